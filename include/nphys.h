@@ -1,0 +1,197 @@
+/* include/nphys.h -- C ABI of the MI355X-native nphysics contact pipeline (libnphys_hip.so).
+ *
+ * One handle (nphys_world) is one MechanicalWorld + GeometricalWorld + body/collider sets of
+ * nphysics 0.23 restricted to rigid balls and cuboids. Every entry point below replaces one
+ * reference call; the citation after each declaration names it (paths relative to the
+ * nphysics source tree).
+ *
+ * Conventions
+ *   - Return value: NPHYS_OK (0) or a negative NPHYS_ERR_* code; nphys_last_error() gives a
+ *     message. Invariants the reference enforces with panics (negative dt, bad handles, a non
+ *     dynamic body in the island, ...) are errors here, never silent skips.
+ *   - Buffers are caller allocated host memory; the world owns all device memory.
+ *   - A world handle is not reentrant and must be externally synchronised; distinct handles are
+ *     independent. Each world owns one HIP stream on the current device.
+ *   - Body handles are dense indices 0..n-1 in insertion order. Static bodies (ground, walls)
+ *     have 0 status-dependent dofs; every body carries exactly one collider at the identity
+ *     position_wrt_body.
+ *   - Quaternions are (i, j, k, w), the nalgebra storage order.
+ *   - Scalars on the device are FP32 (nphysics is generic over RealField; this build fixes f32).
+ */
+#ifndef NPHYS_H
+#define NPHYS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NPHYS_OK 0
+#define NPHYS_ERR_INVALID (-1)   /* bad argument or handle */
+#define NPHYS_ERR_HIP (-2)       /* HIP runtime failure (message in nphys_last_error) */
+#define NPHYS_ERR_CAPACITY (-3)  /* caller buffer too small */
+#define NPHYS_ERR_STATE (-4)     /* call not valid in the current world state */
+#define NPHYS_ERR_NO_DEVICE (-5) /* no usable gfx950 device */
+
+enum { NPHYS_STATUS_DYNAMIC = 0, NPHYS_STATUS_STATIC = 1 };
+enum { NPHYS_SHAPE_BALL = 0, NPHYS_SHAPE_CUBOID = 1 };
+enum {
+  NPHYS_CONTACT_MODEL_SIGNORINI_COULOMB_PYRAMID = 0, /* plain warm start from the impulse cache */
+  NPHYS_CONTACT_MODEL_LEARNED_WARMSTART = 1          /* MLP-predicted initial impulses */
+};
+
+typedef struct nphys_world nphys_world;
+
+/* IntegrationParameters (src/solver/integration_parameters.rs:5-76) field for field, plus the
+ * world's gravity (src/world/mechanical_world.rs:67). inv_dt is derived from dt as set_dt does
+ * (:142-157). Values are converted to the world's FP32 scalar on entry. */
+typedef struct nphys_params {
+  double t;
+  double dt;
+  double erp;
+  double warmstart_coeff;
+  double restitution_velocity_threshold;
+  double allowed_linear_error;
+  double allowed_angular_error;
+  double max_linear_correction;
+  double max_angular_correction;
+  double max_stabilization_multiplier;
+  int32_t max_velocity_iterations;
+  int32_t max_position_iterations;
+  int32_t max_ccd_position_iterations;
+  int32_t max_ccd_substeps;
+  int32_t return_after_ccd_substep;
+  int32_t multiple_ccd_substep_sensor_events_enabled;
+  int32_t ccd_on_penetration_enabled;
+  int32_t reserved0;
+  double gravity[3];
+} nphys_params;
+
+/* RigidBodyDesc + ColliderDesc for n bodies, structure of arrays
+ * (src/object/rigid_body.rs:949-1086, src/object/collider.rs:455-572). */
+typedef struct nphys_body_desc {
+  int32_t n;
+  const int32_t* status;            /* n:  NPHYS_STATUS_* */
+  const int32_t* shape;             /* n:  NPHYS_SHAPE_* */
+  const float* dims;                /* 3n: ball (r, -, -); cuboid half extents */
+  const float* pos;                 /* 3n */
+  const float* rot;                 /* 4n: unit quaternion (i, j, k, w) */
+  const float* linvel;              /* 3n */
+  const float* angvel;              /* 3n */
+  const float* density;             /* n */
+  const float* margin;              /* n: collider margin (default 0.01) */
+  const float* linear_prediction;   /* n: default 0.001 */
+  const float* friction;            /* n: BasicMaterial friction (default 0.5) */
+  const float* restitution;         /* n: BasicMaterial restitution (default 0) */
+} nphys_body_desc;
+
+/* Counters (src/counters/mod.rs:17-219): accumulated stage seconds plus sizes of the last step. */
+typedef struct nphys_counters {
+  double step;
+  double collision_detection;
+  double broad_phase;
+  double narrow_phase;
+  double assembly;
+  double velocity_resolution;
+  double velocity_update;
+  double position_resolution;
+  int64_t nsteps;
+  int64_t npairs;
+  int64_t ncontacts;
+  int64_t nmanifolds;
+  int64_t nconstraints;
+  int64_t ncolours;
+} nphys_counters;
+
+/* ---------------------------------------------------------------- lifecycle */
+
+/* IntegrationParameters::default() (integration_parameters.rs:169-189) and gravity (0,-9.81,0). */
+void nphys_params_default(nphys_params* out);
+
+/* MechanicalWorld::new + GeometricalWorld::new (mechanical_world.rs:76-102,
+ * geometrical_world.rs:57-63). Selects the current HIP device; fails with NPHYS_ERR_NO_DEVICE
+ * when there is none. */
+int nphys_world_create(const nphys_params* params, nphys_world** out);
+int nphys_world_destroy(nphys_world* w);
+
+/* integration_parameters / gravity pub fields (mechanical_world.rs:55-68); set_timestep
+ * (:108-110) is nphys_set_params with a new dt. A negative dt is NPHYS_ERR_INVALID
+ * (the reference asserts, integration_parameters.rs:143-146). */
+int nphys_set_params(nphys_world* w, const nphys_params* params);
+int nphys_get_params(const nphys_world* w, nphys_params* out);
+
+/* ---------------------------------------------------------------- scene */
+
+/* BodySet::insert + ColliderSet::insert, then the insertion half of MechanicalWorld::maintain
+ * (mechanical_world.rs:114-179 -> geometrical_world.rs:104-140): mass properties from the
+ * density (volumetric.rs:46-59), proxy creation. *first_handle receives the first new index. */
+int nphys_add_bodies(nphys_world* w, const nphys_body_desc* desc, int32_t* first_handle);
+int nphys_num_bodies(const nphys_world* w, int32_t* n);
+
+/* RigidBody::set_position / set_velocity for every body (rigid_body.rs:305-324); marks the
+ * bodies moved so the next step or nphys_collide re-runs collision detection. Static bodies
+ * ignore velocities. NULL arrays are left unchanged. */
+int nphys_set_state(nphys_world* w, const float* pos, const float* rot, const float* linvel,
+                    const float* angvel);
+int nphys_get_state(nphys_world* w, float* pos, float* rot, float* linvel, float* angvel);
+
+/* ---------------------------------------------------------------- stepping */
+
+/* MechanicalWorld::step (mechanical_world.rs:182-196), nsteps times, on the world's stream;
+ * returns after the last step is complete on the device. */
+int nphys_step(nphys_world* w, int32_t nsteps);
+
+/* GeometricalWorld::sync_colliders + perform_broad_phase + perform_narrow_phase
+ * (geometrical_world.rs:230-320) on the current poses, as MechanicalWorld::step does before the
+ * solver when bodies were moved by the user (mechanical_world.rs:250-253). */
+int nphys_collide(nphys_world* w);
+
+/* MoreauJeanSolver::set_contact_model (moreau_jean_solver.rs:42-44) restricted to the two models
+ * this build provides. The learned model falls back to the plain cache warm start until a
+ * network is loaded with nphys_warmstart_load. */
+int nphys_contact_model_set(nphys_world* w, int32_t model);
+
+/* ---------------------------------------------------------------- readback */
+
+/* Broad-phase pairs, sorted: (first, second) with first dynamic and second static or a larger
+ * index (the interaction graph of geometrical_world.rs:301-311). pairs holds 2*cap int32. */
+int nphys_num_pairs(nphys_world* w, int64_t* n);
+int nphys_read_pairs(nphys_world* w, int32_t* pairs, int64_t cap);
+
+/* Contact manifolds of the current interaction graph (GeometricalWorld::contact_pairs,
+ * geometrical_world.rs:534-553): per contact its two bodies, feature id, world points (no
+ * margin), normal (body 1 -> body 2) and depth (> 0 penetrating). Any pointer may be NULL. */
+int nphys_num_contacts(nphys_world* w, int64_t* n);
+int nphys_read_contacts(nphys_world* w, int32_t* bodies, uint32_t* feature, float* world1,
+                        float* world2, float* normal, float* depth, int64_t cap);
+
+/* Impulses (normal, tangent 1, tangent 2) of the contacts solved by the last step, in the order
+ * and with the keys of that step's contacts (the ImpulseCache of
+ * signorini_coulomb_pyramid_model.rs:226-261). keys may be NULL. */
+int nphys_num_solved_contacts(nphys_world* w, int64_t* n);
+int nphys_read_impulses(nphys_world* w, uint64_t* keys, float* impulses, int64_t cap);
+
+/* Natural-map LCP residual after the last velocity solve: max and RMS over all rows of
+ * |l - proj(l - r w)| / r, w = J dv + rhs (m/s). The reference computes no residual; this is the
+ * measure both this library and its CPU checker report. */
+int nphys_residual(nphys_world* w, double* r_inf, double* r_rms);
+
+/* Counters::{step,...}_started/completed totals (counters/mod.rs:148-209). */
+int nphys_read_counters(nphys_world* w, nphys_counters* out);
+int nphys_reset_counters(nphys_world* w);
+
+/* Colour of every manifold solved by the last step, in manifold order. Recording starts with the
+ * first call (a call with colours == NULL only enables it), so the first step after enabling is
+ * the first one reported. */
+int nphys_read_colours(nphys_world* w, int32_t* colours, int64_t cap, int32_t* ncolours);
+
+/* Message of the last error on this thread ("" if none). */
+const char* nphys_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NPHYS_H */

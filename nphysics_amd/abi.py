@@ -1,0 +1,162 @@
+"""ctypes mirrors of the structs declared in include/nphys.h.
+
+The same structs are accepted by the product library (libnphys_hip.so) and by the CPU checker
+in oracle/, so scene descriptions are built once and handed to either.
+"""
+import ctypes as C
+
+import numpy as np
+
+NPHYS_OK = 0
+NPHYS_ERR_INVALID = -1
+NPHYS_ERR_HIP = -2
+NPHYS_ERR_CAPACITY = -3
+NPHYS_ERR_STATE = -4
+NPHYS_ERR_NO_DEVICE = -5
+
+STATUS_DYNAMIC = 0
+STATUS_STATIC = 1
+SHAPE_BALL = 0
+SHAPE_CUBOID = 1
+CONTACT_MODEL_SIGNORINI_COULOMB_PYRAMID = 0
+CONTACT_MODEL_LEARNED_WARMSTART = 1
+
+
+class Params(C.Structure):
+    """IntegrationParameters (src/solver/integration_parameters.rs:5-76) + gravity."""
+
+    _fields_ = [
+        ("t", C.c_double),
+        ("dt", C.c_double),
+        ("erp", C.c_double),
+        ("warmstart_coeff", C.c_double),
+        ("restitution_velocity_threshold", C.c_double),
+        ("allowed_linear_error", C.c_double),
+        ("allowed_angular_error", C.c_double),
+        ("max_linear_correction", C.c_double),
+        ("max_angular_correction", C.c_double),
+        ("max_stabilization_multiplier", C.c_double),
+        ("max_velocity_iterations", C.c_int32),
+        ("max_position_iterations", C.c_int32),
+        ("max_ccd_position_iterations", C.c_int32),
+        ("max_ccd_substeps", C.c_int32),
+        ("return_after_ccd_substep", C.c_int32),
+        ("multiple_ccd_substep_sensor_events_enabled", C.c_int32),
+        ("ccd_on_penetration_enabled", C.c_int32),
+        ("reserved0", C.c_int32),
+        ("gravity", C.c_double * 3),
+    ]
+
+
+def default_params(**overrides):
+    """IntegrationParameters::default() (integration_parameters.rs:169-189), gravity -9.81 y."""
+    p = Params()
+    p.t = 0.0
+    p.dt = 1.0 / 60.0
+    p.erp = 0.2
+    p.warmstart_coeff = 1.0
+    p.restitution_velocity_threshold = 1.0
+    p.allowed_linear_error = 0.001
+    p.allowed_angular_error = 0.001
+    p.max_linear_correction = 0.2
+    p.max_angular_correction = 0.2
+    p.max_stabilization_multiplier = 0.2
+    p.max_velocity_iterations = 8
+    p.max_position_iterations = 3
+    p.max_ccd_position_iterations = 10
+    p.max_ccd_substeps = 1
+    p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, -9.81, 0.0
+    for k, v in overrides.items():
+        if k == "gravity":
+            for i in range(3):
+                p.gravity[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class BodyDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32),
+        ("status", C.POINTER(C.c_int32)),
+        ("shape", C.POINTER(C.c_int32)),
+        ("dims", C.POINTER(C.c_float)),
+        ("pos", C.POINTER(C.c_float)),
+        ("rot", C.POINTER(C.c_float)),
+        ("linvel", C.POINTER(C.c_float)),
+        ("angvel", C.POINTER(C.c_float)),
+        ("density", C.POINTER(C.c_float)),
+        ("margin", C.POINTER(C.c_float)),
+        ("linear_prediction", C.POINTER(C.c_float)),
+        ("friction", C.POINTER(C.c_float)),
+        ("restitution", C.POINTER(C.c_float)),
+    ]
+
+
+class Counters(C.Structure):
+    _fields_ = [
+        ("step", C.c_double),
+        ("collision_detection", C.c_double),
+        ("broad_phase", C.c_double),
+        ("narrow_phase", C.c_double),
+        ("assembly", C.c_double),
+        ("velocity_resolution", C.c_double),
+        ("velocity_update", C.c_double),
+        ("position_resolution", C.c_double),
+        ("nsteps", C.c_int64),
+        ("npairs", C.c_int64),
+        ("ncontacts", C.c_int64),
+        ("nmanifolds", C.c_int64),
+        ("nconstraints", C.c_int64),
+        ("ncolours", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_}
+
+
+def fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def u32ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def u64ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def make_body_desc(scene):
+    """Builds a BodyDesc over the arrays of a scene dict; returns (desc, keepalive)."""
+    keep = {}
+    n = int(scene["status"].shape[0])
+
+    def arr(name, dtype, width):
+        a = np.ascontiguousarray(scene[name], dtype=dtype).reshape(n * width if width > 1 else n)
+        keep[name] = a
+        return a
+
+    d = BodyDesc()
+    d.n = n
+    d.status = iptr(arr("status", np.int32, 1))
+    d.shape = iptr(arr("shape", np.int32, 1))
+    d.dims = fptr(arr("dims", np.float32, 3))
+    d.pos = fptr(arr("pos", np.float32, 3))
+    d.rot = fptr(arr("rot", np.float32, 4))
+    d.linvel = fptr(arr("linvel", np.float32, 3))
+    d.angvel = fptr(arr("angvel", np.float32, 3))
+    d.density = fptr(arr("density", np.float32, 1))
+    d.margin = fptr(arr("margin", np.float32, 1))
+    d.linear_prediction = fptr(arr("linear_prediction", np.float32, 1))
+    d.friction = fptr(arr("friction", np.float32, 1))
+    d.restitution = fptr(arr("restitution", np.float32, 1))
+    return d, keep

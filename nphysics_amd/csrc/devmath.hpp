@@ -1,0 +1,212 @@
+// nphysics_amd/csrc/devmath.hpp -- FP32 vector, quaternion and 3x3 helpers for the kernels.
+//
+// Every helper fixes its evaluation order (sums left to right, products as written) and the
+// library is compiled with -ffp-contract=off, so each result is the correctly rounded FP32 value
+// of the expression as written: the same value on the GPU and in any IEEE FP32 implementation
+// that evaluates the same expression. Formulas:
+//   dot (a.x b.x + a.y b.y) + a.z b.z; cross as usual;
+//   quaternion (i, j, k, w), Hamilton product, rotation of a vector v + w t + u x t with
+//   t = 2 u x v (the nalgebra forms behind src/object/rigid_body.rs:371-381);
+//   rotation matrix in homogeneous form; 3x3 inverse by cofactors, zero if singular
+//   (src/algebra/inertia3.rs:77-86); exp of a pure quaternion with a Maclaurin polynomial for
+//   half angles below 1/4 rad (src/algebra/velocity3.rs:57-59 -> Isometry3::new).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#define NPH_HD __host__ __device__ __forceinline__
+
+namespace nph {
+
+struct f3 {
+  float x, y, z;
+};
+
+NPH_HD f3 mk3(float x, float y, float z) { return f3{x, y, z}; }
+NPH_HD f3 add(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+NPH_HD f3 sub(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+NPH_HD f3 neg(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+NPH_HD f3 scale(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+NPH_HD float dot(f3 a, f3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+NPH_HD f3 cross(f3 a, f3 b) {
+  return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+NPH_HD float fminf_(float a, float b) { return a < b ? a : b; }
+NPH_HD float fmaxf_(float a, float b) { return a > b ? a : b; }
+NPH_HD f3 vmin(f3 a, f3 b) { return f3{fminf_(a.x, b.x), fminf_(a.y, b.y), fminf_(a.z, b.z)}; }
+NPH_HD f3 vmax(f3 a, f3 b) { return f3{fmaxf_(a.x, b.x), fmaxf_(a.y, b.y), fmaxf_(a.z, b.z)}; }
+NPH_HD float comp(f3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+NPH_HD void set_comp(f3& a, int i, float v) {
+  if (i == 0) a.x = v;
+  else if (i == 1) a.y = v;
+  else a.z = v;
+}
+NPH_HD f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+NPH_HD float4 f4(f3 v, float w) { return make_float4(v.x, v.y, v.z, w); }
+
+struct q4 {
+  float i, j, k, w;
+};
+NPH_HD q4 toq(float4 v) { return q4{v.x, v.y, v.z, v.w}; }
+NPH_HD float4 fromq(q4 q) { return make_float4(q.i, q.j, q.k, q.w); }
+
+NPH_HD q4 qmul(q4 a, q4 b) {
+  q4 r;
+  r.w = ((a.w * b.w - a.i * b.i) - a.j * b.j) - a.k * b.k;
+  r.i = ((a.w * b.i + a.i * b.w) + a.j * b.k) - a.k * b.j;
+  r.j = ((a.w * b.j - a.i * b.k) + a.j * b.w) + a.k * b.i;
+  r.k = ((a.w * b.k + a.i * b.j) - a.j * b.i) + a.k * b.w;
+  return r;
+}
+
+NPH_HD f3 qrot(q4 q, f3 v) {
+  const f3 u{q.i, q.j, q.k};
+  const f3 t = scale(cross(u, v), 2.0f);
+  const f3 c = cross(u, t);
+  return add(add(scale(t, q.w), c), v);
+}
+
+// Row-major 3x3.
+struct m33 {
+  float m[3][3];
+};
+
+NPH_HD m33 qmat(q4 q) {
+  const float ww = q.w * q.w, ii = q.i * q.i, jj = q.j * q.j, kk = q.k * q.k;
+  const float ij = q.i * q.j * 2.0f, wk = q.w * q.k * 2.0f, wj = q.w * q.j * 2.0f;
+  const float ik = q.i * q.k * 2.0f, jk = q.j * q.k * 2.0f, wi = q.w * q.i * 2.0f;
+  m33 r;
+  r.m[0][0] = ((ww + ii) - jj) - kk; r.m[0][1] = ij - wk;               r.m[0][2] = wj + ik;
+  r.m[1][0] = wk + ij;               r.m[1][1] = ((ww - ii) + jj) - kk; r.m[1][2] = jk - wi;
+  r.m[2][0] = ik - wj;               r.m[2][1] = wi + jk;               r.m[2][2] = ((ww - ii) - jj) + kk;
+  return r;
+}
+
+NPH_HD m33 mmul(const m33& a, const m33& b) {
+  m33 r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      r.m[i][j] = (a.m[i][0] * b.m[0][j] + a.m[i][1] * b.m[1][j]) + a.m[i][2] * b.m[2][j];
+  return r;
+}
+NPH_HD m33 mtrans(const m33& a) {
+  m33 r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.m[i][j] = a.m[j][i];
+  return r;
+}
+NPH_HD f3 mulv(const m33& a, f3 v) {
+  return f3{(a.m[0][0] * v.x + a.m[0][1] * v.y) + a.m[0][2] * v.z,
+            (a.m[1][0] * v.x + a.m[1][1] * v.y) + a.m[1][2] * v.z,
+            (a.m[2][0] * v.x + a.m[2][1] * v.y) + a.m[2][2] * v.z};
+}
+NPH_HD f3 mulv_t(const m33& a, f3 v) {
+  return f3{(a.m[0][0] * v.x + a.m[1][0] * v.y) + a.m[2][0] * v.z,
+            (a.m[0][1] * v.x + a.m[1][1] * v.y) + a.m[2][1] * v.z,
+            (a.m[0][2] * v.x + a.m[1][2] * v.y) + a.m[2][2] * v.z};
+}
+NPH_HD f3 mcol(const m33& a, int j) { return f3{a.m[0][j], a.m[1][j], a.m[2][j]}; }
+
+NPH_HD m33 cross_matrix(f3 v) {
+  m33 r;
+  r.m[0][0] = 0.0f; r.m[0][1] = -v.z; r.m[0][2] = v.y;
+  r.m[1][0] = v.z;  r.m[1][1] = 0.0f; r.m[1][2] = -v.x;
+  r.m[2][0] = -v.y; r.m[2][1] = v.x;  r.m[2][2] = 0.0f;
+  return r;
+}
+
+NPH_HD bool try_inverse(const m33& a, m33& out) {
+  const float c00 = a.m[1][1] * a.m[2][2] - a.m[2][1] * a.m[1][2];
+  const float c01 = a.m[1][0] * a.m[2][2] - a.m[2][0] * a.m[1][2];
+  const float c02 = a.m[1][0] * a.m[2][1] - a.m[2][0] * a.m[1][1];
+  const float det = (a.m[0][0] * c00 - a.m[0][1] * c01) + a.m[0][2] * c02;
+  if (det == 0.0f) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) out.m[i][j] = 0.0f;
+    return false;
+  }
+  out.m[0][0] = c00 / det;
+  out.m[0][1] = (a.m[2][1] * a.m[0][2] - a.m[0][1] * a.m[2][2]) / det;
+  out.m[0][2] = (a.m[0][1] * a.m[1][2] - a.m[1][1] * a.m[0][2]) / det;
+  out.m[1][0] = -c01 / det;
+  out.m[1][1] = (a.m[0][0] * a.m[2][2] - a.m[2][0] * a.m[0][2]) / det;
+  out.m[1][2] = (a.m[1][0] * a.m[0][2] - a.m[0][0] * a.m[1][2]) / det;
+  out.m[2][0] = c02 / det;
+  out.m[2][1] = (a.m[2][0] * a.m[0][1] - a.m[0][0] * a.m[2][1]) / det;
+  out.m[2][2] = (a.m[0][0] * a.m[1][1] - a.m[1][0] * a.m[0][1]) / det;
+  return true;
+}
+
+// exp of the pure quaternion (0, axisangle / 2).
+NPH_HD q4 qexp_scaled_axis(f3 axisangle) {
+  const f3 h = scale(axisangle, 0.5f);
+  const float nn = dot(h, h);
+  const float e = 1.1920928955078125e-07f;  // FP32 machine epsilon
+  if (nn <= e * e) return q4{0.0f, 0.0f, 0.0f, 1.0f};
+  const float n = sqrtf(nn);
+  float s, c;
+  if (n < 0.25f) {
+    const float n2 = n * n;
+    s = 1.0f + n2 * ((float)(-1.0 / 6.0) + n2 * ((float)(1.0 / 120.0) + n2 * ((float)(-1.0 / 5040.0) + n2 * (float)(1.0 / 362880.0))));
+    c = 1.0f + n2 * (-0.5f + n2 * ((float)(1.0 / 24.0) + n2 * ((float)(-1.0 / 720.0) + n2 * (float)(1.0 / 40320.0))));
+  } else {
+    s = sinf(n) / n;
+    c = cosf(n);
+  }
+  return q4{h.x * s, h.y * s, h.z * s, c};
+}
+
+// Pose update of apply_displacement: T(com) * Iso(lin, exp(ang)) * T(-com) * pose, com = t.
+NPH_HD void apply_displacement(f3& t, q4& q, f3 lin, f3 ang) {
+  const q4 qd = qexp_scaled_axis(ang);
+  const f3 com = t;
+  const f3 dt_ = add(com, add(lin, qrot(qd, neg(com))));
+  t = add(dt_, qrot(qd, t));
+  q = qmul(qd, q);
+}
+
+// Axis-aligned boxes.
+struct aabb {
+  f3 mn, mx;
+};
+NPH_HD aabb loosened(aabb a, float amount) {
+  const f3 d{amount, amount, amount};
+  return aabb{sub(a.mn, d), add(a.mx, d)};
+}
+NPH_HD aabb merged(aabb a, aabb b) { return aabb{vmin(a.mn, b.mn), vmax(a.mx, b.mx)}; }
+NPH_HD bool contains(aabb a, aabb b) {
+  return a.mn.x <= b.mn.x && a.mn.y <= b.mn.y && a.mn.z <= b.mn.z && a.mx.x >= b.mx.x &&
+         a.mx.y >= b.mx.y && a.mx.z >= b.mx.z;
+}
+NPH_HD bool intersects(aabb a, aabb b) {
+  return a.mn.x <= b.mx.x && a.mn.y <= b.mx.y && a.mn.z <= b.mx.z && a.mx.x >= b.mn.x &&
+         a.mx.y >= b.mn.y && a.mx.z >= b.mn.z;
+}
+
+enum : int { SHAPE_BALL = 0, SHAPE_CUBOID = 1 };
+enum : int { STATUS_DYNAMIC = 0, STATUS_STATIC = 1 };
+
+// Shape AABB at a pose: ball centre -/+ r, cuboid centre -/+ |R| h.
+NPH_HD aabb shape_aabb(int kind, f3 he, f3 t, q4 q) {
+  if (kind == SHAPE_BALL) {
+    const f3 r{he.x, he.x, he.x};
+    return aabb{sub(t, r), add(t, r)};
+  }
+  const m33 R = qmat(q);
+  m33 A;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) A.m[i][j] = fabsf(R.m[i][j]);
+  const f3 w = mulv(A, he);
+  return aabb{sub(t, w), add(t, w)};
+}
+
+}  // namespace nph

@@ -1,0 +1,290 @@
+// nphysics_amd/csrc/k_broadphase.hip -- broad phase: loose AABBs, Morton LBVH, pair emission.
+//
+// Replaces ncollide's DBVTBroadPhase behind GeometricalWorld::perform_broad_phase
+// (src/world/geometrical_world.rs:60, 285-312) and its DefaultCollisionFilter (:926-971):
+//   * loose AABB rule (k_loose_update): tight = shape AABB at the collider position, merged with
+//     the one at the predicted position when the CCD-phase sync sweeps the collider
+//     (geometrical_world.rs:250-261, position0 -> position), loosened by margin + linear
+//     prediction (collider.rs:542-549); the loose AABB is replaced by tight loosened by 0.01 only
+//     when it no longer contains tight (the DBVT margin of geometrical_world.rs:60).
+//   * pairs = every two bodies whose loose AABBs overlap (closed intervals), not the same body and
+//     not both static; stored as (first, second) with first dynamic and second static or larger,
+//     sorted.
+// The pair set is found with a linear BVH rebuilt whenever a loose AABB changed: Morton codes of
+// the loose-AABB centroids, stable LDS radix sort, Karras (HPG 2012) radix-tree construction, a
+// bottom-up refit with arrival counters, then one thread per dynamic body (in Morton order, so
+// a wavefront walks neighbouring subtrees) traverses the tree and writes its partners into its
+// own fixed-capacity segment; segments are sorted and compacted with a scan. The result is
+// deterministic and independent of thread timing. Algorithmic bytes: 124 per body + 8 per pair.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "stages.hpp"
+
+namespace nph {
+
+namespace {
+
+constexpr int kStack = 64;
+
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+__global__ void __launch_bounds__(256) k_loose_update(DevBodies b, int mode, uint32_t* changed) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  if (mode != 0 && b.status[i] != STATUS_DYNAMIC) return;
+  const int kind = b.shape[i];
+  const f3 he = xyz(b.dims[i]);
+  aabb a = shape_aabb(kind, he, xyz(b.pos[i]), toq(b.rot[i]));
+  if (mode == 2) a = merged(shape_aabb(kind, he, xyz(b.pos0[i]), toq(b.rot0[i])), a);
+  else a = merged(a, a);
+  const aabb tight = loosened(a, b.qlim[i]);
+  const aabb loose{xyz(b.loose_mn[i]), xyz(b.loose_mx[i])};
+  if (!contains(loose, tight)) {
+    const aabb nl = loosened(tight, 0.01f);
+    b.loose_mn[i] = f4(nl.mn, 0.0f);
+    b.loose_mx[i] = f4(nl.mx, 0.0f);
+    atomicAdd(changed, 1u);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_bounds(DevBodies b, uint32_t* bounds) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  const float4 mn = b.loose_mn[i], mx = b.loose_mx[i];
+  const float cx = (mn.x + mx.x) * 0.5f, cy = (mn.y + mx.y) * 0.5f, cz = (mn.z + mx.z) * 0.5f;
+  atomicMin(&bounds[0], f2ord(cx));
+  atomicMin(&bounds[1], f2ord(cy));
+  atomicMin(&bounds[2], f2ord(cz));
+  atomicMax(&bounds[3], f2ord(cx));
+  atomicMax(&bounds[4], f2ord(cy));
+  atomicMax(&bounds[5], f2ord(cz));
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t quant10(float c, float lo, float hi) {
+  const float ext = hi - lo;
+  float u = ext > 0.0f ? (c - lo) / ext : 0.5f;
+  u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
+  const uint32_t q = uint32_t(u * 1023.0f);
+  return q > 1023u ? 1023u : q;
+}
+
+__global__ void __launch_bounds__(256) k_morton(DevBodies b, const uint32_t* bounds, uint32_t* keys,
+                                               uint32_t* vals) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  const float lx = ord2f(bounds[0]), ly = ord2f(bounds[1]), lz = ord2f(bounds[2]);
+  const float hx = ord2f(bounds[3]), hy = ord2f(bounds[4]), hz = ord2f(bounds[5]);
+  const float4 mn = b.loose_mn[i], mx = b.loose_mx[i];
+  const float cx = (mn.x + mx.x) * 0.5f, cy = (mn.y + mx.y) * 0.5f, cz = (mn.z + mx.z) * 0.5f;
+  const uint32_t x = quant10(cx, lx, hx), y = quant10(cy, ly, hy), z = quant10(cz, lz, hz);
+  keys[i] = (expand10(x) << 2) | (expand10(y) << 1) | expand10(z);
+  vals[i] = uint32_t(i);
+}
+
+__device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = keys[i], c = keys[j];
+  if (a == c) return 32 + __clz(uint32_t(i ^ j));
+  return __clz(a ^ c);
+}
+
+// Karras 2012: internal node i spans a key range with one split; children < n-1 are internal,
+// children >= n-1 are leaves (leaf t is node n-1+t).
+__global__ void __launch_bounds__(256) k_lbvh(const uint32_t* __restrict__ keys, int n, int* child,
+                                             int* parent) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int s = 0;
+  int t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int gamma = i + s * d + (d < 0 ? d : 0);
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  const int left = lo == gamma ? (n - 1 + gamma) : gamma;
+  const int right = hi == gamma + 1 ? (n - 1 + gamma + 1) : gamma + 1;
+  child[2 * i] = left;
+  child[2 * i + 1] = right;
+  parent[left] = i;
+  parent[right] = i;
+  if (i == 0) parent[0] = -1;
+}
+
+__global__ void __launch_bounds__(256) k_refit(DevBodies b, const uint32_t* __restrict__ vals, int n,
+                                              const int* __restrict__ child,
+                                              const int* __restrict__ parent, uint32_t* visit,
+                                              float4* node_mn, float4* node_mx) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int body = int(vals[t]);
+  int node = n - 1 + t;
+  node_mn[node] = b.loose_mn[body];
+  node_mx[node] = b.loose_mx[body];
+  __threadfence();
+  int p = parent[node];
+  while (p >= 0) {
+    const uint32_t old = atomicAdd(&visit[p], 1u);
+    if (old == 0u) return;  // the other child arrives later and continues
+    __threadfence();
+    const int l = child[2 * p], r = child[2 * p + 1];
+    const float4 lmn = node_mn[l], lmx = node_mx[l];
+    const float4 rmn = node_mn[r], rmx = node_mx[r];
+    node_mn[p] = make_float4(fminf_(lmn.x, rmn.x), fminf_(lmn.y, rmn.y), fminf_(lmn.z, rmn.z), 0.0f);
+    node_mx[p] = make_float4(fmaxf_(lmx.x, rmx.x), fmaxf_(lmx.y, rmx.y), fmaxf_(lmx.z, rmx.z), 0.0f);
+    __threadfence();
+    p = parent[p];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* __restrict__ vals, int n,
+                                                 const int* __restrict__ child,
+                                                 const float4* __restrict__ node_mn,
+                                                 const float4* __restrict__ node_mx, int* cand,
+                                                 uint32_t* cand_count, int cap, uint32_t* stats) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int q = int(vals[t]);
+  if (b.status[q] != STATUS_DYNAMIC || n < 2) {
+    cand_count[q] = 0;
+    return;
+  }
+  const aabb Q{xyz(b.loose_mn[q]), xyz(b.loose_mx[q])};
+  const long base = long(q) * cap;
+  uint32_t cnt = 0;
+  int stack[kStack];
+  int sp = 0;
+  int node = 0;
+  bool overflow = false;
+  while (true) {
+    const int c0 = child[2 * node], c1 = child[2 * node + 1];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int c = k == 0 ? c0 : c1;
+      const aabb B{xyz(node_mn[c]), xyz(node_mx[c])};
+      if (!intersects(Q, B)) continue;
+      if (c >= n - 1) {
+        const int o = int(vals[c - (n - 1)]);
+        if (o != q && (b.status[o] != STATUS_DYNAMIC || o > q)) {
+          if (cnt < uint32_t(cap)) cand[base + cnt] = o;
+          ++cnt;
+        }
+      } else if (sp < kStack) {
+        stack[sp++] = c;
+      } else {
+        overflow = true;
+      }
+    }
+    if (sp == 0) break;
+    node = stack[--sp];
+  }
+  cand_count[q] = cnt;
+  atomicMax(&stats[0], cnt);
+  if (overflow) atomicOr(&stats[1], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_clamp_counts(const uint32_t* cnt, uint32_t* out, int n, int cap) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t c = cnt[i];
+  out[i] = c < uint32_t(cap) ? c : uint32_t(cap);
+}
+
+__global__ void __launch_bounds__(256) k_sort_compact(int n, int* cand, const uint32_t* __restrict__ cnt,
+                                                     const uint32_t* __restrict__ off, int cap,
+                                                     int2* pairs) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const uint32_t c0 = cnt[q];
+  const int c = int(c0 < uint32_t(cap) ? c0 : uint32_t(cap));
+  if (c == 0) return;
+  int* seg = cand + long(q) * cap;
+  for (int i = 1; i < c; ++i) {
+    const int v = seg[i];
+    int j = i - 1;
+    while (j >= 0 && seg[j] > v) {
+      seg[j + 1] = seg[j];
+      --j;
+    }
+    seg[j + 1] = v;
+  }
+  const uint32_t o = off[q];
+  for (int i = 0; i < c; ++i) pairs[o + i] = make_int2(q, seg[i]);
+}
+
+inline unsigned nb(long n) { return unsigned((n + 255) / 256); }
+
+}  // namespace
+
+void launch_loose_update(const DevBodies& b, int mode, uint32_t* changed, hipStream_t s) {
+  if (b.n == 0) return;
+  hipLaunchKernelGGL(k_loose_update, dim3(nb(b.n)), dim3(256), 0, s, b, mode, changed);
+}
+
+void launch_build_lbvh(const DevBodies& b, BroadPhaseBufs& bp, hipStream_t s) {
+  const int n = b.n;
+  if (n < 2) return;
+  const uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  (void)hipMemcpyAsync(bp.bounds, init, sizeof(init), hipMemcpyHostToDevice, s);
+  hipLaunchKernelGGL(k_bounds, dim3(nb(n)), dim3(256), 0, s, b, bp.bounds);
+  hipLaunchKernelGGL(k_morton, dim3(nb(n)), dim3(256), 0, s, b, bp.bounds, bp.keys, bp.vals);
+  radix_sort_pairs(bp.keys, bp.vals, bp.keys_alt, bp.vals_alt, n, 30, bp.tmp, s);
+  hipLaunchKernelGGL(k_lbvh, dim3(nb(n - 1)), dim3(256), 0, s, bp.keys, n, bp.child, bp.parent);
+  fill_u32(bp.visit, 0u, size_t(n - 1), s);
+  hipLaunchKernelGGL(k_refit, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.parent, bp.visit,
+                     bp.node_mn, bp.node_mx);
+}
+
+void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s) {
+  const int n = b.n;
+  if (n == 0) return;
+  fill_u32(bp.stats, 0u, 4, s);
+  if (n < 2) {
+    fill_u32(bp.cand_count, 0u, size_t(n), s);
+    return;
+  }
+  hipLaunchKernelGGL(k_traverse, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
+                     bp.node_mx, bp.cand, bp.cand_count, cap, bp.stats);
+}
+
+void launch_compact_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s) {
+  const int n = b.n;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_sort_compact, dim3(nb(n)), dim3(256), 0, s, n, bp.cand, bp.cand_count, bp.cand_off,
+                     cap, bp.pairs);
+}
+
+// Exclusive scan of min(count, cap) into cand_off; the clamped counts reuse keys_alt.
+void launch_pair_offsets(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s) {
+  const int n = b.n;
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_clamp_counts, dim3(nb(n)), dim3(256), 0, s, bp.cand_count, bp.keys_alt, n, cap);
+  scan_exclusive_u32(bp.keys_alt, bp.cand_off, n, bp.tmp, s);
+}
+
+}  // namespace nph

@@ -1,0 +1,106 @@
+// nphysics_amd/csrc/stages.hpp -- launch wrappers of the per-stage kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "devworld.hpp"
+
+namespace nph {
+
+// k_bodies.hip
+void launch_prestep(const DevBodies& b, const SimParams& p, hipStream_t s);
+void launch_integrate(const DevBodies& b, const SimParams& p, hipStream_t s);
+
+// k_broadphase.hip ------------------------------------------------------------------------
+struct BroadPhaseBufs {
+  uint32_t* changed;      // [1] number of loose AABBs replaced
+  uint32_t* bounds;       // [6] ordered-int encoded centroid bounds
+  uint32_t* keys;         // [n] Morton codes
+  uint32_t* vals;         // [n] body indices
+  uint32_t* keys_alt;
+  uint32_t* vals_alt;
+  int* child;             // [2(n-1)] children of internal nodes (>= n-1: leaf)
+  int* parent;            // [2n-1]
+  uint32_t* visit;        // [n-1] refit arrival counters
+  float4* node_mn;        // [2n-1]
+  float4* node_mx;
+  int* cand;              // [n * cap] candidate partners per body
+  uint32_t* cand_count;   // [n] partner count (may exceed cap)
+  uint32_t* cand_off;     // [n] exclusive scan of min(count, cap)
+  uint32_t* stats;        // [4] 0: max count, 1: traversal stack overflow
+  int2* pairs;            // [pair capacity]
+  uint32_t* tmp;          // scan / radix scratch
+};
+
+// mode 0: current pose of every body (static ones included); mode 1: dynamic bodies, current
+// pose; mode 2: dynamic bodies, AABB swept from pos0/rot0 to pos/rot.
+void launch_loose_update(const DevBodies& b, int mode, uint32_t* changed, hipStream_t s);
+void launch_build_lbvh(const DevBodies& b, BroadPhaseBufs& bp, hipStream_t s);
+void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s);
+void launch_pair_offsets(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s);
+void launch_compact_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s);
+
+// k_narrowphase.hip -------------------------------------------------------------------------
+struct NarrowPhaseBufs {
+  float4* sw1;            // [4 P] per-pair contact slots
+  float4* sw2;
+  float4* sn;
+  uint32_t* sfeat;
+  int* skind;
+  float4* slp1;
+  float4* slp2;
+  float4* sln;
+  uint32_t* count;        // [P] contacts per pair
+  uint32_t* coff;         // [P] exclusive scan of count
+  uint32_t* has;          // [P] count > 0
+  uint32_t* moff;         // [P] exclusive scan of has
+  uint32_t* totals;       // [2] contacts, manifolds
+  uint32_t* tmp;
+};
+void launch_narrow_phase(const DevBodies& b, const int2* pairs, int npairs, NarrowPhaseBufs& np,
+                         DevContacts out, hipStream_t s);
+
+// k_solver.hip -----------------------------------------------------------------------------
+struct SolverBufs {
+  uint32_t* deg;          // [n] dynamic-body degree in the manifold graph
+  uint32_t* adj_off;      // [n]
+  uint32_t* adj_fill;     // [n]
+  int* adj;               // [2M]
+  uint32_t* state;        // [M] colouring state: 0xffffffff, or round << 16 | colour
+  uint32_t* remaining;    // [1]
+  uint32_t* ckeys;        // [M] colour keys
+  uint32_t* order;        // [M] manifold ids in colour order
+  uint32_t* ckeys_alt;
+  uint32_t* order_alt;
+  uint32_t* col_start;    // [max colours + 1]
+  uint32_t* mcount_ord;   // [M] contact counts in colour order
+  uint32_t* soff;         // [M] first row record of each slot
+  uint32_t* rec2c;        // [C] contact index of each row record
+  float4* rec;            // [16 C] row records
+  double* red;            // residual partials
+  uint32_t* tmp;
+};
+constexpr int kMaxColours = 4096;
+constexpr int kRecFields = 16;
+
+void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb,
+                           hipStream_t s);
+void launch_colour_round(const DevBodies& b, const DevContacts& c, int M, int round, SolverBufs& sb,
+                         hipStream_t s);
+void launch_colour_keys(int M, SolverBufs& sb, uint32_t* ncol, hipStream_t s);
+void launch_colour_order(const DevContacts& c, int M, int ncolours, SolverBufs& sb, hipStream_t s);
+int residual_blocks(int M);
+void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContacts& prev,
+                     int prevM, int M, const SimParams& p, SolverBufs& sb, int C, hipStream_t s);
+void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int first_slot,
+                             int nslots, SolverBufs& sb, int C, hipStream_t s);
+void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int first_slot, int nslots,
+                       SolverBufs& sb, int C, hipStream_t s);
+void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, int C,
+                     hipStream_t s);
+void launch_cache_impulses(const DevContacts& c, SolverBufs& sb, int C, hipStream_t s);
+
+// k_position.hip ---------------------------------------------------------------------------
+void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p,
+                            int first_slot, int nslots, SolverBufs& sb, hipStream_t s);
+
+}  // namespace nph

@@ -1,0 +1,935 @@
+// nphysics_amd/csrc/world.hip -- host orchestration of one MechanicalWorld::step on one GPU and
+// the C ABI of include/nphys.h.
+//
+// Step order (src/world/mechanical_world.rs:199-396):
+//   1. collision detection on the current poses when bodies were added or moved by the caller
+//      (:250-253);
+//   2. per-body dynamics, accelerations, ext = dt acc, dv = 0 (k_prestep; :230-243 and
+//      moreau_jean_solver.rs:137-174);
+//   3. MoreauJeanSolver::step (moreau_jean_solver.rs:47-90): colouring, assembly with warm start,
+//      coloured SOR-Prox sweeps, residual, impulse cache, integration, coloured position
+//      correction;
+//   4. the CCD-phase sync + broad phase over swept AABBs (:604-606), the final sync (a no-op for
+//      the loose AABBs, checked anyway) and the broad + narrow phase that produce the next step's
+//      manifolds (:375-385).
+// The host reads back a handful of counters per step (loose-AABB changes, pair/contact totals,
+// colouring progress, colour offsets, residual partials); everything else stays on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/nphys.h"
+#include "devmath.hpp"
+#include "kernels.hpp"
+#include "stages.hpp"
+
+namespace nph {
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCK(expr)                                                                   \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return set_err(NPHYS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Growable device buffer (contents are not preserved on growth unless asked).
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t ensure(size_t need, bool keep = false, hipStream_t s = nullptr) {
+    if (need <= bytes) return hipSuccess;
+    size_t nb = std::max(need, bytes + bytes / 2);
+    nb = (nb + 255) & ~size_t(255);
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, nb);
+    if (e != hipSuccess) return e;
+    if (keep && p && bytes) {
+      e = hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) return e;
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) return e;
+    }
+    if (p) (void)hipFree(p);
+    p = q;
+    bytes = nb;
+    return hipSuccess;
+  }
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+struct ContactBufs {
+  DBuf mbody, mfirst, mcount, w1, w2, n, feature, kind, lp1, lp2, ln, imp;
+  int M = 0, C = 0;
+  hipError_t ensure(size_t mcap, size_t ccap) {
+    hipError_t e;
+    if ((e = mbody.ensure(mcap * sizeof(int2))) != hipSuccess) return e;
+    if ((e = mfirst.ensure(mcap * 4)) != hipSuccess) return e;
+    if ((e = mcount.ensure(mcap * 4)) != hipSuccess) return e;
+    for (DBuf* b : {&w1, &w2, &n, &lp1, &lp2, &ln, &imp})
+      if ((e = b->ensure(ccap * sizeof(float4))) != hipSuccess) return e;
+    if ((e = feature.ensure(ccap * 4)) != hipSuccess) return e;
+    if ((e = kind.ensure(ccap * 4)) != hipSuccess) return e;
+    return hipSuccess;
+  }
+  DevContacts view() const {
+    DevContacts d;
+    d.mbody = mbody.as<int2>();
+    d.mfirst = mfirst.as<int>();
+    d.mcount = mcount.as<int>();
+    d.w1 = w1.as<float4>();
+    d.w2 = w2.as<float4>();
+    d.n = n.as<float4>();
+    d.feature = feature.as<uint32_t>();
+    d.kind = kind.as<int>();
+    d.lp1 = lp1.as<float4>();
+    d.lp2 = lp2.as<float4>();
+    d.ln = ln.as<float4>();
+    d.imp = imp.as<float4>();
+    return d;
+  }
+};
+
+enum StageEv { EV_START, EV_PRE_CD, EV_ASM, EV_VEL, EV_INT, EV_POS, EV_BP, EV_NP, EV_COUNT };
+
+}  // namespace
+
+struct World {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  nphys_params prm{};
+  SimParams sp{};
+  int32_t contact_model = NPHYS_CONTACT_MODEL_SIGNORINI_COULOMB_PYRAMID;
+
+  // host copies of the per-body static attributes
+  std::vector<int> h_status, h_shape;
+  std::vector<float4> h_dims, h_local_i;
+  std::vector<float> h_margin, h_qlim, h_friction, h_restitution, h_mass;
+  int n = 0;
+
+  // device body arrays
+  DBuf status, shape, dims, margin, qlim, friction, restitution, mass, local_i;
+  DBuf pos, rot, pos0, rot0, linvel, angvel, ia0, ia1, ia2, ext_lin, ext_ang, dv_lin, dv_ang;
+  DBuf loose_mn, loose_mx;
+
+  // broad phase
+  DBuf bp_small, keys, vals, keys_alt, vals_alt, child, parent, visit, node_mn, node_mx, cand,
+      cand_count, cand_off, pairs, bp_tmp;
+  int cap = 32;
+  int npairs = 0;
+  bool pairs_dirty = true;
+  bool moved = true;
+
+  // narrow phase
+  DBuf np_sw1, np_sw2, np_sn, np_sfeat, np_skind, np_slp1, np_slp2, np_sln, np_count, np_coff,
+      np_has, np_moff, np_tmp;
+  ContactBufs cb[2];
+  int cur = 0;      // index of the manifolds produced by the last collision detection
+  bool have_solved = false;
+
+  // solver
+  DBuf sv_deg, sv_adj_off, sv_adj_fill, sv_adj, sv_state, sv_ckeys, sv_order, sv_ckeys_alt,
+      sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_rec2c, sv_rec, sv_red, sv_tmp;
+  std::vector<uint32_t> col_start;
+  int ncolours = 0;
+  int solved_M = 0, solved_C = 0;
+  double res_inf = 0.0, res_rms = 0.0;
+  std::vector<int> last_colours;
+  bool keep_colours = false;
+
+  // pinned host scratch for small readbacks
+  uint32_t* h_small = nullptr;
+  hipEvent_t ev[EV_COUNT] = {};
+  nphys_counters counters{};
+
+  ~World() {
+    if (h_small) (void)hipHostFree(h_small);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void set_params(const nphys_params& p) {
+    prm = p;
+    sp.dt = float(p.dt);
+    sp.inv_dt = sp.dt == 0.0f ? 0.0f : 1.0f / sp.dt;
+    sp.erp = float(p.erp);
+    sp.warmstart_coeff = float(p.warmstart_coeff);
+    sp.restitution_threshold = float(p.restitution_velocity_threshold);
+    sp.allowed_linear_error = float(p.allowed_linear_error);
+    sp.max_linear_correction = float(p.max_linear_correction);
+    sp.gx = float(p.gravity[0]);
+    sp.gy = float(p.gravity[1]);
+    sp.gz = float(p.gravity[2]);
+  }
+
+  int init(const nphys_params& p) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      return set_err(NPHYS_ERR_NO_DEVICE, "no HIP device");
+    HIPCK(hipGetDevice(&device));
+    HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_small), 64 * sizeof(uint32_t)));
+    for (auto& e : ev) HIPCK(hipEventCreate(&e));
+    set_params(p);
+    HIPCK(bp_small.ensure(64 * sizeof(uint32_t)));
+    return NPHYS_OK;
+  }
+
+  DevBodies bodies() const {
+    DevBodies b;
+    b.n = n;
+    b.status = status.as<int>();
+    b.shape = shape.as<int>();
+    b.dims = dims.as<float4>();
+    b.margin = margin.as<float>();
+    b.qlim = qlim.as<float>();
+    b.friction = friction.as<float>();
+    b.restitution = restitution.as<float>();
+    b.mass = mass.as<float>();
+    b.local_i = local_i.as<float4>();
+    b.pos = pos.as<float4>();
+    b.rot = rot.as<float4>();
+    b.pos0 = pos0.as<float4>();
+    b.rot0 = rot0.as<float4>();
+    b.linvel = linvel.as<float4>();
+    b.angvel = angvel.as<float4>();
+    b.ia0 = ia0.as<float4>();
+    b.ia1 = ia1.as<float4>();
+    b.ia2 = ia2.as<float4>();
+    b.ext_lin = ext_lin.as<float4>();
+    b.ext_ang = ext_ang.as<float4>();
+    b.dv_lin = dv_lin.as<float4>();
+    b.dv_ang = dv_ang.as<float4>();
+    b.loose_mn = loose_mn.as<float4>();
+    b.loose_mx = loose_mx.as<float4>();
+    return b;
+  }
+
+  // ------------------------------------------------------------------ scene
+  int add_bodies(const nphys_body_desc& d, int32_t* first) {
+    if (d.n < 0) return set_err(NPHYS_ERR_INVALID, "negative body count");
+    if (!d.status || !d.shape || !d.dims || !d.pos || !d.rot || !d.density || !d.margin ||
+        !d.linear_prediction || !d.friction || !d.restitution)
+      return set_err(NPHYS_ERR_INVALID, "missing body array");
+    if (int64_t(n) + d.n >= (int64_t(1) << 24)) return set_err(NPHYS_ERR_INVALID, "too many bodies");
+    if (first) *first = n;
+    if (d.n == 0) return NPHYS_OK;
+    const int n0 = n, n1 = n + d.n;
+    std::vector<float4> hp(d.n), hr(d.n), hv(d.n), hw(d.n), hmn(d.n), hmx(d.n);
+    for (int k = 0; k < d.n; ++k) {
+      const int st = d.status[k], sh = d.shape[k];
+      if (st != NPHYS_STATUS_DYNAMIC && st != NPHYS_STATUS_STATIC)
+        return set_err(NPHYS_ERR_INVALID, "bad body status");
+      if (sh != NPHYS_SHAPE_BALL && sh != NPHYS_SHAPE_CUBOID) return set_err(NPHYS_ERR_INVALID, "bad shape");
+      const float* dm = d.dims + 3 * k;
+      if (sh == NPHYS_SHAPE_BALL ? !(dm[0] > 0.0f) : !(dm[0] > 0.0f && dm[1] > 0.0f && dm[2] > 0.0f))
+        return set_err(NPHYS_ERR_INVALID, "shape dimensions must be positive");
+    }
+    for (int k = 0; k < d.n; ++k) {
+      const int st = d.status[k], sh = d.shape[k];
+      const float* dm = d.dims + 3 * k;
+      const float4 he = sh == NPHYS_SHAPE_BALL ? make_float4(dm[0], dm[0], dm[0], 0.0f)
+                                               : make_float4(dm[0], dm[1], dm[2], 0.0f);
+      // mass properties (volumetric_ball.rs:9-44, volumetric_cuboid.rs:10-75, volumetric.rs:46-59)
+      float m = 0.0f;
+      float4 li = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      const float density = d.density[k];
+      if (st == NPHYS_STATUS_DYNAMIC && density != 0.0f) {
+        if (sh == NPHYS_SHAPE_BALL) {
+          const float r = dm[0];
+          const float vol = ((((float)M_PI * r) * r) * r) * (float)(4.0 / 3.0);
+          m = vol * density;
+          const float diag = (r * r) * (float)(2.0 / 5.0);
+          li = make_float4(diag * m, diag * m, diag * m, 0.0f);
+        } else {
+          float vol = 1.0f;
+          vol = vol * dm[0] * 2.0f;
+          vol = vol * dm[1] * 2.0f;
+          vol = vol * dm[2] * 2.0f;
+          m = vol * density;
+          const float w = ((float)(1.0 / 12.0) * 2.0f) * 2.0f;
+          const float ix = (w * dm[0]) * dm[0], iy = (w * dm[1]) * dm[1], iz = (w * dm[2]) * dm[2];
+          li = make_float4((iy + iz) * m, (ix + iz) * m, (ix + iy) * m, 0.0f);
+        }
+      }
+      h_status.push_back(st);
+      h_shape.push_back(sh);
+      h_dims.push_back(he);
+      h_local_i.push_back(li);
+      h_mass.push_back(m);
+      h_margin.push_back(d.margin[k]);
+      h_qlim.push_back(d.margin[k] + d.linear_prediction[k]);
+      h_friction.push_back(d.friction[k]);
+      h_restitution.push_back(d.restitution[k]);
+      const float* p = d.pos + 3 * k;
+      const float* q = d.rot + 4 * k;
+      hp[k] = make_float4(p[0], p[1], p[2], 0.0f);
+      hr[k] = make_float4(q[0], q[1], q[2], q[3]);
+      const bool dyn = st == NPHYS_STATUS_DYNAMIC;
+      hv[k] = (dyn && d.linvel) ? make_float4(d.linvel[3 * k], d.linvel[3 * k + 1], d.linvel[3 * k + 2], 0.0f)
+                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      hw[k] = (dyn && d.angvel) ? make_float4(d.angvel[3 * k], d.angvel[3 * k + 1], d.angvel[3 * k + 2], 0.0f)
+                                : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      // proxy creation: loose = aabb(pos) loosened by the query limit, then by 0.01
+      const aabb a = shape_aabb(sh, xyz(he), xyz(hp[k]), toq(hr[k]));
+      const aabb lo = loosened(loosened(a, h_qlim.back()), 0.01f);
+      hmn[k] = f4(lo.mn, 0.0f);
+      hmx[k] = f4(lo.mx, 0.0f);
+    }
+    const size_t f4b = sizeof(float4);
+    // grow the per-body device arrays, keeping the existing bodies' state
+    for (DBuf* b : {&pos, &rot, &linvel, &angvel, &loose_mn, &loose_mx})
+      HIPCK(b->ensure(size_t(n1) * f4b, true, stream));
+    for (DBuf* b : {&pos0, &rot0, &ia0, &ia1, &ia2, &ext_lin, &ext_ang, &dv_lin, &dv_ang, &dims, &local_i})
+      HIPCK(b->ensure(size_t(n1) * f4b));
+    for (DBuf* b : {&status, &shape, &margin, &qlim, &friction, &restitution, &mass})
+      HIPCK(b->ensure(size_t(n1) * 4));
+    auto up = [&](DBuf& b, const void* src, size_t bytes, size_t off) {
+      return hipMemcpyAsync(static_cast<char*>(b.p) + off, src, bytes, hipMemcpyHostToDevice, stream);
+    };
+    HIPCK(up(status, h_status.data(), size_t(n1) * 4, 0));
+    HIPCK(up(shape, h_shape.data(), size_t(n1) * 4, 0));
+    HIPCK(up(dims, h_dims.data(), size_t(n1) * f4b, 0));
+    HIPCK(up(local_i, h_local_i.data(), size_t(n1) * f4b, 0));
+    HIPCK(up(mass, h_mass.data(), size_t(n1) * 4, 0));
+    HIPCK(up(margin, h_margin.data(), size_t(n1) * 4, 0));
+    HIPCK(up(qlim, h_qlim.data(), size_t(n1) * 4, 0));
+    HIPCK(up(friction, h_friction.data(), size_t(n1) * 4, 0));
+    HIPCK(up(restitution, h_restitution.data(), size_t(n1) * 4, 0));
+    HIPCK(up(pos, hp.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
+    HIPCK(up(rot, hr.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
+    HIPCK(up(linvel, hv.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
+    HIPCK(up(angvel, hw.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
+    HIPCK(up(loose_mn, hmn.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
+    HIPCK(up(loose_mx, hmx.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
+    HIPCK(hipStreamSynchronize(stream));
+    n = n1;
+    pairs_dirty = true;
+    moved = true;
+    return NPHYS_OK;
+  }
+
+  int set_state(const float* p, const float* r, const float* lv, const float* av) {
+    if (n == 0) return NPHYS_OK;
+    std::vector<float4> hp(n), hr(n), hv(n), hw(n);
+    const size_t f4b = sizeof(float4);
+    HIPCK(hipMemcpyAsync(hp.data(), pos.p, n * f4b, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(hr.data(), rot.p, n * f4b, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(hv.data(), linvel.p, n * f4b, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(hw.data(), angvel.p, n * f4b, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    for (int i = 0; i < n; ++i) {
+      if (p) hp[i] = make_float4(p[3 * i], p[3 * i + 1], p[3 * i + 2], 0.0f);
+      if (r) hr[i] = make_float4(r[4 * i], r[4 * i + 1], r[4 * i + 2], r[4 * i + 3]);
+      if (h_status[i] == NPHYS_STATUS_DYNAMIC) {
+        if (lv) hv[i] = make_float4(lv[3 * i], lv[3 * i + 1], lv[3 * i + 2], 0.0f);
+        if (av) hw[i] = make_float4(av[3 * i], av[3 * i + 1], av[3 * i + 2], 0.0f);
+      }
+    }
+    HIPCK(hipMemcpyAsync(pos.p, hp.data(), n * f4b, hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(rot.p, hr.data(), n * f4b, hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(linvel.p, hv.data(), n * f4b, hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(angvel.p, hw.data(), n * f4b, hipMemcpyHostToDevice, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    moved = true;
+    return NPHYS_OK;
+  }
+
+  int get_state(float* p, float* r, float* lv, float* av) {
+    if (n == 0) return NPHYS_OK;
+    std::vector<float4> h(n);
+    const size_t f4b = sizeof(float4);
+    struct Item { DBuf* b; float* out; int w; };
+    const Item items[4] = {{&pos, p, 3}, {&rot, r, 4}, {&linvel, lv, 3}, {&angvel, av, 3}};
+    for (const Item& it : items) {
+      if (!it.out) continue;
+      HIPCK(hipMemcpyAsync(h.data(), it.b->p, n * f4b, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipStreamSynchronize(stream));
+      for (int i = 0; i < n; ++i) {
+        it.out[it.w * i + 0] = h[i].x;
+        it.out[it.w * i + 1] = h[i].y;
+        it.out[it.w * i + 2] = h[i].z;
+        if (it.w == 4) it.out[4 * i + 3] = h[i].w;
+      }
+    }
+    return NPHYS_OK;
+  }
+
+  // ------------------------------------------------------------------ broad phase
+  int ensure_bp() {
+    const size_t nn = size_t(std::max(n, 2));
+    for (DBuf* b : {&keys, &vals, &keys_alt, &vals_alt, &cand_count, &cand_off, &visit})
+      HIPCK(b->ensure(nn * 4));
+    HIPCK(child.ensure(2 * nn * 4));
+    HIPCK(parent.ensure(2 * nn * 4));
+    HIPCK(node_mn.ensure(2 * nn * sizeof(float4)));
+    HIPCK(node_mx.ensure(2 * nn * sizeof(float4)));
+    HIPCK(cand.ensure(nn * size_t(cap) * 4));
+    const size_t tw = std::max(radix_temp_words(int(nn)), scan_temp_words(int(nn)));
+    HIPCK(bp_tmp.ensure(tw * 4));
+    return NPHYS_OK;
+  }
+
+  BroadPhaseBufs bp_view() {
+    BroadPhaseBufs v;
+    uint32_t* sm = bp_small.as<uint32_t>();
+    v.changed = sm;
+    v.bounds = sm + 8;
+    v.stats = sm + 16;
+    v.keys = keys.as<uint32_t>();
+    v.vals = vals.as<uint32_t>();
+    v.keys_alt = keys_alt.as<uint32_t>();
+    v.vals_alt = vals_alt.as<uint32_t>();
+    v.child = child.as<int>();
+    v.parent = parent.as<int>();
+    v.visit = visit.as<uint32_t>();
+    v.node_mn = node_mn.as<float4>();
+    v.node_mx = node_mx.as<float4>();
+    v.cand = cand.as<int>();
+    v.cand_count = cand_count.as<uint32_t>();
+    v.cand_off = cand_off.as<uint32_t>();
+    v.pairs = pairs.as<int2>();
+    v.tmp = bp_tmp.as<uint32_t>();
+    return v;
+  }
+
+  // Loose-AABB pass; returns the number of replaced loose AABBs through *changed.
+  int loose_pass(int mode, uint32_t* changed) {
+    BroadPhaseBufs v = bp_view();
+    fill_u32(v.changed, 0u, 1, stream);
+    launch_loose_update(bodies(), mode, v.changed, stream);
+    HIPCK(hipMemcpyAsync(h_small, v.changed, 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    *changed = h_small[0];
+    return NPHYS_OK;
+  }
+
+  int rebuild_pairs() {
+    if (n < 2) {
+      npairs = 0;
+      pairs_dirty = false;
+      return NPHYS_OK;
+    }
+    int rc = ensure_bp();
+    if (rc) return rc;
+    DevBodies b = bodies();
+    BroadPhaseBufs v = bp_view();
+    launch_build_lbvh(b, v, stream);
+    for (;;) {
+      launch_traverse(b, v, cap, stream);
+      HIPCK(hipMemcpyAsync(h_small, v.stats, 8, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipStreamSynchronize(stream));
+      if (h_small[1]) return set_err(NPHYS_ERR_STATE, "BVH traversal stack overflow");
+      if (int(h_small[0]) <= cap) break;
+      cap = int(h_small[0]) + 8;
+      if ((rc = ensure_bp())) return rc;
+      v = bp_view();
+    }
+    launch_pair_offsets(b, v, cap, stream);
+    // total = off[n-1] + min(count[n-1], cap)
+    HIPCK(hipMemcpyAsync(h_small, v.cand_off + (n - 1), 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(h_small + 1, v.keys_alt + (n - 1), 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    npairs = int(h_small[0] + h_small[1]);
+    HIPCK(pairs.ensure(size_t(std::max(npairs, 1)) * sizeof(int2)));
+    v = bp_view();
+    launch_compact_pairs(b, v, cap, stream);
+    pairs_dirty = false;
+    return NPHYS_OK;
+  }
+
+  // ------------------------------------------------------------------ narrow phase
+  int narrow_phase() {
+    ContactBufs& out = cb[cur];
+    const size_t P = size_t(std::max(npairs, 1));
+    for (DBuf* b : {&np_sw1, &np_sw2, &np_sn, &np_slp1, &np_slp2, &np_sln})
+      HIPCK(b->ensure(4 * P * sizeof(float4)));
+    HIPCK(np_sfeat.ensure(4 * P * 4));
+    HIPCK(np_skind.ensure(4 * P * 4));
+    for (DBuf* b : {&np_count, &np_coff, &np_has, &np_moff}) HIPCK(b->ensure(P * 4));
+    HIPCK(np_tmp.ensure(scan_temp_words(int(P)) * 4 + 64));
+    HIPCK(out.ensure(P, 4 * P));
+    NarrowPhaseBufs v;
+    v.sw1 = np_sw1.as<float4>();
+    v.sw2 = np_sw2.as<float4>();
+    v.sn = np_sn.as<float4>();
+    v.sfeat = np_sfeat.as<uint32_t>();
+    v.skind = np_skind.as<int>();
+    v.slp1 = np_slp1.as<float4>();
+    v.slp2 = np_slp2.as<float4>();
+    v.sln = np_sln.as<float4>();
+    v.count = np_count.as<uint32_t>();
+    v.coff = np_coff.as<uint32_t>();
+    v.has = np_has.as<uint32_t>();
+    v.moff = np_moff.as<uint32_t>();
+    v.totals = bp_small.as<uint32_t>() + 24;
+    v.tmp = np_tmp.as<uint32_t>();
+    launch_narrow_phase(bodies(), pairs.as<int2>(), npairs, v, out.view(), stream);
+    HIPCK(hipMemcpyAsync(h_small, v.totals, 8, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    out.C = int(h_small[0]);
+    out.M = int(h_small[1]);
+    return NPHYS_OK;
+  }
+
+  // sync_colliders + perform_broad_phase + perform_narrow_phase on the current poses.
+  int collide_now(int mode) {
+    uint32_t changed = 0;
+    int rc = loose_pass(mode, &changed);
+    if (rc) return rc;
+    if (changed || pairs_dirty) {
+      if ((rc = rebuild_pairs())) return rc;
+    }
+    return narrow_phase();
+  }
+
+  // ------------------------------------------------------------------ solver
+  int ensure_solver(int M, int C) {
+    const size_t nn = size_t(std::max(n, 1));
+    const size_t m = size_t(std::max(M, 1)), c = size_t(std::max(C, 1));
+    for (DBuf* b : {&sv_deg, &sv_adj_off, &sv_adj_fill}) HIPCK(b->ensure(nn * 4));
+    HIPCK(sv_adj.ensure(2 * m * 4));
+    for (DBuf* b : {&sv_state, &sv_ckeys, &sv_order, &sv_ckeys_alt, &sv_order_alt, &sv_mcount_ord, &sv_soff})
+      HIPCK(b->ensure(m * 4));
+    HIPCK(sv_col_start.ensure((kMaxColours + 2) * 4));
+    HIPCK(sv_rec2c.ensure(c * 4));
+    HIPCK(sv_rec.ensure(c * kRecFields * sizeof(float4)));
+    HIPCK(sv_red.ensure(size_t(residual_blocks(M)) * 2 * sizeof(double)));
+    const size_t tw = std::max({radix_temp_words(int(m)), scan_temp_words(int(m)), scan_temp_words(int(nn))});
+    HIPCK(sv_tmp.ensure(tw * 4 + 64));
+    return NPHYS_OK;
+  }
+
+  SolverBufs sv_view() {
+    SolverBufs v;
+    v.deg = sv_deg.as<uint32_t>();
+    v.adj_off = sv_adj_off.as<uint32_t>();
+    v.adj_fill = sv_adj_fill.as<uint32_t>();
+    v.adj = sv_adj.as<int>();
+    v.state = sv_state.as<uint32_t>();
+    v.remaining = bp_small.as<uint32_t>() + 32;
+    v.ckeys = sv_ckeys.as<uint32_t>();
+    v.order = sv_order.as<uint32_t>();
+    v.ckeys_alt = sv_ckeys_alt.as<uint32_t>();
+    v.order_alt = sv_order_alt.as<uint32_t>();
+    v.col_start = sv_col_start.as<uint32_t>();
+    v.mcount_ord = sv_mcount_ord.as<uint32_t>();
+    v.soff = sv_soff.as<uint32_t>();
+    v.rec2c = sv_rec2c.as<uint32_t>();
+    v.rec = sv_rec.as<float4>();
+    v.red = sv_red.as<double>();
+    v.tmp = sv_tmp.as<uint32_t>();
+    return v;
+  }
+
+  int colour(const DevBodies& b, const DevContacts& c, int M, SolverBufs& v) {
+    launch_colour_prepare(b, c, M, v, stream);
+    for (int round = 0;; ++round) {
+      if (round >= 0xffff) return set_err(NPHYS_ERR_STATE, "colouring did not converge");
+      launch_colour_round(b, c, M, round, v, stream);
+      if ((round & 3) == 3 || round < 4) {
+        HIPCK(hipMemcpyAsync(h_small, v.remaining, 4, hipMemcpyDeviceToHost, stream));
+        HIPCK(hipStreamSynchronize(stream));
+        if (h_small[0] == 0) break;
+      }
+    }
+    uint32_t* ncol = bp_small.as<uint32_t>() + 40;
+    launch_colour_keys(M, v, ncol, stream);
+    HIPCK(hipMemcpyAsync(h_small, ncol, 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    ncolours = int(h_small[0]);
+    if (ncolours > kMaxColours) return set_err(NPHYS_ERR_STATE, "too many colours");
+    launch_colour_order(c, M, ncolours, v, stream);
+    col_start.assign(size_t(ncolours) + 1, 0);
+    HIPCK(hipMemcpyAsync(col_start.data(), v.col_start, (size_t(ncolours) + 1) * 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    return NPHYS_OK;
+  }
+
+  int solve() {
+    ContactBufs& cc = cb[cur];
+    ContactBufs& pc = cb[cur ^ 1];
+    const int M = cc.M, C = cc.C;
+    const int prevM = have_solved ? pc.M : 0;
+    solved_M = M;
+    solved_C = C;
+    res_inf = res_rms = 0.0;
+    ncolours = 0;
+    DevBodies b = bodies();
+    if (M == 0) {
+      HIPCK(hipEventRecord(ev[EV_ASM], stream));
+      HIPCK(hipEventRecord(ev[EV_VEL], stream));
+      return NPHYS_OK;
+    }
+    int rc = ensure_solver(M, C);
+    if (rc) return rc;
+    SolverBufs v = sv_view();
+    DevContacts cv = cc.view();
+    if ((rc = colour(b, cv, M, v))) return rc;
+    launch_assemble(b, cv, pc.view(), prevM, M, sp, v, C, stream);
+    HIPCK(hipEventRecord(ev[EV_ASM], stream));
+    for (int k = 0; k < ncolours; ++k)
+      launch_warmstart_colour(b, cv, int(col_start[k]), int(col_start[k + 1] - col_start[k]), v, C, stream);
+    for (int it = 0; it < prm.max_velocity_iterations; ++it)
+      for (int k = 0; k < ncolours; ++k)
+        launch_pgs_colour(b, cv, int(col_start[k]), int(col_start[k + 1] - col_start[k]), v, C, stream);
+    launch_residual(b, cv, M, v, C, stream);
+    launch_cache_impulses(cv, v, C, stream);
+    HIPCK(hipEventRecord(ev[EV_VEL], stream));
+    const int nbk = residual_blocks(M);
+    std::vector<double> red(size_t(nbk) * 2);
+    HIPCK(hipMemcpyAsync(red.data(), v.red, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    double mx = 0.0, ss = 0.0;
+    for (int k = 0; k < nbk; ++k) {
+      mx = std::max(mx, red[2 * k]);
+      ss += red[2 * k + 1];
+    }
+    res_inf = mx;
+    res_rms = C > 0 ? std::sqrt(ss / double(3 * C)) : 0.0;
+    if (keep_colours) {
+      std::vector<uint32_t> st(M);
+      HIPCK(hipMemcpyAsync(st.data(), v.state, size_t(M) * 4, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipStreamSynchronize(stream));
+      last_colours.resize(M);
+      for (int m = 0; m < M; ++m) last_colours[m] = int(st[m] & 0xffffu);
+    }
+    return NPHYS_OK;
+  }
+
+  int position_correction() {
+    ContactBufs& cc = cb[cur];
+    if (cc.M == 0) return NPHYS_OK;
+    DevBodies b = bodies();
+    SolverBufs v = sv_view();
+    DevContacts cv = cc.view();
+    for (int it = 0; it < prm.max_position_iterations; ++it)
+      for (int k = 0; k < ncolours; ++k)
+        launch_position_colour(b, cv, sp, int(col_start[k]), int(col_start[k + 1] - col_start[k]), v, stream);
+    return NPHYS_OK;
+  }
+
+  // ------------------------------------------------------------------ step
+  int step_once() {
+    int rc;
+    HIPCK(hipEventRecord(ev[EV_START], stream));
+    if (moved) {
+      if ((rc = collide_now(0))) return rc;
+      moved = false;
+    }
+    HIPCK(hipEventRecord(ev[EV_PRE_CD], stream));
+    launch_prestep(bodies(), sp, stream);
+    if ((rc = solve())) return rc;
+    launch_integrate(bodies(), sp, stream);
+    HIPCK(hipEventRecord(ev[EV_INT], stream));
+    if ((rc = position_correction())) return rc;
+    HIPCK(hipEventRecord(ev[EV_POS], stream));
+    // this step's manifolds and impulses become the warm-start cache
+    cur ^= 1;
+    have_solved = true;
+    uint32_t ch1 = 0, ch2 = 0;
+    if ((rc = loose_pass(2, &ch1))) return rc;  // CCD-phase sync: swept AABBs
+    if ((rc = loose_pass(1, &ch2))) return rc;  // final sync
+    if (ch1 || ch2 || pairs_dirty) {
+      if ((rc = rebuild_pairs())) return rc;
+    }
+    HIPCK(hipEventRecord(ev[EV_BP], stream));
+    if ((rc = narrow_phase())) return rc;
+    HIPCK(hipEventRecord(ev[EV_NP], stream));
+    HIPCK(hipEventSynchronize(ev[EV_NP]));
+    auto el = [&](int a, int c) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ev[a], ev[c]);
+      return double(ms) * 1e-3;
+    };
+    counters.step += el(EV_START, EV_NP);
+    counters.collision_detection += el(EV_START, EV_PRE_CD) + el(EV_POS, EV_NP);
+    counters.broad_phase += el(EV_POS, EV_BP);
+    counters.narrow_phase += el(EV_BP, EV_NP);
+    counters.assembly += el(EV_PRE_CD, EV_ASM);
+    counters.velocity_resolution += el(EV_ASM, EV_VEL);
+    counters.velocity_update += el(EV_VEL, EV_INT);
+    counters.position_resolution += el(EV_INT, EV_POS);
+    counters.nsteps += 1;
+    counters.npairs = npairs;
+    counters.ncontacts = solved_C;
+    counters.nmanifolds = solved_M;
+    counters.nconstraints = int64_t(3) * solved_C;
+    counters.ncolours = ncolours;
+    prm.t += prm.dt;
+    return NPHYS_OK;
+  }
+};
+
+}  // namespace nph
+
+using nph::World;
+using nph::set_err;
+
+struct nphys_world {
+  World w;
+};
+
+extern "C" {
+
+void nphys_params_default(nphys_params* out) {
+  if (!out) return;
+  std::memset(out, 0, sizeof(*out));
+  out->t = 0.0;
+  out->dt = 1.0 / 60.0;
+  out->erp = 0.2;
+  out->warmstart_coeff = 1.0;
+  out->restitution_velocity_threshold = 1.0;
+  out->allowed_linear_error = 0.001;
+  out->allowed_angular_error = 0.001;
+  out->max_linear_correction = 0.2;
+  out->max_angular_correction = 0.2;
+  out->max_stabilization_multiplier = 0.2;
+  out->max_velocity_iterations = 8;
+  out->max_position_iterations = 3;
+  out->max_ccd_position_iterations = 10;
+  out->max_ccd_substeps = 1;
+  out->gravity[0] = 0.0;
+  out->gravity[1] = -9.81;
+  out->gravity[2] = 0.0;
+}
+
+static int check_params(const nphys_params* p) {
+  if (!p) return set_err(NPHYS_ERR_INVALID, "null params");
+  if (!(p->dt >= 0.0)) return set_err(NPHYS_ERR_INVALID, "dt must be non-negative");
+  if (p->max_velocity_iterations < 0 || p->max_position_iterations < 0)
+    return set_err(NPHYS_ERR_INVALID, "negative iteration count");
+  return NPHYS_OK;
+}
+
+int nphys_world_create(const nphys_params* params, nphys_world** out) {
+  if (!out) return set_err(NPHYS_ERR_INVALID, "null out");
+  *out = nullptr;
+  int rc = check_params(params);
+  if (rc) return rc;
+  nphys_world* w = new nphys_world();
+  rc = w->w.init(*params);
+  if (rc) {
+    delete w;
+    return rc;
+  }
+  *out = w;
+  return NPHYS_OK;
+}
+
+int nphys_world_destroy(nphys_world* w) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  if (w->w.stream) (void)hipStreamSynchronize(w->w.stream);
+  delete w;
+  return NPHYS_OK;
+}
+
+int nphys_set_params(nphys_world* w, const nphys_params* params) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  const int rc = check_params(params);
+  if (rc) return rc;
+  w->w.set_params(*params);
+  return NPHYS_OK;
+}
+
+int nphys_get_params(const nphys_world* w, nphys_params* out) {
+  if (!w || !out) return set_err(NPHYS_ERR_INVALID, "null argument");
+  *out = w->w.prm;
+  return NPHYS_OK;
+}
+
+int nphys_add_bodies(nphys_world* w, const nphys_body_desc* desc, int32_t* first_handle) {
+  if (!w || !desc) return set_err(NPHYS_ERR_INVALID, "null argument");
+  return w->w.add_bodies(*desc, first_handle);
+}
+
+int nphys_num_bodies(const nphys_world* w, int32_t* n) {
+  if (!w || !n) return set_err(NPHYS_ERR_INVALID, "null argument");
+  *n = w->w.n;
+  return NPHYS_OK;
+}
+
+int nphys_set_state(nphys_world* w, const float* pos, const float* rot, const float* linvel,
+                    const float* angvel) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  return w->w.set_state(pos, rot, linvel, angvel);
+}
+
+int nphys_get_state(nphys_world* w, float* pos, float* rot, float* linvel, float* angvel) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  return w->w.get_state(pos, rot, linvel, angvel);
+}
+
+int nphys_step(nphys_world* w, int32_t nsteps) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  if (nsteps < 0) return set_err(NPHYS_ERR_INVALID, "negative step count");
+  for (int i = 0; i < nsteps; ++i) {
+    const int rc = w->w.step_once();
+    if (rc) return rc;
+  }
+  return NPHYS_OK;
+}
+
+int nphys_collide(nphys_world* w) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  const int rc = w->w.collide_now(0);
+  if (rc) return rc;
+  w->w.moved = false;
+  if (hipStreamSynchronize(w->w.stream) != hipSuccess) return set_err(NPHYS_ERR_HIP, "sync failed");
+  return NPHYS_OK;
+}
+
+int nphys_contact_model_set(nphys_world* w, int32_t model) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  if (model != NPHYS_CONTACT_MODEL_SIGNORINI_COULOMB_PYRAMID && model != NPHYS_CONTACT_MODEL_LEARNED_WARMSTART)
+    return set_err(NPHYS_ERR_INVALID, "unknown contact model");
+  w->w.contact_model = model;
+  return NPHYS_OK;
+}
+
+int nphys_num_pairs(nphys_world* w, int64_t* n) {
+  if (!w || !n) return set_err(NPHYS_ERR_INVALID, "null argument");
+  *n = w->w.npairs;
+  return NPHYS_OK;
+}
+
+int nphys_read_pairs(nphys_world* w, int32_t* pairs, int64_t cap) {
+  if (!w || (!pairs && w->w.npairs)) return set_err(NPHYS_ERR_INVALID, "null argument");
+  if (cap < w->w.npairs) return set_err(NPHYS_ERR_CAPACITY, "pair buffer too small");
+  if (w->w.npairs == 0) return NPHYS_OK;
+  if (hipMemcpy(pairs, w->w.pairs.p, size_t(w->w.npairs) * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess)
+    return set_err(NPHYS_ERR_HIP, "pair copy failed");
+  return NPHYS_OK;
+}
+
+int nphys_num_contacts(nphys_world* w, int64_t* n) {
+  if (!w || !n) return set_err(NPHYS_ERR_INVALID, "null argument");
+  *n = w->w.cb[w->w.cur].C;
+  return NPHYS_OK;
+}
+
+int nphys_read_contacts(nphys_world* w, int32_t* bodies, uint32_t* feature, float* world1,
+                        float* world2, float* normal, float* depth, int64_t cap) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  nph::ContactBufs& c = w->w.cb[w->w.cur];
+  const int C = c.C, M = c.M;
+  if (cap < C) return set_err(NPHYS_ERR_CAPACITY, "contact buffer too small");
+  if (C == 0) return NPHYS_OK;
+  std::vector<int2> mb(M);
+  std::vector<int> mf(M), mc(M);
+  std::vector<float4> a(C), b(C), n(C);
+  std::vector<uint32_t> f(C);
+  if (hipMemcpy(mb.data(), c.mbody.p, M * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(mf.data(), c.mfirst.p, M * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(mc.data(), c.mcount.p, M * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(a.data(), c.w1.p, C * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(b.data(), c.w2.p, C * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(n.data(), c.n.p, C * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(f.data(), c.feature.p, C * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return set_err(NPHYS_ERR_HIP, "contact copy failed");
+  for (int m = 0; m < M; ++m)
+    for (int k = 0; k < mc[m]; ++k) {
+      const int i = mf[m] + k;
+      if (bodies) { bodies[2 * i] = mb[m].x; bodies[2 * i + 1] = mb[m].y; }
+    }
+  for (int i = 0; i < C; ++i) {
+    if (feature) feature[i] = f[i];
+    if (world1) { world1[3 * i] = a[i].x; world1[3 * i + 1] = a[i].y; world1[3 * i + 2] = a[i].z; }
+    if (world2) { world2[3 * i] = b[i].x; world2[3 * i + 1] = b[i].y; world2[3 * i + 2] = b[i].z; }
+    if (normal) { normal[3 * i] = n[i].x; normal[3 * i + 1] = n[i].y; normal[3 * i + 2] = n[i].z; }
+    if (depth) depth[i] = a[i].w;
+  }
+  return NPHYS_OK;
+}
+
+int nphys_num_solved_contacts(nphys_world* w, int64_t* n) {
+  if (!w || !n) return set_err(NPHYS_ERR_INVALID, "null argument");
+  *n = w->w.have_solved ? w->w.cb[w->w.cur ^ 1].C : 0;
+  return NPHYS_OK;
+}
+
+int nphys_read_impulses(nphys_world* w, uint64_t* keys, float* impulses, int64_t cap) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  if (!w->w.have_solved) return NPHYS_OK;
+  nph::ContactBufs& c = w->w.cb[w->w.cur ^ 1];
+  const int C = c.C, M = c.M;
+  if (cap < C) return set_err(NPHYS_ERR_CAPACITY, "impulse buffer too small");
+  if (C == 0) return NPHYS_OK;
+  std::vector<float4> imp(C);
+  std::vector<uint32_t> f(C);
+  std::vector<int2> mb(M);
+  std::vector<int> mf(M), mc(M);
+  if (hipMemcpy(imp.data(), c.imp.p, C * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(f.data(), c.feature.p, C * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(mb.data(), c.mbody.p, M * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(mf.data(), c.mfirst.p, M * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(mc.data(), c.mcount.p, M * 4, hipMemcpyDeviceToHost) != hipSuccess)
+    return set_err(NPHYS_ERR_HIP, "impulse copy failed");
+  for (int m = 0; m < M; ++m)
+    for (int k = 0; k < mc[m]; ++k) {
+      const int i = mf[m] + k;
+      if (keys)
+        keys[i] = (uint64_t(uint32_t(mb[m].x)) << 40) | (uint64_t(uint32_t(mb[m].y)) << 16) | uint64_t(f[i] & 0xffffu);
+    }
+  if (impulses)
+    for (int i = 0; i < C; ++i) {
+      impulses[3 * i] = imp[i].x;
+      impulses[3 * i + 1] = imp[i].y;
+      impulses[3 * i + 2] = imp[i].z;
+    }
+  return NPHYS_OK;
+}
+
+int nphys_residual(nphys_world* w, double* r_inf, double* r_rms) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  if (r_inf) *r_inf = w->w.res_inf;
+  if (r_rms) *r_rms = w->w.res_rms;
+  return NPHYS_OK;
+}
+
+int nphys_read_counters(nphys_world* w, nphys_counters* out) {
+  if (!w || !out) return set_err(NPHYS_ERR_INVALID, "null argument");
+  *out = w->w.counters;
+  return NPHYS_OK;
+}
+
+int nphys_reset_counters(nphys_world* w) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  std::memset(&w->w.counters, 0, sizeof(w->w.counters));
+  return NPHYS_OK;
+}
+
+int nphys_read_colours(nphys_world* w, int32_t* colours, int64_t cap, int32_t* ncolours) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  w->w.keep_colours = true;
+  const int M = int(w->w.last_colours.size());
+  if (ncolours) *ncolours = w->w.ncolours;
+  if (!colours) return NPHYS_OK;
+  if (cap < M) return set_err(NPHYS_ERR_CAPACITY, "colour buffer too small");
+  for (int m = 0; m < M; ++m) colours[m] = w->w.last_colours[m];
+  return NPHYS_OK;
+}
+
+const char* nphys_last_error(void) { return nph::g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,218 @@
+"""Python host mirror of the reference's MechanicalWorld surface over libnphys_hip.so.
+
+The reference (nphysics 0.23, Rust) drives a step as
+``MechanicalWorld::step(&mut GeometricalWorld, &mut bodies, &mut colliders, &mut joints,
+&mut forces)`` (src/world/mechanical_world.rs:182-196); here one MechanicalWorld object owns the
+geometrical world and the body/collider sets of balls and cuboids, on one GPU, and every call
+goes through the C ABI of include/nphys.h. There is no CPU fallback: constructing a world
+without the HIP library or without a GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnphys_hip.so")
+
+_lib = None
+
+EXPORTED = [
+    "nphys_params_default", "nphys_world_create", "nphys_world_destroy", "nphys_set_params",
+    "nphys_get_params", "nphys_add_bodies", "nphys_num_bodies", "nphys_set_state",
+    "nphys_get_state", "nphys_step", "nphys_collide", "nphys_contact_model_set",
+    "nphys_num_pairs", "nphys_read_pairs", "nphys_num_contacts", "nphys_read_contacts",
+    "nphys_num_solved_contacts", "nphys_read_impulses", "nphys_residual", "nphys_read_counters",
+    "nphys_reset_counters", "nphys_read_colours", "nphys_last_error",
+]
+
+
+class NphysError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Loads the HIP library and declares the C signatures; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise NphysError(f"{path} is missing; build it with python -m nphysics_amd.build")
+    L = C.CDLL(path)
+    W = C.c_void_p
+    fp = C.POINTER(C.c_float)
+    ip = C.POINTER(C.c_int32)
+    L.nphys_last_error.restype = C.c_char_p
+    L.nphys_params_default.argtypes = [C.POINTER(abi.Params)]
+    L.nphys_params_default.restype = None
+    L.nphys_world_create.argtypes = [C.POINTER(abi.Params), C.POINTER(W)]
+    L.nphys_world_destroy.argtypes = [W]
+    L.nphys_set_params.argtypes = [W, C.POINTER(abi.Params)]
+    L.nphys_get_params.argtypes = [W, C.POINTER(abi.Params)]
+    L.nphys_add_bodies.argtypes = [W, C.POINTER(abi.BodyDesc), ip]
+    L.nphys_num_bodies.argtypes = [W, ip]
+    L.nphys_set_state.argtypes = [W, fp, fp, fp, fp]
+    L.nphys_get_state.argtypes = [W, fp, fp, fp, fp]
+    L.nphys_step.argtypes = [W, C.c_int32]
+    L.nphys_collide.argtypes = [W]
+    L.nphys_contact_model_set.argtypes = [W, C.c_int32]
+    L.nphys_num_pairs.argtypes = [W, C.POINTER(C.c_int64)]
+    L.nphys_read_pairs.argtypes = [W, ip, C.c_int64]
+    L.nphys_num_contacts.argtypes = [W, C.POINTER(C.c_int64)]
+    L.nphys_read_contacts.argtypes = [W, ip, C.POINTER(C.c_uint32), fp, fp, fp, fp, C.c_int64]
+    L.nphys_num_solved_contacts.argtypes = [W, C.POINTER(C.c_int64)]
+    L.nphys_read_impulses.argtypes = [W, C.POINTER(C.c_uint64), fp, C.c_int64]
+    L.nphys_residual.argtypes = [W, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.nphys_read_counters.argtypes = [W, C.POINTER(abi.Counters)]
+    L.nphys_reset_counters.argtypes = [W]
+    L.nphys_read_colours.argtypes = [W, ip, C.c_int64, ip]
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != 0:
+        raise NphysError(f"nphys error {rc}: {_lib.nphys_last_error().decode()}")
+
+
+def default_params(**kw):
+    return abi.default_params(**kw)
+
+
+class MechanicalWorld:
+    """One world on the current HIP device (MechanicalWorld + GeometricalWorld + sets)."""
+
+    def __init__(self, gravity=(0.0, -9.81, 0.0), integration_parameters=None):
+        self._L = load_library()
+        p = integration_parameters if integration_parameters is not None else abi.default_params()
+        p.gravity[0], p.gravity[1], p.gravity[2] = gravity
+        self._h = C.c_void_p()
+        _check(self._L.nphys_world_create(C.byref(p), C.byref(self._h)))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.nphys_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- parameters (integration_parameters / gravity pub fields, set_timestep)
+    @property
+    def integration_parameters(self):
+        p = abi.Params()
+        _check(self._L.nphys_get_params(self._h, C.byref(p)))
+        return p
+
+    @integration_parameters.setter
+    def integration_parameters(self, p):
+        _check(self._L.nphys_set_params(self._h, C.byref(p)))
+
+    def timestep(self):
+        return self.integration_parameters.dt
+
+    def set_timestep(self, dt):
+        p = self.integration_parameters
+        p.dt = dt
+        self.integration_parameters = p
+
+    def set_contact_model(self, model):
+        _check(self._L.nphys_contact_model_set(self._h, model))
+
+    # -- scene
+    def add_bodies(self, scene):
+        desc, keep = abi.make_body_desc(scene)
+        first = C.c_int32()
+        _check(self._L.nphys_add_bodies(self._h, C.byref(desc), C.byref(first)))
+        return first.value
+
+    @property
+    def num_bodies(self):
+        n = C.c_int32()
+        _check(self._L.nphys_num_bodies(self._h, C.byref(n)))
+        return n.value
+
+    def set_state(self, pos=None, rot=None, linvel=None, angvel=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (pos, rot, linvel, angvel)]
+        ptrs = [None if a is None else abi.fptr(a) for a in arrs]
+        _check(self._L.nphys_set_state(self._h, *ptrs))
+
+    def get_state(self):
+        n = self.num_bodies
+        pos = np.zeros((n, 3), np.float32)
+        rot = np.zeros((n, 4), np.float32)
+        lv = np.zeros((n, 3), np.float32)
+        av = np.zeros((n, 3), np.float32)
+        _check(self._L.nphys_get_state(self._h, abi.fptr(pos), abi.fptr(rot), abi.fptr(lv), abi.fptr(av)))
+        return {"pos": pos, "rot": rot, "linvel": lv, "angvel": av}
+
+    # -- stepping
+    def step(self, nsteps=1):
+        _check(self._L.nphys_step(self._h, nsteps))
+
+    def collide(self):
+        _check(self._L.nphys_collide(self._h))
+
+    # -- readback
+    def pairs(self):
+        n = C.c_int64()
+        _check(self._L.nphys_num_pairs(self._h, C.byref(n)))
+        out = np.zeros((n.value, 2), np.int32)
+        _check(self._L.nphys_read_pairs(self._h, abi.iptr(out), n.value))
+        return out
+
+    def contacts(self):
+        n = C.c_int64()
+        _check(self._L.nphys_num_contacts(self._h, C.byref(n)))
+        c = n.value
+        d = {
+            "bodies": np.zeros((c, 2), np.int32),
+            "feature": np.zeros(c, np.uint32),
+            "world1": np.zeros((c, 3), np.float32),
+            "world2": np.zeros((c, 3), np.float32),
+            "normal": np.zeros((c, 3), np.float32),
+            "depth": np.zeros(c, np.float32),
+        }
+        _check(self._L.nphys_read_contacts(self._h, abi.iptr(d["bodies"]), abi.u32ptr(d["feature"]),
+                                           abi.fptr(d["world1"]), abi.fptr(d["world2"]),
+                                           abi.fptr(d["normal"]), abi.fptr(d["depth"]), c))
+        return d
+
+    def impulses(self):
+        n = C.c_int64()
+        _check(self._L.nphys_num_solved_contacts(self._h, C.byref(n)))
+        keys = np.zeros(n.value, np.uint64)
+        imp = np.zeros((n.value, 3), np.float32)
+        _check(self._L.nphys_read_impulses(self._h, abi.u64ptr(keys), abi.fptr(imp), n.value))
+        return keys, imp
+
+    def residual(self):
+        a, b = C.c_double(), C.c_double()
+        _check(self._L.nphys_residual(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def counters(self):
+        c = abi.Counters()
+        _check(self._L.nphys_read_counters(self._h, C.byref(c)))
+        return c.as_dict()
+
+    def reset_counters(self):
+        _check(self._L.nphys_reset_counters(self._h))
+
+    def record_colours(self):
+        """Starts recording the manifold colours of every following step."""
+        nc = C.c_int32()
+        _check(self._L.nphys_read_colours(self._h, None, 0, C.byref(nc)))
+
+    def colours(self):
+        """Colours of the manifolds solved by the last step (needs record_colours() before it)."""
+        m = self.counters()["nmanifolds"]
+        out = np.zeros(max(m, 1), np.int32)
+        nc = C.c_int32()
+        _check(self._L.nphys_read_colours(self._h, abi.iptr(out), out.shape[0], C.byref(nc)))
+        return out[:m], nc.value

@@ -1,0 +1,387 @@
+// oracle/orc_capi.cpp -- TEST INFRASTRUCTURE ONLY (the oracle).
+//
+// C entry points of the CPU restatement (liboracle.so), loaded with ctypes by tests/ and by
+// bench.py's cpu_baseline leg only. They mirror the product ABI in include/nphys.h (same
+// parameter/body structs, same call order) with an `orc_` prefix, a precision switch (FP32 or
+// FP64, nphysics being generic over RealField) and a row-order switch (reference order or the
+// GPU's colour order). Read-back is always in double.
+#include <cstring>
+#include <string>
+#include <type_traits>
+
+#include "../include/nphys.h"
+#include "orc_world.hpp"
+
+namespace {
+
+struct AnyWorld {
+  virtual ~AnyWorld() {}
+  virtual int precision() const = 0;
+};
+
+template <class T>
+struct TWorld : AnyWorld {
+  orc::World<T> w;
+  int precision() const override { return sizeof(T) == 4 ? 32 : 64; }
+};
+
+thread_local std::string g_err;
+
+int fail(int code, const char* msg) {
+  g_err = msg;
+  return code;
+}
+
+template <class T>
+void apply_params(orc::World<T>& w, const nphys_params* p) {
+  w.prm.t = T(p->t);
+  w.prm.set_dt(T(p->dt));
+  w.prm.erp = T(p->erp);
+  w.prm.warmstart_coeff = T(p->warmstart_coeff);
+  w.prm.restitution_velocity_threshold = T(p->restitution_velocity_threshold);
+  w.prm.allowed_linear_error = T(p->allowed_linear_error);
+  w.prm.allowed_angular_error = T(p->allowed_angular_error);
+  w.prm.max_linear_correction = T(p->max_linear_correction);
+  w.prm.max_angular_correction = T(p->max_angular_correction);
+  w.prm.max_stabilization_multiplier = T(p->max_stabilization_multiplier);
+  w.prm.max_velocity_iterations = p->max_velocity_iterations;
+  w.prm.max_position_iterations = p->max_position_iterations;
+  w.prm.max_ccd_position_iterations = p->max_ccd_position_iterations;
+  w.prm.max_ccd_substeps = p->max_ccd_substeps;
+  w.gravity = orc::V3<T>{T(p->gravity[0]), T(p->gravity[1]), T(p->gravity[2])};
+}
+
+template <class F>
+int with_world(void* h, F&& f) {
+  if (!h) return fail(NPHYS_ERR_INVALID, "null world");
+  AnyWorld* a = static_cast<AnyWorld*>(h);
+  if (a->precision() == 32) return f(static_cast<TWorld<float>*>(a)->w);
+  return f(static_cast<TWorld<double>*>(a)->w);
+}
+
+template <class T, class S>
+orc::V3<T> v3_of(const S* p) { return orc::V3<T>{T(p[0]), T(p[1]), T(p[2])}; }
+template <class T, class S>
+orc::Q<T> q_of(const S* q) { return orc::Q<T>{T(q[0]), T(q[1]), T(q[2]), T(q[3])}; }
+template <class T>
+void put3(double* o, orc::V3<T> v) {
+  o[0] = double(v.x); o[1] = double(v.y); o[2] = double(v.z);
+}
+
+template <class T>
+int add_bodies_t(orc::World<T>& w, const nphys_body_desc* d, int32_t* first) {
+  if (!d->status || !d->shape || !d->dims || !d->pos || !d->rot || !d->density || !d->margin ||
+      !d->linear_prediction || !d->friction || !d->restitution)
+    return fail(NPHYS_ERR_INVALID, "missing body array");
+  if (first) *first = int32_t(w.bodies.size());
+  for (int i = 0; i < d->n; ++i) {
+    if (d->status[i] != orc::STATUS_DYNAMIC && d->status[i] != orc::STATUS_STATIC)
+      return fail(NPHYS_ERR_INVALID, "bad status");
+    if (d->shape[i] != orc::SHAPE_BALL && d->shape[i] != orc::SHAPE_CUBOID)
+      return fail(NPHYS_ERR_INVALID, "bad shape");
+    const orc::V3<T> zero{T(0), T(0), T(0)};
+    const orc::V3<T> lv = d->linvel ? v3_of<T>(d->linvel + 3 * i) : zero;
+    const orc::V3<T> av = d->angvel ? v3_of<T>(d->angvel + 3 * i) : zero;
+    w.add_body(d->status[i], d->shape[i], v3_of<T>(d->dims + 3 * i),
+               orc::Iso<T>{v3_of<T>(d->pos + 3 * i), q_of<T>(d->rot + 4 * i)}, lv, av,
+               T(d->density[i]), T(d->margin[i]), T(d->linear_prediction[i]), T(d->friction[i]),
+               T(d->restitution[i]));
+  }
+  return NPHYS_OK;
+}
+
+template <class T, class S>
+int set_state_t(orc::World<T>& w, const S* pos, const S* rot, const S* lv, const S* av) {
+  for (size_t i = 0; i < w.bodies.size(); ++i) {
+    orc::Body<T>& b = w.bodies[i];
+    orc::Iso<T> p = b.pos;
+    if (pos) p.t = v3_of<T>(pos + 3 * i);
+    if (rot) p.q = q_of<T>(rot + 4 * i);
+    w.set_state(int(i), p, lv ? v3_of<T>(lv + 3 * i) : b.linvel, av ? v3_of<T>(av + 3 * i) : b.angvel);
+  }
+  return NPHYS_OK;
+}
+
+template <class T>
+int get_state_t(orc::World<T>& w, double* pos, double* rot, double* lv, double* av) {
+  for (size_t i = 0; i < w.bodies.size(); ++i) {
+    const orc::Body<T>& b = w.bodies[i];
+    if (pos) put3(pos + 3 * i, b.pos.t);
+    if (rot) {
+      rot[4 * i + 0] = double(b.pos.q.i);
+      rot[4 * i + 1] = double(b.pos.q.j);
+      rot[4 * i + 2] = double(b.pos.q.k);
+      rot[4 * i + 3] = double(b.pos.q.w);
+    }
+    if (lv) put3(lv + 3 * i, b.linvel);
+    if (av) put3(av + 3 * i, b.angvel);
+  }
+  return NPHYS_OK;
+}
+
+template <class T>
+int read_contacts_t(orc::World<T>& w, int32_t* bodies, uint32_t* feature, double* w1, double* w2,
+                    double* n, double* depth, int32_t* kind, double* lp1, double* lp2, double* ln,
+                    int64_t cap) {
+  if (int64_t(w.contacts.size()) > cap) return fail(NPHYS_ERR_CAPACITY, "contact buffer too small");
+  int64_t c = 0;
+  for (const auto& m : w.manifolds) {
+    for (int k = 0; k < m.count; ++k, ++c) {
+      const orc::Contact<T>& ct = w.contacts[m.first + k];
+      if (bodies) { bodies[2 * c] = m.b1; bodies[2 * c + 1] = m.b2; }
+      if (feature) feature[c] = ct.feature;
+      if (w1) put3(w1 + 3 * c, ct.world1);
+      if (w2) put3(w2 + 3 * c, ct.world2);
+      if (n) put3(n + 3 * c, ct.normal);
+      if (depth) depth[c] = double(ct.depth);
+      if (kind) kind[c] = ct.kind;
+      if (lp1) put3(lp1 + 3 * c, ct.lp1);
+      if (lp2) put3(lp2 + 3 * c, ct.lp2);
+      if (ln) put3(ln + 3 * c, ct.ln);
+    }
+  }
+  return NPHYS_OK;
+}
+
+template <class T>
+int read_impulses_t(orc::World<T>& w, uint64_t* keys, double* imp, int64_t cap) {
+  if (int64_t(w.crow.size()) > cap) return fail(NPHYS_ERR_CAPACITY, "impulse buffer too small");
+  for (size_t i = 0; i < w.crow.size(); ++i) {
+    const orc::ContactRows& cr = w.crow[i];
+    const auto& nl = cr.ground ? w.uni_g : w.uni;
+    const auto& bl = cr.ground ? w.bil_g : w.bil;
+    if (keys) keys[i] = nl[cr.normal].key;
+    if (imp) {
+      imp[3 * i + 0] = double(nl[cr.normal].impulse);
+      imp[3 * i + 1] = double(bl[cr.fric[0]].impulse);
+      imp[3 * i + 2] = double(bl[cr.fric[1]].impulse);
+    }
+  }
+  return NPHYS_OK;
+}
+
+template <class T>
+int read_body_dynamics_t(orc::World<T>& w, double* inv_mass, double* inv_aug, double* aabb) {
+  for (size_t i = 0; i < w.bodies.size(); ++i) {
+    const orc::Body<T>& b = w.bodies[i];
+    if (inv_mass) inv_mass[i] = double(b.inv_mass);
+    if (inv_aug)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) inv_aug[9 * i + 3 * r + c] = double(b.inv_aug.m[r][c]);
+    if (aabb) {
+      put3(aabb + 6 * i, b.loose.mn);
+      put3(aabb + 6 * i + 3, b.loose.mx);
+    }
+  }
+  return NPHYS_OK;
+}
+
+template <class T>
+int collide_shapes_t(int k1, const double* d1, const double* p1, int k2, const double* d2,
+                     const double* p2, double prediction, uint32_t* feature, double* w1,
+                     double* w2, double* n, double* depth, int32_t* kind) {
+  orc::ShapeDesc<T> s1{k1, k1 == orc::SHAPE_BALL ? orc::V3<T>{T(d1[0]), T(d1[0]), T(d1[0])} : v3_of<T>(d1)};
+  orc::ShapeDesc<T> s2{k2, k2 == orc::SHAPE_BALL ? orc::V3<T>{T(d2[0]), T(d2[0]), T(d2[0])} : v3_of<T>(d2)};
+  orc::Iso<T> i1{v3_of<T>(p1), q_of<T>(p1 + 3)}, i2{v3_of<T>(p2), q_of<T>(p2 + 3)};
+  orc::Contact<T> buf[8];
+  const int nc = orc::collide_pair(s1, i1, s2, i2, T(prediction), buf);
+  for (int c = 0; c < nc; ++c) {
+    if (feature) feature[c] = buf[c].feature;
+    if (w1) put3(w1 + 3 * c, buf[c].world1);
+    if (w2) put3(w2 + 3 * c, buf[c].world2);
+    if (n) put3(n + 3 * c, buf[c].normal);
+    if (depth) depth[c] = double(buf[c].depth);
+    if (kind) kind[c] = buf[c].kind;
+  }
+  return nc;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error(void) { return g_err.c_str(); }
+
+int orc_world_create(const nphys_params* p, int32_t precision, void** out) {
+  if (!p || !out) return fail(NPHYS_ERR_INVALID, "null argument");
+  if (p->dt < 0) return fail(NPHYS_ERR_INVALID, "negative dt");
+  if (precision == 32) {
+    auto* w = new TWorld<float>();
+    apply_params(w->w, p);
+    *out = static_cast<AnyWorld*>(w);
+  } else if (precision == 64) {
+    auto* w = new TWorld<double>();
+    apply_params(w->w, p);
+    *out = static_cast<AnyWorld*>(w);
+  } else {
+    return fail(NPHYS_ERR_INVALID, "precision must be 32 or 64");
+  }
+  return NPHYS_OK;
+}
+
+int orc_world_destroy(void* h) {
+  delete static_cast<AnyWorld*>(h);
+  return NPHYS_OK;
+}
+
+int orc_set_params(void* h, const nphys_params* p) {
+  if (!p) return fail(NPHYS_ERR_INVALID, "null params");
+  if (p->dt < 0) return fail(NPHYS_ERR_INVALID, "negative dt");
+  return with_world(h, [&](auto& w) { apply_params(w, p); return NPHYS_OK; });
+}
+
+int orc_set_order_mode(void* h, int32_t mode) {
+  if (mode != orc::ORDER_REFERENCE && mode != orc::ORDER_COLOURED)
+    return fail(NPHYS_ERR_INVALID, "bad order mode");
+  return with_world(h, [&](auto& w) { w.order_mode = mode; return NPHYS_OK; });
+}
+
+int orc_set_brute_force_limit(void* h, int32_t n) {
+  return with_world(h, [&](auto& w) { w.brute_force_limit = n; return NPHYS_OK; });
+}
+
+int orc_add_bodies(void* h, const nphys_body_desc* d, int32_t* first) {
+  if (!d || d->n < 0) return fail(NPHYS_ERR_INVALID, "bad body desc");
+  return with_world(h, [&](auto& w) { return add_bodies_t(w, d, first); });
+}
+
+int orc_num_bodies(void* h, int32_t* n) {
+  return with_world(h, [&](auto& w) { *n = int32_t(w.bodies.size()); return NPHYS_OK; });
+}
+
+int orc_set_state(void* h, const float* pos, const float* rot, const float* lv, const float* av) {
+  return with_world(h, [&](auto& w) { return set_state_t(w, pos, rot, lv, av); });
+}
+
+int orc_set_state_f64(void* h, const double* pos, const double* rot, const double* lv,
+                      const double* av) {
+  return with_world(h, [&](auto& w) { return set_state_t(w, pos, rot, lv, av); });
+}
+
+int orc_get_state(void* h, double* pos, double* rot, double* lv, double* av) {
+  return with_world(h, [&](auto& w) { return get_state_t(w, pos, rot, lv, av); });
+}
+
+int orc_step(void* h, int32_t nsteps) {
+  if (nsteps < 0) return fail(NPHYS_ERR_INVALID, "negative step count");
+  return with_world(h, [&](auto& w) {
+    for (int i = 0; i < nsteps; ++i) w.step();
+    return NPHYS_OK;
+  });
+}
+
+int orc_collide(void* h) {
+  return with_world(h, [&](auto& w) {
+    w.collide();
+    return NPHYS_OK;
+  });
+}
+
+int orc_num_pairs(void* h, int64_t* n) {
+  return with_world(h, [&](auto& w) { *n = int64_t(w.pairs.size()); return NPHYS_OK; });
+}
+
+int orc_read_pairs(void* h, int32_t* pairs, int64_t cap) {
+  return with_world(h, [&](auto& w) {
+    if (int64_t(w.pairs.size()) > cap) return fail(NPHYS_ERR_CAPACITY, "pair buffer too small");
+    for (size_t i = 0; i < w.pairs.size(); ++i) {
+      pairs[2 * i] = w.pairs[i].first;
+      pairs[2 * i + 1] = w.pairs[i].second;
+    }
+    return NPHYS_OK;
+  });
+}
+
+int orc_num_contacts(void* h, int64_t* n) {
+  return with_world(h, [&](auto& w) { *n = int64_t(w.contacts.size()); return NPHYS_OK; });
+}
+
+int orc_read_contacts(void* h, int32_t* bodies, uint32_t* feature, double* w1, double* w2,
+                      double* n, double* depth, int32_t* kind, double* lp1, double* lp2,
+                      double* ln, int64_t cap) {
+  return with_world(h, [&](auto& w) {
+    return read_contacts_t(w, bodies, feature, w1, w2, n, depth, kind, lp1, lp2, ln, cap);
+  });
+}
+
+int orc_num_solved_contacts(void* h, int64_t* n) {
+  return with_world(h, [&](auto& w) { *n = int64_t(w.crow.size()); return NPHYS_OK; });
+}
+
+int orc_read_impulses(void* h, uint64_t* keys, double* imp, int64_t cap) {
+  return with_world(h, [&](auto& w) { return read_impulses_t(w, keys, imp, cap); });
+}
+
+int orc_residual(void* h, double* rinf, double* rrms) {
+  return with_world(h, [&](auto& w) {
+    *rinf = double(w.res_inf);
+    *rrms = double(w.res_rms);
+    return NPHYS_OK;
+  });
+}
+
+int orc_read_counters(void* h, nphys_counters* c) {
+  return with_world(h, [&](auto& w) {
+    std::memset(c, 0, sizeof(*c));
+    c->step = w.counters.step;
+    c->collision_detection = w.counters.collision_detection;
+    c->broad_phase = w.counters.broad_phase;
+    c->narrow_phase = w.counters.narrow_phase;
+    c->assembly = w.counters.assembly;
+    c->velocity_resolution = w.counters.velocity_resolution;
+    c->velocity_update = w.counters.velocity_update;
+    c->position_resolution = w.counters.position_resolution;
+    c->nsteps = w.counters.nsteps;
+    c->npairs = int64_t(w.pairs.size());
+    c->ncontacts = int64_t(w.crow.size());
+    c->nmanifolds = int64_t(w.solved.size());
+    c->nconstraints = w.n_rows;
+    c->ncolours = w.ncolours;
+    return NPHYS_OK;
+  });
+}
+
+int orc_reset_counters(void* h) {
+  return with_world(h, [&](auto& w) {
+    w.counters = orc::Counters();
+    return NPHYS_OK;
+  });
+}
+
+int orc_read_colours(void* h, int32_t* colours, int64_t cap, int32_t* ncol) {
+  return with_world(h, [&](auto& w) {
+    if (int64_t(w.colours.size()) > cap) return fail(NPHYS_ERR_CAPACITY, "colour buffer too small");
+    for (size_t i = 0; i < w.colours.size(); ++i) colours[i] = w.colours[i];
+    *ncol = w.ncolours;
+    return NPHYS_OK;
+  });
+}
+
+int orc_read_body_dynamics(void* h, double* inv_mass, double* inv_aug, double* loose_aabb) {
+  return with_world(h, [&](auto& w) { return read_body_dynamics_t(w, inv_mass, inv_aug, loose_aabb); });
+}
+
+// Stand-alone narrow phase on one shape pair; pose = (x, y, z, i, j, k, w). Returns the number of
+// contacts (<= 4) written, or a negative error.
+int orc_collide_shapes(int32_t precision, int32_t k1, const double* d1, const double* p1,
+                       int32_t k2, const double* d2, const double* p2, double prediction,
+                       uint32_t* feature, double* w1, double* w2, double* n, double* depth,
+                       int32_t* kind) {
+  if (precision == 32)
+    return collide_shapes_t<float>(k1, d1, p1, k2, d2, p2, prediction, feature, w1, w2, n, depth, kind);
+  return collide_shapes_t<double>(k1, d1, p1, k2, d2, p2, prediction, feature, w1, w2, n, depth, kind);
+}
+
+// The deterministic manifold colouring on its own (colours[m] for manifold m).
+int orc_colour_manifolds(int32_t nman, const int32_t* b1, const int32_t* b2, int32_t nbodies,
+                         const int32_t* is_dynamic, int32_t* colours) {
+  std::vector<orc::Manifold> ms(nman);
+  for (int m = 0; m < nman; ++m) ms[m] = orc::Manifold{b1[m], b2[m], 0, 1};
+  std::vector<int> dyn(is_dynamic, is_dynamic + nbodies);
+  std::vector<int> col;
+  const int nc = orc::colour_manifolds(ms, dyn, nbodies, col);
+  for (int m = 0; m < nman; ++m) colours[m] = col[m];
+  return nc;
+}
+
+}  // extern "C"

@@ -1,0 +1,919 @@
+// oracle/orc_world.hpp -- TEST INFRASTRUCTURE ONLY (the oracle).
+//
+// A CPU restatement of one nphysics 0.23 MechanicalWorld::step over rigid balls and cuboids,
+// templated on the scalar type (nphysics is generic over `N: RealField`). It is the checker for
+// the HIP pipeline and the timed CPU baseline; nothing in the product library uses it.
+//
+// Stage order follows src/world/mechanical_world.rs:199-396 (step_with_filter):
+//   step_started + update_dynamics (:230-234), update_acceleration (:241-243), pre-solve
+//   sync/broad/narrow phase (:250-253), manifold collection (:287-300), MoreauJeanSolver::step
+//   (:316-326), post-solve update_dynamics (:343-346), the no-op CCD sync + broad phase
+//   (:604-606, swept AABBs), validate_advancement (:364-367), the real CD of the step (:375-385).
+// Solver: src/solver/moreau_jean_solver.rs:47-90,129-347; contact model
+//   src/solver/signorini_coulomb_pyramid_model.rs:56-261, src/solver/signorini_model.rs:37-197,
+//   src/solver/helper.rs:53-135, src/object/rigid_body.rs:672-722 (fill_constraint_geometry);
+//   SOR-Prox src/solver/sor_prox.rs:19-436; position correction
+//   src/solver/nonlinear_sor_prox.rs:17-55,121-309.
+// Bodies: src/object/rigid_body.rs:305-315 (set_position), :371-381 (apply_displacement),
+//   :467-505 (integrate), :558-619 (update_dynamics / update_acceleration), :747-766
+//   (add_local_inertia_and_com); mass properties src/volumetric/volumetric_ball.rs:9-44,
+//   src/volumetric/volumetric_cuboid.rs:10-75; material combine src/material/material.rs:72-178.
+//
+// Two row orders are provided for the velocity and position sweeps:
+//   ORDER_REFERENCE -- the reference's: friction rows (non-ground, then ground), then normal rows
+//                      (non-ground, then ground), manifolds in pair order (sor_prox.rs:159-179);
+//   ORDER_COLOURED  -- the same rows visited manifold by manifold in the deterministic greedy
+//                      colouring order the GPU uses (friction rows of the manifold's contacts,
+//                      then its normal rows). Used to check the GPU row arithmetic tightly.
+#pragma once
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+
+#include "orc_geometry.hpp"
+
+namespace orc {
+
+enum BodyStatus : int { STATUS_DYNAMIC = 0, STATUS_STATIC = 1 };
+enum OrderMode : int { ORDER_REFERENCE = 0, ORDER_COLOURED = 1 };
+
+// BodyUpdateStatus flags (src/object/body.rs:450-527).
+enum : unsigned {
+  F_POSITION = 1u,
+  F_VELOCITY = 2u,
+  F_LOCAL_INERTIA = 4u,
+  F_LOCAL_COM = 8u,
+  F_DAMPING = 16u,
+  F_STATUS = 32u,
+  F_ALL = 63u
+};
+
+template <class T>
+struct Params {  // IntegrationParameters (src/solver/integration_parameters.rs:5-76)
+  T t, dt, inv_dt;
+  T erp, warmstart_coeff, restitution_velocity_threshold;
+  T allowed_linear_error, allowed_angular_error, max_linear_correction, max_angular_correction;
+  T max_stabilization_multiplier;
+  int max_velocity_iterations, max_position_iterations, max_ccd_position_iterations;
+  int max_ccd_substeps;
+  void set_dt(T d) {
+    dt = d;
+    inv_dt = d == T(0) ? T(0) : T(1) / d;
+  }
+};
+
+template <class T>
+struct Body {
+  int status, shape;
+  ShapeDesc<T> sd;
+  T density, margin, lin_pred, friction, restitution;
+  Iso<T> pos, pos0;
+  V3<T> linvel, angvel;
+  V3<T> acc_lin, acc_ang;
+  T mass;
+  M3<T> local_I, I_world, aug, inv_aug;
+  T inv_mass;
+  V3<T> com;
+  bool gravity;
+  T lin_damping, ang_damping, max_linvel, max_angvel;
+  int companion;
+  unsigned flags;
+  AABB<T> loose;
+  T query_limit() const { return margin + lin_pred; }
+};
+
+struct Manifold {
+  int b1, b2;
+  int first, count;
+};
+
+template <class T>
+struct VelRow {  // Unilateral/Bilateral(+Ground)Constraint, src/solver/constraint.rs
+  T impulse, r, rhs;
+  uint64_t key;
+  int a1, a2;
+  size_t j1, j2, wj1, wj2;
+  int dep;   // friction rows: index of the normal row in its list
+  T mu;
+  int ground;
+};
+
+template <class T>
+struct PosRow {  // NonlinearUnilateralConstraint, src/solver/nonlinear_constraint.rs:74-105
+  int b1, b2, ndofs1, ndofs2;
+  Contact<T> kin;
+  T r, rhs;
+};
+
+struct ContactRows {
+  int ground;
+  int normal;      // index in uni or uni_g
+  int fric[2];     // index in bil or bil_g
+};
+
+struct Counters {  // stage names of src/counters/mod.rs:148-209
+  double step = 0, collision_detection = 0, broad_phase = 0, narrow_phase = 0, assembly = 0,
+         velocity_resolution = 0, velocity_update = 0, position_resolution = 0;
+  long nsteps = 0;
+};
+
+inline double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+inline uint64_t contact_key(int a, int b, uint32_t feature) {
+  return (uint64_t(uint32_t(a)) << 40) | (uint64_t(uint32_t(b)) << 16) | uint64_t(feature & 0xffffu);
+}
+
+// 32-bit integer hash (bijective mixer) used as the colouring priority.
+inline uint32_t colour_hash(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352dU;
+  x ^= x >> 15;
+  x *= 0x846ca68bU;
+  x ^= x >> 16;
+  return x;
+}
+
+// Deterministic Jones-Plassmann greedy colouring of manifolds; two manifolds conflict when they
+// share a dynamic body. Priority (hash(m), m); in each round every uncoloured manifold that beats
+// all its uncoloured neighbours takes the smallest colour unused by its coloured neighbours.
+// The result depends only on the manifold list, never on evaluation order.
+inline int colour_manifolds(const std::vector<Manifold>& ms, const std::vector<int>& is_dyn,
+                            int nbodies, std::vector<int>& colour) {
+  const int M = int(ms.size());
+  colour.assign(M, -1);
+  std::vector<int> off(nbodies + 1, 0);
+  for (const auto& m : ms) {
+    if (is_dyn[m.b1]) off[m.b1 + 1]++;
+    if (is_dyn[m.b2]) off[m.b2 + 1]++;
+  }
+  for (int i = 0; i < nbodies; ++i) off[i + 1] += off[i];
+  std::vector<int> adj(off[nbodies]);
+  std::vector<int> fill(off.begin(), off.end() - 1);
+  for (int m = 0; m < M; ++m) {
+    if (is_dyn[ms[m].b1]) adj[fill[ms[m].b1]++] = m;
+    if (is_dyn[ms[m].b2]) adj[fill[ms[m].b2]++] = m;
+  }
+  std::vector<uint32_t> pri(M);
+  for (int m = 0; m < M; ++m) pri[m] = colour_hash(uint32_t(m));
+  auto beats = [&](int x, int y) { return pri[x] > pri[y] || (pri[x] == pri[y] && x > y); };
+  int remaining = M, ncol = 0;
+  std::vector<int> sel;
+  while (remaining > 0) {
+    sel.clear();
+    for (int m = 0; m < M; ++m) {
+      if (colour[m] >= 0) continue;
+      bool best = true;
+      const int bs[2] = {ms[m].b1, ms[m].b2};
+      for (int s = 0; s < 2 && best; ++s) {
+        const int b = bs[s];
+        if (!is_dyn[b]) continue;
+        for (int k = off[b]; k < off[b + 1]; ++k) {
+          const int o = adj[k];
+          if (o != m && colour[o] < 0 && beats(o, m)) { best = false; break; }
+        }
+      }
+      if (best) sel.push_back(m);
+    }
+    for (int m : sel) {
+      // smallest colour not used by a coloured neighbour
+      std::vector<char> used;
+      const int bs[2] = {ms[m].b1, ms[m].b2};
+      for (int s = 0; s < 2; ++s) {
+        const int b = bs[s];
+        if (!is_dyn[b]) continue;
+        for (int k = off[b]; k < off[b + 1]; ++k) {
+          const int c = colour[adj[k]];
+          if (c >= 0) {
+            if (c >= int(used.size())) used.resize(c + 1, 0);
+            used[c] = 1;
+          }
+        }
+      }
+      int c = 0;
+      while (c < int(used.size()) && used[c]) ++c;
+      colour[m] = c;
+      ncol = std::max(ncol, c + 1);
+    }
+    remaining -= int(sel.size());
+  }
+  return ncol;
+}
+
+template <class T>
+class World {
+ public:
+  Params<T> prm;
+  V3<T> gravity;
+  std::vector<Body<T>> bodies;
+  int order_mode = ORDER_REFERENCE;
+
+  // geometrical world
+  std::vector<std::pair<int, int>> pairs;
+  std::vector<Manifold> manifolds;  // pairs with >= 1 contact, pair order
+  std::vector<Contact<T>> contacts;
+  std::unordered_map<uint64_t, V3<T>> cache;  // ImpulseCache: ContactId -> (n, t1, t2)
+
+  // solver state of the last step
+  std::vector<T> jacobians, mj_lambda, ext_vels;
+  std::vector<VelRow<T>> uni, uni_g, bil, bil_g;
+  std::vector<PosRow<T>> posc;
+  std::vector<Manifold> solved;        // manifolds handed to the solver
+  std::vector<ContactRows> crow;       // per contact of `solved`
+  std::vector<int> colours;
+  int ncolours = 0;
+  T res_inf = T(0), res_rms = T(0);
+  long n_rows = 0;
+  Counters counters;
+
+  World() {
+    prm.t = T(0);
+    prm.set_dt(T(1.0 / 60.0));
+    prm.erp = T(0.2);
+    prm.warmstart_coeff = T(1.0);
+    prm.restitution_velocity_threshold = T(1.0);
+    prm.allowed_linear_error = T(0.001);
+    prm.allowed_angular_error = T(0.001);
+    prm.max_linear_correction = T(0.2);
+    prm.max_angular_correction = T(0.2);
+    prm.max_stabilization_multiplier = T(0.2);
+    prm.max_velocity_iterations = 8;
+    prm.max_position_iterations = 3;
+    prm.max_ccd_position_iterations = 10;
+    prm.max_ccd_substeps = 1;
+    gravity = V3<T>{T(0), T(-9.81), T(0)};
+  }
+
+  // ------------------------------------------------------------------ body insertion
+  // RigidBodyDesc::build + ColliderDesc::build + GeometricalWorld::handle_insertions
+  // (geometrical_world.rs:104-140): mass properties from the density, proxy creation with
+  // loose AABB = aabb(pos) loosened by the query limit, then by the broad-phase margin 0.01.
+  int add_body(int status, int shape, V3<T> dims, Iso<T> pos, V3<T> lv, V3<T> av, T density,
+               T margin, T lin_pred, T friction, T restitution) {
+    Body<T> b;
+    std::memset(static_cast<void*>(&b), 0, sizeof(b));
+    b.status = status;
+    b.shape = shape;
+    b.sd.kind = shape;
+    b.sd.he = shape == SHAPE_BALL ? V3<T>{dims.x, dims.x, dims.x} : dims;
+    b.density = density;
+    b.margin = margin;
+    b.lin_pred = lin_pred;
+    b.friction = friction;
+    b.restitution = restitution;
+    b.pos = pos;
+    b.pos0 = pos;
+    b.linvel = status == STATUS_DYNAMIC ? lv : V3<T>{T(0), T(0), T(0)};
+    b.angvel = status == STATUS_DYNAMIC ? av : V3<T>{T(0), T(0), T(0)};
+    b.gravity = true;
+    b.lin_damping = T(0);
+    b.ang_damping = T(0);
+    b.max_linvel = std::numeric_limits<T>::max();
+    b.max_angvel = std::numeric_limits<T>::max();
+    b.flags = F_ALL;
+    b.com = pos.t;
+    b.mass = T(0);
+    b.local_I = m3_zero<T>();
+    if (status == STATUS_DYNAMIC && density != T(0)) {
+      T mass;
+      M3<T> ai;
+      if (shape == SHAPE_BALL) {
+        const T r = dims.x;
+        const T vol = (((T(M_PI) * r) * r) * r) * T(4.0 / 3.0);
+        mass = vol * density;
+        const T diag = (r * r) * T(2.0 / 5.0);
+        ai = m3_diag(diag * mass, diag * mass, diag * mass);
+      } else {
+        T vol = T(1);
+        vol = vol * dims.x * T(2);
+        vol = vol * dims.y * T(2);
+        vol = vol * dims.z * T(2);
+        mass = vol * density;
+        const T w = (T(1.0 / 12.0) * T(2)) * T(2);
+        const T ix = (w * dims.x) * dims.x, iy = (w * dims.y) * dims.y, iz = (w * dims.z) * dims.z;
+        ai = m3_diag((iy + iz) * mass, (ix + iz) * mass, (ix + iy) * mass);
+      }
+      b.mass = mass;
+      b.local_I = ai;
+    }
+    const AABB<T> tight = loosened(shape_aabb(b.sd, pos), b.query_limit());
+    b.loose = loosened(tight, T(0.01));
+    bodies.push_back(b);
+    pairs_dirty = true;
+    return int(bodies.size()) - 1;
+  }
+
+  void set_state(int i, Iso<T> pos, V3<T> lv, V3<T> av) {
+    Body<T>& b = bodies[i];
+    b.pos = pos;
+    b.com = pos.t;
+    if (b.status == STATUS_DYNAMIC) {
+      b.linvel = lv;
+      b.angvel = av;
+    }
+    b.flags |= F_POSITION | F_VELOCITY;
+  }
+
+  // ------------------------------------------------------------------ body dynamics
+  void update_dynamics(Body<T>& b) {  // rigid_body.rs:558-588
+    if (!(b.flags & F_ALL) || b.status != STATUS_DYNAMIC) return;
+    const M3<T> R = qmat(b.pos.q);
+    b.I_world = mul(mul(R, b.local_I), transpose(R));
+    b.aug = b.I_world;
+    const V3<T> iw = mulv(b.I_world, b.angvel);
+    const V3<T> w_dt = b.angvel * prm.dt;
+    const M3<T> wx = cross_matrix(w_dt);
+    const M3<T> iwx = cross_matrix(iw * prm.dt);
+    b.aug = madd(b.aug, msub(mul(wx, b.I_world), iwx));
+    b.inv_mass = b.mass == T(0) ? T(0) : T(1) / b.mass;
+    try_inverse(b.aug, b.inv_aug);
+  }
+
+  void update_acceleration(Body<T>& b) {  // rigid_body.rs:590-619
+    b.acc_lin = V3<T>{T(0), T(0), T(0)};
+    b.acc_ang = V3<T>{T(0), T(0), T(0)};
+    if (b.status != STATUS_DYNAMIC) return;
+    const V3<T> iw = mulv(b.I_world, b.angvel);
+    const V3<T> gyro = -cross(b.angvel, iw);
+    b.acc_ang = mulv(b.inv_aug, gyro);
+    if (b.inv_mass != T(0) && b.gravity) b.acc_lin = gravity;
+  }
+
+  // Isometry composition of apply_displacement (rigid_body.rs:371-381):
+  // new_pos = T(com) * Iso(lin, exp(ang)) * T(-com) * pos.
+  void apply_displacement(Body<T>& b, V3<T> lin, V3<T> ang) {
+    const Q<T> qd = qexp_scaled_axis(ang);
+    const V3<T> com = b.com;
+    const V3<T> dt_ = com + (lin + qrot(qd, -com));
+    Iso<T> np;
+    np.t = dt_ + qrot(qd, b.pos.t);
+    np.q = qmul(qd, b.pos.q);
+    b.pos = np;
+    b.com = np.t;
+    b.flags |= F_POSITION;
+  }
+
+  void integrate(Body<T>& b) {  // rigid_body.rs:467-505
+    b.linvel = b.linvel * (T(1) / (T(1) + prm.dt * b.lin_damping));
+    b.angvel = b.angvel * (T(1) / (T(1) + prm.dt * b.ang_damping));
+    const T ln = norm(b.linvel);
+    if (ln > b.max_linvel) b.linvel = b.max_linvel == T(0) ? V3<T>{T(0), T(0), T(0)} : b.linvel * (b.max_linvel / ln);
+    const T an = norm(b.angvel);
+    if (an > b.max_angvel) b.angvel = b.max_angvel == T(0) ? V3<T>{T(0), T(0), T(0)} : b.angvel * (b.max_angvel / an);
+    apply_displacement(b, b.linvel * prm.dt, b.angvel * prm.dt);
+  }
+
+  // ------------------------------------------------------------------ broad phase
+  // Restated DBVT rule (ncollide DBVTBroadPhase, margin 0.01; geometrical_world.rs:60):
+  // tight = shape AABB at the collider position merged with the one at the predicted position,
+  // loosened by margin + linear_prediction; the proxy's loose AABB is replaced by tight + 0.01
+  // only when it no longer contains tight. Pairs = overlapping loose AABBs that pass
+  // DefaultCollisionFilter (geometrical_world.rs:933-971): not the same body, not both static.
+  bool pairs_dirty = true;
+
+  void sync_and_broad_phase(bool swept) {
+    const double t0 = now_s();
+    for (auto& b : bodies) {
+      if (!(b.flags & F_POSITION)) continue;  // colliders_need_update (body.rs:510-512)
+      AABB<T> a1 = shape_aabb(b.sd, swept ? b.pos0 : b.pos);
+      if (swept) a1 = merged(a1, shape_aabb(b.sd, b.pos));
+      else a1 = merged(a1, a1);
+      const AABB<T> tight = loosened(a1, b.query_limit());
+      if (!contains(b.loose, tight)) {
+        b.loose = loosened(tight, T(0.01));
+        pairs_dirty = true;
+      }
+    }
+    if (pairs_dirty) {
+      recompute_pairs();
+      pairs_dirty = false;
+    }
+    counters.broad_phase += now_s() - t0;
+  }
+
+  static bool pair_allowed(const Body<T>& a, const Body<T>& b) {
+    return a.status == STATUS_DYNAMIC || b.status == STATUS_DYNAMIC;
+  }
+
+  // canonical pair: first is dynamic; second is static or a higher index
+  void push_pair(int i, int j) {
+    const Body<T>& a = bodies[i];
+    const Body<T>& b = bodies[j];
+    if (!pair_allowed(a, b)) return;
+    if (a.status == STATUS_DYNAMIC && (b.status != STATUS_DYNAMIC || j > i)) pairs.emplace_back(i, j);
+    else if (b.status == STATUS_DYNAMIC && (a.status != STATUS_DYNAMIC || i > j)) pairs.emplace_back(j, i);
+  }
+
+  int brute_force_limit = 512;
+
+  void recompute_pairs() {
+    pairs.clear();
+    const int n = int(bodies.size());
+    if (n <= brute_force_limit) {
+      for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j)
+          if (intersects(bodies[i].loose, bodies[j].loose)) push_pair(i, j);
+    } else {
+      recompute_pairs_grid();
+    }
+    std::sort(pairs.begin(), pairs.end());
+  }
+
+  // Hashed uniform grid for large scenes; returns exactly the brute-force set.
+  void recompute_pairs_grid() {
+    const int n = int(bodies.size());
+    std::vector<int> dyn, stat;
+    V3<T> ext{T(0), T(0), T(0)};
+    for (int i = 0; i < n; ++i) {
+      if (bodies[i].status == STATUS_DYNAMIC) {
+        dyn.push_back(i);
+        ext = vmax(ext, bodies[i].loose.mx - bodies[i].loose.mn);
+      } else {
+        stat.push_back(i);
+      }
+    }
+    // statics against everything dynamic
+    for (int s : stat)
+      for (int d : dyn)
+        if (intersects(bodies[s].loose, bodies[d].loose)) push_pair(s, d);
+    const double cs = double(std::max(ext.x, std::max(ext.y, ext.z))) * 1.0001 + 1e-6;
+    auto cell = [&](V3<T> p, int64_t& cx, int64_t& cy, int64_t& cz) {
+      cx = int64_t(std::floor(double(p.x) / cs));
+      cy = int64_t(std::floor(double(p.y) / cs));
+      cz = int64_t(std::floor(double(p.z) / cs));
+    };
+    auto hkey = [](int64_t x, int64_t y, int64_t z) {
+      return (uint64_t(x + (1 << 20)) << 42) ^ (uint64_t(y + (1 << 20)) << 21) ^ uint64_t(z + (1 << 20));
+    };
+    std::unordered_map<uint64_t, std::vector<int>> grid;
+    grid.reserve(dyn.size() * 2);
+    for (int d : dyn) {
+      int64_t cx, cy, cz;
+      cell(bodies[d].loose.mn, cx, cy, cz);
+      grid[hkey(cx, cy, cz)].push_back(d);
+    }
+    for (int d : dyn) {
+      int64_t cx, cy, cz;
+      cell(bodies[d].loose.mn, cx, cy, cz);
+      for (int64_t x = cx - 1; x <= cx + 1; ++x)
+        for (int64_t y = cy - 1; y <= cy + 1; ++y)
+          for (int64_t z = cz - 1; z <= cz + 1; ++z) {
+            auto it = grid.find(hkey(x, y, z));
+            if (it == grid.end()) continue;
+            for (int o : it->second)
+              if (o > d && intersects(bodies[d].loose, bodies[o].loose)) push_pair(d, o);
+          }
+    }
+  }
+
+  // ------------------------------------------------------------------ narrow phase
+  void narrow_phase() {
+    const double t0 = now_s();
+    manifolds.clear();
+    contacts.clear();
+    Contact<T> buf[8];
+    for (const auto& p : pairs) {
+      const Body<T>& a = bodies[p.first];
+      const Body<T>& b = bodies[p.second];
+      const T prediction = a.query_limit() + b.query_limit();
+      const int nc = collide_pair(a.sd, a.pos, b.sd, b.pos, prediction, buf);
+      if (nc == 0) continue;
+      Manifold m{p.first, p.second, int(contacts.size()), nc};
+      for (int k = 0; k < nc; ++k) contacts.push_back(buf[k]);
+      manifolds.push_back(m);
+    }
+    counters.narrow_phase += now_s() - t0;
+  }
+
+  void collide() {  // sync_colliders + perform_broad_phase + perform_narrow_phase
+    const double t0 = now_s();
+    sync_and_broad_phase(false);
+    narrow_phase();
+    counters.collision_detection += now_s() - t0;
+  }
+
+  // ------------------------------------------------------------------ contact model
+  // fill_constraint_geometry for one body: J = [d; (p - com) x d], W J = M^-1 J,
+  // inv_r += m^-1 + (J_ang . I^-1 J_ang), out_vel += J . v + J . ext (rigid_body.rs:672-722).
+  void fill_geom(const Body<T>& b, V3<T> point, V3<T> dir, size_t j_id, size_t wj_id, T& inv_r,
+                 const T* ext, T* out_vel) {
+    if (b.status != STATUS_DYNAMIC) return;
+    const V3<T> p = point - b.com;
+    const V3<T> ang = cross(p, dir);
+    T* J = &jacobians[j_id];
+    J[0] = dir.x; J[1] = dir.y; J[2] = dir.z; J[3] = ang.x; J[4] = ang.y; J[5] = ang.z;
+    const V3<T> wl = dir * b.inv_mass;
+    const V3<T> wa = mulv(b.inv_aug, ang);
+    T* W = &jacobians[wj_id];
+    W[0] = wl.x; W[1] = wl.y; W[2] = wl.z; W[3] = wa.x; W[4] = wa.y; W[5] = wa.z;
+    inv_r = inv_r + (b.inv_mass + dot(ang, wa));
+    if (out_vel) {
+      const T v6[6] = {b.linvel.x, b.linvel.y, b.linvel.z, b.angvel.x, b.angvel.y, b.angvel.z};
+      T s = T(0);
+      for (int k = 0; k < 6; ++k) s = s + J[k] * v6[k];
+      *out_vel = *out_vel + s;
+      T e = T(0);
+      for (int k = 0; k < 6; ++k) e = e + J[k] * ext[k];
+      *out_vel = *out_vel + e;
+    }
+  }
+
+  // helper::constraint_pair_geometry (helper.rs:53-135)
+  struct Geom {
+    size_t j1, j2, wj1, wj2;
+    int nd1, nd2;
+    T r;
+  };
+  Geom pair_geometry(const Body<T>& b1, const Body<T>& b2, V3<T> c1, V3<T> c2, V3<T> dir,
+                     size_t& ground_j_id, size_t& j_id, T* out_vel) {
+    Geom g;
+    g.nd1 = b1.status == STATUS_DYNAMIC ? 6 : 0;
+    g.nd2 = b2.status == STATUS_DYNAMIC ? 6 : 0;
+    size_t* out_j = (g.nd1 == 0 || g.nd2 == 0) ? &ground_j_id : &j_id;
+    g.j1 = *out_j;
+    g.j2 = g.j1 + g.nd1;
+    g.wj1 = g.j2 + g.nd2;
+    g.wj2 = g.wj1 + g.nd1;
+    T inv_r = T(0);
+    const T* e1 = g.nd1 ? &ext_vels[b1.companion] : nullptr;
+    const T* e2 = g.nd2 ? &ext_vels[b2.companion] : nullptr;
+    fill_geom(b1, c1, dir, g.j1, g.wj1, inv_r, e1, out_vel);
+    fill_geom(b2, c2, -dir, g.j2, g.wj2, inv_r, e2, out_vel);
+    g.r = inv_r != T(0) ? T(1) / inv_r : T(1);
+    *out_j += size_t(g.nd1 + g.nd2) * 2;
+    return g;
+  }
+
+  VelRow<T> make_row(const Geom& g, const Body<T>& b1, const Body<T>& b2, T rhs, T impulse,
+                     uint64_t key) {
+    VelRow<T> r;
+    r.impulse = impulse;
+    r.r = g.r;
+    r.rhs = rhs;
+    r.key = key;
+    r.dep = -1;
+    r.mu = T(0);
+    r.ground = (g.nd1 == 0 || g.nd2 == 0);
+    if (!r.ground) {
+      r.a1 = b1.companion; r.a2 = b2.companion;
+      r.j1 = g.j1; r.j2 = g.j2; r.wj1 = g.wj1; r.wj2 = g.wj2;
+    } else if (g.nd1 == 0) {  // UnilateralGroundConstraint::new keeps the dynamic side
+      r.a1 = b2.companion; r.a2 = -1;
+      r.j1 = g.j2; r.wj1 = g.wj2; r.j2 = r.wj2 = 0;
+    } else {
+      r.a1 = b1.companion; r.a2 = -1;
+      r.j1 = g.j1; r.wj1 = g.wj1; r.j2 = r.wj2 = 0;
+    }
+    return r;
+  }
+
+  // nalgebra orthonormal_subspace_basis for one 3D vector (unvendored; restated):
+  // a = normalize(|n.x| > |n.y| ? (n.z, 0, -n.x) : (0, -n.z, n.y)); tangents a x n, then a.
+  static void tangents(V3<T> n, V3<T>& t1, V3<T>& t2) {
+    V3<T> a = std::fabs(n.x) > std::fabs(n.y) ? V3<T>{n.z, T(0), -n.x} : V3<T>{T(0), -n.z, n.y};
+    const T l = norm(a);
+    a = V3<T>{a.x / l, a.y / l, a.z / l};
+    t1 = cross(a, n);
+    t2 = a;
+  }
+
+  void assemble(const std::vector<Manifold>& ms) {
+    // companion ids and buffers (moreau_jean_solver.rs:137-174, 322-326)
+    int ndofs = 0;
+    for (auto& b : bodies) {
+      b.companion = 0;
+      if (b.status == STATUS_DYNAMIC) {
+        b.companion = ndofs;
+        ndofs += 6;
+      }
+    }
+    mj_lambda.assign(ndofs, T(0));
+    ext_vels.assign(ndofs, T(0));
+    for (auto& b : bodies) {
+      if (b.status != STATUS_DYNAMIC) continue;
+      const T acc[6] = {b.acc_lin.x, b.acc_lin.y, b.acc_lin.z, b.acc_ang.x, b.acc_ang.y, b.acc_ang.z};
+      for (int k = 0; k < 6; ++k) ext_vels[b.companion + k] = prm.dt * acc[k];
+    }
+    // jacobian sizing: non-ground region first, ground region after (:203-224)
+    size_t jac_sz = 0, gjac_sz = 0;
+    for (const auto& m : ms) {
+      const int nd1 = bodies[m.b1].status == STATUS_DYNAMIC ? 6 : 0;
+      const int nd2 = bodies[m.b2].status == STATUS_DYNAMIC ? 6 : 0;
+      const size_t sz = size_t(3 * m.count) * size_t(nd1 + nd2) * 2;
+      if (nd1 == 0 || nd2 == 0) gjac_sz += sz; else jac_sz += sz;
+    }
+    jacobians.assign(jac_sz + gjac_sz, T(0));
+    size_t j_id = 0, ground_j_id = jac_sz;
+    uni.clear(); uni_g.clear(); bil.clear(); bil_g.clear(); posc.clear(); crow.clear();
+    for (const auto& m : ms) {
+      const Body<T>& b1 = bodies[m.b1];
+      const Body<T>& b2 = bodies[m.b2];
+      // Material::combine, Average mode (material.rs:72-86, 134-178)
+      const T mu = (b1.friction + b2.friction) * T(0.5);
+      const T e = (b1.restitution + b2.restitution) * T(0.5);
+      for (int k = 0; k < m.count; ++k) {
+        const Contact<T>& c = contacts[m.first + k];
+        const uint64_t key = contact_key(m.b1, m.b2, c.feature);
+        V3<T> imp{T(0), T(0), T(0)};
+        auto it = cache.find(key);
+        if (it != cache.end()) imp = it->second;
+        // SignoriniModel::build_velocity_constraint (signorini_model.rs:37-138)
+        const V3<T> center1 = c.world1 + c.normal * b1.margin;
+        const V3<T> center2 = c.world2 - c.normal * b2.margin;
+        T rhs = T(0);  // n . surface_velocity (zero for BasicMaterial)
+        const Geom g = pair_geometry(b1, b2, center1, center2, -c.normal, ground_j_id, j_id, &rhs);
+        if (rhs <= -prm.restitution_velocity_threshold) rhs = rhs + e * rhs;
+        const T depth = (c.depth + b1.margin) + b2.margin;
+        if (depth < T(0)) rhs = rhs + (-depth) * prm.inv_dt;
+        VelRow<T> nr = make_row(g, b1, b2, rhs, imp.x * prm.warmstart_coeff, key);
+        ContactRows cr;
+        cr.ground = nr.ground;
+        std::vector<VelRow<T>>& ulist = nr.ground ? uni_g : uni;
+        ulist.push_back(nr);
+        cr.normal = int(ulist.size()) - 1;
+        // build_position_constraint (signorini_model.rs:153-197)
+        PosRow<T> pr;
+        pr.b1 = m.b1; pr.b2 = m.b2;
+        pr.ndofs1 = b1.status == STATUS_DYNAMIC ? 6 : 0;
+        pr.ndofs2 = b2.status == STATUS_DYNAMIC ? 6 : 0;
+        pr.kin = c;
+        pr.r = T(0); pr.rhs = T(0);
+        posc.push_back(pr);
+        // friction rows (signorini_coulomb_pyramid_model.rs:145-216)
+        V3<T> t[2];
+        tangents(c.normal, t[0], t[1]);
+        for (int f = 0; f < 2; ++f) {
+          T frhs = T(0);
+          const Geom fg = pair_geometry(b1, b2, center1, center2, t[f], ground_j_id, j_id, &frhs);
+          VelRow<T> fr = make_row(fg, b1, b2, frhs, imp_comp(imp, f + 1) * prm.warmstart_coeff, key);
+          fr.dep = cr.normal;
+          fr.mu = mu;
+          std::vector<VelRow<T>>& blist = fr.ground ? bil_g : bil;
+          blist.push_back(fr);
+          cr.fric[f] = int(blist.size()) - 1;
+        }
+        crow.push_back(cr);
+      }
+    }
+    n_rows = long(uni.size() + uni_g.size() + bil.size() + bil_g.size());
+  }
+  static T imp_comp(V3<T> v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+  // ------------------------------------------------------------------ SOR-Prox rows
+  T dot6(size_t j, int a) const {
+    T s = T(0);
+    for (int k = 0; k < 6; ++k) s = s + jacobians[j + k] * mj_lambda[a + k];
+    return s;
+  }
+  void axpy6(T alpha, size_t wj, int a) {
+    for (int k = 0; k < 6; ++k) mj_lambda[a + k] = mj_lambda[a + k] + alpha * jacobians[wj + k];
+  }
+  T row_w(const VelRow<T>& c) const {
+    if (!c.ground) return (dot6(c.j1, c.a1) + dot6(c.j2, c.a2)) + c.rhs;
+    return dot6(c.j1, c.a1) + c.rhs;
+  }
+  void row_add(const VelRow<T>& c, T alpha) {
+    axpy6(alpha, c.wj1, c.a1);
+    if (!c.ground) axpy6(alpha, c.wj2, c.a2);
+  }
+  void warm_row(const VelRow<T>& c) {
+    if (c.impulse != T(0)) row_add(c, c.impulse);
+  }
+  void solve_normal(VelRow<T>& c) {  // sor_prox.rs:181-230
+    const T w = row_w(c);
+    const T ni = tmax(c.impulse - c.r * w, T(0));
+    const T dl = ni - c.impulse;
+    c.impulse = ni;
+    row_add(c, dl);
+  }
+  void solve_friction(VelRow<T>& c, const std::vector<VelRow<T>>& normals) {  // :232-343
+    const T ln = normals[c.dep].impulse;
+    if (ln == T(0)) {
+      if (c.impulse != T(0)) {
+        row_add(c, -c.impulse);
+        c.impulse = T(0);
+      }
+      return;
+    }
+    const T mx = c.mu * ln;
+    const T mn = -mx;
+    const T w = row_w(c);
+    const T v = c.impulse - c.r * w;
+    const T ni = v > mn ? (v < mx ? v : mx) : mn;  // na::clamp
+    const T dl = ni - c.impulse;
+    c.impulse = ni;
+    row_add(c, dl);
+  }
+
+  void sor_prox() {  // SORProx::solve (sor_prox.rs:48-80)
+    if (order_mode == ORDER_REFERENCE) {
+      for (auto& c : uni) warm_row(c);
+      for (auto& c : uni_g) warm_row(c);
+      for (auto& c : bil) warm_row(c);
+      for (auto& c : bil_g) warm_row(c);
+      for (int it = 0; it < prm.max_velocity_iterations; ++it) {
+        for (auto& c : bil) solve_friction(c, uni);
+        for (auto& c : bil_g) solve_friction(c, uni_g);
+        for (auto& c : uni) solve_normal(c);
+        for (auto& c : uni_g) solve_normal(c);
+      }
+    } else {
+      for (int m : colour_order) {
+        const Manifold& mf = solved[m];
+        for (int k = 0; k < mf.count; ++k) {
+          const ContactRows& cr = crow[mf.first + k];
+          warm_row(cr.ground ? uni_g[cr.normal] : uni[cr.normal]);
+        }
+        for (int k = 0; k < mf.count; ++k) {
+          const ContactRows& cr = crow[mf.first + k];
+          for (int f = 0; f < 2; ++f) warm_row(cr.ground ? bil_g[cr.fric[f]] : bil[cr.fric[f]]);
+        }
+      }
+      for (int it = 0; it < prm.max_velocity_iterations; ++it) {
+        for (int m : colour_order) {
+          const Manifold& mf = solved[m];
+          for (int k = 0; k < mf.count; ++k) {
+            const ContactRows& cr = crow[mf.first + k];
+            for (int f = 0; f < 2; ++f) {
+              if (cr.ground) solve_friction(bil_g[cr.fric[f]], uni_g);
+              else solve_friction(bil[cr.fric[f]], uni);
+            }
+          }
+          for (int k = 0; k < mf.count; ++k) {
+            const ContactRows& cr = crow[mf.first + k];
+            solve_normal(cr.ground ? uni_g[cr.normal] : uni[cr.normal]);
+          }
+        }
+      }
+    }
+  }
+
+  // Natural-map residual of the LCP in velocity units, per row:
+  //   normal:   (l - max(0, l - r w)) / r;  friction: (l - clamp(l - r w, -mu ln, mu ln)) / r
+  void compute_residual() {
+    double mx = 0, ss = 0;
+    long n = 0;
+    auto acc = [&](T phi) {
+      const double a = std::fabs(double(phi));
+      mx = std::max(mx, a);
+      ss += a * a;
+      ++n;
+    };
+    auto normal = [&](const VelRow<T>& c) {
+      const T w = row_w(c);
+      const T p = tmax(c.impulse - c.r * w, T(0));
+      acc((c.impulse - p) / c.r);
+    };
+    auto fric = [&](const VelRow<T>& c, const std::vector<VelRow<T>>& ns) {
+      const T ln = ns[c.dep].impulse;
+      const T w = row_w(c);
+      const T mxv = c.mu * ln, mnv = -mxv;
+      const T v = c.impulse - c.r * w;
+      const T p = v > mnv ? (v < mxv ? v : mxv) : mnv;
+      acc((c.impulse - p) / c.r);
+    };
+    for (auto& c : uni) normal(c);
+    for (auto& c : uni_g) normal(c);
+    for (auto& c : bil) fric(c, uni);
+    for (auto& c : bil_g) fric(c, uni_g);
+    res_inf = T(mx);
+    res_rms = n ? T(std::sqrt(ss / double(n))) : T(0);
+  }
+
+  void cache_impulses() {  // signorini_coulomb_pyramid_model.rs:226-261
+    cache.clear();
+    for (const auto& c : uni) cache[c.key] = V3<T>{c.impulse, T(0), T(0)};
+    for (const auto& c : uni_g) cache[c.key] = V3<T>{c.impulse, T(0), T(0)};
+    for (size_t i = 0; i < crow.size(); ++i) {
+      const ContactRows& cr = crow[i];
+      const std::vector<VelRow<T>>& bl = cr.ground ? bil_g : bil;
+      const uint64_t key = bl[cr.fric[0]].key;
+      V3<T>& v = cache[key];
+      v.y = bl[cr.fric[0]].impulse;
+      v.z = bl[cr.fric[1]].impulse;
+    }
+  }
+
+  // ------------------------------------------------------------------ position correction
+  T clamp_rhs(T rhs) const {  // nonlinear_sor_prox.rs:296-309 (linear case)
+    return tmax((rhs + prm.allowed_linear_error) * prm.erp, -prm.max_linear_correction);
+  }
+
+  void solve_position_row(PosRow<T>& pc) {  // nonlinear_sor_prox.rs:121-294
+    Body<T>& b1 = bodies[pc.b1];
+    Body<T>& b2 = bodies[pc.b2];
+    Contact<T> g = reevaluate(pc.kin, b1.sd, b1.pos, b2.sd, b2.pos);
+    // ContactKinematic dilations: margins on both sides (signorini_model.rs:174-181)
+    const V3<T> w1 = g.world1 + g.normal * b1.margin;
+    const V3<T> w2 = g.world2 - g.normal * b2.margin;
+    const T depth = (g.depth + b1.margin) + b2.margin;
+    pc.rhs = clamp_rhs(-depth);
+    if (pc.rhs >= T(0)) return;
+    T inv_r = T(0);
+    V3<T> wl1{}, wa1{}, wl2{}, wa2{};
+    if (pc.ndofs1) {
+      const V3<T> dir = -g.normal;
+      const V3<T> ang = cross(w1 - b1.com, dir);
+      wl1 = dir * b1.inv_mass;
+      wa1 = mulv(b1.inv_aug, ang);
+      inv_r = inv_r + (b1.inv_mass + dot(ang, wa1));
+    }
+    if (pc.ndofs2) {
+      const V3<T> dir = g.normal;
+      const V3<T> ang = cross(w2 - b2.com, dir);
+      wl2 = dir * b2.inv_mass;
+      wa2 = mulv(b2.inv_aug, ang);
+      inv_r = inv_r + (b2.inv_mass + dot(ang, wa2));
+    }
+    if (inv_r == T(0)) return;
+    pc.r = T(1) / inv_r;
+    const T impulse = -pc.rhs * pc.r;
+    if (pc.ndofs1) apply_displacement(b1, wl1 * impulse, wa1 * impulse);
+    if (pc.ndofs2) apply_displacement(b2, wl2 * impulse, wa2 * impulse);
+  }
+
+  void position_correction() {  // NonlinearSORProx::solve (nonlinear_sor_prox.rs:17-55)
+    for (int it = 0; it < prm.max_position_iterations; ++it) {
+      if (order_mode == ORDER_REFERENCE) {
+        for (auto& pc : posc) solve_position_row(pc);
+      } else {
+        for (int m : colour_order) {
+          const Manifold& mf = solved[m];
+          for (int k = 0; k < mf.count; ++k) solve_position_row(posc[mf.first + k]);
+        }
+      }
+    }
+  }
+
+  // ------------------------------------------------------------------ colouring order
+  std::vector<int> colour_order;
+  void build_colour_order() {
+    std::vector<int> is_dyn(bodies.size());
+    for (size_t i = 0; i < bodies.size(); ++i) is_dyn[i] = bodies[i].status == STATUS_DYNAMIC;
+    ncolours = colour_manifolds(solved, is_dyn, int(bodies.size()), colours);
+    colour_order.clear();
+    for (int c = 0; c < ncolours; ++c)
+      for (int m = 0; m < int(solved.size()); ++m)
+        if (colours[m] == c) colour_order.push_back(m);
+  }
+
+  // ------------------------------------------------------------------ the step
+  void solver_step() {  // MoreauJeanSolver::step (moreau_jean_solver.rs:47-90)
+    double t0 = now_s();
+    // manifold collection (mechanical_world.rs:287-300): every kept manifold has >= 1 contact
+    // and a dynamic side; narrow_phase() already stores them with contiguous contacts 0..C-1.
+    solved = manifolds;
+    assemble(solved);
+    counters.assembly += now_s() - t0;
+    t0 = now_s();
+    if (order_mode == ORDER_COLOURED) build_colour_order();
+    sor_prox();
+    compute_residual();
+    cache_impulses();
+    counters.velocity_resolution += now_s() - t0;
+    t0 = now_s();
+    for (auto& b : bodies) {  // update_velocities_and_integrate (:328-347)
+      if (b.status != STATUS_DYNAMIC) continue;
+      const int id = b.companion;
+      b.linvel = V3<T>{b.linvel.x + ext_vels[id + 0], b.linvel.y + ext_vels[id + 1], b.linvel.z + ext_vels[id + 2]};
+      b.angvel = V3<T>{b.angvel.x + ext_vels[id + 3], b.angvel.y + ext_vels[id + 4], b.angvel.z + ext_vels[id + 5]};
+      b.linvel = V3<T>{b.linvel.x + mj_lambda[id + 0], b.linvel.y + mj_lambda[id + 1], b.linvel.z + mj_lambda[id + 2]};
+      b.angvel = V3<T>{b.angvel.x + mj_lambda[id + 3], b.angvel.y + mj_lambda[id + 4], b.angvel.z + mj_lambda[id + 5]};
+      integrate(b);
+    }
+    counters.velocity_update += now_s() - t0;
+    t0 = now_s();
+    position_correction();
+    counters.position_resolution += now_s() - t0;
+  }
+  void step() {  // MechanicalWorld::step_with_filter (mechanical_world.rs:199-396)
+    const double ts = now_s();
+    bool any_moved = false;
+    for (auto& b : bodies) {
+      b.pos0 = b.pos;  // step_started
+      update_dynamics(b);
+      if (b.flags & F_POSITION) any_moved = true;
+    }
+    for (auto& b : bodies) update_acceleration(b);
+    if (any_moved) collide();  // pre-solve CD: only when bodies were moved outside the solver
+    solver_step();
+    for (auto& b : bodies) update_dynamics(b);  // post-solve update_dynamics (:343-346)
+    // CCD phase: sync with (position0 -> position) and broad phase over swept AABBs (:604-606)
+    sync_and_broad_phase(true);
+    for (auto& b : bodies) b.pos0 = b.pos;  // validate_advancement
+    const double tc = now_s();
+    sync_and_broad_phase(false);
+    narrow_phase();
+    counters.collision_detection += now_s() - tc;
+    prm.t = prm.t + prm.dt;
+    for (auto& b : bodies) b.flags = 0;
+    counters.step += now_s() - ts;
+    counters.nsteps++;
+  }
+};
+
+}  // namespace orc

@@ -1,0 +1,153 @@
+"""GPU parity: the HIP pipeline against the FP32 CPU restatement on identical inputs.
+
+Collision detection (pairs, contacts) must match bit for bit. Full steps are compared with the
+oracle run in the GPU's colour order, where every row update is the same FP32 expression in the
+same order, so states must also match bit for bit; against the oracle in the reference's own row
+order the residual at equal iteration count is compared.
+"""
+import numpy as np
+import pytest
+
+from nphysics_amd import abi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu_world(nphys, scene, **params):
+    w = nphys.MechanicalWorld(integration_parameters=abi.default_params(**params))
+    w.add_bodies(scene)
+    return w
+
+
+def _oracle_world(oracle_mod, scene, order, precision=32, **params):
+    o = oracle_mod.OracleWorld(abi.default_params(**params), precision=precision, order=order)
+    o.add_bodies(scene)
+    return o
+
+
+def _f32(a):
+    return np.asarray(a, np.float64).astype(np.float32)
+
+
+def _assert_same_contacts(g, o):
+    assert g["bodies"].shape == o["bodies"].shape
+    np.testing.assert_array_equal(g["bodies"], o["bodies"])
+    np.testing.assert_array_equal(g["feature"], o["feature"])
+    for k in ("world1", "world2", "normal", "depth"):
+        np.testing.assert_array_equal(g[k], _f32(o[k]), err_msg=k)
+
+
+SCENES = {
+    "balls": lambda: scenes.balls(),
+    "boxes_small": lambda: scenes.boxes(nx=8, ny=6, nz=8),
+    "pile_small": lambda: scenes.small_pile(n=512),
+}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_collide_bitexact(nphys, oracle_mod, name):
+    s = SCENES[name]()
+    g = _gpu_world(nphys, s)
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED)
+    g.collide()
+    o.collide()
+    gp, op = g.pairs(), o.pairs()
+    print(name, "pairs", gp.shape[0], "contacts", g.contacts()["depth"].shape[0])
+    np.testing.assert_array_equal(gp, op)
+    _assert_same_contacts(g.contacts(), o.contacts())
+
+
+def _state_diff(gs, os_):
+    out = {}
+    for k in ("pos", "rot", "linvel", "angvel"):
+        out[k] = float(np.max(np.abs(gs[k].astype(np.float64) - os_[k]))) if gs[k].size else 0.0
+    return out
+
+
+@pytest.mark.parametrize("name,steps", [("balls", 1), ("balls", 30), ("pile_small", 1), ("pile_small", 20),
+                                        ("boxes_small", 1), ("boxes_small", 20)])
+def test_step_bitexact_vs_coloured_oracle(nphys, oracle_mod, name, steps):
+    s = SCENES[name]()
+    g = _gpu_world(nphys, s)
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED)
+    for k in range(steps):
+        g.step(1)
+        o.step(1)
+        d = _state_diff(g.get_state(), o.get_state())
+        if any(v != 0.0 for v in d.values()):
+            print(f"{name}: first difference at step {k + 1}: {d}")
+            break
+    gs, os_ = g.get_state(), o.get_state()
+    for key in ("pos", "rot", "linvel", "angvel"):
+        np.testing.assert_array_equal(gs[key], _f32(os_[key]), err_msg=key)
+    gk, gi = g.impulses()
+    ok, oi = o.impulses()
+    np.testing.assert_array_equal(gk, ok)
+    np.testing.assert_array_equal(gi, _f32(oi))
+    gr, orr = g.residual(), o.residual()
+    assert gr[0] == pytest.approx(orr[0], rel=1e-12, abs=0.0)
+    assert gr[1] == pytest.approx(orr[1], rel=1e-6)
+    _assert_same_contacts(g.contacts(), o.contacts())
+
+
+def test_residual_vs_reference_order(nphys, oracle_mod):
+    """Colour order converges like the reference's row order at equal iteration count."""
+    s = scenes.small_pile(n=512)
+    g = _gpu_world(nphys, s)
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_REFERENCE)
+    rg, ro = [], []
+    for _ in range(60):
+        g.step(1)
+        o.step(1)
+        rg.append(g.residual()[1])
+        ro.append(o.residual()[1])
+    rg, ro = np.array(rg[20:]), np.array(ro[20:])
+    print("rms residual gpu/colour", rg.mean(), "oracle/reference order", ro.mean())
+    assert rg.mean() <= 1.5 * ro.mean()
+
+
+def test_determinism(nphys):
+    s = scenes.small_pile(n=512)
+    a = _gpu_world(nphys, s)
+    b = _gpu_world(nphys, s)
+    a.step(25)
+    b.step(25)
+    sa, sb = a.get_state(), b.get_state()
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k])
+
+
+def test_complementarity_invariants(nphys):
+    s = scenes.small_pile(n=512)
+    g = _gpu_world(nphys, s)
+    g.step(40)
+    keys, imp = g.impulses()
+    assert imp.shape[0] > 0
+    assert np.all(imp[:, 0] >= 0.0)
+    mu = 0.5
+    assert np.all(np.abs(imp[:, 1]) <= mu * imp[:, 0] * (1 + 1e-6) + 1e-12)
+    assert np.all(np.abs(imp[:, 2]) <= mu * imp[:, 0] * (1 + 1e-6) + 1e-12)
+
+
+def test_empty_and_static_only(nphys):
+    w = nphys.MechanicalWorld()
+    w.step(3)
+    assert w.pairs().shape[0] == 0
+    g = scenes.ground()
+    w.add_bodies(g)
+    w.step(2)
+    assert w.pairs().shape[0] == 0
+    assert w.contacts()["depth"].shape[0] == 0
+
+
+def test_free_fall_closed_form(nphys):
+    """tests.rs:51-69 restated in 3D: x = g dt^2 n(n+1)/2 under semi-implicit Euler."""
+    s = scenes.balls(nx=1, ny=1, nz=1)
+    s = {k: v[1:] for k, v in s.items()}  # no ground
+    p = abi.default_params(dt=1e-3, gravity=(0.0, -20.0, 0.0))
+    w = nphys.MechanicalWorld(gravity=(0.0, -20.0, 0.0), integration_parameters=p)
+    w.add_bodies(s)
+    y0 = float(w.get_state()["pos"][0, 1])
+    w.step(1000)
+    y = float(w.get_state()["pos"][0, 1])
+    assert (y0 - y) == pytest.approx(20.0 * 1e-6 * 1000 * 1001 / 2, rel=1e-3)
