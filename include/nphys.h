@@ -103,6 +103,7 @@ typedef struct nphys_counters {
   int64_t nmanifolds;
   int64_t nconstraints;
   int64_t ncolours;
+  int64_t colour_rounds;    /* accumulated greedy-colouring rounds */
 } nphys_counters;
 
 /* ---------------------------------------------------------------- lifecycle */

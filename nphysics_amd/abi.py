@@ -1,7 +1,7 @@
 """ctypes mirrors of the structs declared in include/nphys.h.
 
 The same structs are accepted by the product library (libnphys_hip.so) and by the CPU checker
-in oracle/, so scene descriptions are built once and handed to either.
+used in tests, so scene descriptions are built once and handed to either.
 """
 import ctypes as C
 
@@ -109,6 +109,7 @@ class Counters(C.Structure):
         ("nmanifolds", C.c_int64),
         ("nconstraints", C.c_int64),
         ("ncolours", C.c_int64),
+        ("colour_rounds", C.c_int64),
     ]
 
     def as_dict(self):

@@ -6,6 +6,8 @@
 // the smallest colour unused by neighbours coloured in earlier rounds. The state word packs
 // (round << 16 | colour), so a neighbour coloured concurrently in the current round is always
 // seen as "this round" whichever value the load returns; the result is independent of timing.
+// Colours persist across steps: manifolds whose body pair was solved in the previous step keep
+// their colour and only new manifolds run the rounds, so at rest colouring costs one pass.
 // Manifolds are then sorted by colour (stable radix sort) and row records laid out in that order.
 //
 // Assembly (k_assemble, one lane per manifold) restates SignoriniCoulombPyramidModel::constraints
@@ -48,14 +50,34 @@ __device__ __forceinline__ uint32_t colour_hash(uint32_t x) {
   return x;
 }
 
-__global__ void __launch_bounds__(256) k_degree(DevBodies b, DevContacts c, int M, uint32_t* deg,
-                                               uint32_t* state) {
+__global__ void __launch_bounds__(256) k_degree(DevBodies b, DevContacts c, int M, uint32_t* deg) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   const int2 bb = c.mbody[m];
   if (b.status[bb.x] == STATUS_DYNAMIC) atomicAdd(&deg[bb.x], 1u);
   if (b.status[bb.y] == STATUS_DYNAMIC) atomicAdd(&deg[bb.y], 1u);
-  state[m] = kUncoloured;
+}
+
+// Colours persist across steps: a manifold whose body pair was a manifold of the previous solve
+// keeps that colour (marked as coloured in round 0); the others start uncoloured.
+__global__ void __launch_bounds__(256) k_inherit(DevContacts c, int M, DevContacts prev, int prevM,
+                                                const uint32_t* __restrict__ prev_state, uint32_t* state) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const int2 key = c.mbody[m];
+  int lo = 0, hi = prevM;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int2 k = prev.mbody[mid];
+    if (k.x < key.x || (k.x == key.x && k.y < key.y)) lo = mid + 1;
+    else hi = mid;
+  }
+  uint32_t st = kUncoloured;
+  if (lo < prevM) {
+    const int2 k = prev.mbody[lo];
+    if (k.x == key.x && k.y == key.y) st = prev_state[lo] & 0xffffu;
+  }
+  state[m] = st;
 }
 
 __global__ void __launch_bounds__(256) k_adjacency(DevBodies b, DevContacts c, int M,
@@ -92,7 +114,7 @@ __global__ void __launch_bounds__(256) k_colour_round(DevBodies b, DevContacts c
       if (so == kUncoloured || (so >> 16) == round) {
         const uint32_t po = colour_hash(uint32_t(o));
         if (po > pm || (po == pm && o > m)) {
-          atomicAdd(remaining, 1u);
+          *remaining = 1u;  // any uncoloured manifold left; a plain store, every writer stores 1
           return;
         }
       }
@@ -123,12 +145,22 @@ __global__ void __launch_bounds__(256) k_colour_round(DevBodies b, DevContacts c
 
 __global__ void __launch_bounds__(256) k_colour_keys(int M, const uint32_t* __restrict__ state,
                                                     uint32_t* keys, uint32_t* order, uint32_t* ncol) {
+  __shared__ uint32_t s_max[256];
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= M) return;
-  const uint32_t c = state[m] & 0xffffu;
-  keys[m] = c;
-  order[m] = uint32_t(m);
-  atomicMax(ncol, c + 1u);
+  uint32_t c1 = 0;
+  if (m < M) {
+    const uint32_t c = state[m] & 0xffffu;
+    keys[m] = c;
+    order[m] = uint32_t(m);
+    c1 = c + 1u;
+  }
+  s_max[threadIdx.x] = c1;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s_max[threadIdx.x] = s_max[threadIdx.x] > s_max[threadIdx.x + off] ? s_max[threadIdx.x] : s_max[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(ncol, s_max[0]);
 }
 
 __global__ void __launch_bounds__(256) k_colour_starts(DevContacts c, int M,
@@ -512,12 +544,13 @@ inline unsigned nb(long n) { return unsigned((n + 255) / 256); }
 
 }  // namespace
 
-void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb,
-                           hipStream_t s) {
+void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, const DevContacts& prev,
+                           int prevM, const uint32_t* prev_state, SolverBufs& sb, hipStream_t s) {
   fill_u32(sb.deg, 0u, size_t(b.n), s);
   fill_u32(sb.adj_fill, 0u, size_t(b.n), s);
   if (M == 0) return;
-  hipLaunchKernelGGL(k_degree, dim3(nb(M)), dim3(256), 0, s, b, c, M, sb.deg, sb.state);
+  hipLaunchKernelGGL(k_inherit, dim3(nb(M)), dim3(256), 0, s, c, M, prev, prevM, prev_state, sb.state);
+  hipLaunchKernelGGL(k_degree, dim3(nb(M)), dim3(256), 0, s, b, c, M, sb.deg);
   scan_exclusive_u32(sb.deg, sb.adj_off, b.n, sb.tmp, s);
   hipLaunchKernelGGL(k_adjacency, dim3(nb(M)), dim3(256), 0, s, b, c, M, sb.adj_off, sb.adj_fill, sb.adj);
 }
@@ -540,6 +573,7 @@ void launch_colour_order(const DevContacts& c, int M, int ncolours, SolverBufs& 
   if (M == 0) return;
   const int bits = ncolours <= 256 ? 8 : 16;
   radix_sort_pairs(sb.ckeys, sb.order, sb.ckeys_alt, sb.order_alt, M, bits, sb.tmp, s);
+  fill_u32(sb.col_start, 0xffffffffu, size_t(ncolours) + 1, s);
   hipLaunchKernelGGL(k_colour_starts, dim3(nb(M)), dim3(256), 0, s, c, M, sb.ckeys, sb.order, sb.col_start,
                      sb.mcount_ord);
   scan_exclusive_u32(sb.mcount_ord, sb.soff, M, sb.tmp, s);

@@ -82,8 +82,8 @@ struct SolverBufs {
 constexpr int kMaxColours = 4096;
 constexpr int kRecFields = 16;
 
-void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb,
-                           hipStream_t s);
+void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, const DevContacts& prev,
+                           int prevM, const uint32_t* prev_state, SolverBufs& sb, hipStream_t s);
 void launch_colour_round(const DevBodies& b, const DevContacts& c, int M, int round, SolverBufs& sb,
                          hipStream_t s);
 void launch_colour_keys(int M, SolverBufs& sb, uint32_t* ncol, hipStream_t s);

@@ -147,10 +147,14 @@ struct World {
   bool have_solved = false;
 
   // solver
-  DBuf sv_deg, sv_adj_off, sv_adj_fill, sv_adj, sv_state, sv_ckeys, sv_order, sv_ckeys_alt,
+  DBuf sv_state2[2];  // colouring state of the manifolds of cb[k]
+  DBuf sv_deg, sv_adj_off, sv_adj_fill, sv_adj, sv_ckeys, sv_order, sv_ckeys_alt,
       sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_rec2c, sv_rec, sv_red, sv_tmp;
   std::vector<uint32_t> col_start;
   int ncolours = 0;
+  int colour_rounds = 0;
+  std::vector<int> colour_visit;  // colour classes in visiting order
+  long solve_count = 0;    // solves performed, never reset
   int solved_M = 0, solved_C = 0;
   double res_inf = 0.0, res_rms = 0.0;
   std::vector<int> last_colours;
@@ -509,8 +513,10 @@ struct World {
     const size_t m = size_t(std::max(M, 1)), c = size_t(std::max(C, 1));
     for (DBuf* b : {&sv_deg, &sv_adj_off, &sv_adj_fill}) HIPCK(b->ensure(nn * 4));
     HIPCK(sv_adj.ensure(2 * m * 4));
-    for (DBuf* b : {&sv_state, &sv_ckeys, &sv_order, &sv_ckeys_alt, &sv_order_alt, &sv_mcount_ord, &sv_soff})
+    for (DBuf* b : {&sv_ckeys, &sv_order, &sv_ckeys_alt, &sv_order_alt, &sv_mcount_ord, &sv_soff})
       HIPCK(b->ensure(m * 4));
+    HIPCK(sv_state2[cur].ensure(m * 4));
+    HIPCK(sv_state2[cur ^ 1].ensure(4));
     HIPCK(sv_col_start.ensure((kMaxColours + 2) * 4));
     HIPCK(sv_rec2c.ensure(c * 4));
     HIPCK(sv_rec.ensure(c * kRecFields * sizeof(float4)));
@@ -526,7 +532,7 @@ struct World {
     v.adj_off = sv_adj_off.as<uint32_t>();
     v.adj_fill = sv_adj_fill.as<uint32_t>();
     v.adj = sv_adj.as<int>();
-    v.state = sv_state.as<uint32_t>();
+    v.state = sv_state2[cur].as<uint32_t>();
     v.remaining = bp_small.as<uint32_t>() + 32;
     v.ckeys = sv_ckeys.as<uint32_t>();
     v.order = sv_order.as<uint32_t>();
@@ -542,12 +548,34 @@ struct World {
     return v;
   }
 
+  // Visit order of the colour classes for one solve: Fisher-Yates driven by SplitMix64 seeded
+  // with the solve index, so every step orders conflicting manifolds afresh while the colours
+  // themselves persist between steps.
+  static void colour_visit_order(uint64_t solve_index, int k, std::vector<int>& perm) {
+    perm.resize(k);
+    for (int c = 0; c < k; ++c) perm[c] = c;
+    uint64_t st = solve_index * 0x9E3779B97F4A7C15ull;
+    for (int i = k - 1; i > 0; --i) {
+      st += 0x9E3779B97F4A7C15ull;
+      uint64_t z = st;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      const int j = int(z % uint64_t(i + 1));
+      std::swap(perm[i], perm[j]);
+    }
+  }
+
   int colour(const DevBodies& b, const DevContacts& c, int M, SolverBufs& v) {
-    launch_colour_prepare(b, c, M, v, stream);
-    for (int round = 0;; ++round) {
+    const ContactBufs& pc = cb[cur ^ 1];
+    const int prevM = have_solved ? pc.M : 0;
+    launch_colour_prepare(b, c, M, pc.view(), prevM, sv_state2[cur ^ 1].as<uint32_t>(), v, stream);
+    colour_rounds = 0;
+    for (int round = 1;; ++round) {
       if (round >= 0xffff) return set_err(NPHYS_ERR_STATE, "colouring did not converge");
       launch_colour_round(b, c, M, round, v, stream);
-      if ((round & 3) == 3 || round < 4) {
+      colour_rounds = round;
+      if ((round & 3) == 0 || round < 4) {
         HIPCK(hipMemcpyAsync(h_small, v.remaining, 4, hipMemcpyDeviceToHost, stream));
         HIPCK(hipStreamSynchronize(stream));
         if (h_small[0] == 0) break;
@@ -563,10 +591,16 @@ struct World {
     col_start.assign(size_t(ncolours) + 1, 0);
     HIPCK(hipMemcpyAsync(col_start.data(), v.col_start, (size_t(ncolours) + 1) * 4, hipMemcpyDeviceToHost, stream));
     HIPCK(hipStreamSynchronize(stream));
+    // colours left empty (inherited colours can leave gaps) start where the next one does
+    col_start[ncolours] = uint32_t(M);
+    for (int k = ncolours - 1; k >= 0; --k)
+      if (col_start[k] == 0xffffffffu) col_start[k] = col_start[k + 1];
+    colour_visit_order(uint64_t(solve_count), ncolours, colour_visit);
     return NPHYS_OK;
   }
 
   int solve() {
+    ++solve_count;  // counts every solve, as the checker does
     ContactBufs& cc = cb[cur];
     ContactBufs& pc = cb[cur ^ 1];
     const int M = cc.M, C = cc.C;
@@ -588,10 +622,10 @@ struct World {
     if ((rc = colour(b, cv, M, v))) return rc;
     launch_assemble(b, cv, pc.view(), prevM, M, sp, v, C, stream);
     HIPCK(hipEventRecord(ev[EV_ASM], stream));
-    for (int k = 0; k < ncolours; ++k)
+    for (int k : colour_visit)
       launch_warmstart_colour(b, cv, int(col_start[k]), int(col_start[k + 1] - col_start[k]), v, C, stream);
     for (int it = 0; it < prm.max_velocity_iterations; ++it)
-      for (int k = 0; k < ncolours; ++k)
+      for (int k : colour_visit)
         launch_pgs_colour(b, cv, int(col_start[k]), int(col_start[k + 1] - col_start[k]), v, C, stream);
     launch_residual(b, cv, M, v, C, stream);
     launch_cache_impulses(cv, v, C, stream);
@@ -609,7 +643,7 @@ struct World {
     res_rms = C > 0 ? std::sqrt(ss / double(3 * C)) : 0.0;
     if (keep_colours) {
       std::vector<uint32_t> st(M);
-      HIPCK(hipMemcpyAsync(st.data(), v.state, size_t(M) * 4, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipMemcpyAsync(st.data(), sv_state2[cur].p, size_t(M) * 4, hipMemcpyDeviceToHost, stream));
       HIPCK(hipStreamSynchronize(stream));
       last_colours.resize(M);
       for (int m = 0; m < M; ++m) last_colours[m] = int(st[m] & 0xffffu);
@@ -624,7 +658,7 @@ struct World {
     SolverBufs v = sv_view();
     DevContacts cv = cc.view();
     for (int it = 0; it < prm.max_position_iterations; ++it)
-      for (int k = 0; k < ncolours; ++k)
+      for (int k : colour_visit)
         launch_position_colour(b, cv, sp, int(col_start[k]), int(col_start[k + 1] - col_start[k]), v, stream);
     return NPHYS_OK;
   }
@@ -676,6 +710,7 @@ struct World {
     counters.nmanifolds = solved_M;
     counters.nconstraints = int64_t(3) * solved_C;
     counters.ncolours = ncolours;
+    counters.colour_rounds += colour_rounds;
     prm.t += prm.dt;
     return NPHYS_OK;
   }

@@ -227,11 +227,12 @@ def collide_shapes(kind1, dims1, pose1, kind2, dims2, pose2, prediction, precisi
             "depth": dep[:nc], "kind": kind[:nc]}
 
 
-def colour_manifolds(b1, b2, is_dynamic):
+def colour_manifolds(b1, b2, is_dynamic, initial=None):
+    """Greedy colouring; entries >= 0 of `initial` are kept as given."""
     L = lib()
     b1 = np.ascontiguousarray(b1, np.int32)
     b2 = np.ascontiguousarray(b2, np.int32)
     dyn = np.ascontiguousarray(is_dynamic, np.int32)
-    out = np.zeros(b1.shape[0], np.int32)
+    out = np.full(b1.shape[0], -1, np.int32) if initial is None else np.array(initial, np.int32)
     nc = L.orc_colour_manifolds(b1.shape[0], abi.iptr(b1), abi.iptr(b2), dyn.shape[0], abi.iptr(dyn), abi.iptr(out))
     return out, nc

@@ -378,7 +378,8 @@ int orc_colour_manifolds(int32_t nman, const int32_t* b1, const int32_t* b2, int
   std::vector<orc::Manifold> ms(nman);
   for (int m = 0; m < nman; ++m) ms[m] = orc::Manifold{b1[m], b2[m], 0, 1};
   std::vector<int> dyn(is_dynamic, is_dynamic + nbodies);
-  std::vector<int> col;
+  std::vector<int> col(nman);
+  for (int m = 0; m < nman; ++m) col[m] = colours[m];  // entries >= 0 are kept
   const int nc = orc::colour_manifolds(ms, dyn, nbodies, col);
   for (int m = 0; m < nman; ++m) colours[m] = col[m];
   return nc;

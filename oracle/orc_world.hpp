@@ -138,14 +138,35 @@ inline uint32_t colour_hash(uint32_t x) {
   return x;
 }
 
+// Order in which the colour classes of solve number `solve_index` are visited: a Fisher-Yates
+// shuffle of 0..ncolours-1 driven by SplitMix64 seeded with the solve index. Visiting the classes
+// in a new order every step re-randomises the relative order of every pair of conflicting
+// manifolds, as a fresh colouring would, while colours themselves persist between steps.
+inline void colour_visit_order(uint64_t solve_index, int ncolours, std::vector<int>& perm) {
+  perm.resize(ncolours);
+  for (int c = 0; c < ncolours; ++c) perm[c] = c;
+  uint64_t st = solve_index * 0x9E3779B97F4A7C15ull;
+  for (int i = ncolours - 1; i > 0; --i) {
+    st += 0x9E3779B97F4A7C15ull;
+    uint64_t z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    const int j = int(z % uint64_t(i + 1));
+    std::swap(perm[i], perm[j]);
+  }
+}
+
 // Deterministic Jones-Plassmann greedy colouring of manifolds; two manifolds conflict when they
-// share a dynamic body. Priority (hash(m), m); in each round every uncoloured manifold that beats
-// all its uncoloured neighbours takes the smallest colour unused by its coloured neighbours.
-// The result depends only on the manifold list, never on evaluation order.
+// share a dynamic body. Entries of `colour` that are >= 0 on entry are kept (colours inherited
+// from the previous step). Priority (hash(m), m); in each round every uncoloured manifold that
+// beats all its uncoloured neighbours takes the smallest colour unused by its coloured
+// neighbours. The result depends only on the inputs, never on evaluation order. Returns one more
+// than the largest colour.
 inline int colour_manifolds(const std::vector<Manifold>& ms, const std::vector<int>& is_dyn,
                             int nbodies, std::vector<int>& colour) {
   const int M = int(ms.size());
-  colour.assign(M, -1);
+  if (int(colour.size()) != M) colour.assign(M, -1);
   std::vector<int> off(nbodies + 1, 0);
   for (const auto& m : ms) {
     if (is_dyn[m.b1]) off[m.b1 + 1]++;
@@ -161,7 +182,11 @@ inline int colour_manifolds(const std::vector<Manifold>& ms, const std::vector<i
   std::vector<uint32_t> pri(M);
   for (int m = 0; m < M; ++m) pri[m] = colour_hash(uint32_t(m));
   auto beats = [&](int x, int y) { return pri[x] > pri[y] || (pri[x] == pri[y] && x > y); };
-  int remaining = M, ncol = 0;
+  int remaining = 0, ncol = 0;
+  for (int m = 0; m < M; ++m) {
+    if (colour[m] < 0) ++remaining;
+    else ncol = std::max(ncol, colour[m] + 1);
+  }
   std::vector<int> sel;
   while (remaining > 0) {
     sel.clear();
@@ -851,15 +876,39 @@ class World {
 
   // ------------------------------------------------------------------ colouring order
   std::vector<int> colour_order;
+  std::vector<Manifold> prev_coloured;  // manifolds coloured by the previous step, pair order
+  std::vector<int> prev_colours;
+
+  // Colours persist across steps: a manifold whose body pair was also a manifold in the previous
+  // step keeps that colour (the pair's conflicts are the same, so the partial colouring is valid),
+  // and only new manifolds are coloured greedily around them.
   void build_colour_order() {
     std::vector<int> is_dyn(bodies.size());
     for (size_t i = 0; i < bodies.size(); ++i) is_dyn[i] = bodies[i].status == STATUS_DYNAMIC;
+    colours.assign(solved.size(), -1);
+    for (size_t m = 0; m < solved.size(); ++m) {
+      const std::pair<int, int> key{solved[m].b1, solved[m].b2};
+      size_t lo = 0, hi = prev_coloured.size();
+      while (lo < hi) {
+        const size_t mid = (lo + hi) / 2;
+        const std::pair<int, int> k{prev_coloured[mid].b1, prev_coloured[mid].b2};
+        if (k < key) lo = mid + 1;
+        else hi = mid;
+      }
+      if (lo < prev_coloured.size() && prev_coloured[lo].b1 == key.first && prev_coloured[lo].b2 == key.second)
+        colours[m] = prev_colours[lo];
+    }
     ncolours = colour_manifolds(solved, is_dyn, int(bodies.size()), colours);
+    std::vector<int> perm;
+    colour_visit_order(uint64_t(solve_count), ncolours, perm);
     colour_order.clear();
-    for (int c = 0; c < ncolours; ++c)
+    for (int k = 0; k < ncolours; ++k)
       for (int m = 0; m < int(solved.size()); ++m)
-        if (colours[m] == c) colour_order.push_back(m);
+        if (colours[m] == perm[k]) colour_order.push_back(m);
+    prev_coloured = solved;
+    prev_colours = colours;
   }
+  long solve_count = 0;  // solves performed, never reset
 
   // ------------------------------------------------------------------ the step
   void solver_step() {  // MoreauJeanSolver::step (moreau_jean_solver.rs:47-90)
@@ -870,6 +919,7 @@ class World {
     assemble(solved);
     counters.assembly += now_s() - t0;
     t0 = now_s();
+    ++solve_count;
     if (order_mode == ORDER_COLOURED) build_colour_order();
     sor_prox();
     compute_residual();
