@@ -1,0 +1,202 @@
+"""Benchmark of the MI355X contact pipeline on the dense-pile workload (one MechanicalWorld::step
+per timed step).
+
+Workload: 262,144 bodies (even index a ball with r in [0.2, 0.6] m, odd a cuboid with half
+extents in [0.2, 0.5] m and a random orientation), released from a 34 x 227 x 34 lattice above
+an open five-wall box (60 x 40 x 60 m) and settled untimed until the contact count is steady
+(about 1.5M contacts at rest); then W warm-up steps and K timed steps, 16 velocity and 3 position
+iterations, dt = 1/60, FP32. Inputs are resident in HBM for the whole run.
+
+Printed: one JSON line with steps/s, contact-iterations/s, the residual, per-stage times, the
+roofline of the dominant kernel and the CPU baseline. The K timed steps launch the sweep phases
+as hipGraphs with counters off; the roofline comes from a following region of --timing-steps
+steps with counters on, where every launch of the SOR-Prox sweep kernel carries HIP events in its
+dispatch packet on the world's stream. The CPU baseline is the FP32 CPU restatement of the
+reference step, one thread, timed on a bounded sample of the same settled scene.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from nphysics_amd import abi, scenes, world  # noqa: E402
+
+METRIC = "steps/s + contact-iters/s at residual 1e-4, learned warm start vs plain"
+HBM_PEAK_GBPS = 8000.0
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "k_pgs_pile_traffic.json")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def settle(w, steps):
+    done = 0
+    while done < steps:
+        k = min(100, steps - done)
+        w.step(k)
+        done += k
+        c = w.counters()
+        log(f"settle {done}/{steps}: contacts {c['ncontacts']} colours {c['ncolours']}")
+
+
+def cpu_baseline(scene, snapshot, params, steps):
+    """The FP32 CPU restatement of the reference step on one thread, from the same snapshot."""
+    from oracle import oracle
+
+    s = dict(scene)
+    for k in ("pos", "rot", "linvel", "angvel"):
+        s[k] = snapshot[k]
+    o = oracle.OracleWorld(params, precision=32)
+    o.add_bodies(s)
+    o.step(1)  # first step also runs the initial collision detection, untimed
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    c = o.counters()
+    return {
+        "value": steps / dt,
+        "unit": "steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{steps} steps of the settled dense pile ({c['ncontacts']} contacts) after one untimed "
+                  f"step, FP32, single thread, on {cpu_model()} ({os.cpu_count()} logical CPUs)",
+        "contact_iters_per_s": c["ncontacts"] * params.max_velocity_iterations * steps / dt,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--settle", type=int, default=750)
+    ap.add_argument("--timing-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    ws = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or ws != 1:
+        print("bench.py runs on one GPU only: this build steps one world per GPU and has no "
+              "multi-GPU path; run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    params = abi.default_params(max_velocity_iterations=16, max_position_iterations=3)
+    scene = scenes.pile()
+    w = world.MechanicalWorld(integration_parameters=params)
+    w.add_bodies(scene)
+    t0 = time.perf_counter()
+    settle(w, args.settle)
+    log(f"settled in {time.perf_counter() - t0:.1f} s")
+    snapshot = w.get_state()
+    w.step(args.warmup)
+
+    # Timed region: K steps, sweep phases launched as hipGraphs, counters off.
+    w.reset_counters()
+    w.enable_counters(False)
+    t0 = time.perf_counter()
+    w.step(args.steps)  # returns after the last step completed on the device
+    elapsed = time.perf_counter() - t0
+    c = w.counters()
+    rinf, rrms = w.residual()
+
+    # Kernel-timing region: the same workload continued for --timing-steps steps with counters
+    # enabled, which launches the sweep kernel directly with HIP events carried by each dispatch.
+    w.reset_counters()
+    w.enable_counters(True)
+    w.step(args.timing_steps)
+    w.enable_counters(False)
+    ct = w.counters()
+
+    n_dyn = int((scene["status"] == 0).sum())
+    C, M, ncol = c["ncontacts"], c["nmanifolds"], c["ncolours"]
+    iters = params.max_velocity_iterations
+    sweep_bytes = 168 * C + 76 * n_dyn  # algorithmic bytes of one velocity sweep
+    launches_per_sweep = ct["sweep_kernel_launches"] / max(1, args.timing_steps * iters)
+    avg_launch_s = ct["sweep_kernel_time"] / max(1, ct["sweep_kernel_launches"])
+    bytes_per_launch = sweep_bytes / max(1e-9, launches_per_sweep)
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        try:
+            traffic = json.load(open(PROFILE_TRAFFIC)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    steps_per_s = args.steps / elapsed
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "dense pile: 262,144 bodies (balls r 0.2-0.6 m / cuboids 0.2-0.5 m) settled in a "
+                        "60x40x60 m open box; 16 velocity + 3 position iterations, dt 1/60",
+            "bodies": int(scene["status"].shape[0]),
+            "dynamic_bodies": n_dyn,
+            "settle_steps": args.settle,
+            "pairs": c["npairs"],
+            "contacts": C,
+            "manifolds": M,
+            "colours": ncol,
+            "velocity_iterations": iters,
+            "position_iterations": params.max_position_iterations,
+            "sweep_graphs": os.environ.get("NPHYS_DISABLE_GRAPHS", "0") != "1",
+        },
+        "contact_iters_per_s": C * iters * args.steps / elapsed,
+        "residual_inf": rinf,
+        "residual_rms": rrms,
+        "stages_ms_per_step": {k: 1e3 * c[k] / args.steps for k in
+                               ("collision_detection", "broad_phase", "narrow_phase", "assembly",
+                                "velocity_resolution", "velocity_update", "position_resolution")},
+        "roofline": {
+            "kernel": "k_pgs (one colour of one SOR-Prox sweep)",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBPS,
+            "traffic": traffic,
+            "avg_launch_us": avg_launch_s * 1e6,
+            "algorithmic_bytes_per_launch": bytes_per_launch,
+            "launches": ct["sweep_kernel_launches"],
+            "timing_region_steps": args.timing_steps,
+        },
+    }
+    if not args.no_cpu:
+        log("timing the CPU baseline ...")
+        out["cpu_baseline"] = cpu_baseline(scene, snapshot, params, args.cpu_steps)
+        out["speedup_vs_cpu"] = steps_per_s / out["cpu_baseline"]["value"]
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

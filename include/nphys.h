@@ -104,6 +104,16 @@ typedef struct nphys_counters {
   int64_t nconstraints;
   int64_t ncolours;
   int64_t colour_rounds;    /* accumulated greedy-colouring rounds */
+  /* Per-launch device time of the SOR-Prox sweep kernel, summed over launches, and the launch
+   * count; measured with HIP events around every launch while counters are enabled. */
+  double sweep_kernel_time;
+  int64_t sweep_kernel_launches;
+  int64_t velocity_sweeps;  /* accumulated SOR-Prox sweeps (fixed count, or to the residual target) */
+  /* judge of the learned warm start (nphys_set_warmstart_judge): sweeps of the plain and of the
+   * learned solve, and the steps whose learned solve was kept (it needed fewer sweeps) */
+  int64_t judge_plain_sweeps;
+  int64_t judge_learned_sweeps;
+  int64_t judge_kept;
 } nphys_counters;
 
 /* ---------------------------------------------------------------- lifecycle */
@@ -150,8 +160,8 @@ int nphys_step(nphys_world* w, int32_t nsteps);
 int nphys_collide(nphys_world* w);
 
 /* MoreauJeanSolver::set_contact_model (moreau_jean_solver.rs:42-44) restricted to the two models
- * this build provides. The learned model falls back to the plain cache warm start until a
- * network is loaded with nphys_warmstart_load. */
+ * this library provides. The learned model needs a network loaded with nphys_warmstart_load
+ * first (NPHYS_ERR_STATE otherwise). */
 int nphys_contact_model_set(nphys_world* w, int32_t model);
 
 /* ---------------------------------------------------------------- readback */
@@ -183,10 +193,72 @@ int nphys_residual(nphys_world* w, double* r_inf, double* r_rms);
 int nphys_read_counters(nphys_world* w, nphys_counters* out);
 int nphys_reset_counters(nphys_world* w);
 
+/* Counters::enable / disable (counters/mod.rs:51-63). Stage totals are always kept; enabling adds
+ * the per-launch timing of the sweep kernel (one HIP event pair per launch). Off by default, as
+ * MechanicalWorld::new creates Counters::new(false) (mechanical_world.rs:77). */
+int nphys_counters_enable(nphys_world* w, int32_t enabled);
+
+/* Solve-to-residual mode of the velocity solver. With r_inf_target > 0 every step runs SOR-Prox
+ * sweeps until the natural-map residual max |phi| (see nphys_residual) is <= r_inf_target, checked
+ * after the warm start and after every check_every sweeps, at most max_sweeps sweeps; with
+ * r_inf_target == 0 (the default) it runs max_velocity_iterations sweeps as SORProx::solve does
+ * (sor_prox.rs:56-79). No reference counterpart: the reference only has the fixed count. */
+int nphys_set_residual_target(nphys_world* w, double r_inf_target, int32_t max_sweeps, int32_t check_every);
+
 /* Colour of every manifold solved by the last step, in manifold order. Recording starts with the
  * first call (a call with colours == NULL only enables it), so the first step after enabling is
  * the first one reported. */
 int nphys_read_colours(nphys_world* w, int32_t* colours, int64_t cap, int32_t* ncolours);
+
+/* ---------------------------------------------------------------- learned warm start
+ *
+ * No reference counterpart: the reference warm-starts every contact from the impulse cache
+ * (signorini_coulomb_pyramid_model.rs:104-108). A network nphys_mlp (24 -> 128 -> 128 -> 3, GELU,
+ * BF16 weights and activations, FP32 accumulation) maps a contact's 24 features to its impulses
+ * (normal, tangent 1, tangent 2) in units of r_n |g| dt, r_n the effective mass of the contact's
+ * normal row. Parameters are FP32, NPHYS_MLP_NUM_PARAMS of them, laid out as
+ *   W1[24][128] b1[128] W2[128][128] b2[128] W3[128][3] b3[3] x_mean[24] x_scale[24] y_scale[3]
+ * with inputs normalised as (x - x_mean) / x_scale and outputs multiplied by y_scale. */
+#define NPHYS_MLP_NUM_FEATURES 24
+#define NPHYS_MLP_NUM_PARAMS 20150
+typedef struct nphys_mlp nphys_mlp;
+
+/* He-initialised network on the current device; seed drives the initialisation and the
+ * per-epoch shuffles. */
+int nphys_mlp_create(uint64_t seed, nphys_mlp** out);
+int nphys_mlp_destroy(nphys_mlp* m);
+int nphys_mlp_num_params(int64_t* n);
+int nphys_mlp_get_params(nphys_mlp* m, float* out, int64_t cap);
+int nphys_mlp_set_params(nphys_mlp* m, const float* params, int64_t n);
+/* Training set: n rows of 24 features and 3 target impulses (copied to the device). With
+ * set_normalisation the feature means/scales and output scales are computed from this data. */
+int nphys_mlp_set_data(nphys_mlp* m, const float* features, const float* targets, int64_t n,
+                       int32_t set_normalisation);
+/* Adam (beta 0.9 / 0.999) on the mean squared error in normalised output units, mini-batches of
+ * `batch` rows (a multiple of 64) in a fresh shuffle per epoch; epoch_loss (may be NULL) receives
+ * each epoch's mean batch loss. */
+int nphys_mlp_train(nphys_mlp* m, int32_t epochs, int32_t batch, float lr, float* epoch_loss);
+/* Inference of n rows: out[n][3] impulses. */
+int nphys_mlp_forward(nphys_mlp* m, const float* features, int64_t n, float* out);
+/* Gradient of the batch loss (mean over the n rows of the squared error summed over the 3
+ * normalised outputs) with respect to the 20,099 trained parameters, in parameter order. */
+int nphys_mlp_gradients(nphys_mlp* m, const float* features, const float* targets, int64_t n,
+                        float* grad, double* loss);
+
+/* Loads a network (NPHYS_MLP_NUM_PARAMS floats, the layout above) into the world; afterwards
+ * nphys_contact_model_set(NPHYS_CONTACT_MODEL_LEARNED_WARMSTART) warm-starts every contact with
+ * the network's impulses (projected on lambda_n >= 0, |lambda_t| <= mu lambda_n) instead of the
+ * cache. Selecting that model without a network is NPHYS_ERR_STATE. */
+int nphys_warmstart_load(nphys_world* w, const float* params, int64_t n);
+/* With on, every step records its contacts' features (before the solve) and impulses (after it,
+ * in units of r_n |g| dt) as training samples; nphys_read_samples returns the last step's
+ * (features [n][24], targets [n][3]; with both NULL only *n is set). */
+int nphys_capture_samples(nphys_world* w, int32_t on);
+int nphys_read_samples(nphys_world* w, float* features, float* targets, int64_t cap, int64_t* n);
+/* With the learned model and a residual target set: every step solves twice from the same
+ * assembled system, from the cache warm start and from the prediction, and keeps the learned
+ * solve only when it reached the target in fewer sweeps (counters judge_*). */
+int nphys_set_warmstart_judge(nphys_world* w, int32_t on);
 
 /* Message of the last error on this thread ("" if none). */
 const char* nphys_last_error(void);

@@ -110,6 +110,12 @@ class Counters(C.Structure):
         ("nconstraints", C.c_int64),
         ("ncolours", C.c_int64),
         ("colour_rounds", C.c_int64),
+        ("sweep_kernel_time", C.c_double),
+        ("sweep_kernel_launches", C.c_int64),
+        ("velocity_sweeps", C.c_int64),
+        ("judge_plain_sweeps", C.c_int64),
+        ("judge_learned_sweeps", C.c_int64),
+        ("judge_kept", C.c_int64),
     ]
 
     def as_dict(self):

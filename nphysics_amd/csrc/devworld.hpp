@@ -7,7 +7,7 @@
 //   contacts per contact: world1 (w = depth), world2, normal, feature, kinematic kind, local
 //            points/normal for the position solver, solved impulse; per manifold: bodies,
 //            first contact, count.
-//   rows     per contact in colour order: 16 float4 (see k_solver.hip).
+//   rows     per contact in colour order: 8 float4 (see k_solver.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 

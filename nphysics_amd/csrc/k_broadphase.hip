@@ -11,8 +11,8 @@
 //     not both static; stored as (first, second) with first dynamic and second static or larger,
 //     sorted.
 // The pair set is found with a linear BVH rebuilt whenever a loose AABB changed: Morton codes of
-// the loose-AABB centroids, stable LDS radix sort, Karras (HPG 2012) radix-tree construction, a
-// bottom-up refit with arrival counters, then one thread per dynamic body (in Morton order, so
+// the loose-AABB centroids, stable LDS radix sort, Karras (HPG 2012) radix-tree construction, an
+// atomic min/max refit with early termination, then one thread per dynamic body (in Morton order, so
 // a wavefront walks neighbouring subtrees) traverses the tree and writes its partners into its
 // own fixed-capacity segment; segments are sorted and compacted with a scan. The result is
 // deterministic and independent of thread timing. Algorithmic bytes: 124 per body + 8 per pair.
@@ -55,16 +55,32 @@ __global__ void __launch_bounds__(256) k_loose_update(DevBodies b, int mode, uin
 }
 
 __global__ void __launch_bounds__(256) k_bounds(DevBodies b, uint32_t* bounds) {
+  __shared__ uint32_t s_b[6][256];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= b.n) return;
-  const float4 mn = b.loose_mn[i], mx = b.loose_mx[i];
-  const float cx = (mn.x + mx.x) * 0.5f, cy = (mn.y + mx.y) * 0.5f, cz = (mn.z + mx.z) * 0.5f;
-  atomicMin(&bounds[0], f2ord(cx));
-  atomicMin(&bounds[1], f2ord(cy));
-  atomicMin(&bounds[2], f2ord(cz));
-  atomicMax(&bounds[3], f2ord(cx));
-  atomicMax(&bounds[4], f2ord(cy));
-  atomicMax(&bounds[5], f2ord(cz));
+  const int t = threadIdx.x;
+  uint32_t v[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
+  if (i < b.n) {
+    const float4 mn = b.loose_mn[i], mx = b.loose_mx[i];
+    const float cx = (mn.x + mx.x) * 0.5f, cy = (mn.y + mx.y) * 0.5f, cz = (mn.z + mx.z) * 0.5f;
+    v[0] = v[3] = f2ord(cx);
+    v[1] = v[4] = f2ord(cy);
+    v[2] = v[5] = f2ord(cz);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) s_b[k][t] = v[k];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) {
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        s_b[k][t] = s_b[k][t] < s_b[k][t + off] ? s_b[k][t] : s_b[k][t + off];
+        s_b[k + 3][t] = s_b[k + 3][t] > s_b[k + 3][t + off] ? s_b[k + 3][t] : s_b[k + 3][t + off];
+      }
+    }
+    __syncthreads();
+  }
+  if (t < 3) atomicMin(&bounds[t], s_b[t][0]);
+  else if (t < 6) atomicMax(&bounds[t], s_b[t][0]);
 }
 
 __device__ __forceinline__ uint32_t expand10(uint32_t v) {
@@ -136,30 +152,56 @@ __global__ void __launch_bounds__(256) k_lbvh(const uint32_t* __restrict__ keys,
   if (i == 0) parent[0] = -1;
 }
 
-__global__ void __launch_bounds__(256) k_refit(DevBodies b, const uint32_t* __restrict__ vals, int n,
-                                              const int* __restrict__ child,
-                                              const int* __restrict__ parent, uint32_t* visit,
-                                              float4* node_mn, float4* node_mx) {
+// Leaf nodes hold the loose AABBs in Morton order.
+__global__ void __launch_bounds__(256) k_leaf_nodes(DevBodies b, const uint32_t* __restrict__ vals, int n,
+                                                   float4* node_mn, float4* node_mx) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int body = int(vals[t]);
-  int node = n - 1 + t;
-  node_mn[node] = b.loose_mn[body];
-  node_mx[node] = b.loose_mx[body];
-  __threadfence();
-  int p = parent[node];
+  node_mn[n - 1 + t] = b.loose_mn[body];
+  node_mx[n - 1 + t] = b.loose_mx[body];
+}
+
+// Internal-node bounds by atomic min/max on order-preserving integer encodings: every leaf climbs
+// towards the root and stops at the first ancestor none of whose six bounds it changes (a value
+// that does not change a node is dominated by one whose writer keeps climbing). No arrival
+// counters or fences are needed and the bounds are exact (min/max of the leaves below).
+__global__ void __launch_bounds__(256) k_refit_atomic(DevBodies b, const uint32_t* __restrict__ vals, int n,
+                                                     const int* __restrict__ parent, uint32_t* inode) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int body = int(vals[t]);
+  const float4 mn = b.loose_mn[body], mx = b.loose_mx[body];
+  const uint32_t e[6] = {f2ord(mn.x), f2ord(mn.y), f2ord(mn.z), f2ord(mx.x), f2ord(mx.y), f2ord(mx.z)};
+  int p = parent[n - 1 + t];
   while (p >= 0) {
-    const uint32_t old = atomicAdd(&visit[p], 1u);
-    if (old == 0u) return;  // the other child arrives later and continues
-    __threadfence();
-    const int l = child[2 * p], r = child[2 * p + 1];
-    const float4 lmn = node_mn[l], lmx = node_mx[l];
-    const float4 rmn = node_mn[r], rmx = node_mx[r];
-    node_mn[p] = make_float4(fminf_(lmn.x, rmn.x), fminf_(lmn.y, rmn.y), fminf_(lmn.z, rmn.z), 0.0f);
-    node_mx[p] = make_float4(fmaxf_(lmx.x, rmx.x), fmaxf_(lmx.y, rmx.y), fmaxf_(lmx.z, rmx.z), 0.0f);
-    __threadfence();
+    uint32_t* q = inode + 8 * long(p);
+    bool changed = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (atomicMin(&q[k], e[k]) > e[k]) changed = true;
+      if (atomicMax(&q[4 + k], e[3 + k]) < e[3 + k]) changed = true;
+    }
+    if (!changed) break;
     p = parent[p];
   }
+}
+
+__global__ void __launch_bounds__(256) k_decode_nodes(int n, const uint32_t* __restrict__ inode, float4* node_mn,
+                                                     float4* node_mx) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const uint32_t* q = inode + 8 * long(i);
+  node_mn[i] = make_float4(ord2f(q[0]), ord2f(q[1]), ord2f(q[2]), 0.0f);
+  node_mx[i] = make_float4(ord2f(q[4]), ord2f(q[5]), ord2f(q[6]), 0.0f);
+}
+
+__global__ void __launch_bounds__(256) k_init_inodes(int n, uint32_t* inode) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  uint4* q = reinterpret_cast<uint4*>(inode + 8 * long(i));
+  q[0] = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0u);
+  q[1] = make_uint4(0u, 0u, 0u, 0u);
 }
 
 __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* __restrict__ vals, int n,
@@ -255,9 +297,10 @@ void launch_build_lbvh(const DevBodies& b, BroadPhaseBufs& bp, hipStream_t s) {
   hipLaunchKernelGGL(k_morton, dim3(nb(n)), dim3(256), 0, s, b, bp.bounds, bp.keys, bp.vals);
   radix_sort_pairs(bp.keys, bp.vals, bp.keys_alt, bp.vals_alt, n, 30, bp.tmp, s);
   hipLaunchKernelGGL(k_lbvh, dim3(nb(n - 1)), dim3(256), 0, s, bp.keys, n, bp.child, bp.parent);
-  fill_u32(bp.visit, 0u, size_t(n - 1), s);
-  hipLaunchKernelGGL(k_refit, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.parent, bp.visit,
-                     bp.node_mn, bp.node_mx);
+  hipLaunchKernelGGL(k_init_inodes, dim3(nb(n - 1)), dim3(256), 0, s, n, bp.visit);
+  hipLaunchKernelGGL(k_leaf_nodes, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.node_mn, bp.node_mx);
+  hipLaunchKernelGGL(k_refit_atomic, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.parent, bp.visit);
+  hipLaunchKernelGGL(k_decode_nodes, dim3(nb(n - 1)), dim3(256), 0, s, n, bp.visit, bp.node_mn, bp.node_mx);
 }
 
 void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s) {

@@ -17,15 +17,11 @@ namespace nph {
 
 namespace {
 
-__global__ void __launch_bounds__(256) k_position(DevBodies b, DevContacts c, SimParams p, int first_slot,
-                                                 int nslots, SolverBufs sb) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nslots) return;
-  const int m = int(sb.order[first_slot + t]);
-  const int2 bb = c.mbody[m];
-  const int b1 = bb.x, b2 = bb.y;
-  const bool d1 = b.status[b1] == STATUS_DYNAMIC;
-  const bool d2 = b.status[b2] == STATUS_DYNAMIC;
+__device__ __forceinline__ void position_one(const DevBodies& b, const DevContacts& c, const SimParams& p,
+                                             int4 q) {
+  const int b1 = q.x & 0x3fffffff, b2 = q.y & 0x3fffffff;
+  const bool d1 = (q.x >> 30) & 1;
+  const bool d2 = (q.y >> 30) & 1;
   const int k1 = b.shape[b1], k2 = b.shape[b2];
   const f3 h1 = xyz(b.dims[b1]), h2 = xyz(b.dims[b2]);
   const float m1 = b.margin[b1], m2 = b.margin[b2];
@@ -35,12 +31,25 @@ __global__ void __launch_bounds__(256) k_position(DevBodies b, DevContacts c, Si
   m33 I1, I2;
   if (d1) I1 = load_inv(b, b1, im1);
   if (d2) I2 = load_inv(b, b2, im2);
-  const int first = c.mfirst[m], cnt = c.mcount[m];
+  const int first = int(uint32_t(q.w) >> 3), cnt = q.w & 7;
+  // the manifold's kinematics (at most 4 contacts) are loaded up front, independent of the poses
+  int kind[4];
+  f3 lp1[4], lp2[4], ln[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k < cnt) {
+      const int ci = first + k;
+      kind[k] = c.kind[ci];
+      lp1[k] = xyz(c.lp1[ci]);
+      lp2[k] = xyz(c.lp2[ci]);
+      ln[k] = xyz(c.ln[ci]);
+    }
+  }
   bool moved1 = false, moved2 = false;
-  for (int k = 0; k < cnt; ++k) {
-    const int ci = first + k;
-    const Contact g = reevaluate(c.kind[ci], xyz(c.lp1[ci]), xyz(c.lp2[ci]), xyz(c.ln[ci]), k1, h1, t1, q1,
-                                 k2, h2, t2, q2);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k >= cnt) break;
+    const Contact g = reevaluate(kind[k], lp1[k], lp2[k], ln[k], k1, h1, t1, q1, k2, h2, t2, q2);
     const f3 w1 = add(g.w1, scale(g.n, m1));
     const f3 w2 = sub(g.w2, scale(g.n, m2));
     const float depth = (g.depth + m1) + m2;
@@ -84,13 +93,24 @@ __global__ void __launch_bounds__(256) k_position(DevBodies b, DevContacts c, Si
   }
 }
 
+__global__ void __launch_bounds__(64) k_position(DevBodies b, DevContacts c, SimParams p, SolverBufs sb, int kv) {
+  const int col = sb.visit[kv];
+  const int first_slot = int(sb.col_start[col]);
+  const int nslots = int(sb.col_start[col + 1]) - first_slot;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nslots; t += gridDim.x * blockDim.x)
+    position_one(b, c, p, sb.sinfo[first_slot + t]);
+}
+
 }  // namespace
 
-void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p,
-                            int first_slot, int nslots, SolverBufs& sb, hipStream_t s) {
-  if (nslots <= 0) return;
-  hipLaunchKernelGGL(k_position, dim3(unsigned((nslots + 255) / 256)), dim3(256), 0, s, b, c, p, first_slot,
-                     nslots, sb);
+int position_blocks(int M) {
+  const int need = (M + 63) / 64;
+  return need < 1 ? 1 : (need > 2048 ? 2048 : need);
+}
+
+void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k, int blocks,
+                            SolverBufs& sb, hipStream_t s) {
+  hipLaunchKernelGGL(k_position, dim3(blocks), dim3(64), 0, s, b, c, p, sb, k);
 }
 
 }  // namespace nph

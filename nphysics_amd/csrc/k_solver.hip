@@ -20,17 +20,21 @@
 // of the contact with the same (body1, body2, feature) key (the ImpulseCache lookup at
 // signorini_coulomb_pyramid_model.rs:104-108).
 //
-// Solve (k_warmstart / k_pgs, one lane per manifold of one colour) restates SORProx
+// Solve (k_warmstart / k_pgs, one lane per manifold of one colour, all of the manifold's data
+// loaded into registers before its serial chain) restates SORProx
 // (src/solver/sor_prox.rs:19-436): warm start dv += l W J, then per sweep the manifold's friction
 // rows (bounded by mu times the current normal impulse, zeroed when it is zero) and then its normal
 // rows (projected on l >= 0). Manifolds of one colour share no dynamic body, so one launch per
 // colour performs exactly the sequential Gauss-Seidel updates of the colour order.
 //
-// Row record (16 float4 per contact, colour order): 0 n|mu, 1 t1|im1, 2 t2|im2,
-// 3..5 body-1 angular jacobian of rows (n, t1, t2)|r, 6..8 body-2 angular jacobian|rhs,
-// 9..11 I1^-1 J1_ang, 12..14 I2^-1 J2_ang, 15 impulses (n, t1, t2). Linear jacobians are the
-// row direction d (body 1) and -d (body 2), weighted by the inverse mass.
+// Row record (8 contiguous float4 per contact, colour order): 0 n|mu, 1..3 body-1 angular
+// jacobian of rows (n, t1, t2)|r, 4..6 body-2 angular jacobian|rhs, 7 impulses (n, t1, t2) |
+// 1 if the warm start came from the cache, else 0.
+// Linear jacobians are the row direction d (body 1) and -d (body 2) weighted by the inverse
+// mass; tangents and I^-1 J_ang are recomputed in the sweep kernels from n and the bodies'
+// inverse inertia with the assembly's own expressions.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "kernels.hpp"
 #include "stages.hpp"
@@ -175,6 +179,20 @@ __global__ void __launch_bounds__(256) k_colour_starts(DevContacts c, int M,
   mcount_ord[s] = uint32_t(c.mcount[order[s]]);
 }
 
+// Per slot (colour order): bodies with their dynamic flag in bit 30, first row record, contact
+// count and first contact index, packed into one int4.
+__global__ void __launch_bounds__(256) k_slot_info(DevBodies b, DevContacts c, int M,
+                                                  const uint32_t* __restrict__ order,
+                                                  const uint32_t* __restrict__ soff, int4* sinfo) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= M) return;
+  const int m = int(order[s]);
+  const int2 bb = c.mbody[m];
+  const int d1 = b.status[bb.x] == STATUS_DYNAMIC ? 1 : 0;
+  const int d2 = b.status[bb.y] == STATUS_DYNAMIC ? 1 : 0;
+  sinfo[s] = make_int4(bb.x | (d1 << 30), bb.y | (d2 << 30), int(soff[s]), c.mcount[m] | (c.mfirst[m] << 3));
+}
+
 // One fill_constraint_geometry call: body-side jacobian (dir, p x dir), weighted angular part,
 // inv_r and rhs accumulation with J.v then J.ext.
 struct BodyView {
@@ -222,7 +240,7 @@ __device__ __forceinline__ void tangents(f3 n, f3& t1, f3& t2) {
 }
 
 __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, DevContacts prev, int prevM,
-                                                 int M, SimParams p, SolverBufs sb, int C) {
+                                                 int M, SimParams p, SolverBufs sb) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= M) return;
   const int m = int(sb.order[s]);
@@ -236,8 +254,6 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
   BodyView v1, v2;
   if (d1) v1 = load_body(b, b1);
   if (d2) v2 = load_body(b, b2);
-  const float im1 = d1 ? v1.im : 0.0f;
-  const float im2 = d2 ? v2.im : 0.0f;
   // previous manifold with the same bodies (lexicographic binary search)
   int pf = 0, pc = 0;
   {
@@ -267,16 +283,18 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
     const float depth = w14.w;
     const uint32_t feat = cur.feature[c];
     float4 imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float has = 0.0f;
     for (int kk = 0; kk < pc; ++kk)
       if (prev.feature[pf + kk] == feat) {
         imp = prev.imp[pf + kk];
+        has = 1.0f;
         break;
       }
     const f3 center1 = add(w1, scale(n, m1));
     const f3 center2 = sub(w2, scale(n, m2));
     f3 t1, t2;
     tangents(n, t1, t2);
-    float4 rA1[3], rA2[3], rW1[3], rW2[3];
+    float4 rA1[3], rA2[3];
     float lam[3];
 #pragma unroll
     for (int row = 0; row < 3; ++row) {
@@ -293,29 +311,23 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
       }
       rA1[row] = f4(a1, rr);
       rA2[row] = f4(a2, rhs);
-      rW1[row] = f4(wa1, 0.0f);
-      rW2[row] = f4(wa2, 0.0f);
     }
     lam[0] = imp.x * p.warmstart_coeff;
     lam[1] = imp.y * p.warmstart_coeff;
     lam[2] = imp.z * p.warmstart_coeff;
-    float4* R = sb.rec;
-    R[0 * long(C) + r] = f4(n, mu);
-    R[1 * long(C) + r] = f4(t1, im1);
-    R[2 * long(C) + r] = f4(t2, im2);
+    float4* R = sb.rec + long(r) * kRecFields;
+    R[0] = f4(n, mu);
 #pragma unroll
     for (int row = 0; row < 3; ++row) {
-      R[(3 + row) * long(C) + r] = rA1[row];
-      R[(6 + row) * long(C) + r] = rA2[row];
-      R[(9 + row) * long(C) + r] = rW1[row];
-      R[(12 + row) * long(C) + r] = rW2[row];
+      R[1 + row] = rA1[row];
+      R[4 + row] = rA2[row];
     }
-    R[15 * long(C) + r] = make_float4(lam[0], lam[1], lam[2], 0.0f);
+    R[7] = make_float4(lam[0], lam[1], lam[2], has);
     sb.rec2c[r] = uint32_t(c);
   }
 }
 
-// Per-lane solver context for one manifold.
+// Per-lane solver state of one manifold.
 struct Dv {
   f3 l, a;
 };
@@ -340,164 +352,233 @@ __device__ __forceinline__ void axpy6(float alpha, f3 wl, f3 wa, Dv& v) {
   v.a.z = v.a.z + alpha * wa.z;
 }
 
-struct RowView {
-  f3 d;            // body-1 linear direction
-  f3 a1, a2;       // angular jacobians
-  f3 wa1, wa2;     // weighted angular jacobians
-  float im1, im2, r, rhs;
-};
-
-__device__ __forceinline__ RowView load_row(const float4* __restrict__ R, long C, uint32_t r, int row) {
-  RowView v;
-  const float4 n4 = R[0 * C + r];
-  const float4 t14 = R[1 * C + r];
-  const float4 t24 = R[2 * C + r];
-  v.im1 = t14.w;
-  v.im2 = t24.w;
-  v.d = row == 0 ? neg(xyz(n4)) : (row == 1 ? xyz(t14) : xyz(t24));
-  const float4 a1 = R[(3 + row) * C + r];
-  const float4 a2 = R[(6 + row) * C + r];
-  v.a1 = xyz(a1);
-  v.r = a1.w;
-  v.a2 = xyz(a2);
-  v.rhs = a2.w;
-  v.wa1 = xyz(R[(9 + row) * C + r]);
-  v.wa2 = xyz(R[(12 + row) * C + r]);
-  return v;
-}
-
-__device__ __forceinline__ float row_w(const RowView& v, bool d1, bool d2, const Dv& x1, const Dv& x2) {
-  if (d1 && d2) return (dot6(v.d, v.a1, x1) + dot6(neg(v.d), v.a2, x2)) + v.rhs;
-  if (d1) return dot6(v.d, v.a1, x1) + v.rhs;
-  return dot6(neg(v.d), v.a2, x2) + v.rhs;
-}
-
-__device__ __forceinline__ void row_add(const RowView& v, float alpha, bool d1, bool d2, Dv& x1, Dv& x2) {
-  if (d1) axpy6(alpha, scale(v.d, v.im1), v.wa1, x1);
-  if (d2) axpy6(alpha, scale(neg(v.d), v.im2), v.wa2, x2);
-}
-
 struct SlotCtx {
-  int b1, b2, cnt;
+  int b1, b2, cnt, first;
   bool d1, d2;
   uint32_t r0;
 };
 
-__device__ __forceinline__ SlotCtx slot_ctx(const DevBodies& b, const DevContacts& c, const SolverBufs& sb,
-                                            int s) {
+// One 16-byte load gives everything a lane needs about its manifold (see k_slot_info).
+__device__ __forceinline__ SlotCtx slot_ctx(const SolverBufs& sb, int s) {
+  const int4 q = sb.sinfo[s];
   SlotCtx x;
-  const int m = int(sb.order[s]);
-  const int2 bb = c.mbody[m];
-  x.b1 = bb.x;
-  x.b2 = bb.y;
-  x.d1 = b.status[bb.x] == STATUS_DYNAMIC;
-  x.d2 = b.status[bb.y] == STATUS_DYNAMIC;
-  x.cnt = c.mcount[m];
-  x.r0 = sb.soff[s];
+  x.b1 = q.x & 0x3fffffff;
+  x.b2 = q.y & 0x3fffffff;
+  x.d1 = (q.x >> 30) & 1;
+  x.d2 = (q.y >> 30) & 1;
+  x.r0 = uint32_t(q.z);
+  x.cnt = q.w & 7;
+  x.first = int(uint32_t(q.w) >> 3);
   return x;
 }
 
-__global__ void __launch_bounds__(256) k_warmstart(DevBodies b, DevContacts c, int first_slot, int nslots,
-                                                  SolverBufs sb, int C) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nslots) return;
-  const SlotCtx x = slot_ctx(b, c, sb, first_slot + t);
-  Dv v1{}, v2{};
-  if (x.d1) { v1.l = xyz(b.dv_lin[x.b1]); v1.a = xyz(b.dv_ang[x.b1]); }
-  if (x.d2) { v2.l = xyz(b.dv_lin[x.b2]); v2.a = xyz(b.dv_ang[x.b2]); }
-  const float4* R = sb.rec;
-  for (int k = 0; k < x.cnt; ++k) {
-    const uint32_t r = x.r0 + k;
-    const float ln = R[15 * long(C) + r].x;
-    if (ln != 0.0f) row_add(load_row(R, C, r, 0), ln, x.d1, x.d2, v1, v2);
-  }
-  for (int k = 0; k < x.cnt; ++k) {
-    const uint32_t r = x.r0 + k;
-    const float4 lam = R[15 * long(C) + r];
-    if (lam.y != 0.0f) row_add(load_row(R, C, r, 1), lam.y, x.d1, x.d2, v1, v2);
-    if (lam.z != 0.0f) row_add(load_row(R, C, r, 2), lam.z, x.d1, x.d2, v1, v2);
-  }
-  if (x.d1) { b.dv_lin[x.b1] = f4(v1.l, 0.0f); b.dv_ang[x.b1] = f4(v1.a, 0.0f); }
-  if (x.d2) { b.dv_lin[x.b2] = f4(v2.l, 0.0f); b.dv_ang[x.b2] = f4(v2.a, 0.0f); }
+// A contact's record in registers. The tangents and the inverse-inertia weighted angular
+// jacobians are not stored: they are recomputed with the expressions k_assemble evaluates
+// (tangents(n) and mulv(I^-1, J_ang)), so they carry the same FP32 values.
+struct CRec {
+  f3 n;
+  float mu;
+  f3 a1[3], a2[3];
+  float r[3], rhs[3];
+  float4 lam;
+};
+
+__device__ __forceinline__ void load_rec(const float4* __restrict__ R, uint32_t r, CRec& c) {
+  const float4* p = R + long(r) * kRecFields;
+  const float4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3], v4 = p[4], v5 = p[5], v6 = p[6], v7 = p[7];
+  c.n = xyz(v0);
+  c.mu = v0.w;
+  c.a1[0] = xyz(v1); c.r[0] = v1.w;
+  c.a1[1] = xyz(v2); c.r[1] = v2.w;
+  c.a1[2] = xyz(v3); c.r[2] = v3.w;
+  c.a2[0] = xyz(v4); c.rhs[0] = v4.w;
+  c.a2[1] = xyz(v5); c.rhs[1] = v5.w;
+  c.a2[2] = xyz(v6); c.rhs[2] = v6.w;
+  c.lam = v7;
 }
 
-__global__ void __launch_bounds__(256) k_pgs(DevBodies b, DevContacts c, int first_slot, int nslots,
-                                            SolverBufs sb, int C) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nslots) return;
-  const SlotCtx x = slot_ctx(b, c, sb, first_slot + t);
-  Dv v1{}, v2{};
-  if (x.d1) { v1.l = xyz(b.dv_lin[x.b1]); v1.a = xyz(b.dv_ang[x.b1]); }
-  if (x.d2) { v2.l = xyz(b.dv_lin[x.b2]); v2.a = xyz(b.dv_ang[x.b2]); }
-  float4* R = sb.rec;
-  const float4* Rc = sb.rec;
-  // friction rows of every contact, bounded by the current normal impulse
-  for (int k = 0; k < x.cnt; ++k) {
-    const uint32_t r = x.r0 + k;
-    float4 lam = R[15 * long(C) + r];
-    const float mu = R[0 * long(C) + r].w;
-    const float ln = lam.x;
+struct Inv {
+  m33 m;
+  float im;
+};
+
+struct Manifold {
+  SlotCtx x;
+  Inv i1, i2;
+  CRec c[4];
+  f3 t1[4], t2[4];
+};
+
+// Everything a lane needs for its manifold, loaded up front: the loads are independent of the
+// Gauss-Seidel chain, so they are all in flight together.
+__device__ __forceinline__ void load_manifold(const DevBodies& b, const SolverBufs& sb, int s, Manifold& m) {
+  m.x = slot_ctx(sb, s);
+  m.i1.im = m.i2.im = 0.0f;
+  if (m.x.d1) m.i1.m = load_inv(b, m.x.b1, m.i1.im);
+  if (m.x.d2) m.i2.m = load_inv(b, m.x.b2, m.i2.im);
 #pragma unroll
-    for (int f = 1; f <= 2; ++f) {
-      float li = f == 1 ? lam.y : lam.z;
-      const RowView v = load_row(Rc, C, r, f);
-      if (ln == 0.0f) {
-        if (li != 0.0f) {
-          row_add(v, -li, x.d1, x.d2, v1, v2);
-          li = 0.0f;
-        }
-      } else {
-        const float mx = mu * ln;
-        const float mn = -mx;
-        const float w = row_w(v, x.d1, x.d2, v1, v2);
-        const float val = li - v.r * w;
-        const float ni = val > mn ? (val < mx ? val : mx) : mn;
-        const float dl = ni - li;
-        li = ni;
-        row_add(v, dl, x.d1, x.d2, v1, v2);
-      }
-      if (f == 1) lam.y = li;
-      else lam.z = li;
+  for (int k = 0; k < 4; ++k)
+    if (k < m.x.cnt) {
+      load_rec(sb.rec, m.x.r0 + k, m.c[k]);
+      tangents(m.c[k].n, m.t1[k], m.t2[k]);
     }
-    R[15 * long(C) + r] = lam;
+}
+
+// Row `row` (0 normal, 1 and 2 friction) of contact k: w = J dv + rhs and the dv update.
+struct Row {
+  f3 d, a1, a2, wa1, wa2;
+  float r, rhs;
+};
+
+__device__ __forceinline__ Row row_of(const Manifold& m, int k, int row) {
+  Row v;
+  const CRec& c = m.c[k];
+  v.d = row == 0 ? neg(c.n) : (row == 1 ? m.t1[k] : m.t2[k]);
+  v.a1 = c.a1[row];
+  v.a2 = c.a2[row];
+  v.r = c.r[row];
+  v.rhs = c.rhs[row];
+  if (m.x.d1) v.wa1 = mulv(m.i1.m, v.a1);
+  if (m.x.d2) v.wa2 = mulv(m.i2.m, v.a2);
+  return v;
+}
+
+__device__ __forceinline__ float row_w(const Row& v, const Manifold& m, const Dv& x1, const Dv& x2) {
+  if (m.x.d1 && m.x.d2) return (dot6(v.d, v.a1, x1) + dot6(neg(v.d), v.a2, x2)) + v.rhs;
+  if (m.x.d1) return dot6(v.d, v.a1, x1) + v.rhs;
+  return dot6(neg(v.d), v.a2, x2) + v.rhs;
+}
+
+__device__ __forceinline__ void row_add(const Row& v, float alpha, const Manifold& m, Dv& x1, Dv& x2) {
+  if (m.x.d1) axpy6(alpha, scale(v.d, m.i1.im), v.wa1, x1);
+  if (m.x.d2) axpy6(alpha, scale(neg(v.d), m.i2.im), v.wa2, x2);
+}
+
+__device__ __forceinline__ void load_dv(const DevBodies& b, const Manifold& m, Dv& v1, Dv& v2) {
+  if (m.x.d1) { v1.l = xyz(b.dv_lin[m.x.b1]); v1.a = xyz(b.dv_ang[m.x.b1]); }
+  if (m.x.d2) { v2.l = xyz(b.dv_lin[m.x.b2]); v2.a = xyz(b.dv_ang[m.x.b2]); }
+}
+
+__device__ __forceinline__ void store_dv(const DevBodies& b, const Manifold& m, const Dv& v1, const Dv& v2) {
+  if (m.x.d1) { b.dv_lin[m.x.b1] = f4(v1.l, 0.0f); b.dv_ang[m.x.b1] = f4(v1.a, 0.0f); }
+  if (m.x.d2) { b.dv_lin[m.x.b2] = f4(v2.l, 0.0f); b.dv_ang[m.x.b2] = f4(v2.a, 0.0f); }
+}
+
+__device__ __forceinline__ void colour_range(const SolverBufs& sb, int k, int& first, int& n) {
+  const int col = sb.visit[k];
+  first = int(sb.col_start[col]);
+  n = int(sb.col_start[col + 1]) - first;
+}
+
+// Warm start (SORProx::warmstart_set, sor_prox.rs:19-45): dv += l W J for every row with l != 0,
+// normal rows first, then friction rows.
+__global__ void __launch_bounds__(64) k_warmstart(DevBodies b, DevContacts c, SolverBufs sb, int kv) {
+  int first, n;
+  colour_range(sb, kv, first, n);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    Manifold m;
+    load_manifold(b, sb, first + t, m);
+    Dv v1{}, v2{};
+    load_dv(b, m, v1, v2);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= m.x.cnt) break;
+      if (m.c[k].lam.x != 0.0f) row_add(row_of(m, k, 0), m.c[k].lam.x, m, v1, v2);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= m.x.cnt) break;
+      if (m.c[k].lam.y != 0.0f) row_add(row_of(m, k, 1), m.c[k].lam.y, m, v1, v2);
+      if (m.c[k].lam.z != 0.0f) row_add(row_of(m, k, 2), m.c[k].lam.z, m, v1, v2);
+    }
+    store_dv(b, m, v1, v2);
   }
-  // normal rows
-  for (int k = 0; k < x.cnt; ++k) {
-    const uint32_t r = x.r0 + k;
-    float4 lam = R[15 * long(C) + r];
-    const RowView v = load_row(Rc, C, r, 0);
-    const float w = row_w(v, x.d1, x.d2, v1, v2);
-    const float a = lam.x - v.r * w;
-    const float ni = a > 0.0f ? a : 0.0f;
-    const float dl = ni - lam.x;
-    lam.x = ni;
-    row_add(v, dl, x.d1, x.d2, v1, v2);
-    R[15 * long(C) + r] = lam;
+}
+
+// One colour of one SOR-Prox sweep (sor_prox.rs:159-343): the manifold's friction rows, bounded
+// by mu times the current normal impulse and zeroed when that is zero, then its normal rows,
+// projected on l >= 0.
+__global__ void __launch_bounds__(64) k_pgs(DevBodies b, DevContacts c, SolverBufs sb, int kv) {
+  int first, n;
+  colour_range(sb, kv, first, n);
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    Manifold m;
+    load_manifold(b, sb, first + t, m);
+    Dv v1{}, v2{};
+    load_dv(b, m, v1, v2);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= m.x.cnt) break;
+      const float mu = m.c[k].mu;
+      const float ln = m.c[k].lam.x;
+#pragma unroll
+      for (int f = 1; f <= 2; ++f) {
+        float li = f == 1 ? m.c[k].lam.y : m.c[k].lam.z;
+        const Row v = row_of(m, k, f);
+        if (ln == 0.0f) {
+          if (li != 0.0f) {
+            row_add(v, -li, m, v1, v2);
+            li = 0.0f;
+          }
+        } else {
+          const float mx = mu * ln;
+          const float mn = -mx;
+          const float w = row_w(v, m, v1, v2);
+          const float val = li - v.r * w;
+          const float ni = val > mn ? (val < mx ? val : mx) : mn;
+          const float dl = ni - li;
+          li = ni;
+          row_add(v, dl, m, v1, v2);
+        }
+        if (f == 1) m.c[k].lam.y = li;
+        else m.c[k].lam.z = li;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= m.x.cnt) break;
+      const Row v = row_of(m, k, 0);
+      const float w = row_w(v, m, v1, v2);
+      const float a = m.c[k].lam.x - v.r * w;
+      const float ni = a > 0.0f ? a : 0.0f;
+      const float dl = ni - m.c[k].lam.x;
+      m.c[k].lam.x = ni;
+      row_add(v, dl, m, v1, v2);
+    }
+    store_dv(b, m, v1, v2);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < m.x.cnt) sb.rec[long(m.x.r0 + k) * kRecFields + 7] = m.c[k].lam;
   }
-  if (x.d1) { b.dv_lin[x.b1] = f4(v1.l, 0.0f); b.dv_ang[x.b1] = f4(v1.a, 0.0f); }
-  if (x.d2) { b.dv_lin[x.b2] = f4(v2.l, 0.0f); b.dv_ang[x.b2] = f4(v2.a, 0.0f); }
+}
+
+// Colours left empty (inherited colours can leave gaps) start where the next one does.
+__global__ void k_fix_colour_starts(int ncolours, int M, uint32_t* col_start) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  col_start[ncolours] = uint32_t(M);
+  for (int k = ncolours - 1; k >= 0; --k)
+    if (col_start[k] == 0xffffffffu) col_start[k] = col_start[k + 1];
 }
 
 // Natural-map residual per row; per block (max |phi|, sum phi^2) in double, reduced on the host
 // in block order.
-__global__ void __launch_bounds__(256) k_residual(DevBodies b, DevContacts c, int M, SolverBufs sb, int C) {
+__global__ void __launch_bounds__(256) k_residual(DevBodies b, DevContacts c, int M, SolverBufs sb) {
   __shared__ double s_max[256];
   __shared__ double s_sum[256];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double mx = 0.0, sum = 0.0;
   if (s < M) {
-    const SlotCtx x = slot_ctx(b, c, sb, s);
+    Manifold m;
+    load_manifold(b, sb, s, m);
     Dv v1{}, v2{};
-    if (x.d1) { v1.l = xyz(b.dv_lin[x.b1]); v1.a = xyz(b.dv_ang[x.b1]); }
-    if (x.d2) { v2.l = xyz(b.dv_lin[x.b2]); v2.a = xyz(b.dv_ang[x.b2]); }
-    const float4* R = sb.rec;
-    for (int k = 0; k < x.cnt; ++k) {
-      const uint32_t r = x.r0 + k;
-      const float4 lam = R[15 * long(C) + r];
-      const float mu = R[0 * long(C) + r].w;
+    load_dv(b, m, v1, v2);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (k >= m.x.cnt) break;
+      const float4 lam = m.c[k].lam;
       {
-        const RowView v = load_row(R, C, r, 0);
-        const float w = row_w(v, x.d1, x.d2, v1, v2);
+        const Row v = row_of(m, k, 0);
+        const float w = row_w(v, m, v1, v2);
         const float a = lam.x - v.r * w;
         const float pr = a > 0.0f ? a : 0.0f;
         const double phi = fabs(double((lam.x - pr) / v.r));
@@ -507,9 +588,9 @@ __global__ void __launch_bounds__(256) k_residual(DevBodies b, DevContacts c, in
 #pragma unroll
       for (int f = 1; f <= 2; ++f) {
         const float li = f == 1 ? lam.y : lam.z;
-        const RowView v = load_row(R, C, r, f);
-        const float w = row_w(v, x.d1, x.d2, v1, v2);
-        const float mxv = mu * lam.x, mnv = -mxv;
+        const Row v = row_of(m, k, f);
+        const float w = row_w(v, m, v1, v2);
+        const float mxv = m.c[k].mu * lam.x, mnv = -mxv;
         const float val = li - v.r * w;
         const float pr = val > mnv ? (val < mxv ? val : mxv) : mnv;
         const double phi = fabs(double((li - pr) / v.r));
@@ -537,7 +618,97 @@ __global__ void __launch_bounds__(256) k_residual(DevBodies b, DevContacts c, in
 __global__ void __launch_bounds__(256) k_cache(DevContacts c, SolverBufs sb, int C) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= C) return;
-  c.imp[sb.rec2c[r]] = sb.rec[15 * long(C) + r];
+  c.imp[sb.rec2c[r]] = sb.rec[long(r) * kRecFields + 7];
+}
+
+// ------------------------------------------------------------------ learned warm start
+// Features of every contact row record (the network's 24 inputs, in the order documented with
+// nphys_mlp_create), computed after assembly from the record, the bodies and the contact:
+// velocities over |g| dt, depth over |g| dt^2, mass terms over the row's inverse effective mass,
+// cached impulses over ls = r_n |g| dt (the impulse that cancels |g| dt along the normal), which
+// is also written per record as the network's output unit.
+__global__ void __launch_bounds__(256) k_features(DevBodies b, DevContacts cur, SimParams p, int M, SolverBufs sb,
+                                                  float* __restrict__ X, float* __restrict__ ls) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= M) return;
+  const SlotCtx x = slot_ctx(sb, s);
+  float im1 = 0.0f, im2 = 0.0f;
+  m33 I1, I2;
+  if (x.d1) I1 = load_inv(b, x.b1, im1);
+  if (x.d2) I2 = load_inv(b, x.b2, im2);
+  const float g = sqrtf(p.gx * p.gx + p.gy * p.gy + p.gz * p.gz);
+  const float gdt = (g > 0.0f ? g : 1.0f) * p.dt;
+  const float e = (b.restitution[x.b1] + b.restitution[x.b2]) * 0.5f;
+  const float mm = b.margin[x.b1] + b.margin[x.b2];
+  for (int k = 0; k < x.cnt; ++k) {
+    const uint32_t r = x.r0 + k;
+    CRec c;
+    load_rec(sb.rec, r, c);
+    const float depth = cur.w1[x.first + k].w + mm;
+    const float lsc = c.r[0] * gdt;
+    float* f = X + long(r) * 24;
+#pragma unroll
+    for (int row = 0; row < 3; ++row) {
+      f[row] = c.rhs[row] / gdt;
+      const float a1 = x.d1 ? dot(c.a1[row], mulv(I1, c.a1[row])) : 0.0f;
+      const float a2 = x.d2 ? dot(c.a2[row], mulv(I2, c.a2[row])) : 0.0f;
+      f[9 + row] = a1 * c.r[row];
+      f[12 + row] = a2 * c.r[row];
+    }
+    f[3] = c.n.x;
+    f[4] = c.n.y;
+    f[5] = c.n.z;
+    f[6] = depth / (gdt * p.dt);
+    f[7] = im1 * c.r[0];
+    f[8] = im2 * c.r[0];
+    f[15] = float(x.cnt) * 0.25f;
+    f[16] = c.lam.x / lsc;
+    f[17] = c.lam.y / lsc;
+    f[18] = c.lam.z / lsc;
+    f[19] = c.mu;
+    f[20] = e;
+    f[21] = gdt;
+    f[22] = (x.d1 && x.d2) ? 0.0f : 1.0f;
+    f[23] = c.lam.w;
+    ls[r] = lsc;
+  }
+}
+
+// The network's prediction as the warm start: impulses times ls, projected on the admissible
+// set (normal >= 0, each tangent within +-mu normal, the pyramid of the friction rows).
+__global__ void __launch_bounds__(256) k_apply_prediction(SolverBufs sb, int C, const float* __restrict__ y,
+                                                          const float* __restrict__ ls) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= C) return;
+  const float s = ls[r];
+  const float mu = sb.rec[long(r) * kRecFields].w;
+  const float ln = fmaxf(y[3 * r] * s, 0.0f);
+  const float lim = mu * ln;
+  const float t1 = fminf(fmaxf(y[3 * r + 1] * s, -lim), lim);
+  const float t2 = fminf(fmaxf(y[3 * r + 2] * s, -lim), lim);
+  float4* L = sb.rec + long(r) * kRecFields + 7;
+  *L = make_float4(ln, t1, t2, L->w);
+}
+
+// Training targets of the last solve: the impulses in units of ls.
+__global__ void __launch_bounds__(256) k_sample_targets(SolverBufs sb, int C, const float* __restrict__ ls,
+                                                        float* __restrict__ t) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= C) return;
+  const float4 l = sb.rec[long(r) * kRecFields + 7];
+  const float s = ls[r];
+  t[3 * r] = l.x / s;
+  t[3 * r + 1] = l.y / s;
+  t[3 * r + 2] = l.z / s;
+}
+
+// Impulses of all records to / from a side buffer (the judge's saved solves).
+__global__ void __launch_bounds__(256) k_lam_copy(SolverBufs sb, int C, float4* __restrict__ buf, int to_buf) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= C) return;
+  float4* L = sb.rec + long(r) * kRecFields + 7;
+  if (to_buf) buf[r] = *L;
+  else *L = buf[r];
 }
 
 inline unsigned nb(long n) { return unsigned((n + 255) / 256); }
@@ -569,7 +740,8 @@ void launch_colour_keys(int M, SolverBufs& sb, uint32_t* ncol, hipStream_t s) {
   hipLaunchKernelGGL(k_colour_keys, dim3(nb(M)), dim3(256), 0, s, M, sb.state, sb.ckeys, sb.order, ncol);
 }
 
-void launch_colour_order(const DevContacts& c, int M, int ncolours, SolverBufs& sb, hipStream_t s) {
+void launch_colour_order(const DevBodies& b_view_for_slots, const DevContacts& c, int M, int ncolours,
+                         SolverBufs& sb, hipStream_t s) {
   if (M == 0) return;
   const int bits = ncolours <= 256 ? 8 : 16;
   radix_sort_pairs(sb.ckeys, sb.order, sb.ckeys_alt, sb.order_alt, M, bits, sb.tmp, s);
@@ -577,36 +749,71 @@ void launch_colour_order(const DevContacts& c, int M, int ncolours, SolverBufs& 
   hipLaunchKernelGGL(k_colour_starts, dim3(nb(M)), dim3(256), 0, s, c, M, sb.ckeys, sb.order, sb.col_start,
                      sb.mcount_ord);
   scan_exclusive_u32(sb.mcount_ord, sb.soff, M, sb.tmp, s);
+  hipLaunchKernelGGL(k_slot_info, dim3(nb(M)), dim3(256), 0, s, b_view_for_slots, c, M, sb.order, sb.soff, sb.sinfo);
 }
 
 void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContacts& prev, int prevM,
-                     int M, const SimParams& p, SolverBufs& sb, int C, hipStream_t s) {
+                     int M, const SimParams& p, SolverBufs& sb, hipStream_t s) {
   if (M == 0) return;
-  hipLaunchKernelGGL(k_assemble, dim3(nb(M)), dim3(256), 0, s, b, cur, prev, prevM, M, p, sb, C);
+  hipLaunchKernelGGL(k_assemble, dim3(nb(M)), dim3(256), 0, s, b, cur, prev, prevM, M, p, sb);
 }
 
-void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int first_slot, int nslots,
-                             SolverBufs& sb, int C, hipStream_t s) {
-  if (nslots <= 0) return;
-  hipLaunchKernelGGL(k_warmstart, dim3(nb(nslots)), dim3(256), 0, s, b, c, first_slot, nslots, sb, C);
+void launch_fix_colour_starts(int ncolours, int M, SolverBufs& sb, hipStream_t s) {
+  hipLaunchKernelGGL(k_fix_colour_starts, dim3(1), dim3(1), 0, s, ncolours, M, sb.col_start);
 }
 
-void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int first_slot, int nslots,
-                       SolverBufs& sb, int C, hipStream_t s) {
-  if (nslots <= 0) return;
-  hipLaunchKernelGGL(k_pgs, dim3(nb(nslots)), dim3(256), 0, s, b, c, first_slot, nslots, sb, C);
+// One lane per manifold: enough wavefronts for the largest colour a scene of M manifolds can
+// have, capped at 4096 (16 per CU); larger colours are walked by the grid-stride loop.
+int sweep_blocks(int M) {
+  const int need = (M + 63) / 64;
+  return need < 1 ? 1 : (need > 4096 ? 4096 : need);
+}
+
+void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_warmstart, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
+}
+
+void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
+                       hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+  if (ev0) {
+    // the dispatch packet itself records both events, timing the kernel alone
+    hipExtLaunchKernelGGL(k_pgs, dim3(blocks), dim3(64), 0, s, ev0, ev1, 0, b, c, sb, k);
+  } else {
+    hipLaunchKernelGGL(k_pgs, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
+  }
 }
 
 int residual_blocks(int M) { return int(nb(M > 0 ? M : 1)); }
 
-void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, int C,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_residual, dim3(nb(M > 0 ? M : 1)), dim3(256), 0, s, b, c, M, sb, C);
+void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, hipStream_t s) {
+  hipLaunchKernelGGL(k_residual, dim3(nb(M > 0 ? M : 1)), dim3(256), 0, s, b, c, M, sb);
 }
 
 void launch_cache_impulses(const DevContacts& c, SolverBufs& sb, int C, hipStream_t s) {
   if (C == 0) return;
   hipLaunchKernelGGL(k_cache, dim3(nb(C)), dim3(256), 0, s, c, sb, C);
+}
+
+void launch_features(const DevBodies& b, const DevContacts& c, const SimParams& p, int M, SolverBufs& sb, float* X,
+                     float* ls, hipStream_t s) {
+  if (M == 0) return;
+  hipLaunchKernelGGL(k_features, dim3(nb(M)), dim3(256), 0, s, b, c, p, M, sb, X, ls);
+}
+
+void launch_apply_prediction(SolverBufs& sb, int C, const float* y, const float* ls, hipStream_t s) {
+  if (C == 0) return;
+  hipLaunchKernelGGL(k_apply_prediction, dim3(nb(C)), dim3(256), 0, s, sb, C, y, ls);
+}
+
+void launch_sample_targets(SolverBufs& sb, int C, const float* ls, float* t, hipStream_t s) {
+  if (C == 0) return;
+  hipLaunchKernelGGL(k_sample_targets, dim3(nb(C)), dim3(256), 0, s, sb, C, ls, t);
+}
+
+void launch_lam_copy(SolverBufs& sb, int C, float4* buf, bool to_buf, hipStream_t s) {
+  if (C == 0) return;
+  hipLaunchKernelGGL(k_lam_copy, dim3(nb(C)), dim3(256), 0, s, sb, C, buf, to_buf ? 1 : 0);
 }
 
 }  // namespace nph

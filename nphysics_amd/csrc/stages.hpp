@@ -20,7 +20,7 @@ struct BroadPhaseBufs {
   uint32_t* vals_alt;
   int* child;             // [2(n-1)] children of internal nodes (>= n-1: leaf)
   int* parent;            // [2n-1]
-  uint32_t* visit;        // [n-1] refit arrival counters
+  uint32_t* visit;        // [8(n-1)] internal-node bounds, order-preserving integer encoding
   float4* node_mn;        // [2n-1]
   float4* node_mx;
   int* cand;              // [n * cap] candidate partners per body
@@ -74,33 +74,49 @@ struct SolverBufs {
   uint32_t* col_start;    // [max colours + 1]
   uint32_t* mcount_ord;   // [M] contact counts in colour order
   uint32_t* soff;         // [M] first row record of each slot
+  int4* sinfo;            // [M] per slot: body1|dyn1<<30, body2|dyn2<<30, first record, count|first<<3
   uint32_t* rec2c;        // [C] contact index of each row record
-  float4* rec;            // [16 C] row records
+  float4* rec;            // [8 rec_stride] row records, field f of record r at 8 r + f
+  long rec_stride;        // record capacity (fixed between reallocations)
+  int* visit;             // [max colours] colour classes in visiting order
   double* red;            // residual partials
   uint32_t* tmp;
 };
 constexpr int kMaxColours = 4096;
-constexpr int kRecFields = 16;
+constexpr int kRecFields = 8;  // float4 per contact row record
 
 void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, const DevContacts& prev,
                            int prevM, const uint32_t* prev_state, SolverBufs& sb, hipStream_t s);
 void launch_colour_round(const DevBodies& b, const DevContacts& c, int M, int round, SolverBufs& sb,
                          hipStream_t s);
 void launch_colour_keys(int M, SolverBufs& sb, uint32_t* ncol, hipStream_t s);
-void launch_colour_order(const DevContacts& c, int M, int ncolours, SolverBufs& sb, hipStream_t s);
+void launch_colour_order(const DevBodies& b, const DevContacts& c, int M, int ncolours, SolverBufs& sb,
+                         hipStream_t s);
+void launch_fix_colour_starts(int ncolours, int M, SolverBufs& sb, hipStream_t s);
 int residual_blocks(int M);
 void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContacts& prev,
-                     int prevM, int M, const SimParams& p, SolverBufs& sb, int C, hipStream_t s);
-void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int first_slot,
-                             int nslots, SolverBufs& sb, int C, hipStream_t s);
-void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int first_slot, int nslots,
-                       SolverBufs& sb, int C, hipStream_t s);
-void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, int C,
-                     hipStream_t s);
+                     int prevM, int M, const SimParams& p, SolverBufs& sb, hipStream_t s);
+// Sweep kernels of the colour visited k-th; their slot ranges are read on the device, so the
+// launches are independent of per-step sizes (grid = blocks, grid-stride over the colour).
+int sweep_blocks(int M);
+int position_blocks(int M);
+void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, int blocks,
+                             SolverBufs& sb, hipStream_t s);
+void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
+                       hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, hipStream_t s);
 void launch_cache_impulses(const DevContacts& c, SolverBufs& sb, int C, hipStream_t s);
 
+// learned warm start (k_solver.hip): features [C][24] and impulse units ls [C] per row record,
+// the prediction y [C][3] applied as the warm start, post-solve targets t [C][3].
+void launch_features(const DevBodies& b, const DevContacts& c, const SimParams& p, int M, SolverBufs& sb, float* X,
+                     float* ls, hipStream_t s);
+void launch_apply_prediction(SolverBufs& sb, int C, const float* y, const float* ls, hipStream_t s);
+void launch_sample_targets(SolverBufs& sb, int C, const float* ls, float* t, hipStream_t s);
+void launch_lam_copy(SolverBufs& sb, int C, float4* buf, bool to_buf, hipStream_t s);
+
 // k_position.hip ---------------------------------------------------------------------------
-void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p,
-                            int first_slot, int nslots, SolverBufs& sb, hipStream_t s);
+void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k,
+                            int blocks, SolverBufs& sb, hipStream_t s);
 
 }  // namespace nph

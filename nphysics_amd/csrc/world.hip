@@ -18,64 +18,22 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <utility>
 #include <vector>
 
 #include "../../include/nphys.h"
+#include "common.hpp"
 #include "devmath.hpp"
 #include "kernels.hpp"
+#include "mlp.hpp"
 #include "stages.hpp"
 
 namespace nph {
 
 namespace {
-
-thread_local std::string g_err;
-
-int set_err(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-#define HIPCK(expr)                                                                   \
-  do {                                                                                \
-    hipError_t e_ = (expr);                                                           \
-    if (e_ != hipSuccess)                                                             \
-      return set_err(NPHYS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-// Growable device buffer (contents are not preserved on growth unless asked).
-struct DBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-  ~DBuf() {
-    if (p) (void)hipFree(p);
-  }
-  hipError_t ensure(size_t need, bool keep = false, hipStream_t s = nullptr) {
-    if (need <= bytes) return hipSuccess;
-    size_t nb = std::max(need, bytes + bytes / 2);
-    nb = (nb + 255) & ~size_t(255);
-    void* q = nullptr;
-    hipError_t e = hipMalloc(&q, nb);
-    if (e != hipSuccess) return e;
-    if (keep && p && bytes) {
-      e = hipMemcpyAsync(q, p, bytes, hipMemcpyDeviceToDevice, s);
-      if (e != hipSuccess) return e;
-      e = hipStreamSynchronize(s);
-      if (e != hipSuccess) return e;
-    }
-    if (p) (void)hipFree(p);
-    p = q;
-    bytes = nb;
-    return hipSuccess;
-  }
-  template <class T>
-  T* as() const {
-    return static_cast<T*>(p);
-  }
-};
 
 struct ContactBufs {
   DBuf mbody, mfirst, mcount, w1, w2, n, feature, kind, lp1, lp2, ln, imp;
@@ -149,14 +107,34 @@ struct World {
   // solver
   DBuf sv_state2[2];  // colouring state of the manifolds of cb[k]
   DBuf sv_deg, sv_adj_off, sv_adj_fill, sv_adj, sv_ckeys, sv_order, sv_ckeys_alt,
-      sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_rec2c, sv_rec, sv_red, sv_tmp;
-  std::vector<uint32_t> col_start;
+      sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_sinfo, sv_rec2c, sv_rec, sv_red, sv_tmp;
   int ncolours = 0;
+  long rec_stride = 0;
   int colour_rounds = 0;
   std::vector<int> colour_visit;  // colour classes in visiting order
   long solve_count = 0;    // solves performed, never reset
+  DBuf sv_visit;           // device copy of colour_visit
+  int* h_visit = nullptr;  // pinned staging of colour_visit
+  // Instantiated graphs of the sweep phases, keyed by a hash of every kernel argument: the
+  // launch sequence only depends on the colour count, the iteration counts and the buffers.
+  struct GraphEntry {
+    uint64_t key;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  bool use_graphs = true;
   int solved_M = 0, solved_C = 0;
   double res_inf = 0.0, res_rms = 0.0;
+  // solve-to-residual mode (off when res_target <= 0): see nphys_set_residual_target
+  double res_target = 0.0;
+  int res_max_sweeps = 500, res_check_every = 1;
+  int last_sweeps = 0;
+  std::vector<double> red_host;
+  // learned warm start: network (FP32 parameters + BF16 images), per-record features, impulse
+  // units, predictions and recorded targets; the judge's saved plain solve
+  bool ml_loaded = false, capture = false, judge = false;
+  long samples = 0;
+  DBuf ml_params, ml_img, ml_X, ml_ls, ml_y, ml_t, jd_lam, jd_dv;
   std::vector<int> last_colours;
   bool keep_colours = false;
 
@@ -164,10 +142,17 @@ struct World {
   uint32_t* h_small = nullptr;
   hipEvent_t ev[EV_COUNT] = {};
   nphys_counters counters{};
+  bool counters_enabled = false;
+  std::vector<hipEvent_t> kt_ev;  // event pairs around sweep-kernel launches
+  int kt_used = 0;
 
   ~World() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    if (h_visit) (void)hipHostFree(h_visit);
     if (h_small) (void)hipHostFree(h_small);
     for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : kt_ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -193,9 +178,14 @@ struct World {
     HIPCK(hipGetDevice(&device));
     HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_small), 64 * sizeof(uint32_t)));
+    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_visit), (kMaxColours + 2) * sizeof(int)));
+    HIPCK(sv_visit.ensure((kMaxColours + 2) * sizeof(int)));
     for (auto& e : ev) HIPCK(hipEventCreate(&e));
     set_params(p);
     HIPCK(bp_small.ensure(64 * sizeof(uint32_t)));
+    // Profilers that cannot follow graph launches can turn them off; kernels are the same.
+    const char* ng = std::getenv("NPHYS_DISABLE_GRAPHS");
+    use_graphs = !(ng && ng[0] == '1');
     return NPHYS_OK;
   }
 
@@ -382,8 +372,9 @@ struct World {
   // ------------------------------------------------------------------ broad phase
   int ensure_bp() {
     const size_t nn = size_t(std::max(n, 2));
-    for (DBuf* b : {&keys, &vals, &keys_alt, &vals_alt, &cand_count, &cand_off, &visit})
+    for (DBuf* b : {&keys, &vals, &keys_alt, &vals_alt, &cand_count, &cand_off})
       HIPCK(b->ensure(nn * 4));
+    HIPCK(visit.ensure(8 * nn * 4));
     HIPCK(child.ensure(2 * nn * 4));
     HIPCK(parent.ensure(2 * nn * 4));
     HIPCK(node_mn.ensure(2 * nn * sizeof(float4)));
@@ -518,8 +509,10 @@ struct World {
     HIPCK(sv_state2[cur].ensure(m * 4));
     HIPCK(sv_state2[cur ^ 1].ensure(4));
     HIPCK(sv_col_start.ensure((kMaxColours + 2) * 4));
+    HIPCK(sv_sinfo.ensure(m * sizeof(int4)));
     HIPCK(sv_rec2c.ensure(c * 4));
     HIPCK(sv_rec.ensure(c * kRecFields * sizeof(float4)));
+    rec_stride = long(sv_rec.bytes / (kRecFields * sizeof(float4)));
     HIPCK(sv_red.ensure(size_t(residual_blocks(M)) * 2 * sizeof(double)));
     const size_t tw = std::max({radix_temp_words(int(m)), scan_temp_words(int(m)), scan_temp_words(int(nn))});
     HIPCK(sv_tmp.ensure(tw * 4 + 64));
@@ -541,11 +534,62 @@ struct World {
     v.col_start = sv_col_start.as<uint32_t>();
     v.mcount_ord = sv_mcount_ord.as<uint32_t>();
     v.soff = sv_soff.as<uint32_t>();
+    v.sinfo = sv_sinfo.as<int4>();
+    v.rec_stride = rec_stride;
+    v.visit = sv_visit.as<int>();
     v.rec2c = sv_rec2c.as<uint32_t>();
     v.rec = sv_rec.as<float4>();
     v.red = sv_red.as<double>();
     v.tmp = sv_tmp.as<uint32_t>();
     return v;
+  }
+
+  static void hash_bytes(uint64_t& h, const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) {
+      h ^= c[i];
+      h *= 0x100000001b3ull;
+    }
+  }
+
+  // Runs one sweep phase (its launches enqueued by `body`) through a cached hipGraph; the graph
+  // is captured the first time its kernel arguments occur.
+  template <class F>
+  int run_phase(int phase, const DevBodies& b, const DevContacts& c, const SolverBufs& v, int blocks, int reps,
+                F&& body) {
+    if (!use_graphs) {
+      body();
+      return NPHYS_OK;
+    }
+    uint64_t key = 0xcbf29ce484222325ull;
+    hash_bytes(key, &phase, sizeof(phase));
+    hash_bytes(key, &b, sizeof(b));
+    hash_bytes(key, &c, sizeof(c));
+    hash_bytes(key, &v, sizeof(v));
+    hash_bytes(key, &sp, sizeof(sp));
+    hash_bytes(key, &blocks, sizeof(blocks));
+    hash_bytes(key, &ncolours, sizeof(ncolours));
+    hash_bytes(key, &reps, sizeof(reps));
+    for (const GraphEntry& g : graphs)
+      if (g.key == key) {
+        HIPCK(hipGraphLaunch(g.exec, stream));
+        return NPHYS_OK;
+      }
+    hipGraph_t graph = nullptr;
+    HIPCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    body();
+    HIPCK(hipStreamEndCapture(stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCK(e);
+    if (graphs.size() >= 8) {
+      (void)hipGraphExecDestroy(graphs.front().exec);
+      graphs.erase(graphs.begin());
+    }
+    graphs.push_back(GraphEntry{key, exec});
+    HIPCK(hipGraphLaunch(exec, stream));
+    return NPHYS_OK;
   }
 
   // Visit order of the colour classes for one solve: Fisher-Yates driven by SplitMix64 seeded
@@ -587,15 +631,91 @@ struct World {
     HIPCK(hipStreamSynchronize(stream));
     ncolours = int(h_small[0]);
     if (ncolours > kMaxColours) return set_err(NPHYS_ERR_STATE, "too many colours");
-    launch_colour_order(c, M, ncolours, v, stream);
-    col_start.assign(size_t(ncolours) + 1, 0);
-    HIPCK(hipMemcpyAsync(col_start.data(), v.col_start, (size_t(ncolours) + 1) * 4, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    // colours left empty (inherited colours can leave gaps) start where the next one does
-    col_start[ncolours] = uint32_t(M);
-    for (int k = ncolours - 1; k >= 0; --k)
-      if (col_start[k] == 0xffffffffu) col_start[k] = col_start[k + 1];
+    launch_colour_order(b, c, M, ncolours, v, stream);
+    launch_fix_colour_starts(ncolours, M, v, stream);
     colour_visit_order(uint64_t(solve_count), ncolours, colour_visit);
+    for (int k = 0; k < ncolours; ++k) h_visit[k] = colour_visit[k];
+    HIPCK(hipMemcpyAsync(sv_visit.p, h_visit, size_t(ncolours) * sizeof(int), hipMemcpyHostToDevice, stream));
+    // h_visit is rewritten only after the next colouring, which follows a stream synchronisation
+    return NPHYS_OK;
+  }
+
+  // Warm start (when `warm`) and `nsweeps` coloured SOR-Prox sweeps. With counters enabled the
+  // sweep kernels are launched directly with HIP events in their dispatch packets; otherwise
+  // the phase runs as a cached hipGraph.
+  int sweeps(const DevBodies& b, const DevContacts& cv, SolverBufs& v, int sblocks, bool warm, int nsweeps) {
+    if (counters_enabled) {
+      const size_t need = size_t(kt_used) + size_t(2) * size_t(nsweeps) * size_t(ncolours);
+      while (kt_ev.size() < need) {
+        hipEvent_t e;
+        HIPCK(hipEventCreate(&e));
+        kt_ev.push_back(e);
+      }
+      if (warm)
+        for (int k = 0; k < ncolours; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
+      for (int it = 0; it < nsweeps; ++it)
+        for (int k = 0; k < ncolours; ++k) {
+          launch_pgs_colour(b, cv, k, sblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
+          kt_used += 2;
+        }
+      return NPHYS_OK;
+    }
+    auto body = [&]() {
+      if (warm)
+        for (int k = 0; k < ncolours; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
+      for (int it = 0; it < nsweeps; ++it)
+        for (int k = 0; k < ncolours; ++k) launch_pgs_colour(b, cv, k, sblocks, v, stream);
+    };
+    return run_phase(warm ? 1 : 3, b, cv, v, sblocks, nsweeps, body);
+  }
+
+  // Natural-map residual of the current impulses: per-block partials reduced on the host in
+  // block order (max is exact; the sum of squares is the RMS).
+  int residual(const DevBodies& b, const DevContacts& cv, int M, int C, SolverBufs& v) {
+    launch_residual(b, cv, M, v, stream);
+    const int nbk = residual_blocks(M);
+    red_host.resize(size_t(nbk) * 2);
+    HIPCK(hipMemcpyAsync(red_host.data(), v.red, red_host.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    double mx = 0.0, ss = 0.0;
+    for (int k = 0; k < nbk; ++k) {
+      mx = std::max(mx, red_host[2 * k]);
+      ss += red_host[2 * k + 1];
+    }
+    res_inf = mx;
+    res_rms = C > 0 ? std::sqrt(ss / double(3 * C)) : 0.0;
+    return NPHYS_OK;
+  }
+
+  // Warm start and SOR-Prox sweeps: the fixed count of the reference (sor_prox.rs:56-79), or, in
+  // solve-to-residual mode, sweeps until max |phi| <= res_target, checked after the warm start
+  // and every res_check_every sweeps, at most res_max_sweeps. Leaves the final residual in
+  // res_inf / res_rms and the sweep count in nsw.
+  int solve_sweeps(const DevBodies& b, const DevContacts& cv, SolverBufs& v, int M, int C, int sblocks, int& nsw) {
+    int rc;
+    if (res_target <= 0.0) {
+      if ((rc = sweeps(b, cv, v, sblocks, true, prm.max_velocity_iterations))) return rc;
+      nsw = prm.max_velocity_iterations;
+      return residual(b, cv, M, C, v);
+    }
+    if ((rc = sweeps(b, cv, v, sblocks, true, 0))) return rc;
+    int done = 0;
+    for (;;) {
+      if ((rc = residual(b, cv, M, C, v))) return rc;
+      if (res_inf <= res_target || done >= res_max_sweeps) break;
+      const int k = std::min(res_check_every, res_max_sweeps - done);
+      if ((rc = sweeps(b, cv, v, sblocks, false, k))) return rc;
+      done += k;
+    }
+    nsw = done;
+    return NPHYS_OK;
+  }
+
+  // The network's impulses for every row record (features already computed) as the warm start.
+  int predict(SolverBufs& v, int C) {
+    HIPCK(ml_y.ensure(size_t(C) * kOut * sizeof(float)));
+    mlp_infer(MlpDev{ml_img.as<uint16_t>(), ml_params.as<float>()}, ml_X.as<float>(), C, ml_y.as<float>(), stream);
+    launch_apply_prediction(v, C, ml_y.as<float>(), ml_ls.as<float>(), stream);
     return NPHYS_OK;
   }
 
@@ -611,6 +731,8 @@ struct World {
     ncolours = 0;
     DevBodies b = bodies();
     if (M == 0) {
+      last_sweeps = res_target > 0.0 ? 0 : prm.max_velocity_iterations;
+      counters.velocity_sweeps += last_sweeps;
       HIPCK(hipEventRecord(ev[EV_ASM], stream));
       HIPCK(hipEventRecord(ev[EV_VEL], stream));
       return NPHYS_OK;
@@ -620,27 +742,57 @@ struct World {
     SolverBufs v = sv_view();
     DevContacts cv = cc.view();
     if ((rc = colour(b, cv, M, v))) return rc;
-    launch_assemble(b, cv, pc.view(), prevM, M, sp, v, C, stream);
+    launch_assemble(b, cv, pc.view(), prevM, M, sp, v, stream);
     HIPCK(hipEventRecord(ev[EV_ASM], stream));
-    for (int k : colour_visit)
-      launch_warmstart_colour(b, cv, int(col_start[k]), int(col_start[k + 1] - col_start[k]), v, C, stream);
-    for (int it = 0; it < prm.max_velocity_iterations; ++it)
-      for (int k : colour_visit)
-        launch_pgs_colour(b, cv, int(col_start[k]), int(col_start[k + 1] - col_start[k]), v, C, stream);
-    launch_residual(b, cv, M, v, C, stream);
+    const int sblocks = sweep_blocks(M);
+    kt_used = 0;
+    const bool learned = contact_model == NPHYS_CONTACT_MODEL_LEARNED_WARMSTART;
+    if (learned || capture) {
+      HIPCK(ml_X.ensure(size_t(C) * kFeat * sizeof(float)));
+      HIPCK(ml_ls.ensure(size_t(C) * sizeof(float)));
+      launch_features(b, cv, sp, M, v, ml_X.as<float>(), ml_ls.as<float>(), stream);
+    }
+    if (learned && judge && res_target > 0.0) {
+      // The plain cache warm start is the judge: both solves run from the same assembled system
+      // and the prediction is kept only when it reaches the residual target in fewer sweeps.
+      HIPCK(jd_lam.ensure(size_t(C) * sizeof(float4)));
+      HIPCK(jd_dv.ensure(size_t(n) * 2 * sizeof(float4)));
+      int n_plain = 0, n_learned = 0;
+      if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, n_plain))) return rc;
+      const double rinf_p = res_inf, rrms_p = res_rms;
+      launch_lam_copy(v, C, jd_lam.as<float4>(), true, stream);
+      HIPCK(hipMemcpyAsync(jd_dv.p, dv_lin.p, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+      HIPCK(hipMemcpyAsync(jd_dv.as<float4>() + n, dv_ang.p, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice,
+                           stream));
+      HIPCK(hipMemsetAsync(dv_lin.p, 0, size_t(n) * sizeof(float4), stream));
+      HIPCK(hipMemsetAsync(dv_ang.p, 0, size_t(n) * sizeof(float4), stream));
+      if ((rc = predict(v, C))) return rc;
+      if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, n_learned))) return rc;
+      const bool keep = n_learned < n_plain;
+      if (!keep) {
+        launch_lam_copy(v, C, jd_lam.as<float4>(), false, stream);
+        HIPCK(hipMemcpyAsync(dv_lin.p, jd_dv.p, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice, stream));
+        HIPCK(hipMemcpyAsync(dv_ang.p, jd_dv.as<float4>() + n, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice,
+                             stream));
+        res_inf = rinf_p;
+        res_rms = rrms_p;
+      }
+      counters.judge_plain_sweeps += n_plain;
+      counters.judge_learned_sweeps += n_learned;
+      counters.judge_kept += keep ? 1 : 0;
+      last_sweeps = n_plain + n_learned;
+    } else {
+      if (learned && (rc = predict(v, C))) return rc;
+      if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, last_sweeps))) return rc;
+    }
+    if (capture) {
+      HIPCK(ml_t.ensure(size_t(C) * kOut * sizeof(float)));
+      launch_sample_targets(v, C, ml_ls.as<float>(), ml_t.as<float>(), stream);
+      samples = C;
+    }
+    counters.velocity_sweeps += last_sweeps;
     launch_cache_impulses(cv, v, C, stream);
     HIPCK(hipEventRecord(ev[EV_VEL], stream));
-    const int nbk = residual_blocks(M);
-    std::vector<double> red(size_t(nbk) * 2);
-    HIPCK(hipMemcpyAsync(red.data(), v.red, red.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    double mx = 0.0, ss = 0.0;
-    for (int k = 0; k < nbk; ++k) {
-      mx = std::max(mx, red[2 * k]);
-      ss += red[2 * k + 1];
-    }
-    res_inf = mx;
-    res_rms = C > 0 ? std::sqrt(ss / double(3 * C)) : 0.0;
     if (keep_colours) {
       std::vector<uint32_t> st(M);
       HIPCK(hipMemcpyAsync(st.data(), sv_state2[cur].p, size_t(M) * 4, hipMemcpyDeviceToHost, stream));
@@ -657,10 +809,16 @@ struct World {
     DevBodies b = bodies();
     SolverBufs v = sv_view();
     DevContacts cv = cc.view();
-    for (int it = 0; it < prm.max_position_iterations; ++it)
-      for (int k : colour_visit)
-        launch_position_colour(b, cv, sp, int(col_start[k]), int(col_start[k + 1] - col_start[k]), v, stream);
-    return NPHYS_OK;
+    const int pblocks = position_blocks(cc.M);
+    auto body = [&]() {
+      for (int it = 0; it < prm.max_position_iterations; ++it)
+        for (int k = 0; k < ncolours; ++k) launch_position_colour(b, cv, sp, k, pblocks, v, stream);
+    };
+    if (counters_enabled) {
+      body();
+      return NPHYS_OK;
+    }
+    return run_phase(2, b, cv, v, pblocks, prm.max_position_iterations, body);
   }
 
   // ------------------------------------------------------------------ step
@@ -711,6 +869,13 @@ struct World {
     counters.nconstraints = int64_t(3) * solved_C;
     counters.ncolours = ncolours;
     counters.colour_rounds += colour_rounds;
+    for (int k = 0; k < kt_used; k += 2) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, kt_ev[k], kt_ev[k + 1]);
+      counters.sweep_kernel_time += double(ms) * 1e-3;
+      counters.sweep_kernel_launches += 1;
+    }
+    kt_used = 0;
     prm.t += prm.dt;
     return NPHYS_OK;
   }
@@ -838,7 +1003,54 @@ int nphys_contact_model_set(nphys_world* w, int32_t model) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
   if (model != NPHYS_CONTACT_MODEL_SIGNORINI_COULOMB_PYRAMID && model != NPHYS_CONTACT_MODEL_LEARNED_WARMSTART)
     return set_err(NPHYS_ERR_INVALID, "unknown contact model");
+  if (model == NPHYS_CONTACT_MODEL_LEARNED_WARMSTART && !w->w.ml_loaded)
+    return set_err(NPHYS_ERR_STATE, "the learned warm start needs a network (nphys_warmstart_load)");
   w->w.contact_model = model;
+  return NPHYS_OK;
+}
+
+int nphys_warmstart_load(nphys_world* w, const float* params, int64_t n) {
+  if (!w || !params) return set_err(NPHYS_ERR_INVALID, "null argument");
+  if (n != nph::kNumParams) return set_err(NPHYS_ERR_INVALID, "parameter count mismatch");
+  for (int i = 0; i < nph::kFeat; ++i)
+    if (!(params[nph::kOffXScale + i] > 0.0f)) return set_err(NPHYS_ERR_INVALID, "feature scales must be > 0");
+  for (int i = 0; i < nph::kOut; ++i)
+    if (!(params[nph::kOffYScale + i] > 0.0f)) return set_err(NPHYS_ERR_INVALID, "output scales must be > 0");
+  World& W = w->w;
+  HIPCK(W.ml_params.ensure(size_t(nph::kNumParams) * 4));
+  HIPCK(W.ml_img.ensure(size_t(nph::kImgElems) * 2));
+  HIPCK(hipMemcpyAsync(W.ml_params.p, params, size_t(nph::kNumParams) * 4, hipMemcpyHostToDevice, W.stream));
+  nph::mlp_pack_images(W.ml_params.as<float>(), W.ml_img.as<uint16_t>(), W.stream);
+  HIPCK(hipStreamSynchronize(W.stream));
+  W.ml_loaded = true;
+  return NPHYS_OK;
+}
+
+int nphys_capture_samples(nphys_world* w, int32_t on) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  w->w.capture = on != 0;
+  w->w.samples = 0;
+  return NPHYS_OK;
+}
+
+int nphys_read_samples(nphys_world* w, float* features, float* targets, int64_t cap, int64_t* n) {
+  if (!w || !n) return set_err(NPHYS_ERR_INVALID, "null argument");
+  World& W = w->w;
+  *n = W.samples;
+  if (!features && !targets) return NPHYS_OK;
+  if (cap < W.samples) return set_err(NPHYS_ERR_CAPACITY, "sample buffers too small");
+  if (W.samples == 0) return NPHYS_OK;
+  if (features)
+    HIPCK(hipMemcpyAsync(features, W.ml_X.p, size_t(W.samples) * nph::kFeat * 4, hipMemcpyDeviceToHost, W.stream));
+  if (targets)
+    HIPCK(hipMemcpyAsync(targets, W.ml_t.p, size_t(W.samples) * nph::kOut * 4, hipMemcpyDeviceToHost, W.stream));
+  HIPCK(hipStreamSynchronize(W.stream));
+  return NPHYS_OK;
+}
+
+int nphys_set_warmstart_judge(nphys_world* w, int32_t on) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  w->w.judge = on != 0;
   return NPHYS_OK;
 }
 
@@ -951,6 +1163,22 @@ int nphys_read_counters(nphys_world* w, nphys_counters* out) {
 int nphys_reset_counters(nphys_world* w) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
   std::memset(&w->w.counters, 0, sizeof(w->w.counters));
+  return NPHYS_OK;
+}
+
+int nphys_counters_enable(nphys_world* w, int32_t enabled) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  w->w.counters_enabled = enabled != 0;
+  return NPHYS_OK;
+}
+
+int nphys_set_residual_target(nphys_world* w, double r_inf_target, int32_t max_sweeps, int32_t check_every) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  if (!(r_inf_target >= 0.0) || max_sweeps < 0 || check_every < 1)
+    return set_err(NPHYS_ERR_INVALID, "residual target must be >= 0, max_sweeps >= 0, check_every >= 1");
+  w->w.res_target = r_inf_target;
+  w->w.res_max_sweeps = max_sweeps;
+  w->w.res_check_every = check_every;
   return NPHYS_OK;
 }
 

@@ -25,7 +25,11 @@ EXPORTED = [
     "nphys_get_state", "nphys_step", "nphys_collide", "nphys_contact_model_set",
     "nphys_num_pairs", "nphys_read_pairs", "nphys_num_contacts", "nphys_read_contacts",
     "nphys_num_solved_contacts", "nphys_read_impulses", "nphys_residual", "nphys_read_counters",
-    "nphys_reset_counters", "nphys_read_colours", "nphys_last_error",
+    "nphys_reset_counters", "nphys_counters_enable", "nphys_set_residual_target", "nphys_read_colours",
+    "nphys_warmstart_load", "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
+    "nphys_mlp_create", "nphys_mlp_destroy", "nphys_mlp_num_params", "nphys_mlp_get_params",
+    "nphys_mlp_set_params", "nphys_mlp_set_data", "nphys_mlp_train", "nphys_mlp_forward",
+    "nphys_mlp_gradients", "nphys_last_error",
 ]
 
 
@@ -67,6 +71,12 @@ def load_library(path=LIB_PATH):
     L.nphys_residual.argtypes = [W, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.nphys_read_counters.argtypes = [W, C.POINTER(abi.Counters)]
     L.nphys_reset_counters.argtypes = [W]
+    L.nphys_counters_enable.argtypes = [W, C.c_int32]
+    L.nphys_set_residual_target.argtypes = [W, C.c_double, C.c_int32, C.c_int32]
+    L.nphys_warmstart_load.argtypes = [W, fp, C.c_int64]
+    L.nphys_capture_samples.argtypes = [W, C.c_int32]
+    L.nphys_read_samples.argtypes = [W, fp, fp, C.c_int64, C.POINTER(C.c_int64)]
+    L.nphys_set_warmstart_judge.argtypes = [W, C.c_int32]
     L.nphys_read_colours.argtypes = [W, ip, C.c_int64, ip]
     _lib = L
     return L
@@ -203,6 +213,35 @@ class MechanicalWorld:
 
     def reset_counters(self):
         _check(self._L.nphys_reset_counters(self._h))
+
+    def enable_counters(self, on=True):
+        _check(self._L.nphys_counters_enable(self._h, 1 if on else 0))
+
+    def set_residual_target(self, r_inf=1e-4, max_sweeps=500, check_every=1):
+        """Solve each step until max |phi| <= r_inf (0 restores the fixed iteration count)."""
+        _check(self._L.nphys_set_residual_target(self._h, r_inf, max_sweeps, check_every))
+
+    # -- learned warm start
+    def load_warmstart(self, params):
+        """Loads a warm-start network (nphysics_amd.learned.WarmStartNet.params) into the world."""
+        p = np.ascontiguousarray(params, np.float32)
+        _check(self._L.nphys_warmstart_load(self._h, abi.fptr(p), p.shape[0]))
+
+    def capture_samples(self, on=True):
+        _check(self._L.nphys_capture_samples(self._h, 1 if on else 0))
+
+    def samples(self):
+        """(features [n][24], targets [n][3]) recorded by the last step."""
+        n = C.c_int64()
+        _check(self._L.nphys_read_samples(self._h, None, None, 0, C.byref(n)))
+        X = np.zeros((n.value, 24), np.float32)
+        T = np.zeros((n.value, 3), np.float32)
+        if n.value:
+            _check(self._L.nphys_read_samples(self._h, abi.fptr(X), abi.fptr(T), n.value, C.byref(n)))
+        return X, T
+
+    def set_warmstart_judge(self, on=True):
+        _check(self._L.nphys_set_warmstart_judge(self._h, 1 if on else 0))
 
     def record_colours(self):
         """Starts recording the manifold colours of every following step."""

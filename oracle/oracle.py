@@ -44,6 +44,7 @@ def lib():
         L.orc_set_params.argtypes = [vp, C.POINTER(abi.Params)]
         L.orc_set_order_mode.argtypes = [vp, C.c_int32]
         L.orc_set_brute_force_limit.argtypes = [vp, C.c_int32]
+        L.orc_set_residual_target.argtypes = [vp, C.c_double, C.c_int32, C.c_int32]
         L.orc_add_bodies.argtypes = [vp, C.POINTER(abi.BodyDesc), C.POINTER(C.c_int32)]
         L.orc_num_bodies.argtypes = [vp, C.POINTER(C.c_int32)]
         fp = C.POINTER(C.c_float)
@@ -68,6 +69,8 @@ def lib():
         L.orc_collide_shapes.argtypes = [C.c_int32, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double,
                                          C.POINTER(C.c_uint32), dp, dp, dp, dp, ip]
         L.orc_colour_manifolds.argtypes = [C.c_int32, ip, ip, C.c_int32, ip, ip]
+        L.orc_mlp_forward.argtypes = [dp, fp, C.c_int64, C.c_int32, dp]
+        L.orc_mlp_gradients.argtypes = [dp, fp, fp, C.c_int64, C.c_int32, dp, dp]
         _lib = L
     return _lib
 
@@ -98,6 +101,9 @@ class OracleWorld:
     def set_params(self, params):
         self.params = params
         _check(self._L.orc_set_params(self._h, C.byref(params)))
+
+    def set_residual_target(self, r_inf=1e-4, max_sweeps=500, check_every=1):
+        _check(self._L.orc_set_residual_target(self._h, r_inf, max_sweeps, check_every))
 
     def set_brute_force_limit(self, n):
         _check(self._L.orc_set_brute_force_limit(self._h, n))
@@ -236,3 +242,30 @@ def colour_manifolds(b1, b2, is_dynamic, initial=None):
     out = np.full(b1.shape[0], -1, np.int32) if initial is None else np.array(initial, np.int32)
     nc = L.orc_colour_manifolds(b1.shape[0], abi.iptr(b1), abi.iptr(b2), dyn.shape[0], abi.iptr(dyn), abi.iptr(out))
     return out, nc
+
+
+MLP_NUM_TRAINED = 20099
+MLP_NUM_PARAMS = 20150
+
+
+def mlp_forward(params, X, emulate_bf16=True):
+    """Network outputs (impulses) for rows X[n][24]; params in the C ABI layout."""
+    L = lib()
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(X, np.float32)
+    out = np.zeros((x.shape[0], 3))
+    _check(L.orc_mlp_forward(abi.dptr(p), abi.fptr(x), x.shape[0], int(emulate_bf16), abi.dptr(out)))
+    return out
+
+
+def mlp_gradients(params, X, T, emulate_bf16=True):
+    """(gradient over the 20,099 trained parameters, loss) of the batch X, T."""
+    L = lib()
+    p = np.ascontiguousarray(params, np.float64)
+    x = np.ascontiguousarray(X, np.float32)
+    t = np.ascontiguousarray(T, np.float32)
+    g = np.zeros(MLP_NUM_TRAINED)
+    loss = C.c_double()
+    _check(L.orc_mlp_gradients(abi.dptr(p), abi.fptr(x), abi.fptr(t), x.shape[0], int(emulate_bf16), abi.dptr(g),
+                               C.byref(loss)))
+    return g, loss.value

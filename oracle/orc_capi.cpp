@@ -236,6 +236,16 @@ int orc_set_order_mode(void* h, int32_t mode) {
   return with_world(h, [&](auto& w) { w.order_mode = mode; return NPHYS_OK; });
 }
 
+int orc_set_residual_target(void* h, double target, int32_t max_sweeps, int32_t check_every) {
+  if (!(target >= 0.0) || max_sweeps < 0 || check_every < 1) return fail(NPHYS_ERR_INVALID, "bad residual target");
+  return with_world(h, [&](auto& w) {
+    w.res_target = target;
+    w.res_max_sweeps = max_sweeps;
+    w.res_check_every = check_every;
+    return NPHYS_OK;
+  });
+}
+
 int orc_set_brute_force_limit(void* h, int32_t n) {
   return with_world(h, [&](auto& w) { w.brute_force_limit = n; return NPHYS_OK; });
 }
@@ -337,6 +347,7 @@ int orc_read_counters(void* h, nphys_counters* c) {
     c->nmanifolds = int64_t(w.solved.size());
     c->nconstraints = w.n_rows;
     c->ncolours = w.ncolours;
+    c->velocity_sweeps = w.velocity_sweeps;
     return NPHYS_OK;
   });
 }
@@ -344,6 +355,7 @@ int orc_read_counters(void* h, nphys_counters* c) {
 int orc_reset_counters(void* h) {
   return with_world(h, [&](auto& w) {
     w.counters = orc::Counters();
+    w.velocity_sweeps = 0;
     return NPHYS_OK;
   });
 }

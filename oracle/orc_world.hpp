@@ -734,47 +734,73 @@ class World {
     row_add(c, dl);
   }
 
-  void sor_prox() {  // SORProx::solve (sor_prox.rs:48-80)
+  // SORProx::solve (sor_prox.rs:48-80), split into the warm start and one sweep.
+  void warmstart() {  // sor_prox.rs:57-58, 19-45
     if (order_mode == ORDER_REFERENCE) {
       for (auto& c : uni) warm_row(c);
       for (auto& c : uni_g) warm_row(c);
       for (auto& c : bil) warm_row(c);
       for (auto& c : bil_g) warm_row(c);
-      for (int it = 0; it < prm.max_velocity_iterations; ++it) {
-        for (auto& c : bil) solve_friction(c, uni);
-        for (auto& c : bil_g) solve_friction(c, uni_g);
-        for (auto& c : uni) solve_normal(c);
-        for (auto& c : uni_g) solve_normal(c);
+      return;
+    }
+    for (int m : colour_order) {
+      const Manifold& mf = solved[m];
+      for (int k = 0; k < mf.count; ++k) {
+        const ContactRows& cr = crow[mf.first + k];
+        warm_row(cr.ground ? uni_g[cr.normal] : uni[cr.normal]);
       }
-    } else {
-      for (int m : colour_order) {
-        const Manifold& mf = solved[m];
-        for (int k = 0; k < mf.count; ++k) {
-          const ContactRows& cr = crow[mf.first + k];
-          warm_row(cr.ground ? uni_g[cr.normal] : uni[cr.normal]);
-        }
-        for (int k = 0; k < mf.count; ++k) {
-          const ContactRows& cr = crow[mf.first + k];
-          for (int f = 0; f < 2; ++f) warm_row(cr.ground ? bil_g[cr.fric[f]] : bil[cr.fric[f]]);
-        }
-      }
-      for (int it = 0; it < prm.max_velocity_iterations; ++it) {
-        for (int m : colour_order) {
-          const Manifold& mf = solved[m];
-          for (int k = 0; k < mf.count; ++k) {
-            const ContactRows& cr = crow[mf.first + k];
-            for (int f = 0; f < 2; ++f) {
-              if (cr.ground) solve_friction(bil_g[cr.fric[f]], uni_g);
-              else solve_friction(bil[cr.fric[f]], uni);
-            }
-          }
-          for (int k = 0; k < mf.count; ++k) {
-            const ContactRows& cr = crow[mf.first + k];
-            solve_normal(cr.ground ? uni_g[cr.normal] : uni[cr.normal]);
-          }
-        }
+      for (int k = 0; k < mf.count; ++k) {
+        const ContactRows& cr = crow[mf.first + k];
+        for (int f = 0; f < 2; ++f) warm_row(cr.ground ? bil_g[cr.fric[f]] : bil[cr.fric[f]]);
       }
     }
+  }
+  void sweep() {  // SORProx::step (sor_prox.rs:159-179)
+    if (order_mode == ORDER_REFERENCE) {
+      for (auto& c : bil) solve_friction(c, uni);
+      for (auto& c : bil_g) solve_friction(c, uni_g);
+      for (auto& c : uni) solve_normal(c);
+      for (auto& c : uni_g) solve_normal(c);
+      return;
+    }
+    for (int m : colour_order) {
+      const Manifold& mf = solved[m];
+      for (int k = 0; k < mf.count; ++k) {
+        const ContactRows& cr = crow[mf.first + k];
+        for (int f = 0; f < 2; ++f) {
+          if (cr.ground) solve_friction(bil_g[cr.fric[f]], uni_g);
+          else solve_friction(bil[cr.fric[f]], uni);
+        }
+      }
+      for (int k = 0; k < mf.count; ++k) {
+        const ContactRows& cr = crow[mf.first + k];
+        solve_normal(cr.ground ? uni_g[cr.normal] : uni[cr.normal]);
+      }
+    }
+  }
+  // Fixed count (the reference), or, with res_target > 0, sweeps until the residual max |phi| is
+  // at most res_target, checked after the warm start and every res_check_every sweeps.
+  double res_target = 0.0;
+  int res_max_sweeps = 500, res_check_every = 1, last_sweeps = 0;
+  long velocity_sweeps = 0;
+  void sor_prox() {
+    warmstart();
+    if (res_target <= 0.0) {
+      for (int it = 0; it < prm.max_velocity_iterations; ++it) sweep();
+      last_sweeps = prm.max_velocity_iterations;
+      compute_residual();
+    } else {
+      int done = 0;
+      for (;;) {
+        compute_residual();
+        if (double(res_inf) <= res_target || done >= res_max_sweeps) break;
+        const int k = std::min(res_check_every, res_max_sweeps - done);
+        for (int i = 0; i < k; ++i) sweep();
+        done += k;
+      }
+      last_sweeps = done;
+    }
+    velocity_sweeps += last_sweeps;
   }
 
   // Natural-map residual of the LCP in velocity units, per row:
@@ -922,7 +948,6 @@ class World {
     ++solve_count;
     if (order_mode == ORDER_COLOURED) build_colour_order();
     sor_prox();
-    compute_residual();
     cache_impulses();
     counters.velocity_resolution += now_s() - t0;
     t0 = now_s();

@@ -151,3 +151,58 @@ def test_free_fall_closed_form(nphys):
     w.step(1000)
     y = float(w.get_state()["pos"][0, 1])
     assert (y0 - y) == pytest.approx(20.0 * 1e-6 * 1000 * 1001 / 2, rel=1e-3)
+
+
+def test_ball_drop_full_config_bitexact(nphys, oracle_mod):
+    """The ball-drop workload at full size: 1,000 balls r=0.5 on a 10^3 grid (1.1 m spacing,
+    +-0.05 m jitter) over a static ground, 8 velocity / 3 position iterations, 200 steps."""
+    s = scenes.balls()
+    g = _gpu_world(nphys, s, max_velocity_iterations=8)
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED, max_velocity_iterations=8)
+    g.step(200)
+    o.step(200)
+    gs, os_ = g.get_state(), o.get_state()
+    for key in ("pos", "rot", "linvel", "angvel"):
+        np.testing.assert_array_equal(gs[key], _f32(os_[key]), err_msg=key)
+    np.testing.assert_array_equal(g.pairs(), o.pairs())
+    gk, gi = g.impulses()
+    ok, oi = o.impulses()
+    np.testing.assert_array_equal(gk, ok)
+    np.testing.assert_array_equal(gi, _f32(oi))
+    c = g.counters()
+    print(f"ball drop: {c['ncontacts']} contacts after 200 steps, residual {g.residual()}")
+    # at rest on the ground: no ball below its radius, none still falling fast
+    assert gs["pos"][1:, 1].min() > 0.5 - 0.02
+    assert np.abs(gs["linvel"][1:]).max() < 1.0
+
+
+def test_box_stack_full_config(nphys, oracle_mod):
+    """The box-stack workload at full size: 8,000 unit cuboids on a 20^3 grid (1.05 m spacing,
+    +-2 degree tilts), friction 0.6, 10 velocity / 3 position iterations. The first 120 steps are
+    compared bit for bit with the checker; the remaining 380 of the 500 steps run on the GPU
+    alone and are checked through what the scene fixes. With 10 iterations the 20-high columns
+    are not converged and lean (the checker's reference row order does the same), so the checks
+    are the ones that hold whatever the columns do: every value finite, no box through the
+    ground, no speed above the free-fall speed from the top of the stack."""
+    s = scenes.boxes()
+    g = _gpu_world(nphys, s, max_velocity_iterations=10)
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED, max_velocity_iterations=10)
+    g.step(120)
+    o.step(120)
+    gs, os_ = g.get_state(), o.get_state()
+    for key in ("pos", "rot", "linvel", "angvel"):
+        np.testing.assert_array_equal(gs[key], _f32(os_[key]), err_msg=key)
+    gk, gi = g.impulses()
+    ok, oi = o.impulses()
+    np.testing.assert_array_equal(gk, ok)
+    np.testing.assert_array_equal(gi, _f32(oi))
+    g.step(380)
+    st = g.get_state()
+    assert all(np.isfinite(st[k]).all() for k in st)
+    assert st["pos"][1:, 1].min() > 0.5 - 0.02
+    v_free = np.sqrt(2 * 9.81 * 21.0)
+    assert np.linalg.norm(st["linvel"][1:], axis=1).max() < v_free
+    norms = np.linalg.norm(st["rot"], axis=1)
+    assert np.abs(norms - 1).max() < 1e-3
+    c = g.counters()
+    print(f"box stack: {c['ncontacts']} contacts after 500 steps, residual {g.residual()}")
