@@ -82,6 +82,51 @@ def cpu_baseline(scene, snapshot, params, steps):
     }
 
 
+def bench_batched(args):
+    """4,096 independent worlds (lognormal sizes, median 64 bodies, each with its own ground)
+    stepped as one handle; 8 velocity / 3 position iterations. Prints one JSON line."""
+    params = abi.default_params(max_velocity_iterations=8, max_position_iterations=3)
+    scene = scenes.batched()
+    w = world.MechanicalWorld(integration_parameters=params)
+    w.add_bodies(scene)
+    settle_steps = min(args.settle, 200)
+    t0 = time.perf_counter()
+    settle(w, settle_steps)
+    log(f"settled in {time.perf_counter() - t0:.1f} s")
+    w.step(args.warmup)
+    w.reset_counters()
+    t0 = time.perf_counter()
+    w.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    c = w.counters()
+    n_dyn = int((scene["status"] == 0).sum())
+    out = {
+        "metric": "steps/s of 4,096 independent worlds stepped as one batch",
+        "value": args.steps / elapsed,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched worlds: 4,096 scenes, lognormal sizes (median 64, clip 8-2048), "
+                               "balls/cuboids, own ground each; 8 velocity + 3 position iterations, dt 1/60",
+                   "bodies": int(scene["status"].shape[0]), "dynamic_bodies": n_dyn,
+                   "settle_steps": settle_steps, "contacts": c["ncontacts"], "manifolds": c["nmanifolds"],
+                   "colours": c["ncolours"]},
+        "contact_iters_per_s": c["ncontacts"] * 8 * args.steps / elapsed,
+        "world_steps_per_s": 4096 * args.steps / elapsed,
+        "stages_ms_per_step": {k: 1e3 * c[k] / args.steps for k in
+                               ("collision_detection", "broad_phase", "narrow_phase", "assembly",
+                                "velocity_resolution", "velocity_update", "position_resolution")},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +136,13 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--save-state", default=None, help="write the settled pile's state (.npz)")
+    ap.add_argument("--load-state", default=None, help="start from a state written by --save-state")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 512 MiB device copy 5 times (the reference kernel of the "
+                         "FETCH_SIZE / WRITE_SIZE counter calibration) and exit")
+    ap.add_argument("--workload", choices=("pile", "batched"), default="pile",
+                    help="pile: the dense pile (default); batched: 4,096 independent worlds in one handle")
     args = ap.parse_args()
 
     ws = int(os.environ.get("WORLD_SIZE", "1"))
@@ -99,14 +151,35 @@ def main():
               "multi-GPU path; run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        import torch
+
+        a = torch.empty(512 << 20, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 512 << 20, "launches": 5}), flush=True)
+        return
+    if args.workload == "batched":
+        return bench_batched(args)
     params = abi.default_params(max_velocity_iterations=16, max_position_iterations=3)
     scene = scenes.pile()
     w = world.MechanicalWorld(integration_parameters=params)
     w.add_bodies(scene)
-    t0 = time.perf_counter()
-    settle(w, args.settle)
-    log(f"settled in {time.perf_counter() - t0:.1f} s")
+    if args.load_state:
+        # a settled pile saved by an earlier run (--save-state); the impulse cache starts empty
+        st = np.load(args.load_state)
+        w.set_state(**{k: st[k] for k in ("pos", "rot", "linvel", "angvel")})
+        log(f"loaded settled state from {args.load_state}")
+    else:
+        t0 = time.perf_counter()
+        settle(w, args.settle)
+        log(f"settled in {time.perf_counter() - t0:.1f} s")
     snapshot = w.get_state()
+    if args.save_state:
+        np.savez(args.save_state, **snapshot)
     w.step(args.warmup)
 
     # Timed region: K steps, sweep phases launched as hipGraphs, counters off.

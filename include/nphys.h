@@ -85,7 +85,12 @@ typedef struct nphys_body_desc {
   const float* linear_prediction;   /* n: default 0.001 */
   const float* friction;            /* n: BasicMaterial friction (default 0.5) */
   const float* restitution;         /* n: BasicMaterial restitution (default 0) */
+  /* n: scene index in [0, NPHYS_MAX_SCENES), or NULL for scene 0. Bodies of different scenes
+   * never interact: one handle then steps a batch of independent worlds (the reference needs one
+   * MechanicalWorld per scene). */
+  const int32_t* scene;
 } nphys_body_desc;
+#define NPHYS_MAX_SCENES 32768
 
 /* Counters (src/counters/mod.rs:17-219): accumulated stage seconds plus sizes of the last step. */
 typedef struct nphys_counters {
@@ -114,6 +119,7 @@ typedef struct nphys_counters {
   int64_t judge_plain_sweeps;
   int64_t judge_learned_sweeps;
   int64_t judge_kept;
+  int64_t sweep_variant;  /* sweeps of the last step: 0 per-colour launches, 64/128/256 scene-blocked */
 } nphys_counters;
 
 /* ---------------------------------------------------------------- lifecycle */
@@ -198,12 +204,24 @@ int nphys_reset_counters(nphys_world* w);
  * MechanicalWorld::new creates Counters::new(false) (mechanical_world.rs:77). */
 int nphys_counters_enable(nphys_world* w, int32_t enabled);
 
-/* Solve-to-residual mode of the velocity solver. With r_inf_target > 0 every step runs SOR-Prox
- * sweeps until the natural-map residual max |phi| (see nphys_residual) is <= r_inf_target, checked
- * after the warm start and after every check_every sweeps, at most max_sweeps sweeps; with
- * r_inf_target == 0 (the default) it runs max_velocity_iterations sweeps as SORProx::solve does
- * (sor_prox.rs:56-79). No reference counterpart: the reference only has the fixed count. */
-int nphys_set_residual_target(nphys_world* w, double r_inf_target, int32_t max_sweeps, int32_t check_every);
+/* Solve-to-residual mode of the velocity solver. With target > 0 every step runs SOR-Prox sweeps
+ * until the natural-map residual (see nphys_residual) is <= target, checked after the warm start
+ * and after every check_every sweeps, at most max_sweeps sweeps; the residual is max |phi| with
+ * norm == NPHYS_RESIDUAL_MAX, the root mean square with NPHYS_RESIDUAL_RMS. With target == 0 (the
+ * default) it runs max_velocity_iterations sweeps as SORProx::solve does (sor_prox.rs:56-79).
+ * No reference counterpart: the reference only has the fixed count. */
+#define NPHYS_RESIDUAL_MAX 0
+#define NPHYS_RESIDUAL_RMS 1
+int nphys_set_residual_target(nphys_world* w, double target, int32_t max_sweeps, int32_t check_every,
+                              int32_t norm);
+
+/* Launch shape of the velocity sweeps. 0: one launch per colour and sweep over all manifolds of
+ * the colour; 64, 128, 256: one launch per solve with one workgroup of that size per scene,
+ * the scene's velocity increments in LDS (only when every scene has at most 2048 dynamic
+ * bodies; otherwise 0 is used); -1 (default): an autotuner times every eligible variant over
+ * the first solves and keeps the fastest. All variants give bit-identical results. No reference
+ * counterpart (the reference has one sequential loop). */
+int nphys_set_sweep_variant(nphys_world* w, int32_t variant);
 
 /* Colour of every manifold solved by the last step, in manifold order. Recording starts with the
  * first call (a call with colours == NULL only enables it), so the first step after enabling is

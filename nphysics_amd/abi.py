@@ -90,6 +90,7 @@ class BodyDesc(C.Structure):
         ("linear_prediction", C.POINTER(C.c_float)),
         ("friction", C.POINTER(C.c_float)),
         ("restitution", C.POINTER(C.c_float)),
+        ("scene", C.POINTER(C.c_int32)),
     ]
 
 
@@ -116,6 +117,7 @@ class Counters(C.Structure):
         ("judge_plain_sweeps", C.c_int64),
         ("judge_learned_sweeps", C.c_int64),
         ("judge_kept", C.c_int64),
+        ("sweep_variant", C.c_int64),
     ]
 
     def as_dict(self):
@@ -166,4 +168,6 @@ def make_body_desc(scene):
     d.linear_prediction = fptr(arr("linear_prediction", np.float32, 1))
     d.friction = fptr(arr("friction", np.float32, 1))
     d.restitution = fptr(arr("restitution", np.float32, 1))
+    if "scene" in scene:
+        d.scene = iptr(arr("scene", np.int32, 1))
     return d, keep

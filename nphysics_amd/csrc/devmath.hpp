@@ -151,14 +151,26 @@ NPH_HD q4 qexp_scaled_axis(f3 axisangle) {
   const float e = 1.1920928955078125e-07f;  // FP32 machine epsilon
   if (nn <= e * e) return q4{0.0f, 0.0f, 0.0f, 1.0f};
   const float n = sqrtf(nn);
-  float s, c;
-  if (n < 0.25f) {
-    const float n2 = n * n;
-    s = 1.0f + n2 * ((float)(-1.0 / 6.0) + n2 * ((float)(1.0 / 120.0) + n2 * ((float)(-1.0 / 5040.0) + n2 * (float)(1.0 / 362880.0))));
-    c = 1.0f + n2 * (-0.5f + n2 * ((float)(1.0 / 24.0) + n2 * ((float)(-1.0 / 720.0) + n2 * (float)(1.0 / 40320.0))));
-  } else {
-    s = sinf(n) / n;
-    c = cosf(n);
+  // sin(n)/n and cos(n): Maclaurin polynomials below 0.25; larger angles halved exactly until
+  // below 0.25 and rebuilt with the double-angle formulas (the checker's arithmetic, bit for bit)
+  float x = n;
+  int k = 0;
+  while (x >= 0.25f && k < 64) {
+    x = x * 0.5f;
+    ++k;
+  }
+  const float x2 = x * x;
+  float s = 1.0f + x2 * ((float)(-1.0 / 6.0) + x2 * ((float)(1.0 / 120.0) + x2 * ((float)(-1.0 / 5040.0) + x2 * (float)(1.0 / 362880.0))));
+  float c = 1.0f + x2 * (-0.5f + x2 * ((float)(1.0 / 24.0) + x2 * ((float)(-1.0 / 720.0) + x2 * (float)(1.0 / 40320.0))));
+  if (k > 0) {
+    float sn = x * s;
+    for (int i = 0; i < k; ++i) {
+      const float s2 = 2.0f * sn * c;
+      const float c2 = c * c - sn * sn;
+      sn = s2;
+      c = c2;
+    }
+    s = sn / n;
   }
   return q4{h.x * s, h.y * s, h.z * s, c};
 }

@@ -25,6 +25,7 @@ struct DevBodies {
   int n;
   const int* status;
   const int* shape;
+  const int* scene;      // independent-world index; bodies of different scenes never interact
   const float4* dims;    // half extents; balls (r, r, r)
   const float* margin;
   const float* qlim;     // margin + linear prediction

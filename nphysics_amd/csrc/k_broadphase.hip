@@ -92,24 +92,32 @@ __device__ __forceinline__ uint32_t expand10(uint32_t v) {
   return v;
 }
 
-__device__ __forceinline__ uint32_t quant10(float c, float lo, float hi) {
+// Coordinate quantised to `levels` (2^bits - 1) steps of the centroid bounds.
+__device__ __forceinline__ uint32_t quant(float c, float lo, float hi, float levels, uint32_t top) {
   const float ext = hi - lo;
   float u = ext > 0.0f ? (c - lo) / ext : 0.5f;
   u = u < 0.0f ? 0.0f : (u > 1.0f ? 1.0f : u);
-  const uint32_t q = uint32_t(u * 1023.0f);
-  return q > 1023u ? 1023u : q;
+  const uint32_t q = uint32_t(u * levels);
+  return q > top ? top : q;
 }
 
-__global__ void __launch_bounds__(256) k_morton(DevBodies b, const uint32_t* bounds, uint32_t* keys,
-                                               uint32_t* vals) {
+// Key = scene << 3m | Morton code of the loose-AABB centroid with m bits per axis,
+// m = (30 - scene_bits) / 3 (10 for a single scene).
+__global__ void __launch_bounds__(256) k_morton(DevBodies b, const uint32_t* bounds, int scene_bits,
+                                               uint32_t* keys, uint32_t* vals) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
+  const int m = (30 - scene_bits) / 3;
+  const uint32_t top = (1u << m) - 1u;
+  const float levels = float(top);
   const float lx = ord2f(bounds[0]), ly = ord2f(bounds[1]), lz = ord2f(bounds[2]);
   const float hx = ord2f(bounds[3]), hy = ord2f(bounds[4]), hz = ord2f(bounds[5]);
   const float4 mn = b.loose_mn[i], mx = b.loose_mx[i];
   const float cx = (mn.x + mx.x) * 0.5f, cy = (mn.y + mx.y) * 0.5f, cz = (mn.z + mx.z) * 0.5f;
-  const uint32_t x = quant10(cx, lx, hx), y = quant10(cy, ly, hy), z = quant10(cz, lz, hz);
-  keys[i] = (expand10(x) << 2) | (expand10(y) << 1) | expand10(z);
+  const uint32_t x = quant(cx, lx, hx, levels, top), y = quant(cy, ly, hy, levels, top),
+                 z = quant(cz, lz, hz, levels, top);
+  const uint32_t morton = (expand10(x) << 2) | (expand10(y) << 1) | expand10(z);
+  keys[i] = scene_bits ? (uint32_t(b.scene[i]) << (3 * m)) | morton : morton;
   vals[i] = uint32_t(i);
 }
 
@@ -122,8 +130,8 @@ __device__ __forceinline__ int delta(const uint32_t* __restrict__ keys, int n, i
 
 // Karras 2012: internal node i spans a key range with one split; children < n-1 are internal,
 // children >= n-1 are leaves (leaf t is node n-1+t).
-__global__ void __launch_bounds__(256) k_lbvh(const uint32_t* __restrict__ keys, int n, int* child,
-                                             int* parent) {
+__global__ void __launch_bounds__(256) k_lbvh(const uint32_t* __restrict__ keys, int n, int scene_shift,
+                                             int* child, int* parent, int2* node_scene) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -147,6 +155,7 @@ __global__ void __launch_bounds__(256) k_lbvh(const uint32_t* __restrict__ keys,
   const int right = hi == gamma + 1 ? (n - 1 + gamma + 1) : gamma + 1;
   child[2 * i] = left;
   child[2 * i + 1] = right;
+  node_scene[i] = make_int2(int(keys[lo] >> scene_shift), int(keys[hi] >> scene_shift));
   parent[left] = i;
   parent[right] = i;
   if (i == 0) parent[0] = -1;
@@ -207,7 +216,8 @@ __global__ void __launch_bounds__(256) k_init_inodes(int n, uint32_t* inode) {
 __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* __restrict__ vals, int n,
                                                  const int* __restrict__ child,
                                                  const float4* __restrict__ node_mn,
-                                                 const float4* __restrict__ node_mx, int* cand,
+                                                 const float4* __restrict__ node_mx,
+                                                 const int2* __restrict__ node_scene, int* cand,
                                                  uint32_t* cand_count, int cap, uint32_t* stats) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -217,6 +227,7 @@ __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* _
     return;
   }
   const aabb Q{xyz(b.loose_mn[q]), xyz(b.loose_mx[q])};
+  const int qs = b.scene[q];
   const long base = long(q) * cap;
   uint32_t cnt = 0;
   int stack[kStack];
@@ -228,11 +239,15 @@ __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* _
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const int c = k == 0 ? c0 : c1;
+      if (c < n - 1) {
+        const int2 sr = node_scene[c];
+        if (qs < sr.x || qs > sr.y) continue;
+      }
       const aabb B{xyz(node_mn[c]), xyz(node_mx[c])};
       if (!intersects(Q, B)) continue;
       if (c >= n - 1) {
         const int o = int(vals[c - (n - 1)]);
-        if (o != q && (b.status[o] != STATUS_DYNAMIC || o > q)) {
+        if (o != q && b.scene[o] == qs && (b.status[o] != STATUS_DYNAMIC || o > q)) {
           if (cnt < uint32_t(cap)) cand[base + cnt] = o;
           ++cnt;
         }
@@ -294,9 +309,11 @@ void launch_build_lbvh(const DevBodies& b, BroadPhaseBufs& bp, hipStream_t s) {
   const uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0u, 0u, 0u};
   (void)hipMemcpyAsync(bp.bounds, init, sizeof(init), hipMemcpyHostToDevice, s);
   hipLaunchKernelGGL(k_bounds, dim3(nb(n)), dim3(256), 0, s, b, bp.bounds);
-  hipLaunchKernelGGL(k_morton, dim3(nb(n)), dim3(256), 0, s, b, bp.bounds, bp.keys, bp.vals);
+  hipLaunchKernelGGL(k_morton, dim3(nb(n)), dim3(256), 0, s, b, bp.bounds, bp.scene_bits, bp.keys, bp.vals);
   radix_sort_pairs(bp.keys, bp.vals, bp.keys_alt, bp.vals_alt, n, 30, bp.tmp, s);
-  hipLaunchKernelGGL(k_lbvh, dim3(nb(n - 1)), dim3(256), 0, s, bp.keys, n, bp.child, bp.parent);
+  const int scene_shift = 3 * ((30 - bp.scene_bits) / 3);
+  hipLaunchKernelGGL(k_lbvh, dim3(nb(n - 1)), dim3(256), 0, s, bp.keys, n, scene_shift, bp.child, bp.parent,
+                     bp.node_scene);
   hipLaunchKernelGGL(k_init_inodes, dim3(nb(n - 1)), dim3(256), 0, s, n, bp.visit);
   hipLaunchKernelGGL(k_leaf_nodes, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.node_mn, bp.node_mx);
   hipLaunchKernelGGL(k_refit_atomic, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.parent, bp.visit);
@@ -312,7 +329,7 @@ void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_
     return;
   }
   hipLaunchKernelGGL(k_traverse, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
-                     bp.node_mx, bp.cand, bp.cand_count, cap, bp.stats);
+                     bp.node_mx, bp.node_scene, bp.cand, bp.cand_count, cap, bp.stats);
 }
 
 void launch_compact_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s) {

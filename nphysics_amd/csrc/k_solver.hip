@@ -470,86 +470,202 @@ __device__ __forceinline__ void colour_range(const SolverBufs& sb, int k, int& f
   n = int(sb.col_start[col + 1]) - first;
 }
 
-// Warm start (SORProx::warmstart_set, sor_prox.rs:19-45): dv += l W J for every row with l != 0,
-// normal rows first, then friction rows.
+// Solver velocity increments of the manifold's bodies in HBM (the per-colour kernels) ...
+struct GlobalDv {
+  DevBodies b;
+  __device__ void load(const Manifold& m, Dv& v1, Dv& v2) const { load_dv(b, m, v1, v2); }
+  __device__ void store(const Manifold& m, const Dv& v1, const Dv& v2) const { store_dv(b, m, v1, v2); }
+};
+
+// ... or in LDS, indexed by the body's rank among its scene's dynamic bodies (the scene-blocked
+// kernel). The values are the same FP32 numbers either way.
+struct LdsDv {
+  float4* lin;
+  float4* ang;
+  const int* local;
+  __device__ void load(const Manifold& m, Dv& v1, Dv& v2) const {
+    if (m.x.d1) {
+      const int i = local[m.x.b1];
+      v1.l = xyz(lin[i]);
+      v1.a = xyz(ang[i]);
+    }
+    if (m.x.d2) {
+      const int i = local[m.x.b2];
+      v2.l = xyz(lin[i]);
+      v2.a = xyz(ang[i]);
+    }
+  }
+  __device__ void store(const Manifold& m, const Dv& v1, const Dv& v2) const {
+    if (m.x.d1) {
+      const int i = local[m.x.b1];
+      lin[i] = f4(v1.l, 0.0f);
+      ang[i] = f4(v1.a, 0.0f);
+    }
+    if (m.x.d2) {
+      const int i = local[m.x.b2];
+      lin[i] = f4(v2.l, 0.0f);
+      ang[i] = f4(v2.a, 0.0f);
+    }
+  }
+};
+
+// Warm start of one manifold (SORProx::warmstart_set, sor_prox.rs:19-45): dv += l W J for every
+// row with l != 0, normal rows first, then friction rows.
+template <class A>
+__device__ __forceinline__ void warm_manifold(const A& acc, const DevBodies& b, const SolverBufs& sb, int slot) {
+  Manifold m;
+  load_manifold(b, sb, slot, m);
+  Dv v1{}, v2{};
+  acc.load(m, v1, v2);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k >= m.x.cnt) break;
+    if (m.c[k].lam.x != 0.0f) row_add(row_of(m, k, 0), m.c[k].lam.x, m, v1, v2);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k >= m.x.cnt) break;
+    if (m.c[k].lam.y != 0.0f) row_add(row_of(m, k, 1), m.c[k].lam.y, m, v1, v2);
+    if (m.c[k].lam.z != 0.0f) row_add(row_of(m, k, 2), m.c[k].lam.z, m, v1, v2);
+  }
+  acc.store(m, v1, v2);
+}
+
+// One SOR-Prox step of one manifold (sor_prox.rs:159-343): its friction rows, bounded by mu times
+// the current normal impulse and zeroed when that is zero, then its normal rows, projected on
+// l >= 0.
+template <class A>
+__device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, const SolverBufs& sb, int slot) {
+  Manifold m;
+  load_manifold(b, sb, slot, m);
+  Dv v1{}, v2{};
+  acc.load(m, v1, v2);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k >= m.x.cnt) break;
+    const float mu = m.c[k].mu;
+    const float ln = m.c[k].lam.x;
+#pragma unroll
+    for (int f = 1; f <= 2; ++f) {
+      float li = f == 1 ? m.c[k].lam.y : m.c[k].lam.z;
+      const Row v = row_of(m, k, f);
+      if (ln == 0.0f) {
+        if (li != 0.0f) {
+          row_add(v, -li, m, v1, v2);
+          li = 0.0f;
+        }
+      } else {
+        const float mx = mu * ln;
+        const float mn = -mx;
+        const float w = row_w(v, m, v1, v2);
+        const float val = li - v.r * w;
+        const float ni = val > mn ? (val < mx ? val : mx) : mn;
+        const float dl = ni - li;
+        li = ni;
+        row_add(v, dl, m, v1, v2);
+      }
+      if (f == 1) m.c[k].lam.y = li;
+      else m.c[k].lam.z = li;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k >= m.x.cnt) break;
+    const Row v = row_of(m, k, 0);
+    const float w = row_w(v, m, v1, v2);
+    const float a = m.c[k].lam.x - v.r * w;
+    const float ni = a > 0.0f ? a : 0.0f;
+    const float dl = ni - m.c[k].lam.x;
+    m.c[k].lam.x = ni;
+    row_add(v, dl, m, v1, v2);
+  }
+  acc.store(m, v1, v2);
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < m.x.cnt) sb.rec[long(m.x.r0 + k) * kRecFields + 7] = m.c[k].lam;
+}
+
+// One colour of the warm start / of one sweep: one lane per manifold of the colour visited kv-th.
 __global__ void __launch_bounds__(64) k_warmstart(DevBodies b, DevContacts c, SolverBufs sb, int kv) {
   int first, n;
   colour_range(sb, kv, first, n);
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-    Manifold m;
-    load_manifold(b, sb, first + t, m);
-    Dv v1{}, v2{};
-    load_dv(b, m, v1, v2);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k >= m.x.cnt) break;
-      if (m.c[k].lam.x != 0.0f) row_add(row_of(m, k, 0), m.c[k].lam.x, m, v1, v2);
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k >= m.x.cnt) break;
-      if (m.c[k].lam.y != 0.0f) row_add(row_of(m, k, 1), m.c[k].lam.y, m, v1, v2);
-      if (m.c[k].lam.z != 0.0f) row_add(row_of(m, k, 2), m.c[k].lam.z, m, v1, v2);
-    }
-    store_dv(b, m, v1, v2);
-  }
+  const GlobalDv acc{b};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    warm_manifold(acc, b, sb, first + t);
 }
 
-// One colour of one SOR-Prox sweep (sor_prox.rs:159-343): the manifold's friction rows, bounded
-// by mu times the current normal impulse and zeroed when that is zero, then its normal rows,
-// projected on l >= 0.
 __global__ void __launch_bounds__(64) k_pgs(DevBodies b, DevContacts c, SolverBufs sb, int kv) {
   int first, n;
   colour_range(sb, kv, first, n);
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-    Manifold m;
-    load_manifold(b, sb, first + t, m);
-    Dv v1{}, v2{};
-    load_dv(b, m, v1, v2);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k >= m.x.cnt) break;
-      const float mu = m.c[k].mu;
-      const float ln = m.c[k].lam.x;
-#pragma unroll
-      for (int f = 1; f <= 2; ++f) {
-        float li = f == 1 ? m.c[k].lam.y : m.c[k].lam.z;
-        const Row v = row_of(m, k, f);
-        if (ln == 0.0f) {
-          if (li != 0.0f) {
-            row_add(v, -li, m, v1, v2);
-            li = 0.0f;
-          }
-        } else {
-          const float mx = mu * ln;
-          const float mn = -mx;
-          const float w = row_w(v, m, v1, v2);
-          const float val = li - v.r * w;
-          const float ni = val > mn ? (val < mx ? val : mx) : mn;
-          const float dl = ni - li;
-          li = ni;
-          row_add(v, dl, m, v1, v2);
-        }
-        if (f == 1) m.c[k].lam.y = li;
-        else m.c[k].lam.z = li;
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (k >= m.x.cnt) break;
-      const Row v = row_of(m, k, 0);
-      const float w = row_w(v, m, v1, v2);
-      const float a = m.c[k].lam.x - v.r * w;
-      const float ni = a > 0.0f ? a : 0.0f;
-      const float dl = ni - m.c[k].lam.x;
-      m.c[k].lam.x = ni;
-      row_add(v, dl, m, v1, v2);
-    }
-    store_dv(b, m, v1, v2);
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (k < m.x.cnt) sb.rec[long(m.x.r0 + k) * kRecFields + 7] = m.c[k].lam;
+  const GlobalDv acc{b};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    pgs_manifold(acc, b, sb, first + t);
+}
+
+// Scene-blocked sweeps: one workgroup per scene runs the warm start (when warm) and nsweeps
+// sweeps over all colours in the visiting order, with the scene's dynamic-body velocity
+// increments resident in LDS and a workgroup barrier between colours. Every body sees the same
+// sequence of row updates as under the per-colour launches (manifolds of one colour share no
+// dynamic body; colours and sweeps keep their order), so the results are identical.
+template <int kBlock>
+__global__ void __launch_bounds__(kBlock) k_sweeps_blocked(DevBodies b, SolverBufs sb, SceneBlocks sc, int ncolours,
+                                                         int nsweeps, int warm) {
+  __shared__ float4 s_lin[kLdsBodies];
+  __shared__ float4 s_ang[kLdsBodies];
+  const int s = blockIdx.x;
+  const int d0 = sc.dyn_start[s], nd = sc.dyn_start[s + 1] - d0;
+  for (int i = threadIdx.x; i < nd; i += kBlock) {
+    const int body = sc.dyn_list[d0 + i];
+    s_lin[i] = b.dv_lin[body];
+    s_ang[i] = b.dv_ang[body];
   }
+  __syncthreads();
+  const LdsDv acc{s_lin, s_ang, sc.local};
+  const uint32_t* st = sc.start + long(s) * (ncolours + 1);
+  for (int pass = warm ? -1 : 0; pass < nsweeps; ++pass) {
+    for (int k = 0; k < ncolours; ++k) {
+      const int col = sb.visit[k];
+      const int e = int(st[col + 1]);
+      for (int i = int(st[col]) + threadIdx.x; i < e; i += kBlock) {
+        const int slot = int(sc.slot[i]);
+        if (pass < 0) warm_manifold(acc, b, sb, slot);
+        else pgs_manifold(acc, b, sb, slot);
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < nd; i += kBlock) {
+    const int body = sc.dyn_list[d0 + i];
+    b.dv_lin[body] = s_lin[i];
+    b.dv_ang[body] = s_ang[i];
+  }
+}
+
+// Scene-major slot order for the blocked kernel: key = scene << 12 | colour per slot.
+__global__ void __launch_bounds__(256) k_scene_keys(DevBodies b, int M, SolverBufs sb, uint32_t* keys,
+                                                   uint32_t* slots) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= M) return;
+  const int b1 = sb.sinfo[i].x & 0x3fffffff;
+  keys[i] = (uint32_t(b.scene[b1]) << 12) | sb.ckeys[i];
+  slots[i] = uint32_t(i);
+}
+
+// start[s][c] = first position of (scene s, colour >= c) in the sorted keys (lower bound).
+__global__ void __launch_bounds__(256) k_scene_starts(const uint32_t* __restrict__ keys, int M, int nscenes,
+                                                     int ncolours, uint32_t* start) {
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  const long total = long(nscenes) * (ncolours + 1);
+  if (t >= total) return;
+  const int s = int(t / (ncolours + 1)), c = int(t % (ncolours + 1));
+  const uint32_t key = (uint32_t(s) << 12) | uint32_t(c);
+  int lo = 0, hi = M;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (keys[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  start[t] = uint32_t(lo);
 }
 
 // Colours left empty (inherited colours can leave gaps) start where the next one does.
@@ -814,6 +930,28 @@ void launch_sample_targets(SolverBufs& sb, int C, const float* ls, float* t, hip
 void launch_lam_copy(SolverBufs& sb, int C, float4* buf, bool to_buf, hipStream_t s) {
   if (C == 0) return;
   hipLaunchKernelGGL(k_lam_copy, dim3(nb(C)), dim3(256), 0, s, sb, C, buf, to_buf ? 1 : 0);
+}
+
+void launch_scene_order(const DevBodies& b, int M, int nscenes, int ncolours, SolverBufs& sb, SceneBlocks& sc,
+                        hipStream_t s) {
+  if (M == 0) return;
+  hipLaunchKernelGGL(k_scene_keys, dim3(nb(M)), dim3(256), 0, s, b, M, sb, sc.keys, sc.slot);
+  radix_sort_pairs(sc.keys, sc.slot, sc.keys_alt, sc.slot_alt, M, 28, sb.tmp, s);
+  const long total = long(nscenes) * (ncolours + 1);
+  hipLaunchKernelGGL(k_scene_starts, dim3(nb(total)), dim3(256), 0, s, sc.keys, M, nscenes, ncolours, sc.start);
+}
+
+void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int nscenes, int ncolours,
+                           int nsweeps, bool warm, int block, hipStream_t s) {
+  if (nscenes == 0) return;
+  if (block == 64)
+    hipLaunchKernelGGL(k_sweeps_blocked<64>, dim3(nscenes), dim3(64), 0, s, b, sb, sc, ncolours, nsweeps, warm ? 1 : 0);
+  else if (block == 128)
+    hipLaunchKernelGGL(k_sweeps_blocked<128>, dim3(nscenes), dim3(128), 0, s, b, sb, sc, ncolours, nsweeps,
+                       warm ? 1 : 0);
+  else
+    hipLaunchKernelGGL(k_sweeps_blocked<256>, dim3(nscenes), dim3(256), 0, s, b, sb, sc, ncolours, nsweeps,
+                       warm ? 1 : 0);
 }
 
 }  // namespace nph

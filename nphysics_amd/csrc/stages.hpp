@@ -23,6 +23,8 @@ struct BroadPhaseBufs {
   uint32_t* visit;        // [8(n-1)] internal-node bounds, order-preserving integer encoding
   float4* node_mn;        // [2n-1]
   float4* node_mx;
+  int2* node_scene;       // [n-1] scene range (first, last) of every internal node's leaves
+  int scene_bits;         // Morton keys carry the scene in their top scene_bits bits
   int* cand;              // [n * cap] candidate partners per body
   uint32_t* cand_count;   // [n] partner count (may exceed cap)
   uint32_t* cand_off;     // [n] exclusive scan of min(count, cap)
@@ -106,6 +108,24 @@ void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int bloc
                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, hipStream_t s);
 void launch_cache_impulses(const DevContacts& c, SolverBufs& sb, int C, hipStream_t s);
+
+// Scene-blocked sweeps (k_solver.hip): every scene's slots in (scene, colour) order, per scene
+// and colour the first position, and the scene's dynamic bodies (CSR) with each body's rank.
+constexpr int kLdsBodies = 2048;  // dynamic bodies of one scene the blocked kernel keeps in LDS
+struct SceneBlocks {
+  uint32_t* keys;        // [M] scene << 12 | colour, sorted
+  uint32_t* slot;        // [M] slots in that order
+  uint32_t* keys_alt;
+  uint32_t* slot_alt;
+  uint32_t* start;       // [nscenes][ncolours + 1]
+  const int* dyn_start;  // [nscenes + 1]
+  const int* dyn_list;   // dynamic bodies scene by scene
+  const int* local;      // [n] rank of a dynamic body within its scene
+};
+void launch_scene_order(const DevBodies& b, int M, int nscenes, int ncolours, SolverBufs& sb, SceneBlocks& sc,
+                        hipStream_t s);
+void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int nscenes, int ncolours,
+                           int nsweeps, bool warm, int block, hipStream_t s);
 
 // learned warm start (k_solver.hip): features [C][24] and impulse units ls [C] per row record,
 // the prediction y [C][3] applied as the warm start, post-solve targets t [C][3].

@@ -79,18 +79,19 @@ struct World {
   int32_t contact_model = NPHYS_CONTACT_MODEL_SIGNORINI_COULOMB_PYRAMID;
 
   // host copies of the per-body static attributes
-  std::vector<int> h_status, h_shape;
+  std::vector<int> h_status, h_shape, h_scene;
+  int scene_bits = 0;  // bits of the scene index above the Morton code (0: one scene)
   std::vector<float4> h_dims, h_local_i;
   std::vector<float> h_margin, h_qlim, h_friction, h_restitution, h_mass;
   int n = 0;
 
   // device body arrays
-  DBuf status, shape, dims, margin, qlim, friction, restitution, mass, local_i;
+  DBuf status, shape, scene, dims, margin, qlim, friction, restitution, mass, local_i;
   DBuf pos, rot, pos0, rot0, linvel, angvel, ia0, ia1, ia2, ext_lin, ext_ang, dv_lin, dv_ang;
   DBuf loose_mn, loose_mx;
 
   // broad phase
-  DBuf bp_small, keys, vals, keys_alt, vals_alt, child, parent, visit, node_mn, node_mx, cand,
+  DBuf bp_small, keys, vals, keys_alt, vals_alt, child, parent, visit, node_mn, node_mx, node_scene, cand,
       cand_count, cand_off, pairs, bp_tmp;
   int cap = 32;
   int npairs = 0;
@@ -127,11 +128,24 @@ struct World {
   double res_inf = 0.0, res_rms = 0.0;
   // solve-to-residual mode (off when res_target <= 0): see nphys_set_residual_target
   double res_target = 0.0;
-  int res_max_sweeps = 500, res_check_every = 1;
+  int res_max_sweeps = 500, res_check_every = 1, res_norm = NPHYS_RESIDUAL_MAX;
   int last_sweeps = 0;
   std::vector<double> red_host;
   // learned warm start: network (FP32 parameters + BF16 images), per-record features, impulse
   // units, predictions and recorded targets; the judge's saved plain solve
+  // scene-blocked sweeps: per-scene dynamic-body lists (host-built at insertion), the per-solve
+  // scene-major slot order, and the variant chosen by the autotuner (0 per-colour launches,
+  // 64/128/256 the blocked kernel's workgroup size)
+  int nscenes = 1, max_scene_dyn = 0;
+  DBuf sc_dyn_start, sc_dyn_list, sc_local, sc_keys, sc_slot, sc_keys_alt, sc_slot_alt, sc_start;
+  int variant_request = -1;  // -1: autotune; else forced
+  int cur_variant = 0;
+  static constexpr int kTuneReps = 3;
+  std::vector<int> tune_cands;
+  std::vector<double> tune_time;
+  std::vector<int> tune_count;
+  bool tuned = false, tune_pending = false;
+  hipEvent_t tune_ev[2] = {nullptr, nullptr};
   bool ml_loaded = false, capture = false, judge = false;
   long samples = 0;
   DBuf ml_params, ml_img, ml_X, ml_ls, ml_y, ml_t, jd_lam, jd_dv;
@@ -153,6 +167,8 @@ struct World {
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : kt_ev)
+      if (e) (void)hipEventDestroy(e);
+    for (auto& e : tune_ev)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -181,6 +197,7 @@ struct World {
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_visit), (kMaxColours + 2) * sizeof(int)));
     HIPCK(sv_visit.ensure((kMaxColours + 2) * sizeof(int)));
     for (auto& e : ev) HIPCK(hipEventCreate(&e));
+    for (auto& e : tune_ev) HIPCK(hipEventCreate(&e));
     set_params(p);
     HIPCK(bp_small.ensure(64 * sizeof(uint32_t)));
     // Profilers that cannot follow graph launches can turn them off; kernels are the same.
@@ -194,6 +211,7 @@ struct World {
     b.n = n;
     b.status = status.as<int>();
     b.shape = shape.as<int>();
+    b.scene = scene.as<int>();
     b.dims = dims.as<float4>();
     b.margin = margin.as<float>();
     b.qlim = qlim.as<float>();
@@ -235,6 +253,8 @@ struct World {
       if (st != NPHYS_STATUS_DYNAMIC && st != NPHYS_STATUS_STATIC)
         return set_err(NPHYS_ERR_INVALID, "bad body status");
       if (sh != NPHYS_SHAPE_BALL && sh != NPHYS_SHAPE_CUBOID) return set_err(NPHYS_ERR_INVALID, "bad shape");
+      if (d.scene && (d.scene[k] < 0 || d.scene[k] >= NPHYS_MAX_SCENES))
+        return set_err(NPHYS_ERR_INVALID, "scene index out of range");
       const float* dm = d.dims + 3 * k;
       if (sh == NPHYS_SHAPE_BALL ? !(dm[0] > 0.0f) : !(dm[0] > 0.0f && dm[1] > 0.0f && dm[2] > 0.0f))
         return set_err(NPHYS_ERR_INVALID, "shape dimensions must be positive");
@@ -268,6 +288,7 @@ struct World {
       }
       h_status.push_back(st);
       h_shape.push_back(sh);
+      h_scene.push_back(d.scene ? d.scene[k] : 0);
       h_dims.push_back(he);
       h_local_i.push_back(li);
       h_mass.push_back(m);
@@ -296,13 +317,19 @@ struct World {
       HIPCK(b->ensure(size_t(n1) * f4b, true, stream));
     for (DBuf* b : {&pos0, &rot0, &ia0, &ia1, &ia2, &ext_lin, &ext_ang, &dv_lin, &dv_ang, &dims, &local_i})
       HIPCK(b->ensure(size_t(n1) * f4b));
-    for (DBuf* b : {&status, &shape, &margin, &qlim, &friction, &restitution, &mass})
+    for (DBuf* b : {&status, &shape, &scene, &margin, &qlim, &friction, &restitution, &mass})
       HIPCK(b->ensure(size_t(n1) * 4));
+    // scene bits above the Morton code: the BVH keeps every scene in its own key range
+    int max_scene = 0;
+    for (int sc : h_scene) max_scene = std::max(max_scene, sc);
+    scene_bits = 0;
+    while ((1 << scene_bits) <= max_scene) ++scene_bits;
     auto up = [&](DBuf& b, const void* src, size_t bytes, size_t off) {
       return hipMemcpyAsync(static_cast<char*>(b.p) + off, src, bytes, hipMemcpyHostToDevice, stream);
     };
     HIPCK(up(status, h_status.data(), size_t(n1) * 4, 0));
     HIPCK(up(shape, h_shape.data(), size_t(n1) * 4, 0));
+    HIPCK(up(scene, h_scene.data(), size_t(n1) * 4, 0));
     HIPCK(up(dims, h_dims.data(), size_t(n1) * f4b, 0));
     HIPCK(up(local_i, h_local_i.data(), size_t(n1) * f4b, 0));
     HIPCK(up(mass, h_mass.data(), size_t(n1) * 4, 0));
@@ -316,6 +343,35 @@ struct World {
     HIPCK(up(angvel, hw.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
     HIPCK(up(loose_mn, hmn.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
     HIPCK(up(loose_mx, hmx.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
+    // per-scene dynamic-body lists of the scene-blocked sweeps
+    nscenes = 1;
+    for (int sc : h_scene) nscenes = std::max(nscenes, sc + 1);
+    std::vector<int> dstart(size_t(nscenes) + 1, 0), dlist, local(size_t(n1), -1);
+    for (int i = 0; i < n1; ++i)
+      if (h_status[i] == NPHYS_STATUS_DYNAMIC) ++dstart[size_t(h_scene[i]) + 1];
+    max_scene_dyn = 0;
+    for (int sc = 0; sc < nscenes; ++sc) {
+      max_scene_dyn = std::max(max_scene_dyn, dstart[size_t(sc) + 1]);
+      dstart[size_t(sc) + 1] += dstart[size_t(sc)];
+    }
+    dlist.resize(size_t(dstart[size_t(nscenes)]));
+    {
+      std::vector<int> fill(dstart.begin(), dstart.end() - 1);
+      for (int i = 0; i < n1; ++i)
+        if (h_status[i] == NPHYS_STATUS_DYNAMIC) {
+          const int sc = h_scene[i];
+          local[size_t(i)] = fill[size_t(sc)] - dstart[size_t(sc)];
+          dlist[size_t(fill[size_t(sc)]++)] = i;
+        }
+    }
+    HIPCK(sc_dyn_start.ensure(dstart.size() * 4));
+    HIPCK(sc_dyn_list.ensure(std::max<size_t>(dlist.size(), 1) * 4));
+    HIPCK(sc_local.ensure(local.size() * 4));
+    HIPCK(up(sc_dyn_start, dstart.data(), dstart.size() * 4, 0));
+    if (!dlist.empty()) HIPCK(up(sc_dyn_list, dlist.data(), dlist.size() * 4, 0));
+    HIPCK(up(sc_local, local.data(), local.size() * 4, 0));
+    tune_cands.clear();
+    tuned = false;
     HIPCK(hipStreamSynchronize(stream));
     n = n1;
     pairs_dirty = true;
@@ -379,6 +435,7 @@ struct World {
     HIPCK(parent.ensure(2 * nn * 4));
     HIPCK(node_mn.ensure(2 * nn * sizeof(float4)));
     HIPCK(node_mx.ensure(2 * nn * sizeof(float4)));
+    HIPCK(node_scene.ensure(nn * sizeof(int2)));
     HIPCK(cand.ensure(nn * size_t(cap) * 4));
     const size_t tw = std::max(radix_temp_words(int(nn)), scan_temp_words(int(nn)));
     HIPCK(bp_tmp.ensure(tw * 4));
@@ -400,6 +457,8 @@ struct World {
     v.visit = visit.as<uint32_t>();
     v.node_mn = node_mn.as<float4>();
     v.node_mx = node_mx.as<float4>();
+    v.node_scene = node_scene.as<int2>();
+    v.scene_bits = scene_bits;
     v.cand = cand.as<int>();
     v.cand_count = cand_count.as<uint32_t>();
     v.cand_off = cand_off.as<uint32_t>();
@@ -644,6 +703,24 @@ struct World {
   // sweep kernels are launched directly with HIP events in their dispatch packets; otherwise
   // the phase runs as a cached hipGraph.
   int sweeps(const DevBodies& b, const DevContacts& cv, SolverBufs& v, int sblocks, bool warm, int nsweeps) {
+    if (cur_variant > 0) {
+      // scene-blocked: one launch for the warm start and all nsweeps sweeps
+      SceneBlocks sc = sc_view();
+      if (counters_enabled) {
+        while (kt_ev.size() < size_t(kt_used) + 2) {
+          hipEvent_t e;
+          HIPCK(hipEventCreate(&e));
+          kt_ev.push_back(e);
+        }
+        HIPCK(hipEventRecord(kt_ev[kt_used], stream));
+      }
+      launch_sweeps_blocked(b, v, sc, nscenes, ncolours, nsweeps, warm, cur_variant, stream);
+      if (counters_enabled) {
+        HIPCK(hipEventRecord(kt_ev[kt_used + 1], stream));
+        kt_used += 2;
+      }
+      return NPHYS_OK;
+    }
     if (counters_enabled) {
       const size_t need = size_t(kt_used) + size_t(2) * size_t(nsweeps) * size_t(ncolours);
       while (kt_ev.size() < need) {
@@ -687,6 +764,68 @@ struct World {
     return NPHYS_OK;
   }
 
+  SceneBlocks sc_view() {
+    SceneBlocks v;
+    v.keys = sc_keys.as<uint32_t>();
+    v.slot = sc_slot.as<uint32_t>();
+    v.keys_alt = sc_keys_alt.as<uint32_t>();
+    v.slot_alt = sc_slot_alt.as<uint32_t>();
+    v.start = sc_start.as<uint32_t>();
+    v.dyn_start = sc_dyn_start.as<int>();
+    v.dyn_list = sc_dyn_list.as<int>();
+    v.local = sc_local.as<int>();
+    return v;
+  }
+
+  bool blocked_eligible(int ncol) const {
+    return max_scene_dyn <= kLdsBodies && int64_t(nscenes) * (ncol + 1) <= (int64_t(1) << 24);
+  }
+
+  // Variant of this solve's sweeps: forced, or the autotuner's pick, or (while tuning) the next
+  // candidate to time. All variants give identical results.
+  void choose_variant() {
+    tune_pending = false;
+    if (!blocked_eligible(ncolours)) {
+      cur_variant = 0;
+      return;
+    }
+    if (variant_request >= 0) {
+      cur_variant = variant_request;
+      return;
+    }
+    if (tune_cands.empty()) {
+      tune_cands = {0, 64, 128, 256};
+      tune_time.assign(tune_cands.size(), 0.0);
+      tune_count.assign(tune_cands.size(), 0);
+    }
+    if (!tuned) {
+      size_t k = 0;
+      while (k < tune_cands.size() && tune_count[k] >= kTuneReps) ++k;
+      if (k < tune_cands.size()) {
+        cur_variant = tune_cands[k];
+        tune_pending = true;
+        return;
+      }
+      size_t best = 0;
+      for (size_t i = 1; i < tune_cands.size(); ++i)
+        if (tune_time[i] < tune_time[best]) best = i;
+      cur_variant = tune_cands[best];
+      tuned = true;
+    }
+  }
+
+  void tune_record() {
+    if (!tune_pending) return;
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, tune_ev[0], tune_ev[1]) != hipSuccess) return;
+    for (size_t i = 0; i < tune_cands.size(); ++i)
+      if (tune_cands[i] == cur_variant) {
+        tune_time[i] += double(ms);
+        ++tune_count[i];
+      }
+    tune_pending = false;
+  }
+
   // Warm start and SOR-Prox sweeps: the fixed count of the reference (sor_prox.rs:56-79), or, in
   // solve-to-residual mode, sweeps until max |phi| <= res_target, checked after the warm start
   // and every res_check_every sweeps, at most res_max_sweeps. Leaves the final residual in
@@ -702,7 +841,7 @@ struct World {
     int done = 0;
     for (;;) {
       if ((rc = residual(b, cv, M, C, v))) return rc;
-      if (res_inf <= res_target || done >= res_max_sweeps) break;
+      if ((res_norm == NPHYS_RESIDUAL_RMS ? res_rms : res_inf) <= res_target || done >= res_max_sweeps) break;
       const int k = std::min(res_check_every, res_max_sweeps - done);
       if ((rc = sweeps(b, cv, v, sblocks, false, k))) return rc;
       done += k;
@@ -746,6 +885,17 @@ struct World {
     HIPCK(hipEventRecord(ev[EV_ASM], stream));
     const int sblocks = sweep_blocks(M);
     kt_used = 0;
+    choose_variant();
+    if (cur_variant > 0) {
+      HIPCK(sc_keys.ensure(size_t(M) * 4));
+      HIPCK(sc_slot.ensure(size_t(M) * 4));
+      HIPCK(sc_keys_alt.ensure(size_t(M) * 4));
+      HIPCK(sc_slot_alt.ensure(size_t(M) * 4));
+      HIPCK(sc_start.ensure(size_t(nscenes) * size_t(ncolours + 1) * 4));
+      SceneBlocks sc = sc_view();
+      launch_scene_order(b, M, nscenes, ncolours, v, sc, stream);
+    }
+    if (tune_pending) HIPCK(hipEventRecord(tune_ev[0], stream));
     const bool learned = contact_model == NPHYS_CONTACT_MODEL_LEARNED_WARMSTART;
     if (learned || capture) {
       HIPCK(ml_X.ensure(size_t(C) * kFeat * sizeof(float)));
@@ -785,6 +935,7 @@ struct World {
       if (learned && (rc = predict(v, C))) return rc;
       if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, last_sweeps))) return rc;
     }
+    if (tune_pending) HIPCK(hipEventRecord(tune_ev[1], stream));
     if (capture) {
       HIPCK(ml_t.ensure(size_t(C) * kOut * sizeof(float)));
       launch_sample_targets(v, C, ml_ls.as<float>(), ml_t.as<float>(), stream);
@@ -869,6 +1020,8 @@ struct World {
     counters.nconstraints = int64_t(3) * solved_C;
     counters.ncolours = ncolours;
     counters.colour_rounds += colour_rounds;
+    tune_record();
+    counters.sweep_variant = cur_variant;
     for (int k = 0; k < kt_used; k += 2) {
       float ms = 0.0f;
       (void)hipEventElapsedTime(&ms, kt_ev[k], kt_ev[k + 1]);
@@ -1048,6 +1201,16 @@ int nphys_read_samples(nphys_world* w, float* features, float* targets, int64_t 
   return NPHYS_OK;
 }
 
+int nphys_set_sweep_variant(nphys_world* w, int32_t variant) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  if (variant != -1 && variant != 0 && variant != 64 && variant != 128 && variant != 256)
+    return set_err(NPHYS_ERR_INVALID, "variant must be -1 (autotune), 0, 64, 128 or 256");
+  w->w.variant_request = variant;
+  w->w.tune_cands.clear();
+  w->w.tuned = false;
+  return NPHYS_OK;
+}
+
 int nphys_set_warmstart_judge(nphys_world* w, int32_t on) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
   w->w.judge = on != 0;
@@ -1172,13 +1335,16 @@ int nphys_counters_enable(nphys_world* w, int32_t enabled) {
   return NPHYS_OK;
 }
 
-int nphys_set_residual_target(nphys_world* w, double r_inf_target, int32_t max_sweeps, int32_t check_every) {
+int nphys_set_residual_target(nphys_world* w, double target, int32_t max_sweeps, int32_t check_every,
+                              int32_t norm) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
-  if (!(r_inf_target >= 0.0) || max_sweeps < 0 || check_every < 1)
-    return set_err(NPHYS_ERR_INVALID, "residual target must be >= 0, max_sweeps >= 0, check_every >= 1");
-  w->w.res_target = r_inf_target;
+  if (!(target >= 0.0) || max_sweeps < 0 || check_every < 1 ||
+      (norm != NPHYS_RESIDUAL_MAX && norm != NPHYS_RESIDUAL_RMS))
+    return set_err(NPHYS_ERR_INVALID, "target >= 0, max_sweeps >= 0, check_every >= 1, norm MAX or RMS");
+  w->w.res_target = target;
   w->w.res_max_sweeps = max_sweeps;
   w->w.res_check_every = check_every;
+  w->w.res_norm = norm;
   return NPHYS_OK;
 }
 

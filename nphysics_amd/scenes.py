@@ -190,3 +190,44 @@ def small_pile(seed=7, n=512, lattice=(8, 16, 8), extent=8.0, y0=2.0, friction=0
 
 def num_static(scene):
     return int((scene["status"] == 1).sum())
+
+
+def batched(seed=4, worlds=4096, median=64, sigma=1.0, n_min=8, n_max=2048, cell=1.8, half_width=9.0,
+            ground_half=(10.0, 0.5, 10.0), r_range=(0.2, 0.6), he_range=(0.2, 0.5), friction=0.5):
+    """Independent worlds in one batch: world w has n_w = clip(round(exp(N(ln median, sigma^2))),
+    n_min, n_max) bodies (even index in the world a ball, odd a cuboid, the pile's size ranges,
+    uniform random orientation) on a jittered lattice of `cell` spacing over its own static
+    ground of half extents ground_half; every body of world w has scene index w. Static grounds
+    come first (one per world), then the bodies world by world."""
+    u = splitmix_uniform(seed, 2 * worlds).reshape(worlds, 2)
+    z = np.sqrt(-2.0 * np.log(np.maximum(u[:, 0], 1e-300))) * np.cos(2.0 * np.pi * u[:, 1])
+    nw = np.clip(np.rint(np.exp(np.log(median) + sigma * z)), n_min, n_max).astype(np.int64)
+    per_layer_side = int(2 * half_width / cell)
+    per_layer = per_layer_side * per_layer_side
+    total = int(nw.sum())
+    world = np.repeat(np.arange(worlds), nw)
+    first = np.concatenate([[0], np.cumsum(nw)[:-1]])
+    k = np.arange(total) - first[world]          # index inside the world
+    layer, rem = k // per_layer, k % per_layer
+    ix, iz = rem // per_layer_side, rem % per_layer_side
+    v = splitmix_uniform(seed + 1, 9 * total).reshape(total, 9)
+    is_ball = (k % 2) == 0
+    s = _empty(total)
+    s["shape"][:] = np.where(is_ball, 0, 1)
+    r = r_range[0] + (r_range[1] - r_range[0]) * v[:, 0]
+    he = he_range[0] + (he_range[1] - he_range[0]) * v[:, 0:3]
+    s["dims"][:] = np.where(is_ball[:, None], np.stack([r, np.zeros(total), np.zeros(total)], 1), he)
+    u1, u2, u3 = v[:, 3], 2 * np.pi * v[:, 4], 2 * np.pi * v[:, 5]
+    q = np.stack([np.sqrt(1 - u1) * np.sin(u2), np.sqrt(1 - u1) * np.cos(u2),
+                  np.sqrt(u1) * np.sin(u3), np.sqrt(u1) * np.cos(u3)], 1)
+    q[is_ball] = (0, 0, 0, 1)
+    s["rot"][:] = q
+    bound = np.where(is_ball, r, np.sqrt((he * he).sum(1)))
+    jit = np.maximum(0.0, cell / 2 - bound)[:, None] * (v[:, 6:9] * 2 - 1)
+    c = np.stack([-half_width + (ix + 0.5) * cell, cell * 0.5 + 0.1 + layer * cell, -half_width + (iz + 0.5) * cell], 1)
+    s["pos"][:] = c + jit
+    s["friction"][:] = friction
+    s["scene"] = world.astype(np.int32)
+    g = statics([ground_half] * worlds, [(0.0, -ground_half[1], 0.0)] * worlds, friction)
+    g["scene"] = np.arange(worlds, dtype=np.int32)
+    return concat(g, s)

@@ -26,7 +26,7 @@ EXPORTED = [
     "nphys_num_pairs", "nphys_read_pairs", "nphys_num_contacts", "nphys_read_contacts",
     "nphys_num_solved_contacts", "nphys_read_impulses", "nphys_residual", "nphys_read_counters",
     "nphys_reset_counters", "nphys_counters_enable", "nphys_set_residual_target", "nphys_read_colours",
-    "nphys_warmstart_load", "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
+    "nphys_set_sweep_variant", "nphys_warmstart_load", "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
     "nphys_mlp_create", "nphys_mlp_destroy", "nphys_mlp_num_params", "nphys_mlp_get_params",
     "nphys_mlp_set_params", "nphys_mlp_set_data", "nphys_mlp_train", "nphys_mlp_forward",
     "nphys_mlp_gradients", "nphys_last_error",
@@ -72,11 +72,12 @@ def load_library(path=LIB_PATH):
     L.nphys_read_counters.argtypes = [W, C.POINTER(abi.Counters)]
     L.nphys_reset_counters.argtypes = [W]
     L.nphys_counters_enable.argtypes = [W, C.c_int32]
-    L.nphys_set_residual_target.argtypes = [W, C.c_double, C.c_int32, C.c_int32]
+    L.nphys_set_residual_target.argtypes = [W, C.c_double, C.c_int32, C.c_int32, C.c_int32]
     L.nphys_warmstart_load.argtypes = [W, fp, C.c_int64]
     L.nphys_capture_samples.argtypes = [W, C.c_int32]
     L.nphys_read_samples.argtypes = [W, fp, fp, C.c_int64, C.POINTER(C.c_int64)]
     L.nphys_set_warmstart_judge.argtypes = [W, C.c_int32]
+    L.nphys_set_sweep_variant.argtypes = [W, C.c_int32]
     L.nphys_read_colours.argtypes = [W, ip, C.c_int64, ip]
     _lib = L
     return L
@@ -217,9 +218,11 @@ class MechanicalWorld:
     def enable_counters(self, on=True):
         _check(self._L.nphys_counters_enable(self._h, 1 if on else 0))
 
-    def set_residual_target(self, r_inf=1e-4, max_sweeps=500, check_every=1):
-        """Solve each step until max |phi| <= r_inf (0 restores the fixed iteration count)."""
-        _check(self._L.nphys_set_residual_target(self._h, r_inf, max_sweeps, check_every))
+    def set_residual_target(self, target=1e-4, max_sweeps=500, check_every=1, norm="max"):
+        """Solve each step until the residual (norm "max" or "rms") is <= target (0 restores the
+        fixed iteration count)."""
+        _check(self._L.nphys_set_residual_target(self._h, target, max_sweeps, check_every,
+                                                 0 if norm == "max" else 1))
 
     # -- learned warm start
     def load_warmstart(self, params):
@@ -239,6 +242,10 @@ class MechanicalWorld:
         if n.value:
             _check(self._L.nphys_read_samples(self._h, abi.fptr(X), abi.fptr(T), n.value, C.byref(n)))
         return X, T
+
+    def set_sweep_variant(self, variant=-1):
+        """-1 autotune (default), 0 per-colour launches, 64/128/256 scene-blocked workgroups."""
+        _check(self._L.nphys_set_sweep_variant(self._h, variant))
 
     def set_warmstart_judge(self, on=True):
         _check(self._L.nphys_set_warmstart_judge(self._h, 1 if on else 0))

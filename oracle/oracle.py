@@ -44,7 +44,7 @@ def lib():
         L.orc_set_params.argtypes = [vp, C.POINTER(abi.Params)]
         L.orc_set_order_mode.argtypes = [vp, C.c_int32]
         L.orc_set_brute_force_limit.argtypes = [vp, C.c_int32]
-        L.orc_set_residual_target.argtypes = [vp, C.c_double, C.c_int32, C.c_int32]
+        L.orc_set_residual_target.argtypes = [vp, C.c_double, C.c_int32, C.c_int32, C.c_int32]
         L.orc_add_bodies.argtypes = [vp, C.POINTER(abi.BodyDesc), C.POINTER(C.c_int32)]
         L.orc_num_bodies.argtypes = [vp, C.POINTER(C.c_int32)]
         fp = C.POINTER(C.c_float)
@@ -102,8 +102,8 @@ class OracleWorld:
         self.params = params
         _check(self._L.orc_set_params(self._h, C.byref(params)))
 
-    def set_residual_target(self, r_inf=1e-4, max_sweeps=500, check_every=1):
-        _check(self._L.orc_set_residual_target(self._h, r_inf, max_sweeps, check_every))
+    def set_residual_target(self, target=1e-4, max_sweeps=500, check_every=1, norm="max"):
+        _check(self._L.orc_set_residual_target(self._h, target, max_sweeps, check_every, 0 if norm == "max" else 1))
 
     def set_brute_force_limit(self, n):
         _check(self._L.orc_set_brute_force_limit(self._h, n))

@@ -79,13 +79,14 @@ int add_bodies_t(orc::World<T>& w, const nphys_body_desc* d, int32_t* first) {
       return fail(NPHYS_ERR_INVALID, "bad status");
     if (d->shape[i] != orc::SHAPE_BALL && d->shape[i] != orc::SHAPE_CUBOID)
       return fail(NPHYS_ERR_INVALID, "bad shape");
+    if (d->scene && (d->scene[i] < 0 || d->scene[i] >= NPHYS_MAX_SCENES)) return fail(NPHYS_ERR_INVALID, "bad scene");
     const orc::V3<T> zero{T(0), T(0), T(0)};
     const orc::V3<T> lv = d->linvel ? v3_of<T>(d->linvel + 3 * i) : zero;
     const orc::V3<T> av = d->angvel ? v3_of<T>(d->angvel + 3 * i) : zero;
     w.add_body(d->status[i], d->shape[i], v3_of<T>(d->dims + 3 * i),
                orc::Iso<T>{v3_of<T>(d->pos + 3 * i), q_of<T>(d->rot + 4 * i)}, lv, av,
                T(d->density[i]), T(d->margin[i]), T(d->linear_prediction[i]), T(d->friction[i]),
-               T(d->restitution[i]));
+               T(d->restitution[i]), d->scene ? int(d->scene[i]) : 0);
   }
   return NPHYS_OK;
 }
@@ -236,9 +237,11 @@ int orc_set_order_mode(void* h, int32_t mode) {
   return with_world(h, [&](auto& w) { w.order_mode = mode; return NPHYS_OK; });
 }
 
-int orc_set_residual_target(void* h, double target, int32_t max_sweeps, int32_t check_every) {
-  if (!(target >= 0.0) || max_sweeps < 0 || check_every < 1) return fail(NPHYS_ERR_INVALID, "bad residual target");
+int orc_set_residual_target(void* h, double target, int32_t max_sweeps, int32_t check_every, int32_t norm) {
+  if (!(target >= 0.0) || max_sweeps < 0 || check_every < 1 || (norm != 0 && norm != 1))
+    return fail(NPHYS_ERR_INVALID, "bad residual target");
   return with_world(h, [&](auto& w) {
+    w.res_norm = norm;
     w.res_target = target;
     w.res_max_sweeps = max_sweeps;
     w.res_check_every = check_every;

@@ -218,14 +218,32 @@ inline T eps_of() { return std::numeric_limits<T>::epsilon(); }
 // build and the GPU kernels round identically; larger angles use the C library.
 template <class T>
 inline void sinc_cos(T n, T& s, T& c) {
-  if (n < T(0.25)) {
-    const T n2 = n * n;
-    s = T(1) + n2 * (T(-1.0 / 6.0) + n2 * (T(1.0 / 120.0) + n2 * (T(-1.0 / 5040.0) + n2 * T(1.0 / 362880.0))));
-    c = T(1) + n2 * (T(-0.5) + n2 * (T(1.0 / 24.0) + n2 * (T(-1.0 / 720.0) + n2 * T(1.0 / 40320.0))));
-  } else {
-    s = std::sin(n) / n;
-    c = std::cos(n);
+  // sin(x)/x and cos(x) by their Maclaurin polynomials on [0, 0.25); a larger angle is halved
+  // (exactly) until below 0.25 and rebuilt with sin 2x = 2 sin x cos x, cos 2x = cos^2 x - sin^2 x,
+  // so every angle is plain arithmetic that the GPU reproduces bit for bit
+  T x = n;
+  int k = 0;
+  while (x >= T(0.25) && k < 64) {
+    x = x * T(0.5);
+    ++k;
   }
+  const T x2 = x * x;
+  const T sx = T(1) + x2 * (T(-1.0 / 6.0) + x2 * (T(1.0 / 120.0) + x2 * (T(-1.0 / 5040.0) + x2 * T(1.0 / 362880.0))));
+  T cx = T(1) + x2 * (T(-0.5) + x2 * (T(1.0 / 24.0) + x2 * (T(-1.0 / 720.0) + x2 * T(1.0 / 40320.0))));
+  if (k == 0) {
+    s = sx;
+    c = cx;
+    return;
+  }
+  T sn = x * sx;
+  for (int i = 0; i < k; ++i) {
+    const T s2 = T(2) * sn * cx;
+    const T c2 = cx * cx - sn * sn;
+    sn = s2;
+    cx = c2;
+  }
+  s = sn / n;
+  c = cx;
 }
 
 template <class T>

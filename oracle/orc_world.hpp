@@ -68,6 +68,7 @@ struct Params {  // IntegrationParameters (src/solver/integration_parameters.rs:
 template <class T>
 struct Body {
   int status, shape;
+  int scene;  // bodies of different scenes never interact
   ShapeDesc<T> sd;
   T density, margin, lin_pred, friction, restitution;
   Iso<T> pos, pos0;
@@ -278,9 +279,10 @@ class World {
   // (geometrical_world.rs:104-140): mass properties from the density, proxy creation with
   // loose AABB = aabb(pos) loosened by the query limit, then by the broad-phase margin 0.01.
   int add_body(int status, int shape, V3<T> dims, Iso<T> pos, V3<T> lv, V3<T> av, T density,
-               T margin, T lin_pred, T friction, T restitution) {
+               T margin, T lin_pred, T friction, T restitution, int scene = 0) {
     Body<T> b;
     std::memset(static_cast<void*>(&b), 0, sizeof(b));
+    b.scene = scene;
     b.status = status;
     b.shape = shape;
     b.sd.kind = shape;
@@ -420,8 +422,10 @@ class World {
     counters.broad_phase += now_s() - t0;
   }
 
+  // DefaultCollisionFilter::is_pair_valid (geometrical_world.rs:933-971) for single-collider
+  // bodies: at least one side has dofs; plus the scene rule (independent worlds in one batch).
   static bool pair_allowed(const Body<T>& a, const Body<T>& b) {
-    return a.status == STATUS_DYNAMIC || b.status == STATUS_DYNAMIC;
+    return (a.status == STATUS_DYNAMIC || b.status == STATUS_DYNAMIC) && a.scene == b.scene;
   }
 
   // canonical pair: first is dynamic; second is static or a higher index
@@ -448,38 +452,44 @@ class World {
     std::sort(pairs.begin(), pairs.end());
   }
 
-  // Hashed uniform grid for large scenes; returns exactly the brute-force set.
+  // Hashed uniform grid (keyed by scene and cell) for large scenes; returns exactly the
+  // brute-force set.
   void recompute_pairs_grid() {
     const int n = int(bodies.size());
-    std::vector<int> dyn, stat;
+    std::vector<int> dyn;
+    std::unordered_map<int, std::vector<int>> stat;
     V3<T> ext{T(0), T(0), T(0)};
     for (int i = 0; i < n; ++i) {
       if (bodies[i].status == STATUS_DYNAMIC) {
         dyn.push_back(i);
         ext = vmax(ext, bodies[i].loose.mx - bodies[i].loose.mn);
       } else {
-        stat.push_back(i);
+        stat[bodies[i].scene].push_back(i);
       }
     }
-    // statics against everything dynamic
-    for (int s : stat)
-      for (int d : dyn)
+    // statics against every dynamic body of their scene
+    for (int d : dyn) {
+      auto it = stat.find(bodies[d].scene);
+      if (it == stat.end()) continue;
+      for (int s : it->second)
         if (intersects(bodies[s].loose, bodies[d].loose)) push_pair(s, d);
+    }
     const double cs = double(std::max(ext.x, std::max(ext.y, ext.z))) * 1.0001 + 1e-6;
     auto cell = [&](V3<T> p, int64_t& cx, int64_t& cy, int64_t& cz) {
       cx = int64_t(std::floor(double(p.x) / cs));
       cy = int64_t(std::floor(double(p.y) / cs));
       cz = int64_t(std::floor(double(p.z) / cs));
     };
-    auto hkey = [](int64_t x, int64_t y, int64_t z) {
-      return (uint64_t(x + (1 << 20)) << 42) ^ (uint64_t(y + (1 << 20)) << 21) ^ uint64_t(z + (1 << 20));
+    auto hkey = [](int scene, int64_t x, int64_t y, int64_t z) {
+      return (uint64_t(x + (1 << 20)) << 42) ^ (uint64_t(y + (1 << 20)) << 21) ^ uint64_t(z + (1 << 20)) ^
+             (uint64_t(uint32_t(scene)) * 0x9E3779B97F4A7C15ull);
     };
     std::unordered_map<uint64_t, std::vector<int>> grid;
     grid.reserve(dyn.size() * 2);
     for (int d : dyn) {
       int64_t cx, cy, cz;
       cell(bodies[d].loose.mn, cx, cy, cz);
-      grid[hkey(cx, cy, cz)].push_back(d);
+      grid[hkey(bodies[d].scene, cx, cy, cz)].push_back(d);
     }
     for (int d : dyn) {
       int64_t cx, cy, cz;
@@ -487,7 +497,7 @@ class World {
       for (int64_t x = cx - 1; x <= cx + 1; ++x)
         for (int64_t y = cy - 1; y <= cy + 1; ++y)
           for (int64_t z = cz - 1; z <= cz + 1; ++z) {
-            auto it = grid.find(hkey(x, y, z));
+            auto it = grid.find(hkey(bodies[d].scene, x, y, z));
             if (it == grid.end()) continue;
             for (int o : it->second)
               if (o > d && intersects(bodies[d].loose, bodies[o].loose)) push_pair(d, o);
@@ -781,7 +791,7 @@ class World {
   // Fixed count (the reference), or, with res_target > 0, sweeps until the residual max |phi| is
   // at most res_target, checked after the warm start and every res_check_every sweeps.
   double res_target = 0.0;
-  int res_max_sweeps = 500, res_check_every = 1, last_sweeps = 0;
+  int res_max_sweeps = 500, res_check_every = 1, last_sweeps = 0, res_norm = 0;  // 0 max |phi|, 1 RMS
   long velocity_sweeps = 0;
   void sor_prox() {
     warmstart();
@@ -793,7 +803,7 @@ class World {
       int done = 0;
       for (;;) {
         compute_residual();
-        if (double(res_inf) <= res_target || done >= res_max_sweeps) break;
+        if (double(res_norm ? res_rms : res_inf) <= res_target || done >= res_max_sweeps) break;
         const int k = std::min(res_check_every, res_max_sweeps - done);
         for (int i = 0; i < k; ++i) sweep();
         done += k;

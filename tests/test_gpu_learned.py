@@ -19,14 +19,19 @@ def _world(nphys, scene, **params):
     return w
 
 
+def low_pile(seed):
+    """512 bodies in 8 layers from just above the floor of a 16 m box: in contact within 80 steps."""
+    return scenes.small_pile(seed=seed, n=512, lattice=(8, 8, 8), extent=8.0, y0=0.3)
+
+
 @pytest.mark.parametrize("check_every", [1, 4])
 def test_residual_target_bitexact_vs_checker(nphys, oracle_mod, check_every):
-    s = scenes.small_pile(n=512)
+    s = low_pile(5)
     g = _world(nphys, s)
     o = oracle_mod.OracleWorld(abi.default_params(), precision=32, order=oracle_mod.ORDER_COLOURED)
     o.add_bodies(s)
-    g.step(30)
-    o.step(30)
+    g.step(80)
+    o.step(80)
     g.set_residual_target(2e-3, 80, check_every)
     o.set_residual_target(2e-3, 80, check_every)
     g.reset_counters()
@@ -52,9 +57,9 @@ def test_learned_model_needs_a_network(nphys):
 
 
 def test_features_and_targets_identities(nphys):
-    s = scenes.small_pile(n=512)
+    s = low_pile(7)
     w = _world(nphys, s)
-    w.step(40)
+    w.step(80)
     w.capture_samples(True)
     w.step(1)
     X, T = w.samples()
@@ -85,14 +90,15 @@ def test_learned_warm_start_end_to_end(nphys):
     consistent with what it kept."""
     from nphysics_amd import learned
 
-    s = scenes.small_pile(seed=21, n=512)
+    s = low_pile(21)
     w = _world(nphys, s)
+    w.step(60)
     w.set_residual_target(1e-5, 200, 4)
     w.capture_samples(True)
     Xs, Ts = [], []
-    for k in range(60):
+    for k in range(40):
         w.step(1)
-        if k >= 20:
+        if k >= 5:
             X, T = w.samples()
             Xs.append(X)
             Ts.append(T)
@@ -104,8 +110,8 @@ def test_learned_warm_start_end_to_end(nphys):
     print("losses", losses[0], losses[-1])
     assert losses[-1] < losses[0]
 
-    e = _world(nphys, scenes.small_pile(seed=22, n=512))
-    e.step(40)
+    e = _world(nphys, low_pile(22))
+    e.step(80)
     e.load_warmstart(net.params)
     e.set_contact_model(abi.CONTACT_MODEL_LEARNED_WARMSTART)
     e.set_residual_target(1e-4, 120, 1)

@@ -206,3 +206,67 @@ def test_box_stack_full_config(nphys, oracle_mod):
     assert np.abs(norms - 1).max() < 1e-3
     c = g.counters()
     print(f"box stack: {c['ncontacts']} contacts after 500 steps, residual {g.residual()}")
+
+
+def test_batched_worlds_bitexact(nphys, oracle_mod):
+    """Independent worlds in one handle (scene indices): pairs never cross scenes, and the batch
+    steps bit for bit like the checker on the same batch."""
+    s = scenes.batched(worlds=24, seed=9)
+    g = _gpu_world(nphys, s, max_velocity_iterations=8)
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED, max_velocity_iterations=8)
+    g.step(60)
+    o.step(60)
+    gp = g.pairs()
+    np.testing.assert_array_equal(gp, o.pairs())
+    sc = s["scene"]
+    assert (sc[gp[:, 0]] == sc[gp[:, 1]]).all()
+    gs, os_ = g.get_state(), o.get_state()
+    for key in ("pos", "rot", "linvel", "angvel"):
+        np.testing.assert_array_equal(gs[key], _f32(os_[key]), err_msg=key)
+    print(f"batched: {s['status'].shape[0]} bodies in 24 scenes, {g.counters()['ncontacts']} contacts")
+
+
+@pytest.mark.parametrize("variant", [0, 64, 128, 256])
+def test_sweep_variants_identical(nphys, oracle_mod, variant):
+    """Per-colour launches and the scene-blocked kernel (LDS-resident velocity increments) give
+    the same bits, on a batch of scenes and on a single scene, with the fixed iteration count and
+    in solve-to-residual mode."""
+    for s, vi in ((scenes.batched(worlds=16, seed=5), 8), (scenes.boxes(nx=4, ny=6, nz=4), 10)):
+        g = _gpu_world(nphys, s, max_velocity_iterations=vi)
+        g.set_sweep_variant(variant)
+        o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED, max_velocity_iterations=vi)
+        g.step(25)
+        o.step(25)
+        g.set_residual_target(1e-3, 40, 3)
+        o.set_residual_target(1e-3, 40, 3)
+        g.step(5)
+        o.step(5)
+        assert g.counters()["sweep_variant"] == variant
+        gs, os_ = g.get_state(), o.get_state()
+        for key in ("pos", "rot", "linvel", "angvel"):
+            np.testing.assert_array_equal(gs[key], _f32(os_[key]), err_msg=f"variant {variant} {key}")
+
+
+def test_sweep_autotuner_picks_a_variant(nphys):
+    s = scenes.batched(worlds=64, seed=6)
+    g = _gpu_world(nphys, s, max_velocity_iterations=8)
+    g.step(20)
+    v = g.counters()["sweep_variant"]
+    print("autotuned sweep variant", v)
+    assert v in (0, 64, 128, 256)
+
+
+def test_fast_spin_exponential_bitexact(nphys, oracle_mod):
+    """Balls and cuboids spinning at up to 3000 rad/s (up to 25 rad of half angle per step): the
+    exponential map's halving / double-angle path is plain arithmetic on both sides."""
+    s = scenes.small_pile(seed=4, n=64, lattice=(4, 4, 4), extent=6.0, y0=5.0)
+    rng = np.random.default_rng(0)
+    s["angvel"] = rng.uniform(-3000, 3000, s["angvel"].shape).astype(np.float32)
+    s["angvel"][s["status"] != 0] = 0
+    g = _gpu_world(nphys, s, gravity=(0, 0, 0))
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED, gravity=(0, 0, 0))
+    g.step(5)
+    o.step(5)
+    gs, os_ = g.get_state(), o.get_state()
+    for key in ("pos", "rot", "linvel", "angvel"):
+        np.testing.assert_array_equal(gs[key], _f32(os_[key]), err_msg=key)

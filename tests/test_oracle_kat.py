@@ -358,3 +358,24 @@ def test_coloured_order_matches_reference_order_physically(oracle_mod):
 def test_invalid_params_rejected(oracle_mod):
     with pytest.raises(RuntimeError):
         oracle_mod.OracleWorld(abi.default_params(dt=-1.0))
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("omega", [0.3, 7.0, 60.0, 400.0, 3000.0])
+def test_spinning_ball_quaternion_closed_form(oracle_mod, precision, omega):
+    """A free ball (isotropic inertia: no gyroscopic term) spinning at omega about z turns by
+    omega dt per step: after k steps q = (0, 0, sin(k omega dt / 2), cos(k omega dt / 2)). The
+    angles cover both the polynomial range (half angle < 0.25 rad) and the halving / double-angle
+    range of the exponential map, up to 25 rad per step."""
+    p = abi.default_params(gravity=(0, 0, 0))
+    o = oracle_mod.OracleWorld(p, precision=precision)
+    o.add_bodies(single(0, 0, (0.5, 0, 0), (0, 0, 0), angvel=(0, 0, omega)))
+    steps = 3
+    o.step(steps)
+    q = o.get_state()["rot"][0]
+    half = steps * omega * p.dt / 2
+    expect = np.array([0, 0, math.sin(half), math.cos(half)])
+    sign = 1.0 if np.dot(q, expect) >= 0 else -1.0
+    tol = 2e-5 if precision == 32 else 1e-12
+    np.testing.assert_allclose(sign * q, expect, atol=tol)
+    assert o.get_state()["angvel"][0, 2] == pytest.approx(omega, rel=1e-6)
