@@ -89,6 +89,7 @@ def bench_batched(args):
     scene = scenes.batched()
     w = world.MechanicalWorld(integration_parameters=params)
     w.add_bodies(scene)
+    w.set_sweep_variant(args.sweep_variant)
     settle_steps = min(args.settle, 200)
     t0 = time.perf_counter()
     settle(w, settle_steps)
@@ -117,7 +118,7 @@ def bench_batched(args):
                                "balls/cuboids, own ground each; 8 velocity + 3 position iterations, dt 1/60",
                    "bodies": int(scene["status"].shape[0]), "dynamic_bodies": n_dyn,
                    "settle_steps": settle_steps, "contacts": c["ncontacts"], "manifolds": c["nmanifolds"],
-                   "colours": c["ncolours"]},
+                   "colours": c["ncolours"], "sweep_variant": c["sweep_variant"]},
         "contact_iters_per_s": c["ncontacts"] * 8 * args.steps / elapsed,
         "world_steps_per_s": 4096 * args.steps / elapsed,
         "stages_ms_per_step": {k: 1e3 * c[k] / args.steps for k in
@@ -141,6 +142,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 512 MiB device copy 5 times (the reference kernel of the "
                          "FETCH_SIZE / WRITE_SIZE counter calibration) and exit")
+    ap.add_argument("--sweep-variant", type=int, default=-1,
+                    help="-1 autotune (default), 0 per-colour launches, 64/128/256 scene-blocked")
     ap.add_argument("--workload", choices=("pile", "batched"), default="pile",
                     help="pile: the dense pile (default); batched: 4,096 independent worlds in one handle")
     args = ap.parse_args()

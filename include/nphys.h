@@ -232,9 +232,10 @@ int nphys_read_colours(nphys_world* w, int32_t* colours, int64_t cap, int32_t* n
  *
  * No reference counterpart: the reference warm-starts every contact from the impulse cache
  * (signorini_coulomb_pyramid_model.rs:104-108). A network nphys_mlp (24 -> 128 -> 128 -> 3, GELU,
- * BF16 weights and activations, FP32 accumulation) maps a contact's 24 features to its impulses
- * (normal, tangent 1, tangent 2) in units of r_n |g| dt, r_n the effective mass of the contact's
- * normal row. Parameters are FP32, NPHYS_MLP_NUM_PARAMS of them, laid out as
+ * BF16 weights and activations, FP32 accumulation) maps a contact's 24 features to the correction
+ * of its cached impulses (normal, tangent 1, tangent 2; the cache is 0 for a new contact) in units
+ * of r_n |g| dt, r_n the effective mass of the contact's normal row. Parameters are FP32,
+ * NPHYS_MLP_NUM_PARAMS of them, laid out as
  *   W1[24][128] b1[128] W2[128][128] b2[128] W3[128][3] b3[3] x_mean[24] x_scale[24] y_scale[3]
  * with inputs normalised as (x - x_mean) / x_scale and outputs multiplied by y_scale. */
 #define NPHYS_MLP_NUM_FEATURES 24
@@ -265,12 +266,13 @@ int nphys_mlp_gradients(nphys_mlp* m, const float* features, const float* target
 
 /* Loads a network (NPHYS_MLP_NUM_PARAMS floats, the layout above) into the world; afterwards
  * nphys_contact_model_set(NPHYS_CONTACT_MODEL_LEARNED_WARMSTART) warm-starts every contact with
- * the network's impulses (projected on lambda_n >= 0, |lambda_t| <= mu lambda_n) instead of the
- * cache. Selecting that model without a network is NPHYS_ERR_STATE. */
+ * its cached impulses plus the network's correction (projected on lambda_n >= 0,
+ * |lambda_t| <= mu lambda_n). Selecting that model without a network is NPHYS_ERR_STATE. */
 int nphys_warmstart_load(nphys_world* w, const float* params, int64_t n);
 /* With on, every step records its contacts' features (before the solve) and impulses (after it,
  * in units of r_n |g| dt) as training samples; nphys_read_samples returns the last step's
- * (features [n][24], targets [n][3]; with both NULL only *n is set). */
+ * (features [n][24], targets [n][3]; with both NULL only *n is set). The network's training target
+ * is targets minus features 16..18 (the cached impulses in the same unit). */
 int nphys_capture_samples(nphys_world* w, int32_t on);
 int nphys_read_samples(nphys_world* w, float* features, float* targets, int64_t cap, int64_t* n);
 /* With the learned model and a residual target set: every step solves twice from the same

@@ -213,6 +213,7 @@ __global__ void __launch_bounds__(256) k_init_inodes(int n, uint32_t* inode) {
   q[1] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+template <bool kScenes>
 __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* __restrict__ vals, int n,
                                                  const int* __restrict__ child,
                                                  const float4* __restrict__ node_mn,
@@ -227,7 +228,7 @@ __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* _
     return;
   }
   const aabb Q{xyz(b.loose_mn[q]), xyz(b.loose_mx[q])};
-  const int qs = b.scene[q];
+  const int qs = kScenes ? b.scene[q] : 0;
   const long base = long(q) * cap;
   uint32_t cnt = 0;
   int stack[kStack];
@@ -239,7 +240,7 @@ __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* _
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
       const int c = k == 0 ? c0 : c1;
-      if (c < n - 1) {
+      if (kScenes && c < n - 1) {
         const int2 sr = node_scene[c];
         if (qs < sr.x || qs > sr.y) continue;
       }
@@ -247,7 +248,7 @@ __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* _
       if (!intersects(Q, B)) continue;
       if (c >= n - 1) {
         const int o = int(vals[c - (n - 1)]);
-        if (o != q && b.scene[o] == qs && (b.status[o] != STATUS_DYNAMIC || o > q)) {
+        if (o != q && (!kScenes || b.scene[o] == qs) && (b.status[o] != STATUS_DYNAMIC || o > q)) {
           if (cnt < uint32_t(cap)) cand[base + cnt] = o;
           ++cnt;
         }
@@ -328,8 +329,12 @@ void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_
     fill_u32(bp.cand_count, 0u, size_t(n), s);
     return;
   }
-  hipLaunchKernelGGL(k_traverse, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
-                     bp.node_mx, bp.node_scene, bp.cand, bp.cand_count, cap, bp.stats);
+  if (bp.scene_bits)
+    hipLaunchKernelGGL(k_traverse<true>, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
+                       bp.node_mx, bp.node_scene, bp.cand, bp.cand_count, cap, bp.stats);
+  else
+    hipLaunchKernelGGL(k_traverse<false>, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
+                       bp.node_mx, bp.node_scene, bp.cand, bp.cand_count, cap, bp.stats);
 }
 
 void launch_compact_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s) {

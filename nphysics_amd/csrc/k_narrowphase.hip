@@ -17,10 +17,32 @@ namespace nph {
 
 namespace {
 
-__global__ void __launch_bounds__(256) k_np_pairs(DevBodies b, const int2* __restrict__ pairs,
-                                                 int npairs, NarrowPhaseBufs np) {
+// Lanes are binned by pair kind so a wavefront runs one contact generator: cuboid-cuboid pairs
+// (SAT + clipping, the expensive one) first, all other pairs after. Only the lane -> pair mapping
+// changes; every pair still writes its own slots, so the output is the unbinned one.
+__global__ void __launch_bounds__(256) k_np_flag(DevBodies b, const int2* __restrict__ pairs, int npairs,
+                                                uint32_t* flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npairs) return;
+  const int2 pr = pairs[i];
+  flag[i] = (b.shape[pr.x] == SHAPE_CUBOID && b.shape[pr.y] == SHAPE_CUBOID) ? 1u : 0u;
+}
+
+__global__ void __launch_bounds__(256) k_np_perm(int npairs, const uint32_t* __restrict__ flag,
+                                                const uint32_t* __restrict__ before, uint32_t* perm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= npairs) return;
+  const uint32_t nbb = before[npairs - 1] + flag[npairs - 1];
+  const uint32_t b = before[i];
+  perm[flag[i] ? b : nbb + (uint32_t(i) - b)] = uint32_t(i);
+}
+
+__global__ void __launch_bounds__(256) k_np_pairs(DevBodies b, const int2* __restrict__ pairs,
+                                                 int npairs, const uint32_t* __restrict__ perm,
+                                                 NarrowPhaseBufs np) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npairs) return;
+  const int i = int(perm[j]);
   const int2 pr = pairs[i];
   const int a = pr.x, c = pr.y;
   const float prediction = b.qlim[a] + b.qlim[c];
@@ -80,7 +102,11 @@ void launch_narrow_phase(const DevBodies& b, const int2* pairs, int npairs, Narr
     return;
   }
   const unsigned g = unsigned((npairs + 255) / 256);
-  hipLaunchKernelGGL(k_np_pairs, dim3(g), dim3(256), 0, s, b, pairs, npairs, np);
+  // binning by kind: flags in np.has, their exclusive scan in np.moff (both rewritten below)
+  hipLaunchKernelGGL(k_np_flag, dim3(g), dim3(256), 0, s, b, pairs, npairs, np.has);
+  scan_exclusive_u32(np.has, np.moff, npairs, np.tmp, s);
+  hipLaunchKernelGGL(k_np_perm, dim3(g), dim3(256), 0, s, npairs, np.has, np.moff, np.perm);
+  hipLaunchKernelGGL(k_np_pairs, dim3(g), dim3(256), 0, s, b, pairs, npairs, np.perm, np);
   scan_exclusive_u32(np.count, np.coff, npairs, np.tmp, s);
   scan_exclusive_u32(np.has, np.moff, npairs, np.tmp, s);
   hipLaunchKernelGGL(k_np_compact, dim3(g), dim3(256), 0, s, pairs, npairs, np, out);

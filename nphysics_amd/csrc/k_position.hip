@@ -103,10 +103,8 @@ __global__ void __launch_bounds__(64) k_position(DevBodies b, DevContacts c, Sim
 
 }  // namespace
 
-int position_blocks(int M) {
-  const int need = (M + 63) / 64;
-  return need < 1 ? 1 : (need > 2048 ? 2048 : need);
-}
+// Sized like the sweep grid: from the largest colour class.
+int position_blocks(int max_class) { return sweep_blocks(max_class); }
 
 void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k, int blocks,
                             SolverBufs& sb, hipStream_t s) {

@@ -669,11 +669,17 @@ __global__ void __launch_bounds__(256) k_scene_starts(const uint32_t* __restrict
 }
 
 // Colours left empty (inherited colours can leave gaps) start where the next one does.
-__global__ void k_fix_colour_starts(int ncolours, int M, uint32_t* col_start) {
+// Also reports the largest colour class, which sizes the sweep grids.
+__global__ void k_fix_colour_starts(int ncolours, int M, uint32_t* col_start, uint32_t* max_class) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   col_start[ncolours] = uint32_t(M);
-  for (int k = ncolours - 1; k >= 0; --k)
+  uint32_t mx = 0;
+  for (int k = ncolours - 1; k >= 0; --k) {
     if (col_start[k] == 0xffffffffu) col_start[k] = col_start[k + 1];
+    const uint32_t sz = col_start[k + 1] - col_start[k];
+    mx = sz > mx ? sz : mx;
+  }
+  *max_class = mx;
 }
 
 // Natural-map residual per row; per block (max |phi|, sum phi^2) in double, reduced on the host
@@ -790,20 +796,22 @@ __global__ void __launch_bounds__(256) k_features(DevBodies b, DevContacts cur, 
   }
 }
 
-// The network's prediction as the warm start: impulses times ls, projected on the admissible
-// set (normal >= 0, each tangent within +-mu normal, the pyramid of the friction rows).
+// The network predicts the correction of the cache warm start in units of ls: the warm start
+// becomes cache + y ls (cache 0 for a new contact), projected on the admissible set (normal >= 0,
+// each tangent within +-mu normal, the pyramid of the friction rows).
 __global__ void __launch_bounds__(256) k_apply_prediction(SolverBufs sb, int C, const float* __restrict__ y,
                                                           const float* __restrict__ ls) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= C) return;
   const float s = ls[r];
   const float mu = sb.rec[long(r) * kRecFields].w;
-  const float ln = fmaxf(y[3 * r] * s, 0.0f);
-  const float lim = mu * ln;
-  const float t1 = fminf(fmaxf(y[3 * r + 1] * s, -lim), lim);
-  const float t2 = fminf(fmaxf(y[3 * r + 2] * s, -lim), lim);
   float4* L = sb.rec + long(r) * kRecFields + 7;
-  *L = make_float4(ln, t1, t2, L->w);
+  const float4 c = *L;
+  const float ln = fmaxf(c.x + y[3 * r] * s, 0.0f);
+  const float lim = mu * ln;
+  const float t1 = fminf(fmaxf(c.y + y[3 * r + 1] * s, -lim), lim);
+  const float t2 = fminf(fmaxf(c.z + y[3 * r + 2] * s, -lim), lim);
+  *L = make_float4(ln, t1, t2, c.w);
 }
 
 // Training targets of the last solve: the impulses in units of ls.
@@ -874,15 +882,18 @@ void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContac
   hipLaunchKernelGGL(k_assemble, dim3(nb(M)), dim3(256), 0, s, b, cur, prev, prevM, M, p, sb);
 }
 
-void launch_fix_colour_starts(int ncolours, int M, SolverBufs& sb, hipStream_t s) {
-  hipLaunchKernelGGL(k_fix_colour_starts, dim3(1), dim3(1), 0, s, ncolours, M, sb.col_start);
+void launch_fix_colour_starts(int ncolours, int M, SolverBufs& sb, uint32_t* max_class, hipStream_t s) {
+  hipLaunchKernelGGL(k_fix_colour_starts, dim3(1), dim3(1), 0, s, ncolours, M, sb.col_start, max_class);
 }
 
-// One lane per manifold: enough wavefronts for the largest colour a scene of M manifolds can
-// have, capped at 4096 (16 per CU); larger colours are walked by the grid-stride loop.
-int sweep_blocks(int M) {
-  const int need = (M + 63) / 64;
-  return need < 1 ? 1 : (need > 4096 ? 4096 : need);
+// One lane per manifold: one 64-lane workgroup per 64 manifolds of the largest colour class,
+// rounded up to a multiple of 64 workgroups (so the launch shape, and with it the captured
+// sweep graph, changes only when the largest class crosses a multiple of 4,096) and capped at
+// 8,192; larger classes are walked by the grid-stride loop.
+int sweep_blocks(int max_class) {
+  int need = (max_class + 63) / 64;
+  need = ((need + 63) / 64) * 64;
+  return need < 64 ? 64 : (need > 8192 ? 8192 : need);
 }
 
 void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,

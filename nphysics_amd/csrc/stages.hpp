@@ -55,6 +55,7 @@ struct NarrowPhaseBufs {
   uint32_t* coff;         // [P] exclusive scan of count
   uint32_t* has;          // [P] count > 0
   uint32_t* moff;         // [P] exclusive scan of has
+  uint32_t* perm;         // [P] pairs binned by kind (lane -> pair)
   uint32_t* totals;       // [2] contacts, manifolds
   uint32_t* tmp;
 };
@@ -94,14 +95,15 @@ void launch_colour_round(const DevBodies& b, const DevContacts& c, int M, int ro
 void launch_colour_keys(int M, SolverBufs& sb, uint32_t* ncol, hipStream_t s);
 void launch_colour_order(const DevBodies& b, const DevContacts& c, int M, int ncolours, SolverBufs& sb,
                          hipStream_t s);
-void launch_fix_colour_starts(int ncolours, int M, SolverBufs& sb, hipStream_t s);
+void launch_fix_colour_starts(int ncolours, int M, SolverBufs& sb, uint32_t* max_class, hipStream_t s);
 int residual_blocks(int M);
 void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContacts& prev,
                      int prevM, int M, const SimParams& p, SolverBufs& sb, hipStream_t s);
 // Sweep kernels of the colour visited k-th; their slot ranges are read on the device, so the
-// launches are independent of per-step sizes (grid = blocks, grid-stride over the colour).
-int sweep_blocks(int M);
-int position_blocks(int M);
+// launches are independent of per-step sizes (grid = blocks from the largest colour class,
+// grid-stride over the colour).
+int sweep_blocks(int max_class);
+int position_blocks(int max_class);
 void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, int blocks,
                              SolverBufs& sb, hipStream_t s);
 void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,

@@ -100,7 +100,7 @@ struct World {
 
   // narrow phase
   DBuf np_sw1, np_sw2, np_sn, np_sfeat, np_skind, np_slp1, np_slp2, np_sln, np_count, np_coff,
-      np_has, np_moff, np_tmp;
+      np_has, np_moff, np_perm, np_tmp;
   ContactBufs cb[2];
   int cur = 0;      // index of the manifolds produced by the last collision detection
   bool have_solved = false;
@@ -130,6 +130,8 @@ struct World {
   double res_target = 0.0;
   int res_max_sweeps = 500, res_check_every = 1, res_norm = NPHYS_RESIDUAL_MAX;
   int last_sweeps = 0;
+  int max_class = 0;             // largest colour class of the current solve
+  hipEvent_t ev_class = nullptr;  // after its readback
   std::vector<double> red_host;
   // learned warm start: network (FP32 parameters + BF16 images), per-record features, impulse
   // units, predictions and recorded targets; the judge's saved plain solve
@@ -170,6 +172,7 @@ struct World {
       if (e) (void)hipEventDestroy(e);
     for (auto& e : tune_ev)
       if (e) (void)hipEventDestroy(e);
+    if (ev_class) (void)hipEventDestroy(ev_class);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -198,6 +201,7 @@ struct World {
     HIPCK(sv_visit.ensure((kMaxColours + 2) * sizeof(int)));
     for (auto& e : ev) HIPCK(hipEventCreate(&e));
     for (auto& e : tune_ev) HIPCK(hipEventCreate(&e));
+    HIPCK(hipEventCreateWithFlags(&ev_class, hipEventDisableTiming));
     set_params(p);
     HIPCK(bp_small.ensure(64 * sizeof(uint32_t)));
     // Profilers that cannot follow graph launches can turn them off; kernels are the same.
@@ -520,7 +524,7 @@ struct World {
       HIPCK(b->ensure(4 * P * sizeof(float4)));
     HIPCK(np_sfeat.ensure(4 * P * 4));
     HIPCK(np_skind.ensure(4 * P * 4));
-    for (DBuf* b : {&np_count, &np_coff, &np_has, &np_moff}) HIPCK(b->ensure(P * 4));
+    for (DBuf* b : {&np_count, &np_coff, &np_has, &np_moff, &np_perm}) HIPCK(b->ensure(P * 4));
     HIPCK(np_tmp.ensure(scan_temp_words(int(P)) * 4 + 64));
     HIPCK(out.ensure(P, 4 * P));
     NarrowPhaseBufs v;
@@ -535,6 +539,7 @@ struct World {
     v.count = np_count.as<uint32_t>();
     v.coff = np_coff.as<uint32_t>();
     v.has = np_has.as<uint32_t>();
+    v.perm = np_perm.as<uint32_t>();
     v.moff = np_moff.as<uint32_t>();
     v.totals = bp_small.as<uint32_t>() + 24;
     v.tmp = np_tmp.as<uint32_t>();
@@ -691,7 +696,10 @@ struct World {
     ncolours = int(h_small[0]);
     if (ncolours > kMaxColours) return set_err(NPHYS_ERR_STATE, "too many colours");
     launch_colour_order(b, c, M, ncolours, v, stream);
-    launch_fix_colour_starts(ncolours, M, v, stream);
+    uint32_t* max_class_dev = bp_small.as<uint32_t>() + 44;
+    launch_fix_colour_starts(ncolours, M, v, max_class_dev, stream);
+    HIPCK(hipMemcpyAsync(h_small + 44, max_class_dev, 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipEventRecord(ev_class, stream));
     colour_visit_order(uint64_t(solve_count), ncolours, colour_visit);
     for (int k = 0; k < ncolours; ++k) h_visit[k] = colour_visit[k];
     HIPCK(hipMemcpyAsync(sv_visit.p, h_visit, size_t(ncolours) * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -883,7 +891,10 @@ struct World {
     if ((rc = colour(b, cv, M, v))) return rc;
     launch_assemble(b, cv, pc.view(), prevM, M, sp, v, stream);
     HIPCK(hipEventRecord(ev[EV_ASM], stream));
-    const int sblocks = sweep_blocks(M);
+    // the largest colour class sizes the sweep grids (read back while the assembly runs)
+    HIPCK(hipEventSynchronize(ev_class));
+    max_class = int(h_small[44]);
+    const int sblocks = sweep_blocks(max_class);
     kt_used = 0;
     choose_variant();
     if (cur_variant > 0) {
@@ -960,7 +971,7 @@ struct World {
     DevBodies b = bodies();
     SolverBufs v = sv_view();
     DevContacts cv = cc.view();
-    const int pblocks = position_blocks(cc.M);
+    const int pblocks = position_blocks(max_class);
     auto body = [&]() {
       for (int it = 0; it < prm.max_position_iterations; ++it)
         for (int k = 0; k < ncolours; ++k) launch_position_colour(b, cv, sp, k, pblocks, v, stream);

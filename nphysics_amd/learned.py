@@ -1,7 +1,8 @@
-"""The learned warm start: a network that predicts each contact's initial impulses.
+"""The learned warm start: a network that corrects each contact's cached initial impulses.
 
 WarmStartNet wraps the C ABI's nphys_mlp (include/nphys.h): 24 contact features -> 128 -> 128 ->
-3 impulses (normal, tangent 1, tangent 2) in units of r_n |g| dt, GELU, BF16 weights and
+3 impulse corrections (normal, tangent 1, tangent 2) in units of r_n |g| dt added to the cached
+impulses (0 for a new contact), GELU, BF16 weights and
 activations with FP32 accumulation on MFMA. It is trained on the GPU by the library's own
 forward/backward kernels and Adam, on samples a MechanicalWorld records (its features before the
 solve and its impulses after a solve to a tight residual), and loaded into a world with
@@ -112,3 +113,57 @@ class WarmStartNet:
         _check(self._L, self._L.nphys_mlp_gradients(self._h, abi.fptr(x), abi.fptr(t), x.shape[0], abi.fptr(g),
                                                     C.byref(loss)))
         return g, loss.value
+
+
+CACHE = slice(16, 19)  # features holding the cached impulses
+
+
+def correction_targets(features, impulses):
+    """Training targets of the network: solved impulses minus the cached ones (same unit)."""
+    return impulses - features[:, CACHE]
+
+
+def collect_samples(world, steps, target=1e-6, max_sweeps=500, check_every=4, norm="max"):
+    """Steps `world` with its solve run to a tight residual target and returns the recorded
+    (features, impulses) of every step: the impulses the plain solver converged to (or reached at
+    max_sweeps). The world's residual mode is restored to the fixed count after."""
+    world.set_residual_target(target, max_sweeps, check_every, norm)
+    world.capture_samples(True)
+    Xs, Ts = [], []
+    for _ in range(steps):
+        world.step(1)
+        X, T = world.samples()
+        Xs.append(X)
+        Ts.append(T)
+    world.capture_samples(False)
+    world.set_residual_target(0.0)
+    return np.concatenate(Xs), np.concatenate(Ts)
+
+
+def judge(world, params, steps, target=1e-4, max_sweeps=200, check_every=2, norm="rms"):
+    """Runs `steps` steps of `world` with the learned warm start under the judge (both solves from
+    the same assembled system each step, the learned one kept only if it reached the target in
+    fewer sweeps) and returns the judge's counters and the mean sweeps of each start."""
+    world.load_warmstart(params)
+    world.set_contact_model(abi.CONTACT_MODEL_LEARNED_WARMSTART)
+    world.set_residual_target(target, max_sweeps, check_every, norm)
+    world.set_warmstart_judge(True)
+    world.reset_counters()
+    world.step(steps)
+    c = world.counters()
+    world.set_warmstart_judge(False)
+    world.set_contact_model(abi.CONTACT_MODEL_SIGNORINI_COULOMB_PYRAMID)
+    world.set_residual_target(0.0)
+    plain = c["judge_plain_sweeps"] / steps
+    learned = c["judge_learned_sweeps"] / steps
+    return {
+        "steps": steps,
+        "target": target,
+        "norm": norm,
+        "max_sweeps": max_sweeps,
+        "check_every": check_every,
+        "plain_sweeps_per_step": plain,
+        "learned_sweeps_per_step": learned,
+        "sweep_reduction": (plain - learned) / plain if plain > 0 else 0.0,
+        "learned_kept_steps": c["judge_kept"],
+    }

@@ -95,10 +95,14 @@ def default_params(**kw):
 class MechanicalWorld:
     """One world on the current HIP device (MechanicalWorld + GeometricalWorld + sets)."""
 
-    def __init__(self, gravity=(0.0, -9.81, 0.0), integration_parameters=None):
+    def __init__(self, gravity=None, integration_parameters=None):
+        """MechanicalWorld::new(gravity) (mechanical_world.rs:76-102). The gravity argument, when
+        given, replaces the one carried by integration_parameters (default (0, -9.81, 0))."""
         self._L = load_library()
-        p = integration_parameters if integration_parameters is not None else abi.default_params()
-        p.gravity[0], p.gravity[1], p.gravity[2] = gravity
+        p = abi.Params()
+        C.pointer(p)[0] = integration_parameters if integration_parameters is not None else abi.default_params()
+        if gravity is not None:
+            p.gravity[0], p.gravity[1], p.gravity[2] = gravity
         self._h = C.c_void_p()
         _check(self._L.nphys_world_create(C.byref(p), C.byref(self._h)))
 
