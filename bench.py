@@ -82,6 +82,77 @@ def cpu_baseline(scene, snapshot, params, steps):
     }
 
 
+def train_warmstart():
+    """The learned warm start's network, trained on the GPU on samples from scenes disjoint from
+    the benchmark's (seeds 1000-1002: two 8,192-body piles and a 10^3 box stack), each sample a
+    contact's features and the impulses its solve reached at RMS residual 1e-5 (cap 1,000 sweeps)."""
+    from nphysics_amd import learned
+
+    t0 = time.perf_counter()
+    Xs, Ts = [], []
+    jobs = [(scenes.small_pile(seed=1000, n=8192, lattice=(32, 8, 32), extent=24.0, y0=0.3), 16, 200),
+            (scenes.small_pile(seed=1001, n=8192, lattice=(32, 8, 32), extent=24.0, y0=0.3), 16, 200),
+            (scenes.boxes(seed=1002, nx=10, ny=10, nz=10), 10, 100)]
+    for scene, iters, settle_steps in jobs:
+        w = world.MechanicalWorld(integration_parameters=abi.default_params(max_velocity_iterations=iters))
+        w.add_bodies(scene)
+        w.step(settle_steps)
+        X, T = learned.collect_samples(w, 25, target=1e-5, max_sweeps=1000, check_every=8, norm="rms")
+        Xs.append(X)
+        Ts.append(T)
+        w.close()
+    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    t1 = time.perf_counter()
+    net = learned.WarmStartNet(seed=1)
+    net.set_data(X, learned.correction_targets(X, T))
+    losses = net.train(epochs=15, batch=4096, lr=1e-3)
+    t2 = time.perf_counter()
+    params = net.params
+    net.close()
+    log(f"warm-start network: {X.shape[0]} samples in {t1 - t0:.1f} s, trained in {t2 - t1:.1f} s, "
+        f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
+    return params, {"train_samples": int(X.shape[0]), "data_s": t1 - t0, "train_s": t2 - t1,
+                    "epochs": 15, "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
+
+
+def residual_target_section(scene, snapshot, params, target, steps, max_sweeps):
+    """Steps/s and contact-iterations/s of the pile solved to RMS residual `target` per step, from
+    the plain cache warm start and from the learned one (each from the same settled snapshot, so
+    the trajectories differ only through the warm start), and the judge's sweep counts on
+    identical assembled systems."""
+    from nphysics_amd import learned
+
+    net_params, train = train_warmstart()
+    out = {"norm": "rms", "target": target, "max_sweeps": max_sweeps, "steps": steps, "training": train}
+    for mode in ("plain", "learned"):
+        w = world.MechanicalWorld(integration_parameters=params)
+        w.add_bodies(scene)
+        w.set_state(**{k: snapshot[k] for k in ("pos", "rot", "linvel", "angvel")})
+        w.step(3)  # rebuild the impulse cache after the state upload
+        if mode == "learned":
+            w.load_warmstart(net_params)
+            w.set_contact_model(abi.CONTACT_MODEL_LEARNED_WARMSTART)
+        w.set_residual_target(target, max_sweeps, 1, "rms")
+        w.step(1)
+        w.reset_counters()
+        t0 = time.perf_counter()
+        w.step(steps)
+        dt = time.perf_counter() - t0
+        c = w.counters()
+        out[mode] = {"steps_per_s": steps / dt, "sweeps_per_step": c["velocity_sweeps"] / steps,
+                     "contact_iters_per_s": c["ncontacts"] * c["velocity_sweeps"] / dt,
+                     "residual_rms": w.residual()[1]}
+        if mode == "learned":
+            w.set_residual_target(0.0)
+            out["judge"] = learned.judge(w, net_params, steps, target=target, max_sweeps=max_sweeps,
+                                         check_every=1, norm="rms")
+        w.close()
+    p, l = out["plain"]["sweeps_per_step"], out["learned"]["sweeps_per_step"]
+    out["sweep_reduction"] = (p - l) / p if p > 0 else 0.0
+    out["speedup"] = out["learned"]["steps_per_s"] / out["plain"]["steps_per_s"]
+    return out
+
+
 def bench_batched(args):
     """4,096 independent worlds (lognormal sizes, median 64 bodies, each with its own ground)
     stepped as one handle; 8 velocity / 3 position iterations. Prints one JSON line."""
@@ -137,6 +208,10 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the residual-target / learned runs")
+    ap.add_argument("--target-rms", type=float, default=1e-4)
+    ap.add_argument("--target-steps", type=int, default=4)
+    ap.add_argument("--target-max-sweeps", type=int, default=1000)
     ap.add_argument("--save-state", default=None, help="write the settled pile's state (.npz)")
     ap.add_argument("--load-state", default=None, help="start from a state written by --save-state")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -265,6 +340,11 @@ def main():
             "timing_region_steps": args.timing_steps,
         },
     }
+    if not args.no_learned:
+        log("residual-target runs, plain vs learned warm start ...")
+        w.close()
+        out["residual_target"] = residual_target_section(scene, snapshot, params, args.target_rms,
+                                                         args.target_steps, args.target_max_sweeps)
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(scene, snapshot, params, args.cpu_steps)

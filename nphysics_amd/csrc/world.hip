@@ -147,6 +147,7 @@ struct World {
   std::vector<double> tune_time;
   std::vector<int> tune_count;
   bool tuned = false, tune_pending = false;
+  int tune_M = 0;  // manifold count when the current pick was timed
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
   bool ml_loaded = false, capture = false, judge = false;
   long samples = 0;
@@ -791,7 +792,7 @@ struct World {
 
   // Variant of this solve's sweeps: forced, or the autotuner's pick, or (while tuning) the next
   // candidate to time. All variants give identical results.
-  void choose_variant() {
+  void choose_variant(int M) {
     tune_pending = false;
     if (!blocked_eligible(ncolours)) {
       cur_variant = 0;
@@ -801,7 +802,14 @@ struct World {
       cur_variant = variant_request;
       return;
     }
+    // a pick is kept while the problem stays of the size it was timed at (a batch of falling
+    // scenes has few manifolds at first): retime when the manifold count leaves [2/3, 3/2] of it
+    if (tuned && (2 * int64_t(M) > 3 * int64_t(tune_M) || 3 * int64_t(M) < 2 * int64_t(tune_M))) {
+      tune_cands.clear();
+      tuned = false;
+    }
     if (tune_cands.empty()) {
+      tune_M = M;
       tune_cands = {0, 64, 128, 256};
       tune_time.assign(tune_cands.size(), 0.0);
       tune_count.assign(tune_cands.size(), 0);
@@ -896,7 +904,7 @@ struct World {
     max_class = int(h_small[44]);
     const int sblocks = sweep_blocks(max_class);
     kt_used = 0;
-    choose_variant();
+    choose_variant(M);
     if (cur_variant > 0) {
       HIPCK(sc_keys.ensure(size_t(M) * 4));
       HIPCK(sc_slot.ensure(size_t(M) * 4));
