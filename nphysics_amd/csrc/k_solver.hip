@@ -602,17 +602,35 @@ __global__ void __launch_bounds__(64) k_pgs(DevBodies b, DevContacts c, SolverBu
     pgs_manifold(acc, b, sb, first + t);
 }
 
+// The small colour classes of a sweep (visit positions k0..k1-1, each at most kTailClass
+// manifolds) in one workgroup: the classes in visiting order with a workgroup barrier between
+// them, which orders the global-memory updates of the workgroup's waves like kernel boundaries do.
+template <bool kWarm>
+__global__ void __launch_bounds__(kTailThreads) k_pgs_tail(DevBodies b, DevContacts c, SolverBufs sb, int k0, int k1) {
+  const GlobalDv acc{b};
+  for (int k = k0; k < k1; ++k) {
+    int first, n;
+    colour_range(sb, k, first, n);
+    for (int t = threadIdx.x; t < n; t += kTailThreads) {
+      if (kWarm) warm_manifold(acc, b, sb, first + t);
+      else pgs_manifold(acc, b, sb, first + t);
+    }
+    __syncthreads();
+  }
+}
+
 // Scene-blocked sweeps: one workgroup per scene runs the warm start (when warm) and nsweeps
 // sweeps over all colours in the visiting order, with the scene's dynamic-body velocity
 // increments resident in LDS and a workgroup barrier between colours. Every body sees the same
 // sequence of row updates as under the per-colour launches (manifolds of one colour share no
 // dynamic body; colours and sweeps keep their order), so the results are identical.
-template <int kBlock>
-__global__ void __launch_bounds__(kBlock) k_sweeps_blocked(DevBodies b, SolverBufs sb, SceneBlocks sc, int ncolours,
+template <int kBlock, int kCap>
+__global__ void __launch_bounds__(kBlock) k_sweeps_blocked(DevBodies b, SolverBufs sb, SceneBlocks sc,
+                                                         const int* __restrict__ scene_list, int ncolours,
                                                          int nsweeps, int warm) {
-  __shared__ float4 s_lin[kLdsBodies];
-  __shared__ float4 s_ang[kLdsBodies];
-  const int s = blockIdx.x;
+  __shared__ float4 s_lin[kCap];
+  __shared__ float4 s_ang[kCap];
+  const int s = scene_list[blockIdx.x];
   const int d0 = sc.dyn_start[s], nd = sc.dyn_start[s + 1] - d0;
   for (int i = threadIdx.x; i < nd; i += kBlock) {
     const int body = sc.dyn_list[d0 + i];
@@ -952,17 +970,35 @@ void launch_scene_order(const DevBodies& b, int M, int nscenes, int ncolours, So
   hipLaunchKernelGGL(k_scene_starts, dim3(nb(total)), dim3(256), 0, s, sc.keys, M, nscenes, ncolours, sc.start);
 }
 
-void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int nscenes, int ncolours,
-                           int nsweeps, bool warm, int block, hipStream_t s) {
-  if (nscenes == 0) return;
+// Two launches: scenes with at most kSmallScene dynamic bodies (64-lane workgroups, 8 KB of LDS,
+// many resident per CU), then the larger ones (`block` lanes, 64 KB of LDS).
+void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int ncolours, int nsweeps,
+                           bool warm, int block, hipStream_t s) {
+  const int w = warm ? 1 : 0;
+  if (sc.n_small > 0)
+    hipLaunchKernelGGL((k_sweeps_blocked<64, kSmallScene>), dim3(sc.n_small), dim3(64), 0, s, b, sb, sc,
+                       sc.small_list, ncolours, nsweeps, w);
+  if (sc.n_large == 0) return;
   if (block == 64)
-    hipLaunchKernelGGL(k_sweeps_blocked<64>, dim3(nscenes), dim3(64), 0, s, b, sb, sc, ncolours, nsweeps, warm ? 1 : 0);
+    hipLaunchKernelGGL((k_sweeps_blocked<64, kLdsBodies>), dim3(sc.n_large), dim3(64), 0, s, b, sb, sc,
+                       sc.large_list, ncolours, nsweeps, w);
   else if (block == 128)
-    hipLaunchKernelGGL(k_sweeps_blocked<128>, dim3(nscenes), dim3(128), 0, s, b, sb, sc, ncolours, nsweeps,
-                       warm ? 1 : 0);
+    hipLaunchKernelGGL((k_sweeps_blocked<128, kLdsBodies>), dim3(sc.n_large), dim3(128), 0, s, b, sb, sc,
+                       sc.large_list, ncolours, nsweeps, w);
   else
-    hipLaunchKernelGGL(k_sweeps_blocked<256>, dim3(nscenes), dim3(256), 0, s, b, sb, sc, ncolours, nsweeps,
-                       warm ? 1 : 0);
+    hipLaunchKernelGGL((k_sweeps_blocked<256, kLdsBodies>), dim3(sc.n_large), dim3(256), 0, s, b, sb, sc,
+                       sc.large_list, ncolours, nsweeps, w);
+}
+
+void launch_tail_colours(const DevBodies& b, const DevContacts& c, int k0, int k1, bool warm, SolverBufs& sb,
+                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+  if (k1 <= k0) return;
+  if (warm)
+    hipLaunchKernelGGL(k_pgs_tail<true>, dim3(1), dim3(kTailThreads), 0, s, b, c, sb, k0, k1);
+  else if (ev0)
+    hipExtLaunchKernelGGL(k_pgs_tail<false>, dim3(1), dim3(kTailThreads), 0, s, ev0, ev1, 0, b, c, sb, k0, k1);
+  else
+    hipLaunchKernelGGL(k_pgs_tail<false>, dim3(1), dim3(kTailThreads), 0, s, b, c, sb, k0, k1);
 }
 
 }  // namespace nph

@@ -86,6 +86,10 @@ struct SolverBufs {
   uint32_t* tmp;
 };
 constexpr int kMaxColours = 4096;
+// Colour classes of at most kTailClass manifolds are visited last in each sweep and run together
+// in one workgroup of kTailThreads lanes (the CPU restatement groups them identically).
+constexpr int kTailClass = 4096;
+constexpr int kTailThreads = 512;
 constexpr int kRecFields = 8;  // float4 per contact row record
 
 void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, const DevContacts& prev,
@@ -108,12 +112,15 @@ void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, in
                              SolverBufs& sb, hipStream_t s);
 void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
+void launch_tail_colours(const DevBodies& b, const DevContacts& c, int k0, int k1, bool warm, SolverBufs& sb,
+                         hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, hipStream_t s);
 void launch_cache_impulses(const DevContacts& c, SolverBufs& sb, int C, hipStream_t s);
 
 // Scene-blocked sweeps (k_solver.hip): every scene's slots in (scene, colour) order, per scene
 // and colour the first position, and the scene's dynamic bodies (CSR) with each body's rank.
 constexpr int kLdsBodies = 2048;  // dynamic bodies of one scene the blocked kernel keeps in LDS
+constexpr int kSmallScene = 256;  // scenes up to this many dynamic bodies use the small workgroups
 struct SceneBlocks {
   uint32_t* keys;        // [M] scene << 12 | colour, sorted
   uint32_t* slot;        // [M] slots in that order
@@ -123,11 +130,14 @@ struct SceneBlocks {
   const int* dyn_start;  // [nscenes + 1]
   const int* dyn_list;   // dynamic bodies scene by scene
   const int* local;      // [n] rank of a dynamic body within its scene
+  const int* small_list; // scenes with at most kSmallScene dynamic bodies
+  const int* large_list; // the other scenes
+  int n_small, n_large;
 };
 void launch_scene_order(const DevBodies& b, int M, int nscenes, int ncolours, SolverBufs& sb, SceneBlocks& sc,
                         hipStream_t s);
-void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int nscenes, int ncolours,
-                           int nsweeps, bool warm, int block, hipStream_t s);
+void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int ncolours, int nsweeps,
+                           bool warm, int block, hipStream_t s);
 
 // learned warm start (k_solver.hip): features [C][24] and impulse units ls [C] per row record,
 // the prediction y [C][3] applied as the warm start, post-solve targets t [C][3].
@@ -140,5 +150,7 @@ void launch_lam_copy(SolverBufs& sb, int C, float4* buf, bool to_buf, hipStream_
 // k_position.hip ---------------------------------------------------------------------------
 void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k,
                             int blocks, SolverBufs& sb, hipStream_t s);
+void launch_position_tail(const DevBodies& b, const DevContacts& c, const SimParams& p, int k0, int k1,
+                          SolverBufs& sb, hipStream_t s);
 
 }  // namespace nph

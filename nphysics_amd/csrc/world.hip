@@ -131,6 +131,8 @@ struct World {
   int res_max_sweeps = 500, res_check_every = 1, res_norm = NPHYS_RESIDUAL_MAX;
   int last_sweeps = 0;
   int max_class = 0;             // largest colour class of the current solve
+  int nbig = 0;                  // classes above kTailClass (launched one per class)
+  uint32_t* h_colstart = nullptr;  // pinned: class offsets of the current solve
   hipEvent_t ev_class = nullptr;  // after its readback
   std::vector<double> red_host;
   // learned warm start: network (FP32 parameters + BF16 images), per-record features, impulse
@@ -139,7 +141,8 @@ struct World {
   // scene-major slot order, and the variant chosen by the autotuner (0 per-colour launches,
   // 64/128/256 the blocked kernel's workgroup size)
   int nscenes = 1, max_scene_dyn = 0;
-  DBuf sc_dyn_start, sc_dyn_list, sc_local, sc_keys, sc_slot, sc_keys_alt, sc_slot_alt, sc_start;
+  DBuf sc_dyn_start, sc_dyn_list, sc_local, sc_keys, sc_slot, sc_keys_alt, sc_slot_alt, sc_start, sc_small, sc_large;
+  int n_small_scenes = 0, n_large_scenes = 0;
   int variant_request = -1;  // -1: autotune; else forced
   int cur_variant = 0;
   static constexpr int kTuneReps = 3;
@@ -167,6 +170,7 @@ struct World {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (h_visit) (void)hipHostFree(h_visit);
     if (h_small) (void)hipHostFree(h_small);
+    if (h_colstart) (void)hipHostFree(h_colstart);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : kt_ev)
@@ -199,6 +203,7 @@ struct World {
     HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_small), 64 * sizeof(uint32_t)));
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_visit), (kMaxColours + 2) * sizeof(int)));
+    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_colstart), (kMaxColours + 2) * sizeof(uint32_t)));
     HIPCK(sv_visit.ensure((kMaxColours + 2) * sizeof(int)));
     for (auto& e : ev) HIPCK(hipEventCreate(&e));
     for (auto& e : tune_ev) HIPCK(hipEventCreate(&e));
@@ -375,6 +380,20 @@ struct World {
     HIPCK(up(sc_dyn_start, dstart.data(), dstart.size() * 4, 0));
     if (!dlist.empty()) HIPCK(up(sc_dyn_list, dlist.data(), dlist.size() * 4, 0));
     HIPCK(up(sc_local, local.data(), local.size() * 4, 0));
+    {
+      std::vector<int> small, large;
+      for (int sc = 0; sc < nscenes; ++sc) {
+        const int nd = dstart[size_t(sc) + 1] - dstart[size_t(sc)];
+        if (nd == 0) continue;
+        (nd <= kSmallScene ? small : large).push_back(sc);
+      }
+      n_small_scenes = int(small.size());
+      n_large_scenes = int(large.size());
+      HIPCK(sc_small.ensure(std::max<size_t>(small.size(), 1) * 4));
+      HIPCK(sc_large.ensure(std::max<size_t>(large.size(), 1) * 4));
+      if (!small.empty()) HIPCK(up(sc_small, small.data(), small.size() * 4, 0));
+      if (!large.empty()) HIPCK(up(sc_large, large.data(), large.size() * 4, 0));
+    }
     tune_cands.clear();
     tuned = false;
     HIPCK(hipStreamSynchronize(stream));
@@ -635,6 +654,7 @@ struct World {
     hash_bytes(key, &blocks, sizeof(blocks));
     hash_bytes(key, &ncolours, sizeof(ncolours));
     hash_bytes(key, &reps, sizeof(reps));
+    hash_bytes(key, &nbig, sizeof(nbig));
     for (const GraphEntry& g : graphs)
       if (g.key == key) {
         HIPCK(hipGraphLaunch(g.exec, stream));
@@ -699,12 +719,33 @@ struct World {
     launch_colour_order(b, c, M, ncolours, v, stream);
     uint32_t* max_class_dev = bp_small.as<uint32_t>() + 44;
     launch_fix_colour_starts(ncolours, M, v, max_class_dev, stream);
-    HIPCK(hipMemcpyAsync(h_small + 44, max_class_dev, 4, hipMemcpyDeviceToHost, stream));
+    // class offsets to the host (read after the assembly is enqueued: visit_order())
+    HIPCK(hipMemcpyAsync(h_colstart, v.col_start, size_t(ncolours + 1) * 4, hipMemcpyDeviceToHost, stream));
     HIPCK(hipEventRecord(ev_class, stream));
+    return NPHYS_OK;
+  }
+
+  // Visit order of this solve's colour classes: the SplitMix64 Fisher-Yates shuffle seeded by the
+  // solve index, then the classes of at most kTailClass manifolds moved behind the others (each
+  // group keeps its shuffled order); uploads it and sets max_class and nbig (the number of
+  // classes launched one by one; the rest run as one tail workgroup).
+  int visit_order() {
+    HIPCK(hipEventSynchronize(ev_class));
     colour_visit_order(uint64_t(solve_count), ncolours, colour_visit);
-    for (int k = 0; k < ncolours; ++k) h_visit[k] = colour_visit[k];
+    max_class = 0;
+    std::vector<int> big, small;
+    for (int c : colour_visit) {
+      const int sz = int(h_colstart[c + 1] - h_colstart[c]);
+      max_class = std::max(max_class, sz);
+      (sz > kTailClass ? big : small).push_back(c);
+    }
+    nbig = int(big.size());
+    int k = 0;
+    for (int c : big) h_visit[k++] = c;
+    for (int c : small) h_visit[k++] = c;
+    // h_visit is rewritten only by the next solve, after this copy has completed (ev_class above
+    // is recorded after it in stream order)
     HIPCK(hipMemcpyAsync(sv_visit.p, h_visit, size_t(ncolours) * sizeof(int), hipMemcpyHostToDevice, stream));
-    // h_visit is rewritten only after the next colouring, which follows a stream synchronisation
     return NPHYS_OK;
   }
 
@@ -723,34 +764,47 @@ struct World {
         }
         HIPCK(hipEventRecord(kt_ev[kt_used], stream));
       }
-      launch_sweeps_blocked(b, v, sc, nscenes, ncolours, nsweeps, warm, cur_variant, stream);
+      launch_sweeps_blocked(b, v, sc, ncolours, nsweeps, warm, cur_variant, stream);
       if (counters_enabled) {
         HIPCK(hipEventRecord(kt_ev[kt_used + 1], stream));
         kt_used += 2;
       }
       return NPHYS_OK;
     }
+    // large classes one launch each, the small ones (visit positions nbig..) in one workgroup
+    const bool tail = nbig < ncolours;
     if (counters_enabled) {
-      const size_t need = size_t(kt_used) + size_t(2) * size_t(nsweeps) * size_t(ncolours);
+      const size_t need = size_t(kt_used) + size_t(2) * size_t(nsweeps) * size_t(nbig + (tail ? 1 : 0));
       while (kt_ev.size() < need) {
         hipEvent_t e;
         HIPCK(hipEventCreate(&e));
         kt_ev.push_back(e);
       }
-      if (warm)
-        for (int k = 0; k < ncolours; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
-      for (int it = 0; it < nsweeps; ++it)
-        for (int k = 0; k < ncolours; ++k) {
+      if (warm) {
+        for (int k = 0; k < nbig; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
+        launch_tail_colours(b, cv, nbig, ncolours, true, v, stream);
+      }
+      for (int it = 0; it < nsweeps; ++it) {
+        for (int k = 0; k < nbig; ++k) {
           launch_pgs_colour(b, cv, k, sblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
           kt_used += 2;
         }
+        if (tail) {
+          launch_tail_colours(b, cv, nbig, ncolours, false, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
+          kt_used += 2;
+        }
+      }
       return NPHYS_OK;
     }
     auto body = [&]() {
-      if (warm)
-        for (int k = 0; k < ncolours; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
-      for (int it = 0; it < nsweeps; ++it)
-        for (int k = 0; k < ncolours; ++k) launch_pgs_colour(b, cv, k, sblocks, v, stream);
+      if (warm) {
+        for (int k = 0; k < nbig; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
+        launch_tail_colours(b, cv, nbig, ncolours, true, v, stream);
+      }
+      for (int it = 0; it < nsweeps; ++it) {
+        for (int k = 0; k < nbig; ++k) launch_pgs_colour(b, cv, k, sblocks, v, stream);
+        launch_tail_colours(b, cv, nbig, ncolours, false, v, stream);
+      }
     };
     return run_phase(warm ? 1 : 3, b, cv, v, sblocks, nsweeps, body);
   }
@@ -783,6 +837,10 @@ struct World {
     v.dyn_start = sc_dyn_start.as<int>();
     v.dyn_list = sc_dyn_list.as<int>();
     v.local = sc_local.as<int>();
+    v.small_list = sc_small.as<int>();
+    v.large_list = sc_large.as<int>();
+    v.n_small = n_small_scenes;
+    v.n_large = n_large_scenes;
     return v;
   }
 
@@ -899,9 +957,8 @@ struct World {
     if ((rc = colour(b, cv, M, v))) return rc;
     launch_assemble(b, cv, pc.view(), prevM, M, sp, v, stream);
     HIPCK(hipEventRecord(ev[EV_ASM], stream));
-    // the largest colour class sizes the sweep grids (read back while the assembly runs)
-    HIPCK(hipEventSynchronize(ev_class));
-    max_class = int(h_small[44]);
+    // class sizes (read back while the assembly runs) give the visit order and the sweep grids
+    if ((rc = visit_order())) return rc;
     const int sblocks = sweep_blocks(max_class);
     kt_used = 0;
     choose_variant(M);
@@ -981,8 +1038,10 @@ struct World {
     DevContacts cv = cc.view();
     const int pblocks = position_blocks(max_class);
     auto body = [&]() {
-      for (int it = 0; it < prm.max_position_iterations; ++it)
-        for (int k = 0; k < ncolours; ++k) launch_position_colour(b, cv, sp, k, pblocks, v, stream);
+      for (int it = 0; it < prm.max_position_iterations; ++it) {
+        for (int k = 0; k < nbig; ++k) launch_position_colour(b, cv, sp, k, pblocks, v, stream);
+        launch_position_tail(b, cv, sp, nbig, ncolours, v, stream);
+      }
     };
     if (counters_enabled) {
       body();

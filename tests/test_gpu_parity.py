@@ -230,10 +230,13 @@ def test_batched_worlds_bitexact(nphys, oracle_mod):
 
 @pytest.mark.parametrize("variant", [0, 64, 128, 256])
 def test_sweep_variants_identical(nphys, oracle_mod, variant):
-    """Per-colour launches and the scene-blocked kernel (LDS-resident velocity increments) give
-    the same bits, on a batch of scenes and on a single scene, with the fixed iteration count and
-    in solve-to-residual mode."""
-    for s, vi in ((scenes.batched(worlds=16, seed=5), 8), (scenes.boxes(nx=4, ny=6, nz=4), 10)):
+    """Per-colour launches and the scene-blocked kernels (LDS-resident velocity increments; small
+    and large scene workgroups) give the same bits, on batches of scenes and on a single scene,
+    with the fixed iteration count and in solve-to-residual mode."""
+    cases = ((scenes.batched(worlds=16, seed=5), 8),                                  # small-scene workgroups
+             (scenes.batched(worlds=5, seed=8, median=500, n_min=300, n_max=900), 8),  # large-scene workgroups
+             (scenes.boxes(nx=4, ny=6, nz=4), 10))                                     # one scene
+    for s, vi in cases:
         g = _gpu_world(nphys, s, max_velocity_iterations=vi)
         g.set_sweep_variant(variant)
         o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED, max_velocity_iterations=vi)
@@ -280,3 +283,25 @@ def test_fast_spin_exponential_bitexact(nphys, oracle_mod):
                       f"contacts {g.counters()['ncontacts']} / {o.counters()['ncontacts']}")
         for key in ("pos", "rot", "linvel", "angvel"):
             np.testing.assert_array_equal(gs[key], _f32(os_[key]), err_msg=f"step {k + 1} {key}")
+
+
+def test_large_and_small_classes_bitexact(nphys, oracle_mod):
+    """A 150 x 150 layer of touching balls: colour classes above 4,096 manifolds run one launch
+    each and the small ones together in one workgroup after them; both sides visit the classes in
+    that grouped order, bit for bit."""
+    s = scenes.balls(seed=13, nx=150, ny=1, nz=150, spacing=1.0, jitter=0.0)
+    g = _gpu_world(nphys, s)
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED)
+    g.record_colours()
+    g.step(10)
+    o.step(10)
+    cols, ncol = g.colours()
+    sizes = np.bincount(cols, minlength=ncol)
+    print("class sizes", sizes)
+    assert (sizes > 4096).any() and (sizes <= 4096).any()
+    gs, os_ = g.get_state(), o.get_state()
+    for key in ("pos", "rot", "linvel", "angvel"):
+        np.testing.assert_array_equal(gs[key], _f32(os_[key]), err_msg=key)
+    gk, gi = g.impulses()
+    ok, oi = o.impulses()
+    np.testing.assert_array_equal(gi, _f32(oi))
