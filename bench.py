@@ -280,10 +280,14 @@ def main():
     n_dyn = int((scene["status"] == 0).sum())
     C, M, ncol = c["ncontacts"], c["nmanifolds"], c["ncolours"]
     iters = params.max_velocity_iterations
-    sweep_bytes = 168 * C + 76 * n_dyn  # algorithmic bytes of one velocity sweep
-    launches_per_sweep = ct["sweep_kernel_launches"] / max(1, args.timing_steps * iters)
-    avg_launch_s = ct["sweep_kernel_time"] / max(1, ct["sweep_kernel_launches"])
-    bytes_per_launch = sweep_bytes / max(1e-9, launches_per_sweep)
+    # Algorithmic bytes of one velocity sweep, 168 C + 76 N, attributed to the launches in
+    # proportion to the contacts each processes; the timed launches are those of the large colour
+    # classes (one launch per class), so bytes per launch = bytes per contact x their contacts.
+    sweep_bytes = 168 * C + 76 * n_dyn
+    bytes_per_contact = sweep_bytes / max(1, C)
+    launches = max(1, ct["sweep_kernel_launches"])
+    avg_launch_s = ct["sweep_kernel_time"] / launches
+    bytes_per_launch = bytes_per_contact * ct["sweep_kernel_contacts"] / launches
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     traffic = None
     if os.path.exists(PROFILE_TRAFFIC):
@@ -327,7 +331,7 @@ def main():
                                ("collision_detection", "broad_phase", "narrow_phase", "assembly",
                                 "velocity_resolution", "velocity_update", "position_resolution")},
         "roofline": {
-            "kernel": "k_pgs (one colour of one SOR-Prox sweep)",
+            "kernel": "k_pgs (one large colour class of one SOR-Prox sweep)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,

@@ -109,10 +109,13 @@ typedef struct nphys_counters {
   int64_t nconstraints;
   int64_t ncolours;
   int64_t colour_rounds;    /* accumulated greedy-colouring rounds */
-  /* Per-launch device time of the SOR-Prox sweep kernel, summed over launches, and the launch
-   * count; measured with HIP events around every launch while counters are enabled. */
+  /* Per-launch device time of the SOR-Prox sweep kernel over one large colour class, summed over
+   * launches, the launch count and the contacts those launches processed; measured with HIP events
+   * in every such launch while counters are enabled (the single-workgroup launch of the small
+   * classes is not included). */
   double sweep_kernel_time;
   int64_t sweep_kernel_launches;
+  int64_t sweep_kernel_contacts;
   int64_t velocity_sweeps;  /* accumulated SOR-Prox sweeps (fixed count, or to the residual target) */
   /* judge of the learned warm start (nphys_set_warmstart_judge): sweeps of the plain and of the
    * learned solve, and the steps whose learned solve was kept (it needed fewer sweeps) */

@@ -113,6 +113,7 @@ class Counters(C.Structure):
         ("colour_rounds", C.c_int64),
         ("sweep_kernel_time", C.c_double),
         ("sweep_kernel_launches", C.c_int64),
+        ("sweep_kernel_contacts", C.c_int64),
         ("velocity_sweeps", C.c_int64),
         ("judge_plain_sweeps", C.c_int64),
         ("judge_learned_sweeps", C.c_int64),

@@ -687,17 +687,17 @@ __global__ void __launch_bounds__(256) k_scene_starts(const uint32_t* __restrict
 }
 
 // Colours left empty (inherited colours can leave gaps) start where the next one does.
-// Also reports the largest colour class, which sizes the sweep grids.
-__global__ void k_fix_colour_starts(int ncolours, int M, uint32_t* col_start, uint32_t* max_class) {
+// Also writes the first row record of every class (class_rec[k], class_rec[ncolours] = C),
+// which gives the host the contacts per class.
+__global__ void k_fix_colour_starts(int ncolours, int M, int C, uint32_t* col_start, const uint32_t* __restrict__ soff,
+                                    uint32_t* class_rec) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   col_start[ncolours] = uint32_t(M);
-  uint32_t mx = 0;
+  class_rec[ncolours] = uint32_t(C);
   for (int k = ncolours - 1; k >= 0; --k) {
     if (col_start[k] == 0xffffffffu) col_start[k] = col_start[k + 1];
-    const uint32_t sz = col_start[k + 1] - col_start[k];
-    mx = sz > mx ? sz : mx;
+    class_rec[k] = col_start[k] < uint32_t(M) ? soff[col_start[k]] : uint32_t(C);
   }
-  *max_class = mx;
 }
 
 // Natural-map residual per row; per block (max |phi|, sum phi^2) in double, reduced on the host
@@ -900,8 +900,8 @@ void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContac
   hipLaunchKernelGGL(k_assemble, dim3(nb(M)), dim3(256), 0, s, b, cur, prev, prevM, M, p, sb);
 }
 
-void launch_fix_colour_starts(int ncolours, int M, SolverBufs& sb, uint32_t* max_class, hipStream_t s) {
-  hipLaunchKernelGGL(k_fix_colour_starts, dim3(1), dim3(1), 0, s, ncolours, M, sb.col_start, max_class);
+void launch_fix_colour_starts(int ncolours, int M, int C, SolverBufs& sb, uint32_t* class_rec, hipStream_t s) {
+  hipLaunchKernelGGL(k_fix_colour_starts, dim3(1), dim3(1), 0, s, ncolours, M, C, sb.col_start, sb.soff, class_rec);
 }
 
 // One lane per manifold: one 64-lane workgroup per 64 manifolds of the largest colour class,

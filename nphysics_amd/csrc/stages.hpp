@@ -99,7 +99,7 @@ void launch_colour_round(const DevBodies& b, const DevContacts& c, int M, int ro
 void launch_colour_keys(int M, SolverBufs& sb, uint32_t* ncol, hipStream_t s);
 void launch_colour_order(const DevBodies& b, const DevContacts& c, int M, int ncolours, SolverBufs& sb,
                          hipStream_t s);
-void launch_fix_colour_starts(int ncolours, int M, SolverBufs& sb, uint32_t* max_class, hipStream_t s);
+void launch_fix_colour_starts(int ncolours, int M, int C, SolverBufs& sb, uint32_t* class_rec, hipStream_t s);
 int residual_blocks(int M);
 void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContacts& prev,
                      int prevM, int M, const SimParams& p, SolverBufs& sb, hipStream_t s);

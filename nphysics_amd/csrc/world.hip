@@ -203,7 +203,7 @@ struct World {
     HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_small), 64 * sizeof(uint32_t)));
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_visit), (kMaxColours + 2) * sizeof(int)));
-    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_colstart), (kMaxColours + 2) * sizeof(uint32_t)));
+    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_colstart), 2 * (kMaxColours + 2) * sizeof(uint32_t)));
     HIPCK(sv_visit.ensure((kMaxColours + 2) * sizeof(int)));
     for (auto& e : ev) HIPCK(hipEventCreate(&e));
     for (auto& e : tune_ev) HIPCK(hipEventCreate(&e));
@@ -592,7 +592,7 @@ struct World {
       HIPCK(b->ensure(m * 4));
     HIPCK(sv_state2[cur].ensure(m * 4));
     HIPCK(sv_state2[cur ^ 1].ensure(4));
-    HIPCK(sv_col_start.ensure((kMaxColours + 2) * 4));
+    HIPCK(sv_col_start.ensure(2 * (kMaxColours + 2) * 4));
     HIPCK(sv_sinfo.ensure(m * sizeof(int4)));
     HIPCK(sv_rec2c.ensure(c * 4));
     HIPCK(sv_rec.ensure(c * kRecFields * sizeof(float4)));
@@ -717,10 +717,12 @@ struct World {
     ncolours = int(h_small[0]);
     if (ncolours > kMaxColours) return set_err(NPHYS_ERR_STATE, "too many colours");
     launch_colour_order(b, c, M, ncolours, v, stream);
-    uint32_t* max_class_dev = bp_small.as<uint32_t>() + 44;
-    launch_fix_colour_starts(ncolours, M, v, max_class_dev, stream);
-    // class offsets to the host (read after the assembly is enqueued: visit_order())
+    uint32_t* class_rec = v.col_start + (kMaxColours + 2);
+    launch_fix_colour_starts(ncolours, M, cb[cur].C, v, class_rec, stream);
+    // class offsets (slots and row records) to the host, read after the assembly is enqueued
     HIPCK(hipMemcpyAsync(h_colstart, v.col_start, size_t(ncolours + 1) * 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(h_colstart + (kMaxColours + 2), class_rec, size_t(ncolours + 1) * 4,
+                         hipMemcpyDeviceToHost, stream));
     HIPCK(hipEventRecord(ev_class, stream));
     return NPHYS_OK;
   }
@@ -774,7 +776,7 @@ struct World {
     // large classes one launch each, the small ones (visit positions nbig..) in one workgroup
     const bool tail = nbig < ncolours;
     if (counters_enabled) {
-      const size_t need = size_t(kt_used) + size_t(2) * size_t(nsweeps) * size_t(nbig + (tail ? 1 : 0));
+      const size_t need = size_t(kt_used) + size_t(2) * size_t(nsweeps) * size_t(nbig);
       while (kt_ev.size() < need) {
         hipEvent_t e;
         HIPCK(hipEventCreate(&e));
@@ -788,11 +790,11 @@ struct World {
         for (int k = 0; k < nbig; ++k) {
           launch_pgs_colour(b, cv, k, sblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
           kt_used += 2;
+          const int c = h_visit[k];
+          counters.sweep_kernel_contacts +=
+              int64_t(h_colstart[kMaxColours + 2 + c + 1]) - int64_t(h_colstart[kMaxColours + 2 + c]);
         }
-        if (tail) {
-          launch_tail_colours(b, cv, nbig, ncolours, false, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
-          kt_used += 2;
-        }
+        if (tail) launch_tail_colours(b, cv, nbig, ncolours, false, v, stream);
       }
       return NPHYS_OK;
     }
