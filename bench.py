@@ -323,6 +323,8 @@ def main():
             "velocity_iterations": iters,
             "position_iterations": params.max_position_iterations,
             "sweep_graphs": os.environ.get("NPHYS_DISABLE_GRAPHS", "0") != "1",
+            "loose_changes_per_step": c["loose_changes"] / args.steps,
+            "pair_rebuilds_per_step": c["pair_rebuilds"] / args.steps,
         },
         "contact_iters_per_s": C * iters * args.steps / elapsed,
         "residual_inf": rinf,

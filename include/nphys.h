@@ -123,6 +123,8 @@ typedef struct nphys_counters {
   int64_t judge_learned_sweeps;
   int64_t judge_kept;
   int64_t sweep_variant;  /* sweeps of the last step: 0 per-colour launches, 64/128/256 scene-blocked */
+  int64_t loose_changes;  /* accumulated loose AABBs replaced by the end-of-step syncs */
+  int64_t pair_rebuilds;  /* accumulated end-of-step broad-phase pair rebuilds */
 } nphys_counters;
 
 /* ---------------------------------------------------------------- lifecycle */

@@ -119,6 +119,8 @@ class Counters(C.Structure):
         ("judge_learned_sweeps", C.c_int64),
         ("judge_kept", C.c_int64),
         ("sweep_variant", C.c_int64),
+        ("loose_changes", C.c_int64),
+        ("pair_rebuilds", C.c_int64),
     ]
 
     def as_dict(self):

@@ -905,13 +905,14 @@ void launch_fix_colour_starts(int ncolours, int M, int C, SolverBufs& sb, uint32
 }
 
 // One lane per manifold: one 64-lane workgroup per 64 manifolds of the largest colour class,
-// rounded up to a multiple of 64 workgroups (so the launch shape, and with it the captured
-// sweep graph, changes only when the largest class crosses a multiple of 4,096) and capped at
-// 8,192; larger classes are walked by the grid-stride loop.
+// rounded up to a power of two (so the launch shape, and with it the captured sweep graph,
+// changes only when the largest class doubles or halves), at least 64 and at most 8,192;
+// larger classes are walked by the grid-stride loop.
 int sweep_blocks(int max_class) {
   int need = (max_class + 63) / 64;
-  need = ((need + 63) / 64) * 64;
-  return need < 64 ? 64 : (need > 8192 ? 8192 : need);
+  int blocks = 64;
+  while (blocks < need && blocks < 8192) blocks *= 2;
+  return blocks;
 }
 
 void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,

@@ -655,8 +655,12 @@ struct World {
     hash_bytes(key, &ncolours, sizeof(ncolours));
     hash_bytes(key, &reps, sizeof(reps));
     hash_bytes(key, &nbig, sizeof(nbig));
-    for (const GraphEntry& g : graphs)
-      if (g.key == key) {
+    for (size_t i = 0; i < graphs.size(); ++i)
+      if (graphs[i].key == key) {
+        const GraphEntry g = graphs[i];
+        // least recently used first: the hit moves to the back
+        graphs.erase(graphs.begin() + long(i));
+        graphs.push_back(g);
         HIPCK(hipGraphLaunch(g.exec, stream));
         return NPHYS_OK;
       }
@@ -669,6 +673,8 @@ struct World {
     (void)hipGraphDestroy(graph);
     HIPCK(e);
     if (graphs.size() >= 8) {
+      // the evicted graph may still be executing from an earlier launch on this stream
+      HIPCK(hipStreamSynchronize(stream));
       (void)hipGraphExecDestroy(graphs.front().exec);
       graphs.erase(graphs.begin());
     }
@@ -964,6 +970,7 @@ struct World {
     const int sblocks = sweep_blocks(max_class);
     kt_used = 0;
     choose_variant(M);
+    if (tune_pending) HIPCK(hipEventRecord(tune_ev[0], stream));
     if (cur_variant > 0) {
       HIPCK(sc_keys.ensure(size_t(M) * 4));
       HIPCK(sc_slot.ensure(size_t(M) * 4));
@@ -973,7 +980,6 @@ struct World {
       SceneBlocks sc = sc_view();
       launch_scene_order(b, M, nscenes, ncolours, v, sc, stream);
     }
-    if (tune_pending) HIPCK(hipEventRecord(tune_ev[0], stream));
     const bool learned = contact_model == NPHYS_CONTACT_MODEL_LEARNED_WARMSTART;
     if (learned || capture) {
       HIPCK(ml_X.ensure(size_t(C) * kFeat * sizeof(float)));
@@ -1073,8 +1079,10 @@ struct World {
     uint32_t ch1 = 0, ch2 = 0;
     if ((rc = loose_pass(2, &ch1))) return rc;  // CCD-phase sync: swept AABBs
     if ((rc = loose_pass(1, &ch2))) return rc;  // final sync
+    counters.loose_changes += int64_t(ch1) + int64_t(ch2);
     if (ch1 || ch2 || pairs_dirty) {
       if ((rc = rebuild_pairs())) return rc;
+      ++counters.pair_rebuilds;
     }
     HIPCK(hipEventRecord(ev[EV_BP], stream));
     if ((rc = narrow_phase())) return rc;

@@ -305,3 +305,20 @@ def test_large_and_small_classes_bitexact(nphys, oracle_mod):
     gk, gi = g.impulses()
     ok, oi = o.impulses()
     np.testing.assert_array_equal(gi, _f32(oi))
+
+
+def test_settling_pile_long_run_bitexact(nphys, oracle_mod):
+    """300 steps of a settling pile: the colour count, the large/small class split and the grid
+    size change many times, so the cached sweep graphs are replaced over and over (an evicted
+    graph may still be executing when it is replaced); the trajectory stays bit-exact."""
+    s = scenes.small_pile(seed=17, n=1024, lattice=(16, 4, 16), extent=12.0, y0=0.3)
+    g = _gpu_world(nphys, s)
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED)
+    for chunk in range(6):
+        g.step(50)
+        o.step(50)
+        gs, os_ = g.get_state(), o.get_state()
+        same = all(np.array_equal(gs[k], _f32(os_[k])) for k in ("pos", "rot", "linvel", "angvel"))
+        print(f"step {50 * (chunk + 1)}: contacts {g.counters()['ncontacts']} / {o.counters()['ncontacts']}, "
+              f"colours {g.counters()['ncolours']}, identical {same}")
+        assert same, f"diverged by step {50 * (chunk + 1)}"
