@@ -101,28 +101,10 @@ __global__ void __launch_bounds__(64) k_position(DevBodies b, DevContacts c, Sim
     position_one(b, c, p, sb.sinfo[first_slot + t]);
 }
 
-// The small colour classes (visit positions k0..k1-1) in one workgroup, barrier between classes.
-__global__ void __launch_bounds__(kTailThreads) k_position_tail(DevBodies b, DevContacts c, SimParams p, SolverBufs sb,
-                                                                int k0, int k1) {
-  for (int k = k0; k < k1; ++k) {
-    const int col = sb.visit[k];
-    const int first_slot = int(sb.col_start[col]);
-    const int nslots = int(sb.col_start[col + 1]) - first_slot;
-    for (int t = threadIdx.x; t < nslots; t += kTailThreads) position_one(b, c, p, sb.sinfo[first_slot + t]);
-    __syncthreads();
-  }
-}
-
 }  // namespace
 
 // Sized like the sweep grid: from the largest colour class.
 int position_blocks(int max_class) { return sweep_blocks(max_class); }
-
-void launch_position_tail(const DevBodies& b, const DevContacts& c, const SimParams& p, int k0, int k1,
-                          SolverBufs& sb, hipStream_t s) {
-  if (k1 <= k0) return;
-  hipLaunchKernelGGL(k_position_tail, dim3(1), dim3(kTailThreads), 0, s, b, c, p, sb, k0, k1);
-}
 
 void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k, int blocks,
                             SolverBufs& sb, hipStream_t s) {

@@ -602,23 +602,6 @@ __global__ void __launch_bounds__(64) k_pgs(DevBodies b, DevContacts c, SolverBu
     pgs_manifold(acc, b, sb, first + t);
 }
 
-// The small colour classes of a sweep (visit positions k0..k1-1, each at most kTailClass
-// manifolds) in one workgroup: the classes in visiting order with a workgroup barrier between
-// them, which orders the global-memory updates of the workgroup's waves like kernel boundaries do.
-template <bool kWarm>
-__global__ void __launch_bounds__(kTailThreads) k_pgs_tail(DevBodies b, DevContacts c, SolverBufs sb, int k0, int k1) {
-  const GlobalDv acc{b};
-  for (int k = k0; k < k1; ++k) {
-    int first, n;
-    colour_range(sb, k, first, n);
-    for (int t = threadIdx.x; t < n; t += kTailThreads) {
-      if (kWarm) warm_manifold(acc, b, sb, first + t);
-      else pgs_manifold(acc, b, sb, first + t);
-    }
-    __syncthreads();
-  }
-}
-
 // Scene-blocked sweeps: one workgroup per scene runs the warm start (when warm) and nsweeps
 // sweeps over all colours in the visiting order, with the scene's dynamic-body velocity
 // increments resident in LDS and a workgroup barrier between colours. Every body sees the same
@@ -989,17 +972,6 @@ void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks
   else
     hipLaunchKernelGGL((k_sweeps_blocked<256, kLdsBodies>), dim3(sc.n_large), dim3(256), 0, s, b, sb, sc,
                        sc.large_list, ncolours, nsweeps, w);
-}
-
-void launch_tail_colours(const DevBodies& b, const DevContacts& c, int k0, int k1, bool warm, SolverBufs& sb,
-                         hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
-  if (k1 <= k0) return;
-  if (warm)
-    hipLaunchKernelGGL(k_pgs_tail<true>, dim3(1), dim3(kTailThreads), 0, s, b, c, sb, k0, k1);
-  else if (ev0)
-    hipExtLaunchKernelGGL(k_pgs_tail<false>, dim3(1), dim3(kTailThreads), 0, s, ev0, ev1, 0, b, c, sb, k0, k1);
-  else
-    hipLaunchKernelGGL(k_pgs_tail<false>, dim3(1), dim3(kTailThreads), 0, s, b, c, sb, k0, k1);
 }
 
 }  // namespace nph

@@ -86,10 +86,6 @@ struct SolverBufs {
   uint32_t* tmp;
 };
 constexpr int kMaxColours = 4096;
-// Colour classes of at most kTailClass manifolds are visited last in each sweep and run together
-// in one workgroup of kTailThreads lanes (the CPU restatement groups them identically).
-constexpr int kTailClass = 4096;
-constexpr int kTailThreads = 512;
 constexpr int kRecFields = 8;  // float4 per contact row record
 
 void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, const DevContacts& prev,
@@ -112,8 +108,6 @@ void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, in
                              SolverBufs& sb, hipStream_t s);
 void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
-void launch_tail_colours(const DevBodies& b, const DevContacts& c, int k0, int k1, bool warm, SolverBufs& sb,
-                         hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, hipStream_t s);
 void launch_cache_impulses(const DevContacts& c, SolverBufs& sb, int C, hipStream_t s);
 
@@ -150,7 +144,5 @@ void launch_lam_copy(SolverBufs& sb, int C, float4* buf, bool to_buf, hipStream_
 // k_position.hip ---------------------------------------------------------------------------
 void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k,
                             int blocks, SolverBufs& sb, hipStream_t s);
-void launch_position_tail(const DevBodies& b, const DevContacts& c, const SimParams& p, int k0, int k1,
-                          SolverBufs& sb, hipStream_t s);
 
 }  // namespace nph

@@ -131,7 +131,6 @@ struct World {
   int res_max_sweeps = 500, res_check_every = 1, res_norm = NPHYS_RESIDUAL_MAX;
   int last_sweeps = 0;
   int max_class = 0;             // largest colour class of the current solve
-  int nbig = 0;                  // classes above kTailClass (launched one per class)
   uint32_t* h_colstart = nullptr;  // pinned: class offsets of the current solve
   hipEvent_t ev_class = nullptr;  // after its readback
   std::vector<double> red_host;
@@ -654,7 +653,6 @@ struct World {
     hash_bytes(key, &blocks, sizeof(blocks));
     hash_bytes(key, &ncolours, sizeof(ncolours));
     hash_bytes(key, &reps, sizeof(reps));
-    hash_bytes(key, &nbig, sizeof(nbig));
     for (size_t i = 0; i < graphs.size(); ++i)
       if (graphs[i].key == key) {
         const GraphEntry g = graphs[i];
@@ -733,26 +731,20 @@ struct World {
     return NPHYS_OK;
   }
 
-  // Visit order of this solve's colour classes: the SplitMix64 Fisher-Yates shuffle seeded by the
-  // solve index, then the classes of at most kTailClass manifolds moved behind the others (each
-  // group keeps its shuffled order); uploads it and sets max_class and nbig (the number of
-  // classes launched one by one; the rest run as one tail workgroup).
+  // Visit order of this solve's colour classes (the SplitMix64 Fisher-Yates shuffle seeded by the
+  // solve index, computed on the host) and the largest class (sizes the sweep grids), from the
+  // class offsets read back in colour().
   int visit_order() {
     HIPCK(hipEventSynchronize(ev_class));
     colour_visit_order(uint64_t(solve_count), ncolours, colour_visit);
     max_class = 0;
-    std::vector<int> big, small;
-    for (int c : colour_visit) {
-      const int sz = int(h_colstart[c + 1] - h_colstart[c]);
-      max_class = std::max(max_class, sz);
-      (sz > kTailClass ? big : small).push_back(c);
+    for (int k = 0; k < ncolours; ++k) {
+      const int c = colour_visit[k];
+      max_class = std::max(max_class, int(h_colstart[c + 1] - h_colstart[c]));
+      h_visit[k] = c;
     }
-    nbig = int(big.size());
-    int k = 0;
-    for (int c : big) h_visit[k++] = c;
-    for (int c : small) h_visit[k++] = c;
-    // h_visit is rewritten only by the next solve, after this copy has completed (ev_class above
-    // is recorded after it in stream order)
+    // h_visit is rewritten only by the next solve, after this copy has completed (that solve's
+    // ev_class is recorded after it in stream order)
     HIPCK(hipMemcpyAsync(sv_visit.p, h_visit, size_t(ncolours) * sizeof(int), hipMemcpyHostToDevice, stream));
     return NPHYS_OK;
   }
@@ -779,40 +771,30 @@ struct World {
       }
       return NPHYS_OK;
     }
-    // large classes one launch each, the small ones (visit positions nbig..) in one workgroup
-    const bool tail = nbig < ncolours;
     if (counters_enabled) {
-      const size_t need = size_t(kt_used) + size_t(2) * size_t(nsweeps) * size_t(nbig);
+      const size_t need = size_t(kt_used) + size_t(2) * size_t(nsweeps) * size_t(ncolours);
       while (kt_ev.size() < need) {
         hipEvent_t e;
         HIPCK(hipEventCreate(&e));
         kt_ev.push_back(e);
       }
-      if (warm) {
-        for (int k = 0; k < nbig; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
-        launch_tail_colours(b, cv, nbig, ncolours, true, v, stream);
-      }
-      for (int it = 0; it < nsweeps; ++it) {
-        for (int k = 0; k < nbig; ++k) {
+      if (warm)
+        for (int k = 0; k < ncolours; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
+      for (int it = 0; it < nsweeps; ++it)
+        for (int k = 0; k < ncolours; ++k) {
           launch_pgs_colour(b, cv, k, sblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
           kt_used += 2;
           const int c = h_visit[k];
           counters.sweep_kernel_contacts +=
               int64_t(h_colstart[kMaxColours + 2 + c + 1]) - int64_t(h_colstart[kMaxColours + 2 + c]);
         }
-        if (tail) launch_tail_colours(b, cv, nbig, ncolours, false, v, stream);
-      }
       return NPHYS_OK;
     }
     auto body = [&]() {
-      if (warm) {
-        for (int k = 0; k < nbig; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
-        launch_tail_colours(b, cv, nbig, ncolours, true, v, stream);
-      }
-      for (int it = 0; it < nsweeps; ++it) {
-        for (int k = 0; k < nbig; ++k) launch_pgs_colour(b, cv, k, sblocks, v, stream);
-        launch_tail_colours(b, cv, nbig, ncolours, false, v, stream);
-      }
+      if (warm)
+        for (int k = 0; k < ncolours; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
+      for (int it = 0; it < nsweeps; ++it)
+        for (int k = 0; k < ncolours; ++k) launch_pgs_colour(b, cv, k, sblocks, v, stream);
     };
     return run_phase(warm ? 1 : 3, b, cv, v, sblocks, nsweeps, body);
   }
@@ -1046,10 +1028,8 @@ struct World {
     DevContacts cv = cc.view();
     const int pblocks = position_blocks(max_class);
     auto body = [&]() {
-      for (int it = 0; it < prm.max_position_iterations; ++it) {
-        for (int k = 0; k < nbig; ++k) launch_position_colour(b, cv, sp, k, pblocks, v, stream);
-        launch_position_tail(b, cv, sp, nbig, ncolours, v, stream);
-      }
+      for (int it = 0; it < prm.max_position_iterations; ++it)
+        for (int k = 0; k < ncolours; ++k) launch_position_colour(b, cv, sp, k, pblocks, v, stream);
     };
     if (counters_enabled) {
       body();

@@ -918,21 +918,6 @@ class World {
   // Colours persist across steps: a manifold whose body pair was also a manifold in the previous
   // step keeps that colour (the pair's conflicts are the same, so the partial colouring is valid),
   // and only new manifolds are coloured greedily around them.
-  // Classes of at most kTailClass manifolds are visited after the larger ones, each group in the
-  // shuffled order (the GPU runs the small classes of a sweep in one workgroup).
-  static constexpr int kTailClass = 4096;
-  void group_small_classes(std::vector<int>& perm) const {
-    std::vector<int> cnt(size_t(ncolours), 0);
-    for (int c : colours) ++cnt[size_t(c)];
-    std::vector<int> out;
-    out.reserve(perm.size());
-    for (int c : perm)
-      if (cnt[size_t(c)] > kTailClass) out.push_back(c);
-    for (int c : perm)
-      if (cnt[size_t(c)] <= kTailClass) out.push_back(c);
-    perm.swap(out);
-  }
-
   void build_colour_order() {
     std::vector<int> is_dyn(bodies.size());
     for (size_t i = 0; i < bodies.size(); ++i) is_dyn[i] = bodies[i].status == STATUS_DYNAMIC;
@@ -952,7 +937,6 @@ class World {
     ncolours = colour_manifolds(solved, is_dyn, int(bodies.size()), colours);
     std::vector<int> perm;
     colour_visit_order(uint64_t(solve_count), ncolours, perm);
-    group_small_classes(perm);
     colour_order.clear();
     for (int k = 0; k < ncolours; ++k)
       for (int m = 0; m < int(solved.size()); ++m)
