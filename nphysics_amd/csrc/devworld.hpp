@@ -1,9 +1,14 @@
 // nphysics_amd/csrc/devworld.hpp -- device-side views of one world, passed to kernels by value.
 //
-// HBM layout (structure of arrays, one float4 per 3-vector so every access is a 16-byte load):
-//   bodies   pos/rot/pos0/rot0/linvel/angvel, inverse augmented inertia as three rows with the
-//            inverse mass in row 0's w, external velocity increment dt*acc (lin, ang), solver
-//            velocity increment dv (lin, ang), loose AABB (mn, mx), static shape data.
+// HBM layout (one float4 per 3-vector so every access is a 16-byte load):
+//   bodies   a 32-byte pose block per body (pos, rot as quaternion xyzw) and a 64-byte solver
+//            block holding exactly what a velocity sweep reads and writes per body, so a body
+//            costs one 64-byte line per manifold:
+//              [0] (I00, I01, I02, 1/m)  [1] (I10, I11, I12, I20)
+//              [2] (I21, I22, dv_lin.x, dv_lin.y)  [3] (dv_lin.z, dv_ang.x, dv_ang.y, dv_ang.z)
+//            with I the inverse augmented inertia. Structure of arrays for the rest:
+//            pos0/rot0/linvel/angvel, external velocity increment dt*acc (lin, ang), loose AABB
+//            (mn, mx), shape data.
 //   contacts per contact: world1 (w = depth), world2, normal, feature, kinematic kind, local
 //            points/normal for the position solver, solved impulse; per manifold: bodies,
 //            first contact, count.
@@ -21,6 +26,10 @@ struct SimParams {
   float gx, gy, gz;
 };
 
+// float4 per pose block and per solver block
+constexpr int kPoseBlock = 2;
+constexpr int kSolverBlock = 4;
+
 struct DevBodies {
   int n;
   const int* status;
@@ -33,21 +42,52 @@ struct DevBodies {
   const float* restitution;
   const float* mass;
   const float4* local_i; // diagonal of the local angular inertia
-  float4* pos;
-  float4* rot;
+  float4* pose;          // kPoseBlock float4 per body
+  float4* sol;           // kSolverBlock float4 per body
   float4* pos0;
   float4* rot0;
   float4* linvel;
   float4* angvel;
-  float4* ia0;
-  float4* ia1;
-  float4* ia2;
   float4* ext_lin;
   float4* ext_ang;
-  float4* dv_lin;
-  float4* dv_ang;
   float4* loose_mn;
   float4* loose_mx;
+
+  NPH_HD float4& pos(int i) const { return pose[long(i) * kPoseBlock]; }
+  NPH_HD float4& rot(int i) const { return pose[long(i) * kPoseBlock + 1]; }
+  NPH_HD float4* solver(int i) const { return sol + long(i) * kSolverBlock; }
+
+  // Inverse augmented inertia and inverse mass of body i.
+  NPH_HD m33 inv_inertia(int i, float& inv_mass) const {
+    const float4* q = solver(i);
+    const float4 r0 = q[0], r1 = q[1], r2 = q[2];
+    m33 m;
+    m.m[0][0] = r0.x; m.m[0][1] = r0.y; m.m[0][2] = r0.z;
+    m.m[1][0] = r1.x; m.m[1][1] = r1.y; m.m[1][2] = r1.z;
+    m.m[2][0] = r1.w; m.m[2][1] = r2.x; m.m[2][2] = r2.y;
+    inv_mass = r0.w;
+    return m;
+  }
+  // Writes the inverse inertia and mass of body i and clears its velocity increment.
+  NPH_HD void set_inv_inertia(int i, const m33& m, float inv_mass) const {
+    float4* q = solver(i);
+    q[0] = make_float4(m.m[0][0], m.m[0][1], m.m[0][2], inv_mass);
+    q[1] = make_float4(m.m[1][0], m.m[1][1], m.m[1][2], m.m[2][0]);
+    q[2] = make_float4(m.m[2][1], m.m[2][2], 0.0f, 0.0f);
+    q[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  NPH_HD void load_dv(int i, f3& lin, f3& ang) const {
+    const float4* q = solver(i);
+    const float2 a = *reinterpret_cast<const float2*>(&q[2].z);
+    const float4 c = q[3];
+    lin = f3{a.x, a.y, c.x};
+    ang = f3{c.y, c.z, c.w};
+  }
+  NPH_HD void store_dv(int i, f3 lin, f3 ang) const {
+    float4* q = solver(i);
+    *reinterpret_cast<float2*>(&q[2].z) = make_float2(lin.x, lin.y);
+    q[3] = make_float4(lin.z, ang.x, ang.y, ang.z);
+  }
 };
 
 struct DevContacts {
@@ -65,14 +105,6 @@ struct DevContacts {
   float4* imp;       // (normal, tangent 1, tangent 2) impulses after the solve
 };
 
-NPH_HD m33 load_inv(const DevBodies& b, int i, float& inv_mass) {
-  const float4 r0 = b.ia0[i], r1 = b.ia1[i], r2 = b.ia2[i];
-  m33 m;
-  m.m[0][0] = r0.x; m.m[0][1] = r0.y; m.m[0][2] = r0.z;
-  m.m[1][0] = r1.x; m.m[1][1] = r1.y; m.m[1][2] = r1.z;
-  m.m[2][0] = r2.x; m.m[2][1] = r2.y; m.m[2][2] = r2.z;
-  inv_mass = r0.w;
-  return m;
-}
+NPH_HD m33 load_inv(const DevBodies& b, int i, float& inv_mass) { return b.inv_inertia(i, inv_mass); }
 
 }  // namespace nph

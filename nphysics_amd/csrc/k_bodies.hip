@@ -26,14 +26,13 @@ namespace {
 __global__ void __launch_bounds__(256) k_prestep(DevBodies b, SimParams p) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
-  const float4 t = b.pos[i];
-  const float4 q = b.rot[i];
+  const float4 t = b.pos(i);
+  const float4 q = b.rot(i);
   b.pos0[i] = t;
   b.rot0[i] = q;
   const float4 z4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-  b.dv_lin[i] = z4;
-  b.dv_ang[i] = z4;
   if (b.status[i] != STATUS_DYNAMIC) {
+    b.set_inv_inertia(i, m33{}, 0.0f);
     b.ext_lin[i] = z4;
     b.ext_ang[i] = z4;
     return;
@@ -60,9 +59,7 @@ __global__ void __launch_bounds__(256) k_prestep(DevBodies b, SimParams p) {
   const float inv_mass = mass == 0.0f ? 0.0f : 1.0f / mass;
   m33 inv;
   try_inverse(aug, inv);
-  b.ia0[i] = make_float4(inv.m[0][0], inv.m[0][1], inv.m[0][2], inv_mass);
-  b.ia1[i] = make_float4(inv.m[1][0], inv.m[1][1], inv.m[1][2], 0.0f);
-  b.ia2[i] = make_float4(inv.m[2][0], inv.m[2][1], inv.m[2][2], 0.0f);
+  b.set_inv_inertia(i, inv, inv_mass);
   const f3 gyro = neg(cross(w, iw));
   const f3 acc_ang = mulv(inv, gyro);
   const f3 acc_lin = inv_mass != 0.0f ? f3{p.gx, p.gy, p.gz} : f3{0.0f, 0.0f, 0.0f};
@@ -76,16 +73,17 @@ __global__ void __launch_bounds__(256) k_integrate(DevBodies b, SimParams p) {
   if (b.status[i] != STATUS_DYNAMIC) return;
   const float4 v = b.linvel[i], w = b.angvel[i];
   const float4 el = b.ext_lin[i], ea = b.ext_ang[i];
-  const float4 dl = b.dv_lin[i], da = b.dv_ang[i];
+  f3 dl, da;
+  b.load_dv(i, dl, da);
   const f3 lv{(v.x + el.x) + dl.x, (v.y + el.y) + dl.y, (v.z + el.z) + dl.z};
   const f3 av{(w.x + ea.x) + da.x, (w.y + ea.y) + da.y, (w.z + ea.z) + da.z};
   b.linvel[i] = f4(lv, 0.0f);
   b.angvel[i] = f4(av, 0.0f);
-  f3 t = xyz(b.pos[i]);
-  q4 q = toq(b.rot[i]);
+  f3 t = xyz(b.pos(i));
+  q4 q = toq(b.rot(i));
   apply_displacement(t, q, scale(lv, p.dt), scale(av, p.dt));
-  b.pos[i] = f4(t, 0.0f);
-  b.rot[i] = fromq(q);
+  b.pos(i) = f4(t, 0.0f);
+  b.rot(i) = fromq(q);
 }
 
 }  // namespace

@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(256) k_loose_update(DevBodies b, int mode, uin
   if (mode != 0 && b.status[i] != STATUS_DYNAMIC) return;
   const int kind = b.shape[i];
   const f3 he = xyz(b.dims[i]);
-  aabb a = shape_aabb(kind, he, xyz(b.pos[i]), toq(b.rot[i]));
+  aabb a = shape_aabb(kind, he, xyz(b.pos(i)), toq(b.rot(i)));
   if (mode == 2) a = merged(shape_aabb(kind, he, xyz(b.pos0[i]), toq(b.rot0[i])), a);
   else a = merged(a, a);
   const aabb tight = loosened(a, b.qlim[i]);

@@ -47,8 +47,8 @@ __global__ void __launch_bounds__(256) k_np_pairs(DevBodies b, const int2* __res
   const int a = pr.x, c = pr.y;
   const float prediction = b.qlim[a] + b.qlim[c];
   Contact out[4];
-  const int nc = collide_pair(b.shape[a], xyz(b.dims[a]), xyz(b.pos[a]), toq(b.rot[a]), b.shape[c],
-                              xyz(b.dims[c]), xyz(b.pos[c]), toq(b.rot[c]), prediction, out);
+  const int nc = collide_pair(b.shape[a], xyz(b.dims[a]), xyz(b.pos(a)), toq(b.rot(a)), b.shape[c],
+                              xyz(b.dims[c]), xyz(b.pos(c)), toq(b.rot(c)), prediction, out);
   np.count[i] = uint32_t(nc);
   np.has[i] = nc > 0 ? 1u : 0u;
   for (int k = 0; k < nc; ++k) {

@@ -25,8 +25,8 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const DevContac
   const int k1 = b.shape[b1], k2 = b.shape[b2];
   const f3 h1 = xyz(b.dims[b1]), h2 = xyz(b.dims[b2]);
   const float m1 = b.margin[b1], m2 = b.margin[b2];
-  f3 t1 = xyz(b.pos[b1]), t2 = xyz(b.pos[b2]);
-  q4 q1 = toq(b.rot[b1]), q2 = toq(b.rot[b2]);
+  f3 t1 = xyz(b.pos(b1)), t2 = xyz(b.pos(b2));
+  q4 q1 = toq(b.rot(b1)), q2 = toq(b.rot(b2));
   float im1 = 0.0f, im2 = 0.0f;
   m33 I1, I2;
   if (d1) I1 = load_inv(b, b1, im1);
@@ -84,12 +84,12 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const DevContac
     }
   }
   if (moved1) {
-    b.pos[b1] = f4(t1, 0.0f);
-    b.rot[b1] = fromq(q1);
+    b.pos(b1) = f4(t1, 0.0f);
+    b.rot(b1) = fromq(q1);
   }
   if (moved2) {
-    b.pos[b2] = f4(t2, 0.0f);
-    b.rot[b2] = fromq(q2);
+    b.pos(b2) = f4(t2, 0.0f);
+    b.rot(b2) = fromq(q2);
   }
 }
 

@@ -27,9 +27,11 @@
 // rows (projected on l >= 0). Manifolds of one colour share no dynamic body, so one launch per
 // colour performs exactly the sequential Gauss-Seidel updates of the colour order.
 //
-// Row record (8 contiguous float4 per contact, colour order): 0 n|mu, 1..3 body-1 angular
-// jacobian of rows (n, t1, t2)|r, 4..6 body-2 angular jacobian|rhs, 7 impulses (n, t1, t2) |
-// 1 if the warm start came from the cache, else 0.
+// Row record (7 contiguous float4 per contact, colour order): 0 n|mu, 1..3 body-1 angular
+// jacobian of rows (n, t1, t2)|r, 4..6 body-2 angular jacobian|rhs. The impulses live in their
+// own array, one float4 per record (n, t1, t2 | 1 if the warm start came from the cache, else
+// 0), so a sweep's impulse writes are contiguous 16-byte runs rather than one dirty line per
+// contact.
 // Linear jacobians are the row direction d (body 1) and -d (body 2) weighted by the inverse
 // mass; tangents and I^-1 J_ang are recomputed in the sweep kernels from n and the bodies'
 // inverse inertia with the assembly's own expressions.
@@ -222,7 +224,7 @@ __device__ __forceinline__ void fill_geom(const BodyView& bv, f3 point, f3 dir, 
 
 __device__ __forceinline__ BodyView load_body(const DevBodies& b, int i) {
   BodyView v;
-  v.com = xyz(b.pos[i]);
+  v.com = xyz(b.pos(i));
   v.v = xyz(b.linvel[i]);
   v.w = xyz(b.angvel[i]);
   v.el = xyz(b.ext_lin[i]);
@@ -322,7 +324,7 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
       R[1 + row] = rA1[row];
       R[4 + row] = rA2[row];
     }
-    R[7] = make_float4(lam[0], lam[1], lam[2], has);
+    sb.lam[r] = make_float4(lam[0], lam[1], lam[2], has);
     sb.rec2c[r] = uint32_t(c);
   }
 }
@@ -383,9 +385,9 @@ struct CRec {
   float4 lam;
 };
 
-__device__ __forceinline__ void load_rec(const float4* __restrict__ R, uint32_t r, CRec& c) {
-  const float4* p = R + long(r) * kRecFields;
-  const float4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3], v4 = p[4], v5 = p[5], v6 = p[6], v7 = p[7];
+__device__ __forceinline__ void load_rec(const SolverBufs& sb, uint32_t r, CRec& c) {
+  const float4* p = sb.rec + long(r) * kRecFields;
+  const float4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3], v4 = p[4], v5 = p[5], v6 = p[6];
   c.n = xyz(v0);
   c.mu = v0.w;
   c.a1[0] = xyz(v1); c.r[0] = v1.w;
@@ -394,7 +396,7 @@ __device__ __forceinline__ void load_rec(const float4* __restrict__ R, uint32_t 
   c.a2[0] = xyz(v4); c.rhs[0] = v4.w;
   c.a2[1] = xyz(v5); c.rhs[1] = v5.w;
   c.a2[2] = xyz(v6); c.rhs[2] = v6.w;
-  c.lam = v7;
+  c.lam = sb.lam[r];
 }
 
 struct Inv {
@@ -419,7 +421,7 @@ __device__ __forceinline__ void load_manifold(const DevBodies& b, const SolverBu
 #pragma unroll
   for (int k = 0; k < 4; ++k)
     if (k < m.x.cnt) {
-      load_rec(sb.rec, m.x.r0 + k, m.c[k]);
+      load_rec(sb, m.x.r0 + k, m.c[k]);
       tangents(m.c[k].n, m.t1[k], m.t2[k]);
     }
 }
@@ -455,13 +457,13 @@ __device__ __forceinline__ void row_add(const Row& v, float alpha, const Manifol
 }
 
 __device__ __forceinline__ void load_dv(const DevBodies& b, const Manifold& m, Dv& v1, Dv& v2) {
-  if (m.x.d1) { v1.l = xyz(b.dv_lin[m.x.b1]); v1.a = xyz(b.dv_ang[m.x.b1]); }
-  if (m.x.d2) { v2.l = xyz(b.dv_lin[m.x.b2]); v2.a = xyz(b.dv_ang[m.x.b2]); }
+  if (m.x.d1) b.load_dv(m.x.b1, v1.l, v1.a);
+  if (m.x.d2) b.load_dv(m.x.b2, v2.l, v2.a);
 }
 
 __device__ __forceinline__ void store_dv(const DevBodies& b, const Manifold& m, const Dv& v1, const Dv& v2) {
-  if (m.x.d1) { b.dv_lin[m.x.b1] = f4(v1.l, 0.0f); b.dv_ang[m.x.b1] = f4(v1.a, 0.0f); }
-  if (m.x.d2) { b.dv_lin[m.x.b2] = f4(v2.l, 0.0f); b.dv_ang[m.x.b2] = f4(v2.a, 0.0f); }
+  if (m.x.d1) b.store_dv(m.x.b1, v1.l, v1.a);
+  if (m.x.d2) b.store_dv(m.x.b2, v2.l, v2.a);
 }
 
 __device__ __forceinline__ void colour_range(const SolverBufs& sb, int k, int& first, int& n) {
@@ -582,7 +584,7 @@ __device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, c
   acc.store(m, v1, v2);
 #pragma unroll
   for (int k = 0; k < 4; ++k)
-    if (k < m.x.cnt) sb.rec[long(m.x.r0 + k) * kRecFields + 7] = m.c[k].lam;
+    if (k < m.x.cnt) sb.lam[m.x.r0 + k] = m.c[k].lam;
 }
 
 // One colour of the warm start / of one sweep: one lane per manifold of the colour visited kv-th.
@@ -617,8 +619,10 @@ __global__ void __launch_bounds__(kBlock) k_sweeps_blocked(DevBodies b, SolverBu
   const int d0 = sc.dyn_start[s], nd = sc.dyn_start[s + 1] - d0;
   for (int i = threadIdx.x; i < nd; i += kBlock) {
     const int body = sc.dyn_list[d0 + i];
-    s_lin[i] = b.dv_lin[body];
-    s_ang[i] = b.dv_ang[body];
+    f3 l, a;
+    b.load_dv(body, l, a);
+    s_lin[i] = f4(l, 0.0f);
+    s_ang[i] = f4(a, 0.0f);
   }
   __syncthreads();
   const LdsDv acc{s_lin, s_ang, sc.local};
@@ -637,8 +641,7 @@ __global__ void __launch_bounds__(kBlock) k_sweeps_blocked(DevBodies b, SolverBu
   }
   for (int i = threadIdx.x; i < nd; i += kBlock) {
     const int body = sc.dyn_list[d0 + i];
-    b.dv_lin[body] = s_lin[i];
-    b.dv_ang[body] = s_ang[i];
+    b.store_dv(body, xyz(s_lin[i]), xyz(s_ang[i]));
   }
 }
 
@@ -741,7 +744,7 @@ __global__ void __launch_bounds__(256) k_residual(DevBodies b, DevContacts c, in
 __global__ void __launch_bounds__(256) k_cache(DevContacts c, SolverBufs sb, int C) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= C) return;
-  c.imp[sb.rec2c[r]] = sb.rec[long(r) * kRecFields + 7];
+  c.imp[sb.rec2c[r]] = sb.lam[r];
 }
 
 // ------------------------------------------------------------------ learned warm start
@@ -766,7 +769,7 @@ __global__ void __launch_bounds__(256) k_features(DevBodies b, DevContacts cur, 
   for (int k = 0; k < x.cnt; ++k) {
     const uint32_t r = x.r0 + k;
     CRec c;
-    load_rec(sb.rec, r, c);
+    load_rec(sb, r, c);
     const float depth = cur.w1[x.first + k].w + mm;
     const float lsc = c.r[0] * gdt;
     float* f = X + long(r) * 24;
@@ -806,7 +809,7 @@ __global__ void __launch_bounds__(256) k_apply_prediction(SolverBufs sb, int C, 
   if (r >= C) return;
   const float s = ls[r];
   const float mu = sb.rec[long(r) * kRecFields].w;
-  float4* L = sb.rec + long(r) * kRecFields + 7;
+  float4* L = sb.lam + r;
   const float4 c = *L;
   const float ln = fmaxf(c.x + y[3 * r] * s, 0.0f);
   const float lim = mu * ln;
@@ -820,7 +823,7 @@ __global__ void __launch_bounds__(256) k_sample_targets(SolverBufs sb, int C, co
                                                         float* __restrict__ t) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= C) return;
-  const float4 l = sb.rec[long(r) * kRecFields + 7];
+  const float4 l = sb.lam[r];
   const float s = ls[r];
   t[3 * r] = l.x / s;
   t[3 * r + 1] = l.y / s;
@@ -831,7 +834,7 @@ __global__ void __launch_bounds__(256) k_sample_targets(SolverBufs sb, int C, co
 __global__ void __launch_bounds__(256) k_lam_copy(SolverBufs sb, int C, float4* __restrict__ buf, int to_buf) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= C) return;
-  float4* L = sb.rec + long(r) * kRecFields + 7;
+  float4* L = sb.lam + r;
   if (to_buf) buf[r] = *L;
   else *L = buf[r];
 }

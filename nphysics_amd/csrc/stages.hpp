@@ -79,14 +79,15 @@ struct SolverBufs {
   uint32_t* soff;         // [M] first row record of each slot
   int4* sinfo;            // [M] per slot: body1|dyn1<<30, body2|dyn2<<30, first record, count|first<<3
   uint32_t* rec2c;        // [C] contact index of each row record
-  float4* rec;            // [8 rec_stride] row records, field f of record r at 8 r + f
+  float4* rec;            // [kRecFields rec_stride] row records, field f of record r at kRecFields r + f
+  float4* lam;            // [rec_stride] impulses of each row record (normal, tangent 1, tangent 2, has cache)
   long rec_stride;        // record capacity (fixed between reallocations)
   int* visit;             // [max colours] colour classes in visiting order
   double* red;            // residual partials
   uint32_t* tmp;
 };
 constexpr int kMaxColours = 4096;
-constexpr int kRecFields = 8;  // float4 per contact row record
+constexpr int kRecFields = 7;  // float4 per contact row record
 
 void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, const DevContacts& prev,
                            int prevM, const uint32_t* prev_state, SolverBufs& sb, hipStream_t s);

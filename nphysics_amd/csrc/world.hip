@@ -87,7 +87,9 @@ struct World {
 
   // device body arrays
   DBuf status, shape, scene, dims, margin, qlim, friction, restitution, mass, local_i;
-  DBuf pos, rot, pos0, rot0, linvel, angvel, ia0, ia1, ia2, ext_lin, ext_ang, dv_lin, dv_ang;
+  DBuf pose;  // kPoseBlock float4 per body: position, rotation
+  DBuf sol;   // kSolverBlock float4 per body: inverse inertia and mass, solver velocity increment
+  DBuf pos0, rot0, linvel, angvel, ext_lin, ext_ang;
   DBuf loose_mn, loose_mx;
 
   // broad phase
@@ -108,7 +110,7 @@ struct World {
   // solver
   DBuf sv_state2[2];  // colouring state of the manifolds of cb[k]
   DBuf sv_deg, sv_adj_off, sv_adj_fill, sv_adj, sv_ckeys, sv_order, sv_ckeys_alt,
-      sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_sinfo, sv_rec2c, sv_rec, sv_red, sv_tmp;
+      sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_sinfo, sv_rec2c, sv_rec, sv_lam, sv_red, sv_tmp;
   int ncolours = 0;
   long rec_stride = 0;
   int colour_rounds = 0;
@@ -228,19 +230,14 @@ struct World {
     b.restitution = restitution.as<float>();
     b.mass = mass.as<float>();
     b.local_i = local_i.as<float4>();
-    b.pos = pos.as<float4>();
-    b.rot = rot.as<float4>();
+    b.pose = pose.as<float4>();
+    b.sol = sol.as<float4>();
     b.pos0 = pos0.as<float4>();
     b.rot0 = rot0.as<float4>();
     b.linvel = linvel.as<float4>();
     b.angvel = angvel.as<float4>();
-    b.ia0 = ia0.as<float4>();
-    b.ia1 = ia1.as<float4>();
-    b.ia2 = ia2.as<float4>();
     b.ext_lin = ext_lin.as<float4>();
     b.ext_ang = ext_ang.as<float4>();
-    b.dv_lin = dv_lin.as<float4>();
-    b.dv_ang = dv_ang.as<float4>();
     b.loose_mn = loose_mn.as<float4>();
     b.loose_mx = loose_mx.as<float4>();
     return b;
@@ -322,9 +319,11 @@ struct World {
     }
     const size_t f4b = sizeof(float4);
     // grow the per-body device arrays, keeping the existing bodies' state
-    for (DBuf* b : {&pos, &rot, &linvel, &angvel, &loose_mn, &loose_mx})
+    for (DBuf* b : {&linvel, &angvel, &loose_mn, &loose_mx})
       HIPCK(b->ensure(size_t(n1) * f4b, true, stream));
-    for (DBuf* b : {&pos0, &rot0, &ia0, &ia1, &ia2, &ext_lin, &ext_ang, &dv_lin, &dv_ang, &dims, &local_i})
+    HIPCK(pose.ensure(size_t(n1) * kPoseBlock * f4b, true, stream));
+    HIPCK(sol.ensure(size_t(n1) * kSolverBlock * f4b));
+    for (DBuf* b : {&pos0, &rot0, &ext_lin, &ext_ang, &dims, &local_i})
       HIPCK(b->ensure(size_t(n1) * f4b));
     for (DBuf* b : {&status, &shape, &scene, &margin, &qlim, &friction, &restitution, &mass})
       HIPCK(b->ensure(size_t(n1) * 4));
@@ -346,8 +345,9 @@ struct World {
     HIPCK(up(qlim, h_qlim.data(), size_t(n1) * 4, 0));
     HIPCK(up(friction, h_friction.data(), size_t(n1) * 4, 0));
     HIPCK(up(restitution, h_restitution.data(), size_t(n1) * 4, 0));
-    HIPCK(up(pos, hp.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
-    HIPCK(up(rot, hr.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
+    HIPCK(hipMemsetAsync(sol.p, 0, size_t(n1) * kSolverBlock * f4b, stream));
+    HIPCK(pose_copy(0, hp.data(), n0, d.n, hipMemcpyHostToDevice));
+    HIPCK(pose_copy(1, hr.data(), n0, d.n, hipMemcpyHostToDevice));
     HIPCK(up(linvel, hv.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
     HIPCK(up(angvel, hw.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
     HIPCK(up(loose_mn, hmn.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
@@ -402,12 +402,23 @@ struct World {
     return NPHYS_OK;
   }
 
+  // Copies the positions (slot 0) or rotations (slot 1) of `count` pose blocks from body `first`
+  // on, to or from a packed host array of float4.
+  hipError_t pose_copy(int slot, float4* host, int first, int count, hipMemcpyKind kind) {
+    if (count <= 0) return hipSuccess;
+    const size_t pitch = kPoseBlock * sizeof(float4), w = sizeof(float4);
+    float4* dev = pose.as<float4>() + size_t(first) * kPoseBlock + slot;
+    if (kind == hipMemcpyHostToDevice)
+      return hipMemcpy2DAsync(dev, pitch, host, w, w, size_t(count), kind, stream);
+    return hipMemcpy2DAsync(host, w, dev, pitch, w, size_t(count), kind, stream);
+  }
+
   int set_state(const float* p, const float* r, const float* lv, const float* av) {
     if (n == 0) return NPHYS_OK;
     std::vector<float4> hp(n), hr(n), hv(n), hw(n);
     const size_t f4b = sizeof(float4);
-    HIPCK(hipMemcpyAsync(hp.data(), pos.p, n * f4b, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipMemcpyAsync(hr.data(), rot.p, n * f4b, hipMemcpyDeviceToHost, stream));
+    HIPCK(pose_copy(0, hp.data(), 0, n, hipMemcpyDeviceToHost));
+    HIPCK(pose_copy(1, hr.data(), 0, n, hipMemcpyDeviceToHost));
     HIPCK(hipMemcpyAsync(hv.data(), linvel.p, n * f4b, hipMemcpyDeviceToHost, stream));
     HIPCK(hipMemcpyAsync(hw.data(), angvel.p, n * f4b, hipMemcpyDeviceToHost, stream));
     HIPCK(hipStreamSynchronize(stream));
@@ -419,8 +430,8 @@ struct World {
         if (av) hw[i] = make_float4(av[3 * i], av[3 * i + 1], av[3 * i + 2], 0.0f);
       }
     }
-    HIPCK(hipMemcpyAsync(pos.p, hp.data(), n * f4b, hipMemcpyHostToDevice, stream));
-    HIPCK(hipMemcpyAsync(rot.p, hr.data(), n * f4b, hipMemcpyHostToDevice, stream));
+    HIPCK(pose_copy(0, hp.data(), 0, n, hipMemcpyHostToDevice));
+    HIPCK(pose_copy(1, hr.data(), 0, n, hipMemcpyHostToDevice));
     HIPCK(hipMemcpyAsync(linvel.p, hv.data(), n * f4b, hipMemcpyHostToDevice, stream));
     HIPCK(hipMemcpyAsync(angvel.p, hw.data(), n * f4b, hipMemcpyHostToDevice, stream));
     HIPCK(hipStreamSynchronize(stream));
@@ -432,11 +443,14 @@ struct World {
     if (n == 0) return NPHYS_OK;
     std::vector<float4> h(n);
     const size_t f4b = sizeof(float4);
-    struct Item { DBuf* b; float* out; int w; };
-    const Item items[4] = {{&pos, p, 3}, {&rot, r, 4}, {&linvel, lv, 3}, {&angvel, av, 3}};
+    struct Item { DBuf* b; int slot; float* out; int w; };
+    const Item items[4] = {{nullptr, 0, p, 3}, {nullptr, 1, r, 4}, {&linvel, 0, lv, 3}, {&angvel, 0, av, 3}};
     for (const Item& it : items) {
       if (!it.out) continue;
-      HIPCK(hipMemcpyAsync(h.data(), it.b->p, n * f4b, hipMemcpyDeviceToHost, stream));
+      if (it.b)
+        HIPCK(hipMemcpyAsync(h.data(), it.b->p, n * f4b, hipMemcpyDeviceToHost, stream));
+      else
+        HIPCK(pose_copy(it.slot, h.data(), 0, n, hipMemcpyDeviceToHost));
       HIPCK(hipStreamSynchronize(stream));
       for (int i = 0; i < n; ++i) {
         it.out[it.w * i + 0] = h[i].x;
@@ -595,7 +609,8 @@ struct World {
     HIPCK(sv_sinfo.ensure(m * sizeof(int4)));
     HIPCK(sv_rec2c.ensure(c * 4));
     HIPCK(sv_rec.ensure(c * kRecFields * sizeof(float4)));
-    rec_stride = long(sv_rec.bytes / (kRecFields * sizeof(float4)));
+    HIPCK(sv_lam.ensure(c * sizeof(float4)));
+    rec_stride = long(std::min(sv_rec.bytes / (kRecFields * sizeof(float4)), sv_lam.bytes / sizeof(float4)));
     HIPCK(sv_red.ensure(size_t(residual_blocks(M)) * 2 * sizeof(double)));
     const size_t tw = std::max({radix_temp_words(int(m)), scan_temp_words(int(m)), scan_temp_words(int(nn))});
     HIPCK(sv_tmp.ensure(tw * 4 + 64));
@@ -622,6 +637,7 @@ struct World {
     v.visit = sv_visit.as<int>();
     v.rec2c = sv_rec2c.as<uint32_t>();
     v.rec = sv_rec.as<float4>();
+    v.lam = sv_lam.as<float4>();
     v.red = sv_red.as<double>();
     v.tmp = sv_tmp.as<uint32_t>();
     return v;
@@ -977,19 +993,17 @@ struct World {
       if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, n_plain))) return rc;
       const double rinf_p = res_inf, rrms_p = res_rms;
       launch_lam_copy(v, C, jd_lam.as<float4>(), true, stream);
-      HIPCK(hipMemcpyAsync(jd_dv.p, dv_lin.p, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-      HIPCK(hipMemcpyAsync(jd_dv.as<float4>() + n, dv_ang.p, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice,
-                           stream));
-      HIPCK(hipMemsetAsync(dv_lin.p, 0, size_t(n) * sizeof(float4), stream));
-      HIPCK(hipMemsetAsync(dv_ang.p, 0, size_t(n) * sizeof(float4), stream));
+      // the velocity increment is the last 24 bytes of each 64-byte solver block
+      char* dv0 = sol.as<char>() + 10 * sizeof(float);
+      const size_t pitch = kSolverBlock * sizeof(float4), dvb = 6 * sizeof(float);
+      HIPCK(hipMemcpy2DAsync(jd_dv.p, dvb, dv0, pitch, dvb, size_t(n), hipMemcpyDeviceToDevice, stream));
+      HIPCK(hipMemset2DAsync(dv0, pitch, 0, dvb, size_t(n), stream));
       if ((rc = predict(v, C))) return rc;
       if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, n_learned))) return rc;
       const bool keep = n_learned < n_plain;
       if (!keep) {
         launch_lam_copy(v, C, jd_lam.as<float4>(), false, stream);
-        HIPCK(hipMemcpyAsync(dv_lin.p, jd_dv.p, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice, stream));
-        HIPCK(hipMemcpyAsync(dv_ang.p, jd_dv.as<float4>() + n, size_t(n) * sizeof(float4), hipMemcpyDeviceToDevice,
-                             stream));
+        HIPCK(hipMemcpy2DAsync(dv0, pitch, jd_dv.p, dvb, dvb, size_t(n), hipMemcpyDeviceToDevice, stream));
         res_inf = rinf_p;
         res_rms = rrms_p;
       }
