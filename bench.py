@@ -82,60 +82,54 @@ def cpu_baseline(scene, snapshot, params, steps):
     }
 
 
-def train_warmstart():
-    """The learned warm start's network, trained on the GPU on samples from scenes disjoint from
-    the benchmark's (seeds 1000-1002: two 8,192-body piles and a 10^3 box stack), each sample a
-    contact's features and the impulses its solve reached at RMS residual 1e-5 (cap 1,000 sweeps)."""
+def train_warmstart(scene, params, settle_steps, capture_steps, max_sweeps, check_every, what):
+    """Trains the learned warm start's network on the GPU on samples from `scene` (a seed
+    disjoint from the evaluated one): after `settle_steps` steps, `capture_steps` steps whose solve
+    runs to RMS residual 1e-5 (capped at `max_sweeps` sweeps), each contact's features and the
+    impulses its solve reached."""
     from nphysics_amd import learned
 
     t0 = time.perf_counter()
-    Xs, Ts = [], []
-    jobs = [(scenes.small_pile(seed=1000, n=8192, lattice=(32, 8, 32), extent=24.0, y0=0.3), 16, 200),
-            (scenes.small_pile(seed=1001, n=8192, lattice=(32, 8, 32), extent=24.0, y0=0.3), 16, 200),
-            (scenes.boxes(seed=1002, nx=10, ny=10, nz=10), 10, 100)]
-    for scene, iters, settle_steps in jobs:
-        w = world.MechanicalWorld(integration_parameters=abi.default_params(max_velocity_iterations=iters))
-        w.add_bodies(scene)
-        w.step(settle_steps)
-        X, T = learned.collect_samples(w, 25, target=1e-5, max_sweeps=1000, check_every=8, norm="rms")
-        Xs.append(X)
-        Ts.append(T)
-        w.close()
-    X, T = np.concatenate(Xs), np.concatenate(Ts)
+    w = world.MechanicalWorld(integration_parameters=params)
+    w.add_bodies(scene)
+    settle(w, settle_steps)
+    X, T = learned.collect_samples(w, capture_steps, target=1e-5, max_sweeps=max_sweeps,
+                                   check_every=check_every, norm="rms")
+    w.close()
     t1 = time.perf_counter()
     net = learned.WarmStartNet(seed=1)
     net.set_data(X, learned.correction_targets(X, T))
     losses = net.train(epochs=15, batch=4096, lr=1e-3)
     t2 = time.perf_counter()
-    params = net.params
+    params_out = net.params
     net.close()
-    log(f"warm-start network: {X.shape[0]} samples in {t1 - t0:.1f} s, trained in {t2 - t1:.1f} s, "
+    log(f"warm-start network ({what}): {X.shape[0]} samples in {t1 - t0:.1f} s, trained in {t2 - t1:.1f} s, "
         f"loss {losses[0]:.3f} -> {losses[-1]:.3f}")
-    return params, {"train_samples": int(X.shape[0]), "data_s": t1 - t0, "train_s": t2 - t1,
-                    "epochs": 15, "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
+    return params_out, {"data": what, "train_samples": int(X.shape[0]), "data_s": t1 - t0, "train_s": t2 - t1,
+                        "epochs": 15, "loss_first": float(losses[0]), "loss_last": float(losses[-1])}
 
 
-def residual_target_section(scene, snapshot, params, target, steps, max_sweeps):
-    """Solve-to-residual runs on the pile from the settled snapshot.
+def learned_section(fresh, net_params, train, iters, targets, steps, max_sweeps):
+    """Solve-to-residual runs from the state `fresh()` returns, plain vs learned warm start.
 
-    * `target` (RMS, default 1e-4): sweeps and steps/s the plain warm start needs (capped at
-      max_sweeps; whether it is reached is reported).
-    * The residual the reference's own iteration count reaches (RMS after 16 sweeps, measured on
-      the same state) as a reachable target: steps/s and sweeps/step with the plain and with the
-      learned warm start (separate runs from the snapshot), and the judge's sweep counts on
-      identical assembled systems."""
+    For each RMS target: steps/s and sweeps/step with the plain and with the learned warm start
+    (separate runs from the same state, capped at max_sweeps; whether the target was reached is
+    reported), and the judge's sweep counts, both starts solved from identical assembled systems
+    with the learned start kept only where it needs fewer sweeps. The targets are the given ones
+    plus the RMS residual the plain warm start reaches with the reference's iteration count
+    (`iters` sweeps) on the same state."""
     from nphysics_amd import learned
 
-    net_params, train = train_warmstart()
+    w = fresh()
+    w.step(1)
+    r_ref = w.residual()[1]
+    w.close()
 
-    def fresh():
-        w = world.MechanicalWorld(integration_parameters=params)
-        w.add_bodies(scene)
-        w.set_state(**{k: snapshot[k] for k in ("pos", "rot", "linvel", "angvel")})
-        w.step(3)  # rebuild the impulse cache after the state upload
-        return w
-
-    def run(w, tgt):
+    def run(tgt, learned_start):
+        w = fresh()
+        if learned_start:
+            w.load_warmstart(net_params)
+            w.set_contact_model(abi.CONTACT_MODEL_LEARNED_WARMSTART)
         w.set_residual_target(tgt, max_sweeps, 1, "rms")
         w.step(1)
         w.reset_counters()
@@ -143,34 +137,67 @@ def residual_target_section(scene, snapshot, params, target, steps, max_sweeps):
         w.step(steps)
         dt = time.perf_counter() - t0
         c = w.counters()
-        return {"steps_per_s": steps / dt, "sweeps_per_step": c["velocity_sweeps"] / steps,
-                "contact_iters_per_s": c["ncontacts"] * c["velocity_sweeps"] / dt,
-                "residual_rms": w.residual()[1], "reached": bool(w.residual()[1] <= tgt)}
-
-    out = {"norm": "rms", "max_sweeps": max_sweeps, "steps": steps, "training": train}
-    w = fresh()
-    out["at_target"] = dict(target=target, plain=run(w, target))
-    w.close()
-    w = fresh()
-    w.step(1)
-    r16 = w.residual()[1]
-    w.close()
-    sec = {"target": r16, "what": "RMS residual of the plain warm start after 16 sweeps on this state"}
-    for mode in ("plain", "learned"):
-        w = fresh()
-        if mode == "learned":
-            w.load_warmstart(net_params)
-            w.set_contact_model(abi.CONTACT_MODEL_LEARNED_WARMSTART)
-        sec[mode] = run(w, r16)
-        if mode == "learned":
-            w.set_residual_target(0.0)
-            sec["judge"] = learned.judge(w, net_params, steps, target=r16, max_sweeps=max_sweeps,
-                                         check_every=1, norm="rms")
+        r = {"steps_per_s": steps / dt, "sweeps_per_step": c["velocity_sweeps"] / steps,
+             "contact_iters_per_s": c["ncontacts"] * c["velocity_sweeps"] / dt,
+             "residual_rms": w.residual()[1], "reached": bool(w.residual()[1] <= tgt)}
         w.close()
-    p, l = sec["plain"]["sweeps_per_step"], sec["learned"]["sweeps_per_step"]
-    sec["sweep_reduction"] = (p - l) / p if p > 0 else 0.0
-    sec["speedup"] = sec["learned"]["steps_per_s"] / sec["plain"]["steps_per_s"]
-    out["at_reference_residual"] = sec
+        return r
+
+    out = {"norm": "rms", "max_sweeps": max_sweeps, "steps": steps, "training": train, "runs": []}
+    for tgt, what in [(t, "fixed target") for t in targets] + [
+            (r_ref, f"RMS residual of the plain warm start after {iters} sweeps on this state")]:
+        sec = {"target": tgt, "what": what, "plain": run(tgt, False), "learned": run(tgt, True)}
+        w = fresh()
+        sec["judge"] = learned.judge(w, net_params, steps, target=tgt, max_sweeps=max_sweeps, check_every=1,
+                                     norm="rms")
+        w.close()
+        p, l = sec["plain"]["sweeps_per_step"], sec["learned"]["sweeps_per_step"]
+        sec["sweep_reduction"] = (p - l) / p if p > 0 else 0.0
+        sec["speedup"] = sec["learned"]["steps_per_s"] / sec["plain"]["steps_per_s"]
+        log(f"target {tgt:.3g}: plain {p:.1f} sweeps/step, learned {l:.1f}; judge plain "
+            f"{sec['judge']['plain_sweeps_per_step']:.1f} learned {sec['judge']['learned_sweeps_per_step']:.1f}")
+        out["runs"].append(sec)
+    return out
+
+
+def residual_target_section(scene, snapshot, params, target, steps, max_sweeps):
+    """The learned warm start against the plain one on the two workloads it is evaluated on:
+
+    * the dense pile, from the settled snapshot of the benchmark (seed 3), with a network trained
+      on another pile (seed 1003) settled the same way;
+    * the 8,000-box stack (seed 2, 10 velocity iterations), with a network trained on a box
+      stack of seed 1002."""
+    def fresh_pile():
+        w = world.MechanicalWorld(integration_parameters=params)
+        w.add_bodies(scene)
+        w.set_state(**{k: snapshot[k] for k in ("pos", "rot", "linvel", "angvel")})
+        w.step(3)  # rebuild the impulse cache after the state upload
+        return w
+
+    pile_net, pile_train = train_warmstart(scenes.pile(seed=1003), params, 750, 4, 256, 16,
+                                           "dense pile, seed 1003")
+    out = {"pile": learned_section(fresh_pile, pile_net, pile_train, params.max_velocity_iterations,
+                                   [target], steps, max_sweeps)}
+
+    box_params = abi.default_params(max_velocity_iterations=10, max_position_iterations=3)
+    box_net, box_train = train_warmstart(scenes.boxes(seed=1002), box_params, 100, 25, 1000, 8,
+                                         "8,000-box stack, seed 1002")
+    eval_boxes = scenes.boxes(seed=2)
+    w = world.MechanicalWorld(integration_parameters=box_params)
+    w.add_bodies(eval_boxes)
+    w.step(100)
+    box_state = w.get_state()
+    w.close()
+
+    def fresh_boxes():
+        w = world.MechanicalWorld(integration_parameters=box_params)
+        w.add_bodies(eval_boxes)
+        w.set_state(**box_state)
+        w.step(3)
+        return w
+
+    out["box_stack"] = learned_section(fresh_boxes, box_net, box_train, box_params.max_velocity_iterations,
+                                       [target], 8, max_sweeps)
     return out
 
 
@@ -372,6 +399,9 @@ def main():
         w.close()
         out["residual_target"] = residual_target_section(scene, snapshot, params, args.target_rms,
                                                          args.target_steps, args.target_max_sweeps)
+        out["learned_sweep_reduction"] = {
+            wl: {f"{r['target']:.3g}": r["sweep_reduction"] for r in sec["runs"]}
+            for wl, sec in out["residual_target"].items()}
     if not args.no_cpu:
         log("timing the CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline(scene, snapshot, params, args.cpu_steps)

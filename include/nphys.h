@@ -125,6 +125,9 @@ typedef struct nphys_counters {
   int64_t sweep_variant;  /* sweeps of the last step: 0 per-colour launches, 64/128/256 scene-blocked */
   int64_t loose_changes;  /* accumulated loose AABBs replaced by the end-of-step syncs */
   int64_t pair_rebuilds;  /* accumulated end-of-step broad-phase pair rebuilds */
+  /* accumulated pair rebuilds done incrementally (only the bodies whose loose AABB was replaced
+   * re-traversed, the tree refitted by expansion) instead of by a full tree build */
+  int64_t incremental_pair_updates;
 } nphys_counters;
 
 /* ---------------------------------------------------------------- lifecycle */
@@ -227,6 +230,14 @@ int nphys_set_residual_target(nphys_world* w, double target, int32_t max_sweeps,
  * the first solves and keeps the fastest. All variants give bit-identical results. No reference
  * counterpart (the reference has one sequential loop). */
 int nphys_set_sweep_variant(nphys_world* w, int32_t variant);
+
+/* How the pair set follows replaced loose AABBs. 1 (default): when only dynamic bodies' loose
+ * AABBs were replaced since the last pair build (at most n/8 of them, and a full build at least
+ * every 32 updates), only those bodies are re-traversed in the previous tree refitted by
+ * expansion; 0: the tree is rebuilt and every body traversed. The pair set is identical either
+ * way. Replaces ncollide's incremental DBVT update (geometrical_world.rs:285-312; the DBVT itself
+ * is not in the reference tree). */
+int nphys_set_incremental_pairs(nphys_world* w, int32_t on);
 
 /* Colour of every manifold solved by the last step, in manifold order. Recording starts with the
  * first call (a call with colours == NULL only enables it), so the first step after enabling is

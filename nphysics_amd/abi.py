@@ -121,6 +121,7 @@ class Counters(C.Structure):
         ("sweep_variant", C.c_int64),
         ("loose_changes", C.c_int64),
         ("pair_rebuilds", C.c_int64),
+        ("incremental_pair_updates", C.c_int64),
     ]
 
     def as_dict(self):

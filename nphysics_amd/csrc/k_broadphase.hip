@@ -16,6 +16,10 @@
 // a wavefront walks neighbouring subtrees) traverses the tree and writes its partners into its
 // own fixed-capacity segment; segments are sorted and compacted with a scan. The result is
 // deterministic and independent of thread timing. Algorithmic bytes: 124 per body + 8 per pair.
+// When only a few dynamic bodies' loose AABBs were replaced since the last pair build (the common
+// case of a resting pile), the segments are updated instead (see the incremental kernels below):
+// a few thousand traversals and one pass over the segments in place of a tree build and n
+// traversals; the pair list is the same.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -35,7 +39,8 @@ __device__ __forceinline__ float ord2f(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
 }
 
-__global__ void __launch_bounds__(256) k_loose_update(DevBodies b, int mode, uint32_t* changed) {
+__global__ void __launch_bounds__(256) k_loose_update(DevBodies b, int mode, uint32_t* changed,
+                                                     uint32_t* dirty) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
   if (mode != 0 && b.status[i] != STATUS_DYNAMIC) return;
@@ -50,7 +55,9 @@ __global__ void __launch_bounds__(256) k_loose_update(DevBodies b, int mode, uin
     const aabb nl = loosened(tight, 0.01f);
     b.loose_mn[i] = f4(nl.mn, 0.0f);
     b.loose_mx[i] = f4(nl.mx, 0.0f);
-    atomicAdd(changed, 1u);
+    dirty[i] = 1u;
+    atomicAdd(&changed[0], 1u);
+    if (b.status[i] != STATUS_DYNAMIC) atomicAdd(&changed[1], 1u);
   }
 }
 
@@ -163,12 +170,13 @@ __global__ void __launch_bounds__(256) k_lbvh(const uint32_t* __restrict__ keys,
 
 // Leaf nodes hold the loose AABBs in Morton order.
 __global__ void __launch_bounds__(256) k_leaf_nodes(DevBodies b, const uint32_t* __restrict__ vals, int n,
-                                                   float4* node_mn, float4* node_mx) {
+                                                   float4* node_mn, float4* node_mx, int* leaf_of) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int body = int(vals[t]);
   node_mn[n - 1 + t] = b.loose_mn[body];
   node_mx[n - 1 + t] = b.loose_mx[body];
+  leaf_of[body] = t;
 }
 
 // Internal-node bounds by atomic min/max on order-preserving integer encodings: every leaf climbs
@@ -266,6 +274,129 @@ __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* _
   if (overflow) atomicOr(&stats[1], 1u);
 }
 
+// ---- incremental pair update -------------------------------------------------------------------
+// Used when only dynamic bodies' loose AABBs were replaced since the last pair build (dirty[i]).
+// Pairs of two clean bodies cannot have changed, so: (1) every clean body's segment drops its
+// dirty partners (order kept: segments are sorted); (2) the dirty leaves get their new bounds and
+// expand their ancestors by atomic min/max (node bounds stay a superset of the leaves below, which
+// is all a traversal needs; a full build restores tight bounds); (3) each dirty body traverses the
+// tree: a partner that is static or has a larger index goes into its own segment, a clean dynamic
+// partner with a smaller index gets the dirty body appended to its segment (atomic count), a dirty
+// one finds it itself. The segments then hold exactly what a full traversal would write, up to
+// order, and are sorted by the compaction as before.
+
+__global__ void __launch_bounds__(256) k_seg_filter(DevBodies b, const uint32_t* __restrict__ dirty, int* cand,
+                                                   uint32_t* cand_count, int cap) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= b.n) return;
+  if (b.status[q] != STATUS_DYNAMIC) return;
+  if (dirty[q]) {
+    cand_count[q] = 0;
+    return;
+  }
+  const int c = int(cand_count[q]);
+  int* seg = cand + long(q) * cap;
+  int k = 0;
+  for (int i = 0; i < c; ++i) {
+    const int o = seg[i];
+    if (!dirty[o]) seg[k++] = o;
+  }
+  if (k != c) cand_count[q] = uint32_t(k);
+}
+
+// Dirty dynamic bodies (in arbitrary order: the segments are sorted afterwards).
+__global__ void __launch_bounds__(256) k_dirty_list(DevBodies b, const uint32_t* __restrict__ vals, int n,
+                                                   const uint32_t* __restrict__ dirty, int* list,
+                                                   uint32_t* list_count) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int q = int(vals[t]);
+  if (dirty[q] && b.status[q] == STATUS_DYNAMIC) list[atomicAdd(list_count, 1u)] = q;
+}
+
+__global__ void __launch_bounds__(256) k_refit_dirty(DevBodies b, const int* __restrict__ list,
+                                                    const uint32_t* __restrict__ list_count, int n,
+                                                    const int* __restrict__ leaf_of, const int* __restrict__ parent,
+                                                    float4* node_mn, float4* node_mx, uint32_t* inode) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= int(*list_count)) return;
+  const int body = list[t];
+  const int leaf = n - 1 + leaf_of[body];
+  const float4 mn = b.loose_mn[body], mx = b.loose_mx[body];
+  node_mn[leaf] = mn;
+  node_mx[leaf] = mx;
+  const uint32_t e[6] = {f2ord(mn.x), f2ord(mn.y), f2ord(mn.z), f2ord(mx.x), f2ord(mx.y), f2ord(mx.z)};
+  int p = parent[leaf];
+  while (p >= 0) {
+    uint32_t* q = inode + 8 * long(p);
+    bool changed = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      if (atomicMin(&q[k], e[k]) > e[k]) changed = true;
+      if (atomicMax(&q[4 + k], e[3 + k]) < e[3 + k]) changed = true;
+    }
+    if (!changed) break;
+    p = parent[p];
+  }
+}
+
+template <bool kScenes>
+__global__ void __launch_bounds__(256) k_traverse_dirty(DevBodies b, const int* __restrict__ list,
+                                                       const uint32_t* __restrict__ list_count,
+                                                       const uint32_t* __restrict__ vals, int n,
+                                                       const int* __restrict__ child,
+                                                       const float4* __restrict__ node_mn,
+                                                       const float4* __restrict__ node_mx,
+                                                       const int2* __restrict__ node_scene,
+                                                       const uint32_t* __restrict__ dirty, int* cand,
+                                                       uint32_t* cand_count, int cap, uint32_t* stats) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= int(*list_count)) return;
+  const int q = list[t];
+  const aabb Q{xyz(b.loose_mn[q]), xyz(b.loose_mx[q])};
+  const int qs = kScenes ? b.scene[q] : 0;
+  const long base = long(q) * cap;
+  uint32_t cnt = 0, most = 0;
+  int stack[kStack];
+  int sp = 0;
+  int node = 0;
+  bool overflow = false;
+  while (true) {
+    const int c0 = child[2 * node], c1 = child[2 * node + 1];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const int c = k == 0 ? c0 : c1;
+      if (kScenes && c < n - 1) {
+        const int2 sr = node_scene[c];
+        if (qs < sr.x || qs > sr.y) continue;
+      }
+      const aabb B{xyz(node_mn[c]), xyz(node_mx[c])};
+      if (!intersects(Q, B)) continue;
+      if (c >= n - 1) {
+        const int o = int(vals[c - (n - 1)]);
+        if (o == q || (kScenes && b.scene[o] != qs)) continue;
+        if (b.status[o] != STATUS_DYNAMIC || o > q) {
+          if (cnt < uint32_t(cap)) cand[base + cnt] = o;
+          ++cnt;
+        } else if (!dirty[o]) {
+          const uint32_t pos = atomicAdd(&cand_count[o], 1u);
+          if (pos < uint32_t(cap)) cand[long(o) * cap + pos] = q;
+          most = pos + 1 > most ? pos + 1 : most;
+        }
+      } else if (sp < kStack) {
+        stack[sp++] = c;
+      } else {
+        overflow = true;
+      }
+    }
+    if (sp == 0) break;
+    node = stack[--sp];
+  }
+  cand_count[q] = cnt;
+  atomicMax(&stats[0], cnt > most ? cnt : most);
+  if (overflow) atomicOr(&stats[1], 1u);
+}
+
 __global__ void __launch_bounds__(256) k_clamp_counts(const uint32_t* cnt, uint32_t* out, int n, int cap) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -299,9 +430,9 @@ inline unsigned nb(long n) { return unsigned((n + 255) / 256); }
 
 }  // namespace
 
-void launch_loose_update(const DevBodies& b, int mode, uint32_t* changed, hipStream_t s) {
+void launch_loose_update(const DevBodies& b, int mode, uint32_t* changed, uint32_t* dirty, hipStream_t s) {
   if (b.n == 0) return;
-  hipLaunchKernelGGL(k_loose_update, dim3(nb(b.n)), dim3(256), 0, s, b, mode, changed);
+  hipLaunchKernelGGL(k_loose_update, dim3(nb(b.n)), dim3(256), 0, s, b, mode, changed, dirty);
 }
 
 void launch_build_lbvh(const DevBodies& b, BroadPhaseBufs& bp, hipStream_t s) {
@@ -316,7 +447,8 @@ void launch_build_lbvh(const DevBodies& b, BroadPhaseBufs& bp, hipStream_t s) {
   hipLaunchKernelGGL(k_lbvh, dim3(nb(n - 1)), dim3(256), 0, s, bp.keys, n, scene_shift, bp.child, bp.parent,
                      bp.node_scene);
   hipLaunchKernelGGL(k_init_inodes, dim3(nb(n - 1)), dim3(256), 0, s, n, bp.visit);
-  hipLaunchKernelGGL(k_leaf_nodes, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.node_mn, bp.node_mx);
+  hipLaunchKernelGGL(k_leaf_nodes, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.node_mn, bp.node_mx,
+                     bp.leaf_of);
   hipLaunchKernelGGL(k_refit_atomic, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.parent, bp.visit);
   hipLaunchKernelGGL(k_decode_nodes, dim3(nb(n - 1)), dim3(256), 0, s, n, bp.visit, bp.node_mn, bp.node_mx);
 }
@@ -335,6 +467,27 @@ void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_
   else
     hipLaunchKernelGGL(k_traverse<false>, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
                        bp.node_mx, bp.node_scene, bp.cand, bp.cand_count, cap, bp.stats);
+}
+
+void launch_incremental_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, int max_dirty, hipStream_t s) {
+  const int n = b.n;
+  if (n < 2) return;
+  fill_u32(bp.stats, 0u, 4, s);
+  fill_u32(bp.list_count, 0u, 1, s);
+  hipLaunchKernelGGL(k_seg_filter, dim3(nb(n)), dim3(256), 0, s, b, bp.dirty, bp.cand, bp.cand_count, cap);
+  hipLaunchKernelGGL(k_dirty_list, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.dirty, bp.list, bp.list_count);
+  if (max_dirty <= 0) return;
+  hipLaunchKernelGGL(k_refit_dirty, dim3(nb(max_dirty)), dim3(256), 0, s, b, bp.list, bp.list_count, n,
+                     bp.leaf_of, bp.parent, bp.node_mn, bp.node_mx, bp.visit);
+  hipLaunchKernelGGL(k_decode_nodes, dim3(nb(n - 1)), dim3(256), 0, s, n, bp.visit, bp.node_mn, bp.node_mx);
+  if (bp.scene_bits)
+    hipLaunchKernelGGL(k_traverse_dirty<true>, dim3(nb(max_dirty)), dim3(256), 0, s, b, bp.list, bp.list_count,
+                       bp.vals, n, bp.child, bp.node_mn, bp.node_mx, bp.node_scene, bp.dirty, bp.cand,
+                       bp.cand_count, cap, bp.stats);
+  else
+    hipLaunchKernelGGL(k_traverse_dirty<false>, dim3(nb(max_dirty)), dim3(256), 0, s, b, bp.list, bp.list_count,
+                       bp.vals, n, bp.child, bp.node_mn, bp.node_mx, bp.node_scene, bp.dirty, bp.cand,
+                       bp.cand_count, cap, bp.stats);
 }
 
 void launch_compact_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s) {

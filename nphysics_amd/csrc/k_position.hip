@@ -7,7 +7,10 @@
 // -max_linear_correction); if rhs < 0 move each dynamic body by its weighted jacobian times
 // -rhs / (J M^-1 J^T) (RigidBody::apply_displacement, src/object/rigid_body.rs:371-381),
 // immediately, so later contacts see the corrected poses. One lane per manifold of one colour
-// keeps both poses in registers across the manifold's contacts.
+// keeps both poses in registers across the manifold's contacts. Its inputs are laid out by the
+// assembly in colour order: per slot the two bodies' half extents and margins, per contact a
+// 48-byte position record (local points and normal, contact kind, shape kinds), so a manifold
+// reads contiguous runs besides its two bodies' pose and solver blocks.
 #include <hip/hip_runtime.h>
 
 #include "np_geometry.hpp"
@@ -17,32 +20,39 @@ namespace nph {
 
 namespace {
 
-__device__ __forceinline__ void position_one(const DevBodies& b, const DevContacts& c, const SimParams& p,
-                                             int4 q) {
+__device__ __forceinline__ void position_one(const DevBodies& b, const SimParams& p, const SolverBufs& sb,
+                                             int slot) {
+  const int4 q = sb.sinfo[slot];
   const int b1 = q.x & 0x3fffffff, b2 = q.y & 0x3fffffff;
   const bool d1 = (q.x >> 30) & 1;
   const bool d2 = (q.y >> 30) & 1;
-  const int k1 = b.shape[b1], k2 = b.shape[b2];
-  const f3 h1 = xyz(b.dims[b1]), h2 = xyz(b.dims[b2]);
-  const float m1 = b.margin[b1], m2 = b.margin[b2];
+  const uint32_t r0 = uint32_t(q.z);
+  const int cnt = q.w & 7;
+  const float4 g1 = sb.pgeo[2 * long(slot)], g2 = sb.pgeo[2 * long(slot) + 1];
+  const f3 h1 = xyz(g1), h2 = xyz(g2);
+  const float m1 = g1.w, m2 = g2.w;
   f3 t1 = xyz(b.pos(b1)), t2 = xyz(b.pos(b2));
   q4 q1 = toq(b.rot(b1)), q2 = toq(b.rot(b2));
   float im1 = 0.0f, im2 = 0.0f;
   m33 I1, I2;
   if (d1) I1 = load_inv(b, b1, im1);
   if (d2) I2 = load_inv(b, b2, im2);
-  const int first = int(uint32_t(q.w) >> 3), cnt = q.w & 7;
-  // the manifold's kinematics (at most 4 contacts) are loaded up front, independent of the poses
+  // the manifold's kinematics (at most 4 contacts, contiguous position records written by the
+  // assembly in colour order) are loaded up front, independent of the poses
   int kind[4];
   f3 lp1[4], lp2[4], ln[4];
+  int k1 = 0, k2 = 0;
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (k < cnt) {
-      const int ci = first + k;
-      kind[k] = c.kind[ci];
-      lp1[k] = xyz(c.lp1[ci]);
-      lp2[k] = xyz(c.lp2[ci]);
-      ln[k] = xyz(c.ln[ci]);
+      const float4* P = sb.prec + 3 * long(r0 + k);
+      const float4 a = P[0], c = P[1], e = P[2];
+      kind[k] = __float_as_int(a.w);
+      lp1[k] = xyz(a);
+      lp2[k] = xyz(c);
+      ln[k] = xyz(e);
+      k1 = __float_as_int(c.w);
+      k2 = __float_as_int(e.w);
     }
   }
   bool moved1 = false, moved2 = false;
@@ -93,12 +103,12 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const DevContac
   }
 }
 
-__global__ void __launch_bounds__(64) k_position(DevBodies b, DevContacts c, SimParams p, SolverBufs sb, int kv) {
+__global__ void __launch_bounds__(64) k_position(DevBodies b, SimParams p, SolverBufs sb, int kv) {
   const int col = sb.visit[kv];
   const int first_slot = int(sb.col_start[col]);
   const int nslots = int(sb.col_start[col + 1]) - first_slot;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nslots; t += gridDim.x * blockDim.x)
-    position_one(b, c, p, sb.sinfo[first_slot + t]);
+    position_one(b, p, sb, first_slot + t);
 }
 
 }  // namespace
@@ -108,7 +118,8 @@ int position_blocks(int max_class) { return sweep_blocks(max_class); }
 
 void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k, int blocks,
                             SolverBufs& sb, hipStream_t s) {
-  hipLaunchKernelGGL(k_position, dim3(blocks), dim3(64), 0, s, b, c, p, sb, k);
+  (void)c;
+  hipLaunchKernelGGL(k_position, dim3(blocks), dim3(64), 0, s, b, p, sb, k);
 }
 
 }  // namespace nph

@@ -276,9 +276,17 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
   }
   const int first = cur.mfirst[m], cnt = cur.mcount[m];
   const uint32_t r0 = sb.soff[s];
+  // the position solver's inputs in slot / record order (see k_position.hip)
+  const int k1 = b.shape[b1], k2 = b.shape[b2];
+  sb.pgeo[2 * long(s)] = f4(xyz(b.dims[b1]), m1);
+  sb.pgeo[2 * long(s) + 1] = f4(xyz(b.dims[b2]), m2);
   for (int k = 0; k < cnt; ++k) {
     const int c = first + k;
     const uint32_t r = r0 + k;
+    float4* P = sb.prec + 3 * long(r);
+    P[0] = f4(xyz(cur.lp1[c]), __int_as_float(cur.kind[c]));
+    P[1] = f4(xyz(cur.lp2[c]), __int_as_float(k1));
+    P[2] = f4(xyz(cur.ln[c]), __int_as_float(k2));
     const float4 w14 = cur.w1[c];
     const f3 n = xyz(cur.n[c]);
     const f3 w1 = xyz(w14), w2 = xyz(cur.w2[c]);

@@ -12,7 +12,7 @@ void launch_integrate(const DevBodies& b, const SimParams& p, hipStream_t s);
 
 // k_broadphase.hip ------------------------------------------------------------------------
 struct BroadPhaseBufs {
-  uint32_t* changed;      // [1] number of loose AABBs replaced
+  uint32_t* changed;      // [2] number of loose AABBs replaced: all, of non-dynamic bodies
   uint32_t* bounds;       // [6] ordered-int encoded centroid bounds
   uint32_t* keys;         // [n] Morton codes
   uint32_t* vals;         // [n] body indices
@@ -31,15 +31,23 @@ struct BroadPhaseBufs {
   uint32_t* stats;        // [4] 0: max count, 1: traversal stack overflow
   int2* pairs;            // [pair capacity]
   uint32_t* tmp;          // scan / radix scratch
+  uint32_t* dirty;        // [n] 1 where the loose AABB was replaced since the last pair build
+  int* leaf_of;           // [n] leaf of each body in the current tree
+  int* list;              // [n] dirty dynamic bodies (incremental update)
+  uint32_t* list_count;   // [1]
 };
 
 // mode 0: current pose of every body (static ones included); mode 1: dynamic bodies, current
 // pose; mode 2: dynamic bodies, AABB swept from pos0/rot0 to pos/rot.
-void launch_loose_update(const DevBodies& b, int mode, uint32_t* changed, hipStream_t s);
+void launch_loose_update(const DevBodies& b, int mode, uint32_t* changed, uint32_t* dirty, hipStream_t s);
 void launch_build_lbvh(const DevBodies& b, BroadPhaseBufs& bp, hipStream_t s);
 void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s);
 void launch_pair_offsets(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s);
 void launch_compact_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s);
+// Pair segments updated for the dirty bodies only (the tree of the last full build refitted by
+// expansion); max_dirty bounds the number of dirty bodies. stats[0] > cap or stats[1] means the
+// caller must fall back to a full build.
+void launch_incremental_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, int max_dirty, hipStream_t s);
 
 // k_narrowphase.hip -------------------------------------------------------------------------
 struct NarrowPhaseBufs {
@@ -81,6 +89,8 @@ struct SolverBufs {
   uint32_t* rec2c;        // [C] contact index of each row record
   float4* rec;            // [kRecFields rec_stride] row records, field f of record r at kRecFields r + f
   float4* lam;            // [rec_stride] impulses of each row record (normal, tangent 1, tangent 2, has cache)
+  float4* prec;           // [3 rec_stride] position records: (lp1 | contact kind), (lp2 | shape 1), (ln | shape 2)
+  float4* pgeo;           // [2M] per slot: (half extents 1 | margin 1), (half extents 2 | margin 2)
   long rec_stride;        // record capacity (fixed between reallocations)
   int* visit;             // [max colours] colour classes in visiting order
   double* red;            // residual partials

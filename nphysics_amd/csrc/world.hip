@@ -93,6 +93,15 @@ struct World {
   DBuf loose_mn, loose_mx;
 
   // broad phase
+  DBuf bp_dirty, bp_leaf_of, bp_list;
+  // Incremental pair updates: the tree and segments of the last pair build are valid, how many
+  // incremental updates followed the last full build, and the loose-AABB replacements (all, of
+  // non-dynamic bodies) since the last pair build.
+  bool tree_valid = false;
+  bool incremental_bp = true;
+  int inc_since_full = 0;
+  int64_t dirty_pending = 0, dirty_static_pending = 0;
+  static constexpr int kMaxIncremental = 32;
   DBuf bp_small, keys, vals, keys_alt, vals_alt, child, parent, visit, node_mn, node_mx, node_scene, cand,
       cand_count, cand_off, pairs, bp_tmp;
   int cap = 32;
@@ -110,7 +119,7 @@ struct World {
   // solver
   DBuf sv_state2[2];  // colouring state of the manifolds of cb[k]
   DBuf sv_deg, sv_adj_off, sv_adj_fill, sv_adj, sv_ckeys, sv_order, sv_ckeys_alt,
-      sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_sinfo, sv_rec2c, sv_rec, sv_lam, sv_red, sv_tmp;
+      sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_sinfo, sv_rec2c, sv_rec, sv_lam, sv_prec, sv_pgeo, sv_red, sv_tmp;
   int ncolours = 0;
   long rec_stride = 0;
   int colour_rounds = 0;
@@ -155,7 +164,7 @@ struct World {
   hipEvent_t tune_ev[2] = {nullptr, nullptr};
   bool ml_loaded = false, capture = false, judge = false;
   long samples = 0;
-  DBuf ml_params, ml_img, ml_X, ml_ls, ml_y, ml_t, jd_lam, jd_dv;
+  DBuf ml_params, ml_img, ml_X, ml_ls, ml_y, ml_t, jd_lam, jd_lam0, jd_dv;
   std::vector<int> last_colours;
   bool keep_colours = false;
 
@@ -473,7 +482,14 @@ struct World {
     HIPCK(node_mn.ensure(2 * nn * sizeof(float4)));
     HIPCK(node_mx.ensure(2 * nn * sizeof(float4)));
     HIPCK(node_scene.ensure(nn * sizeof(int2)));
+    if (cand.bytes < nn * size_t(cap) * 4) tree_valid = false;  // segments are lost on growth
     HIPCK(cand.ensure(nn * size_t(cap) * 4));
+    for (DBuf* b : {&bp_leaf_of, &bp_list}) HIPCK(b->ensure(nn * 4));
+    if (bp_dirty.bytes < nn * 4) {
+      tree_valid = false;
+      HIPCK(bp_dirty.ensure(nn * 4));
+      fill_u32(bp_dirty.as<uint32_t>(), 0u, nn, stream);
+    }
     const size_t tw = std::max(radix_temp_words(int(nn)), scan_temp_words(int(nn)));
     HIPCK(bp_tmp.ensure(tw * 4));
     return NPHYS_OK;
@@ -501,17 +517,25 @@ struct World {
     v.cand_off = cand_off.as<uint32_t>();
     v.pairs = pairs.as<int2>();
     v.tmp = bp_tmp.as<uint32_t>();
+    v.dirty = bp_dirty.as<uint32_t>();
+    v.leaf_of = bp_leaf_of.as<int>();
+    v.list = bp_list.as<int>();
+    v.list_count = sm + 28;
     return v;
   }
 
   // Loose-AABB pass; returns the number of replaced loose AABBs through *changed.
   int loose_pass(int mode, uint32_t* changed) {
+    int rc = ensure_bp();  // the dirty flags
+    if (rc) return rc;
     BroadPhaseBufs v = bp_view();
-    fill_u32(v.changed, 0u, 1, stream);
-    launch_loose_update(bodies(), mode, v.changed, stream);
-    HIPCK(hipMemcpyAsync(h_small, v.changed, 4, hipMemcpyDeviceToHost, stream));
+    fill_u32(v.changed, 0u, 2, stream);
+    launch_loose_update(bodies(), mode, v.changed, v.dirty, stream);
+    HIPCK(hipMemcpyAsync(h_small, v.changed, 8, hipMemcpyDeviceToHost, stream));
     HIPCK(hipStreamSynchronize(stream));
     *changed = h_small[0];
+    dirty_pending += h_small[0];
+    dirty_static_pending += h_small[1];
     return NPHYS_OK;
   }
 
@@ -525,6 +549,42 @@ struct World {
     if (rc) return rc;
     DevBodies b = bodies();
     BroadPhaseBufs v = bp_view();
+    bool done = false;
+    if (tree_valid && incremental_bp && !pairs_dirty && dirty_static_pending == 0 &&
+        inc_since_full < kMaxIncremental && dirty_pending <= n / 8) {
+      launch_incremental_pairs(b, v, cap, int(dirty_pending), stream);
+      HIPCK(hipMemcpyAsync(h_small, v.stats, 8, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipStreamSynchronize(stream));
+      done = h_small[1] == 0 && int(h_small[0]) <= cap;  // else the full build below
+      if (done) {
+        ++inc_since_full;
+        ++counters.incremental_pair_updates;
+      }
+    }
+    if (!done) {
+      if ((rc = full_traversal(b, v))) return rc;
+      inc_since_full = 0;
+    }
+    launch_pair_offsets(b, v, cap, stream);
+    // total = off[n-1] + min(count[n-1], cap)
+    HIPCK(hipMemcpyAsync(h_small, v.cand_off + (n - 1), 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(h_small + 1, v.keys_alt + (n - 1), 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    npairs = int(h_small[0] + h_small[1]);
+    HIPCK(pairs.ensure(size_t(std::max(npairs, 1)) * sizeof(int2)));
+    v = bp_view();
+    launch_compact_pairs(b, v, cap, stream);
+    fill_u32(v.dirty, 0u, size_t(n), stream);
+    dirty_pending = dirty_static_pending = 0;
+    tree_valid = true;
+    pairs_dirty = false;
+    return NPHYS_OK;
+  }
+
+  // Full build: the tree from scratch and every dynamic body traversed, the segment capacity grown
+  // until no body overflows it.
+  int full_traversal(DevBodies& b, BroadPhaseBufs& v) {
+    int rc;
     launch_build_lbvh(b, v, stream);
     for (;;) {
       launch_traverse(b, v, cap, stream);
@@ -536,16 +596,6 @@ struct World {
       if ((rc = ensure_bp())) return rc;
       v = bp_view();
     }
-    launch_pair_offsets(b, v, cap, stream);
-    // total = off[n-1] + min(count[n-1], cap)
-    HIPCK(hipMemcpyAsync(h_small, v.cand_off + (n - 1), 4, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipMemcpyAsync(h_small + 1, v.keys_alt + (n - 1), 4, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    npairs = int(h_small[0] + h_small[1]);
-    HIPCK(pairs.ensure(size_t(std::max(npairs, 1)) * sizeof(int2)));
-    v = bp_view();
-    launch_compact_pairs(b, v, cap, stream);
-    pairs_dirty = false;
     return NPHYS_OK;
   }
 
@@ -610,7 +660,10 @@ struct World {
     HIPCK(sv_rec2c.ensure(c * 4));
     HIPCK(sv_rec.ensure(c * kRecFields * sizeof(float4)));
     HIPCK(sv_lam.ensure(c * sizeof(float4)));
-    rec_stride = long(std::min(sv_rec.bytes / (kRecFields * sizeof(float4)), sv_lam.bytes / sizeof(float4)));
+    HIPCK(sv_prec.ensure(c * 3 * sizeof(float4)));
+    HIPCK(sv_pgeo.ensure(m * 2 * sizeof(float4)));
+    rec_stride = long(std::min({sv_rec.bytes / (kRecFields * sizeof(float4)), sv_lam.bytes / sizeof(float4),
+                                sv_prec.bytes / (3 * sizeof(float4))}));
     HIPCK(sv_red.ensure(size_t(residual_blocks(M)) * 2 * sizeof(double)));
     const size_t tw = std::max({radix_temp_words(int(m)), scan_temp_words(int(m)), scan_temp_words(int(nn))});
     HIPCK(sv_tmp.ensure(tw * 4 + 64));
@@ -638,6 +691,8 @@ struct World {
     v.rec2c = sv_rec2c.as<uint32_t>();
     v.rec = sv_rec.as<float4>();
     v.lam = sv_lam.as<float4>();
+    v.prec = sv_prec.as<float4>();
+    v.pgeo = sv_pgeo.as<float4>();
     v.red = sv_red.as<double>();
     v.tmp = sv_tmp.as<uint32_t>();
     return v;
@@ -986,13 +1041,17 @@ struct World {
     }
     if (learned && judge && res_target > 0.0) {
       // The plain cache warm start is the judge: both solves run from the same assembled system
-      // and the prediction is kept only when it reaches the residual target in fewer sweeps.
+      // (the same cached impulses, the prediction added to them for the second) and the
+      // prediction is kept only when it reaches the residual target in fewer sweeps.
       HIPCK(jd_lam.ensure(size_t(C) * sizeof(float4)));
+      HIPCK(jd_lam0.ensure(size_t(C) * sizeof(float4)));
       HIPCK(jd_dv.ensure(size_t(n) * 2 * sizeof(float4)));
       int n_plain = 0, n_learned = 0;
+      launch_lam_copy(v, C, jd_lam0.as<float4>(), true, stream);  // the cache warm start
       if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, n_plain))) return rc;
       const double rinf_p = res_inf, rrms_p = res_rms;
       launch_lam_copy(v, C, jd_lam.as<float4>(), true, stream);
+      launch_lam_copy(v, C, jd_lam0.as<float4>(), false, stream);
       // the velocity increment is the last 24 bytes of each 64-byte solver block
       char* dv0 = sol.as<char>() + 10 * sizeof(float);
       const size_t pitch = kSolverBlock * sizeof(float4), dvb = 6 * sizeof(float);
@@ -1290,6 +1349,12 @@ int nphys_set_sweep_variant(nphys_world* w, int32_t variant) {
   w->w.variant_request = variant;
   w->w.tune_cands.clear();
   w->w.tuned = false;
+  return NPHYS_OK;
+}
+
+int nphys_set_incremental_pairs(nphys_world* w, int32_t on) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  w->w.incremental_bp = on != 0;
   return NPHYS_OK;
 }
 

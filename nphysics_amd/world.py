@@ -26,7 +26,7 @@ EXPORTED = [
     "nphys_num_pairs", "nphys_read_pairs", "nphys_num_contacts", "nphys_read_contacts",
     "nphys_num_solved_contacts", "nphys_read_impulses", "nphys_residual", "nphys_read_counters",
     "nphys_reset_counters", "nphys_counters_enable", "nphys_set_residual_target", "nphys_read_colours",
-    "nphys_set_sweep_variant", "nphys_warmstart_load", "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
+    "nphys_set_sweep_variant", "nphys_set_incremental_pairs", "nphys_warmstart_load", "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
     "nphys_mlp_create", "nphys_mlp_destroy", "nphys_mlp_num_params", "nphys_mlp_get_params",
     "nphys_mlp_set_params", "nphys_mlp_set_data", "nphys_mlp_train", "nphys_mlp_forward",
     "nphys_mlp_gradients", "nphys_last_error",
@@ -78,6 +78,7 @@ def load_library(path=LIB_PATH):
     L.nphys_read_samples.argtypes = [W, fp, fp, C.c_int64, C.POINTER(C.c_int64)]
     L.nphys_set_warmstart_judge.argtypes = [W, C.c_int32]
     L.nphys_set_sweep_variant.argtypes = [W, C.c_int32]
+    L.nphys_set_incremental_pairs.argtypes = [W, C.c_int32]
     L.nphys_read_colours.argtypes = [W, ip, C.c_int64, ip]
     _lib = L
     return L
@@ -250,6 +251,10 @@ class MechanicalWorld:
     def set_sweep_variant(self, variant=-1):
         """-1 autotune (default), 0 per-colour launches, 64/128/256 scene-blocked workgroups."""
         _check(self._L.nphys_set_sweep_variant(self._h, variant))
+
+    def set_incremental_pairs(self, on=True):
+        """Incremental pair-set updates (default) or a full tree build on every change."""
+        _check(self._L.nphys_set_incremental_pairs(self._h, 1 if on else 0))
 
     def set_warmstart_judge(self, on=True):
         _check(self._L.nphys_set_warmstart_judge(self._h, 1 if on else 0))

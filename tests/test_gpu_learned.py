@@ -135,3 +135,38 @@ def test_learned_warm_start_end_to_end(nphys):
     st = e.get_state()
     assert all(np.isfinite(v).all() for v in st.values())
     net.close()
+
+
+def test_judge_solves_match_the_standalone_solves(nphys):
+    """Under the judge, one step solves the assembled system twice: from the cached impulses and
+    from the cached impulses plus the prediction. Each must take exactly the sweeps a world without
+    the judge takes for the same step from the same state (plain contact model, and learned model
+    alone), and the kept result must be that world's state."""
+    from nphysics_amd import learned
+
+    net = learned.WarmStartNet(seed=5)
+    p = net.params
+    net.close()
+    s = low_pile(31)
+    worlds = [_world(nphys, s) for _ in range(3)]  # judge, plain, learned alone
+    for w in worlds:
+        w.step(80)
+        w.set_residual_target(5e-3, 60, 1)
+    judge, plain, alone = worlds
+    for w in (judge, alone):
+        w.load_warmstart(p)
+        w.set_contact_model(abi.CONTACT_MODEL_LEARNED_WARMSTART)
+    judge.set_warmstart_judge(True)
+    for w in worlds:
+        w.reset_counters()
+        w.step(1)
+    cj, cp, cl = (w.counters() for w in worlds)
+    print("judge", cj["judge_plain_sweeps"], cj["judge_learned_sweeps"], cj["judge_kept"],
+          "plain", cp["velocity_sweeps"], "learned alone", cl["velocity_sweeps"])
+    assert cj["judge_plain_sweeps"] == cp["velocity_sweeps"]
+    assert cj["judge_learned_sweeps"] == cl["velocity_sweeps"]
+    assert cj["judge_kept"] == (1 if cl["velocity_sweeps"] < cp["velocity_sweeps"] else 0)
+    ref = alone if cj["judge_kept"] else plain
+    sj, sr = judge.get_state(), ref.get_state()
+    for k in ("pos", "rot", "linvel", "angvel"):
+        np.testing.assert_array_equal(sj[k], sr[k], err_msg=k)
