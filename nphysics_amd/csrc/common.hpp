@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include <algorithm>
 #include <string>
 
@@ -27,6 +29,13 @@ inline int hip_err(hipError_t e) { return set_err(NPHYS_ERR_HIP, std::string("HI
   } while (0)
 
 // Growable device buffer (contents are not preserved on growth unless asked).
+// Bit pattern of a float (host side of __float_as_uint).
+inline uint32_t bits_of(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, sizeof(u));
+  return u;
+}
+
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;

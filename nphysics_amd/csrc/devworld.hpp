@@ -9,9 +9,11 @@
 //            with I the inverse augmented inertia. Structure of arrays for the rest:
 //            pos0/rot0/linvel/angvel, external velocity increment dt*acc (lin, ang), loose AABB
 //            (mn, mx), shape data.
-//   contacts per contact: world1 (w = depth), world2, normal, feature, kinematic kind, local
-//            points/normal for the position solver, solved impulse; per manifold: bodies,
-//            first contact, count.
+//   contacts one 96-byte record per contact slot, each broad-phase pair owning 4 consecutive slots
+//            (a manifold's contacts are slots 4 p .. 4 p + count - 1 of its pair p):
+//            [0] world1 | depth  [1] world2 | feature  [2] normal | kinematic kind
+//            [3] local point 1  [4] local point 2  [5] local normal (the position solver's);
+//            the solved impulse per slot; per manifold: bodies, first slot, count.
 //   rows     per contact in colour order: 8 float4 (see k_solver.hip).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -90,19 +92,18 @@ struct DevBodies {
   }
 };
 
+constexpr int kContactFields = 6;  // float4 per contact record
+constexpr int kPairSlots = 4;      // contact slots per broad-phase pair
+
 struct DevContacts {
   int2* mbody;       // (body1, body2) per manifold, sorted
-  int* mfirst;
+  int* mfirst;       // first contact slot
   int* mcount;
-  float4* w1;        // xyz world point on collider 1, w = depth
-  float4* w2;
-  float4* n;
-  uint32_t* feature;
-  int* kind;
-  float4* lp1;
-  float4* lp2;
-  float4* ln;
-  float4* imp;       // (normal, tangent 1, tangent 2) impulses after the solve
+  float4* rec;       // kContactFields float4 per contact slot, layout above
+  float4* imp;       // per slot: (normal, tangent 1, tangent 2) impulses after the solve
+
+  NPH_HD float4* contact(long slot) const { return rec + slot * kContactFields; }
+  NPH_HD uint32_t feature(long slot) const { return __float_as_uint(rec[slot * kContactFields + 1].w); }
 };
 
 NPH_HD m33 load_inv(const DevBodies& b, int i, float& inv_mass) { return b.inv_inertia(i, inv_mass); }

@@ -221,12 +221,75 @@ __global__ void __launch_bounds__(256) k_init_inodes(int n, uint32_t* inode) {
   q[1] = make_uint4(0u, 0u, 0u, 0u);
 }
 
+// Packed internal nodes for the traversals: one 64-byte record per internal node with both
+// children's bounds and references, so descending one level is a single 64-byte load:
+//   [0] (child0 min, ref0)  [1] (child0 max, ref1)  [2] (child1 min, first scene)
+//   [3] (child1 max, last scene)
+// ref = internal node index, or kLeafRef | dynamic << 30 | body for a leaf; the scene range is
+// that of the node's own leaves.
+constexpr uint32_t kLeafRef = 0x80000000u;
+
+__device__ __forceinline__ uint32_t child_ref(DevBodies b, const uint32_t* __restrict__ vals, int n, int c) {
+  if (c < n - 1) return uint32_t(c);
+  const int o = int(vals[c - (n - 1)]);
+  return kLeafRef | (b.status[o] == STATUS_DYNAMIC ? 1u << 30 : 0u) | uint32_t(o);
+}
+
+__global__ void __launch_bounds__(256) k_pack_nodes(DevBodies b, const uint32_t* __restrict__ vals, int n,
+                                                   const int* __restrict__ child,
+                                                   const float4* __restrict__ node_mn,
+                                                   const float4* __restrict__ node_mx,
+                                                   const int2* __restrict__ node_scene, float4* pnode) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  const int c0 = child[2 * i], c1 = child[2 * i + 1];
+  const int2 sr = node_scene[i];
+  float4* r = pnode + 4 * long(i);
+  r[0] = f4(xyz(node_mn[c0]), __uint_as_float(child_ref(b, vals, n, c0)));
+  r[1] = f4(xyz(node_mx[c0]), __uint_as_float(child_ref(b, vals, n, c1)));
+  r[2] = f4(xyz(node_mn[c1]), __int_as_float(sr.x));
+  r[3] = f4(xyz(node_mx[c1]), __int_as_float(sr.y));
+}
+
+// Walks the packed tree for the loose AABB of body q and calls leaf(o, dynamic) for every other
+// body of q's scene whose loose AABB overlaps it. Returns false on a stack overflow.
+template <bool kScenes, class F>
+__device__ __forceinline__ bool walk_tree(const DevBodies& b, const float4* __restrict__ pnode, int q, F&& leaf) {
+  const aabb Q{xyz(b.loose_mn[q]), xyz(b.loose_mx[q])};
+  const int qs = kScenes ? b.scene[q] : 0;
+  int stack[kStack];
+  int sp = 0;
+  int node = 0;
+  bool ok = true;
+  while (true) {
+    const float4* r = pnode + 4 * long(node);
+    const float4 a0 = r[0], a1 = r[1], a2 = r[2], a3 = r[3];
+    if (!kScenes || (qs >= __float_as_int(a2.w) && qs <= __float_as_int(a3.w))) {
+      const uint32_t ref[2] = {__float_as_uint(a0.w), __float_as_uint(a1.w)};
+      const aabb B[2] = {{xyz(a0), xyz(a1)}, {xyz(a2), xyz(a3)}};
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (!intersects(Q, B[k])) continue;
+        const uint32_t c = ref[k];
+        if (c & kLeafRef) {
+          const int o = int(c & 0x3fffffffu);
+          if (o != q && (!kScenes || b.scene[o] == qs)) leaf(o, ((c >> 30) & 1u) != 0);
+        } else if (sp < kStack) {
+          stack[sp++] = int(c);
+        } else {
+          ok = false;
+        }
+      }
+    }
+    if (sp == 0) break;
+    node = stack[--sp];
+  }
+  return ok;
+}
+
 template <bool kScenes>
 __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* __restrict__ vals, int n,
-                                                 const int* __restrict__ child,
-                                                 const float4* __restrict__ node_mn,
-                                                 const float4* __restrict__ node_mx,
-                                                 const int2* __restrict__ node_scene, int* cand,
+                                                 const float4* __restrict__ pnode, int* cand,
                                                  uint32_t* cand_count, int cap, uint32_t* stats) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -235,43 +298,17 @@ __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* _
     cand_count[q] = 0;
     return;
   }
-  const aabb Q{xyz(b.loose_mn[q]), xyz(b.loose_mx[q])};
-  const int qs = kScenes ? b.scene[q] : 0;
   const long base = long(q) * cap;
   uint32_t cnt = 0;
-  int stack[kStack];
-  int sp = 0;
-  int node = 0;
-  bool overflow = false;
-  while (true) {
-    const int c0 = child[2 * node], c1 = child[2 * node + 1];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const int c = k == 0 ? c0 : c1;
-      if (kScenes && c < n - 1) {
-        const int2 sr = node_scene[c];
-        if (qs < sr.x || qs > sr.y) continue;
-      }
-      const aabb B{xyz(node_mn[c]), xyz(node_mx[c])};
-      if (!intersects(Q, B)) continue;
-      if (c >= n - 1) {
-        const int o = int(vals[c - (n - 1)]);
-        if (o != q && (!kScenes || b.scene[o] == qs) && (b.status[o] != STATUS_DYNAMIC || o > q)) {
-          if (cnt < uint32_t(cap)) cand[base + cnt] = o;
-          ++cnt;
-        }
-      } else if (sp < kStack) {
-        stack[sp++] = c;
-      } else {
-        overflow = true;
-      }
+  const bool ok = walk_tree<kScenes>(b, pnode, q, [&](int o, bool dyn) {
+    if (!dyn || o > q) {
+      if (cnt < uint32_t(cap)) cand[base + cnt] = o;
+      ++cnt;
     }
-    if (sp == 0) break;
-    node = stack[--sp];
-  }
+  });
   cand_count[q] = cnt;
   atomicMax(&stats[0], cnt);
-  if (overflow) atomicOr(&stats[1], 1u);
+  if (!ok) atomicOr(&stats[1], 1u);
 }
 
 // ---- incremental pair update -------------------------------------------------------------------
@@ -343,58 +380,27 @@ __global__ void __launch_bounds__(256) k_refit_dirty(DevBodies b, const int* __r
 template <bool kScenes>
 __global__ void __launch_bounds__(256) k_traverse_dirty(DevBodies b, const int* __restrict__ list,
                                                        const uint32_t* __restrict__ list_count,
-                                                       const uint32_t* __restrict__ vals, int n,
-                                                       const int* __restrict__ child,
-                                                       const float4* __restrict__ node_mn,
-                                                       const float4* __restrict__ node_mx,
-                                                       const int2* __restrict__ node_scene,
+                                                       const float4* __restrict__ pnode,
                                                        const uint32_t* __restrict__ dirty, int* cand,
                                                        uint32_t* cand_count, int cap, uint32_t* stats) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= int(*list_count)) return;
   const int q = list[t];
-  const aabb Q{xyz(b.loose_mn[q]), xyz(b.loose_mx[q])};
-  const int qs = kScenes ? b.scene[q] : 0;
   const long base = long(q) * cap;
   uint32_t cnt = 0, most = 0;
-  int stack[kStack];
-  int sp = 0;
-  int node = 0;
-  bool overflow = false;
-  while (true) {
-    const int c0 = child[2 * node], c1 = child[2 * node + 1];
-#pragma unroll
-    for (int k = 0; k < 2; ++k) {
-      const int c = k == 0 ? c0 : c1;
-      if (kScenes && c < n - 1) {
-        const int2 sr = node_scene[c];
-        if (qs < sr.x || qs > sr.y) continue;
-      }
-      const aabb B{xyz(node_mn[c]), xyz(node_mx[c])};
-      if (!intersects(Q, B)) continue;
-      if (c >= n - 1) {
-        const int o = int(vals[c - (n - 1)]);
-        if (o == q || (kScenes && b.scene[o] != qs)) continue;
-        if (b.status[o] != STATUS_DYNAMIC || o > q) {
-          if (cnt < uint32_t(cap)) cand[base + cnt] = o;
-          ++cnt;
-        } else if (!dirty[o]) {
-          const uint32_t pos = atomicAdd(&cand_count[o], 1u);
-          if (pos < uint32_t(cap)) cand[long(o) * cap + pos] = q;
-          most = pos + 1 > most ? pos + 1 : most;
-        }
-      } else if (sp < kStack) {
-        stack[sp++] = c;
-      } else {
-        overflow = true;
-      }
+  const bool ok = walk_tree<kScenes>(b, pnode, q, [&](int o, bool dyn) {
+    if (!dyn || o > q) {
+      if (cnt < uint32_t(cap)) cand[base + cnt] = o;
+      ++cnt;
+    } else if (!dirty[o]) {
+      const uint32_t pos = atomicAdd(&cand_count[o], 1u);
+      if (pos < uint32_t(cap)) cand[long(o) * cap + pos] = q;
+      most = pos + 1 > most ? pos + 1 : most;
     }
-    if (sp == 0) break;
-    node = stack[--sp];
-  }
+  });
   cand_count[q] = cnt;
   atomicMax(&stats[0], cnt > most ? cnt : most);
-  if (overflow) atomicOr(&stats[1], 1u);
+  if (!ok) atomicOr(&stats[1], 1u);
 }
 
 __global__ void __launch_bounds__(256) k_clamp_counts(const uint32_t* cnt, uint32_t* out, int n, int cap) {
@@ -451,6 +457,8 @@ void launch_build_lbvh(const DevBodies& b, BroadPhaseBufs& bp, hipStream_t s) {
                      bp.leaf_of);
   hipLaunchKernelGGL(k_refit_atomic, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.parent, bp.visit);
   hipLaunchKernelGGL(k_decode_nodes, dim3(nb(n - 1)), dim3(256), 0, s, n, bp.visit, bp.node_mn, bp.node_mx);
+  hipLaunchKernelGGL(k_pack_nodes, dim3(nb(n - 1)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
+                     bp.node_mx, bp.node_scene, bp.pnode);
 }
 
 void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s) {
@@ -462,11 +470,11 @@ void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_
     return;
   }
   if (bp.scene_bits)
-    hipLaunchKernelGGL(k_traverse<true>, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
-                       bp.node_mx, bp.node_scene, bp.cand, bp.cand_count, cap, bp.stats);
+    hipLaunchKernelGGL(k_traverse<true>, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.pnode, bp.cand,
+                       bp.cand_count, cap, bp.stats);
   else
-    hipLaunchKernelGGL(k_traverse<false>, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
-                       bp.node_mx, bp.node_scene, bp.cand, bp.cand_count, cap, bp.stats);
+    hipLaunchKernelGGL(k_traverse<false>, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.pnode, bp.cand,
+                       bp.cand_count, cap, bp.stats);
 }
 
 void launch_incremental_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, int max_dirty, hipStream_t s) {
@@ -480,14 +488,14 @@ void launch_incremental_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, i
   hipLaunchKernelGGL(k_refit_dirty, dim3(nb(max_dirty)), dim3(256), 0, s, b, bp.list, bp.list_count, n,
                      bp.leaf_of, bp.parent, bp.node_mn, bp.node_mx, bp.visit);
   hipLaunchKernelGGL(k_decode_nodes, dim3(nb(n - 1)), dim3(256), 0, s, n, bp.visit, bp.node_mn, bp.node_mx);
+  hipLaunchKernelGGL(k_pack_nodes, dim3(nb(n - 1)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
+                     bp.node_mx, bp.node_scene, bp.pnode);
   if (bp.scene_bits)
     hipLaunchKernelGGL(k_traverse_dirty<true>, dim3(nb(max_dirty)), dim3(256), 0, s, b, bp.list, bp.list_count,
-                       bp.vals, n, bp.child, bp.node_mn, bp.node_mx, bp.node_scene, bp.dirty, bp.cand,
-                       bp.cand_count, cap, bp.stats);
+                       bp.pnode, bp.dirty, bp.cand, bp.cand_count, cap, bp.stats);
   else
     hipLaunchKernelGGL(k_traverse_dirty<false>, dim3(nb(max_dirty)), dim3(256), 0, s, b, bp.list, bp.list_count,
-                       bp.vals, n, bp.child, bp.node_mn, bp.node_mx, bp.node_scene, bp.dirty, bp.cand,
-                       bp.cand_count, cap, bp.stats);
+                       bp.pnode, bp.dirty, bp.cand, bp.cand_count, cap, bp.stats);
 }
 
 void launch_compact_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_t s) {

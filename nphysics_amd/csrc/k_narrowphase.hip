@@ -4,9 +4,10 @@
 // the interaction graph and the manifold collection of MechanicalWorld::step
 // (src/world/mechanical_world.rs:287-300, manifolds with at least one contact). Each lane runs the
 // contact generator of its pair (np_geometry.hpp) with prediction = sum of both query limits and
-// writes up to 4 contacts into the pair's fixed slots; two scans then compact the slots into the
-// contact list (pair order, then generator order) and the manifold list. HBM bound: 100 B per
-// pair + 48 B per contact of algorithmic traffic.
+// writes up to 4 contacts as 96-byte records into the pair's fixed slots, where they stay; a scan
+// then lists the manifolds (pairs with contacts, in pair order) with their first slot and count.
+// The contact order (pair order, then generator order) is the manifold list's. HBM bound: 100 B
+// per pair + 48 B per contact of algorithmic traffic.
 #include <hip/hip_runtime.h>
 
 #include "kernels.hpp"
@@ -39,7 +40,7 @@ __global__ void __launch_bounds__(256) k_np_perm(int npairs, const uint32_t* __r
 
 __global__ void __launch_bounds__(256) k_np_pairs(DevBodies b, const int2* __restrict__ pairs,
                                                  int npairs, const uint32_t* __restrict__ perm,
-                                                 NarrowPhaseBufs np) {
+                                                 NarrowPhaseBufs np, DevContacts out_c) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npairs) return;
   const int i = int(perm[j]);
@@ -52,20 +53,20 @@ __global__ void __launch_bounds__(256) k_np_pairs(DevBodies b, const int2* __res
   np.count[i] = uint32_t(nc);
   np.has[i] = nc > 0 ? 1u : 0u;
   for (int k = 0; k < nc; ++k) {
-    const long s = long(i) * 4 + k;
-    np.sw1[s] = f4(out[k].w1, out[k].depth);
-    np.sw2[s] = f4(out[k].w2, 0.0f);
-    np.sn[s] = f4(out[k].n, 0.0f);
-    np.sfeat[s] = out[k].feature;
-    np.skind[s] = out[k].kind;
-    np.slp1[s] = f4(out[k].lp1, 0.0f);
-    np.slp2[s] = f4(out[k].lp2, 0.0f);
-    np.sln[s] = f4(out[k].ln, 0.0f);
+    float4* r = out_c.contact(long(i) * kPairSlots + k);
+    r[0] = f4(out[k].w1, out[k].depth);
+    r[1] = f4(out[k].w2, __uint_as_float(out[k].feature));
+    r[2] = f4(out[k].n, __int_as_float(out[k].kind));
+    r[3] = f4(out[k].lp1, 0.0f);
+    r[4] = f4(out[k].lp2, 0.0f);
+    r[5] = f4(out[k].ln, 0.0f);
   }
 }
 
-__global__ void __launch_bounds__(256) k_np_compact(const int2* __restrict__ pairs, int npairs,
-                                                   NarrowPhaseBufs np, DevContacts out) {
+// The manifold list: one entry per pair with contacts, in pair order; its contacts stay in the
+// pair's slots.
+__global__ void __launch_bounds__(256) k_np_manifolds(const int2* __restrict__ pairs, int npairs,
+                                                     NarrowPhaseBufs np, DevContacts out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npairs) return;
   const uint32_t nc = np.count[i];
@@ -75,21 +76,22 @@ __global__ void __launch_bounds__(256) k_np_compact(const int2* __restrict__ pai
   }
   if (nc == 0) return;
   const uint32_t m = np.moff[i];
-  const uint32_t c0 = np.coff[i];
   out.mbody[m] = pairs[i];
-  out.mfirst[m] = int(c0);
+  out.mfirst[m] = i * kPairSlots;
   out.mcount[m] = int(nc);
-  for (uint32_t k = 0; k < nc; ++k) {
-    const long s = long(i) * 4 + k;
-    const uint32_t c = c0 + k;
-    out.w1[c] = np.sw1[s];
-    out.w2[c] = np.sw2[s];
-    out.n[c] = np.sn[s];
-    out.feature[c] = np.sfeat[s];
-    out.kind[c] = np.skind[s];
-    out.lp1[c] = np.slp1[s];
-    out.lp2[c] = np.slp2[s];
-    out.ln[c] = np.sln[s];
+}
+
+__global__ void __launch_bounds__(256) k_gather_contacts(DevContacts c, int M, const uint32_t* __restrict__ moff,
+                                                        float4* dense, float4* dense_imp) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const int first = c.mfirst[m], cnt = c.mcount[m];
+  const long d0 = moff[m];
+  for (int k = 0; k < cnt; ++k) {
+    const float4* r = c.contact(first + k);
+    float4* o = dense + (d0 + k) * kContactFields;
+    for (int f = 0; f < kContactFields; ++f) o[f] = r[f];
+    if (dense_imp) dense_imp[d0 + k] = c.imp[first + k];
   }
 }
 
@@ -106,10 +108,17 @@ void launch_narrow_phase(const DevBodies& b, const int2* pairs, int npairs, Narr
   hipLaunchKernelGGL(k_np_flag, dim3(g), dim3(256), 0, s, b, pairs, npairs, np.has);
   scan_exclusive_u32(np.has, np.moff, npairs, np.tmp, s);
   hipLaunchKernelGGL(k_np_perm, dim3(g), dim3(256), 0, s, npairs, np.has, np.moff, np.perm);
-  hipLaunchKernelGGL(k_np_pairs, dim3(g), dim3(256), 0, s, b, pairs, npairs, np.perm, np);
+  hipLaunchKernelGGL(k_np_pairs, dim3(g), dim3(256), 0, s, b, pairs, npairs, np.perm, np, out);
   scan_exclusive_u32(np.count, np.coff, npairs, np.tmp, s);
   scan_exclusive_u32(np.has, np.moff, npairs, np.tmp, s);
-  hipLaunchKernelGGL(k_np_compact, dim3(g), dim3(256), 0, s, pairs, npairs, np, out);
+  hipLaunchKernelGGL(k_np_manifolds, dim3(g), dim3(256), 0, s, pairs, npairs, np, out);
+}
+
+void launch_gather_contacts(const DevContacts& c, int M, const uint32_t* moff, float4* dense,
+                            float4* dense_imp, hipStream_t s) {
+  if (M <= 0) return;
+  hipLaunchKernelGGL(k_gather_contacts, dim3(unsigned((M + 255) / 256)), dim3(256), 0, s, c, M, moff, dense,
+                     dense_imp);
 }
 
 }  // namespace nph

@@ -283,19 +283,20 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
   for (int k = 0; k < cnt; ++k) {
     const int c = first + k;
     const uint32_t r = r0 + k;
+    const float4* K = cur.contact(c);
+    const float4 w14 = K[0], w24 = K[1], n4 = K[2];
     float4* P = sb.prec + 3 * long(r);
-    P[0] = f4(xyz(cur.lp1[c]), __int_as_float(cur.kind[c]));
-    P[1] = f4(xyz(cur.lp2[c]), __int_as_float(k1));
-    P[2] = f4(xyz(cur.ln[c]), __int_as_float(k2));
-    const float4 w14 = cur.w1[c];
-    const f3 n = xyz(cur.n[c]);
-    const f3 w1 = xyz(w14), w2 = xyz(cur.w2[c]);
+    P[0] = f4(xyz(K[3]), n4.w);  // the kinematic kind's bits
+    P[1] = f4(xyz(K[4]), __int_as_float(k1));
+    P[2] = f4(xyz(K[5]), __int_as_float(k2));
+    const f3 n = xyz(n4);
+    const f3 w1 = xyz(w14), w2 = xyz(w24);
     const float depth = w14.w;
-    const uint32_t feat = cur.feature[c];
+    const uint32_t feat = __float_as_uint(w24.w);
     float4 imp = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     float has = 0.0f;
     for (int kk = 0; kk < pc; ++kk)
-      if (prev.feature[pf + kk] == feat) {
+      if (prev.feature(pf + kk) == feat) {
         imp = prev.imp[pf + kk];
         has = 1.0f;
         break;
@@ -778,7 +779,7 @@ __global__ void __launch_bounds__(256) k_features(DevBodies b, DevContacts cur, 
     const uint32_t r = x.r0 + k;
     CRec c;
     load_rec(sb, r, c);
-    const float depth = cur.w1[x.first + k].w + mm;
+    const float depth = cur.contact(x.first + k)[0].w + mm;
     const float lsc = c.r[0] * gdt;
     float* f = X + long(r) * 24;
 #pragma unroll

@@ -24,6 +24,7 @@ struct BroadPhaseBufs {
   float4* node_mn;        // [2n-1]
   float4* node_mx;
   int2* node_scene;       // [n-1] scene range (first, last) of every internal node's leaves
+  float4* pnode;          // [4(n-1)] packed internal nodes for the traversals (k_broadphase.hip)
   int scene_bits;         // Morton keys carry the scene in their top scene_bits bits
   int* cand;              // [n * cap] candidate partners per body
   uint32_t* cand_count;   // [n] partner count (may exceed cap)
@@ -51,14 +52,6 @@ void launch_incremental_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, i
 
 // k_narrowphase.hip -------------------------------------------------------------------------
 struct NarrowPhaseBufs {
-  float4* sw1;            // [4 P] per-pair contact slots
-  float4* sw2;
-  float4* sn;
-  uint32_t* sfeat;
-  int* skind;
-  float4* slp1;
-  float4* slp2;
-  float4* sln;
   uint32_t* count;        // [P] contacts per pair
   uint32_t* coff;         // [P] exclusive scan of count
   uint32_t* has;          // [P] count > 0
@@ -69,6 +62,11 @@ struct NarrowPhaseBufs {
 };
 void launch_narrow_phase(const DevBodies& b, const int2* pairs, int npairs, NarrowPhaseBufs& np,
                          DevContacts out, hipStream_t s);
+// Contacts of all manifolds in manifold order, gathered from their slots into dense records
+// (kContactFields float4 each) and impulses (dense_imp may be null); moff = exclusive scan of the
+// manifolds' counts.
+void launch_gather_contacts(const DevContacts& c, int M, const uint32_t* moff, float4* dense,
+                            float4* dense_imp, hipStream_t s);
 
 // k_solver.hip -----------------------------------------------------------------------------
 struct SolverBufs {

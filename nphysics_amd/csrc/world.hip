@@ -35,18 +35,17 @@ namespace nph {
 
 namespace {
 
+// Manifold list plus the contact slots of every broad-phase pair (kPairSlots per pair).
 struct ContactBufs {
-  DBuf mbody, mfirst, mcount, w1, w2, n, feature, kind, lp1, lp2, ln, imp;
+  DBuf mbody, mfirst, mcount, rec, imp;
   int M = 0, C = 0;
-  hipError_t ensure(size_t mcap, size_t ccap) {
+  hipError_t ensure(size_t mcap, size_t slots) {
     hipError_t e;
     if ((e = mbody.ensure(mcap * sizeof(int2))) != hipSuccess) return e;
     if ((e = mfirst.ensure(mcap * 4)) != hipSuccess) return e;
     if ((e = mcount.ensure(mcap * 4)) != hipSuccess) return e;
-    for (DBuf* b : {&w1, &w2, &n, &lp1, &lp2, &ln, &imp})
-      if ((e = b->ensure(ccap * sizeof(float4))) != hipSuccess) return e;
-    if ((e = feature.ensure(ccap * 4)) != hipSuccess) return e;
-    if ((e = kind.ensure(ccap * 4)) != hipSuccess) return e;
+    if ((e = rec.ensure(slots * kContactFields * sizeof(float4))) != hipSuccess) return e;
+    if ((e = imp.ensure(slots * sizeof(float4))) != hipSuccess) return e;
     return hipSuccess;
   }
   DevContacts view() const {
@@ -54,14 +53,7 @@ struct ContactBufs {
     d.mbody = mbody.as<int2>();
     d.mfirst = mfirst.as<int>();
     d.mcount = mcount.as<int>();
-    d.w1 = w1.as<float4>();
-    d.w2 = w2.as<float4>();
-    d.n = n.as<float4>();
-    d.feature = feature.as<uint32_t>();
-    d.kind = kind.as<int>();
-    d.lp1 = lp1.as<float4>();
-    d.lp2 = lp2.as<float4>();
-    d.ln = ln.as<float4>();
+    d.rec = rec.as<float4>();
     d.imp = imp.as<float4>();
     return d;
   }
@@ -93,7 +85,7 @@ struct World {
   DBuf loose_mn, loose_mx;
 
   // broad phase
-  DBuf bp_dirty, bp_leaf_of, bp_list;
+  DBuf bp_dirty, bp_leaf_of, bp_list, bp_pnode;
   // Incremental pair updates: the tree and segments of the last pair build are valid, how many
   // incremental updates followed the last full build, and the loose-AABB replacements (all, of
   // non-dynamic bodies) since the last pair build.
@@ -110,8 +102,8 @@ struct World {
   bool moved = true;
 
   // narrow phase
-  DBuf np_sw1, np_sw2, np_sn, np_sfeat, np_skind, np_slp1, np_slp2, np_sln, np_count, np_coff,
-      np_has, np_moff, np_perm, np_tmp;
+  DBuf np_count, np_coff, np_has, np_moff, np_perm, np_tmp;
+  DBuf rd_moff, rd_rec, rd_imp;  // contact readback staging
   ContactBufs cb[2];
   int cur = 0;      // index of the manifolds produced by the last collision detection
   bool have_solved = false;
@@ -482,6 +474,7 @@ struct World {
     HIPCK(node_mn.ensure(2 * nn * sizeof(float4)));
     HIPCK(node_mx.ensure(2 * nn * sizeof(float4)));
     HIPCK(node_scene.ensure(nn * sizeof(int2)));
+    HIPCK(bp_pnode.ensure(4 * nn * sizeof(float4)));
     if (cand.bytes < nn * size_t(cap) * 4) tree_valid = false;  // segments are lost on growth
     HIPCK(cand.ensure(nn * size_t(cap) * 4));
     for (DBuf* b : {&bp_leaf_of, &bp_list}) HIPCK(b->ensure(nn * 4));
@@ -511,6 +504,7 @@ struct World {
     v.node_mn = node_mn.as<float4>();
     v.node_mx = node_mx.as<float4>();
     v.node_scene = node_scene.as<int2>();
+    v.pnode = bp_pnode.as<float4>();
     v.scene_bits = scene_bits;
     v.cand = cand.as<int>();
     v.cand_count = cand_count.as<uint32_t>();
@@ -599,26 +593,49 @@ struct World {
     return NPHYS_OK;
   }
 
+  // The contacts of `c` in manifold order (dense records and, when imp is given, impulses), with
+  // the manifolds' bodies and counts, copied to the host.
+  int gather_contacts(ContactBufs& c, std::vector<int2>& mb, std::vector<int>& mc, std::vector<float4>& rec,
+                      std::vector<float4>* imp) {
+    const int M = c.M, C = c.C;
+    mb.resize(size_t(M));
+    mc.resize(size_t(M));
+    HIPCK(hipMemcpyAsync(mb.data(), c.mbody.p, size_t(M) * sizeof(int2), hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(mc.data(), c.mcount.p, size_t(M) * 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    std::vector<uint32_t> off(static_cast<size_t>(M));
+    uint32_t acc = 0;
+    for (int m = 0; m < M; ++m) {
+      off[size_t(m)] = acc;
+      acc += uint32_t(mc[size_t(m)]);
+    }
+    if (int(acc) != C) return set_err(NPHYS_ERR_STATE, "contact count mismatch");
+    HIPCK(rd_moff.ensure(size_t(std::max(M, 1)) * 4));
+    HIPCK(rd_rec.ensure(size_t(std::max(C, 1)) * kContactFields * sizeof(float4)));
+    if (imp) HIPCK(rd_imp.ensure(size_t(std::max(C, 1)) * sizeof(float4)));
+    HIPCK(hipMemcpyAsync(rd_moff.p, off.data(), size_t(M) * 4, hipMemcpyHostToDevice, stream));
+    launch_gather_contacts(c.view(), M, rd_moff.as<uint32_t>(), rd_rec.as<float4>(),
+                           imp ? rd_imp.as<float4>() : nullptr, stream);
+    rec.resize(size_t(C) * kContactFields);
+    HIPCK(hipMemcpyAsync(rec.data(), rd_rec.p, rec.size() * sizeof(float4), hipMemcpyDeviceToHost, stream));
+    if (imp) {
+      imp->resize(size_t(C));
+      HIPCK(hipMemcpyAsync(imp->data(), rd_imp.p, imp->size() * sizeof(float4), hipMemcpyDeviceToHost, stream));
+    }
+    HIPCK(hipStreamSynchronize(stream));
+    return NPHYS_OK;
+  }
+
   // ------------------------------------------------------------------ narrow phase
   int narrow_phase() {
     ContactBufs& out = cb[cur];
     const size_t P = size_t(std::max(npairs, 1));
-    for (DBuf* b : {&np_sw1, &np_sw2, &np_sn, &np_slp1, &np_slp2, &np_sln})
-      HIPCK(b->ensure(4 * P * sizeof(float4)));
-    HIPCK(np_sfeat.ensure(4 * P * 4));
-    HIPCK(np_skind.ensure(4 * P * 4));
+    // a manifold's first slot is packed in 28 bits of the solver's slot record
+    if (kPairSlots * P >= (size_t(1) << 28)) return set_err(NPHYS_ERR_CAPACITY, "too many broad-phase pairs");
     for (DBuf* b : {&np_count, &np_coff, &np_has, &np_moff, &np_perm}) HIPCK(b->ensure(P * 4));
     HIPCK(np_tmp.ensure(scan_temp_words(int(P)) * 4 + 64));
-    HIPCK(out.ensure(P, 4 * P));
+    HIPCK(out.ensure(P, kPairSlots * P));
     NarrowPhaseBufs v;
-    v.sw1 = np_sw1.as<float4>();
-    v.sw2 = np_sw2.as<float4>();
-    v.sn = np_sn.as<float4>();
-    v.sfeat = np_sfeat.as<uint32_t>();
-    v.skind = np_skind.as<int>();
-    v.slp1 = np_slp1.as<float4>();
-    v.slp2 = np_slp2.as<float4>();
-    v.sln = np_sln.as<float4>();
     v.count = np_count.as<uint32_t>();
     v.coff = np_coff.as<uint32_t>();
     v.has = np_has.as<uint32_t>();
@@ -1389,33 +1406,24 @@ int nphys_read_contacts(nphys_world* w, int32_t* bodies, uint32_t* feature, floa
                         float* world2, float* normal, float* depth, int64_t cap) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
   nph::ContactBufs& c = w->w.cb[w->w.cur];
-  const int C = c.C, M = c.M;
-  if (cap < C) return set_err(NPHYS_ERR_CAPACITY, "contact buffer too small");
-  if (C == 0) return NPHYS_OK;
-  std::vector<int2> mb(M);
-  std::vector<int> mf(M), mc(M);
-  std::vector<float4> a(C), b(C), n(C);
-  std::vector<uint32_t> f(C);
-  if (hipMemcpy(mb.data(), c.mbody.p, M * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(mf.data(), c.mfirst.p, M * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(mc.data(), c.mcount.p, M * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(a.data(), c.w1.p, C * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(b.data(), c.w2.p, C * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(n.data(), c.n.p, C * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(f.data(), c.feature.p, C * 4, hipMemcpyDeviceToHost) != hipSuccess)
-    return set_err(NPHYS_ERR_HIP, "contact copy failed");
-  for (int m = 0; m < M; ++m)
-    for (int k = 0; k < mc[m]; ++k) {
-      const int i = mf[m] + k;
+  if (cap < c.C) return set_err(NPHYS_ERR_CAPACITY, "contact buffer too small");
+  if (c.C == 0) return NPHYS_OK;
+  std::vector<int2> mb;
+  std::vector<int> mc;
+  std::vector<float4> rec;
+  int rc = w->w.gather_contacts(c, mb, mc, rec, nullptr);
+  if (rc) return rc;
+  long i = 0;
+  for (size_t m = 0; m < mb.size(); ++m)
+    for (int k = 0; k < mc[m]; ++k, ++i) {
+      const float4* r = &rec[size_t(i) * nph::kContactFields];
       if (bodies) { bodies[2 * i] = mb[m].x; bodies[2 * i + 1] = mb[m].y; }
+      if (feature) feature[i] = nph::bits_of(r[1].w);
+      if (world1) { world1[3 * i] = r[0].x; world1[3 * i + 1] = r[0].y; world1[3 * i + 2] = r[0].z; }
+      if (world2) { world2[3 * i] = r[1].x; world2[3 * i + 1] = r[1].y; world2[3 * i + 2] = r[1].z; }
+      if (normal) { normal[3 * i] = r[2].x; normal[3 * i + 1] = r[2].y; normal[3 * i + 2] = r[2].z; }
+      if (depth) depth[i] = r[0].w;
     }
-  for (int i = 0; i < C; ++i) {
-    if (feature) feature[i] = f[i];
-    if (world1) { world1[3 * i] = a[i].x; world1[3 * i + 1] = a[i].y; world1[3 * i + 2] = a[i].z; }
-    if (world2) { world2[3 * i] = b[i].x; world2[3 * i + 1] = b[i].y; world2[3 * i + 2] = b[i].z; }
-    if (normal) { normal[3 * i] = n[i].x; normal[3 * i + 1] = n[i].y; normal[3 * i + 2] = n[i].z; }
-    if (depth) depth[i] = a[i].w;
-  }
   return NPHYS_OK;
 }
 
@@ -1429,30 +1437,24 @@ int nphys_read_impulses(nphys_world* w, uint64_t* keys, float* impulses, int64_t
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
   if (!w->w.have_solved) return NPHYS_OK;
   nph::ContactBufs& c = w->w.cb[w->w.cur ^ 1];
-  const int C = c.C, M = c.M;
-  if (cap < C) return set_err(NPHYS_ERR_CAPACITY, "impulse buffer too small");
-  if (C == 0) return NPHYS_OK;
-  std::vector<float4> imp(C);
-  std::vector<uint32_t> f(C);
-  std::vector<int2> mb(M);
-  std::vector<int> mf(M), mc(M);
-  if (hipMemcpy(imp.data(), c.imp.p, C * sizeof(float4), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(f.data(), c.feature.p, C * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(mb.data(), c.mbody.p, M * sizeof(int2), hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(mf.data(), c.mfirst.p, M * 4, hipMemcpyDeviceToHost) != hipSuccess ||
-      hipMemcpy(mc.data(), c.mcount.p, M * 4, hipMemcpyDeviceToHost) != hipSuccess)
-    return set_err(NPHYS_ERR_HIP, "impulse copy failed");
-  for (int m = 0; m < M; ++m)
-    for (int k = 0; k < mc[m]; ++k) {
-      const int i = mf[m] + k;
+  if (cap < c.C) return set_err(NPHYS_ERR_CAPACITY, "impulse buffer too small");
+  if (c.C == 0) return NPHYS_OK;
+  std::vector<int2> mb;
+  std::vector<int> mc;
+  std::vector<float4> rec, imp;
+  int rc = w->w.gather_contacts(c, mb, mc, rec, &imp);
+  if (rc) return rc;
+  long i = 0;
+  for (size_t m = 0; m < mb.size(); ++m)
+    for (int k = 0; k < mc[m]; ++k, ++i) {
+      const uint32_t f = nph::bits_of(rec[size_t(i) * nph::kContactFields + 1].w);
       if (keys)
-        keys[i] = (uint64_t(uint32_t(mb[m].x)) << 40) | (uint64_t(uint32_t(mb[m].y)) << 16) | uint64_t(f[i] & 0xffffu);
-    }
-  if (impulses)
-    for (int i = 0; i < C; ++i) {
-      impulses[3 * i] = imp[i].x;
-      impulses[3 * i + 1] = imp[i].y;
-      impulses[3 * i + 2] = imp[i].z;
+        keys[i] = (uint64_t(uint32_t(mb[m].x)) << 40) | (uint64_t(uint32_t(mb[m].y)) << 16) | uint64_t(f & 0xffffu);
+      if (impulses) {
+        impulses[3 * i] = imp[size_t(i)].x;
+        impulses[3 * i + 1] = imp[size_t(i)].y;
+        impulses[3 * i + 2] = imp[size_t(i)].z;
+      }
     }
   return NPHYS_OK;
 }
