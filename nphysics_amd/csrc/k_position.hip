@@ -104,9 +104,9 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const SimParams
 }
 
 __global__ void __launch_bounds__(64) k_position(DevBodies b, SimParams p, SolverBufs sb, int kv) {
-  const int col = sb.visit[kv];
-  const int first_slot = int(sb.col_start[col]);
-  const int nslots = int(sb.col_start[col + 1]) - first_slot;
+  const int2 range = sb.vrange[kv];
+  const int first_slot = range.x;
+  const int nslots = range.y;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nslots; t += gridDim.x * blockDim.x)
     position_one(b, p, sb, first_slot + t);
 }

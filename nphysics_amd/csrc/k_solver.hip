@@ -475,10 +475,12 @@ __device__ __forceinline__ void store_dv(const DevBodies& b, const Manifold& m, 
   if (m.x.d2) b.store_dv(m.x.b2, v2.l, v2.a);
 }
 
+// Slot range of the class visited k-th: one load (the host writes the ranges with the visiting
+// order), not a visit -> class start chain.
 __device__ __forceinline__ void colour_range(const SolverBufs& sb, int k, int& first, int& n) {
-  const int col = sb.visit[k];
-  first = int(sb.col_start[col]);
-  n = int(sb.col_start[col + 1]) - first;
+  const int2 r = sb.vrange[k];
+  first = r.x;
+  n = r.y;
 }
 
 // Solver velocity increments of the manifold's bodies in HBM (the per-colour kernels) ...

@@ -91,6 +91,7 @@ struct SolverBufs {
   float4* pgeo;           // [2M] per slot: (half extents 1 | margin 1), (half extents 2 | margin 2)
   long rec_stride;        // record capacity (fixed between reallocations)
   int* visit;             // [max colours] colour classes in visiting order
+  int2* vrange;           // [max colours] (first slot, slots) of the class visited k-th
   double* red;            // residual partials
   uint32_t* tmp;
 };

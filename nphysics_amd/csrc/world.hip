@@ -117,8 +117,8 @@ struct World {
   int colour_rounds = 0;
   std::vector<int> colour_visit;  // colour classes in visiting order
   long solve_count = 0;    // solves performed, never reset
-  DBuf sv_visit;           // device copy of colour_visit
-  int* h_visit = nullptr;  // pinned staging of colour_visit
+  DBuf sv_visit;           // device copy of colour_visit, then the classes' slot ranges in that order
+  int* h_visit = nullptr;  // pinned staging of the same
   // Instantiated graphs of the sweep phases, keyed by a hash of every kernel argument: the
   // launch sequence only depends on the colour count, the iteration counts and the buffers.
   struct GraphEntry {
@@ -204,9 +204,9 @@ struct World {
     HIPCK(hipGetDevice(&device));
     HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_small), 64 * sizeof(uint32_t)));
-    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_visit), (kMaxColours + 2) * sizeof(int)));
+    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_visit), 3 * (kMaxColours + 2) * sizeof(int)));
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_colstart), 2 * (kMaxColours + 2) * sizeof(uint32_t)));
-    HIPCK(sv_visit.ensure((kMaxColours + 2) * sizeof(int)));
+    HIPCK(sv_visit.ensure(3 * (kMaxColours + 2) * sizeof(int)));
     for (auto& e : ev) HIPCK(hipEventCreate(&e));
     for (auto& e : tune_ev) HIPCK(hipEventCreate(&e));
     HIPCK(hipEventCreateWithFlags(&ev_class, hipEventDisableTiming));
@@ -705,6 +705,7 @@ struct World {
     v.sinfo = sv_sinfo.as<int4>();
     v.rec_stride = rec_stride;
     v.visit = sv_visit.as<int>();
+    v.vrange = reinterpret_cast<int2*>(sv_visit.as<int>() + (kMaxColours + 2));
     v.rec2c = sv_rec2c.as<uint32_t>();
     v.rec = sv_rec.as<float4>();
     v.lam = sv_lam.as<float4>();
@@ -826,14 +827,20 @@ struct World {
     HIPCK(hipEventSynchronize(ev_class));
     colour_visit_order(uint64_t(solve_count), ncolours, colour_visit);
     max_class = 0;
+    int* h_range = h_visit + (kMaxColours + 2);
     for (int k = 0; k < ncolours; ++k) {
       const int c = colour_visit[k];
-      max_class = std::max(max_class, int(h_colstart[c + 1] - h_colstart[c]));
+      const int first = int(h_colstart[c]), size = int(h_colstart[c + 1] - h_colstart[c]);
+      max_class = std::max(max_class, size);
       h_visit[k] = c;
+      h_range[2 * k] = first;
+      h_range[2 * k + 1] = size;
     }
-    // h_visit is rewritten only by the next solve, after this copy has completed (that solve's
-    // ev_class is recorded after it in stream order)
+    // h_visit is rewritten only by the next solve, after these copies have completed (that solve's
+    // ev_class is recorded after them in stream order)
     HIPCK(hipMemcpyAsync(sv_visit.p, h_visit, size_t(ncolours) * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(sv_visit.as<int>() + (kMaxColours + 2), h_range, size_t(ncolours) * 2 * sizeof(int),
+                         hipMemcpyHostToDevice, stream));
     return NPHYS_OK;
   }
 
