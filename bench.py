@@ -381,7 +381,7 @@ def main():
                                ("collision_detection", "broad_phase", "narrow_phase", "assembly",
                                 "velocity_resolution", "velocity_update", "position_resolution")},
         "roofline": {
-            "kernel": "k_pgs (one large colour class of one SOR-Prox sweep)",
+            "kernel": "k_pgs (one colour class of one SOR-Prox sweep; average over every launch of the timing region)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,

@@ -100,7 +100,7 @@ __global__ void __launch_bounds__(256) k_colour_round(DevBodies b, DevContacts c
                                                      const uint32_t* __restrict__ off,
                                                      const uint32_t* __restrict__ deg,
                                                      const int* __restrict__ adj, uint32_t* state,
-                                                     uint32_t* remaining) {
+                                                     uint32_t* remaining, uint32_t* last_round) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   if (state[m] != kUncoloured) return;
@@ -147,6 +147,7 @@ __global__ void __launch_bounds__(256) k_colour_round(DevBodies b, DevContacts c
     }
   }
   state[m] = (round << 16) | colour;
+  atomicMax(last_round, round);
 }
 
 __global__ void __launch_bounds__(256) k_colour_keys(int M, const uint32_t* __restrict__ state,
@@ -185,10 +186,12 @@ __global__ void __launch_bounds__(256) k_colour_starts(DevContacts c, int M,
 // count and first contact index, packed into one int4.
 __global__ void __launch_bounds__(256) k_slot_info(DevBodies b, DevContacts c, int M,
                                                   const uint32_t* __restrict__ order,
-                                                  const uint32_t* __restrict__ soff, int4* sinfo) {
+                                                  const uint32_t* __restrict__ soff, int4* sinfo,
+                                                  uint32_t* slot_of) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= M) return;
   const int m = int(order[s]);
+  slot_of[m] = uint32_t(s);
   const int2 bb = c.mbody[m];
   const int d1 = b.status[bb.x] == STATUS_DYNAMIC ? 1 : 0;
   const int d2 = b.status[bb.y] == STATUS_DYNAMIC ? 1 : 0;
@@ -243,9 +246,11 @@ __device__ __forceinline__ void tangents(f3 n, f3& t1, f3& t2) {
 
 __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, DevContacts prev, int prevM,
                                                  int M, SimParams p, SolverBufs sb) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= M) return;
-  const int m = int(sb.order[s]);
+  // manifold order (neighbouring lanes read neighbouring bodies and contacts); the records go to
+  // the manifold's slot in colour order
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const int s = int(sb.slot_of[m]);
   const int2 bb = cur.mbody[m];
   const int b1 = bb.x, b2 = bb.y;
   const bool d1 = b.status[b1] == STATUS_DYNAMIC;
@@ -425,6 +430,8 @@ struct Manifold {
 __device__ __forceinline__ void load_manifold(const DevBodies& b, const SolverBufs& sb, int s, Manifold& m) {
   m.x = slot_ctx(sb, s);
   m.i1.im = m.i2.im = 0.0f;
+  m.i1.m = m33{};
+  m.i2.m = m33{};
   if (m.x.d1) m.i1.m = load_inv(b, m.x.b1, m.i1.im);
   if (m.x.d2) m.i2.m = load_inv(b, m.x.b2, m.i2.im);
 #pragma unroll
@@ -435,7 +442,11 @@ __device__ __forceinline__ void load_manifold(const DevBodies& b, const SolverBu
     }
 }
 
-// Row `row` (0 normal, 1 and 2 friction) of contact k: w = J dv + rhs and the dv update.
+// Row `row` (0 normal, 1 and 2 friction) of contact k: w = J dv + rhs and the dv update. Both
+// bodies' terms are always evaluated, a static body having zero inverse mass and inertia and a
+// zero dv: its terms are then exact zeros, so w and every dv component equal those of the
+// one-body expression (up to the sign of a zero), and a wavefront mixing ground and body-body
+// manifolds runs one code path.
 struct Row {
   f3 d, a1, a2, wa1, wa2;
   float r, rhs;
@@ -449,20 +460,18 @@ __device__ __forceinline__ Row row_of(const Manifold& m, int k, int row) {
   v.a2 = c.a2[row];
   v.r = c.r[row];
   v.rhs = c.rhs[row];
-  if (m.x.d1) v.wa1 = mulv(m.i1.m, v.a1);
-  if (m.x.d2) v.wa2 = mulv(m.i2.m, v.a2);
+  v.wa1 = mulv(m.i1.m, v.a1);
+  v.wa2 = mulv(m.i2.m, v.a2);
   return v;
 }
 
 __device__ __forceinline__ float row_w(const Row& v, const Manifold& m, const Dv& x1, const Dv& x2) {
-  if (m.x.d1 && m.x.d2) return (dot6(v.d, v.a1, x1) + dot6(neg(v.d), v.a2, x2)) + v.rhs;
-  if (m.x.d1) return dot6(v.d, v.a1, x1) + v.rhs;
-  return dot6(neg(v.d), v.a2, x2) + v.rhs;
+  return (dot6(v.d, v.a1, x1) + dot6(neg(v.d), v.a2, x2)) + v.rhs;
 }
 
 __device__ __forceinline__ void row_add(const Row& v, float alpha, const Manifold& m, Dv& x1, Dv& x2) {
-  if (m.x.d1) axpy6(alpha, scale(v.d, m.i1.im), v.wa1, x1);
-  if (m.x.d2) axpy6(alpha, scale(neg(v.d), m.i2.im), v.wa2, x2);
+  axpy6(alpha, scale(v.d, m.i1.im), v.wa1, x1);
+  axpy6(alpha, scale(neg(v.d), m.i2.im), v.wa2, x2);
 }
 
 __device__ __forceinline__ void load_dv(const DevBodies& b, const Manifold& m, Dv& v1, Dv& v2) {
@@ -870,7 +879,7 @@ void launch_colour_round(const DevBodies& b, const DevContacts& c, int M, int ro
   if (M == 0) return;
   fill_u32(sb.remaining, 0u, 1, s);
   hipLaunchKernelGGL(k_colour_round, dim3(nb(M)), dim3(256), 0, s, b, c, M, uint32_t(round), sb.adj_off,
-                     sb.deg, sb.adj, sb.state, sb.remaining);
+                     sb.deg, sb.adj, sb.state, sb.remaining, sb.remaining + 1);
 }
 
 void launch_colour_keys(int M, SolverBufs& sb, uint32_t* ncol, hipStream_t s) {
@@ -888,7 +897,8 @@ void launch_colour_order(const DevBodies& b_view_for_slots, const DevContacts& c
   hipLaunchKernelGGL(k_colour_starts, dim3(nb(M)), dim3(256), 0, s, c, M, sb.ckeys, sb.order, sb.col_start,
                      sb.mcount_ord);
   scan_exclusive_u32(sb.mcount_ord, sb.soff, M, sb.tmp, s);
-  hipLaunchKernelGGL(k_slot_info, dim3(nb(M)), dim3(256), 0, s, b_view_for_slots, c, M, sb.order, sb.soff, sb.sinfo);
+  hipLaunchKernelGGL(k_slot_info, dim3(nb(M)), dim3(256), 0, s, b_view_for_slots, c, M, sb.order, sb.soff, sb.sinfo,
+                     sb.slot_of);
 }
 
 void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContacts& prev, int prevM,

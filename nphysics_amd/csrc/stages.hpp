@@ -75,7 +75,7 @@ struct SolverBufs {
   uint32_t* adj_fill;     // [n]
   int* adj;               // [2M]
   uint32_t* state;        // [M] colouring state: 0xffffffff, or round << 16 | colour
-  uint32_t* remaining;    // [1]
+  uint32_t* remaining;    // [2] 0: a manifold left uncoloured by the last round, 1: last round that coloured one
   uint32_t* ckeys;        // [M] colour keys
   uint32_t* order;        // [M] manifold ids in colour order
   uint32_t* ckeys_alt;
@@ -84,6 +84,7 @@ struct SolverBufs {
   uint32_t* mcount_ord;   // [M] contact counts in colour order
   uint32_t* soff;         // [M] first row record of each slot
   int4* sinfo;            // [M] per slot: body1|dyn1<<30, body2|dyn2<<30, first record, count|first<<3
+  uint32_t* slot_of;      // [M] slot of each manifold (inverse of order)
   uint32_t* rec2c;        // [C] contact index of each row record
   float4* rec;            // [kRecFields rec_stride] row records, field f of record r at kRecFields r + f
   float4* lam;            // [rec_stride] impulses of each row record (normal, tangent 1, tangent 2, has cache)

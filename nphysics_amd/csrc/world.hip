@@ -111,10 +111,11 @@ struct World {
   // solver
   DBuf sv_state2[2];  // colouring state of the manifolds of cb[k]
   DBuf sv_deg, sv_adj_off, sv_adj_fill, sv_adj, sv_ckeys, sv_order, sv_ckeys_alt,
-      sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_sinfo, sv_rec2c, sv_rec, sv_lam, sv_prec, sv_pgeo, sv_red, sv_tmp;
+      sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_slot_of, sv_sinfo, sv_rec2c, sv_rec, sv_lam, sv_prec, sv_pgeo, sv_red, sv_tmp;
   int ncolours = 0;
   long rec_stride = 0;
   int colour_rounds = 0;
+  static constexpr int kRoundBatch = 4;  // colouring rounds launched between polls
   std::vector<int> colour_visit;  // colour classes in visiting order
   long solve_count = 0;    // solves performed, never reset
   DBuf sv_visit;           // device copy of colour_visit, then the classes' slot ranges in that order
@@ -668,7 +669,7 @@ struct World {
     const size_t m = size_t(std::max(M, 1)), c = size_t(std::max(C, 1));
     for (DBuf* b : {&sv_deg, &sv_adj_off, &sv_adj_fill}) HIPCK(b->ensure(nn * 4));
     HIPCK(sv_adj.ensure(2 * m * 4));
-    for (DBuf* b : {&sv_ckeys, &sv_order, &sv_ckeys_alt, &sv_order_alt, &sv_mcount_ord, &sv_soff})
+    for (DBuf* b : {&sv_ckeys, &sv_order, &sv_ckeys_alt, &sv_order_alt, &sv_mcount_ord, &sv_soff, &sv_slot_of})
       HIPCK(b->ensure(m * 4));
     HIPCK(sv_state2[cur].ensure(m * 4));
     HIPCK(sv_state2[cur ^ 1].ensure(4));
@@ -703,6 +704,7 @@ struct World {
     v.mcount_ord = sv_mcount_ord.as<uint32_t>();
     v.soff = sv_soff.as<uint32_t>();
     v.sinfo = sv_sinfo.as<int4>();
+    v.slot_of = sv_slot_of.as<uint32_t>();
     v.rec_stride = rec_stride;
     v.visit = sv_visit.as<int>();
     v.vrange = reinterpret_cast<int2*>(sv_visit.as<int>() + (kMaxColours + 2));
@@ -792,22 +794,23 @@ struct World {
     const ContactBufs& pc = cb[cur ^ 1];
     const int prevM = have_solved ? pc.M : 0;
     launch_colour_prepare(b, c, M, pc.view(), prevM, sv_state2[cur ^ 1].as<uint32_t>(), v, stream);
-    colour_rounds = 0;
-    for (int round = 1;; ++round) {
-      if (round >= 0xffff) return set_err(NPHYS_ERR_STATE, "colouring did not converge");
-      launch_colour_round(b, c, M, round, v, stream);
-      colour_rounds = round;
-      if ((round & 3) == 0 || round < 4) {
-        HIPCK(hipMemcpyAsync(h_small, v.remaining, 4, hipMemcpyDeviceToHost, stream));
-        HIPCK(hipStreamSynchronize(stream));
-        if (h_small[0] == 0) break;
-      }
+    // Rounds in batches of four with one poll per batch: a round after the last one that colours
+    // anything changes nothing, and the device records the last round that did.
+    fill_u32(v.remaining + 1, 0u, 1, stream);
+    for (int round = 1;; round += kRoundBatch) {
+      if (round + kRoundBatch >= 0xffff) return set_err(NPHYS_ERR_STATE, "colouring did not converge");
+      for (int k = 0; k < kRoundBatch; ++k) launch_colour_round(b, c, M, round + k, v, stream);
+      HIPCK(hipMemcpyAsync(h_small, v.remaining, 4, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipStreamSynchronize(stream));
+      if (h_small[0] == 0) break;
     }
     uint32_t* ncol = bp_small.as<uint32_t>() + 40;
     launch_colour_keys(M, v, ncol, stream);
     HIPCK(hipMemcpyAsync(h_small, ncol, 4, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(h_small + 1, v.remaining + 1, 4, hipMemcpyDeviceToHost, stream));
     HIPCK(hipStreamSynchronize(stream));
     ncolours = int(h_small[0]);
+    colour_rounds = int(h_small[1]);
     if (ncolours > kMaxColours) return set_err(NPHYS_ERR_STATE, "too many colours");
     launch_colour_order(b, c, M, ncolours, v, stream);
     uint32_t* class_rec = v.col_start + (kMaxColours + 2);
