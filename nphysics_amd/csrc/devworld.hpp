@@ -14,7 +14,8 @@
 //            [0] world1 | depth  [1] world2 | feature  [2] normal | kinematic kind
 //            [3] local point 1  [4] local point 2  [5] local normal (the position solver's);
 //            the solved impulse per slot; per manifold: bodies, first slot, count.
-//   rows     per contact in colour order: 8 float4 (see k_solver.hip).
+//   rows     per contact in colour order: a 4-float4 row record and a float4 of impulses (see
+//            k_solver.hip).
 #pragma once
 #include <hip/hip_runtime.h>
 

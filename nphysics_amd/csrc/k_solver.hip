@@ -27,14 +27,14 @@
 // rows (projected on l >= 0). Manifolds of one colour share no dynamic body, so one launch per
 // colour performs exactly the sequential Gauss-Seidel updates of the colour order.
 //
-// Row record (7 contiguous float4 per contact, colour order): 0 n|mu, 1..3 body-1 angular
-// jacobian of rows (n, t1, t2)|r, 4..6 body-2 angular jacobian|rhs. The impulses live in their
-// own array, one float4 per record (n, t1, t2 | 1 if the warm start came from the cache, else
-// 0), so a sweep's impulse writes are contiguous 16-byte runs rather than one dirty line per
-// contact.
+// Row record (4 contiguous float4 per contact, colour order): 0 n|mu, 1 lever arm p1 of body 1
+// (contact point minus centre of mass, zero for a static body)|r_n, 2 p2|r_t1,
+// 3 rhs (n, t1, t2)|r_t2. The impulses live in their own array, one float4 per record (n, t1, t2
+// | 1 if the warm start came from the cache, else 0), so a sweep's impulse writes are contiguous
+// 16-byte runs rather than one dirty line per contact.
 // Linear jacobians are the row direction d (body 1) and -d (body 2) weighted by the inverse
-// mass; tangents and I^-1 J_ang are recomputed in the sweep kernels from n and the bodies'
-// inverse inertia with the assembly's own expressions.
+// mass; the tangents, the angular jacobians p1 x d and p2 x (-d) and I^-1 J_ang are recomputed
+// in the sweep kernels with the assembly's own expressions, so they carry the same FP32 values.
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
     const f3 center2 = sub(w2, scale(n, m2));
     f3 t1, t2;
     tangents(n, t1, t2);
-    float4 rA1[3], rA2[3];
+    float rr3[3], rhs3[3];
     float lam[3];
 #pragma unroll
     for (int row = 0; row < 3; ++row) {
@@ -325,19 +325,22 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
         const float de = (depth + m1) + m2;
         if (de < 0.0f) rhs = rhs + (-de) * p.inv_dt;
       }
-      rA1[row] = f4(a1, rr);
-      rA2[row] = f4(a2, rhs);
+      rr3[row] = rr;
+      rhs3[row] = rhs;
     }
+    // lever arms of the contact point from each dynamic body's centre (fill_geom's p); the sweeps
+    // rebuild every row's angular jacobian from them with fill_geom's own cross product
+    const f3 zero{0.0f, 0.0f, 0.0f};
+    const f3 p1 = d1 ? sub(center1, v1.com) : zero;
+    const f3 p2 = d2 ? sub(center2, v2.com) : zero;
     lam[0] = imp.x * p.warmstart_coeff;
     lam[1] = imp.y * p.warmstart_coeff;
     lam[2] = imp.z * p.warmstart_coeff;
     float4* R = sb.rec + long(r) * kRecFields;
     R[0] = f4(n, mu);
-#pragma unroll
-    for (int row = 0; row < 3; ++row) {
-      R[1 + row] = rA1[row];
-      R[4 + row] = rA2[row];
-    }
+    R[1] = f4(p1, rr3[0]);
+    R[2] = f4(p2, rr3[1]);
+    R[3] = make_float4(rhs3[0], rhs3[1], rhs3[2], rr3[2]);
     sb.lam[r] = make_float4(lam[0], lam[1], lam[2], has);
     sb.rec2c[r] = uint32_t(c);
   }
@@ -394,22 +397,28 @@ __device__ __forceinline__ SlotCtx slot_ctx(const SolverBufs& sb, int s) {
 struct CRec {
   f3 n;
   float mu;
-  f3 a1[3], a2[3];
+  f3 p1, p2;  // lever arms (zero for a static body)
   float r[3], rhs[3];
   float4 lam;
 };
 
+// Angular jacobians of one row along d (d for body 1, -d for body 2), as fill_geom forms them.
+__device__ __forceinline__ f3 ang1(const CRec& c, f3 d) { return cross(c.p1, d); }
+__device__ __forceinline__ f3 ang2(const CRec& c, f3 d) { return cross(c.p2, neg(d)); }
+
 __device__ __forceinline__ void load_rec(const SolverBufs& sb, uint32_t r, CRec& c) {
   const float4* p = sb.rec + long(r) * kRecFields;
-  const float4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3], v4 = p[4], v5 = p[5], v6 = p[6];
+  const float4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
   c.n = xyz(v0);
   c.mu = v0.w;
-  c.a1[0] = xyz(v1); c.r[0] = v1.w;
-  c.a1[1] = xyz(v2); c.r[1] = v2.w;
-  c.a1[2] = xyz(v3); c.r[2] = v3.w;
-  c.a2[0] = xyz(v4); c.rhs[0] = v4.w;
-  c.a2[1] = xyz(v5); c.rhs[1] = v5.w;
-  c.a2[2] = xyz(v6); c.rhs[2] = v6.w;
+  c.p1 = xyz(v1);
+  c.p2 = xyz(v2);
+  c.r[0] = v1.w;
+  c.r[1] = v2.w;
+  c.r[2] = v3.w;
+  c.rhs[0] = v3.x;
+  c.rhs[1] = v3.y;
+  c.rhs[2] = v3.z;
   c.lam = sb.lam[r];
 }
 
@@ -456,8 +465,8 @@ __device__ __forceinline__ Row row_of(const Manifold& m, int k, int row) {
   Row v;
   const CRec& c = m.c[k];
   v.d = row == 0 ? neg(c.n) : (row == 1 ? m.t1[k] : m.t2[k]);
-  v.a1 = c.a1[row];
-  v.a2 = c.a2[row];
+  v.a1 = ang1(c, v.d);
+  v.a2 = ang2(c, v.d);
   v.r = c.r[row];
   v.rhs = c.rhs[row];
   v.wa1 = mulv(m.i1.m, v.a1);
@@ -790,14 +799,18 @@ __global__ void __launch_bounds__(256) k_features(DevBodies b, DevContacts cur, 
     const uint32_t r = x.r0 + k;
     CRec c;
     load_rec(sb, r, c);
+    f3 t1, t2;
+    tangents(c.n, t1, t2);
     const float depth = cur.contact(x.first + k)[0].w + mm;
     const float lsc = c.r[0] * gdt;
     float* f = X + long(r) * 24;
 #pragma unroll
     for (int row = 0; row < 3; ++row) {
       f[row] = c.rhs[row] / gdt;
-      const float a1 = x.d1 ? dot(c.a1[row], mulv(I1, c.a1[row])) : 0.0f;
-      const float a2 = x.d2 ? dot(c.a2[row], mulv(I2, c.a2[row])) : 0.0f;
+      const f3 d = row == 0 ? neg(c.n) : (row == 1 ? t1 : t2);
+      const f3 j1 = ang1(c, d), j2 = ang2(c, d);
+      const float a1 = x.d1 ? dot(j1, mulv(I1, j1)) : 0.0f;
+      const float a2 = x.d2 ? dot(j2, mulv(I2, j2)) : 0.0f;
       f[9 + row] = a1 * c.r[row];
       f[12 + row] = a2 * c.r[row];
     }

@@ -97,7 +97,7 @@ struct SolverBufs {
   uint32_t* tmp;
 };
 constexpr int kMaxColours = 4096;
-constexpr int kRecFields = 7;  // float4 per contact row record
+constexpr int kRecFields = 4;  // float4 per contact row record
 
 void launch_colour_prepare(const DevBodies& b, const DevContacts& c, int M, const DevContacts& prev,
                            int prevM, const uint32_t* prev_state, SolverBufs& sb, hipStream_t s);
