@@ -47,20 +47,20 @@ __global__ void __launch_bounds__(256) k_np_pairs(DevBodies b, const int2* __res
   const int2 pr = pairs[i];
   const int a = pr.x, c = pr.y;
   const float prediction = b.qlim[a] + b.qlim[c];
-  Contact out[4];
+  float4* dst = out_c.contact(long(i) * kPairSlots);
   const int nc = collide_pair(b.shape[a], xyz(b.dims[a]), xyz(b.pos(a)), toq(b.rot(a)), b.shape[c],
-                              xyz(b.dims[c]), xyz(b.pos(c)), toq(b.rot(c)), prediction, out);
+                              xyz(b.dims[c]), xyz(b.pos(c)), toq(b.rot(c)), prediction,
+                              [&](int k, const Contact& ct) {
+                                float4* r = dst + long(k) * kContactFields;
+                                r[0] = f4(ct.w1, ct.depth);
+                                r[1] = f4(ct.w2, __uint_as_float(ct.feature));
+                                r[2] = f4(ct.n, __int_as_float(ct.kind));
+                                r[3] = f4(ct.lp1, 0.0f);
+                                r[4] = f4(ct.lp2, 0.0f);
+                                r[5] = f4(ct.ln, 0.0f);
+                              });
   np.count[i] = uint32_t(nc);
   np.has[i] = nc > 0 ? 1u : 0u;
-  for (int k = 0; k < nc; ++k) {
-    float4* r = out_c.contact(long(i) * kPairSlots + k);
-    r[0] = f4(out[k].w1, out[k].depth);
-    r[1] = f4(out[k].w2, __uint_as_float(out[k].feature));
-    r[2] = f4(out[k].n, __int_as_float(out[k].kind));
-    r[3] = f4(out[k].lp1, 0.0f);
-    r[4] = f4(out[k].lp2, 0.0f);
-    r[5] = f4(out[k].ln, 0.0f);
-  }
 }
 
 // The manifold list: one entry per pair with contacts, in pair order; its contacts stay in the

@@ -124,11 +124,13 @@ NPH_HD int collide_ball_box(f3 c, float r, f3 bt, q4 bq, f3 h, bool ball_first, 
   return 1;
 }
 
+// A clip-polygon vertex: position and tag = code | label << 8 (16 bytes).
 struct ClipVert {
   f3 p;
-  int code;
-  int label;
+  int tag;
 };
+NPH_HD int vcode(const ClipVert& v) { return v.tag & 0xff; }
+NPH_HD int vlabel(const ClipVert& v) { return v.tag >> 8; }
 
 NPH_HD int clip_poly(const ClipVert* in, int nin, f3 n, float off, int plane, ClipVert* out) {
   int no = 0;
@@ -145,12 +147,13 @@ NPH_HD int clip_poly(const ClipVert* in, int nin, f3 n, float off, int plane, Cl
       const float tt = d0 / (d0 - d1);
       ClipVert x;
       x.p = add(prev.p, scale(sub(cur.p, prev.p), tt));
-      x.code = prev.label < 4 ? 4 + prev.label * 4 + plane : 20 + (prev.label - 4) * 4 + plane;
+      const int pl = vlabel(prev);
+      const int code = pl < 4 ? 4 + pl * 4 + plane : 20 + (pl - 4) * 4 + plane;
       if (in0) {
-        x.label = 4 + plane;
+        x.tag = code | ((4 + plane) << 8);
         out[no++] = x;
       } else {
-        x.label = prev.label;
+        x.tag = code | (pl << 8);
         out[no++] = x;
         out[no++] = cur;
       }
@@ -159,8 +162,15 @@ NPH_HD int clip_poly(const ClipVert* in, int nin, f3 n, float off, int plane, Cl
   return no;
 }
 
-NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float prediction,
-                           Contact* out) {
+// Register-friendly element access (selects instead of a dynamically indexed local array).
+NPH_HD f3 sel3(f3 v0, f3 v1, f3 v2, int i) { return i == 0 ? v0 : (i == 1 ? v1 : v2); }
+NPH_HD f3 mcol_sel(const m33& a, int j) {
+  return sel3(f3{a.m[0][0], a.m[1][0], a.m[2][0]}, f3{a.m[0][1], a.m[1][1], a.m[2][1]},
+              f3{a.m[0][2], a.m[1][2], a.m[2][2]}, j);
+}
+
+template <class Emit>
+NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float prediction, Emit&& emit) {
   const m33 RA = qmat(qa);
   const m33 RB = qmat(qb);
   const f3 d = sub(tb, ta);
@@ -177,6 +187,7 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
     for (int j = 0; j < 3; ++j) AC[i][j] = fabsf(dot(a[i], b[j]));
   float sepA = -INFINITY;
   int iA = 0;
+#pragma unroll
   for (int i = 0; i < 3; ++i) {
     const float rb = (hb.x * AC[i][0] + hb.y * AC[i][1]) + hb.z * AC[i][2];
     const float s = fabsf(dot(d, a[i])) - (comp(ha, i) + rb);
@@ -185,6 +196,7 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
   }
   float sepB = -INFINITY;
   int iB = 0;
+#pragma unroll
   for (int j = 0; j < 3; ++j) {
     const float ra = (ha.x * AC[0][j] + ha.y * AC[1][j]) + ha.z * AC[2][j];
     const float s = fabsf(dot(d, b[j])) - (ra + comp(hb, j));
@@ -194,7 +206,9 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
   float sepE = -INFINITY;
   int eI = -1, eJ = -1;
   f3 eAxis{0.0f, 0.0f, 0.0f};
+#pragma unroll
   for (int i = 0; i < 3; ++i)
+#pragma unroll
     for (int j = 0; j < 3; ++j) {
       const f3 L = cross(a[i], b[j]);
       const float l2 = dot(L, L);
@@ -217,18 +231,20 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
     f3 Ln = eAxis;
     if (dot(d, Ln) < 0.0f) Ln = neg(Ln);
     f3 pA = ta;
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
       if (k == eI) continue;
       const float sgn = dot(a[k], Ln) > 0.0f ? 1.0f : -1.0f;
       pA = add(pA, scale(a[k], sgn * comp(ha, k)));
     }
     f3 pB = tb;
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
       if (k == eJ) continue;
       const float sgn = dot(b[k], Ln) > 0.0f ? -1.0f : 1.0f;
       pB = add(pB, scale(b[k], sgn * comp(hb, k)));
     }
-    const f3 da = a[eI], db = b[eJ];
+    const f3 da = sel3(a[0], a[1], a[2], eI), db = sel3(b[0], b[1], b[2], eJ);
     const f3 r = sub(pA, pB);
     const float bb = dot(da, db), c = dot(da, r), f = dot(db, r);
     const float denom = 1.0f - bb * bb;
@@ -245,7 +261,7 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
     }
     const f3 x1 = add(pA, scale(da, s));
     const f3 x2 = add(pB, scale(db, t));
-    Contact& ct = out[0];
+    Contact ct;
     ct.n = Ln;
     ct.w1 = x1;
     ct.w2 = x2;
@@ -255,6 +271,7 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
     ct.lp1 = mulv_t(RA, sub(x1, ta));
     ct.lp2 = mulv_t(RB, sub(x2, tb));
     ct.ln = mulv_t(RA, Ln);
+    emit(0, ct);
     return 1;
   }
 
@@ -266,30 +283,31 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
   const f3 hI = refIsB ? ha : hb;
   const int ir = refIsB ? iB : iA;
   const f3 dRI = sub(tI, tR);
-  const f3 axR = mcol(RR, ir);
+  const f3 axR = mcol_sel(RR, ir);
   const float sR = dot(dRI, axR) >= 0.0f ? 1.0f : -1.0f;
   const f3 nR = scale(axR, sR);
   int ji = 0;
   float bestc = -1.0f;
   float cj = 0.0f;
+#pragma unroll
   for (int j = 0; j < 3; ++j) {
     const float cc = dot(nR, mcol(RI, j));
     if (fabsf(cc) > bestc) { bestc = fabsf(cc); ji = j; cj = cc; }
   }
   const float sI = cj > 0.0f ? -1.0f : 1.0f;
-  const f3 nI = scale(mcol(RI, ji), sI);
+  const f3 nI = scale(mcol_sel(RI, ji), sI);
   const f3 fc = add(tI, scale(nI, comp(hI, ji)));
   const int j1 = (ji + 1) % 3, j2 = (ji + 2) % 3;
-  const f3 u = scale(mcol(RI, j1), comp(hI, j1));
-  const f3 v = scale(mcol(RI, j2), comp(hI, j2));
+  const f3 u = scale(mcol_sel(RI, j1), comp(hI, j1));
+  const f3 v = scale(mcol_sel(RI, j2), comp(hI, j2));
   ClipVert poly0[8], poly1[8];
-  poly0[0] = ClipVert{add(add(fc, u), v), 0, 0};
-  poly0[1] = ClipVert{add(sub(fc, u), v), 1, 1};
-  poly0[2] = ClipVert{sub(sub(fc, u), v), 2, 2};
-  poly0[3] = ClipVert{sub(add(fc, u), v), 3, 3};
+  poly0[0] = ClipVert{add(add(fc, u), v), 0 | (0 << 8)};
+  poly0[1] = ClipVert{add(sub(fc, u), v), 1 | (1 << 8)};
+  poly0[2] = ClipVert{sub(sub(fc, u), v), 2 | (2 << 8)};
+  poly0[3] = ClipVert{sub(add(fc, u), v), 3 | (3 << 8)};
   int np = 4;
   const int r1 = (ir + 1) % 3, r2 = (ir + 2) % 3;
-  const f3 u1 = mcol(RR, r1), u2 = mcol(RR, r2);
+  const f3 u1 = mcol_sel(RR, r1), u2 = mcol_sel(RR, r2);
   const float o1 = dot(u1, tR), o2 = dot(u2, tR);
   const float e1 = comp(hR, r1), e2 = comp(hR, r2);
   np = clip_poly(poly0, np, u1, o1 + e1, 0, poly1);
@@ -297,17 +315,19 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
   np = clip_poly(poly0, np, u2, o2 + e2, 2, poly1);
   np = clip_poly(poly1, np, neg(u2), -o2 + e2, 3, poly0);
   const float offR = dot(nR, tR) + comp(hR, ir);
-  ClipVert cand[8];
+  // candidates: the clipped polygon's vertices within the prediction distance, compacted in place
+  // in poly0 (in order) with their separations in csep
   float csep[8];
   int nc = 0;
   for (int k = 0; k < np; ++k) {
     const float s = dot(nR, poly0[k].p) - offR;
     if (s <= prediction) {
-      cand[nc] = poly0[k];
+      poly0[nc] = poly0[k];
       csep[nc] = s;
       ++nc;
     }
   }
+  ClipVert* cand = poly0;
   if (nc == 0) return 0;
   int keep[8];
   int nk = 0;
@@ -353,9 +373,9 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
     const ClipVert cv = cand[keep[m]];
     const float s = csep[keep[m]];
     const f3 onRef = sub(cv.p, scale(nR, s));
-    Contact& ct = out[m];
+    Contact ct;
     ct.depth = -s;
-    ct.feature = fbase | uint32_t(cv.code);
+    ct.feature = fbase | uint32_t(vcode(cv));
     if (!refIsB) {
       ct.n = nR;
       ct.w1 = onRef;
@@ -374,17 +394,25 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
       ct.lp2 = mulv_t(RB, sub(onRef, tb));
       ct.ln = mulv_t(RB, n);
     }
+    emit(m, ct);
   }
   return nk;
 }
 
-NPH_HD int collide_pair(int k1, f3 h1, f3 t1, q4 q1, int k2, f3 h2, f3 t2, q4 q2,
-                        float prediction, Contact* out) {
-  if (k1 == SHAPE_BALL && k2 == SHAPE_BALL)
-    return collide_ball_ball(t1, h1.x, t2, h2.x, prediction, false, out);
-  if (k1 == SHAPE_BALL) return collide_ball_box(t1, h1.x, t2, q2, h2, true, prediction, false, out);
-  if (k2 == SHAPE_BALL) return collide_ball_box(t2, h2.x, t1, q1, h1, false, prediction, false, out);
-  return collide_box_box(t1, q1, h1, t2, q2, h2, prediction, out);
+// Contact generation of one pair; emit(k, contact) is called for each of its k < 4 contacts.
+template <class Emit>
+NPH_HD int collide_pair(int k1, f3 h1, f3 t1, q4 q1, int k2, f3 h2, f3 t2, q4 q2, float prediction,
+                        Emit&& emit) {
+  if (k1 == SHAPE_BALL || k2 == SHAPE_BALL) {
+    Contact c;
+    int n;
+    if (k1 == SHAPE_BALL && k2 == SHAPE_BALL) n = collide_ball_ball(t1, h1.x, t2, h2.x, prediction, false, &c);
+    else if (k1 == SHAPE_BALL) n = collide_ball_box(t1, h1.x, t2, q2, h2, true, prediction, false, &c);
+    else n = collide_ball_box(t2, h2.x, t1, q1, h1, false, prediction, false, &c);
+    if (n) emit(0, c);
+    return n;
+  }
+  return collide_box_box(t1, q1, h1, t2, q2, h2, prediction, emit);
 }
 
 // Contact re-evaluation at the current poses (geometric, without margins).
