@@ -253,11 +253,16 @@ __global__ void __launch_bounds__(256) k_pack_nodes(DevBodies b, const uint32_t*
 
 // Walks the packed tree for the loose AABB of body q and calls leaf(o, dynamic) for every other
 // body of q's scene whose loose AABB overlaps it. Returns false on a stack overflow.
+// The traversal stack lives in LDS, interleaved across the workgroup's lanes (entry k of lane t at
+// stack[k * kTraverseBlock + t]): a stack in private memory would be scratch, one memory round trip
+// per push and pop on the traversal's dependent chain.
+constexpr int kTraverseBlock = 64;
+
 template <bool kScenes, class F>
-__device__ __forceinline__ bool walk_tree(const DevBodies& b, const float4* __restrict__ pnode, int q, F&& leaf) {
+__device__ __forceinline__ bool walk_tree(const DevBodies& b, const float4* __restrict__ pnode, int q,
+                                          int* __restrict__ stack, F&& leaf) {
   const aabb Q{xyz(b.loose_mn[q]), xyz(b.loose_mx[q])};
   const int qs = kScenes ? b.scene[q] : 0;
-  int stack[kStack];
   int sp = 0;
   int node = 0;
   bool ok = true;
@@ -275,22 +280,23 @@ __device__ __forceinline__ bool walk_tree(const DevBodies& b, const float4* __re
           const int o = int(c & 0x3fffffffu);
           if (o != q && (!kScenes || b.scene[o] == qs)) leaf(o, ((c >> 30) & 1u) != 0);
         } else if (sp < kStack) {
-          stack[sp++] = int(c);
+          stack[(sp++) * kTraverseBlock] = int(c);
         } else {
           ok = false;
         }
       }
     }
     if (sp == 0) break;
-    node = stack[--sp];
+    node = stack[(--sp) * kTraverseBlock];
   }
   return ok;
 }
 
 template <bool kScenes>
-__global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* __restrict__ vals, int n,
+__global__ void __launch_bounds__(kTraverseBlock) k_traverse(DevBodies b, const uint32_t* __restrict__ vals, int n,
                                                  const float4* __restrict__ pnode, int* cand,
                                                  uint32_t* cand_count, int cap, uint32_t* stats) {
+  __shared__ int s_stack[kStack * kTraverseBlock];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const int q = int(vals[t]);
@@ -300,7 +306,7 @@ __global__ void __launch_bounds__(256) k_traverse(DevBodies b, const uint32_t* _
   }
   const long base = long(q) * cap;
   uint32_t cnt = 0;
-  const bool ok = walk_tree<kScenes>(b, pnode, q, [&](int o, bool dyn) {
+  const bool ok = walk_tree<kScenes>(b, pnode, q, s_stack + threadIdx.x, [&](int o, bool dyn) {
     if (!dyn || o > q) {
       if (cnt < uint32_t(cap)) cand[base + cnt] = o;
       ++cnt;
@@ -378,17 +384,18 @@ __global__ void __launch_bounds__(256) k_refit_dirty(DevBodies b, const int* __r
 }
 
 template <bool kScenes>
-__global__ void __launch_bounds__(256) k_traverse_dirty(DevBodies b, const int* __restrict__ list,
+__global__ void __launch_bounds__(kTraverseBlock) k_traverse_dirty(DevBodies b, const int* __restrict__ list,
                                                        const uint32_t* __restrict__ list_count,
                                                        const float4* __restrict__ pnode,
                                                        const uint32_t* __restrict__ dirty, int* cand,
                                                        uint32_t* cand_count, int cap, uint32_t* stats) {
+  __shared__ int s_stack[kStack * kTraverseBlock];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= int(*list_count)) return;
   const int q = list[t];
   const long base = long(q) * cap;
   uint32_t cnt = 0, most = 0;
-  const bool ok = walk_tree<kScenes>(b, pnode, q, [&](int o, bool dyn) {
+  const bool ok = walk_tree<kScenes>(b, pnode, q, s_stack + threadIdx.x, [&](int o, bool dyn) {
     if (!dyn || o > q) {
       if (cnt < uint32_t(cap)) cand[base + cnt] = o;
       ++cnt;
@@ -470,10 +477,10 @@ void launch_traverse(const DevBodies& b, BroadPhaseBufs& bp, int cap, hipStream_
     return;
   }
   if (bp.scene_bits)
-    hipLaunchKernelGGL(k_traverse<true>, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.pnode, bp.cand,
+    hipLaunchKernelGGL(k_traverse<true>, dim3(unsigned((n + kTraverseBlock - 1) / kTraverseBlock)), dim3(kTraverseBlock), 0, s, b, bp.vals, n, bp.pnode, bp.cand,
                        bp.cand_count, cap, bp.stats);
   else
-    hipLaunchKernelGGL(k_traverse<false>, dim3(nb(n)), dim3(256), 0, s, b, bp.vals, n, bp.pnode, bp.cand,
+    hipLaunchKernelGGL(k_traverse<false>, dim3(unsigned((n + kTraverseBlock - 1) / kTraverseBlock)), dim3(kTraverseBlock), 0, s, b, bp.vals, n, bp.pnode, bp.cand,
                        bp.cand_count, cap, bp.stats);
 }
 
@@ -491,10 +498,10 @@ void launch_incremental_pairs(const DevBodies& b, BroadPhaseBufs& bp, int cap, i
   hipLaunchKernelGGL(k_pack_nodes, dim3(nb(n - 1)), dim3(256), 0, s, b, bp.vals, n, bp.child, bp.node_mn,
                      bp.node_mx, bp.node_scene, bp.pnode);
   if (bp.scene_bits)
-    hipLaunchKernelGGL(k_traverse_dirty<true>, dim3(nb(max_dirty)), dim3(256), 0, s, b, bp.list, bp.list_count,
+    hipLaunchKernelGGL(k_traverse_dirty<true>, dim3(unsigned((max_dirty + kTraverseBlock - 1) / kTraverseBlock)), dim3(kTraverseBlock), 0, s, b, bp.list, bp.list_count,
                        bp.pnode, bp.dirty, bp.cand, bp.cand_count, cap, bp.stats);
   else
-    hipLaunchKernelGGL(k_traverse_dirty<false>, dim3(nb(max_dirty)), dim3(256), 0, s, b, bp.list, bp.list_count,
+    hipLaunchKernelGGL(k_traverse_dirty<false>, dim3(unsigned((max_dirty + kTraverseBlock - 1) / kTraverseBlock)), dim3(kTraverseBlock), 0, s, b, bp.list, bp.list_count,
                        bp.pnode, bp.dirty, bp.cand, bp.cand_count, cap, bp.stats);
 }
 

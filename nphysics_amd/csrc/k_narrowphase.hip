@@ -18,9 +18,10 @@ namespace nph {
 
 namespace {
 
-// Lanes are binned by pair kind so a wavefront runs one contact generator: cuboid-cuboid pairs
-// (SAT + clipping, the expensive one) first, all other pairs after. Only the lane -> pair mapping
-// changes; every pair still writes its own slots, so the output is the unbinned one.
+// Lanes are binned by pair kind: cuboid-cuboid pairs (SAT + clipping, the expensive one) first,
+// run by k_np_boxes with their clip polygons in LDS, all other pairs after, run by k_np_balls.
+// Only the lane -> pair mapping changes; every pair still writes its own slots, so the output is
+// the unbinned one.
 __global__ void __launch_bounds__(256) k_np_flag(DevBodies b, const int2* __restrict__ pairs, int npairs,
                                                 uint32_t* flag) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -30,35 +31,65 @@ __global__ void __launch_bounds__(256) k_np_flag(DevBodies b, const int2* __rest
 }
 
 __global__ void __launch_bounds__(256) k_np_perm(int npairs, const uint32_t* __restrict__ flag,
-                                                const uint32_t* __restrict__ before, uint32_t* perm) {
+                                                const uint32_t* __restrict__ before, uint32_t* perm,
+                                                uint32_t* nbox) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= npairs) return;
   const uint32_t nbb = before[npairs - 1] + flag[npairs - 1];
+  if (i == 0) *nbox = nbb;
   const uint32_t b = before[i];
   perm[flag[i] ? b : nbb + (uint32_t(i) - b)] = uint32_t(i);
 }
 
-__global__ void __launch_bounds__(256) k_np_pairs(DevBodies b, const int2* __restrict__ pairs,
-                                                 int npairs, const uint32_t* __restrict__ perm,
-                                                 NarrowPhaseBufs np, DevContacts out_c) {
+// Writes contact k of pair i as a 96-byte record into the pair's slots.
+__device__ __forceinline__ void put_contact(const DevContacts& out_c, int i, int k, const Contact& ct) {
+  float4* r = out_c.contact(long(i) * kPairSlots + k);
+  r[0] = f4(ct.w1, ct.depth);
+  r[1] = f4(ct.w2, __uint_as_float(ct.feature));
+  r[2] = f4(ct.n, __int_as_float(ct.kind));
+  r[3] = f4(ct.lp1, 0.0f);
+  r[4] = f4(ct.lp2, 0.0f);
+  r[5] = f4(ct.ln, 0.0f);
+}
+
+// Cuboid-cuboid pairs (lanes [0, nbox) of the binned order): SAT + clipping, the clip polygons
+// and separations of each lane interleaved in LDS.
+constexpr int kBoxBlock = 64;
+__global__ void __launch_bounds__(kBoxBlock) k_np_boxes(DevBodies b, const int2* __restrict__ pairs, int npairs,
+                                                       const uint32_t* __restrict__ perm,
+                                                       const uint32_t* __restrict__ nbox, NarrowPhaseBufs np,
+                                                       DevContacts out_c) {
+  __shared__ ClipVert s_poly[2 * 8 * kBoxBlock];
+  __shared__ float s_sep[8 * kBoxBlock];
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= int(*nbox)) return;
+  const int i = int(perm[j]);
+  const int2 pr = pairs[i];
+  const int a = pr.x, c = pr.y;
+  const float prediction = b.qlim[a] + b.qlim[c];
+  const PolyBuf p0{s_poly + threadIdx.x, kBoxBlock}, p1{s_poly + 8 * kBoxBlock + threadIdx.x, kBoxBlock};
+  const SepBuf sep{s_sep + threadIdx.x, kBoxBlock};
+  const int nc = collide_box_box(xyz(b.pos(a)), toq(b.rot(a)), xyz(b.dims[a]), xyz(b.pos(c)), toq(b.rot(c)),
+                                 xyz(b.dims[c]), prediction, p0, p1, sep,
+                                 [&](int k, const Contact& ct) { put_contact(out_c, i, k, ct); });
+  np.count[i] = uint32_t(nc);
+  np.has[i] = nc > 0 ? 1u : 0u;
+}
+
+// Pairs with a ball (lanes [nbox, npairs)).
+__global__ void __launch_bounds__(256) k_np_balls(DevBodies b, const int2* __restrict__ pairs, int npairs,
+                                                 const uint32_t* __restrict__ perm,
+                                                 const uint32_t* __restrict__ nbox, NarrowPhaseBufs np,
+                                                 DevContacts out_c) {
+  const int j = int(*nbox) + blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npairs) return;
   const int i = int(perm[j]);
   const int2 pr = pairs[i];
   const int a = pr.x, c = pr.y;
   const float prediction = b.qlim[a] + b.qlim[c];
-  float4* dst = out_c.contact(long(i) * kPairSlots);
-  const int nc = collide_pair(b.shape[a], xyz(b.dims[a]), xyz(b.pos(a)), toq(b.rot(a)), b.shape[c],
-                              xyz(b.dims[c]), xyz(b.pos(c)), toq(b.rot(c)), prediction,
-                              [&](int k, const Contact& ct) {
-                                float4* r = dst + long(k) * kContactFields;
-                                r[0] = f4(ct.w1, ct.depth);
-                                r[1] = f4(ct.w2, __uint_as_float(ct.feature));
-                                r[2] = f4(ct.n, __int_as_float(ct.kind));
-                                r[3] = f4(ct.lp1, 0.0f);
-                                r[4] = f4(ct.lp2, 0.0f);
-                                r[5] = f4(ct.ln, 0.0f);
-                              });
+  const int nc = collide_ball_pair(b.shape[a], xyz(b.dims[a]), xyz(b.pos(a)), toq(b.rot(a)), b.shape[c],
+                                   xyz(b.dims[c]), xyz(b.pos(c)), toq(b.rot(c)), prediction,
+                                   [&](int k, const Contact& ct) { put_contact(out_c, i, k, ct); });
   np.count[i] = uint32_t(nc);
   np.has[i] = nc > 0 ? 1u : 0u;
 }
@@ -107,8 +138,11 @@ void launch_narrow_phase(const DevBodies& b, const int2* pairs, int npairs, Narr
   // binning by kind: flags in np.has, their exclusive scan in np.moff (both rewritten below)
   hipLaunchKernelGGL(k_np_flag, dim3(g), dim3(256), 0, s, b, pairs, npairs, np.has);
   scan_exclusive_u32(np.has, np.moff, npairs, np.tmp, s);
-  hipLaunchKernelGGL(k_np_perm, dim3(g), dim3(256), 0, s, npairs, np.has, np.moff, np.perm);
-  hipLaunchKernelGGL(k_np_pairs, dim3(g), dim3(256), 0, s, b, pairs, npairs, np.perm, np, out);
+  uint32_t* nbox = np.totals + 2;
+  hipLaunchKernelGGL(k_np_perm, dim3(g), dim3(256), 0, s, npairs, np.has, np.moff, np.perm, nbox);
+  hipLaunchKernelGGL(k_np_boxes, dim3(unsigned((npairs + kBoxBlock - 1) / kBoxBlock)), dim3(kBoxBlock), 0, s, b,
+                     pairs, npairs, np.perm, nbox, np, out);
+  hipLaunchKernelGGL(k_np_balls, dim3(g), dim3(256), 0, s, b, pairs, npairs, np.perm, nbox, np, out);
   scan_exclusive_u32(np.count, np.coff, npairs, np.tmp, s);
   scan_exclusive_u32(np.has, np.moff, npairs, np.tmp, s);
   hipLaunchKernelGGL(k_np_manifolds, dim3(g), dim3(256), 0, s, pairs, npairs, np, out);

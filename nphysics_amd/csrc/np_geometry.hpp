@@ -132,7 +132,20 @@ struct ClipVert {
 NPH_HD int vcode(const ClipVert& v) { return v.tag & 0xff; }
 NPH_HD int vlabel(const ClipVert& v) { return v.tag >> 8; }
 
-NPH_HD int clip_poly(const ClipVert* in, int nin, f3 n, float off, int plane, ClipVert* out) {
+// Strided views of per-lane scratch arrays (element k of lane t at base[k * stride]), so that a
+// workgroup's lanes can keep their clip polygons interleaved in LDS.
+struct PolyBuf {
+  ClipVert* base;
+  int stride;
+  NPH_HD ClipVert& operator[](int k) const { return base[long(k) * stride]; }
+};
+struct SepBuf {
+  float* base;
+  int stride;
+  NPH_HD float& operator[](int k) const { return base[long(k) * stride]; }
+};
+
+NPH_HD int clip_poly(PolyBuf in, int nin, f3 n, float off, int plane, PolyBuf out) {
   int no = 0;
   if (nin == 0) return 0;
   for (int k = 0; k < nin; ++k) {
@@ -169,8 +182,10 @@ NPH_HD f3 mcol_sel(const m33& a, int j) {
               f3{a.m[0][2], a.m[1][2], a.m[2][2]}, j);
 }
 
+// poly0 and poly1 hold 8 vertices each and csep 8 floats.
 template <class Emit>
-NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float prediction, Emit&& emit) {
+NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float prediction, PolyBuf poly0,
+                           PolyBuf poly1, SepBuf csep, Emit&& emit) {
   const m33 RA = qmat(qa);
   const m33 RB = qmat(qb);
   const f3 d = sub(tb, ta);
@@ -300,7 +315,6 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
   const int j1 = (ji + 1) % 3, j2 = (ji + 2) % 3;
   const f3 u = scale(mcol_sel(RI, j1), comp(hI, j1));
   const f3 v = scale(mcol_sel(RI, j2), comp(hI, j2));
-  ClipVert poly0[8], poly1[8];
   poly0[0] = ClipVert{add(add(fc, u), v), 0 | (0 << 8)};
   poly0[1] = ClipVert{add(sub(fc, u), v), 1 | (1 << 8)};
   poly0[2] = ClipVert{sub(sub(fc, u), v), 2 | (2 << 8)};
@@ -317,7 +331,6 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
   const float offR = dot(nR, tR) + comp(hR, ir);
   // candidates: the clipped polygon's vertices within the prediction distance, compacted in place
   // in poly0 (in order) with their separations in csep
-  float csep[8];
   int nc = 0;
   for (int k = 0; k < np; ++k) {
     const float s = dot(nR, poly0[k].p) - offR;
@@ -327,17 +340,15 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
       ++nc;
     }
   }
-  ClipVert* cand = poly0;
+  const PolyBuf cand = poly0;
   if (nc == 0) return 0;
-  int keep[8];
-  int nk = 0;
-  if (nc <= 4) {
-    for (int k = 0; k < nc; ++k) keep[nk++] = k;
-  } else {
-    int i0 = 0;
+  // at most 4 kept (in candidate order): all of them, or the deepest one and three spreading the
+  // manifold's area
+  int i0 = -1, i1 = -1, i2 = -1, i3 = -1;
+  if (nc > 4) {
+    i0 = 0;
     for (int k = 1; k < nc; ++k)
       if (csep[k] < csep[i0]) i0 = k;
-    int i1 = -1;
     float bd = -1.0f;
     for (int k = 0; k < nc; ++k) {
       if (k == i0) continue;
@@ -345,14 +356,12 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
       const float dd = dot(dd3, dd3);
       if (dd > bd) { bd = dd; i1 = k; }
     }
-    int i2 = -1;
     float ba = -1.0f;
     for (int k = 0; k < nc; ++k) {
       if (k == i0 || k == i1) continue;
       const float ar = fabsf(dot(cross(sub(cand[i1].p, cand[i0].p), sub(cand[k].p, cand[i0].p)), nR));
       if (ar > ba) { ba = ar; i2 = k; }
     }
-    int i3 = -1;
     float bs = -1.0f;
     for (int k = 0; k < nc; ++k) {
       if (k == i0 || k == i1 || k == i2) continue;
@@ -363,15 +372,15 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
       const float sum = (a01 + a12) + a20;
       if (sum > bs) { bs = sum; i3 = k; }
     }
-    for (int k = 0; k < nc; ++k)
-      if (k == i0 || k == i1 || k == i2 || k == i3) keep[nk++] = k;
   }
   const uint32_t faceR = uint32_t(ir * 2 + (sR < 0.0f ? 1 : 0));
   const uint32_t faceI = uint32_t(ji * 2 + (sI < 0.0f ? 1 : 0));
   const uint32_t fbase = 0x4000u | (refIsB ? 0x2000u : 0u) | (faceR << 10) | (faceI << 7);
-  for (int m = 0; m < nk; ++m) {
-    const ClipVert cv = cand[keep[m]];
-    const float s = csep[keep[m]];
+  int m = 0;
+  for (int k = 0; k < nc; ++k) {
+    if (nc > 4 && k != i0 && k != i1 && k != i2 && k != i3) continue;
+    const ClipVert cv = cand[k];
+    const float s = csep[k];
     const f3 onRef = sub(cv.p, scale(nR, s));
     Contact ct;
     ct.depth = -s;
@@ -395,24 +404,22 @@ NPH_HD int collide_box_box(f3 ta, q4 qa, f3 ha, f3 tb, q4 qb, f3 hb, float predi
       ct.ln = mulv_t(RB, n);
     }
     emit(m, ct);
+    ++m;
   }
-  return nk;
+  return m;
 }
 
-// Contact generation of one pair; emit(k, contact) is called for each of its k < 4 contacts.
+// Contact generation of a pair with at least one ball (at most one contact, emitted as emit(0, c)).
 template <class Emit>
-NPH_HD int collide_pair(int k1, f3 h1, f3 t1, q4 q1, int k2, f3 h2, f3 t2, q4 q2, float prediction,
-                        Emit&& emit) {
-  if (k1 == SHAPE_BALL || k2 == SHAPE_BALL) {
-    Contact c;
-    int n;
-    if (k1 == SHAPE_BALL && k2 == SHAPE_BALL) n = collide_ball_ball(t1, h1.x, t2, h2.x, prediction, false, &c);
-    else if (k1 == SHAPE_BALL) n = collide_ball_box(t1, h1.x, t2, q2, h2, true, prediction, false, &c);
-    else n = collide_ball_box(t2, h2.x, t1, q1, h1, false, prediction, false, &c);
-    if (n) emit(0, c);
-    return n;
-  }
-  return collide_box_box(t1, q1, h1, t2, q2, h2, prediction, emit);
+NPH_HD int collide_ball_pair(int k1, f3 h1, f3 t1, q4 q1, int k2, f3 h2, f3 t2, q4 q2, float prediction,
+                             Emit&& emit) {
+  Contact c;
+  int n;
+  if (k1 == SHAPE_BALL && k2 == SHAPE_BALL) n = collide_ball_ball(t1, h1.x, t2, h2.x, prediction, false, &c);
+  else if (k1 == SHAPE_BALL) n = collide_ball_box(t1, h1.x, t2, q2, h2, true, prediction, false, &c);
+  else n = collide_ball_box(t2, h2.x, t1, q1, h1, false, prediction, false, &c);
+  if (n) emit(0, c);
+  return n;
 }
 
 // Contact re-evaluation at the current poses (geometric, without margins).

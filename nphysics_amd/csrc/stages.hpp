@@ -57,7 +57,7 @@ struct NarrowPhaseBufs {
   uint32_t* has;          // [P] count > 0
   uint32_t* moff;         // [P] exclusive scan of has
   uint32_t* perm;         // [P] pairs binned by kind (lane -> pair)
-  uint32_t* totals;       // [2] contacts, manifolds
+  uint32_t* totals;       // [3] contacts, manifolds, cuboid-cuboid pairs
   uint32_t* tmp;
 };
 void launch_narrow_phase(const DevBodies& b, const int2* pairs, int npairs, NarrowPhaseBufs& np,
