@@ -108,11 +108,11 @@ typedef struct nphys_counters {
   int64_t nmanifolds;
   int64_t nconstraints;
   int64_t ncolours;
-  int64_t colour_rounds;    /* accumulated greedy-colouring rounds */
-  /* Per-launch device time of the SOR-Prox sweep kernel over one large colour class, summed over
+  int64_t colour_rounds;    /* accumulated greedy-colouring rounds (the last round that coloured) */
+  /* Per-launch device time of the SOR-Prox sweep kernel over one colour class, summed over
    * launches, the launch count and the contacts those launches processed; measured with HIP events
-   * in every such launch while counters are enabled (the single-workgroup launch of the small
-   * classes is not included). */
+   * carried by every per-colour sweep launch while counters are enabled (a scene-blocked variant
+   * runs all its sweeps in one launch, counted as one). */
   double sweep_kernel_time;
   int64_t sweep_kernel_launches;
   int64_t sweep_kernel_contacts;
