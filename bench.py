@@ -201,6 +201,51 @@ def residual_target_section(scene, snapshot, params, target, steps, max_sweeps):
     return out
 
 
+def cpu_baseline_batched(scene, snapshot, params, steps):
+    """The FP32 CPU restatement on the batch from the same state, one world per scene: on one
+    thread (the reference's own threading), and on all of the host's cores (at most 16) with a
+    thread pool over the independent worlds. Each world takes one untimed step first."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import oracle
+
+    sc = scene["scene"]
+    n = sc.shape[0]
+    order = np.argsort(sc, kind="stable")
+    bounds = np.searchsorted(sc[order], np.arange(int(sc.max()) + 2))
+    worlds = []
+    for s in range(int(sc.max()) + 1):
+        idx = order[bounds[s]:bounds[s + 1]]
+        sub = {k: v[idx] for k, v in scene.items() if isinstance(v, np.ndarray) and v.shape[:1] == (n,)}
+        for k in ("pos", "rot", "linvel", "angvel"):
+            sub[k] = snapshot[k][idx]
+        sub["scene"] = np.zeros(idx.shape[0], np.int32)
+        o = oracle.OracleWorld(params, precision=32)
+        o.add_bodies(sub)
+        worlds.append(o)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda o: o.step(1), worlds))
+    # one thread over a fixed share of the worlds (every 16th), scaled to the whole batch
+    share = worlds[::16]
+    t0 = time.perf_counter()
+    for o in share:
+        o.step(steps)
+    t_one = (time.perf_counter() - t0) * len(worlds) / len(share)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        list(ex.map(lambda o: o.step(steps), worlds))
+    t_all = time.perf_counter() - t0
+    where = f"on {cpu_model()} ({os.cpu_count()} logical CPUs)"
+    single = {"value": steps / t_one, "unit": "steps/s", "cores": 1, "kind": "port",
+              "sample": f"{steps} steps of every 16th of the {len(worlds)} settled worlds (one oracle world "
+                        f"each) after one untimed step, FP32, single thread, scaled to the whole batch, {where}"}
+    every = {"value": steps / t_all, "unit": "steps/s", "cores": threads, "kind": "port",
+             "sample": f"{steps} steps of all {len(worlds)} settled worlds, FP32, a pool of {threads} threads "
+                       f"over the independent worlds, {where}"}
+    return single, every
+
+
 def bench_batched(args):
     """4,096 independent worlds (lognormal sizes, median 64 bodies, each with its own ground)
     stepped as one handle; 8 velocity / 3 position iterations. Prints one JSON line."""
@@ -214,6 +259,7 @@ def bench_batched(args):
     settle(w, settle_steps)
     log(f"settled in {time.perf_counter() - t0:.1f} s")
     w.step(args.warmup)
+    snapshot = w.get_state()
     w.reset_counters()
     t0 = time.perf_counter()
     w.step(args.steps)
@@ -244,6 +290,14 @@ def bench_batched(args):
                                ("collision_detection", "broad_phase", "narrow_phase", "assembly",
                                 "velocity_resolution", "velocity_update", "position_resolution")},
     }
+    if not args.no_cpu:
+        log("timing the CPU baselines ...")
+        out["cpu_baseline"], out["cpu_baseline_all_cores"] = cpu_baseline_batched(scene, snapshot, params,
+                                                                                  args.cpu_steps)
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        out["speedup_vs_cpu_all_cores"] = out["value"] / out["cpu_baseline_all_cores"]["value"]
+    else:
+        out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
 
 
