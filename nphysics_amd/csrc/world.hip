@@ -130,6 +130,9 @@ struct World {
   bool use_graphs = true;
   int solved_M = 0, solved_C = 0;
   double res_inf = 0.0, res_rms = 0.0;
+  double* red_pinned = nullptr;  // deferred residual partials (pinned)
+  size_t red_pinned_cap = 0;
+  int res_pending_blocks = -1, res_pending_C = 0;
   // solve-to-residual mode (off when res_target <= 0): see nphys_set_residual_target
   double res_target = 0.0;
   int res_max_sweeps = 500, res_check_every = 1, res_norm = NPHYS_RESIDUAL_MAX;
@@ -172,6 +175,7 @@ struct World {
   ~World() {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (h_visit) (void)hipHostFree(h_visit);
+    if (red_pinned) (void)hipHostFree(red_pinned);
     if (h_small) (void)hipHostFree(h_small);
     if (h_colstart) (void)hipHostFree(h_colstart);
     for (auto& e : ev)
@@ -534,6 +538,23 @@ struct World {
     return NPHYS_OK;
   }
 
+  // The end-of-step syncs (CCD-phase swept AABBs, then the final poses) with one readback.
+  int loose_passes_end_of_step(uint32_t* ch_swept, uint32_t* ch_final) {
+    int rc = ensure_bp();
+    if (rc) return rc;
+    BroadPhaseBufs v = bp_view();
+    fill_u32(v.changed, 0u, 4, stream);
+    launch_loose_update(bodies(), 2, v.changed, v.dirty, stream);
+    launch_loose_update(bodies(), 1, v.changed + 2, v.dirty, stream);
+    HIPCK(hipMemcpyAsync(h_small, v.changed, 16, hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    *ch_swept = h_small[0];
+    *ch_final = h_small[2];
+    dirty_pending += int64_t(h_small[0]) + h_small[2];
+    dirty_static_pending += int64_t(h_small[1]) + h_small[3];
+    return NPHYS_OK;
+  }
+
   int rebuild_pairs() {
     if (n < 2) {
       npairs = 0;
@@ -547,25 +568,31 @@ struct World {
     bool done = false;
     if (tree_valid && incremental_bp && !pairs_dirty && dirty_static_pending == 0 &&
         inc_since_full < kMaxIncremental && dirty_pending <= n / 8) {
+      // the pair offsets are computed right away and read back with the overflow check; they are
+      // discarded if the update overflowed
       launch_incremental_pairs(b, v, cap, int(dirty_pending), stream);
+      launch_pair_offsets(b, v, cap, stream);
       HIPCK(hipMemcpyAsync(h_small, v.stats, 8, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipMemcpyAsync(h_small + 2, v.cand_off + (n - 1), 4, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipMemcpyAsync(h_small + 3, v.keys_alt + (n - 1), 4, hipMemcpyDeviceToHost, stream));
       HIPCK(hipStreamSynchronize(stream));
       done = h_small[1] == 0 && int(h_small[0]) <= cap;  // else the full build below
       if (done) {
         ++inc_since_full;
         ++counters.incremental_pair_updates;
+        npairs = int(h_small[2] + h_small[3]);
       }
     }
     if (!done) {
       if ((rc = full_traversal(b, v))) return rc;
       inc_since_full = 0;
+      launch_pair_offsets(b, v, cap, stream);
+      // total = off[n-1] + min(count[n-1], cap)
+      HIPCK(hipMemcpyAsync(h_small, v.cand_off + (n - 1), 4, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipMemcpyAsync(h_small + 1, v.keys_alt + (n - 1), 4, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipStreamSynchronize(stream));
+      npairs = int(h_small[0] + h_small[1]);
     }
-    launch_pair_offsets(b, v, cap, stream);
-    // total = off[n-1] + min(count[n-1], cap)
-    HIPCK(hipMemcpyAsync(h_small, v.cand_off + (n - 1), 4, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipMemcpyAsync(h_small + 1, v.keys_alt + (n - 1), 4, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    npairs = int(h_small[0] + h_small[1]);
     HIPCK(pairs.ensure(size_t(std::max(npairs, 1)) * sizeof(int2)));
     v = bp_view();
     launch_compact_pairs(b, v, cap, stream);
@@ -796,20 +823,20 @@ struct World {
     launch_colour_prepare(b, c, M, pc.view(), prevM, sv_state2[cur ^ 1].as<uint32_t>(), v, stream);
     // Rounds in batches of four with one poll per batch: a round after the last one that colours
     // anything changes nothing, and the device records the last round that did.
+    // The sort keys and colour count are computed after every batch and read back with the poll,
+    // so the usual single batch costs one synchronisation; they are recomputed after a further one.
     fill_u32(v.remaining + 1, 0u, 1, stream);
+    uint32_t* ncol = bp_small.as<uint32_t>() + 40;
     for (int round = 1;; round += kRoundBatch) {
       if (round + kRoundBatch >= 0xffff) return set_err(NPHYS_ERR_STATE, "colouring did not converge");
       for (int k = 0; k < kRoundBatch; ++k) launch_colour_round(b, c, M, round + k, v, stream);
-      HIPCK(hipMemcpyAsync(h_small, v.remaining, 4, hipMemcpyDeviceToHost, stream));
+      launch_colour_keys(M, v, ncol, stream);
+      HIPCK(hipMemcpyAsync(h_small, v.remaining, 8, hipMemcpyDeviceToHost, stream));
+      HIPCK(hipMemcpyAsync(h_small + 2, ncol, 4, hipMemcpyDeviceToHost, stream));
       HIPCK(hipStreamSynchronize(stream));
       if (h_small[0] == 0) break;
     }
-    uint32_t* ncol = bp_small.as<uint32_t>() + 40;
-    launch_colour_keys(M, v, ncol, stream);
-    HIPCK(hipMemcpyAsync(h_small, ncol, 4, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipMemcpyAsync(h_small + 1, v.remaining + 1, 4, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    ncolours = int(h_small[0]);
+    ncolours = int(h_small[2]);
     colour_rounds = int(h_small[1]);
     if (ncolours > kMaxColours) return set_err(NPHYS_ERR_STATE, "too many colours");
     launch_colour_order(b, c, M, ncolours, v, stream);
@@ -895,6 +922,40 @@ struct World {
         for (int k = 0; k < ncolours; ++k) launch_pgs_colour(b, cv, k, sblocks, v, stream);
     };
     return run_phase(warm ? 1 : 3, b, cv, v, sblocks, nsweeps, body);
+  }
+
+  // Residual of the fixed-count mode: the partials are copied into pinned memory without waiting
+  // and reduced after the step's final synchronisation (finish_residual), since nothing in the step
+  // needs them.
+  int residual_deferred(const DevBodies& b, const DevContacts& cv, int M, int C, SolverBufs& v) {
+    launch_residual(b, cv, M, v, stream);
+    const int nbk = residual_blocks(M);
+    const size_t need = size_t(nbk) * 2;
+    if (need > red_pinned_cap) {
+      HIPCK(hipStreamSynchronize(stream));  // the previous buffer may be the target of a copy
+      if (red_pinned) (void)hipHostFree(red_pinned);
+      red_pinned = nullptr;
+      red_pinned_cap = 0;
+      HIPCK(hipHostMalloc(reinterpret_cast<void**>(&red_pinned), std::max<size_t>(need, 1024) * sizeof(double)));
+      red_pinned_cap = std::max<size_t>(need, 1024);
+    }
+    HIPCK(hipMemcpyAsync(red_pinned, v.red, need * sizeof(double), hipMemcpyDeviceToHost, stream));
+    res_pending_blocks = nbk;
+    res_pending_C = C;
+    return NPHYS_OK;
+  }
+
+  // Reduces a deferred residual; the stream must have passed its copy.
+  void finish_residual() {
+    if (res_pending_blocks < 0) return;
+    double mx = 0.0, ss = 0.0;
+    for (int k = 0; k < res_pending_blocks; ++k) {
+      mx = std::max(mx, red_pinned[2 * k]);
+      ss += red_pinned[2 * k + 1];
+    }
+    res_inf = mx;
+    res_rms = res_pending_C > 0 ? std::sqrt(ss / double(3 * res_pending_C)) : 0.0;
+    res_pending_blocks = -1;
   }
 
   // Natural-map residual of the current impulses: per-block partials reduced on the host in
@@ -997,7 +1058,7 @@ struct World {
     if (res_target <= 0.0) {
       if ((rc = sweeps(b, cv, v, sblocks, true, prm.max_velocity_iterations))) return rc;
       nsw = prm.max_velocity_iterations;
-      return residual(b, cv, M, C, v);
+      return residual_deferred(b, cv, M, C, v);
     }
     if ((rc = sweeps(b, cv, v, sblocks, true, 0))) return rc;
     int done = 0;
@@ -1157,8 +1218,7 @@ struct World {
     cur ^= 1;
     have_solved = true;
     uint32_t ch1 = 0, ch2 = 0;
-    if ((rc = loose_pass(2, &ch1))) return rc;  // CCD-phase sync: swept AABBs
-    if ((rc = loose_pass(1, &ch2))) return rc;  // final sync
+    if ((rc = loose_passes_end_of_step(&ch1, &ch2))) return rc;  // CCD-phase sync, then the final one
     counters.loose_changes += int64_t(ch1) + int64_t(ch2);
     if (ch1 || ch2 || pairs_dirty) {
       if ((rc = rebuild_pairs())) return rc;
@@ -1168,6 +1228,7 @@ struct World {
     if ((rc = narrow_phase())) return rc;
     HIPCK(hipEventRecord(ev[EV_NP], stream));
     HIPCK(hipEventSynchronize(ev[EV_NP]));
+    finish_residual();
     auto el = [&](int a, int c) {
       float ms = 0.0f;
       (void)hipEventElapsedTime(&ms, ev[a], ev[c]);
