@@ -89,6 +89,15 @@ typedef struct nphys_body_desc {
    * never interact: one handle then steps a batch of independent worlds (the reference needs one
    * MechanicalWorld per scene). */
   const int32_t* scene;
+  /* n: sleep threshold of the energy running average (RigidBodyDesc::sleep_threshold,
+   * rigid_body.rs:986,1028; activation_manager.rs:61-201), or NULL / a negative value for a body
+   * that never sleeps (sleep_threshold(None)). With any threshold set the step first updates every
+   * awake body's energy, groups the dynamic bodies into islands over the contact manifolds and puts
+   * islands whose bodies all have energy below their threshold to sleep (zero velocity, excluded
+   * from the solve and the integration) or wakes them. NULL differs from RigidBodyDesc's default
+   * (Some(0.01)): pass 0.01 for the reference's default. nphys_set_state wakes every sleeping body
+   * (2 x threshold) at the next step, as setting a pose or velocity does in the reference. */
+  const float* sleep_threshold;
 } nphys_body_desc;
 #define NPHYS_MAX_SCENES 32768
 
@@ -128,6 +137,7 @@ typedef struct nphys_counters {
   /* accumulated pair rebuilds done incrementally (only the bodies whose loose AABB was replaced
    * re-traversed, the tree refitted by expansion) instead of by a full tree build */
   int64_t incremental_pair_updates;
+  int64_t active_bodies;  /* awake dynamic bodies after the last step's activation update */
 } nphys_counters;
 
 /* ---------------------------------------------------------------- lifecycle */
@@ -238,6 +248,11 @@ int nphys_set_sweep_variant(nphys_world* w, int32_t variant);
  * way. Replaces ncollide's incremental DBVT update (geometrical_world.rs:285-312; the DBVT itself
  * is not in the reference tree). */
 int nphys_set_incremental_pairs(nphys_world* w, int32_t on);
+
+/* Activation energy of every body (ActivationStatus::energy, body.rs:65-125): 0 for a sleeping
+ * body; 0.04 (4 x the default threshold) for a new body. No counterpart entry point in the
+ * reference, which exposes it per body (Body::activation_status); cap >= the body count. */
+int nphys_read_activation(nphys_world* w, float* energy, int64_t cap);
 
 /* Colour of every manifold solved by the last step, in manifold order. Recording starts with the
  * first call (a call with colours == NULL only enables it), so the first step after enabling is

@@ -91,6 +91,7 @@ class BodyDesc(C.Structure):
         ("friction", C.POINTER(C.c_float)),
         ("restitution", C.POINTER(C.c_float)),
         ("scene", C.POINTER(C.c_int32)),
+        ("sleep_threshold", C.POINTER(C.c_float)),
     ]
 
 
@@ -122,6 +123,7 @@ class Counters(C.Structure):
         ("loose_changes", C.c_int64),
         ("pair_rebuilds", C.c_int64),
         ("incremental_pair_updates", C.c_int64),
+        ("active_bodies", C.c_int64),
     ]
 
     def as_dict(self):
@@ -174,4 +176,6 @@ def make_body_desc(scene):
     d.restitution = fptr(arr("restitution", np.float32, 1))
     if "scene" in scene:
         d.scene = iptr(arr("scene", np.int32, 1))
+    if "sleep_threshold" in scene:
+        d.sleep_threshold = fptr(arr("sleep_threshold", np.float32, 1))
     return d, keep

@@ -8,7 +8,7 @@
 //              [2] (I21, I22, dv_lin.x, dv_lin.y)  [3] (dv_lin.z, dv_ang.x, dv_ang.y, dv_ang.z)
 //            with I the inverse augmented inertia. Structure of arrays for the rest:
 //            pos0/rot0/linvel/angvel, external velocity increment dt*acc (lin, ang), loose AABB
-//            (mn, mx), shape data.
+//            (mn, mx), shape data, and where bodies can sleep (energy, threshold) per body.
 //   contacts one 96-byte record per contact slot, each broad-phase pair owning 4 consecutive slots
 //            (a manifold's contacts are slots 4 p .. 4 p + count - 1 of its pair p):
 //            [0] world1 | depth  [1] world2 | feature  [2] normal | kinematic kind
@@ -55,6 +55,7 @@ struct DevBodies {
   float4* ext_ang;
   float4* loose_mn;
   float4* loose_mx;
+  float2* act;           // (activation energy, sleep threshold); null when no body can sleep
 
   NPH_HD float4& pos(int i) const { return pose[long(i) * kPoseBlock]; }
   NPH_HD float4& rot(int i) const { return pose[long(i) * kPoseBlock + 1]; }

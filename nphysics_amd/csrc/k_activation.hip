@@ -1,0 +1,219 @@
+// nphysics_amd/csrc/k_activation.hip -- sleeping and activation islands.
+//
+// Restates ActivationManager::update (src/detection/activation_manager.rs:61-201) for rigid
+// bodies without joints or kinematic bodies, then the manifold collection of
+// MechanicalWorld::step (src/world/mechanical_world.rs:287-300) that hands the solver only the
+// manifolds with an awake dynamic side:
+//   k_act_wake     (before the step's dynamics update, after the caller set the bodies' state)
+//                  sleeping bodies woken with 2 x threshold, as RigidBody::update_dynamics does
+//                  for a body whose pose or velocity changed (src/object/rigid_body.rs:558-565);
+//   k_act_energy   the energy running average e <- (1 - 0.01) e + 0.01 |v|^2 capped at
+//                  4 x threshold (activation_manager.rs:47-57) of every awake body that has a
+//                  threshold; union-find parents reset;
+//   k_act_union    islands: union over the manifolds whose bodies are both dynamic (:143-147),
+//                  lock-free (a root is linked under the smaller root by compare-and-swap);
+//   k_act_flags    path compression, and an island cannot sleep as soon as one of its bodies has
+//                  no threshold or an energy >= its threshold (:160-172);
+//   k_act_apply    islands that can sleep are put to sleep (energy and velocity zeroed,
+//                  rigid_body.rs:396-400), the others woken with 2 x threshold (body.rs:338-342);
+//                  counts the awake dynamic bodies;
+//   k_act_manifolds, k_act_compact  the solved-manifold list (stable compaction of the manifolds
+//                  with an awake dynamic side) and, for the other manifolds, the impulse cache
+//                  carried over from the previous step: the reference keeps a contact's cached
+//                  impulse as long as the contact exists (signorini_coulomb_pyramid_model.rs:226-261
+//                  only overwrites the entries of solved contacts).
+// Activation state: one float2 per body (energy, threshold; threshold < 0 means the body never
+// sleeps, energy 0 means asleep). Every kernel is one pass over bodies or manifolds; union-find
+// walks are bounded by the body count (parents only point to smaller indices).
+#include <hip/hip_runtime.h>
+
+#include "devworld.hpp"
+#include "kernels.hpp"
+#include "stages.hpp"
+
+namespace nph {
+
+namespace {
+
+constexpr float kMix = 0.01f;  // ActivationManager::new(0.01) (mechanical_world.rs:80)
+
+__device__ inline int uf_load(const int* p, int x) {
+  return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline int uf_root(const int* p, int x, int n) {
+  for (int it = 0; it < n; ++it) {
+    const int y = uf_load(p, x);
+    if (y == x) break;
+    x = y;
+  }
+  return x;
+}
+
+__global__ void __launch_bounds__(256) k_act_wake(DevBodies b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n || b.status[i] != STATUS_DYNAMIC) return;
+  const float2 a = b.act[i];
+  if (a.x == 0.0f && a.y >= 0.0f) b.act[i] = make_float2(a.y * 2.0f, a.y);
+}
+
+__global__ void __launch_bounds__(256) k_act_energy(DevBodies b, int* parent, uint32_t* can) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  parent[i] = i;
+  can[i] = 1u;
+  if (b.status[i] != STATUS_DYNAMIC) return;
+  float2 a = b.act[i];
+  if (a.x != 0.0f && a.y >= 0.0f) {
+    const float4 v = b.linvel[i], w = b.angvel[i];
+    const float nsq = ((((v.x * v.x + v.y * v.y) + v.z * v.z) + w.x * w.x) + w.y * w.y) + w.z * w.z;
+    const float e = (1.0f - kMix) * a.x + kMix * nsq;
+    const float cap = a.y * 4.0f;
+    a.x = e < cap ? e : cap;
+  }
+  b.act[i] = a;
+}
+
+__global__ void __launch_bounds__(256) k_act_union(DevContacts c, int M, const int* __restrict__ status, int* parent,
+                                                  int n) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const int2 bb = c.mbody[m];
+  if (status[bb.x] != STATUS_DYNAMIC || status[bb.y] != STATUS_DYNAMIC) return;
+  int a = uf_root(parent, bb.x, n), d = uf_root(parent, bb.y, n);
+  // every failed exchange means another thread linked root a, which happens at most n - 1 times
+  for (int it = 0; it < 2 * n && a != d; ++it) {
+    if (a < d) {
+      const int t = a;
+      a = d;
+      d = t;
+    }
+    const int old = atomicCAS(parent + a, a, d);
+    if (old == a) break;
+    a = uf_root(parent, old, n);
+    d = uf_root(parent, d, n);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_act_flags(DevBodies b, int* parent, uint32_t* can) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n) return;
+  if (b.status[i] != STATUS_DYNAMIC) return;
+  const int r = uf_root(parent, i, b.n);
+  __hip_atomic_store(parent + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const float2 a = b.act[i];
+  if (!(a.y >= 0.0f && a.x < a.y)) can[r] = 0u;
+}
+
+__global__ void __launch_bounds__(256) k_act_apply(DevBodies b, const int* __restrict__ parent,
+                                                  const uint32_t* __restrict__ can, uint32_t* awake) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool on = false;
+  if (i < b.n && b.status[i] == STATUS_DYNAMIC) {
+    float2 a = b.act[i];
+    if (can[parent[i]]) {
+      if (a.x != 0.0f) {
+        a.x = 0.0f;
+        b.linvel[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        b.angvel[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+    } else if (a.x == 0.0f && a.y >= 0.0f) {
+      a.x = a.y * 2.0f;
+    }
+    b.act[i] = a;
+    on = a.x != 0.0f;
+  }
+  const unsigned long long mask = __ballot(on);
+  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(awake, uint32_t(__popcll(mask)));
+}
+
+__global__ void __launch_bounds__(256) k_act_manifolds(DevBodies b, DevContacts c, int M, uint32_t* flag) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  const int2 bb = c.mbody[m];
+  const bool on1 = b.status[bb.x] == STATUS_DYNAMIC && b.act[bb.x].x != 0.0f;
+  const bool on2 = b.status[bb.y] == STATUS_DYNAMIC && b.act[bb.y].x != 0.0f;
+  flag[m] = (on1 || on2) ? 1u : 0u;
+}
+
+// Solved manifolds to `out` in manifold order; the other manifolds' contacts get the cached
+// impulse of the same contact in the previous step's manifolds (w = -1 marks "no entry").
+// totals[0] = solved manifolds, totals[1] = their contacts.
+__global__ void __launch_bounds__(256) k_act_compact(DevContacts c, int M, const uint32_t* __restrict__ flag,
+                                                    const uint32_t* __restrict__ pos, DevContacts out,
+                                                    DevContacts prev, int prevM, uint32_t* totals) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t cnt = 0;
+  if (m < M) {
+    const int first = c.mfirst[m], count = c.mcount[m];
+    if (flag[m]) {
+      const uint32_t p = pos[m];
+      out.mbody[p] = c.mbody[m];
+      out.mfirst[p] = first;
+      out.mcount[p] = count;
+      cnt = uint32_t(count);
+    } else {
+      const int2 bb = c.mbody[m];
+      int lo = 0, hi = prevM;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int2 k = prev.mbody[mid];
+        if (k.x < bb.x || (k.x == bb.x && k.y < bb.y)) lo = mid + 1;
+        else hi = mid;
+      }
+      int pf = 0, pc = 0;
+      if (lo < prevM) {
+        const int2 k = prev.mbody[lo];
+        if (k.x == bb.x && k.y == bb.y) {
+          pf = prev.mfirst[lo];
+          pc = prev.mcount[lo];
+        }
+      }
+      for (int k = 0; k < count; ++k) {
+        const uint32_t feat = c.feature(first + k);
+        float4 imp = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+        for (int kk = 0; kk < pc; ++kk)
+          if (prev.feature(pf + kk) == feat) {
+            imp = prev.imp[pf + kk];
+            break;
+          }
+        c.imp[first + k] = imp;
+      }
+    }
+    if (m == M - 1) totals[0] = pos[m] + flag[m];
+  }
+  // one atomic per wave for the contact count
+  uint32_t s = cnt;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0 && s) atomicAdd(totals + 1, s);
+}
+
+int nb(long n) { return int((n + 255) / 256); }
+
+}  // namespace
+
+void launch_wake(const DevBodies& b, hipStream_t s) {
+  if (b.n > 0) hipLaunchKernelGGL(k_act_wake, dim3(nb(b.n)), dim3(256), 0, s, b);
+}
+
+void launch_activation(const DevBodies& b, const DevContacts& c, int M, const ActivationBufs& a, hipStream_t s) {
+  if (b.n > 0) {
+    hipLaunchKernelGGL(k_act_energy, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can);
+    if (M > 0) hipLaunchKernelGGL(k_act_union, dim3(nb(M)), dim3(256), 0, s, c, M, b.status, a.parent, b.n);
+    hipLaunchKernelGGL(k_act_flags, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can);
+    hipLaunchKernelGGL(k_act_apply, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can, a.totals + 2);
+  }
+  if (M > 0) {
+    hipLaunchKernelGGL(k_act_manifolds, dim3(nb(M)), dim3(256), 0, s, b, c, M, a.mflag);
+    scan_exclusive_u32(a.mflag, a.pos, M, a.tmp, s);
+  }
+}
+
+void launch_solve_list(const DevContacts& c, int M, const ActivationBufs& a, const DevContacts& out,
+                       const DevContacts& prev, int prevM, hipStream_t s) {
+  if (M > 0)
+    hipLaunchKernelGGL(k_act_compact, dim3(nb(M)), dim3(256), 0, s, c, M, a.mflag, a.pos, out, prev, prevM,
+                       a.totals);
+}
+
+}  // namespace nph

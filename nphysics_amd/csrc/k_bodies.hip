@@ -8,7 +8,9 @@
 //   MoreauJeanSolver::assemble_system's ext_vels = dt * acc and the zeroed mj_lambda
 //     (src/solver/moreau_jean_solver.rs:166-174, 322-326).
 // update_dynamics is evaluated at the start of every step; the reference evaluates it after the
-// previous step's solve on the same pose and velocity, so the values are identical.
+// previous step's solve on the same pose and velocity, so the values are identical. A sleeping
+// body keeps the inverse inertia of the last step it moved in, as the reference does (its update
+// flags stay clear while it sleeps).
 // k_integrate restates MoreauJeanSolver::update_velocities_and_integrate (:328-347) and
 // RigidBody::integrate (rigid_body.rs:467-505) with zero damping and unbounded velocities (the
 // RigidBodyDesc defaults), i.e. v += ext; v += dv; pose <- displacement(v dt) * pose.
@@ -47,18 +49,23 @@ __global__ void __launch_bounds__(256) k_prestep(DevBodies b, SimParams p) {
   const m33 Iw = mmul(mmul(R, L), mtrans(R));
   const f3 w = xyz(b.angvel[i]);
   const f3 iw = mulv(Iw, w);
-  const m33 wx = cross_matrix(scale(w, dt));
-  const m33 iwx = cross_matrix(scale(iw, dt));
-  const m33 wxI = mmul(wx, Iw);
-  m33 aug;
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) aug.m[r][c] = Iw.m[r][c] + (wxI.m[r][c] - iwx.m[r][c]);
-  const float mass = b.mass[i];
-  const float inv_mass = mass == 0.0f ? 0.0f : 1.0f / mass;
+  float inv_mass;
   m33 inv;
-  try_inverse(aug, inv);
+  if (b.act && b.act[i].x == 0.0f) {
+    inv = b.inv_inertia(i, inv_mass);
+  } else {
+    const m33 wx = cross_matrix(scale(w, dt));
+    const m33 iwx = cross_matrix(scale(iw, dt));
+    const m33 wxI = mmul(wx, Iw);
+    m33 aug;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) aug.m[r][c] = Iw.m[r][c] + (wxI.m[r][c] - iwx.m[r][c]);
+    const float mass = b.mass[i];
+    inv_mass = mass == 0.0f ? 0.0f : 1.0f / mass;
+    try_inverse(aug, inv);
+  }
   b.set_inv_inertia(i, inv, inv_mass);
   const f3 gyro = neg(cross(w, iw));
   const f3 acc_ang = mulv(inv, gyro);
@@ -71,6 +78,7 @@ __global__ void __launch_bounds__(256) k_integrate(DevBodies b, SimParams p) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
   if (b.status[i] != STATUS_DYNAMIC) return;
+  if (b.act && b.act[i].x == 0.0f) return;  // asleep: not in the solver's island
   const float4 v = b.linvel[i], w = b.angvel[i];
   const float4 el = b.ext_lin[i], ea = b.ext_ang[i];
   f3 dl, da;

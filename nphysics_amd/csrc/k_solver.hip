@@ -302,8 +302,11 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
     float has = 0.0f;
     for (int kk = 0; kk < pc; ++kk)
       if (prev.feature(pf + kk) == feat) {
-        imp = prev.imp[pf + kk];
-        has = 1.0f;
+        const float4 q = prev.imp[pf + kk];
+        if (q.w >= 0.0f) {  // w < 0: a sleeping contact that had no cached impulse
+          imp = q;
+          has = 1.0f;
+        }
         break;
       }
     const f3 center1 = add(w1, scale(n, m1));

@@ -10,6 +10,20 @@ namespace nph {
 void launch_prestep(const DevBodies& b, const SimParams& p, hipStream_t s);
 void launch_integrate(const DevBodies& b, const SimParams& p, hipStream_t s);
 
+// k_activation.hip ------------------------------------------------------------------------
+struct ActivationBufs {
+  int* parent;       // [n] union-find parents
+  uint32_t* can;     // [n] per island root: every body may sleep
+  uint32_t* mflag;   // [M] manifold handed to the solver
+  uint32_t* pos;     // [M] its index in the solved list
+  uint32_t* tmp;     // scan temporary
+  uint32_t* totals;  // [3] solved manifolds, their contacts, awake dynamic bodies (zeroed by the caller)
+};
+void launch_wake(const DevBodies& b, hipStream_t s);
+void launch_activation(const DevBodies& b, const DevContacts& c, int M, const ActivationBufs& a, hipStream_t s);
+void launch_solve_list(const DevContacts& c, int M, const ActivationBufs& a, const DevContacts& out,
+                       const DevContacts& prev, int prevM, hipStream_t s);
+
 // k_broadphase.hip ------------------------------------------------------------------------
 struct BroadPhaseBufs {
   uint32_t* changed;      // [2] number of loose AABBs replaced: all, of non-dynamic bodies

@@ -83,6 +83,12 @@ struct World {
   DBuf sol;   // kSolverBlock float4 per body: inverse inertia and mass, solver velocity increment
   DBuf pos0, rot0, linvel, angvel, ext_lin, ext_ang;
   DBuf loose_mn, loose_mx;
+  DBuf act;  // float2 per body: activation energy (0: asleep), sleep threshold (< 0: never sleeps)
+  // Sleeping: on once a dynamic body with a threshold exists; a wake of every sleeping body is
+  // due at the next step after the caller set the bodies' state.
+  bool sleeping = false, wake_pending = false;
+  int64_t n_dynamic = 0, active_bodies = 0;
+  DBuf act_parent, act_can, act_pos, act_tmp, act_mflag[2], sl_mbody[2], sl_mfirst[2], sl_mcount[2];
 
   // broad phase
   DBuf bp_dirty, bp_leaf_of, bp_list, bp_pnode;
@@ -109,7 +115,12 @@ struct World {
   bool have_solved = false;
 
   // solver
-  DBuf sv_state2[2];  // colouring state of the manifolds of cb[k]
+  // The manifolds handed to the solver with cb[k]'s contacts: all of cb[k], or (sleeping) the
+  // compacted list of those with an awake dynamic side in sl_*[k].
+  DevContacts sview[2] = {};
+  int sview_M[2] = {0, 0}, sview_C[2] = {0, 0};
+  bool sview_compact[2] = {false, false};
+  DBuf sv_state2[2];  // colouring state of the manifolds of sview[k]
   DBuf sv_deg, sv_adj_off, sv_adj_fill, sv_adj, sv_ckeys, sv_order, sv_ckeys_alt,
       sv_order_alt, sv_col_start, sv_mcount_ord, sv_soff, sv_slot_of, sv_sinfo, sv_rec2c, sv_rec, sv_lam, sv_prec, sv_pgeo, sv_red, sv_tmp;
   int ncolours = 0;
@@ -246,6 +257,7 @@ struct World {
     b.ext_ang = ext_ang.as<float4>();
     b.loose_mn = loose_mn.as<float4>();
     b.loose_mx = loose_mx.as<float4>();
+    b.act = sleeping ? act.as<float2>() : nullptr;
     return b;
   }
 
@@ -260,6 +272,9 @@ struct World {
     if (d.n == 0) return NPHYS_OK;
     const int n0 = n, n1 = n + d.n;
     std::vector<float4> hp(d.n), hr(d.n), hv(d.n), hw(d.n), hmn(d.n), hmx(d.n);
+    std::vector<float2> ha(d.n);
+    bool any_sleeper = false;
+    int64_t new_dynamic = 0;
     for (int k = 0; k < d.n; ++k) {
       const int st = d.status[k], sh = d.shape[k];
       if (st != NPHYS_STATUS_DYNAMIC && st != NPHYS_STATUS_STATIC)
@@ -313,6 +328,11 @@ struct World {
       hp[k] = make_float4(p[0], p[1], p[2], 0.0f);
       hr[k] = make_float4(q[0], q[1], q[2], q[3]);
       const bool dyn = st == NPHYS_STATUS_DYNAMIC;
+      // ActivationStatus::new_active (body.rs:77-82): energy 4 x the default threshold 0.01
+      const float thr = d.sleep_threshold ? d.sleep_threshold[k] : -1.0f;
+      ha[k] = make_float2(0.01f * 4.0f, thr);
+      any_sleeper = any_sleeper || (dyn && thr >= 0.0f);
+      new_dynamic += dyn ? 1 : 0;
       hv[k] = (dyn && d.linvel) ? make_float4(d.linvel[3 * k], d.linvel[3 * k + 1], d.linvel[3 * k + 2], 0.0f)
                                 : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       hw[k] = (dyn && d.angvel) ? make_float4(d.angvel[3 * k], d.angvel[3 * k + 1], d.angvel[3 * k + 2], 0.0f)
@@ -328,7 +348,9 @@ struct World {
     for (DBuf* b : {&linvel, &angvel, &loose_mn, &loose_mx})
       HIPCK(b->ensure(size_t(n1) * f4b, true, stream));
     HIPCK(pose.ensure(size_t(n1) * kPoseBlock * f4b, true, stream));
-    HIPCK(sol.ensure(size_t(n1) * kSolverBlock * f4b));
+    // kept: a sleeping body's inverse inertia is not recomputed
+    HIPCK(sol.ensure(size_t(n1) * kSolverBlock * f4b, true, stream));
+    HIPCK(act.ensure(size_t(n1) * sizeof(float2), true, stream));
     for (DBuf* b : {&pos0, &rot0, &ext_lin, &ext_ang, &dims, &local_i})
       HIPCK(b->ensure(size_t(n1) * f4b));
     for (DBuf* b : {&status, &shape, &scene, &margin, &qlim, &friction, &restitution, &mass})
@@ -351,7 +373,9 @@ struct World {
     HIPCK(up(qlim, h_qlim.data(), size_t(n1) * 4, 0));
     HIPCK(up(friction, h_friction.data(), size_t(n1) * 4, 0));
     HIPCK(up(restitution, h_restitution.data(), size_t(n1) * 4, 0));
-    HIPCK(hipMemsetAsync(sol.p, 0, size_t(n1) * kSolverBlock * f4b, stream));
+    HIPCK(hipMemsetAsync(sol.as<float4>() + size_t(n0) * kSolverBlock, 0, size_t(d.n) * kSolverBlock * f4b, stream));
+    HIPCK(hipMemcpyAsync(act.as<float2>() + n0, ha.data(), size_t(d.n) * sizeof(float2), hipMemcpyHostToDevice,
+                         stream));
     HIPCK(pose_copy(0, hp.data(), n0, d.n, hipMemcpyHostToDevice));
     HIPCK(pose_copy(1, hr.data(), n0, d.n, hipMemcpyHostToDevice));
     HIPCK(up(linvel, hv.data(), size_t(d.n) * f4b, size_t(n0) * f4b));
@@ -403,6 +427,8 @@ struct World {
     tuned = false;
     HIPCK(hipStreamSynchronize(stream));
     n = n1;
+    n_dynamic += new_dynamic;
+    sleeping = sleeping || any_sleeper;
     pairs_dirty = true;
     moved = true;
     return NPHYS_OK;
@@ -442,6 +468,17 @@ struct World {
     HIPCK(hipMemcpyAsync(angvel.p, hw.data(), n * f4b, hipMemcpyHostToDevice, stream));
     HIPCK(hipStreamSynchronize(stream));
     moved = true;
+    wake_pending = sleeping;
+    return NPHYS_OK;
+  }
+
+  int read_activation(float* energy, int64_t cap) {
+    if (cap < n) return set_err(NPHYS_ERR_CAPACITY, "energy buffer too small");
+    if (n == 0) return NPHYS_OK;
+    std::vector<float2> h(static_cast<size_t>(n));
+    HIPCK(hipMemcpyAsync(h.data(), act.p, size_t(n) * sizeof(float2), hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    for (int i = 0; i < n; ++i) energy[i] = h[size_t(i)].x;
     return NPHYS_OK;
   }
 
@@ -817,10 +854,10 @@ struct World {
     }
   }
 
-  int colour(const DevBodies& b, const DevContacts& c, int M, SolverBufs& v) {
-    const ContactBufs& pc = cb[cur ^ 1];
-    const int prevM = have_solved ? pc.M : 0;
-    launch_colour_prepare(b, c, M, pc.view(), prevM, sv_state2[cur ^ 1].as<uint32_t>(), v, stream);
+  int colour(const DevBodies& b, const DevContacts& c, int M, int C, SolverBufs& v) {
+    // colours are inherited from the manifolds of the previous solve
+    const int prevM = have_solved ? sview_M[cur ^ 1] : 0;
+    launch_colour_prepare(b, c, M, sview[cur ^ 1], prevM, sv_state2[cur ^ 1].as<uint32_t>(), v, stream);
     // Rounds in batches of four with one poll per batch: a round after the last one that colours
     // anything changes nothing, and the device records the last round that did.
     // The sort keys and colour count are computed after every batch and read back with the poll,
@@ -841,7 +878,7 @@ struct World {
     if (ncolours > kMaxColours) return set_err(NPHYS_ERR_STATE, "too many colours");
     launch_colour_order(b, c, M, ncolours, v, stream);
     uint32_t* class_rec = v.col_start + (kMaxColours + 2);
-    launch_fix_colour_starts(ncolours, M, cb[cur].C, v, class_rec, stream);
+    launch_fix_colour_starts(ncolours, M, C, v, class_rec, stream);
     // class offsets (slots and row records) to the host, read after the assembly is enqueued
     HIPCK(hipMemcpyAsync(h_colstart, v.col_start, size_t(ncolours + 1) * 4, hipMemcpyDeviceToHost, stream));
     HIPCK(hipMemcpyAsync(h_colstart + (kMaxColours + 2), class_rec, size_t(ncolours + 1) * 4,
@@ -1081,17 +1118,64 @@ struct World {
     return NPHYS_OK;
   }
 
+  // Activation islands and the solved-manifold list (sleeping worlds): sets cv, M and C to the
+  // manifolds with an awake dynamic side; the others' cached impulses are carried over.
+  int solve_list(const DevBodies& b, ContactBufs& cc, ContactBufs& pc, int prevM, DevContacts& cv, int& M, int& C) {
+    const size_t m = size_t(std::max(cc.M, 1)), nn = size_t(std::max(n, 1));
+    HIPCK(act_parent.ensure(nn * 4));
+    HIPCK(act_can.ensure(nn * 4));
+    HIPCK(act_mflag[cur].ensure(m * 4));
+    HIPCK(act_pos.ensure(m * 4));
+    HIPCK(act_tmp.ensure(scan_temp_words(int(m)) * 4 + 64));
+    HIPCK(sl_mbody[cur].ensure(m * sizeof(int2)));
+    HIPCK(sl_mfirst[cur].ensure(m * 4));
+    HIPCK(sl_mcount[cur].ensure(m * 4));
+    ActivationBufs a;
+    a.parent = act_parent.as<int>();
+    a.can = act_can.as<uint32_t>();
+    a.mflag = act_mflag[cur].as<uint32_t>();
+    a.pos = act_pos.as<uint32_t>();
+    a.tmp = act_tmp.as<uint32_t>();
+    a.totals = bp_small.as<uint32_t>() + 48;
+    fill_u32(a.totals, 0u, 3, stream);
+    const DevContacts full = cc.view();
+    launch_activation(b, full, cc.M, a, stream);
+    DevContacts out = full;
+    out.mbody = sl_mbody[cur].as<int2>();
+    out.mfirst = sl_mfirst[cur].as<int>();
+    out.mcount = sl_mcount[cur].as<int>();
+    launch_solve_list(full, cc.M, a, out, pc.view(), prevM, stream);
+    HIPCK(hipMemcpyAsync(h_small + 48, a.totals, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    M = int(h_small[48]);
+    C = int(h_small[49]);
+    active_bodies = int64_t(h_small[50]);
+    cv = out;
+    return NPHYS_OK;
+  }
+
   int solve() {
     ++solve_count;  // counts every solve, as the checker does
     ContactBufs& cc = cb[cur];
     ContactBufs& pc = cb[cur ^ 1];
-    const int M = cc.M, C = cc.C;
     const int prevM = have_solved ? pc.M : 0;
+    DevBodies b = bodies();
+    DevContacts cv = cc.view();
+    int M = cc.M, C = cc.C;
+    if (sleeping) {
+      const int rc = solve_list(b, cc, pc, prevM, cv, M, C);
+      if (rc) return rc;
+    } else {
+      active_bodies = n_dynamic;
+    }
+    sview[cur] = cv;
+    sview_M[cur] = M;
+    sview_C[cur] = C;
+    sview_compact[cur] = sleeping;
     solved_M = M;
     solved_C = C;
     res_inf = res_rms = 0.0;
     ncolours = 0;
-    DevBodies b = bodies();
     if (M == 0) {
       last_sweeps = res_target > 0.0 ? 0 : prm.max_velocity_iterations;
       counters.velocity_sweeps += last_sweeps;
@@ -1102,8 +1186,7 @@ struct World {
     int rc = ensure_solver(M, C);
     if (rc) return rc;
     SolverBufs v = sv_view();
-    DevContacts cv = cc.view();
-    if ((rc = colour(b, cv, M, v))) return rc;
+    if ((rc = colour(b, cv, M, C, v))) return rc;
     launch_assemble(b, cv, pc.view(), prevM, M, sp, v, stream);
     HIPCK(hipEventRecord(ev[EV_ASM], stream));
     // class sizes (read back while the assembly runs) give the visit order and the sweep grids
@@ -1182,11 +1265,10 @@ struct World {
   }
 
   int position_correction() {
-    ContactBufs& cc = cb[cur];
-    if (cc.M == 0) return NPHYS_OK;
+    if (sview_M[cur] == 0) return NPHYS_OK;
     DevBodies b = bodies();
     SolverBufs v = sv_view();
-    DevContacts cv = cc.view();
+    const DevContacts cv = sview[cur];
     const int pblocks = position_blocks(max_class);
     auto body = [&]() {
       for (int it = 0; it < prm.max_position_iterations; ++it)
@@ -1208,6 +1290,10 @@ struct World {
       moved = false;
     }
     HIPCK(hipEventRecord(ev[EV_PRE_CD], stream));
+    if (wake_pending) {
+      launch_wake(bodies(), stream);
+      wake_pending = false;
+    }
     launch_prestep(bodies(), sp, stream);
     if ((rc = solve())) return rc;
     launch_integrate(bodies(), sp, stream);
@@ -1246,6 +1332,7 @@ struct World {
     counters.npairs = npairs;
     counters.ncontacts = solved_C;
     counters.nmanifolds = solved_M;
+    counters.active_bodies = active_bodies;
     counters.nconstraints = int64_t(3) * solved_C;
     counters.ncolours = ncolours;
     counters.colour_rounds += colour_rounds;
@@ -1500,32 +1587,45 @@ int nphys_read_contacts(nphys_world* w, int32_t* bodies, uint32_t* feature, floa
 
 int nphys_num_solved_contacts(nphys_world* w, int64_t* n) {
   if (!w || !n) return set_err(NPHYS_ERR_INVALID, "null argument");
-  *n = w->w.have_solved ? w->w.cb[w->w.cur ^ 1].C : 0;
+  *n = w->w.have_solved ? w->w.sview_C[w->w.cur ^ 1] : 0;
   return NPHYS_OK;
 }
 
 int nphys_read_impulses(nphys_world* w, uint64_t* keys, float* impulses, int64_t cap) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
   if (!w->w.have_solved) return NPHYS_OK;
-  nph::ContactBufs& c = w->w.cb[w->w.cur ^ 1];
-  if (cap < c.C) return set_err(NPHYS_ERR_CAPACITY, "impulse buffer too small");
-  if (c.C == 0) return NPHYS_OK;
+  const int par = w->w.cur ^ 1;
+  nph::ContactBufs& c = w->w.cb[par];
+  if (cap < w->w.sview_C[par]) return set_err(NPHYS_ERR_CAPACITY, "impulse buffer too small");
+  if (w->w.sview_C[par] == 0) return NPHYS_OK;
   std::vector<int2> mb;
   std::vector<int> mc;
   std::vector<float4> rec, imp;
   int rc = w->w.gather_contacts(c, mb, mc, rec, &imp);
   if (rc) return rc;
-  long i = 0;
+  // sleeping worlds: only the manifolds handed to the solver
+  std::vector<uint32_t> solved;
+  if (w->w.sview_compact[par]) {
+    solved.resize(mb.size());
+    if (!solved.empty()) {
+      HIPCK(hipMemcpyAsync(solved.data(), w->w.act_mflag[par].p, solved.size() * 4, hipMemcpyDeviceToHost,
+                           w->w.stream));
+      HIPCK(hipStreamSynchronize(w->w.stream));
+    }
+  }
+  long i = 0, j = 0;
   for (size_t m = 0; m < mb.size(); ++m)
-    for (int k = 0; k < mc[m]; ++k, ++i) {
-      const uint32_t f = nph::bits_of(rec[size_t(i) * nph::kContactFields + 1].w);
+    for (int k = 0; k < mc[m]; ++k, ++j) {
+      if (!solved.empty() && !solved[m]) continue;
+      const uint32_t f = nph::bits_of(rec[size_t(j) * nph::kContactFields + 1].w);
       if (keys)
         keys[i] = (uint64_t(uint32_t(mb[m].x)) << 40) | (uint64_t(uint32_t(mb[m].y)) << 16) | uint64_t(f & 0xffffu);
       if (impulses) {
-        impulses[3 * i] = imp[size_t(i)].x;
-        impulses[3 * i + 1] = imp[size_t(i)].y;
-        impulses[3 * i + 2] = imp[size_t(i)].z;
+        impulses[3 * i] = imp[size_t(j)].x;
+        impulses[3 * i + 1] = imp[size_t(j)].y;
+        impulses[3 * i + 2] = imp[size_t(j)].z;
       }
+      ++i;
     }
   return NPHYS_OK;
 }
@@ -1566,6 +1666,11 @@ int nphys_set_residual_target(nphys_world* w, double target, int32_t max_sweeps,
   w->w.res_check_every = check_every;
   w->w.res_norm = norm;
   return NPHYS_OK;
+}
+
+int nphys_read_activation(nphys_world* w, float* energy, int64_t cap) {
+  if (!w || !energy) return set_err(NPHYS_ERR_INVALID, "null argument");
+  return w->w.read_activation(energy, cap);
 }
 
 int nphys_read_colours(nphys_world* w, int32_t* colours, int64_t cap, int32_t* ncolours) {

@@ -63,6 +63,15 @@ def concat(*scenes):
     return out
 
 
+def with_sleep_threshold(scene, threshold):
+    """Copy of `scene` whose bodies may sleep below `threshold` (a scalar or one value per body;
+    negative: the body never sleeps). RigidBodyDesc's default is 0.01 (src/object/body.rs:72-74)."""
+    out = dict(scene)
+    n = scene["status"].shape[0]
+    out["sleep_threshold"] = np.broadcast_to(np.asarray(threshold, np.float32), (n,)).copy()
+    return out
+
+
 def statics(half_extents, centres, friction=0.5):
     n = len(centres)
     s = _empty(n)

@@ -26,7 +26,8 @@ EXPORTED = [
     "nphys_num_pairs", "nphys_read_pairs", "nphys_num_contacts", "nphys_read_contacts",
     "nphys_num_solved_contacts", "nphys_read_impulses", "nphys_residual", "nphys_read_counters",
     "nphys_reset_counters", "nphys_counters_enable", "nphys_set_residual_target", "nphys_read_colours",
-    "nphys_set_sweep_variant", "nphys_set_incremental_pairs", "nphys_warmstart_load", "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
+    "nphys_set_sweep_variant", "nphys_set_incremental_pairs", "nphys_read_activation", "nphys_warmstart_load",
+    "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
     "nphys_mlp_create", "nphys_mlp_destroy", "nphys_mlp_num_params", "nphys_mlp_get_params",
     "nphys_mlp_set_params", "nphys_mlp_set_data", "nphys_mlp_train", "nphys_mlp_forward",
     "nphys_mlp_gradients", "nphys_last_error",
@@ -80,6 +81,7 @@ def load_library(path=LIB_PATH):
     L.nphys_set_sweep_variant.argtypes = [W, C.c_int32]
     L.nphys_set_incremental_pairs.argtypes = [W, C.c_int32]
     L.nphys_read_colours.argtypes = [W, ip, C.c_int64, ip]
+    L.nphys_read_activation.argtypes = [W, fp, C.c_int64]
     _lib = L
     return L
 
@@ -258,6 +260,13 @@ class MechanicalWorld:
 
     def set_warmstart_judge(self, on=True):
         _check(self._L.nphys_set_warmstart_judge(self._h, 1 if on else 0))
+
+    def activation(self):
+        """Per-body activation energy (0: asleep; 0.04 for bodies that never sleep)."""
+        n = self.num_bodies
+        e = np.zeros(max(n, 1), np.float32)
+        _check(self._L.nphys_read_activation(self._h, abi.fptr(e), e.shape[0]))
+        return e[:n]
 
     def record_colours(self):
         """Starts recording the manifold colours of every following step."""

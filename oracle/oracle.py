@@ -65,6 +65,7 @@ def lib():
         L.orc_read_counters.argtypes = [vp, C.POINTER(abi.Counters)]
         L.orc_reset_counters.argtypes = [vp]
         L.orc_read_colours.argtypes = [vp, ip, C.c_int64, ip]
+        L.orc_read_activation.argtypes = [vp, dp, C.c_int64]
         L.orc_read_body_dynamics.argtypes = [vp, dp, dp, dp]
         L.orc_collide_shapes.argtypes = [C.c_int32, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double,
                                          C.POINTER(C.c_uint32), dp, dp, dp, dp, ip]
@@ -194,6 +195,12 @@ class OracleWorld:
 
     def reset_counters(self):
         _check(self._L.orc_reset_counters(self._h))
+
+    def activation(self):
+        """Per-body activation energy (0: asleep)."""
+        e = np.zeros(max(1, self.num_bodies))
+        _check(self._L.orc_read_activation(self._h, abi.dptr(e), e.shape[0]))
+        return e[: self.num_bodies]
 
     def colours(self):
         c = self.counters()

@@ -86,7 +86,8 @@ int add_bodies_t(orc::World<T>& w, const nphys_body_desc* d, int32_t* first) {
     w.add_body(d->status[i], d->shape[i], v3_of<T>(d->dims + 3 * i),
                orc::Iso<T>{v3_of<T>(d->pos + 3 * i), q_of<T>(d->rot + 4 * i)}, lv, av,
                T(d->density[i]), T(d->margin[i]), T(d->linear_prediction[i]), T(d->friction[i]),
-               T(d->restitution[i]), d->scene ? int(d->scene[i]) : 0);
+               T(d->restitution[i]), d->scene ? int(d->scene[i]) : 0,
+               d->sleep_threshold ? T(d->sleep_threshold[i]) : T(-1));
   }
   return NPHYS_OK;
 }
@@ -351,6 +352,9 @@ int orc_read_counters(void* h, nphys_counters* c) {
     c->nconstraints = w.n_rows;
     c->ncolours = w.ncolours;
     c->velocity_sweeps = w.velocity_sweeps;
+    int64_t active = 0;
+    for (const auto& b : w.bodies) active += (b.status == orc::STATUS_DYNAMIC && b.energy != 0) ? 1 : 0;
+    c->active_bodies = active;
     return NPHYS_OK;
   });
 }
@@ -359,6 +363,14 @@ int orc_reset_counters(void* h) {
   return with_world(h, [&](auto& w) {
     w.counters = orc::Counters();
     w.velocity_sweeps = 0;
+    return NPHYS_OK;
+  });
+}
+
+int orc_read_activation(void* h, double* energy, int64_t cap) {
+  return with_world(h, [&](auto& w) {
+    if (int64_t(w.bodies.size()) > cap) return fail(NPHYS_ERR_CAPACITY, "energy buffer too small");
+    for (size_t i = 0; i < w.bodies.size(); ++i) energy[i] = double(w.bodies[i].energy);
     return NPHYS_OK;
   });
 }

@@ -83,6 +83,8 @@ struct Body {
   int companion;
   unsigned flags;
   AABB<T> loose;
+  T energy;     // ActivationStatus::energy (0: asleep)
+  T sleep_thr;  // ActivationStatus::threshold, < 0 for None
   T query_limit() const { return margin + lin_pred; }
 };
 
@@ -248,7 +250,10 @@ class World {
   std::vector<T> jacobians, mj_lambda, ext_vels;
   std::vector<VelRow<T>> uni, uni_g, bil, bil_g;
   std::vector<PosRow<T>> posc;
-  std::vector<Manifold> solved;        // manifolds handed to the solver
+  std::vector<Manifold> solved;        // manifolds handed to the solver (first: row index)
+  std::vector<Contact<T>> solved_contacts;  // their contacts, when a subset is solved
+  std::vector<char> manifold_active;   // per manifold: handed to the solver this step
+  bool any_sleeper = false;            // some dynamic body has a sleep threshold
   std::vector<ContactRows> crow;       // per contact of `solved`
   std::vector<int> colours;
   int ncolours = 0;
@@ -279,10 +284,15 @@ class World {
   // (geometrical_world.rs:104-140): mass properties from the density, proxy creation with
   // loose AABB = aabb(pos) loosened by the query limit, then by the broad-phase margin 0.01.
   int add_body(int status, int shape, V3<T> dims, Iso<T> pos, V3<T> lv, V3<T> av, T density,
-               T margin, T lin_pred, T friction, T restitution, int scene = 0) {
+               T margin, T lin_pred, T friction, T restitution, int scene = 0, T sleep_thr = T(-1)) {
     Body<T> b;
     std::memset(static_cast<void*>(&b), 0, sizeof(b));
     b.scene = scene;
+    // ActivationStatus::new_active (body.rs:77-82): 4 x the default threshold 0.01; the body's own
+    // threshold (or None) is set afterwards by the descriptor (rigid_body.rs:1028)
+    b.energy = T(0.01) * T(4.0);
+    b.sleep_thr = sleep_thr;
+    if (status == STATUS_DYNAMIC && sleep_thr >= T(0)) any_sleeper = true;
     b.status = status;
     b.shape = shape;
     b.sd.kind = shape;
@@ -348,6 +358,7 @@ class World {
   // ------------------------------------------------------------------ body dynamics
   void update_dynamics(Body<T>& b) {  // rigid_body.rs:558-588
     if (!(b.flags & F_ALL) || b.status != STATUS_DYNAMIC) return;
+    if (b.energy == T(0) && b.sleep_thr >= T(0)) b.energy = b.sleep_thr * T(2.0);  // wake (:563-565)
     const M3<T> R = qmat(b.pos.q);
     b.I_world = mul(mul(R, b.local_I), transpose(R));
     b.aug = b.I_world;
@@ -616,7 +627,8 @@ class World {
     t2 = a;
   }
 
-  void assemble(const std::vector<Manifold>& ms) {
+  // ms[m].first indexes cs and, as row index, the assembled rows (contiguous over ms)
+  void assemble(const std::vector<Manifold>& ms, const std::vector<Contact<T>>& cs) {
     // companion ids and buffers (moreau_jean_solver.rs:137-174, 322-326)
     int ndofs = 0;
     for (auto& b : bodies) {
@@ -651,7 +663,7 @@ class World {
       const T mu = (b1.friction + b2.friction) * T(0.5);
       const T e = (b1.restitution + b2.restitution) * T(0.5);
       for (int k = 0; k < m.count; ++k) {
-        const Contact<T>& c = contacts[m.first + k];
+        const Contact<T>& c = cs[size_t(m.first + k)];
         const uint64_t key = contact_key(m.b1, m.b2, c.feature);
         V3<T> imp{T(0), T(0), T(0)};
         auto it = cache.find(key);
@@ -846,7 +858,21 @@ class World {
   }
 
   void cache_impulses() {  // signorini_coulomb_pyramid_model.rs:226-261
+    // The reference's cache (a map keyed by the persistent contact id) keeps the entries of
+    // contacts that still exist but were not solved (sleeping); those are carried over here.
+    std::unordered_map<uint64_t, V3<T>> old;
+    if (any_sleeper) old.swap(cache);
     cache.clear();
+    if (any_sleeper)
+      for (size_t m = 0; m < manifolds.size(); ++m) {
+        if (manifold_active[m]) continue;
+        const Manifold& mf = manifolds[m];
+        for (int k = 0; k < mf.count; ++k) {
+          const uint64_t key = contact_key(mf.b1, mf.b2, contacts[size_t(mf.first + k)].feature);
+          const auto it = old.find(key);
+          if (it != old.end()) cache[key] = it->second;
+        }
+      }
     for (const auto& c : uni) cache[c.key] = V3<T>{c.impulse, T(0), T(0)};
     for (const auto& c : uni_g) cache[c.key] = V3<T>{c.impulse, T(0), T(0)};
     for (size_t i = 0; i < crow.size(); ++i) {
@@ -947,12 +973,82 @@ class World {
   long solve_count = 0;  // solves performed, never reset
 
   // ------------------------------------------------------------------ the step
+  // ActivationManager::update (activation_manager.rs:61-201) without kinematic bodies or joints:
+  // energy running average of the awake bodies, islands by union-find over the contact
+  // manifolds between dynamic bodies, then every island sleeps (energy and velocity zeroed) when
+  // all of its bodies have a threshold and an energy below it, and is woken otherwise.
+  int uf_find(std::vector<int>& p, int x) {
+    while (p[size_t(x)] != x) {
+      p[size_t(x)] = p[size_t(p[size_t(x)])];
+      x = p[size_t(x)];
+    }
+    return x;
+  }
+  void activation_update() {
+    const T mix = T(0.01);
+    for (auto& b : bodies) {
+      if (b.status != STATUS_DYNAMIC || b.energy == T(0) || b.sleep_thr < T(0)) continue;
+      // |generalized velocity|^2, components in order (linear, then angular)
+      const T nsq = ((((b.linvel.x * b.linvel.x + b.linvel.y * b.linvel.y) + b.linvel.z * b.linvel.z) +
+                      b.angvel.x * b.angvel.x) + b.angvel.y * b.angvel.y) + b.angvel.z * b.angvel.z;
+      const T e = (T(1) - mix) * b.energy + mix * nsq;
+      const T cap = b.sleep_thr * T(4.0);
+      b.energy = e < cap ? e : cap;
+    }
+    std::vector<int> p(bodies.size());
+    for (size_t i = 0; i < p.size(); ++i) p[i] = int(i);
+    for (const auto& m : manifolds) {
+      if (bodies[size_t(m.b1)].status != STATUS_DYNAMIC || bodies[size_t(m.b2)].status != STATUS_DYNAMIC) continue;
+      const int a = uf_find(p, m.b1), c = uf_find(p, m.b2);
+      if (a != c) p[size_t(std::max(a, c))] = std::min(a, c);
+    }
+    std::vector<char> can(bodies.size(), 1);
+    for (size_t i = 0; i < bodies.size(); ++i) {
+      const Body<T>& b = bodies[i];
+      if (b.status != STATUS_DYNAMIC) continue;
+      if (!(b.sleep_thr >= T(0) && b.energy < b.sleep_thr)) can[size_t(uf_find(p, int(i)))] = 0;
+    }
+    for (size_t i = 0; i < bodies.size(); ++i) {
+      Body<T>& b = bodies[i];
+      if (b.status != STATUS_DYNAMIC) continue;
+      if (can[size_t(uf_find(p, int(i)))]) {
+        if (b.energy != T(0)) {  // RigidBody::deactivate (rigid_body.rs:396-400)
+          b.energy = T(0);
+          b.linvel = V3<T>{T(0), T(0), T(0)};
+          b.angvel = V3<T>{T(0), T(0), T(0)};
+        }
+      } else if (b.energy == T(0) && b.sleep_thr >= T(0)) {
+        b.energy = b.sleep_thr * T(2.0);  // Body::activate (body.rs:338-342)
+      }
+    }
+  }
+
   void solver_step() {  // MoreauJeanSolver::step (moreau_jean_solver.rs:47-90)
     double t0 = now_s();
     // manifold collection (mechanical_world.rs:287-300): every kept manifold has >= 1 contact
-    // and a dynamic side; narrow_phase() already stores them with contiguous contacts 0..C-1.
-    solved = manifolds;
-    assemble(solved);
+    // and a dynamic side that is awake; narrow_phase() stores them with contiguous contacts.
+    manifold_active.assign(manifolds.size(), 1);
+    if (any_sleeper) {
+      activation_update();
+      solved.clear();
+      solved_contacts.clear();
+      for (size_t m = 0; m < manifolds.size(); ++m) {
+        const Body<T>& b1 = bodies[size_t(manifolds[m].b1)];
+        const Body<T>& b2 = bodies[size_t(manifolds[m].b2)];
+        const bool act = (b1.status == STATUS_DYNAMIC && b1.energy != T(0)) ||
+                         (b2.status == STATUS_DYNAMIC && b2.energy != T(0));
+        manifold_active[m] = act ? 1 : 0;
+        if (!act) continue;
+        Manifold s = manifolds[m];
+        s.first = int(solved_contacts.size());
+        for (int k = 0; k < s.count; ++k) solved_contacts.push_back(contacts[size_t(manifolds[m].first + k)]);
+        solved.push_back(s);
+      }
+      assemble(solved, solved_contacts);
+    } else {
+      solved = manifolds;
+      assemble(solved, contacts);
+    }
     counters.assembly += now_s() - t0;
     t0 = now_s();
     ++solve_count;
@@ -961,8 +1057,8 @@ class World {
     cache_impulses();
     counters.velocity_resolution += now_s() - t0;
     t0 = now_s();
-    for (auto& b : bodies) {  // update_velocities_and_integrate (:328-347)
-      if (b.status != STATUS_DYNAMIC) continue;
+    for (auto& b : bodies) {  // update_velocities_and_integrate (:328-347), over the awake bodies
+      if (b.status != STATUS_DYNAMIC || b.energy == T(0)) continue;
       const int id = b.companion;
       b.linvel = V3<T>{b.linvel.x + ext_vels[id + 0], b.linvel.y + ext_vels[id + 1], b.linvel.z + ext_vels[id + 2]};
       b.angvel = V3<T>{b.angvel.x + ext_vels[id + 3], b.angvel.y + ext_vels[id + 4], b.angvel.z + ext_vels[id + 5]};

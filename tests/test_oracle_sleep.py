@@ -1,0 +1,205 @@
+"""Pins the oracle's sleeping / activation islands (ActivationManager::update,
+src/detection/activation_manager.rs:61-201) before it checks the HIP path.
+
+Expected values come from the closed forms the reference's rules fix:
+  * a free body of constant velocity follows e <- (1 - 0.01) e + 0.01 |v|^2 from the
+    ActivationStatus::new_active energy 4 * 0.01 (src/object/body.rs:77-82), capped at 4 x its
+    threshold, and sleeps on the first step where e < threshold; sleeping zeroes energy and
+    velocity (rigid_body.rs:396-400) and the body stops moving;
+  * islands are the connected components of the contact graph over dynamic bodies: a body that
+    cannot sleep keeps everything it touches awake, static bodies join no island;
+  * a sleeping body woken by contact gets energy 2 x threshold (body.rs:338-342) on that step;
+  * manifolds whose bodies all sleep are not solved (mechanical_world.rs:287-300).
+"""
+import numpy as np
+import pytest
+
+from nphysics_amd import abi, scenes
+
+from test_oracle_kat import single
+
+
+def _world(oracle_mod, scene, precision=32, **params):
+    o = oracle_mod.OracleWorld(abi.default_params(**params), precision=precision)
+    o.add_bodies(scene)
+    return o
+
+
+def _energy_recurrence(ftype, thr, nsq, steps):
+    """Energies after each step of an unobstructed body (velocity constant, |v|^2 = nsq) and the
+    step on which it falls asleep."""
+    mix = ftype(0.01)
+    e = ftype(0.01) * ftype(4.0)
+    out = []
+    cap = ftype(thr) * ftype(4.0)
+    for k in range(steps):
+        e = (ftype(1) - mix) * e + mix * ftype(nsq)
+        e = e if e < cap else cap
+        if e < ftype(thr):
+            return out + [ftype(0)], k + 1
+        out.append(e)
+    return out, None
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_free_body_energy_recurrence_and_sleep_step(oracle_mod, precision):
+    ft = np.float32 if precision == 32 else np.float64
+    v, w = (0.03, 0.0, 0.0), (0.0, 0.04, 0.0)  # |v|^2 + |w|^2 = 0.0025 < threshold 0.01
+    s = scenes.with_sleep_threshold(single(0, 0, (0.5, 0, 0), (0, 5, 0), linvel=v, angvel=w), 0.01)
+    o = _world(oracle_mod, s, precision, gravity=(0.0, 0.0, 0.0))
+    thr = ft(np.float32(0.01))
+    f = [ft(np.float32(x)) for x in (*v, *w)]  # the descriptor's float32 values
+    nsq = ((((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2]) + f[3] * f[3]) + f[4] * f[4]) + f[5] * f[5]
+    expected, sleep_step = _energy_recurrence(ft, thr, nsq, 400)
+    assert sleep_step == 161  # 0.0025 + 0.0375 * 0.99^k < 0.01 first at k = 161
+    x_at_sleep = None
+    for k in range(sleep_step + 20):
+        o.step(1)
+        e = o.activation()[0]
+        st = o.get_state()
+        if k + 1 < sleep_step:
+            assert e == expected[k], f"step {k + 1}"
+            assert st["linvel"][0, 0] == pytest.approx(0.03, rel=1e-6)
+        else:
+            assert e == 0.0, f"step {k + 1}"
+            np.testing.assert_array_equal(st["linvel"][0], 0.0)
+            np.testing.assert_array_equal(st["angvel"][0], 0.0)
+            if x_at_sleep is None:
+                x_at_sleep = st["pos"][0].copy()
+            np.testing.assert_array_equal(st["pos"][0], x_at_sleep)
+    # the body moved for sleep_step - 1 steps at 0.03 m/s
+    assert x_at_sleep[0] == pytest.approx(0.03 * (sleep_step - 1) / 60.0, rel=1e-4)
+    assert o.counters()["active_bodies"] == 0
+
+
+def test_fast_body_is_capped_and_never_sleeps(oracle_mod):
+    """Energy is capped at 4 x threshold, so a fast body's energy sits exactly at the cap."""
+    s = scenes.with_sleep_threshold(single(0, 0, (0.5, 0, 0), (0, 5, 0), linvel=(3.0, 0, 0)), 0.01)
+    o = _world(oracle_mod, s, 32, gravity=(0.0, 0.0, 0.0))
+    o.step(50)
+    assert o.activation()[0] == np.float32(0.01) * np.float32(4.0)
+    assert o.counters()["active_bodies"] == 1
+
+
+def test_negative_threshold_never_sleeps_and_energy_is_frozen(oracle_mod):
+    s = scenes.with_sleep_threshold(single(0, 0, (0.5, 0, 0), (0, 5, 0)), -1.0)
+    o = _world(oracle_mod, s, 32, gravity=(0.0, 0.0, 0.0))
+    o.step(400)
+    assert o.activation()[0] == np.float32(0.01) * np.float32(4.0)  # new_active energy, never updated
+    assert o.counters()["active_bodies"] == 1
+
+
+def _two_balls_on_ground(thr_a, thr_b, gap=0.0):
+    r = 0.5
+    a = single(0, 0, (r, 0, 0), (-r - gap / 2, r + 0.005, 0))
+    b = single(0, 0, (r, 0, 0), (r + gap / 2, r + 0.005, 0))
+    s = scenes.concat(scenes.ground(), a, b)
+    return scenes.with_sleep_threshold(s, [0.01, thr_a, thr_b])
+
+
+def test_ball_on_static_ground_sleeps(oracle_mod):
+    """Static bodies join no island: a resting ball sleeps by itself, and then its ground
+    manifold is no longer solved."""
+    r = 0.5
+    s = scenes.with_sleep_threshold(scenes.concat(scenes.ground(), single(0, 0, (r, 0, 0), (0, r + 0.005, 0))),
+                                    0.01)
+    o = _world(oracle_mod, s)
+    step = None
+    for k in range(400):
+        o.step(1)
+        if o.activation()[1] == 0.0:
+            step = k + 1
+            break
+    assert step is not None and 138 <= step <= 180
+    y = o.get_state()["pos"][1, 1]
+    o.step(50)
+    c = o.counters()
+    assert c["nmanifolds"] == 0 and c["ncontacts"] == 0 and c["active_bodies"] == 0
+    assert o.get_state()["pos"][1, 1] == y
+    assert o.impulses()[0].shape[0] == 0
+
+
+def test_island_sleeps_together_and_a_nonsleeper_keeps_it_awake(oracle_mod):
+    # touching balls, both may sleep: they sleep on the same step
+    o = _world(oracle_mod, _two_balls_on_ground(0.01, 0.01))
+    first = {}
+    for k in range(400):
+        o.step(1)
+        e = o.activation()
+        for i in (1, 2):
+            if e[i] == 0.0 and i not in first:
+                first[i] = k + 1
+    assert o.contacts()["bodies"].shape[0] > 0
+    assert 1 in first and first[1] == first[2]
+    # the same pair with one body that never sleeps: neither sleeps
+    o = _world(oracle_mod, _two_balls_on_ground(-1.0, 0.01))
+    o.step(400)
+    e = o.activation()
+    assert e[1] != 0.0 and e[2] != 0.0
+    # apart, the sleeper sleeps alone
+    o = _world(oracle_mod, _two_balls_on_ground(-1.0, 0.01, gap=0.5))
+    o.step(400)
+    e = o.activation()
+    assert e[1] != 0.0 and e[2] == 0.0
+
+
+def test_contact_wakes_a_sleeping_body_with_twice_its_threshold(oracle_mod):
+    r = 0.5
+    rest = single(0, 0, (r, 0, 0), (0, r + 0.005, 0))
+    s = scenes.with_sleep_threshold(scenes.concat(scenes.ground(), rest), 0.01)
+    o = _world(oracle_mod, s)
+    o.step(250)
+    assert o.activation()[1] == 0.0
+    # drop a second ball onto it: add_bodies keeps the world's state
+    o.add_bodies(scenes.with_sleep_threshold(single(0, 0, (r, 0, 0), (0.1, 3 * r + 0.6, 0)), 0.01))
+    woke = None
+    for k in range(120):
+        o.step(1)
+        e = o.activation()
+        if e[1] != 0.0:
+            woke = k + 1
+            assert e[1] == np.float32(0.01) * np.float32(2.0)
+            assert e[2] != 0.0
+            break
+    assert woke is not None and woke > 5
+
+
+def test_set_state_wakes_a_sleeper(oracle_mod):
+    r = 0.5
+    s = scenes.with_sleep_threshold(scenes.concat(scenes.ground(), single(0, 0, (r, 0, 0), (0, r + 0.005, 0))),
+                                    0.01)
+    o = _world(oracle_mod, s)
+    o.step(250)
+    assert o.activation()[1] == 0.0
+    st = o.get_state()
+    o.set_state(pos=st["pos"])
+    assert o.activation()[1] == 0.0  # woken at the next step's update_dynamics
+    o.step(1)
+    # woken with 2 x threshold, then one energy update at zero velocity
+    ft = np.float32
+    assert o.activation()[1] == (ft(1) - ft(0.01)) * (ft(0.01) * ft(2.0)) + ft(0.01) * ft(0.0)
+
+
+def test_partial_sleep_coloured_order_tracks_reference_order(oracle_mod):
+    """Once part of the world sleeps the solved manifolds are a subset with gaps in contact order;
+    the coloured order (the GPU's) and the reference's row order must still solve the same
+    system: never-sleeping resting bodies next to a ball that falls asleep end where they do in
+    the reference order."""
+    r = 0.5
+    sleeper = single(0, 0, (r, 0, 0), (-4.0, r + 0.005, 0))
+    column = scenes.concat(single(0, 1, (0.5, 0.5, 0.5), (3.0, 0.505, 0.0)), single(0, 0, (r, 0, 0), (3.0, 1.0 + r + 0.01, 0)),
+                           single(0, 1, (0.5, 0.5, 0.5), (5.0, 0.505, 0.0)))
+    s = scenes.with_sleep_threshold(scenes.concat(scenes.ground(), sleeper, column), [0.01, 0.01, -1, -1, -1])
+    worlds = []
+    for order in (oracle_mod.ORDER_COLOURED, oracle_mod.ORDER_REFERENCE):
+        o = oracle_mod.OracleWorld(abi.default_params(), precision=64, order=order)
+        o.add_bodies(s)
+        o.step(300)
+        worlds.append(o)
+    for o in worlds:
+        e = o.activation()
+        assert e[1] == 0.0 and (e[2:] != 0.0).all()
+        assert o.counters()["nmanifolds"] == 3  # box-ground, ball-box, box-ground
+    a, b = worlds[0].get_state(), worlds[1].get_state()
+    np.testing.assert_allclose(a["pos"], b["pos"], atol=1e-3)
+    assert np.abs(a["linvel"]).max() < 0.05
