@@ -54,7 +54,7 @@ def settle(w, steps):
         w.step(k)
         done += k
         c = w.counters()
-        log(f"settle {done}/{steps}: contacts {c['ncontacts']} colours {c['ncolours']}")
+        log(f"settle {done}/{steps}: contacts {c['ncontacts']} colours {c['ncolours']} awake {c['active_bodies']}")
 
 
 def cpu_baseline(scene, snapshot, params, steps):
@@ -301,6 +301,61 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_sleeping(args):
+    """The batched worlds with the reference's default sleep threshold (0.01) on every body,
+    settled with sleeping on: islands at rest leave the solve and the integration (a single dense
+    pile is one island and stays awake while any body in it moves). Prints one JSON line."""
+    pile = args.sleeping_scene == "pile"
+    params = abi.default_params(max_velocity_iterations=16 if pile else 8, max_position_iterations=3)
+    scene = scenes.with_sleep_threshold(scenes.pile() if pile else scenes.batched(), 0.01)
+    w = world.MechanicalWorld(integration_parameters=params)
+    w.add_bodies(scene)
+    w.set_sweep_variant(args.sweep_variant)
+    t0 = time.perf_counter()
+    if args.load_state:
+        # a settled state saved by an earlier run: setting it wakes every body (2 x threshold),
+        # so the settle steps that follow let the energies decay again
+        st = np.load(args.load_state)
+        w.set_state(**{k: st[k] for k in ("pos", "rot", "linvel", "angvel")})
+        log(f"loaded settled state from {args.load_state}")
+    settle(w, args.settle)
+    log(f"settled in {time.perf_counter() - t0:.1f} s")
+    w.step(args.warmup)
+    w.reset_counters()
+    t0 = time.perf_counter()
+    w.step(args.steps)
+    elapsed = time.perf_counter() - t0
+    c = w.counters()
+    n_dyn = int((scene["status"] == 0).sum())
+    out = {
+        "metric": ("steps/s of the dense pile" if pile else "steps/s of 4,096 independent worlds")
+                  + " with sleeping (threshold 0.01 on every body)",
+        "value": args.steps / elapsed,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": ("dense pile as the default workload (16 velocity + 3 position iterations)" if pile
+                                else "batched worlds as --workload batched (8 velocity + 3 position iterations)")
+                               + ", every body with sleep threshold 0.01, settled with sleeping on",
+                   "bodies": int(scene["status"].shape[0]), "dynamic_bodies": n_dyn,
+                   "settle_steps": args.settle, "awake_bodies": c["active_bodies"],
+                   "pairs": c["npairs"], "solved_contacts": c["ncontacts"], "solved_manifolds": c["nmanifolds"],
+                   "colours": c["ncolours"]},
+        "stages_ms_per_step": {k: 1e3 * c[k] / args.steps for k in
+                               ("collision_detection", "broad_phase", "narrow_phase", "assembly",
+                                "velocity_resolution", "velocity_update", "position_resolution")},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,8 +376,11 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counter calibration) and exit")
     ap.add_argument("--sweep-variant", type=int, default=-1,
                     help="-1 autotune (default), 0 per-colour launches, 64/128/256 scene-blocked")
-    ap.add_argument("--workload", choices=("pile", "batched"), default="pile",
-                    help="pile: the dense pile (default); batched: 4,096 independent worlds in one handle")
+    ap.add_argument("--workload", choices=("pile", "batched", "sleeping"), default="pile",
+                    help="pile: the dense pile (default); batched: 4,096 independent worlds in one handle; "
+                         "sleeping: the batched worlds with sleep threshold 0.01")
+    ap.add_argument("--sleeping-scene", choices=("batched", "pile"), default="batched",
+                    help="scene of --workload sleeping")
     args = ap.parse_args()
 
     ws = int(os.environ.get("WORLD_SIZE", "1"))
@@ -344,6 +402,8 @@ def main():
         return
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "sleeping":
+        return bench_sleeping(args)
     params = abi.default_params(max_velocity_iterations=16, max_position_iterations=3)
     scene = scenes.pile()
     w = world.MechanicalWorld(integration_parameters=params)

@@ -202,6 +202,15 @@ int nphys_num_contacts(nphys_world* w, int64_t* n);
 int nphys_read_contacts(nphys_world* w, int32_t* bodies, uint32_t* feature, float* world1,
                         float* world2, float* normal, float* depth, int64_t cap);
 
+/* The same contacts grouped as contact manifolds, one per body pair in contact (the manifolds
+ * GeometricalWorld::contact_pairs yields, geometrical_world.rs:534-553, with
+ * ColliderContactManifold's pair, collider_contact_manifold.rs:9-115): per manifold its two bodies
+ * in the broad phase's pair order, the index of its first contact in nphys_read_contacts' order
+ * and its contact count (1..4); manifolds in increasing (body 1, body 2) order. Any pointer may be
+ * NULL. */
+int nphys_num_manifolds(nphys_world* w, int64_t* n);
+int nphys_read_manifolds(nphys_world* w, int32_t* bodies, int32_t* first, int32_t* count, int64_t cap);
+
 /* Impulses (normal, tangent 1, tangent 2) of the contacts solved by the last step, in the order
  * and with the keys of that step's contacts (the ImpulseCache of
  * signorini_coulomb_pyramid_model.rs:226-261). keys may be NULL. */

@@ -11,7 +11,8 @@
 //                  4 x threshold (activation_manager.rs:47-57) of every awake body that has a
 //                  threshold; union-find parents reset;
 //   k_act_union    islands: union over the manifolds whose bodies are both dynamic (:143-147),
-//                  lock-free (a root is linked under the smaller root by compare-and-swap);
+//                  lock-free (a root is linked under the smaller root by compare-and-swap),
+//                  skipping manifolds whose two bodies keep their islands awake anyway;
 //   k_act_flags    path compression, and an island cannot sleep as soon as one of its bodies has
 //                  no threshold or an energy >= its threshold (:160-172);
 //   k_act_apply    islands that can sleep are put to sleep (energy and velocity zeroed,
@@ -24,7 +25,8 @@
 //                  only overwrites the entries of solved contacts).
 // Activation state: one float2 per body (energy, threshold; threshold < 0 means the body never
 // sleeps, energy 0 means asleep). Every kernel is one pass over bodies or manifolds; union-find
-// walks are bounded by the body count (parents only point to smaller indices).
+// walks are bounded by the body count (parents only point to smaller indices) and halve the
+// paths they walk, which keeps the trees shallow when one island spans the whole world.
 #include <hip/hip_runtime.h>
 
 #include "devworld.hpp"
@@ -41,13 +43,30 @@ __device__ inline int uf_load(const int* p, int x) {
   return __hip_atomic_load(p + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ inline int uf_root(const int* p, int x, int n) {
+// Root of x with path halving: every visited node is pointed at its grandparent, which is also
+// one of its ancestors, so concurrent walks and links stay valid.
+__device__ inline int uf_root(int* p, int x, int n) {
   for (int it = 0; it < n; ++it) {
     const int y = uf_load(p, x);
     if (y == x) break;
-    x = y;
+    const int z = uf_load(p, y);
+    if (z != y) __hip_atomic_store(p + x, z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    x = z;
   }
   return x;
+}
+
+// Sum of v over the 256-thread block, added to *out by one atomic (one address shared by every
+// block: per-wave atomics serialise there).
+__device__ inline void block_add(uint32_t v, uint32_t* out) {
+  __shared__ uint32_t part[4];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t s = (part[0] + part[1]) + (part[2] + part[3]);
+    if (s) atomicAdd(out, s);
+  }
 }
 
 __global__ void __launch_bounds__(256) k_act_wake(DevBodies b) {
@@ -74,12 +93,21 @@ __global__ void __launch_bounds__(256) k_act_energy(DevBodies b, int* parent, ui
   b.act[i] = a;
 }
 
-__global__ void __launch_bounds__(256) k_act_union(DevContacts c, int M, const int* __restrict__ status, int* parent,
-                                                  int n) {
+// A body that keeps its island awake: no threshold, or an energy at or above it.
+__device__ inline bool keeps_awake(float2 a) { return !(a.y >= 0.0f && a.x < a.y); }
+
+// An island can sleep iff none of its bodies keeps it awake, and a body that could sleep is
+// connected to one that keeps it awake iff it is through edges with a body that could sleep at
+// one end at least (follow any path to its first such body). So the union skips the manifolds
+// between two bodies that keep their islands awake: the components containing such a body
+// change, the answer for every body does not.
+__global__ void __launch_bounds__(256) k_act_union(DevBodies b, DevContacts c, int M, int* parent) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   const int2 bb = c.mbody[m];
-  if (status[bb.x] != STATUS_DYNAMIC || status[bb.y] != STATUS_DYNAMIC) return;
+  if (b.status[bb.x] != STATUS_DYNAMIC || b.status[bb.y] != STATUS_DYNAMIC) return;
+  if (keeps_awake(b.act[bb.x]) && keeps_awake(b.act[bb.y])) return;
+  const int n = b.n;
   int a = uf_root(parent, bb.x, n), d = uf_root(parent, bb.y, n);
   // every failed exchange means another thread linked root a, which happens at most n - 1 times
   for (int it = 0; it < 2 * n && a != d; ++it) {
@@ -101,8 +129,7 @@ __global__ void __launch_bounds__(256) k_act_flags(DevBodies b, int* parent, uin
   if (b.status[i] != STATUS_DYNAMIC) return;
   const int r = uf_root(parent, i, b.n);
   __hip_atomic_store(parent + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const float2 a = b.act[i];
-  if (!(a.y >= 0.0f && a.x < a.y)) can[r] = 0u;
+  if (keeps_awake(b.act[i])) can[r] = 0u;
 }
 
 __global__ void __launch_bounds__(256) k_act_apply(DevBodies b, const int* __restrict__ parent,
@@ -123,8 +150,7 @@ __global__ void __launch_bounds__(256) k_act_apply(DevBodies b, const int* __res
     b.act[i] = a;
     on = a.x != 0.0f;
   }
-  const unsigned long long mask = __ballot(on);
-  if ((threadIdx.x & 63) == 0 && mask) atomicAdd(awake, uint32_t(__popcll(mask)));
+  block_add(on ? 1u : 0u, awake);
 }
 
 __global__ void __launch_bounds__(256) k_act_manifolds(DevBodies b, DevContacts c, int M, uint32_t* flag) {
@@ -182,10 +208,7 @@ __global__ void __launch_bounds__(256) k_act_compact(DevContacts c, int M, const
     }
     if (m == M - 1) totals[0] = pos[m] + flag[m];
   }
-  // one atomic per wave for the contact count
-  uint32_t s = cnt;
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if ((threadIdx.x & 63) == 0 && s) atomicAdd(totals + 1, s);
+  block_add(cnt, totals + 1);
 }
 
 int nb(long n) { return int((n + 255) / 256); }
@@ -199,7 +222,7 @@ void launch_wake(const DevBodies& b, hipStream_t s) {
 void launch_activation(const DevBodies& b, const DevContacts& c, int M, const ActivationBufs& a, hipStream_t s) {
   if (b.n > 0) {
     hipLaunchKernelGGL(k_act_energy, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can);
-    if (M > 0) hipLaunchKernelGGL(k_act_union, dim3(nb(M)), dim3(256), 0, s, c, M, b.status, a.parent, b.n);
+    if (M > 0) hipLaunchKernelGGL(k_act_union, dim3(nb(M)), dim3(256), 0, s, b, c, M, a.parent);
     hipLaunchKernelGGL(k_act_flags, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can);
     hipLaunchKernelGGL(k_act_apply, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can, a.totals + 2);
   }

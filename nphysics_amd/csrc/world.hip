@@ -17,7 +17,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <initializer_list>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -60,6 +63,24 @@ struct ContactBufs {
 };
 
 enum StageEv { EV_START, EV_PRE_CD, EV_ASM, EV_VEL, EV_INT, EV_POS, EV_BP, EV_NP, EV_COUNT };
+
+// Small device values to pinned host memory without a stream synchronisation: one workgroup copies
+// up to four word ranges into the host buffer, fences at system scope and then writes the
+// sequence number the host waits for. Launched last in stream order, so the flag also means that
+// everything enqueued before it has completed.
+struct RbJob {
+  const uint32_t* src[4];
+  int n[4];
+  int dst[4];
+};
+
+__global__ void __launch_bounds__(256) k_readback(RbJob j, uint32_t* host, int flag, uint32_t seq) {
+  for (int s = 0; s < 4; ++s)
+    for (int i = threadIdx.x; i < j.n[s]; i += blockDim.x) host[j.dst[s] + i] = j.src[s][i];
+  __threadfence_system();
+  __syncthreads();
+  if (threadIdx.x == 0) __hip_atomic_store(host + flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 }  // namespace
 
@@ -149,8 +170,6 @@ struct World {
   int res_max_sweeps = 500, res_check_every = 1, res_norm = NPHYS_RESIDUAL_MAX;
   int last_sweeps = 0;
   int max_class = 0;             // largest colour class of the current solve
-  uint32_t* h_colstart = nullptr;  // pinned: class offsets of the current solve
-  hipEvent_t ev_class = nullptr;  // after its readback
   std::vector<double> red_host;
   // learned warm start: network (FP32 parameters + BF16 images), per-record features, impulse
   // units, predictions and recorded targets; the judge's saved plain solve
@@ -175,8 +194,17 @@ struct World {
   std::vector<int> last_colours;
   bool keep_colours = false;
 
-  // pinned host scratch for small readbacks
-  uint32_t* h_small = nullptr;
+  // Readback buffer (pinned, device-mapped, coherent): words [0, 64) small per-step values,
+  // [kRbClass, +2 (kMaxColours + 2)) the class offsets, kRbFlag the sequence flag.
+  static constexpr int kRbClass = 64, kRbFlag = kRbClass + 2 * (kMaxColours + 2) + 64, kRbWords = kRbFlag + 64;
+  uint32_t* rb_host = nullptr;
+  uint32_t* rb_dev = nullptr;
+  uint32_t rb_seq = 0, class_seq = 0;
+  struct RbSeg {
+    const uint32_t* src;
+    int n;
+    int dst;
+  };
   hipEvent_t ev[EV_COUNT] = {};
   nphys_counters counters{};
   bool counters_enabled = false;
@@ -187,15 +215,13 @@ struct World {
     for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (h_visit) (void)hipHostFree(h_visit);
     if (red_pinned) (void)hipHostFree(red_pinned);
-    if (h_small) (void)hipHostFree(h_small);
-    if (h_colstart) (void)hipHostFree(h_colstart);
+    if (rb_host) (void)hipHostFree(rb_host);
     for (auto& e : ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : kt_ev)
       if (e) (void)hipEventDestroy(e);
     for (auto& e : tune_ev)
       if (e) (void)hipEventDestroy(e);
-    if (ev_class) (void)hipEventDestroy(ev_class);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -219,13 +245,14 @@ struct World {
       return set_err(NPHYS_ERR_NO_DEVICE, "no HIP device");
     HIPCK(hipGetDevice(&device));
     HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_small), 64 * sizeof(uint32_t)));
+    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&rb_host), kRbWords * sizeof(uint32_t),
+                        hipHostMallocMapped | hipHostMallocCoherent));
+    std::memset(rb_host, 0, kRbWords * sizeof(uint32_t));
+    HIPCK(hipHostGetDevicePointer(reinterpret_cast<void**>(&rb_dev), rb_host, 0));
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_visit), 3 * (kMaxColours + 2) * sizeof(int)));
-    HIPCK(hipHostMalloc(reinterpret_cast<void**>(&h_colstart), 2 * (kMaxColours + 2) * sizeof(uint32_t)));
     HIPCK(sv_visit.ensure(3 * (kMaxColours + 2) * sizeof(int)));
     for (auto& e : ev) HIPCK(hipEventCreate(&e));
     for (auto& e : tune_ev) HIPCK(hipEventCreate(&e));
-    HIPCK(hipEventCreateWithFlags(&ev_class, hipEventDisableTiming));
     set_params(p);
     HIPCK(bp_small.ensure(64 * sizeof(uint32_t)));
     // Profilers that cannot follow graph launches can turn them off; kernels are the same.
@@ -560,6 +587,39 @@ struct World {
     return v;
   }
 
+  // Enqueues a readback of up to four device word ranges into rb_host; returns its sequence number.
+  uint32_t rb_post(std::initializer_list<RbSeg> segs) {
+    RbJob j{};
+    int k = 0;
+    for (const RbSeg& s : segs) {
+      j.src[k] = s.src;
+      j.n[k] = s.n;
+      j.dst[k] = s.dst;
+      ++k;
+    }
+    ++rb_seq;
+    hipLaunchKernelGGL(k_readback, dim3(1), dim3(256), 0, stream, j, rb_dev, kRbFlag, rb_seq);
+    return rb_seq;
+  }
+
+  // Waits until readback `seq` (and everything before it on the stream) has completed. After two
+  // seconds the stream is synchronised instead, which reports a failed launch or fault.
+  int rb_wait(uint32_t seq) {
+    const volatile uint32_t* f = rb_host + kRbFlag;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t it = 1; int32_t(*f - seq) < 0; ++it) {
+      if ((it & 0xfffu) == 0 && std::chrono::steady_clock::now() - t0 > std::chrono::seconds(2)) {
+        HIPCK(hipStreamSynchronize(stream));
+        if (int32_t(*f - seq) >= 0) break;
+        return set_err(NPHYS_ERR_HIP, "readback did not complete");
+      }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return NPHYS_OK;
+  }
+
+  int readback(std::initializer_list<RbSeg> segs) { return rb_wait(rb_post(segs)); }
+
   // Loose-AABB pass; returns the number of replaced loose AABBs through *changed.
   int loose_pass(int mode, uint32_t* changed) {
     int rc = ensure_bp();  // the dirty flags
@@ -567,11 +627,10 @@ struct World {
     BroadPhaseBufs v = bp_view();
     fill_u32(v.changed, 0u, 2, stream);
     launch_loose_update(bodies(), mode, v.changed, v.dirty, stream);
-    HIPCK(hipMemcpyAsync(h_small, v.changed, 8, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    *changed = h_small[0];
-    dirty_pending += h_small[0];
-    dirty_static_pending += h_small[1];
+    if ((rc = readback({{v.changed, 2, 0}}))) return rc;
+    *changed = rb_host[0];
+    dirty_pending += rb_host[0];
+    dirty_static_pending += rb_host[1];
     return NPHYS_OK;
   }
 
@@ -583,12 +642,11 @@ struct World {
     fill_u32(v.changed, 0u, 4, stream);
     launch_loose_update(bodies(), 2, v.changed, v.dirty, stream);
     launch_loose_update(bodies(), 1, v.changed + 2, v.dirty, stream);
-    HIPCK(hipMemcpyAsync(h_small, v.changed, 16, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    *ch_swept = h_small[0];
-    *ch_final = h_small[2];
-    dirty_pending += int64_t(h_small[0]) + h_small[2];
-    dirty_static_pending += int64_t(h_small[1]) + h_small[3];
+    if ((rc = readback({{v.changed, 4, 0}}))) return rc;
+    *ch_swept = rb_host[0];
+    *ch_final = rb_host[2];
+    dirty_pending += int64_t(rb_host[0]) + rb_host[2];
+    dirty_static_pending += int64_t(rb_host[1]) + rb_host[3];
     return NPHYS_OK;
   }
 
@@ -609,15 +667,12 @@ struct World {
       // discarded if the update overflowed
       launch_incremental_pairs(b, v, cap, int(dirty_pending), stream);
       launch_pair_offsets(b, v, cap, stream);
-      HIPCK(hipMemcpyAsync(h_small, v.stats, 8, hipMemcpyDeviceToHost, stream));
-      HIPCK(hipMemcpyAsync(h_small + 2, v.cand_off + (n - 1), 4, hipMemcpyDeviceToHost, stream));
-      HIPCK(hipMemcpyAsync(h_small + 3, v.keys_alt + (n - 1), 4, hipMemcpyDeviceToHost, stream));
-      HIPCK(hipStreamSynchronize(stream));
-      done = h_small[1] == 0 && int(h_small[0]) <= cap;  // else the full build below
+      if ((rc = readback({{v.stats, 2, 0}, {v.cand_off + (n - 1), 1, 2}, {v.keys_alt + (n - 1), 1, 3}}))) return rc;
+      done = rb_host[1] == 0 && int(rb_host[0]) <= cap;  // else the full build below
       if (done) {
         ++inc_since_full;
         ++counters.incremental_pair_updates;
-        npairs = int(h_small[2] + h_small[3]);
+        npairs = int(rb_host[2] + rb_host[3]);
       }
     }
     if (!done) {
@@ -625,10 +680,8 @@ struct World {
       inc_since_full = 0;
       launch_pair_offsets(b, v, cap, stream);
       // total = off[n-1] + min(count[n-1], cap)
-      HIPCK(hipMemcpyAsync(h_small, v.cand_off + (n - 1), 4, hipMemcpyDeviceToHost, stream));
-      HIPCK(hipMemcpyAsync(h_small + 1, v.keys_alt + (n - 1), 4, hipMemcpyDeviceToHost, stream));
-      HIPCK(hipStreamSynchronize(stream));
-      npairs = int(h_small[0] + h_small[1]);
+      if ((rc = readback({{v.cand_off + (n - 1), 1, 0}, {v.keys_alt + (n - 1), 1, 1}}))) return rc;
+      npairs = int(rb_host[0] + rb_host[1]);
     }
     HIPCK(pairs.ensure(size_t(std::max(npairs, 1)) * sizeof(int2)));
     v = bp_view();
@@ -647,11 +700,10 @@ struct World {
     launch_build_lbvh(b, v, stream);
     for (;;) {
       launch_traverse(b, v, cap, stream);
-      HIPCK(hipMemcpyAsync(h_small, v.stats, 8, hipMemcpyDeviceToHost, stream));
-      HIPCK(hipStreamSynchronize(stream));
-      if (h_small[1]) return set_err(NPHYS_ERR_STATE, "BVH traversal stack overflow");
-      if (int(h_small[0]) <= cap) break;
-      cap = int(h_small[0]) + 8;
+      if ((rc = readback({{v.stats, 2, 0}}))) return rc;
+      if (rb_host[1]) return set_err(NPHYS_ERR_STATE, "BVH traversal stack overflow");
+      if (int(rb_host[0]) <= cap) break;
+      cap = int(rb_host[0]) + 8;
       if ((rc = ensure_bp())) return rc;
       v = bp_view();
     }
@@ -692,7 +744,9 @@ struct World {
   }
 
   // ------------------------------------------------------------------ narrow phase
-  int narrow_phase() {
+  // `done`, when given, is recorded after the narrow phase and before its readback (the step's last
+  // stage event, complete once the readback is).
+  int narrow_phase(hipEvent_t done = nullptr) {
     ContactBufs& out = cb[cur];
     const size_t P = size_t(std::max(npairs, 1));
     // a manifold's first slot is packed in 28 bits of the solver's slot record
@@ -709,10 +763,11 @@ struct World {
     v.totals = bp_small.as<uint32_t>() + 24;
     v.tmp = np_tmp.as<uint32_t>();
     launch_narrow_phase(bodies(), pairs.as<int2>(), npairs, v, out.view(), stream);
-    HIPCK(hipMemcpyAsync(h_small, v.totals, 8, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    out.C = int(h_small[0]);
-    out.M = int(h_small[1]);
+    if (done) HIPCK(hipEventRecord(done, stream));
+    const int rc = readback({{v.totals, 2, 0}});
+    if (rc) return rc;
+    out.C = int(rb_host[0]);
+    out.M = int(rb_host[1]);
     return NPHYS_OK;
   }
 
@@ -868,22 +923,18 @@ struct World {
       if (round + kRoundBatch >= 0xffff) return set_err(NPHYS_ERR_STATE, "colouring did not converge");
       for (int k = 0; k < kRoundBatch; ++k) launch_colour_round(b, c, M, round + k, v, stream);
       launch_colour_keys(M, v, ncol, stream);
-      HIPCK(hipMemcpyAsync(h_small, v.remaining, 8, hipMemcpyDeviceToHost, stream));
-      HIPCK(hipMemcpyAsync(h_small + 2, ncol, 4, hipMemcpyDeviceToHost, stream));
-      HIPCK(hipStreamSynchronize(stream));
-      if (h_small[0] == 0) break;
+      const int rc = readback({{v.remaining, 2, 0}, {ncol, 1, 2}});
+      if (rc) return rc;
+      if (rb_host[0] == 0) break;
     }
-    ncolours = int(h_small[2]);
-    colour_rounds = int(h_small[1]);
+    ncolours = int(rb_host[2]);
+    colour_rounds = int(rb_host[1]);
     if (ncolours > kMaxColours) return set_err(NPHYS_ERR_STATE, "too many colours");
     launch_colour_order(b, c, M, ncolours, v, stream);
     uint32_t* class_rec = v.col_start + (kMaxColours + 2);
     launch_fix_colour_starts(ncolours, M, C, v, class_rec, stream);
     // class offsets (slots and row records) to the host, read after the assembly is enqueued
-    HIPCK(hipMemcpyAsync(h_colstart, v.col_start, size_t(ncolours + 1) * 4, hipMemcpyDeviceToHost, stream));
-    HIPCK(hipMemcpyAsync(h_colstart + (kMaxColours + 2), class_rec, size_t(ncolours + 1) * 4,
-                         hipMemcpyDeviceToHost, stream));
-    HIPCK(hipEventRecord(ev_class, stream));
+    class_seq = rb_post({{v.col_start, ncolours + 1, kRbClass}, {class_rec, ncolours + 1, kRbClass + kMaxColours + 2}});
     return NPHYS_OK;
   }
 
@@ -891,7 +942,9 @@ struct World {
   // solve index, computed on the host) and the largest class (sizes the sweep grids), from the
   // class offsets read back in colour().
   int visit_order() {
-    HIPCK(hipEventSynchronize(ev_class));
+    const int rc = rb_wait(class_seq);
+    if (rc) return rc;
+    const uint32_t* h_colstart = rb_host + kRbClass;
     colour_visit_order(uint64_t(solve_count), ncolours, colour_visit);
     max_class = 0;
     int* h_range = h_visit + (kMaxColours + 2);
@@ -904,7 +957,7 @@ struct World {
       h_range[2 * k + 1] = size;
     }
     // h_visit is rewritten only by the next solve, after these copies have completed (that solve's
-    // ev_class is recorded after them in stream order)
+    // class readback is enqueued after them in stream order)
     HIPCK(hipMemcpyAsync(sv_visit.p, h_visit, size_t(ncolours) * sizeof(int), hipMemcpyHostToDevice, stream));
     HIPCK(hipMemcpyAsync(sv_visit.as<int>() + (kMaxColours + 2), h_range, size_t(ncolours) * 2 * sizeof(int),
                          hipMemcpyHostToDevice, stream));
@@ -947,8 +1000,8 @@ struct World {
           launch_pgs_colour(b, cv, k, sblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
           kt_used += 2;
           const int c = h_visit[k];
-          counters.sweep_kernel_contacts +=
-              int64_t(h_colstart[kMaxColours + 2 + c + 1]) - int64_t(h_colstart[kMaxColours + 2 + c]);
+          const uint32_t* class_rec = rb_host + kRbClass + kMaxColours + 2;
+          counters.sweep_kernel_contacts += int64_t(class_rec[c + 1]) - int64_t(class_rec[c]);
         }
       return NPHYS_OK;
     }
@@ -1145,11 +1198,11 @@ struct World {
     out.mfirst = sl_mfirst[cur].as<int>();
     out.mcount = sl_mcount[cur].as<int>();
     launch_solve_list(full, cc.M, a, out, pc.view(), prevM, stream);
-    HIPCK(hipMemcpyAsync(h_small + 48, a.totals, 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
-    HIPCK(hipStreamSynchronize(stream));
-    M = int(h_small[48]);
-    C = int(h_small[49]);
-    active_bodies = int64_t(h_small[50]);
+    const int rc = readback({{a.totals, 3, 0}});
+    if (rc) return rc;
+    M = int(rb_host[0]);
+    C = int(rb_host[1]);
+    active_bodies = int64_t(rb_host[2]);
     cv = out;
     return NPHYS_OK;
   }
@@ -1311,9 +1364,8 @@ struct World {
       ++counters.pair_rebuilds;
     }
     HIPCK(hipEventRecord(ev[EV_BP], stream));
-    if ((rc = narrow_phase())) return rc;
-    HIPCK(hipEventRecord(ev[EV_NP], stream));
-    HIPCK(hipEventSynchronize(ev[EV_NP]));
+    // the narrow phase's readback completes the step: every earlier event and copy is done after it
+    if ((rc = narrow_phase(ev[EV_NP]))) return rc;
     finish_residual();
     auto el = [&](int a, int c) {
       float ms = 0.0f;
@@ -1582,6 +1634,35 @@ int nphys_read_contacts(nphys_world* w, int32_t* bodies, uint32_t* feature, floa
       if (normal) { normal[3 * i] = r[2].x; normal[3 * i + 1] = r[2].y; normal[3 * i + 2] = r[2].z; }
       if (depth) depth[i] = r[0].w;
     }
+  return NPHYS_OK;
+}
+
+int nphys_num_manifolds(nphys_world* w, int64_t* n) {
+  if (!w || !n) return set_err(NPHYS_ERR_INVALID, "null argument");
+  *n = w->w.cb[w->w.cur].M;
+  return NPHYS_OK;
+}
+
+int nphys_read_manifolds(nphys_world* w, int32_t* bodies, int32_t* first, int32_t* count, int64_t cap) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  nph::ContactBufs& c = w->w.cb[w->w.cur];
+  if (cap < c.M) return set_err(NPHYS_ERR_CAPACITY, "manifold buffer too small");
+  if (c.M == 0) return NPHYS_OK;
+  std::vector<int2> mb(static_cast<size_t>(c.M));
+  std::vector<int> mc(static_cast<size_t>(c.M));
+  HIPCK(hipMemcpyAsync(mb.data(), c.mbody.p, mb.size() * sizeof(int2), hipMemcpyDeviceToHost, w->w.stream));
+  HIPCK(hipMemcpyAsync(mc.data(), c.mcount.p, mc.size() * 4, hipMemcpyDeviceToHost, w->w.stream));
+  HIPCK(hipStreamSynchronize(w->w.stream));
+  int32_t acc = 0;
+  for (size_t m = 0; m < mb.size(); ++m) {
+    if (bodies) {
+      bodies[2 * m] = mb[m].x;
+      bodies[2 * m + 1] = mb[m].y;
+    }
+    if (first) first[m] = acc;
+    if (count) count[m] = mc[m];
+    acc += mc[m];
+  }
   return NPHYS_OK;
 }
 

@@ -24,7 +24,7 @@ EXPORTED = [
     "nphys_get_params", "nphys_add_bodies", "nphys_num_bodies", "nphys_set_state",
     "nphys_get_state", "nphys_step", "nphys_collide", "nphys_contact_model_set",
     "nphys_num_pairs", "nphys_read_pairs", "nphys_num_contacts", "nphys_read_contacts",
-    "nphys_num_solved_contacts", "nphys_read_impulses", "nphys_residual", "nphys_read_counters",
+    "nphys_num_manifolds", "nphys_read_manifolds", "nphys_num_solved_contacts", "nphys_read_impulses", "nphys_residual", "nphys_read_counters",
     "nphys_reset_counters", "nphys_counters_enable", "nphys_set_residual_target", "nphys_read_colours",
     "nphys_set_sweep_variant", "nphys_set_incremental_pairs", "nphys_read_activation", "nphys_warmstart_load",
     "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
@@ -67,6 +67,8 @@ def load_library(path=LIB_PATH):
     L.nphys_read_pairs.argtypes = [W, ip, C.c_int64]
     L.nphys_num_contacts.argtypes = [W, C.POINTER(C.c_int64)]
     L.nphys_read_contacts.argtypes = [W, ip, C.POINTER(C.c_uint32), fp, fp, fp, fp, C.c_int64]
+    L.nphys_num_manifolds.argtypes = [W, C.POINTER(C.c_int64)]
+    L.nphys_read_manifolds.argtypes = [W, ip, ip, ip, C.c_int64]
     L.nphys_num_solved_contacts.argtypes = [W, C.POINTER(C.c_int64)]
     L.nphys_read_impulses.argtypes = [W, C.POINTER(C.c_uint64), fp, C.c_int64]
     L.nphys_residual.argtypes = [W, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -260,6 +262,18 @@ class MechanicalWorld:
 
     def set_warmstart_judge(self, on=True):
         _check(self._L.nphys_set_warmstart_judge(self._h, 1 if on else 0))
+
+    def manifolds(self):
+        """Contact manifolds of the last collision detection (GeometricalWorld::contact_pairs):
+        bodies (M, 2), index of the first contact in contacts() order, contact count."""
+        n = C.c_int64()
+        _check(self._L.nphys_num_manifolds(self._h, C.byref(n)))
+        m = n.value
+        b = np.zeros((max(m, 1), 2), np.int32)
+        f = np.zeros(max(m, 1), np.int32)
+        k = np.zeros(max(m, 1), np.int32)
+        _check(self._L.nphys_read_manifolds(self._h, abi.iptr(b), abi.iptr(f), abi.iptr(k), max(m, 1)))
+        return {"bodies": b[:m], "first": f[:m], "count": k[:m]}
 
     def activation(self):
         """Per-body activation energy (0: asleep; 0.04 for bodies that never sleep)."""

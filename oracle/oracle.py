@@ -66,6 +66,8 @@ def lib():
         L.orc_reset_counters.argtypes = [vp]
         L.orc_read_colours.argtypes = [vp, ip, C.c_int64, ip]
         L.orc_read_activation.argtypes = [vp, dp, C.c_int64]
+        L.orc_num_manifolds.argtypes = [vp, C.POINTER(C.c_int64)]
+        L.orc_read_manifolds.argtypes = [vp, ip, ip, ip, C.c_int64]
         L.orc_read_body_dynamics.argtypes = [vp, dp, dp, dp]
         L.orc_collide_shapes.argtypes = [C.c_int32, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double,
                                          C.POINTER(C.c_uint32), dp, dp, dp, dp, ip]
@@ -195,6 +197,17 @@ class OracleWorld:
 
     def reset_counters(self):
         _check(self._L.orc_reset_counters(self._h))
+
+    def manifolds(self):
+        """Contact manifolds: bodies (M, 2), first contact index, contact count."""
+        n = C.c_int64()
+        _check(self._L.orc_num_manifolds(self._h, C.byref(n)))
+        m = n.value
+        b = np.zeros((max(m, 1), 2), np.int32)
+        f = np.zeros(max(m, 1), np.int32)
+        k = np.zeros(max(m, 1), np.int32)
+        _check(self._L.orc_read_manifolds(self._h, abi.iptr(b), abi.iptr(f), abi.iptr(k), max(m, 1)))
+        return {"bodies": b[:m], "first": f[:m], "count": k[:m]}
 
     def activation(self):
         """Per-body activation energy (0: asleep)."""

@@ -318,6 +318,26 @@ int orc_read_contacts(void* h, int32_t* bodies, uint32_t* feature, double* w1, d
   });
 }
 
+int orc_num_manifolds(void* h, int64_t* n) {
+  return with_world(h, [&](auto& w) { *n = int64_t(w.manifolds.size()); return NPHYS_OK; });
+}
+
+int orc_read_manifolds(void* h, int32_t* bodies, int32_t* first, int32_t* count, int64_t cap) {
+  return with_world(h, [&](auto& w) {
+    if (int64_t(w.manifolds.size()) > cap) return fail(NPHYS_ERR_CAPACITY, "manifold buffer too small");
+    for (size_t m = 0; m < w.manifolds.size(); ++m) {
+      const orc::Manifold& mf = w.manifolds[m];
+      if (bodies) {
+        bodies[2 * m] = mf.b1;
+        bodies[2 * m + 1] = mf.b2;
+      }
+      if (first) first[m] = mf.first;
+      if (count) count[m] = mf.count;
+    }
+    return NPHYS_OK;
+  });
+}
+
 int orc_num_solved_contacts(void* h, int64_t* n) {
   return with_world(h, [&](auto& w) { *n = int64_t(w.crow.size()); return NPHYS_OK; });
 }

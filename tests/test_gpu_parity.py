@@ -55,6 +55,9 @@ def test_collide_bitexact(nphys, oracle_mod, name):
     print(name, "pairs", gp.shape[0], "contacts", g.contacts()["depth"].shape[0])
     np.testing.assert_array_equal(gp, op)
     _assert_same_contacts(g.contacts(), o.contacts())
+    gm, om_ = g.manifolds(), o.manifolds()
+    for k in ("bodies", "first", "count"):
+        np.testing.assert_array_equal(gm[k], om_[k], err_msg=k)
 
 
 def _state_diff(gs, os_):

@@ -379,3 +379,21 @@ def test_spinning_ball_quaternion_closed_form(oracle_mod, precision, omega):
     tol = 2e-5 if precision == 32 else 1e-12
     np.testing.assert_allclose(sign * q, expect, atol=tol)
     assert o.get_state()["angvel"][0, 2] == pytest.approx(omega, rel=1e-6)
+
+
+def test_manifolds_group_the_contacts(oracle_mod):
+    """Manifold readback (contact_pairs): one manifold per body pair in contact, pairs in
+    increasing (body 1, body 2) order, 1-4 contacts each, covering the contact list in order."""
+    s = scenes.small_pile(n=256)
+    o = oracle_mod.OracleWorld(abi.default_params())
+    o.add_bodies(s)
+    o.step(120)
+    m, c = o.manifolds(), o.contacts()
+    assert m["bodies"].shape[0] > 100
+    assert (m["count"] >= 1).all() and (m["count"] <= 4).all()
+    keys = m["bodies"][:, 0].astype(np.int64) * (1 << 32) + m["bodies"][:, 1]
+    assert (np.diff(keys) > 0).all()
+    np.testing.assert_array_equal(m["first"], np.concatenate([[0], np.cumsum(m["count"])[:-1]]))
+    assert int(m["count"].sum()) == c["bodies"].shape[0]
+    owner = np.repeat(np.arange(m["bodies"].shape[0]), m["count"])
+    np.testing.assert_array_equal(c["bodies"], m["bodies"][owner])

@@ -203,3 +203,22 @@ def test_partial_sleep_coloured_order_tracks_reference_order(oracle_mod):
     a, b = worlds[0].get_state(), worlds[1].get_state()
     np.testing.assert_allclose(a["pos"], b["pos"], atol=1e-3)
     assert np.abs(a["linvel"]).max() < 0.05
+
+
+def test_sleeping_contact_keeps_its_cached_impulse(oracle_mod):
+    """The impulse cache keeps a sleeping contact's impulse (the reference only overwrites the
+    entries of solved contacts): woken after a set state and solved with zero velocity
+    iterations, the contact's impulse is the warm start alone, i.e. the carried m g dt."""
+    r = 0.5
+    s = scenes.with_sleep_threshold(scenes.concat(scenes.ground(), single(0, 0, (r, 0, 0), (0, r + 0.005, 0))),
+                                    0.01)
+    o = _world(oracle_mod, s, 64)
+    o.step(250)
+    assert o.activation()[1] == 0.0 and o.impulses()[0].shape[0] == 0
+    o.set_state(pos=o.get_state()["pos"])
+    o.set_params(abi.default_params(max_velocity_iterations=0))
+    o.step(1)
+    keys, imp = o.impulses()
+    assert keys.shape[0] == 1
+    m = 4.0 / 3.0 * np.pi * r ** 3
+    assert imp[0, 0] == pytest.approx(m * 9.81 / 60.0, rel=1e-4)
