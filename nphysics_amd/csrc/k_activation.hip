@@ -12,7 +12,7 @@
 //                  threshold; union-find parents reset;
 //   k_act_union    islands: union over the manifolds whose bodies are both dynamic (:143-147),
 //                  lock-free (a root is linked under the smaller root by compare-and-swap),
-//                  skipping manifolds whose two bodies keep their islands awake anyway;
+//                  restricted to the manifolds between bodies that could sleep (see there);
 //   k_act_flags    path compression, and an island cannot sleep as soon as one of its bodies has
 //                  no threshold or an energy >= its threshold (:160-172);
 //   k_act_apply    islands that can sleep are put to sleep (energy and velocity zeroed,
@@ -76,11 +76,12 @@ __global__ void __launch_bounds__(256) k_act_wake(DevBodies b) {
   if (a.x == 0.0f && a.y >= 0.0f) b.act[i] = make_float2(a.y * 2.0f, a.y);
 }
 
-__global__ void __launch_bounds__(256) k_act_energy(DevBodies b, int* parent, uint32_t* can) {
+__global__ void __launch_bounds__(256) k_act_energy(DevBodies b, int* parent, uint32_t* can, uint32_t* touch) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
   parent[i] = i;
   can[i] = 1u;
+  touch[i] = 0u;
   if (b.status[i] != STATUS_DYNAMIC) return;
   float2 a = b.act[i];
   if (a.x != 0.0f && a.y >= 0.0f) {
@@ -96,17 +97,24 @@ __global__ void __launch_bounds__(256) k_act_energy(DevBodies b, int* parent, ui
 // A body that keeps its island awake: no threshold, or an energy at or above it.
 __device__ inline bool keeps_awake(float2 a) { return !(a.y >= 0.0f && a.x < a.y); }
 
-// An island can sleep iff none of its bodies keeps it awake, and a body that could sleep is
-// connected to one that keeps it awake iff it is through edges with a body that could sleep at
-// one end at least (follow any path to its first such body). So the union skips the manifolds
-// between two bodies that keep their islands awake: the components containing such a body
-// change, the answer for every body does not.
-__global__ void __launch_bounds__(256) k_act_union(DevBodies b, DevContacts c, int M, int* parent) {
+// An island can sleep iff none of its bodies keeps it awake. Take the components of the bodies
+// that could sleep under the manifolds between two such bodies: if a member of one touches a body
+// that keeps its island awake, its island contains that body; if none does, every neighbour of
+// the component is in it and the component is the whole island. So only those manifolds are
+// united, and a manifold between a body that could sleep and one that keeps its island awake just
+// marks the former (`touch`): the answer for every body is the one of the full islands, with far
+// fewer unions (and smaller trees) while much of a world is moving.
+__global__ void __launch_bounds__(256) k_act_union(DevBodies b, DevContacts c, int M, int* parent, uint32_t* touch) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   const int2 bb = c.mbody[m];
   if (b.status[bb.x] != STATUS_DYNAMIC || b.status[bb.y] != STATUS_DYNAMIC) return;
-  if (keeps_awake(b.act[bb.x]) && keeps_awake(b.act[bb.y])) return;
+  const bool h1 = keeps_awake(b.act[bb.x]), h2 = keeps_awake(b.act[bb.y]);
+  if (h1 || h2) {
+    if (!h1) touch[bb.x] = 1u;
+    if (!h2) touch[bb.y] = 1u;
+    return;
+  }
   const int n = b.n;
   int a = uf_root(parent, bb.x, n), d = uf_root(parent, bb.y, n);
   // every failed exchange means another thread linked root a, which happens at most n - 1 times
@@ -123,13 +131,14 @@ __global__ void __launch_bounds__(256) k_act_union(DevBodies b, DevContacts c, i
   }
 }
 
-__global__ void __launch_bounds__(256) k_act_flags(DevBodies b, int* parent, uint32_t* can) {
+__global__ void __launch_bounds__(256) k_act_flags(DevBodies b, int* parent, uint32_t* can,
+                                                  const uint32_t* __restrict__ touch) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
   if (b.status[i] != STATUS_DYNAMIC) return;
   const int r = uf_root(parent, i, b.n);
   __hip_atomic_store(parent + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  if (keeps_awake(b.act[i])) can[r] = 0u;
+  if (keeps_awake(b.act[i]) || touch[i]) can[r] = 0u;
 }
 
 __global__ void __launch_bounds__(256) k_act_apply(DevBodies b, const int* __restrict__ parent,
@@ -221,9 +230,9 @@ void launch_wake(const DevBodies& b, hipStream_t s) {
 
 void launch_activation(const DevBodies& b, const DevContacts& c, int M, const ActivationBufs& a, hipStream_t s) {
   if (b.n > 0) {
-    hipLaunchKernelGGL(k_act_energy, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can);
-    if (M > 0) hipLaunchKernelGGL(k_act_union, dim3(nb(M)), dim3(256), 0, s, b, c, M, a.parent);
-    hipLaunchKernelGGL(k_act_flags, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can);
+    hipLaunchKernelGGL(k_act_energy, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can, a.touch);
+    if (M > 0) hipLaunchKernelGGL(k_act_union, dim3(nb(M)), dim3(256), 0, s, b, c, M, a.parent, a.touch);
+    hipLaunchKernelGGL(k_act_flags, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can, a.touch);
     hipLaunchKernelGGL(k_act_apply, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can, a.totals + 2);
   }
   if (M > 0) {

@@ -14,6 +14,7 @@ void launch_integrate(const DevBodies& b, const SimParams& p, hipStream_t s);
 struct ActivationBufs {
   int* parent;       // [n] union-find parents
   uint32_t* can;     // [n] per island root: every body may sleep
+  uint32_t* touch;   // [n] body that could sleep touches one that keeps its island awake
   uint32_t* mflag;   // [M] manifold handed to the solver
   uint32_t* pos;     // [M] its index in the solved list
   uint32_t* tmp;     // scan temporary

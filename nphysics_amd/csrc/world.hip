@@ -109,7 +109,7 @@ struct World {
   // due at the next step after the caller set the bodies' state.
   bool sleeping = false, wake_pending = false;
   int64_t n_dynamic = 0, active_bodies = 0;
-  DBuf act_parent, act_can, act_pos, act_tmp, act_mflag[2], sl_mbody[2], sl_mfirst[2], sl_mcount[2];
+  DBuf act_parent, act_can, act_touch, act_pos, act_tmp, act_mflag[2], sl_mbody[2], sl_mfirst[2], sl_mcount[2];
 
   // broad phase
   DBuf bp_dirty, bp_leaf_of, bp_list, bp_pnode;
@@ -1177,6 +1177,7 @@ struct World {
     const size_t m = size_t(std::max(cc.M, 1)), nn = size_t(std::max(n, 1));
     HIPCK(act_parent.ensure(nn * 4));
     HIPCK(act_can.ensure(nn * 4));
+    HIPCK(act_touch.ensure(nn * 4));
     HIPCK(act_mflag[cur].ensure(m * 4));
     HIPCK(act_pos.ensure(m * 4));
     HIPCK(act_tmp.ensure(scan_temp_words(int(m)) * 4 + 64));
@@ -1186,6 +1187,7 @@ struct World {
     ActivationBufs a;
     a.parent = act_parent.as<int>();
     a.can = act_can.as<uint32_t>();
+    a.touch = act_touch.as<uint32_t>();
     a.mflag = act_mflag[cur].as<uint32_t>();
     a.pos = act_pos.as<uint32_t>();
     a.tmp = act_tmp.as<uint32_t>();
