@@ -408,6 +408,7 @@ def main():
     scene = scenes.pile()
     w = world.MechanicalWorld(integration_parameters=params)
     w.add_bodies(scene)
+    w.set_sweep_variant(args.sweep_variant)
     if args.load_state:
         # a settled pile saved by an earlier run (--save-state); the impulse cache starts empty
         st = np.load(args.load_state)
