@@ -103,10 +103,39 @@ struct DevContacts {
   int* mcount;
   float4* rec;       // kContactFields float4 per contact slot, layout above
   float4* imp;       // per slot: (normal, tangent 1, tangent 2) impulses after the solve
+  int* bfirst;       // per body below bcount: its first manifold as body 1, or -1; null: none built
+  int bcount;
 
   NPH_HD float4* contact(long slot) const { return rec + slot * kContactFields; }
   NPH_HD uint32_t feature(long slot) const { return __float_as_uint(rec[slot * kContactFields + 1].w); }
 };
+
+// Index of the manifold of body pair (b1, b2) among the first M manifolds of c (sorted by pair),
+// or -1: through the per-body index (the manifolds of one body 1 are contiguous and sorted by
+// body 2) when there is one, else by binary search.
+__device__ inline int find_manifold(const DevContacts& c, int M, int b1, int b2) {
+  if (c.bfirst) {
+    if (b1 >= c.bcount) return -1;
+    for (int i = c.bfirst[b1]; i >= 0 && i < M; ++i) {
+      const int2 k = c.mbody[i];
+      if (k.x != b1 || k.y > b2) return -1;
+      if (k.y == b2) return i;
+    }
+    return -1;
+  }
+  int lo = 0, hi = M;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const int2 k = c.mbody[mid];
+    if (k.x < b1 || (k.x == b1 && k.y < b2)) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < M) {
+    const int2 k = c.mbody[lo];
+    if (k.x == b1 && k.y == b2) return lo;
+  }
+  return -1;
+}
 
 NPH_HD m33 load_inv(const DevBodies& b, int i, float& inv_mass) { return b.inv_inertia(i, inv_mass); }
 

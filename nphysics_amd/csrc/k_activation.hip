@@ -189,20 +189,11 @@ __global__ void __launch_bounds__(256) k_act_compact(DevContacts c, int M, const
       cnt = uint32_t(count);
     } else {
       const int2 bb = c.mbody[m];
-      int lo = 0, hi = prevM;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        const int2 k = prev.mbody[mid];
-        if (k.x < bb.x || (k.x == bb.x && k.y < bb.y)) lo = mid + 1;
-        else hi = mid;
-      }
       int pf = 0, pc = 0;
-      if (lo < prevM) {
-        const int2 k = prev.mbody[lo];
-        if (k.x == bb.x && k.y == bb.y) {
-          pf = prev.mfirst[lo];
-          pc = prev.mcount[lo];
-        }
+      const int j = find_manifold(prev, prevM, bb.x, bb.y);
+      if (j >= 0) {
+        pf = prev.mfirst[j];
+        pc = prev.mcount[j];
       }
       for (int k = 0; k < count; ++k) {
         const uint32_t feat = c.feature(first + k);

@@ -112,6 +112,14 @@ __global__ void __launch_bounds__(256) k_np_manifolds(const int2* __restrict__ p
   out.mcount[m] = int(nc);
 }
 
+// Per-body index of the manifold list (sorted by body pair): the first manifold of each body 1.
+__global__ void __launch_bounds__(256) k_np_index(int npairs, const uint32_t* __restrict__ totals, DevContacts out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= npairs || m >= int(totals[1])) return;
+  const int b = out.mbody[m].x;
+  if (m == 0 || out.mbody[m - 1].x != b) out.bfirst[b] = m;
+}
+
 __global__ void __launch_bounds__(256) k_gather_contacts(DevContacts c, int M, const uint32_t* __restrict__ moff,
                                                         float4* dense, float4* dense_imp) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -130,6 +138,7 @@ __global__ void __launch_bounds__(256) k_gather_contacts(DevContacts c, int M, c
 
 void launch_narrow_phase(const DevBodies& b, const int2* pairs, int npairs, NarrowPhaseBufs& np,
                          DevContacts out, hipStream_t s) {
+  if (out.bfirst) fill_u32(reinterpret_cast<uint32_t*>(out.bfirst), 0xffffffffu, size_t(out.bcount), s);
   if (npairs <= 0) {
     fill_u32(np.totals, 0u, 2, s);
     return;
@@ -146,6 +155,7 @@ void launch_narrow_phase(const DevBodies& b, const int2* pairs, int npairs, Narr
   scan_exclusive_u32(np.count, np.coff, npairs, np.tmp, s);
   scan_exclusive_u32(np.has, np.moff, npairs, np.tmp, s);
   hipLaunchKernelGGL(k_np_manifolds, dim3(g), dim3(256), 0, s, pairs, npairs, np, out);
+  if (out.bfirst) hipLaunchKernelGGL(k_np_index, dim3(g), dim3(256), 0, s, npairs, np.totals, out);
 }
 
 void launch_gather_contacts(const DevContacts& c, int M, const uint32_t* moff, float4* dense,

@@ -71,19 +71,8 @@ __global__ void __launch_bounds__(256) k_inherit(DevContacts c, int M, DevContac
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
   const int2 key = c.mbody[m];
-  int lo = 0, hi = prevM;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    const int2 k = prev.mbody[mid];
-    if (k.x < key.x || (k.x == key.x && k.y < key.y)) lo = mid + 1;
-    else hi = mid;
-  }
-  uint32_t st = kUncoloured;
-  if (lo < prevM) {
-    const int2 k = prev.mbody[lo];
-    if (k.x == key.x && k.y == key.y) st = prev_state[lo] & 0xffffu;
-  }
-  state[m] = st;
+  const int j = find_manifold(prev, prevM, key.x, key.y);
+  state[m] = j >= 0 ? (prev_state[j] & 0xffffu) : kUncoloured;
 }
 
 __global__ void __launch_bounds__(256) k_adjacency(DevBodies b, DevContacts c, int M,
@@ -261,22 +250,13 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
   BodyView v1, v2;
   if (d1) v1 = load_body(b, b1);
   if (d2) v2 = load_body(b, b2);
-  // previous manifold with the same bodies (lexicographic binary search)
+  // previous manifold with the same bodies
   int pf = 0, pc = 0;
   {
-    int lo = 0, hi = prevM;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      const int2 k = prev.mbody[mid];
-      if (k.x < b1 || (k.x == b1 && k.y < b2)) lo = mid + 1;
-      else hi = mid;
-    }
-    if (lo < prevM) {
-      const int2 k = prev.mbody[lo];
-      if (k.x == b1 && k.y == b2) {
-        pf = prev.mfirst[lo];
-        pc = prev.mcount[lo];
-      }
+    const int j = find_manifold(prev, prevM, b1, b2);
+    if (j >= 0) {
+      pf = prev.mfirst[j];
+      pc = prev.mcount[j];
     }
   }
   const int first = cur.mfirst[m], cnt = cur.mcount[m];

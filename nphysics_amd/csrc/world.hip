@@ -40,8 +40,8 @@ namespace {
 
 // Manifold list plus the contact slots of every broad-phase pair (kPairSlots per pair).
 struct ContactBufs {
-  DBuf mbody, mfirst, mcount, rec, imp;
-  int M = 0, C = 0;
+  DBuf mbody, mfirst, mcount, rec, imp, bfirst;
+  int M = 0, C = 0, nb = 0;  // nb: bodies covered by bfirst
   hipError_t ensure(size_t mcap, size_t slots) {
     hipError_t e;
     if ((e = mbody.ensure(mcap * sizeof(int2))) != hipSuccess) return e;
@@ -58,6 +58,8 @@ struct ContactBufs {
     d.mcount = mcount.as<int>();
     d.rec = rec.as<float4>();
     d.imp = imp.as<float4>();
+    d.bfirst = nb > 0 ? bfirst.as<int>() : nullptr;
+    d.bcount = nb;
     return d;
   }
 };
@@ -754,6 +756,8 @@ struct World {
     for (DBuf* b : {&np_count, &np_coff, &np_has, &np_moff, &np_perm}) HIPCK(b->ensure(P * 4));
     HIPCK(np_tmp.ensure(scan_temp_words(int(P)) * 4 + 64));
     HIPCK(out.ensure(P, kPairSlots * P));
+    HIPCK(out.bfirst.ensure(size_t(std::max(n, 1)) * 4));
+    out.nb = n;
     NarrowPhaseBufs v;
     v.count = np_count.as<uint32_t>();
     v.coff = np_coff.as<uint32_t>();
@@ -1196,6 +1200,8 @@ struct World {
     const DevContacts full = cc.view();
     launch_activation(b, full, cc.M, a, stream);
     DevContacts out = full;
+    out.bfirst = nullptr;  // the solved list has no per-body index
+    out.bcount = 0;
     out.mbody = sl_mbody[cur].as<int2>();
     out.mfirst = sl_mfirst[cur].as<int>();
     out.mcount = sl_mcount[cur].as<int>();
