@@ -616,6 +616,23 @@ __global__ void __launch_bounds__(64) k_pgs(DevBodies b, DevContacts c, SolverBu
     pgs_manifold(acc, b, sb, first + t);
 }
 
+// A run of consecutive small colour classes of the visiting order in one workgroup, a barrier
+// between classes: per body the same updates in the same order as one launch per class, without
+// a launch per class. Each class fits one pass of the workgroup.
+template <bool kWarm>
+__global__ void __launch_bounds__(kRunBlock) k_colour_run(DevBodies b, SolverBufs sb, int kv0, int nk) {
+  const GlobalDv acc{b};
+  for (int kv = kv0; kv < kv0 + nk; ++kv) {
+    int first, n;
+    colour_range(sb, kv, first, n);
+    for (int t = threadIdx.x; t < n; t += kRunBlock) {
+      if (kWarm) warm_manifold(acc, b, sb, first + t);
+      else pgs_manifold(acc, b, sb, first + t);
+    }
+    __syncthreads();
+  }
+}
+
 // Scene-blocked sweeps: one workgroup per scene runs the warm start (when warm) and nsweeps
 // sweeps over all colours in the visiting order, with the scene's dynamic-body velocity
 // increments resident in LDS and a workgroup barrier between colours. Every body sees the same
@@ -931,6 +948,11 @@ void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int bloc
   } else {
     hipLaunchKernelGGL(k_pgs, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
   }
+}
+
+void launch_colour_run(const DevBodies& b, SolverBufs& sb, int kv0, int nk, bool warm, hipStream_t s) {
+  if (warm) hipLaunchKernelGGL(k_colour_run<true>, dim3(1), dim3(kRunBlock), 0, s, b, sb, kv0, nk);
+  else hipLaunchKernelGGL(k_colour_run<false>, dim3(1), dim3(kRunBlock), 0, s, b, sb, kv0, nk);
 }
 
 int residual_blocks(int M) { return int(nb(M > 0 ? M : 1)); }

@@ -132,6 +132,10 @@ int sweep_blocks(int max_class);
 int position_blocks(int max_class);
 void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, int blocks,
                              SolverBufs& sb, hipStream_t s);
+// Consecutive visited classes kv0 .. kv0 + nk - 1, each of at most kRunBlock manifolds, in one
+// workgroup (warm start or sweep).
+constexpr int kRunBlock = 512;
+void launch_colour_run(const DevBodies& b, SolverBufs& sb, int kv0, int nk, bool warm, hipStream_t s);
 void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, hipStream_t s);

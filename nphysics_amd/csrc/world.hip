@@ -1009,13 +1009,26 @@ struct World {
         }
       return NPHYS_OK;
     }
-    auto body = [&]() {
-      if (warm)
-        for (int k = 0; k < ncolours; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
-      for (int it = 0; it < nsweeps; ++it)
-        for (int k = 0; k < ncolours; ++k) launch_pgs_colour(b, cv, k, sblocks, v, stream);
+    // Direct launches (the visiting order changes every solve, so the launch sequence does): two or
+    // more consecutive small classes of the visiting order run back to back in one workgroup.
+    const int* h_range = h_visit + (kMaxColours + 2);
+    auto pass = [&](bool w) {
+      for (int k = 0; k < ncolours;) {
+        int e = k;
+        while (e < ncolours && h_range[2 * e + 1] <= kRunBlock) ++e;
+        if (e - k >= 2) {
+          launch_colour_run(b, v, k, e - k, w, stream);
+          k = e;
+          continue;
+        }
+        if (w) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
+        else launch_pgs_colour(b, cv, k, sblocks, v, stream);
+        ++k;
+      }
     };
-    return run_phase(warm ? 1 : 3, b, cv, v, sblocks, nsweeps, body);
+    if (warm) pass(true);
+    for (int it = 0; it < nsweeps; ++it) pass(false);
+    return NPHYS_OK;
   }
 
   // Residual of the fixed-count mode: the partials are copied into pinned memory without waiting
