@@ -485,7 +485,6 @@ def main():
             "colours": ncol,
             "velocity_iterations": iters,
             "position_iterations": params.max_position_iterations,
-            "sweep_graphs": os.environ.get("NPHYS_DISABLE_GRAPHS", "0") != "1",
             "loose_changes_per_step": c["loose_changes"] / args.steps,
             "pair_rebuilds_per_step": c["pair_rebuilds"] / args.steps,
         },
@@ -496,7 +495,8 @@ def main():
                                ("collision_detection", "broad_phase", "narrow_phase", "assembly",
                                 "velocity_resolution", "velocity_update", "position_resolution")},
         "roofline": {
-            "kernel": "k_pgs (one colour class of one SOR-Prox sweep; average over every launch of the timing region)",
+            "kernel": "k_pgs (one colour class of one SOR-Prox sweep, classes of more than 512 manifolds; "
+                      "average over every k_pgs launch of the timing region)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBPS,

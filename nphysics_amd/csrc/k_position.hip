@@ -111,6 +111,16 @@ __global__ void __launch_bounds__(64) k_position(DevBodies b, SimParams p, Solve
     position_one(b, p, sb, first_slot + t);
 }
 
+// Consecutive small classes of the visiting order in one workgroup, a barrier between classes
+// (the same corrections in the same order per body as one launch per class).
+__global__ void __launch_bounds__(kRunBlock) k_position_run(DevBodies b, SimParams p, SolverBufs sb, int kv0, int nk) {
+  for (int kv = kv0; kv < kv0 + nk; ++kv) {
+    const int2 range = sb.vrange[kv];
+    for (int t = threadIdx.x; t < range.y; t += kRunBlock) position_one(b, p, sb, range.x + t);
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 // Sized like the sweep grid: from the largest colour class.
@@ -120,6 +130,10 @@ void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimP
                             SolverBufs& sb, hipStream_t s) {
   (void)c;
   hipLaunchKernelGGL(k_position, dim3(blocks), dim3(64), 0, s, b, p, sb, k);
+}
+
+void launch_position_run(const DevBodies& b, const SimParams& p, SolverBufs& sb, int kv0, int nk, hipStream_t s) {
+  hipLaunchKernelGGL(k_position_run, dim3(1), dim3(kRunBlock), 0, s, b, p, sb, kv0, nk);
 }
 
 }  // namespace nph

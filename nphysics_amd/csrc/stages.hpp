@@ -174,5 +174,6 @@ void launch_lam_copy(SolverBufs& sb, int C, float4* buf, bool to_buf, hipStream_
 // k_position.hip ---------------------------------------------------------------------------
 void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k,
                             int blocks, SolverBufs& sb, hipStream_t s);
+void launch_position_run(const DevBodies& b, const SimParams& p, SolverBufs& sb, int kv0, int nk, hipStream_t s);
 
 }  // namespace nph

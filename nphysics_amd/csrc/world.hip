@@ -154,14 +154,6 @@ struct World {
   long solve_count = 0;    // solves performed, never reset
   DBuf sv_visit;           // device copy of colour_visit, then the classes' slot ranges in that order
   int* h_visit = nullptr;  // pinned staging of the same
-  // Instantiated graphs of the sweep phases, keyed by a hash of every kernel argument: the
-  // launch sequence only depends on the colour count, the iteration counts and the buffers.
-  struct GraphEntry {
-    uint64_t key;
-    hipGraphExec_t exec;
-  };
-  std::vector<GraphEntry> graphs;
-  bool use_graphs = true;
   int solved_M = 0, solved_C = 0;
   double res_inf = 0.0, res_rms = 0.0;
   double* red_pinned = nullptr;  // deferred residual partials (pinned)
@@ -214,7 +206,6 @@ struct World {
   int kt_used = 0;
 
   ~World() {
-    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
     if (h_visit) (void)hipHostFree(h_visit);
     if (red_pinned) (void)hipHostFree(red_pinned);
     if (rb_host) (void)hipHostFree(rb_host);
@@ -257,9 +248,6 @@ struct World {
     for (auto& e : tune_ev) HIPCK(hipEventCreate(&e));
     set_params(p);
     HIPCK(bp_small.ensure(64 * sizeof(uint32_t)));
-    // Profilers that cannot follow graph launches can turn them off; kernels are the same.
-    const char* ng = std::getenv("NPHYS_DISABLE_GRAPHS");
-    use_graphs = !(ng && ng[0] == '1');
     return NPHYS_OK;
   }
 
@@ -841,60 +829,6 @@ struct World {
     return v;
   }
 
-  static void hash_bytes(uint64_t& h, const void* p, size_t n) {
-    const unsigned char* c = static_cast<const unsigned char*>(p);
-    for (size_t i = 0; i < n; ++i) {
-      h ^= c[i];
-      h *= 0x100000001b3ull;
-    }
-  }
-
-  // Runs one sweep phase (its launches enqueued by `body`) through a cached hipGraph; the graph
-  // is captured the first time its kernel arguments occur.
-  template <class F>
-  int run_phase(int phase, const DevBodies& b, const DevContacts& c, const SolverBufs& v, int blocks, int reps,
-                F&& body) {
-    if (!use_graphs) {
-      body();
-      return NPHYS_OK;
-    }
-    uint64_t key = 0xcbf29ce484222325ull;
-    hash_bytes(key, &phase, sizeof(phase));
-    hash_bytes(key, &b, sizeof(b));
-    hash_bytes(key, &c, sizeof(c));
-    hash_bytes(key, &v, sizeof(v));
-    hash_bytes(key, &sp, sizeof(sp));
-    hash_bytes(key, &blocks, sizeof(blocks));
-    hash_bytes(key, &ncolours, sizeof(ncolours));
-    hash_bytes(key, &reps, sizeof(reps));
-    for (size_t i = 0; i < graphs.size(); ++i)
-      if (graphs[i].key == key) {
-        const GraphEntry g = graphs[i];
-        // least recently used first: the hit moves to the back
-        graphs.erase(graphs.begin() + long(i));
-        graphs.push_back(g);
-        HIPCK(hipGraphLaunch(g.exec, stream));
-        return NPHYS_OK;
-      }
-    hipGraph_t graph = nullptr;
-    HIPCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    body();
-    HIPCK(hipStreamEndCapture(stream, &graph));
-    hipGraphExec_t exec = nullptr;
-    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    HIPCK(e);
-    if (graphs.size() >= 8) {
-      // the evicted graph may still be executing from an earlier launch on this stream
-      HIPCK(hipStreamSynchronize(stream));
-      (void)hipGraphExecDestroy(graphs.front().exec);
-      graphs.erase(graphs.begin());
-    }
-    graphs.push_back(GraphEntry{key, exec});
-    HIPCK(hipGraphLaunch(exec, stream));
-    return NPHYS_OK;
-  }
-
   // Visit order of the colour classes for one solve: Fisher-Yates driven by SplitMix64 seeded
   // with the solve index, so every step orders conflicting manifolds afresh while the colours
   // themselves persist between steps.
@@ -990,6 +924,9 @@ struct World {
       }
       return NPHYS_OK;
     }
+    // Direct launches (the visiting order changes every solve, so the launch sequence does): two or
+    // more consecutive small classes of the visiting order run back to back in one workgroup. With
+    // counters enabled each k_pgs launch carries HIP events in its dispatch packet.
     if (counters_enabled) {
       const size_t need = size_t(kt_used) + size_t(2) * size_t(nsweeps) * size_t(ncolours);
       while (kt_ev.size() < need) {
@@ -997,21 +934,9 @@ struct World {
         HIPCK(hipEventCreate(&e));
         kt_ev.push_back(e);
       }
-      if (warm)
-        for (int k = 0; k < ncolours; ++k) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
-      for (int it = 0; it < nsweeps; ++it)
-        for (int k = 0; k < ncolours; ++k) {
-          launch_pgs_colour(b, cv, k, sblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
-          kt_used += 2;
-          const int c = h_visit[k];
-          const uint32_t* class_rec = rb_host + kRbClass + kMaxColours + 2;
-          counters.sweep_kernel_contacts += int64_t(class_rec[c + 1]) - int64_t(class_rec[c]);
-        }
-      return NPHYS_OK;
     }
-    // Direct launches (the visiting order changes every solve, so the launch sequence does): two or
-    // more consecutive small classes of the visiting order run back to back in one workgroup.
     const int* h_range = h_visit + (kMaxColours + 2);
+    const uint32_t* class_rec = rb_host + kRbClass + kMaxColours + 2;
     auto pass = [&](bool w) {
       for (int k = 0; k < ncolours;) {
         int e = k;
@@ -1021,8 +946,16 @@ struct World {
           k = e;
           continue;
         }
-        if (w) launch_warmstart_colour(b, cv, k, sblocks, v, stream);
-        else launch_pgs_colour(b, cv, k, sblocks, v, stream);
+        if (w) {
+          launch_warmstart_colour(b, cv, k, sblocks, v, stream);
+        } else if (counters_enabled) {
+          launch_pgs_colour(b, cv, k, sblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
+          kt_used += 2;
+          const int c = h_visit[k];
+          counters.sweep_kernel_contacts += int64_t(class_rec[c + 1]) - int64_t(class_rec[c]);
+        } else {
+          launch_pgs_colour(b, cv, k, sblocks, v, stream);
+        }
         ++k;
       }
     };
@@ -1344,15 +1277,21 @@ struct World {
     SolverBufs v = sv_view();
     const DevContacts cv = sview[cur];
     const int pblocks = position_blocks(max_class);
-    auto body = [&]() {
-      for (int it = 0; it < prm.max_position_iterations; ++it)
-        for (int k = 0; k < ncolours; ++k) launch_position_colour(b, cv, sp, k, pblocks, v, stream);
-    };
-    if (counters_enabled) {
-      body();
-      return NPHYS_OK;
-    }
-    return run_phase(2, b, cv, v, pblocks, prm.max_position_iterations, body);
+    // direct launches, runs of two or more consecutive small classes in one workgroup (as the sweeps)
+    const int* h_range = h_visit + (kMaxColours + 2);
+    for (int it = 0; it < prm.max_position_iterations; ++it)
+      for (int k = 0; k < ncolours;) {
+        int e = k;
+        while (e < ncolours && h_range[2 * e + 1] <= kRunBlock) ++e;
+        if (e - k >= 2) {
+          launch_position_run(b, sp, v, k, e - k, stream);
+          k = e;
+          continue;
+        }
+        launch_position_colour(b, cv, sp, k, pblocks, v, stream);
+        ++k;
+      }
+    return NPHYS_OK;
   }
 
   // ------------------------------------------------------------------ step

@@ -310,9 +310,8 @@ def test_large_and_small_classes_bitexact(nphys, oracle_mod):
 
 
 def test_settling_pile_long_run_bitexact(nphys, oracle_mod):
-    """300 steps of a settling pile: the colour count, the large/small class split and the grid
-    size change many times, so the cached sweep graphs are replaced over and over (an evicted
-    graph may still be executing when it is replaced); the trajectory stays bit-exact."""
+    """300 steps of a settling pile: the colour count, the class sizes, the runs of small classes
+    launched together and the grid size change many times; the trajectory stays bit-exact."""
     s = scenes.small_pile(seed=17, n=1024, lattice=(16, 4, 16), extent=12.0, y0=0.3)
     g = _gpu_world(nphys, s)
     o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_COLOURED)
