@@ -8,10 +8,10 @@ an open five-wall box (60 x 40 x 60 m) and settled untimed until the contact cou
 iterations, dt = 1/60, FP32. Inputs are resident in HBM for the whole run.
 
 Printed: one JSON line with steps/s, contact-iterations/s, the residual, per-stage times, the
-roofline of the dominant kernel and the CPU baseline. The K timed steps launch the sweep phases
-as hipGraphs with counters off; the roofline comes from a following region of --timing-steps
-steps with counters on, where every launch of the SOR-Prox sweep kernel carries HIP events in its
-dispatch packet on the world's stream. The CPU baseline is the FP32 CPU restatement of the
+roofline of the dominant kernel and the CPU baseline. The K timed steps run with counters off;
+the roofline comes from a following region of --timing-steps steps with counters on, where every
+launch of the SOR-Prox sweep kernel carries HIP events in its dispatch packet on the world's
+stream. The CPU baseline is the FP32 CPU restatement of the
 reference step, one thread, timed on a bounded sample of the same settled scene.
 """
 import argparse
@@ -423,7 +423,7 @@ def main():
         np.savez(args.save_state, **snapshot)
     w.step(args.warmup)
 
-    # Timed region: K steps, sweep phases launched as hipGraphs, counters off.
+    # Timed region: K steps, counters off.
     w.reset_counters()
     w.enable_counters(False)
     t0 = time.perf_counter()
