@@ -487,6 +487,7 @@ def main():
             "position_iterations": params.max_position_iterations,
             "loose_changes_per_step": c["loose_changes"] / args.steps,
             "pair_rebuilds_per_step": c["pair_rebuilds"] / args.steps,
+            "incremental_pair_updates_per_step": c["incremental_pair_updates"] / args.steps,
         },
         "contact_iters_per_s": C * iters * args.steps / elapsed,
         "residual_inf": rinf,

@@ -663,6 +663,12 @@ struct World {
         ++inc_since_full;
         ++counters.incremental_pair_updates;
         npairs = int(rb_host[2] + rb_host[3]);
+      } else if (rb_host[1] == 0) {
+        // a clean body's segment took more appended partners than it has room for: the full build
+        // below gets segments that large, so the following incremental updates fit
+        cap = int(rb_host[0]) + 8;
+        if ((rc = ensure_bp())) return rc;
+        v = bp_view();
       }
     }
     if (!done) {
