@@ -13,8 +13,9 @@
 //   k_act_union    islands: union over the manifolds whose bodies are both dynamic (:143-147),
 //                  lock-free (a root is linked under the smaller root by compare-and-swap),
 //                  restricted to the manifolds between bodies that could sleep (see there);
-//   k_act_flags    path compression, and an island cannot sleep as soon as one of its bodies has
-//                  no threshold or an energy >= its threshold (:160-172);
+//   k_act_flags    every body stores its root (read-only walks: no other body overwrites it), and
+//                  an island cannot sleep as soon as one of its bodies has no threshold or an
+//                  energy >= its threshold (:160-172);
 //   k_act_apply    islands that can sleep are put to sleep (energy and velocity zeroed,
 //                  rigid_body.rs:396-400), the others woken with 2 x threshold (body.rs:338-342);
 //                  counts the awake dynamic bodies;
@@ -67,6 +68,17 @@ __device__ inline void block_add(uint32_t v, uint32_t* out) {
     const uint32_t s = (part[0] + part[1]) + (part[2] + part[3]);
     if (s) atomicAdd(out, s);
   }
+}
+
+// Root of x without writes: used once the trees are final, where every body stores its own root
+// and no walk may overwrite that with an older ancestor it read before.
+__device__ inline int uf_root_ro(const int* p, int x, int n) {
+  for (int it = 0; it < n; ++it) {
+    const int y = uf_load(p, x);
+    if (y == x) break;
+    x = y;
+  }
+  return x;
 }
 
 __global__ void __launch_bounds__(256) k_act_wake(DevBodies b) {
@@ -136,7 +148,9 @@ __global__ void __launch_bounds__(256) k_act_flags(DevBodies b, int* parent, uin
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
   if (b.status[i] != STATUS_DYNAMIC) return;
-  const int r = uf_root(parent, i, b.n);
+  // read-only walk: a halving write by another body's walk could land after this body stored its
+  // root and leave it pointing at a non-root ancestor, whose flag k_act_apply would then read
+  const int r = uf_root_ro(parent, i, b.n);
   __hip_atomic_store(parent + i, r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (keeps_awake(b.act[i]) || touch[i]) can[r] = 0u;
 }
