@@ -98,7 +98,32 @@ typedef struct nphys_body_desc {
    * (Some(0.01)): pass 0.01 for the reference's default. nphys_set_state wakes every sleeping body
    * (2 x threshold) at the next step, as setting a pose or velocity does in the reference. */
   const float* sleep_threshold;
+  /* Optional rigid-body and material attributes; NULL gives the RigidBodyDesc / BasicMaterial
+   * default (rigid_body.rs:972-990, basic_material.rs:33-43). */
+  const float* linear_damping;       /* n: default 0 (RigidBodyDesc::linear_damping) */
+  const float* angular_damping;      /* n: default 0 */
+  const float* max_linear_velocity;  /* n: default FLT_MAX (set_max_linear_velocity) */
+  const float* max_angular_velocity; /* n: default FLT_MAX */
+  const int32_t* gravity_enabled;    /* n: default 1 (RigidBodyDesc::gravity_enabled) */
+  /* n: jacobian mask as kinematic dofs (set_translations_kinematic / set_rotations_kinematic,
+   * rigid_body.rs:103-118): bit k set makes dof k kinematic, bits 0-2 the translations x, y, z,
+   * bits 3-5 the rotations. Default 0 (every dof free). */
+  const int32_t* kinematic_dofs;
+  const int32_t* friction_combine;    /* n: NPHYS_COMBINE_* (BasicMaterial::friction_combine_mode) */
+  const int32_t* restitution_combine; /* n: NPHYS_COMBINE_* */
+  /* n: BasicMaterial::id for the coefficient lookup table (nphys_set_material_coefficient), a
+   * negative value for None (the default) */
+  const int32_t* material_id;
+  /* 3n: BasicMaterial::surface_velocity in the collider's frame (default none = zero) */
+  const float* surface_velocity;
 } nphys_body_desc;
+
+/* MaterialCombineMode (material.rs:46-66); the mode of higher precedence of the two colliders
+ * decides: Max > Multiply > Min > Average. */
+enum { NPHYS_COMBINE_AVERAGE = 0, NPHYS_COMBINE_MIN = 1, NPHYS_COMBINE_MULTIPLY = 2, NPHYS_COMBINE_MAX = 3 };
+/* ForceType (src/algebra/mod.rs:17-26): a force, an impulse, an acceleration change
+ * and a velocity change. */
+enum { NPHYS_FORCE = 0, NPHYS_IMPULSE = 1, NPHYS_ACCELERATION_CHANGE = 2, NPHYS_VELOCITY_CHANGE = 3 };
 #define NPHYS_MAX_SCENES 32768
 
 /* Counters (src/counters/mod.rs:17-219): accumulated stage seconds plus sizes of the last step. */
@@ -171,6 +196,28 @@ int nphys_num_bodies(const nphys_world* w, int32_t* n);
 int nphys_set_state(nphys_world* w, const float* pos, const float* rot, const float* linvel,
                     const float* angvel);
 int nphys_get_state(nphys_world* w, float* pos, float* rot, float* linvel, float* angvel);
+
+/* Body::apply_force (rigid_body.rs:771-822) for `count` entries: body handle, world-frame force
+ * (fx, fy, fz, torque x, y, z) and the ForceType `type` (NPHYS_FORCE ...). Forces and acceleration
+ * changes accumulate into the body's external forces, applied by the next step and then cleared
+ * (mechanical_world.rs:365); impulses and velocity changes change the velocity now, impulses and
+ * acceleration changes through the (inverse) augmented mass of the body's last dynamics update.
+ * Each value is masked by the body's jacobian mask; non-dynamic bodies are ignored. With
+ * auto_wake_up the body is activated (energy 2 x its sleep threshold, body.rs:338-342). Entries of
+ * one body are applied in the given order. */
+int nphys_apply_force(nphys_world* w, int32_t count, const int32_t* bodies, const float* forces,
+                      int32_t type, int32_t auto_wake_up);
+/* Body::apply_force_at_point (rigid_body.rs:835-845): linear force (3 per entry) at a world point
+ * (3 per entry), i.e. the force (f, (point - com) x f). */
+int nphys_apply_force_at_point(nphys_world* w, int32_t count, const int32_t* bodies, const float* forces,
+                               const float* points, int32_t type, int32_t auto_wake_up);
+
+/* MaterialsCoefficientsTable::set_friction_coefficient / set_restitution_coefficient
+ * (materials_coefficients_table.rs:23-31): the coefficient used for a contact between colliders
+ * whose materials have ids id1 and id2 (either order), instead of the combine modes. which: 0
+ * friction, 1 restitution. At most NPHYS_MAX_MATERIAL_PAIRS pairs per kind. */
+#define NPHYS_MAX_MATERIAL_PAIRS 65536
+int nphys_set_material_coefficient(nphys_world* w, int32_t which, int32_t id1, int32_t id2, float coeff);
 
 /* ---------------------------------------------------------------- stepping */
 

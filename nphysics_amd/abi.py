@@ -20,6 +20,30 @@ SHAPE_BALL = 0
 SHAPE_CUBOID = 1
 CONTACT_MODEL_SIGNORINI_COULOMB_PYRAMID = 0
 CONTACT_MODEL_LEARNED_WARMSTART = 1
+COMBINE_AVERAGE = 0
+COMBINE_MIN = 1
+COMBINE_MULTIPLY = 2
+COMBINE_MAX = 3
+FORCE = 0
+IMPULSE = 1
+ACCELERATION_CHANGE = 2
+VELOCITY_CHANGE = 3
+
+# optional per-body arrays of nphys_body_desc: name -> (dtype, width)
+OPTIONAL_BODY_FIELDS = {
+    "scene": (np.int32, 1),
+    "sleep_threshold": (np.float32, 1),
+    "linear_damping": (np.float32, 1),
+    "angular_damping": (np.float32, 1),
+    "max_linear_velocity": (np.float32, 1),
+    "max_angular_velocity": (np.float32, 1),
+    "gravity_enabled": (np.int32, 1),
+    "kinematic_dofs": (np.int32, 1),
+    "friction_combine": (np.int32, 1),
+    "restitution_combine": (np.int32, 1),
+    "material_id": (np.int32, 1),
+    "surface_velocity": (np.float32, 3),
+}
 
 
 class Params(C.Structure):
@@ -92,6 +116,16 @@ class BodyDesc(C.Structure):
         ("restitution", C.POINTER(C.c_float)),
         ("scene", C.POINTER(C.c_int32)),
         ("sleep_threshold", C.POINTER(C.c_float)),
+        ("linear_damping", C.POINTER(C.c_float)),
+        ("angular_damping", C.POINTER(C.c_float)),
+        ("max_linear_velocity", C.POINTER(C.c_float)),
+        ("max_angular_velocity", C.POINTER(C.c_float)),
+        ("gravity_enabled", C.POINTER(C.c_int32)),
+        ("kinematic_dofs", C.POINTER(C.c_int32)),
+        ("friction_combine", C.POINTER(C.c_int32)),
+        ("restitution_combine", C.POINTER(C.c_int32)),
+        ("material_id", C.POINTER(C.c_int32)),
+        ("surface_velocity", C.POINTER(C.c_float)),
     ]
 
 
@@ -174,8 +208,8 @@ def make_body_desc(scene):
     d.linear_prediction = fptr(arr("linear_prediction", np.float32, 1))
     d.friction = fptr(arr("friction", np.float32, 1))
     d.restitution = fptr(arr("restitution", np.float32, 1))
-    if "scene" in scene:
-        d.scene = iptr(arr("scene", np.int32, 1))
-    if "sleep_threshold" in scene:
-        d.sleep_threshold = fptr(arr("sleep_threshold", np.float32, 1))
+    for name, (dtype, width) in OPTIONAL_BODY_FIELDS.items():
+        if name in scene:
+            a = arr(name, dtype, width)
+            setattr(d, name, iptr(a) if dtype == np.int32 else fptr(a))
     return d, keep

@@ -56,6 +56,23 @@ struct DevBodies {
   float4* loose_mn;
   float4* loose_mx;
   float2* act;           // (activation energy, sleep threshold); null when no body can sleep
+  // Optional per-body attributes, null when every body has the default:
+  const float4* damp;    // (linear damping, angular damping, max |v|, max |w|)
+  const int* flags;      // kinematic dofs (bits 0-5), gravity disabled (bit 6), friction and
+                         // restitution combine modes (bits 8-9, 10-11); null: all defaults
+  const int* kmask;      // = flags when some dynamic body has a kinematic dof, else null
+  const int* mat_id;     // material id for the coefficient table (< 0: none)
+  const float4* surf_vel;  // surface velocity in the collider frame
+  float4* f_lin;         // accumulated external forces, null when none are pending
+  float4* f_ang;
+  float4* w_dyn;         // angular velocity of the last dynamics update (written by k_integrate)
+
+  // Jacobian mask of body i (1 free, 0 kinematic), linear and angular (rigid_body.rs:46,103-118).
+  NPH_HD void mask(int i, f3& ml, f3& ma) const {
+    const int k = kmask ? kmask[i] : 0;
+    ml = f3{(k & 1) ? 0.0f : 1.0f, (k & 2) ? 0.0f : 1.0f, (k & 4) ? 0.0f : 1.0f};
+    ma = f3{(k & 8) ? 0.0f : 1.0f, (k & 16) ? 0.0f : 1.0f, (k & 32) ? 0.0f : 1.0f};
+  }
 
   NPH_HD float4& pos(int i) const { return pose[long(i) * kPoseBlock]; }
   NPH_HD float4& rot(int i) const { return pose[long(i) * kPoseBlock + 1]; }
@@ -138,5 +155,46 @@ __device__ inline int find_manifold(const DevContacts& c, int M, int b1, int b2)
 }
 
 NPH_HD m33 load_inv(const DevBodies& b, int i, float& inv_mass) { return b.inv_inertia(i, inv_mass); }
+
+// Component-wise product (the jacobian mask, rigid_body.rs:615,690).
+NPH_HD f3 cmul(f3 a, f3 m) { return f3{a.x * m.x, a.y * m.y, a.z * m.z}; }
+
+// Body flag bits (DevBodies::flags).
+constexpr int kFlagNoGravity = 64;
+NPH_HD int fric_mode(int flags) { return (flags >> 8) & 3; }
+NPH_HD int rest_mode(int flags) { return (flags >> 10) & 3; }
+
+// MaterialCombineMode::combine (material.rs:72-86): Max > Multiply > Min > Average.
+NPH_HD float combine_coeff(float a, int ma, float b, int mb) {
+  if (ma == 3 || mb == 3) return a > b ? a : b;
+  if (ma == 2 || mb == 2) return a * b;
+  if (ma == 1 || mb == 1) return a < b ? a : b;
+  return (a + b) * 0.5f;
+}
+
+// Sorted material-pair table (MaterialsCoefficientsTable): key (min id << 32 | max id).
+struct MatTable {
+  const unsigned long long* fkey;
+  const float* fval;
+  int nf;
+  const unsigned long long* rkey;
+  const float* rval;
+  int nr;
+};
+
+__device__ inline bool mat_lookup(const unsigned long long* key, const float* val, int n, unsigned long long k,
+                                  float& out) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (key[mid] < k) lo = mid + 1;
+    else hi = mid;
+  }
+  if (lo < n && key[lo] == k) {
+    out = val[lo];
+    return true;
+  }
+  return false;
+}
 
 }  // namespace nph

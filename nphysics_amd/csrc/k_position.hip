@@ -20,6 +20,7 @@ namespace nph {
 
 namespace {
 
+template <bool kMask>
 __device__ __forceinline__ void position_one(const DevBodies& b, const SimParams& p, const SolverBufs& sb,
                                              int slot) {
   const int4 q = sb.sinfo[slot];
@@ -37,6 +38,12 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const SimParams
   m33 I1, I2;
   if (d1) I1 = load_inv(b, b1, im1);
   if (d2) I2 = load_inv(b, b2, im2);
+  // jacobian masks (fill_constraint_geometry masks the jacobian, rigid_body.rs:685-705)
+  f3 ml1{1.0f, 1.0f, 1.0f}, ma1{1.0f, 1.0f, 1.0f}, ml2{1.0f, 1.0f, 1.0f}, ma2{1.0f, 1.0f, 1.0f};
+  if (kMask) {
+    b.mask(b1, ml1, ma1);
+    b.mask(b2, ml2, ma2);
+  }
   // the manifold's kinematics (at most 4 contacts, contiguous position records written by the
   // assembly in colour order) are loaded up front, independent of the poses
   int kind[4];
@@ -69,15 +76,17 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const SimParams
     f3 wl1{0.0f, 0.0f, 0.0f}, wa1{0.0f, 0.0f, 0.0f}, wl2{0.0f, 0.0f, 0.0f}, wa2{0.0f, 0.0f, 0.0f};
     if (d1) {
       const f3 dir = neg(g.n);
-      const f3 ang = cross(sub(w1, t1), dir);
-      wl1 = scale(dir, im1);
+      f3 ang = cross(sub(w1, t1), dir);
+      if (kMask) ang = cmul(ang, ma1);
+      wl1 = scale(kMask ? cmul(dir, ml1) : dir, im1);
       wa1 = mulv(I1, ang);
       inv_r = inv_r + (im1 + dot(ang, wa1));
     }
     if (d2) {
       const f3 dir = g.n;
-      const f3 ang = cross(sub(w2, t2), dir);
-      wl2 = scale(dir, im2);
+      f3 ang = cross(sub(w2, t2), dir);
+      if (kMask) ang = cmul(ang, ma2);
+      wl2 = scale(kMask ? cmul(dir, ml2) : dir, im2);
       wa2 = mulv(I2, ang);
       inv_r = inv_r + (im2 + dot(ang, wa2));
     }
@@ -103,20 +112,22 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const SimParams
   }
 }
 
+template <bool kMask>
 __global__ void __launch_bounds__(64) k_position(DevBodies b, SimParams p, SolverBufs sb, int kv) {
   const int2 range = sb.vrange[kv];
   const int first_slot = range.x;
   const int nslots = range.y;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nslots; t += gridDim.x * blockDim.x)
-    position_one(b, p, sb, first_slot + t);
+    position_one<kMask>(b, p, sb, first_slot + t);
 }
 
 // Consecutive small classes of the visiting order in one workgroup, a barrier between classes
 // (the same corrections in the same order per body as one launch per class).
+template <bool kMask>
 __global__ void __launch_bounds__(kRunBlock) k_position_run(DevBodies b, SimParams p, SolverBufs sb, int kv0, int nk) {
   for (int kv = kv0; kv < kv0 + nk; ++kv) {
     const int2 range = sb.vrange[kv];
-    for (int t = threadIdx.x; t < range.y; t += kRunBlock) position_one(b, p, sb, range.x + t);
+    for (int t = threadIdx.x; t < range.y; t += kRunBlock) position_one<kMask>(b, p, sb, range.x + t);
     __syncthreads();
   }
 }
@@ -129,11 +140,13 @@ int position_blocks(int max_class) { return sweep_blocks(max_class); }
 void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k, int blocks,
                             SolverBufs& sb, hipStream_t s) {
   (void)c;
-  hipLaunchKernelGGL(k_position, dim3(blocks), dim3(64), 0, s, b, p, sb, k);
+  if (b.kmask) hipLaunchKernelGGL(k_position<true>, dim3(blocks), dim3(64), 0, s, b, p, sb, k);
+  else hipLaunchKernelGGL(k_position<false>, dim3(blocks), dim3(64), 0, s, b, p, sb, k);
 }
 
 void launch_position_run(const DevBodies& b, const SimParams& p, SolverBufs& sb, int kv0, int nk, hipStream_t s) {
-  hipLaunchKernelGGL(k_position_run, dim3(1), dim3(kRunBlock), 0, s, b, p, sb, kv0, nk);
+  if (b.kmask) hipLaunchKernelGGL(k_position_run<true>, dim3(1), dim3(kRunBlock), 0, s, b, p, sb, kv0, nk);
+  else hipLaunchKernelGGL(k_position_run<false>, dim3(1), dim3(kRunBlock), 0, s, b, p, sb, kv0, nk);
 }
 
 }  // namespace nph

@@ -191,22 +191,29 @@ __global__ void __launch_bounds__(256) k_slot_info(DevBodies b, DevContacts c, i
 // inv_r and rhs accumulation with J.v then J.ext.
 struct BodyView {
   f3 com, v, w, el, ea;
+  f3 ml, ma;  // jacobian mask
   m33 inv;
   float im;
 };
 
+// The jacobian [dir; p x dir] is masked; the velocity term uses the unmasked force, so kinematic
+// dofs stay controllable at the velocity level, and the external term the masked one
+// (rigid_body.rs:685-718).
 __device__ __forceinline__ void fill_geom(const BodyView& bv, f3 point, f3 dir, f3& ang, f3& wang,
                                           float& inv_r, float& rhs) {
   const f3 p = sub(point, bv.com);
-  ang = cross(p, dir);
+  const f3 ang_u = cross(p, dir);
+  const f3 lin = cmul(dir, bv.ml);
+  ang = cmul(ang_u, bv.ma);
   wang = mulv(bv.inv, ang);
   inv_r = inv_r + (bv.im + dot(ang, wang));
-  const float J[6] = {dir.x, dir.y, dir.z, ang.x, ang.y, ang.z};
+  const float F[6] = {dir.x, dir.y, dir.z, ang_u.x, ang_u.y, ang_u.z};
+  const float J[6] = {lin.x, lin.y, lin.z, ang.x, ang.y, ang.z};
   const float V[6] = {bv.v.x, bv.v.y, bv.v.z, bv.w.x, bv.w.y, bv.w.z};
   const float E[6] = {bv.el.x, bv.el.y, bv.el.z, bv.ea.x, bv.ea.y, bv.ea.z};
   float s = 0.0f;
 #pragma unroll
-  for (int k = 0; k < 6; ++k) s = s + J[k] * V[k];
+  for (int k = 0; k < 6; ++k) s = s + F[k] * V[k];
   rhs = rhs + s;
   float e = 0.0f;
 #pragma unroll
@@ -222,6 +229,7 @@ __device__ __forceinline__ BodyView load_body(const DevBodies& b, int i) {
   v.el = xyz(b.ext_lin[i]);
   v.ea = xyz(b.ext_ang[i]);
   v.inv = load_inv(b, i, v.im);
+  b.mask(i, v.ml, v.ma);
   return v;
 }
 
@@ -233,8 +241,25 @@ __device__ __forceinline__ void tangents(f3 n, f3& t1, f3& t2) {
   t2 = a;
 }
 
+// Material::combine (material.rs:134-178): the table's coefficient when both materials have an
+// id and the table holds the pair, else MaterialCombineMode::combine of the two.
+__device__ __forceinline__ void material_coefficients(const DevBodies& b, const MatTable& mt, int b1, int b2,
+                                                      float& mu, float& e) {
+  const int f1 = b.flags ? b.flags[b1] : 0, f2 = b.flags ? b.flags[b2] : 0;
+  mu = combine_coeff(b.friction[b1], fric_mode(f1), b.friction[b2], fric_mode(f2));
+  e = combine_coeff(b.restitution[b1], rest_mode(f1), b.restitution[b2], rest_mode(f2));
+  if (!b.mat_id) return;
+  const int i1 = b.mat_id[b1], i2 = b.mat_id[b2];
+  if (i1 < 0 || i2 < 0) return;
+  const unsigned long long lo = unsigned(i1 < i2 ? i1 : i2), hi = unsigned(i1 < i2 ? i2 : i1);
+  const unsigned long long key = (lo << 32) | hi;
+  float v;
+  if (mat_lookup(mt.fkey, mt.fval, mt.nf, key, v)) mu = v;
+  if (mat_lookup(mt.rkey, mt.rval, mt.nr, key, v)) e = v;
+}
+
 __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, DevContacts prev, int prevM,
-                                                 int M, SimParams p, SolverBufs sb) {
+                                                 int M, SimParams p, SolverBufs sb, MatTable mt) {
   // manifold order (neighbouring lanes read neighbouring bodies and contacts); the records go to
   // the manifold's slot in colour order
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -244,8 +269,11 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
   const int b1 = bb.x, b2 = bb.y;
   const bool d1 = b.status[b1] == STATUS_DYNAMIC;
   const bool d2 = b.status[b2] == STATUS_DYNAMIC;
-  const float mu = (b.friction[b1] + b.friction[b2]) * 0.5f;
-  const float e = (b.restitution[b1] + b.restitution[b2]) * 0.5f;
+  float mu, e;
+  material_coefficients(b, mt, b1, b2, mu, e);
+  // surface_velocity = position1 * v1 - position2 * v2 (material.rs:175, basic_material.rs:47-56)
+  f3 sv{0.0f, 0.0f, 0.0f};
+  if (b.surf_vel) sv = sub(qrot(toq(b.rot(b1)), xyz(b.surf_vel[b1])), qrot(toq(b.rot(b2)), xyz(b.surf_vel[b2])));
   const float m1 = b.margin[b1], m2 = b.margin[b2];
   BodyView v1, v2;
   if (d1) v1 = load_body(b, b1);
@@ -298,7 +326,8 @@ __global__ void __launch_bounds__(256) k_assemble(DevBodies b, DevContacts cur, 
 #pragma unroll
     for (int row = 0; row < 3; ++row) {
       const f3 dir = row == 0 ? neg(n) : (row == 1 ? t1 : t2);
-      float inv_r = 0.0f, rhs = 0.0f;
+      // n . surface_velocity for the normal row, t . surface_velocity for the friction rows
+      float inv_r = 0.0f, rhs = b.surf_vel ? dot(row == 0 ? n : dir, sv) : 0.0f;
       f3 a1{0.0f, 0.0f, 0.0f}, wa1{0.0f, 0.0f, 0.0f}, a2{0.0f, 0.0f, 0.0f}, wa2{0.0f, 0.0f, 0.0f};
       if (d1) fill_geom(v1, center1, dir, a1, wa1, inv_r, rhs);
       if (d2) fill_geom(v2, center2, neg(dir), a2, wa2, inv_r, rhs);
@@ -410,22 +439,48 @@ struct Inv {
   float im;
 };
 
+// Jacobian masks of a manifold's two bodies (only the masked kernel variants carry them).
+template <bool kMask>
+struct Masks {
+  __device__ void load(const DevBodies&, const SlotCtx&) {}
+  __device__ f3 l1(f3 d) const { return d; }
+  __device__ f3 l2(f3 d) const { return d; }
+  __device__ f3 a1(f3 a) const { return a; }
+  __device__ f3 a2(f3 a) const { return a; }
+};
+template <>
+struct Masks<true> {
+  f3 ml1, ma1, ml2, ma2;
+  __device__ void load(const DevBodies& b, const SlotCtx& x) {
+    b.mask(x.b1, ml1, ma1);
+    b.mask(x.b2, ml2, ma2);
+  }
+  __device__ f3 l1(f3 d) const { return cmul(d, ml1); }
+  __device__ f3 l2(f3 d) const { return cmul(d, ml2); }
+  __device__ f3 a1(f3 a) const { return cmul(a, ma1); }
+  __device__ f3 a2(f3 a) const { return cmul(a, ma2); }
+};
+
+template <bool kMask>
 struct Manifold {
   SlotCtx x;
   Inv i1, i2;
   CRec c[4];
   f3 t1[4], t2[4];
+  Masks<kMask> mk;
 };
 
 // Everything a lane needs for its manifold, loaded up front: the loads are independent of the
 // Gauss-Seidel chain, so they are all in flight together.
-__device__ __forceinline__ void load_manifold(const DevBodies& b, const SolverBufs& sb, int s, Manifold& m) {
+template <bool kMask>
+__device__ __forceinline__ void load_manifold(const DevBodies& b, const SolverBufs& sb, int s, Manifold<kMask>& m) {
   m.x = slot_ctx(sb, s);
   m.i1.im = m.i2.im = 0.0f;
   m.i1.m = m33{};
   m.i2.m = m33{};
   if (m.x.d1) m.i1.m = load_inv(b, m.x.b1, m.i1.im);
   if (m.x.d2) m.i2.m = load_inv(b, m.x.b2, m.i2.im);
+  m.mk.load(b, m.x);
 #pragma unroll
   for (int k = 0; k < 4; ++k)
     if (k < m.x.cnt) {
@@ -438,18 +493,22 @@ __device__ __forceinline__ void load_manifold(const DevBodies& b, const SolverBu
 // bodies' terms are always evaluated, a static body having zero inverse mass and inertia and a
 // zero dv: its terms are then exact zeros, so w and every dv component equal those of the
 // one-body expression (up to the sign of a zero), and a wavefront mixing ground and body-body
-// manifolds runs one code path.
+// manifolds runs one code path. The masked variant multiplies each body's jacobian by its mask,
+// as k_assemble's fill_geom does.
 struct Row {
-  f3 d, a1, a2, wa1, wa2;
+  f3 d1, d2, a1, a2, wa1, wa2;
   float r, rhs;
 };
 
-__device__ __forceinline__ Row row_of(const Manifold& m, int k, int row) {
+template <bool kMask>
+__device__ __forceinline__ Row row_of(const Manifold<kMask>& m, int k, int row) {
   Row v;
   const CRec& c = m.c[k];
-  v.d = row == 0 ? neg(c.n) : (row == 1 ? m.t1[k] : m.t2[k]);
-  v.a1 = ang1(c, v.d);
-  v.a2 = ang2(c, v.d);
+  const f3 d = row == 0 ? neg(c.n) : (row == 1 ? m.t1[k] : m.t2[k]);
+  v.d1 = m.mk.l1(d);
+  v.d2 = m.mk.l2(neg(d));
+  v.a1 = m.mk.a1(ang1(c, d));
+  v.a2 = m.mk.a2(ang2(c, d));
   v.r = c.r[row];
   v.rhs = c.rhs[row];
   v.wa1 = mulv(m.i1.m, v.a1);
@@ -457,21 +516,24 @@ __device__ __forceinline__ Row row_of(const Manifold& m, int k, int row) {
   return v;
 }
 
-__device__ __forceinline__ float row_w(const Row& v, const Manifold& m, const Dv& x1, const Dv& x2) {
-  return (dot6(v.d, v.a1, x1) + dot6(neg(v.d), v.a2, x2)) + v.rhs;
+__device__ __forceinline__ float row_w(const Row& v, const Dv& x1, const Dv& x2) {
+  return (dot6(v.d1, v.a1, x1) + dot6(v.d2, v.a2, x2)) + v.rhs;
 }
 
-__device__ __forceinline__ void row_add(const Row& v, float alpha, const Manifold& m, Dv& x1, Dv& x2) {
-  axpy6(alpha, scale(v.d, m.i1.im), v.wa1, x1);
-  axpy6(alpha, scale(neg(v.d), m.i2.im), v.wa2, x2);
+template <bool kMask>
+__device__ __forceinline__ void row_add(const Row& v, float alpha, const Manifold<kMask>& m, Dv& x1, Dv& x2) {
+  axpy6(alpha, scale(v.d1, m.i1.im), v.wa1, x1);
+  axpy6(alpha, scale(v.d2, m.i2.im), v.wa2, x2);
 }
 
-__device__ __forceinline__ void load_dv(const DevBodies& b, const Manifold& m, Dv& v1, Dv& v2) {
+template <bool kMask>
+__device__ __forceinline__ void load_dv(const DevBodies& b, const Manifold<kMask>& m, Dv& v1, Dv& v2) {
   if (m.x.d1) b.load_dv(m.x.b1, v1.l, v1.a);
   if (m.x.d2) b.load_dv(m.x.b2, v2.l, v2.a);
 }
 
-__device__ __forceinline__ void store_dv(const DevBodies& b, const Manifold& m, const Dv& v1, const Dv& v2) {
+template <bool kMask>
+__device__ __forceinline__ void store_dv(const DevBodies& b, const Manifold<kMask>& m, const Dv& v1, const Dv& v2) {
   if (m.x.d1) b.store_dv(m.x.b1, v1.l, v1.a);
   if (m.x.d2) b.store_dv(m.x.b2, v2.l, v2.a);
 }
@@ -487,8 +549,10 @@ __device__ __forceinline__ void colour_range(const SolverBufs& sb, int k, int& f
 // Solver velocity increments of the manifold's bodies in HBM (the per-colour kernels) ...
 struct GlobalDv {
   DevBodies b;
-  __device__ void load(const Manifold& m, Dv& v1, Dv& v2) const { load_dv(b, m, v1, v2); }
-  __device__ void store(const Manifold& m, const Dv& v1, const Dv& v2) const { store_dv(b, m, v1, v2); }
+  template <bool kMask>
+  __device__ void load(const Manifold<kMask>& m, Dv& v1, Dv& v2) const { load_dv(b, m, v1, v2); }
+  template <bool kMask>
+  __device__ void store(const Manifold<kMask>& m, const Dv& v1, const Dv& v2) const { store_dv(b, m, v1, v2); }
 };
 
 // ... or in LDS, indexed by the body's rank among its scene's dynamic bodies (the scene-blocked
@@ -497,7 +561,8 @@ struct LdsDv {
   float4* lin;
   float4* ang;
   const int* local;
-  __device__ void load(const Manifold& m, Dv& v1, Dv& v2) const {
+  template <bool kMask>
+  __device__ void load(const Manifold<kMask>& m, Dv& v1, Dv& v2) const {
     if (m.x.d1) {
       const int i = local[m.x.b1];
       v1.l = xyz(lin[i]);
@@ -509,7 +574,8 @@ struct LdsDv {
       v2.a = xyz(ang[i]);
     }
   }
-  __device__ void store(const Manifold& m, const Dv& v1, const Dv& v2) const {
+  template <bool kMask>
+  __device__ void store(const Manifold<kMask>& m, const Dv& v1, const Dv& v2) const {
     if (m.x.d1) {
       const int i = local[m.x.b1];
       lin[i] = f4(v1.l, 0.0f);
@@ -525,9 +591,9 @@ struct LdsDv {
 
 // Warm start of one manifold (SORProx::warmstart_set, sor_prox.rs:19-45): dv += l W J for every
 // row with l != 0, normal rows first, then friction rows.
-template <class A>
+template <bool kMask, class A>
 __device__ __forceinline__ void warm_manifold(const A& acc, const DevBodies& b, const SolverBufs& sb, int slot) {
-  Manifold m;
+  Manifold<kMask> m;
   load_manifold(b, sb, slot, m);
   Dv v1{}, v2{};
   acc.load(m, v1, v2);
@@ -548,9 +614,9 @@ __device__ __forceinline__ void warm_manifold(const A& acc, const DevBodies& b, 
 // One SOR-Prox step of one manifold (sor_prox.rs:159-343): its friction rows, bounded by mu times
 // the current normal impulse and zeroed when that is zero, then its normal rows, projected on
 // l >= 0.
-template <class A>
+template <bool kMask, class A>
 __device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, const SolverBufs& sb, int slot) {
-  Manifold m;
+  Manifold<kMask> m;
   load_manifold(b, sb, slot, m);
   Dv v1{}, v2{};
   acc.load(m, v1, v2);
@@ -571,7 +637,7 @@ __device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, c
       } else {
         const float mx = mu * ln;
         const float mn = -mx;
-        const float w = row_w(v, m, v1, v2);
+        const float w = row_w(v, v1, v2);
         const float val = li - v.r * w;
         const float ni = val > mn ? (val < mx ? val : mx) : mn;
         const float dl = ni - li;
@@ -586,7 +652,7 @@ __device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, c
   for (int k = 0; k < 4; ++k) {
     if (k >= m.x.cnt) break;
     const Row v = row_of(m, k, 0);
-    const float w = row_w(v, m, v1, v2);
+    const float w = row_w(v, v1, v2);
     const float a = m.c[k].lam.x - v.r * w;
     const float ni = a > 0.0f ? a : 0.0f;
     const float dl = ni - m.c[k].lam.x;
@@ -600,34 +666,36 @@ __device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, c
 }
 
 // One colour of the warm start / of one sweep: one lane per manifold of the colour visited kv-th.
+template <bool kMask>
 __global__ void __launch_bounds__(64) k_warmstart(DevBodies b, DevContacts c, SolverBufs sb, int kv) {
   int first, n;
   colour_range(sb, kv, first, n);
   const GlobalDv acc{b};
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-    warm_manifold(acc, b, sb, first + t);
+    warm_manifold<kMask>(acc, b, sb, first + t);
 }
 
+template <bool kMask>
 __global__ void __launch_bounds__(64) k_pgs(DevBodies b, DevContacts c, SolverBufs sb, int kv) {
   int first, n;
   colour_range(sb, kv, first, n);
   const GlobalDv acc{b};
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-    pgs_manifold(acc, b, sb, first + t);
+    pgs_manifold<kMask>(acc, b, sb, first + t);
 }
 
 // A run of consecutive small colour classes of the visiting order in one workgroup, a barrier
 // between classes: per body the same updates in the same order as one launch per class, without
 // a launch per class. Each class fits one pass of the workgroup.
-template <bool kWarm>
+template <bool kWarm, bool kMask>
 __global__ void __launch_bounds__(kRunBlock) k_colour_run(DevBodies b, SolverBufs sb, int kv0, int nk) {
   const GlobalDv acc{b};
   for (int kv = kv0; kv < kv0 + nk; ++kv) {
     int first, n;
     colour_range(sb, kv, first, n);
     for (int t = threadIdx.x; t < n; t += kRunBlock) {
-      if (kWarm) warm_manifold(acc, b, sb, first + t);
-      else pgs_manifold(acc, b, sb, first + t);
+      if (kWarm) warm_manifold<kMask>(acc, b, sb, first + t);
+      else pgs_manifold<kMask>(acc, b, sb, first + t);
     }
     __syncthreads();
   }
@@ -638,7 +706,7 @@ __global__ void __launch_bounds__(kRunBlock) k_colour_run(DevBodies b, SolverBuf
 // increments resident in LDS and a workgroup barrier between colours. Every body sees the same
 // sequence of row updates as under the per-colour launches (manifolds of one colour share no
 // dynamic body; colours and sweeps keep their order), so the results are identical.
-template <int kBlock, int kCap>
+template <int kBlock, int kCap, bool kMask>
 __global__ void __launch_bounds__(kBlock) k_sweeps_blocked(DevBodies b, SolverBufs sb, SceneBlocks sc,
                                                          const int* __restrict__ scene_list, int ncolours,
                                                          int nsweeps, int warm) {
@@ -662,8 +730,8 @@ __global__ void __launch_bounds__(kBlock) k_sweeps_blocked(DevBodies b, SolverBu
       const int e = int(st[col + 1]);
       for (int i = int(st[col]) + threadIdx.x; i < e; i += kBlock) {
         const int slot = int(sc.slot[i]);
-        if (pass < 0) warm_manifold(acc, b, sb, slot);
-        else pgs_manifold(acc, b, sb, slot);
+        if (pass < 0) warm_manifold<kMask>(acc, b, sb, slot);
+        else pgs_manifold<kMask>(acc, b, sb, slot);
       }
       __syncthreads();
     }
@@ -717,13 +785,14 @@ __global__ void k_fix_colour_starts(int ncolours, int M, int C, uint32_t* col_st
 
 // Natural-map residual per row; per block (max |phi|, sum phi^2) in double, reduced on the host
 // in block order.
+template <bool kMask>
 __global__ void __launch_bounds__(256) k_residual(DevBodies b, DevContacts c, int M, SolverBufs sb) {
   __shared__ double s_max[256];
   __shared__ double s_sum[256];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   double mx = 0.0, sum = 0.0;
   if (s < M) {
-    Manifold m;
+    Manifold<kMask> m;
     load_manifold(b, sb, s, m);
     Dv v1{}, v2{};
     load_dv(b, m, v1, v2);
@@ -733,7 +802,7 @@ __global__ void __launch_bounds__(256) k_residual(DevBodies b, DevContacts c, in
       const float4 lam = m.c[k].lam;
       {
         const Row v = row_of(m, k, 0);
-        const float w = row_w(v, m, v1, v2);
+        const float w = row_w(v, v1, v2);
         const float a = lam.x - v.r * w;
         const float pr = a > 0.0f ? a : 0.0f;
         const double phi = fabs(double((lam.x - pr) / v.r));
@@ -744,7 +813,7 @@ __global__ void __launch_bounds__(256) k_residual(DevBodies b, DevContacts c, in
       for (int f = 1; f <= 2; ++f) {
         const float li = f == 1 ? lam.y : lam.z;
         const Row v = row_of(m, k, f);
-        const float w = row_w(v, m, v1, v2);
+        const float w = row_w(v, v1, v2);
         const float mxv = m.c[k].mu * lam.x, mnv = -mxv;
         const float val = li - v.r * w;
         const float pr = val > mnv ? (val < mxv ? val : mxv) : mnv;
@@ -915,9 +984,9 @@ void launch_colour_order(const DevBodies& b_view_for_slots, const DevContacts& c
 }
 
 void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContacts& prev, int prevM,
-                     int M, const SimParams& p, SolverBufs& sb, hipStream_t s) {
+                     int M, const SimParams& p, SolverBufs& sb, const MatTable& mt, hipStream_t s) {
   if (M == 0) return;
-  hipLaunchKernelGGL(k_assemble, dim3(nb(M)), dim3(256), 0, s, b, cur, prev, prevM, M, p, sb);
+  hipLaunchKernelGGL(k_assemble, dim3(nb(M)), dim3(256), 0, s, b, cur, prev, prevM, M, p, sb, mt);
 }
 
 void launch_fix_colour_starts(int ncolours, int M, int C, SolverBufs& sb, uint32_t* class_rec, hipStream_t s) {
@@ -935,30 +1004,43 @@ int sweep_blocks(int max_class) {
   return blocks;
 }
 
+// The masked kernel variants run only in worlds where some dynamic body has a kinematic dof.
 void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
                              hipStream_t s) {
-  hipLaunchKernelGGL(k_warmstart, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
+  if (b.kmask) hipLaunchKernelGGL(k_warmstart<true>, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
+  else hipLaunchKernelGGL(k_warmstart<false>, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
 }
 
 void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
                        hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
+  if (b.kmask) {
+    if (ev0) hipExtLaunchKernelGGL(k_pgs<true>, dim3(blocks), dim3(64), 0, s, ev0, ev1, 0, b, c, sb, k);
+    else hipLaunchKernelGGL(k_pgs<true>, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
+    return;
+  }
   if (ev0) {
     // the dispatch packet itself records both events, timing the kernel alone
-    hipExtLaunchKernelGGL(k_pgs, dim3(blocks), dim3(64), 0, s, ev0, ev1, 0, b, c, sb, k);
+    hipExtLaunchKernelGGL(k_pgs<false>, dim3(blocks), dim3(64), 0, s, ev0, ev1, 0, b, c, sb, k);
   } else {
-    hipLaunchKernelGGL(k_pgs, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
+    hipLaunchKernelGGL(k_pgs<false>, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
   }
 }
 
 void launch_colour_run(const DevBodies& b, SolverBufs& sb, int kv0, int nk, bool warm, hipStream_t s) {
-  if (warm) hipLaunchKernelGGL(k_colour_run<true>, dim3(1), dim3(kRunBlock), 0, s, b, sb, kv0, nk);
-  else hipLaunchKernelGGL(k_colour_run<false>, dim3(1), dim3(kRunBlock), 0, s, b, sb, kv0, nk);
+  if (b.kmask) {
+    if (warm) hipLaunchKernelGGL((k_colour_run<true, true>), dim3(1), dim3(kRunBlock), 0, s, b, sb, kv0, nk);
+    else hipLaunchKernelGGL((k_colour_run<false, true>), dim3(1), dim3(kRunBlock), 0, s, b, sb, kv0, nk);
+    return;
+  }
+  if (warm) hipLaunchKernelGGL((k_colour_run<true, false>), dim3(1), dim3(kRunBlock), 0, s, b, sb, kv0, nk);
+  else hipLaunchKernelGGL((k_colour_run<false, false>), dim3(1), dim3(kRunBlock), 0, s, b, sb, kv0, nk);
 }
 
 int residual_blocks(int M) { return int(nb(M > 0 ? M : 1)); }
 
 void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, hipStream_t s) {
-  hipLaunchKernelGGL(k_residual, dim3(nb(M > 0 ? M : 1)), dim3(256), 0, s, b, c, M, sb);
+  if (b.kmask) hipLaunchKernelGGL(k_residual<true>, dim3(nb(M > 0 ? M : 1)), dim3(256), 0, s, b, c, M, sb);
+  else hipLaunchKernelGGL(k_residual<false>, dim3(nb(M > 0 ? M : 1)), dim3(256), 0, s, b, c, M, sb);
 }
 
 void launch_cache_impulses(const DevContacts& c, SolverBufs& sb, int C, hipStream_t s) {
@@ -998,22 +1080,29 @@ void launch_scene_order(const DevBodies& b, int M, int nscenes, int ncolours, So
 
 // Two launches: scenes with at most kSmallScene dynamic bodies (64-lane workgroups, 8 KB of LDS,
 // many resident per CU), then the larger ones (`block` lanes, 64 KB of LDS).
-void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int ncolours, int nsweeps,
-                           bool warm, int block, hipStream_t s) {
+template <bool kMask>
+void sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int ncolours, int nsweeps, bool warm,
+                    int block, hipStream_t s) {
   const int w = warm ? 1 : 0;
   if (sc.n_small > 0)
-    hipLaunchKernelGGL((k_sweeps_blocked<64, kSmallScene>), dim3(sc.n_small), dim3(64), 0, s, b, sb, sc,
+    hipLaunchKernelGGL((k_sweeps_blocked<64, kSmallScene, kMask>), dim3(sc.n_small), dim3(64), 0, s, b, sb, sc,
                        sc.small_list, ncolours, nsweeps, w);
   if (sc.n_large == 0) return;
   if (block == 64)
-    hipLaunchKernelGGL((k_sweeps_blocked<64, kLdsBodies>), dim3(sc.n_large), dim3(64), 0, s, b, sb, sc,
+    hipLaunchKernelGGL((k_sweeps_blocked<64, kLdsBodies, kMask>), dim3(sc.n_large), dim3(64), 0, s, b, sb, sc,
                        sc.large_list, ncolours, nsweeps, w);
   else if (block == 128)
-    hipLaunchKernelGGL((k_sweeps_blocked<128, kLdsBodies>), dim3(sc.n_large), dim3(128), 0, s, b, sb, sc,
+    hipLaunchKernelGGL((k_sweeps_blocked<128, kLdsBodies, kMask>), dim3(sc.n_large), dim3(128), 0, s, b, sb, sc,
                        sc.large_list, ncolours, nsweeps, w);
   else
-    hipLaunchKernelGGL((k_sweeps_blocked<256, kLdsBodies>), dim3(sc.n_large), dim3(256), 0, s, b, sb, sc,
+    hipLaunchKernelGGL((k_sweeps_blocked<256, kLdsBodies, kMask>), dim3(sc.n_large), dim3(256), 0, s, b, sb, sc,
                        sc.large_list, ncolours, nsweeps, w);
+}
+
+void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int ncolours, int nsweeps,
+                           bool warm, int block, hipStream_t s) {
+  if (b.kmask) sweeps_blocked<true>(b, sb, sc, ncolours, nsweeps, warm, block, s);
+  else sweeps_blocked<false>(b, sb, sc, ncolours, nsweeps, warm, block, s);
 }
 
 }  // namespace nph

@@ -9,6 +9,10 @@ namespace nph {
 // k_bodies.hip
 void launch_prestep(const DevBodies& b, const SimParams& p, hipStream_t s);
 void launch_integrate(const DevBodies& b, const SimParams& p, hipStream_t s);
+void launch_clear_dyn_marks(const DevBodies& b, hipStream_t s);
+void launch_apply_force(const DevBodies& b, const SimParams& p, int nu, const int* ubody, const int* off,
+                        const int* order, const float* forces, const float* points, int type, int wake,
+                        const float4* rot_dyn, hipStream_t s);
 
 // k_activation.hip ------------------------------------------------------------------------
 struct ActivationBufs {
@@ -124,7 +128,7 @@ void launch_colour_order(const DevBodies& b, const DevContacts& c, int M, int nc
 void launch_fix_colour_starts(int ncolours, int M, int C, SolverBufs& sb, uint32_t* class_rec, hipStream_t s);
 int residual_blocks(int M);
 void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContacts& prev,
-                     int prevM, int M, const SimParams& p, SolverBufs& sb, hipStream_t s);
+                     int prevM, int M, const SimParams& p, SolverBufs& sb, const MatTable& mt, hipStream_t s);
 // Sweep kernels of the colour visited k-th; their slot ranges are read on the device, so the
 // launches are independent of per-step sizes (grid = blocks from the largest colour class,
 // grid-stride over the colour).

@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <initializer_list>
+#include <limits>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -107,6 +108,19 @@ struct World {
   DBuf pos0, rot0, linvel, angvel, ext_lin, ext_ang;
   DBuf loose_mn, loose_mx;
   DBuf act;  // float2 per body: activation energy (0: asleep), sleep threshold (< 0: never sleeps)
+  // Optional per-body attributes (damping and clamps, flags, material ids, surface velocities);
+  // each is passed to the kernels only when some body differs from the default.
+  std::vector<float4> h_damp, h_surf;
+  std::vector<int> h_flags, h_matid;
+  bool any_damp = false, any_flags = false, any_mask = false, any_matid = false, any_surf = false;
+  DBuf damp, bflags, matid, surf;
+  // External forces accumulated for the next step (cleared after it), the angular velocity of the
+  // last dynamics update, and the rotations from before a set_state of the current interval.
+  DBuf f_lin, f_ang, w_dyn, rot_dyn, af_tmp;
+  bool forces_pending = false, rot_dyn_saved = false;
+  // material coefficient table (sorted keys min id << 32 | max id), friction and restitution
+  std::vector<std::pair<unsigned long long, float>> mt_host[2];
+  DBuf mt_key[2], mt_val[2];
   // Sleeping: on once a dynamic body with a threshold exists; a wake of every sleeping body is
   // due at the next step after the caller set the bodies' state.
   bool sleeping = false, wake_pending = false;
@@ -275,7 +289,26 @@ struct World {
     b.loose_mn = loose_mn.as<float4>();
     b.loose_mx = loose_mx.as<float4>();
     b.act = sleeping ? act.as<float2>() : nullptr;
+    b.damp = any_damp ? damp.as<float4>() : nullptr;
+    b.flags = any_flags ? bflags.as<int>() : nullptr;
+    b.kmask = any_mask ? bflags.as<int>() : nullptr;
+    b.mat_id = any_matid ? matid.as<int>() : nullptr;
+    b.surf_vel = any_surf ? surf.as<float4>() : nullptr;
+    b.f_lin = forces_pending ? f_lin.as<float4>() : nullptr;
+    b.f_ang = forces_pending ? f_ang.as<float4>() : nullptr;
+    b.w_dyn = w_dyn.as<float4>();
     return b;
+  }
+
+  MatTable mat_table() const {
+    MatTable t;
+    t.fkey = mt_key[0].as<unsigned long long>();
+    t.fval = mt_val[0].as<float>();
+    t.nf = int(mt_host[0].size());
+    t.rkey = mt_key[1].as<unsigned long long>();
+    t.rval = mt_val[1].as<float>();
+    t.nr = int(mt_host[1].size());
+    return t;
   }
 
   // ------------------------------------------------------------------ scene
@@ -302,6 +335,9 @@ struct World {
       const float* dm = d.dims + 3 * k;
       if (sh == NPHYS_SHAPE_BALL ? !(dm[0] > 0.0f) : !(dm[0] > 0.0f && dm[1] > 0.0f && dm[2] > 0.0f))
         return set_err(NPHYS_ERR_INVALID, "shape dimensions must be positive");
+      if ((d.friction_combine && (d.friction_combine[k] < 0 || d.friction_combine[k] > 3)) ||
+          (d.restitution_combine && (d.restitution_combine[k] < 0 || d.restitution_combine[k] > 3)))
+        return set_err(NPHYS_ERR_INVALID, "bad combine mode");
     }
     for (int k = 0; k < d.n; ++k) {
       const int st = d.status[k], sh = d.shape[k];
@@ -340,6 +376,30 @@ struct World {
       h_qlim.push_back(d.margin[k] + d.linear_prediction[k]);
       h_friction.push_back(d.friction[k]);
       h_restitution.push_back(d.restitution[k]);
+      // RigidBodyDesc / BasicMaterial attributes (rigid_body.rs:972-990, basic_material.rs:33-43)
+      const float fmax = 3.402823466e38f;
+      const float4 dp = make_float4(d.linear_damping ? d.linear_damping[k] : 0.0f,
+                                    d.angular_damping ? d.angular_damping[k] : 0.0f,
+                                    d.max_linear_velocity ? d.max_linear_velocity[k] : fmax,
+                                    d.max_angular_velocity ? d.max_angular_velocity[k] : fmax);
+      h_damp.push_back(dp);
+      any_damp = any_damp || dp.x != 0.0f || dp.y != 0.0f || dp.z != fmax || dp.w != fmax;
+      const int kin = d.kinematic_dofs ? (d.kinematic_dofs[k] & 63) : 0;
+      const int fl = kin | ((d.gravity_enabled && !d.gravity_enabled[k]) ? kFlagNoGravity : 0) |
+                     ((d.friction_combine ? d.friction_combine[k] : 0) << 8) |
+                     ((d.restitution_combine ? d.restitution_combine[k] : 0) << 10);
+      h_flags.push_back(fl);
+      any_flags = any_flags || fl != 0;
+      any_mask = any_mask || (st == NPHYS_STATUS_DYNAMIC && kin != 0);
+      const int mid = d.material_id ? (d.material_id[k] < 0 ? -1 : d.material_id[k]) : -1;
+      h_matid.push_back(mid);
+      any_matid = any_matid || mid >= 0;
+      const float4 sv = d.surface_velocity
+                            ? make_float4(d.surface_velocity[3 * k], d.surface_velocity[3 * k + 1],
+                                          d.surface_velocity[3 * k + 2], 0.0f)
+                            : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      h_surf.push_back(sv);
+      any_surf = any_surf || sv.x != 0.0f || sv.y != 0.0f || sv.z != 0.0f;
       const float* p = d.pos + 3 * k;
       const float* q = d.rot + 4 * k;
       hp[k] = make_float4(p[0], p[1], p[2], 0.0f);
@@ -368,9 +428,10 @@ struct World {
     // kept: a sleeping body's inverse inertia is not recomputed
     HIPCK(sol.ensure(size_t(n1) * kSolverBlock * f4b, true, stream));
     HIPCK(act.ensure(size_t(n1) * sizeof(float2), true, stream));
-    for (DBuf* b : {&pos0, &rot0, &ext_lin, &ext_ang, &dims, &local_i})
+    for (DBuf* b : {&w_dyn, &f_lin, &f_ang}) HIPCK(b->ensure(size_t(n1) * f4b, true, stream));
+    for (DBuf* b : {&pos0, &rot0, &ext_lin, &ext_ang, &dims, &local_i, &damp, &surf, &rot_dyn})
       HIPCK(b->ensure(size_t(n1) * f4b));
-    for (DBuf* b : {&status, &shape, &scene, &margin, &qlim, &friction, &restitution, &mass})
+    for (DBuf* b : {&status, &shape, &scene, &margin, &qlim, &friction, &restitution, &mass, &bflags, &matid})
       HIPCK(b->ensure(size_t(n1) * 4));
     // scene bits above the Morton code: the BVH keeps every scene in its own key range
     int max_scene = 0;
@@ -390,6 +451,15 @@ struct World {
     HIPCK(up(qlim, h_qlim.data(), size_t(n1) * 4, 0));
     HIPCK(up(friction, h_friction.data(), size_t(n1) * 4, 0));
     HIPCK(up(restitution, h_restitution.data(), size_t(n1) * 4, 0));
+    HIPCK(up(damp, h_damp.data(), size_t(n1) * f4b, 0));
+    HIPCK(up(surf, h_surf.data(), size_t(n1) * f4b, 0));
+    HIPCK(up(bflags, h_flags.data(), size_t(n1) * 4, 0));
+    HIPCK(up(matid, h_matid.data(), size_t(n1) * 4, 0));
+    // new bodies: no dynamics update yet (the augmented mass is the world inertia, w = 0) and no
+    // pending force
+    HIPCK(hipMemsetAsync(w_dyn.as<float4>() + n0, 0, size_t(d.n) * f4b, stream));
+    HIPCK(hipMemsetAsync(f_lin.as<float4>() + n0, 0, size_t(d.n) * f4b, stream));
+    HIPCK(hipMemsetAsync(f_ang.as<float4>() + n0, 0, size_t(d.n) * f4b, stream));
     HIPCK(hipMemsetAsync(sol.as<float4>() + size_t(n0) * kSolverBlock, 0, size_t(d.n) * kSolverBlock * f4b, stream));
     HIPCK(hipMemcpyAsync(act.as<float2>() + n0, ha.data(), size_t(d.n) * sizeof(float2), hipMemcpyHostToDevice,
                          stream));
@@ -464,6 +534,13 @@ struct World {
 
   int set_state(const float* p, const float* r, const float* lv, const float* av) {
     if (n == 0) return NPHYS_OK;
+    if (r && !rot_dyn_saved) {
+      // impulses applied before the next step use the augmented mass of the rotation the last
+      // dynamics update saw (the reference recomputes it only in the next step)
+      HIPCK(hipMemcpy2DAsync(rot_dyn.p, sizeof(float4), pose.as<float4>() + 1, kPoseBlock * sizeof(float4),
+                             sizeof(float4), size_t(n), hipMemcpyDeviceToDevice, stream));
+      rot_dyn_saved = true;
+    }
     std::vector<float4> hp(n), hr(n), hv(n), hw(n);
     const size_t f4b = sizeof(float4);
     HIPCK(pose_copy(0, hp.data(), 0, n, hipMemcpyDeviceToHost));
@@ -483,6 +560,7 @@ struct World {
     HIPCK(pose_copy(1, hr.data(), 0, n, hipMemcpyHostToDevice));
     HIPCK(hipMemcpyAsync(linvel.p, hv.data(), n * f4b, hipMemcpyHostToDevice, stream));
     HIPCK(hipMemcpyAsync(angvel.p, hw.data(), n * f4b, hipMemcpyHostToDevice, stream));
+    if (sleeping) launch_clear_dyn_marks(bodies(), stream);
     HIPCK(hipStreamSynchronize(stream));
     moved = true;
     wake_pending = sleeping;
@@ -909,8 +987,8 @@ struct World {
   }
 
   // Warm start (when `warm`) and `nsweeps` coloured SOR-Prox sweeps. With counters enabled the
-  // sweep kernels are launched directly with HIP events in their dispatch packets; otherwise
-  // the phase runs as a cached hipGraph.
+  // per-colour sweep kernels carry HIP events in their dispatch packets. Every launch is direct:
+  // one per colour class, or one workgroup for a run of consecutive small classes.
   int sweeps(const DevBodies& b, const DevContacts& cv, SolverBufs& v, int sblocks, bool warm, int nsweeps) {
     if (cur_variant > 0) {
       // scene-blocked: one launch for the warm start and all nsweeps sweeps
@@ -1200,7 +1278,7 @@ struct World {
     if (rc) return rc;
     SolverBufs v = sv_view();
     if ((rc = colour(b, cv, M, C, v))) return rc;
-    launch_assemble(b, cv, pc.view(), prevM, M, sp, v, stream);
+    launch_assemble(b, cv, pc.view(), prevM, M, sp, v, mat_table(), stream);
     HIPCK(hipEventRecord(ev[EV_ASM], stream));
     // class sizes (read back while the assembly runs) give the visit order and the sweep grids
     if ((rc = visit_order())) return rc;
@@ -1364,6 +1442,86 @@ struct World {
     }
     kt_used = 0;
     prm.t += prm.dt;
+    // clear_forces (mechanical_world.rs:365): the next accumulation starts from zero
+    if (forces_pending) {
+      fill_u32(f_lin.as<uint32_t>(), 0u, size_t(n) * 4, stream);
+      fill_u32(f_ang.as<uint32_t>(), 0u, size_t(n) * 4, stream);
+      forces_pending = false;
+    }
+    rot_dyn_saved = false;
+    return NPHYS_OK;
+  }
+
+  // Body::apply_force / apply_force_at_point for `count` entries: grouped by body on the host (a
+  // stable sort keeps each body's entries in the caller's order), one thread per body.
+  int apply_force(int32_t count, const int32_t* bodies_, const float* forces, const float* points, int32_t type,
+                  int32_t wake) {
+    if (count < 0 || type < 0 || type > 3) return set_err(NPHYS_ERR_INVALID, "bad force arguments");
+    if (count == 0) return NPHYS_OK;
+    if (!bodies_ || !forces) return set_err(NPHYS_ERR_INVALID, "null force arrays");
+    for (int k = 0; k < count; ++k)
+      if (bodies_[k] < 0 || bodies_[k] >= n) return set_err(NPHYS_ERR_INVALID, "bad body handle");
+    std::vector<int> order(static_cast<size_t>(count));
+    for (int k = 0; k < count; ++k) order[size_t(k)] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return bodies_[a] < bodies_[c]; });
+    std::vector<int> ub, off;
+    for (int k = 0; k < count; ++k) {
+      const int bd = bodies_[order[size_t(k)]];
+      if (ub.empty() || ub.back() != bd) {
+        ub.push_back(bd);
+        off.push_back(k);
+      }
+    }
+    off.push_back(count);
+    const int nu = int(ub.size());
+    const int fw = points ? 3 : 6;
+    const size_t words = size_t(nu) + size_t(nu + 1) + size_t(count) + size_t(count) * fw + (points ? size_t(count) * 3 : 0);
+    HIPCK(af_tmp.ensure(words * 4));
+    char* base = af_tmp.as<char>();
+    int* d_ub = reinterpret_cast<int*>(base);
+    int* d_off = d_ub + nu;
+    int* d_order = d_off + nu + 1;
+    float* d_f = reinterpret_cast<float*>(d_order + count);
+    float* d_p = points ? d_f + size_t(count) * fw : nullptr;
+    HIPCK(hipMemcpyAsync(d_ub, ub.data(), size_t(nu) * 4, hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(d_off, off.data(), size_t(nu + 1) * 4, hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(d_order, order.data(), size_t(count) * 4, hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(d_f, forces, size_t(count) * fw * 4, hipMemcpyHostToDevice, stream));
+    if (points) HIPCK(hipMemcpyAsync(d_p, points, size_t(count) * 12, hipMemcpyHostToDevice, stream));
+    if (type == NPHYS_FORCE || type == NPHYS_ACCELERATION_CHANGE) forces_pending = true;
+    launch_apply_force(bodies(), sp, nu, d_ub, d_off, d_order, d_f, d_p, type, wake, rot_dyn_saved ? rot_dyn.as<float4>() : nullptr,
+                       stream);
+    HIPCK(hipStreamSynchronize(stream));  // the staging buffer is reused by the next call
+    return NPHYS_OK;
+  }
+
+  // MaterialsCoefficientsTable::set_friction_coefficient / set_restitution_coefficient.
+  int set_material_coefficient(int32_t which, int32_t id1, int32_t id2, float coeff) {
+    if ((which != 0 && which != 1) || id1 < 0 || id2 < 0) return set_err(NPHYS_ERR_INVALID, "bad material pair");
+    const unsigned long long lo = unsigned(std::min(id1, id2)), hi = unsigned(std::max(id1, id2));
+    const unsigned long long key = (lo << 32) | hi;
+    auto& t = mt_host[which];
+    auto it = std::lower_bound(t.begin(), t.end(), std::make_pair(key, -std::numeric_limits<float>::infinity()),
+                               [](const std::pair<unsigned long long, float>& a,
+                                  const std::pair<unsigned long long, float>& c) { return a.first < c.first; });
+    if (it != t.end() && it->first == key) {
+      it->second = coeff;
+    } else {
+      if (t.size() >= NPHYS_MAX_MATERIAL_PAIRS) return set_err(NPHYS_ERR_CAPACITY, "material table full");
+      t.insert(it, std::make_pair(key, coeff));
+    }
+    std::vector<unsigned long long> keys(t.size());
+    std::vector<float> vals(t.size());
+    for (size_t i = 0; i < t.size(); ++i) {
+      keys[i] = t[i].first;
+      vals[i] = t[i].second;
+    }
+    HIPCK(hipStreamSynchronize(stream));  // the previous table may be read by enqueued work
+    HIPCK(mt_key[which].ensure(keys.size() * 8));
+    HIPCK(mt_val[which].ensure(vals.size() * 4));
+    HIPCK(hipMemcpyAsync(mt_key[which].p, keys.data(), keys.size() * 8, hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(mt_val[which].p, vals.data(), vals.size() * 4, hipMemcpyHostToDevice, stream));
+    HIPCK(hipStreamSynchronize(stream));
     return NPHYS_OK;
   }
 };
@@ -1465,6 +1623,24 @@ int nphys_set_state(nphys_world* w, const float* pos, const float* rot, const fl
 int nphys_get_state(nphys_world* w, float* pos, float* rot, float* linvel, float* angvel) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
   return w->w.get_state(pos, rot, linvel, angvel);
+}
+
+int nphys_apply_force(nphys_world* w, int32_t count, const int32_t* bodies, const float* forces, int32_t type,
+                      int32_t auto_wake_up) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  return w->w.apply_force(count, bodies, forces, nullptr, type, auto_wake_up);
+}
+
+int nphys_apply_force_at_point(nphys_world* w, int32_t count, const int32_t* bodies, const float* forces,
+                               const float* points, int32_t type, int32_t auto_wake_up) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  if (count > 0 && !points) return set_err(NPHYS_ERR_INVALID, "null points");
+  return w->w.apply_force(count, bodies, forces, points, type, auto_wake_up);
+}
+
+int nphys_set_material_coefficient(nphys_world* w, int32_t which, int32_t id1, int32_t id2, float coeff) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  return w->w.set_material_coefficient(which, id1, id2, coeff);
 }
 
 int nphys_step(nphys_world* w, int32_t nsteps) {

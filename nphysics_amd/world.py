@@ -30,7 +30,8 @@ EXPORTED = [
     "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
     "nphys_mlp_create", "nphys_mlp_destroy", "nphys_mlp_num_params", "nphys_mlp_get_params",
     "nphys_mlp_set_params", "nphys_mlp_set_data", "nphys_mlp_train", "nphys_mlp_forward",
-    "nphys_mlp_gradients", "nphys_last_error",
+    "nphys_mlp_gradients", "nphys_last_error", "nphys_apply_force", "nphys_apply_force_at_point",
+    "nphys_set_material_coefficient",
 ]
 
 
@@ -84,6 +85,9 @@ def load_library(path=LIB_PATH):
     L.nphys_set_incremental_pairs.argtypes = [W, C.c_int32]
     L.nphys_read_colours.argtypes = [W, ip, C.c_int64, ip]
     L.nphys_read_activation.argtypes = [W, fp, C.c_int64]
+    L.nphys_apply_force.argtypes = [W, C.c_int32, ip, fp, C.c_int32, C.c_int32]
+    L.nphys_apply_force_at_point.argtypes = [W, C.c_int32, ip, fp, fp, C.c_int32, C.c_int32]
+    L.nphys_set_material_coefficient.argtypes = [W, C.c_int32, C.c_int32, C.c_int32, C.c_float]
     _lib = L
     return L
 
@@ -170,6 +174,24 @@ class MechanicalWorld:
         av = np.zeros((n, 3), np.float32)
         _check(self._L.nphys_get_state(self._h, abi.fptr(pos), abi.fptr(rot), abi.fptr(lv), abi.fptr(av)))
         return {"pos": pos, "rot": rot, "linvel": lv, "angvel": av}
+
+    def apply_force(self, bodies, forces, force_type=abi.FORCE, auto_wake_up=True, points=None):
+        """Body::apply_force per entry (forces (k, 6): linear, torque; world frame), or, with
+        `points`, apply_force_at_point (forces (k, 3) linear at the world points (k, 3)).
+        force_type: abi.FORCE, IMPULSE, ACCELERATION_CHANGE or VELOCITY_CHANGE."""
+        b = np.ascontiguousarray(bodies, np.int32).reshape(-1)
+        f = np.ascontiguousarray(forces, np.float32).reshape(-1)
+        if points is None:
+            _check(self._L.nphys_apply_force(self._h, b.shape[0], abi.iptr(b), abi.fptr(f), force_type,
+                                             int(auto_wake_up)))
+        else:
+            p = np.ascontiguousarray(points, np.float32).reshape(-1)
+            _check(self._L.nphys_apply_force_at_point(self._h, b.shape[0], abi.iptr(b), abi.fptr(f), abi.fptr(p),
+                                                      force_type, int(auto_wake_up)))
+
+    def set_material_coefficient(self, which, id1, id2, coeff):
+        """MaterialsCoefficientsTable: which 0 friction, 1 restitution, for material ids id1, id2."""
+        _check(self._L.nphys_set_material_coefficient(self._h, which, id1, id2, coeff))
 
     # -- stepping
     def step(self, nsteps=1):

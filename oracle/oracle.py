@@ -72,6 +72,8 @@ def lib():
         L.orc_collide_shapes.argtypes = [C.c_int32, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double,
                                          C.POINTER(C.c_uint32), dp, dp, dp, dp, ip]
         L.orc_colour_manifolds.argtypes = [C.c_int32, ip, ip, C.c_int32, ip, ip]
+        L.orc_apply_force.argtypes = [vp, C.c_int32, ip, fp, fp, C.c_int32, C.c_int32]
+        L.orc_set_material_coefficient.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_float]
         L.orc_mlp_forward.argtypes = [dp, fp, C.c_int64, C.c_int32, dp]
         L.orc_mlp_gradients.argtypes = [dp, fp, fp, C.c_int64, C.c_int32, dp, dp]
         _lib = L
@@ -144,6 +146,18 @@ class OracleWorld:
 
     def step(self, n=1):
         _check(self._L.orc_step(self._h, n))
+
+    def apply_force(self, bodies, forces, force_type=0, auto_wake_up=True, points=None):
+        """Body::apply_force per entry (forces (k, 6): linear, torque), or apply_force_at_point
+        with `points` (forces (k, 3) linear at world points (k, 3))."""
+        b = np.ascontiguousarray(bodies, np.int32).reshape(-1)
+        f = np.ascontiguousarray(forces, np.float32).reshape(-1)
+        p = None if points is None else np.ascontiguousarray(points, np.float32).reshape(-1)
+        _check(self._L.orc_apply_force(self._h, b.shape[0], abi.iptr(b), abi.fptr(f),
+                                       None if p is None else abi.fptr(p), force_type, int(auto_wake_up)))
+
+    def set_material_coefficient(self, which, id1, id2, coeff):
+        _check(self._L.orc_set_material_coefficient(self._h, which, id1, id2, coeff))
 
     def collide(self):
         _check(self._L.orc_collide(self._h))

@@ -83,11 +83,24 @@ int add_bodies_t(orc::World<T>& w, const nphys_body_desc* d, int32_t* first) {
     const orc::V3<T> zero{T(0), T(0), T(0)};
     const orc::V3<T> lv = d->linvel ? v3_of<T>(d->linvel + 3 * i) : zero;
     const orc::V3<T> av = d->angvel ? v3_of<T>(d->angvel + 3 * i) : zero;
+    orc::BodyExtras<T> ex;
+    if (d->linear_damping) ex.lin_damping = T(d->linear_damping[i]);
+    if (d->angular_damping) ex.ang_damping = T(d->angular_damping[i]);
+    if (d->max_linear_velocity) ex.max_linvel = T(d->max_linear_velocity[i]);
+    if (d->max_angular_velocity) ex.max_angvel = T(d->max_angular_velocity[i]);
+    if (d->gravity_enabled) ex.gravity = d->gravity_enabled[i] != 0;
+    if (d->kinematic_dofs) ex.kinematic_dofs = d->kinematic_dofs[i] & 63;
+    if (d->friction_combine) ex.fric_mode = d->friction_combine[i];
+    if (d->restitution_combine) ex.rest_mode = d->restitution_combine[i];
+    if (d->material_id) ex.mat_id = d->material_id[i] < 0 ? -1 : d->material_id[i];
+    if (d->surface_velocity) ex.surf_vel = v3_of<T>(d->surface_velocity + 3 * i);
+    if (ex.fric_mode < 0 || ex.fric_mode > 3 || ex.rest_mode < 0 || ex.rest_mode > 3)
+      return fail(NPHYS_ERR_INVALID, "bad combine mode");
     w.add_body(d->status[i], d->shape[i], v3_of<T>(d->dims + 3 * i),
                orc::Iso<T>{v3_of<T>(d->pos + 3 * i), q_of<T>(d->rot + 4 * i)}, lv, av,
                T(d->density[i]), T(d->margin[i]), T(d->linear_prediction[i]), T(d->friction[i]),
                T(d->restitution[i]), d->scene ? int(d->scene[i]) : 0,
-               d->sleep_threshold ? T(d->sleep_threshold[i]) : T(-1));
+               d->sleep_threshold ? T(d->sleep_threshold[i]) : T(-1), ex);
   }
   return NPHYS_OK;
 }
@@ -274,6 +287,35 @@ int orc_set_state_f64(void* h, const double* pos, const double* rot, const doubl
 
 int orc_get_state(void* h, double* pos, double* rot, double* lv, double* av) {
   return with_world(h, [&](auto& w) { return get_state_t(w, pos, rot, lv, av); });
+}
+
+int orc_apply_force(void* h, int32_t count, const int32_t* bodies, const float* forces, const float* points,
+                    int32_t type, int32_t wake) {
+  if (count < 0 || (count > 0 && (!bodies || !forces)) || type < 0 || type > 3)
+    return fail(NPHYS_ERR_INVALID, "bad force arguments");
+  return with_world(h, [&](auto& w) {
+    using T = typename std::remove_reference<decltype(w)>::type::Scalar;
+    for (int k = 0; k < count; ++k)
+      if (bodies[k] < 0 || bodies[k] >= int(w.bodies.size())) return fail(NPHYS_ERR_INVALID, "bad body handle");
+    for (int k = 0; k < count; ++k) {
+      if (points) {
+        w.apply_force_at_point(bodies[k], v3_of<T>(forces + 3 * k), v3_of<T>(points + 3 * k), type, wake != 0);
+      } else {
+        w.apply_force(bodies[k], v3_of<T>(forces + 6 * k), v3_of<T>(forces + 6 * k + 3), type, wake != 0);
+      }
+    }
+    return NPHYS_OK;
+  });
+}
+
+int orc_set_material_coefficient(void* h, int32_t which, int32_t id1, int32_t id2, float coeff) {
+  if ((which != 0 && which != 1) || id1 < 0 || id2 < 0) return fail(NPHYS_ERR_INVALID, "bad material pair");
+  return with_world(h, [&](auto& w) {
+    using T = typename std::remove_reference<decltype(w)>::type::Scalar;
+    const std::pair<uint32_t, uint32_t> key{uint32_t(std::min(id1, id2)), uint32_t(std::max(id1, id2))};
+    (which == 0 ? w.fric_table : w.rest_table)[key] = T(coeff);
+    return NPHYS_OK;
+  });
 }
 
 int orc_step(void* h, int32_t nsteps) {

@@ -30,6 +30,8 @@
 #include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <limits>
+#include <map>
 #include <unordered_map>
 #include <vector>
 
@@ -80,6 +82,14 @@ struct Body {
   V3<T> com;
   bool gravity;
   T lin_damping, ang_damping, max_linvel, max_angvel;
+  // jacobian_mask (rigid_body.rs:46,78,103-118): 1 for a free dof, 0 for a kinematic one
+  V3<T> mask_lin, mask_ang;
+  // external_forces (rigid_body.rs:771-822), cleared at the end of every step (mechanical_world.rs:365)
+  V3<T> f_lin, f_ang;
+  // BasicMaterial (basic_material.rs:14-63): combine modes, lookup id (-1: None), surface velocity
+  // in the collider's frame
+  int fric_mode, rest_mode, mat_id;
+  V3<T> surf_vel;
   int companion;
   unsigned flags;
   AABB<T> loose;
@@ -232,9 +242,34 @@ inline int colour_manifolds(const std::vector<Manifold>& ms, const std::vector<i
   return ncol;
 }
 
+enum CombineMode : int { COMBINE_AVERAGE = 0, COMBINE_MIN = 1, COMBINE_MULTIPLY = 2, COMBINE_MAX = 3 };
+
+// MaterialCombineMode::combine (material.rs:72-86): the mode of higher precedence decides
+// (Max > Multiply > Min > Average).
+template <class T>
+inline T combine_coeff(T a, int ma, T b, int mb) {
+  if (ma == COMBINE_MAX || mb == COMBINE_MAX) return tmax(a, b);
+  if (ma == COMBINE_MULTIPLY || mb == COMBINE_MULTIPLY) return a * b;
+  if (ma == COMBINE_MIN || mb == COMBINE_MIN) return tmin(a, b);
+  return (a + b) * T(0.5);
+}
+
+// Per-body attributes beyond shape, pose and BasicMaterial coefficients, with the
+// RigidBodyDesc / BasicMaterial defaults (rigid_body.rs:972-990, basic_material.rs:33-43).
+template <class T>
+struct BodyExtras {
+  T lin_damping = T(0), ang_damping = T(0);
+  T max_linvel = std::numeric_limits<T>::max(), max_angvel = std::numeric_limits<T>::max();
+  bool gravity = true;
+  int kinematic_dofs = 0;  // bit k: dof k kinematic (0-2 translations, 3-5 rotations)
+  int fric_mode = COMBINE_AVERAGE, rest_mode = COMBINE_AVERAGE, mat_id = -1;
+  V3<T> surf_vel{T(0), T(0), T(0)};
+};
+
 template <class T>
 class World {
  public:
+  using Scalar = T;
   Params<T> prm;
   V3<T> gravity;
   std::vector<Body<T>> bodies;
@@ -260,6 +295,9 @@ class World {
   T res_inf = T(0), res_rms = T(0);
   long n_rows = 0;
   Counters counters;
+  // MaterialsCoefficientsTable (materials_coefficients_table.rs:8-50), keyed by the sorted pair
+  std::map<std::pair<uint32_t, uint32_t>, T> fric_table, rest_table;
+  bool any_surface_velocity = false;  // some body's material has a surface velocity
 
   World() {
     prm.t = T(0);
@@ -284,7 +322,8 @@ class World {
   // (geometrical_world.rs:104-140): mass properties from the density, proxy creation with
   // loose AABB = aabb(pos) loosened by the query limit, then by the broad-phase margin 0.01.
   int add_body(int status, int shape, V3<T> dims, Iso<T> pos, V3<T> lv, V3<T> av, T density,
-               T margin, T lin_pred, T friction, T restitution, int scene = 0, T sleep_thr = T(-1)) {
+               T margin, T lin_pred, T friction, T restitution, int scene = 0, T sleep_thr = T(-1),
+               const BodyExtras<T>& ex = BodyExtras<T>()) {
     Body<T> b;
     std::memset(static_cast<void*>(&b), 0, sizeof(b));
     b.scene = scene;
@@ -306,11 +345,20 @@ class World {
     b.pos0 = pos;
     b.linvel = status == STATUS_DYNAMIC ? lv : V3<T>{T(0), T(0), T(0)};
     b.angvel = status == STATUS_DYNAMIC ? av : V3<T>{T(0), T(0), T(0)};
-    b.gravity = true;
-    b.lin_damping = T(0);
-    b.ang_damping = T(0);
-    b.max_linvel = std::numeric_limits<T>::max();
-    b.max_angvel = std::numeric_limits<T>::max();
+    b.gravity = ex.gravity;
+    b.lin_damping = ex.lin_damping;
+    b.ang_damping = ex.ang_damping;
+    b.max_linvel = ex.max_linvel;
+    b.max_angvel = ex.max_angvel;
+    for (int k = 0; k < 3; ++k) {
+      set_comp(b.mask_lin, k, (ex.kinematic_dofs >> k) & 1 ? T(0) : T(1));
+      set_comp(b.mask_ang, k, (ex.kinematic_dofs >> (3 + k)) & 1 ? T(0) : T(1));
+    }
+    b.fric_mode = ex.fric_mode;
+    b.rest_mode = ex.rest_mode;
+    b.mat_id = ex.mat_id;
+    b.surf_vel = ex.surf_vel;
+    if (ex.surf_vel.x != T(0) || ex.surf_vel.y != T(0) || ex.surf_vel.z != T(0)) any_surface_velocity = true;
     b.flags = F_ALL;
     b.com = pos.t;
     b.mass = T(0);
@@ -336,6 +384,15 @@ class World {
       }
       b.mass = mass;
       b.local_I = ai;
+    }
+    // RigidBody::new / update_inertia_from_local_inertia (rigid_body.rs:56-81,262-267): the
+    // augmented mass starts as the world inertia, without gyroscopic terms
+    b.inv_mass = b.mass == T(0) ? T(0) : T(1) / b.mass;
+    if (status == STATUS_DYNAMIC) {
+      const M3<T> R = qmat(pos.q);
+      b.I_world = mul(mul(R, b.local_I), transpose(R));
+      b.aug = b.I_world;
+      try_inverse(b.aug, b.inv_aug);
     }
     const AABB<T> tight = loosened(shape_aabb(b.sd, pos), b.query_limit());
     b.loose = loosened(tight, T(0.01));
@@ -379,6 +436,41 @@ class World {
     const V3<T> gyro = -cross(b.angvel, iw);
     b.acc_ang = mulv(b.inv_aug, gyro);
     if (b.inv_mass != T(0) && b.gravity) b.acc_lin = gravity;
+    // acceleration += inv_augmented_mass * external_forces, then (.) jacobian_mask
+    b.acc_lin = b.acc_lin + b.f_lin * b.inv_mass;
+    b.acc_ang = b.acc_ang + mulv(b.inv_aug, b.f_ang);
+    b.acc_lin = cmul(b.acc_lin, b.mask_lin);
+    b.acc_ang = cmul(b.acc_ang, b.mask_ang);
+  }
+  static V3<T> cmul(V3<T> a, V3<T> m) { return V3<T>{a.x * m.x, a.y * m.y, a.z * m.z}; }
+
+  // Body::apply_force (rigid_body.rs:771-822) for a world-frame force (linear, torque), with
+  // the inverse augmented mass / augmented mass of the last update_dynamics. Types: 0 Force,
+  // 1 Impulse, 2 AccelerationChange, 3 VelocityChange (ForceType, algebra/mod.rs:17-26).
+  void apply_force(int i, V3<T> fl, V3<T> fa, int type, bool wake) {
+    Body<T>& b = bodies[size_t(i)];
+    if (b.status != STATUS_DYNAMIC) return;
+    if (wake && b.sleep_thr >= T(0)) b.energy = b.sleep_thr * T(2.0);  // Body::activate (body.rs:338-342)
+    if (type == 0) {
+      b.f_lin = cmul(fl, b.mask_lin) + b.f_lin;
+      b.f_ang = cmul(fa, b.mask_ang) + b.f_ang;
+    } else if (type == 1) {
+      b.linvel = cmul(fl * b.inv_mass, b.mask_lin) + b.linvel;
+      b.angvel = cmul(mulv(b.inv_aug, fa), b.mask_ang) + b.angvel;
+      b.flags |= F_VELOCITY;
+    } else if (type == 2) {
+      b.f_lin = cmul(fl * b.mass, b.mask_lin) + b.f_lin;
+      b.f_ang = cmul(mulv(b.aug, fa), b.mask_ang) + b.f_ang;
+    } else {
+      b.linvel = cmul(fl, b.mask_lin) + b.linvel;
+      b.angvel = cmul(fa, b.mask_ang) + b.angvel;
+      b.flags |= F_VELOCITY;
+    }
+  }
+  // apply_force_at_point (rigid_body.rs:835-845): Force::linear_at_point(f, point - com)
+  void apply_force_at_point(int i, V3<T> f, V3<T> point, int type, bool wake) {
+    const V3<T> p = point - bodies[size_t(i)].com;
+    apply_force(i, f, cross(p, f), type, wake);
   }
 
   // Isometry composition of apply_displacement (rigid_body.rs:371-381):
@@ -549,18 +641,23 @@ class World {
                  const T* ext, T* out_vel) {
     if (b.status != STATUS_DYNAMIC) return;
     const V3<T> p = point - b.com;
-    const V3<T> ang = cross(p, dir);
+    const V3<T> ang_u = cross(p, dir);
+    // the jacobian is masked; the velocity term uses the unmasked force so that kinematic dofs stay
+    // controllable at the velocity level (rigid_body.rs:685-718)
+    const V3<T> lin = cmul(dir, b.mask_lin);
+    const V3<T> ang = cmul(ang_u, b.mask_ang);
     T* J = &jacobians[j_id];
-    J[0] = dir.x; J[1] = dir.y; J[2] = dir.z; J[3] = ang.x; J[4] = ang.y; J[5] = ang.z;
-    const V3<T> wl = dir * b.inv_mass;
+    J[0] = lin.x; J[1] = lin.y; J[2] = lin.z; J[3] = ang.x; J[4] = ang.y; J[5] = ang.z;
+    const V3<T> wl = lin * b.inv_mass;
     const V3<T> wa = mulv(b.inv_aug, ang);
     T* W = &jacobians[wj_id];
     W[0] = wl.x; W[1] = wl.y; W[2] = wl.z; W[3] = wa.x; W[4] = wa.y; W[5] = wa.z;
     inv_r = inv_r + (b.inv_mass + dot(ang, wa));
     if (out_vel) {
       const T v6[6] = {b.linvel.x, b.linvel.y, b.linvel.z, b.angvel.x, b.angvel.y, b.angvel.z};
+      const T F[6] = {dir.x, dir.y, dir.z, ang_u.x, ang_u.y, ang_u.z};
       T s = T(0);
-      for (int k = 0; k < 6; ++k) s = s + J[k] * v6[k];
+      for (int k = 0; k < 6; ++k) s = s + F[k] * v6[k];
       *out_vel = *out_vel + s;
       T e = T(0);
       for (int k = 0; k < 6; ++k) e = e + J[k] * ext[k];
@@ -659,9 +756,13 @@ class World {
     for (const auto& m : ms) {
       const Body<T>& b1 = bodies[m.b1];
       const Body<T>& b2 = bodies[m.b2];
-      // Material::combine, Average mode (material.rs:72-86, 134-178)
-      const T mu = (b1.friction + b2.friction) * T(0.5);
-      const T e = (b1.restitution + b2.restitution) * T(0.5);
+      // Material::combine (material.rs:134-178): the lookup table when both materials have an id
+      // and the table has the pair, else MaterialCombineMode::combine
+      T mu, e;
+      material_coefficients(b1, b2, mu, e);
+      // surface_velocity = position1 * v1 - position2 * v2 (material.rs:175, basic_material.rs:47-56)
+      V3<T> sv{T(0), T(0), T(0)};
+      if (any_surface_velocity) sv = qrot(b1.pos.q, b1.surf_vel) - qrot(b2.pos.q, b2.surf_vel);
       for (int k = 0; k < m.count; ++k) {
         const Contact<T>& c = cs[size_t(m.first + k)];
         const uint64_t key = contact_key(m.b1, m.b2, c.feature);
@@ -671,7 +772,7 @@ class World {
         // SignoriniModel::build_velocity_constraint (signorini_model.rs:37-138)
         const V3<T> center1 = c.world1 + c.normal * b1.margin;
         const V3<T> center2 = c.world2 - c.normal * b2.margin;
-        T rhs = T(0);  // n . surface_velocity (zero for BasicMaterial)
+        T rhs = any_surface_velocity ? dot(c.normal, sv) : T(0);  // n . surface_velocity
         const Geom g = pair_geometry(b1, b2, center1, center2, -c.normal, ground_j_id, j_id, &rhs);
         if (rhs <= -prm.restitution_velocity_threshold) rhs = rhs + e * rhs;
         const T depth = (c.depth + b1.margin) + b2.margin;
@@ -694,7 +795,7 @@ class World {
         V3<T> t[2];
         tangents(c.normal, t[0], t[1]);
         for (int f = 0; f < 2; ++f) {
-          T frhs = T(0);
+          T frhs = any_surface_velocity ? dot(t[f], sv) : T(0);  // t . surface_velocity
           const Geom fg = pair_geometry(b1, b2, center1, center2, t[f], ground_j_id, j_id, &frhs);
           VelRow<T> fr = make_row(fg, b1, b2, frhs, imp_comp(imp, f + 1) * prm.warmstart_coeff, key);
           fr.dep = cr.normal;
@@ -709,6 +810,18 @@ class World {
     n_rows = long(uni.size() + uni_g.size() + bil.size() + bil_g.size());
   }
   static T imp_comp(V3<T> v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+  void material_coefficients(const Body<T>& b1, const Body<T>& b2, T& mu, T& e) const {
+    mu = combine_coeff(b1.friction, b1.fric_mode, b2.friction, b2.fric_mode);
+    e = combine_coeff(b1.restitution, b1.rest_mode, b2.restitution, b2.rest_mode);
+    if (b1.mat_id < 0 || b2.mat_id < 0) return;
+    const uint32_t i1 = uint32_t(b1.mat_id), i2 = uint32_t(b2.mat_id);
+    const std::pair<uint32_t, uint32_t> key{std::min(i1, i2), std::max(i1, i2)};
+    const auto f = fric_table.find(key);
+    if (f != fric_table.end()) mu = f->second;
+    const auto r = rest_table.find(key);
+    if (r != rest_table.end()) e = r->second;
+  }
 
   // ------------------------------------------------------------------ SOR-Prox rows
   T dot6(size_t j, int a) const {
@@ -902,17 +1015,17 @@ class World {
     if (pc.rhs >= T(0)) return;
     T inv_r = T(0);
     V3<T> wl1{}, wa1{}, wl2{}, wa2{};
-    if (pc.ndofs1) {
+    if (pc.ndofs1) {  // fill_constraint_geometry with the masked jacobian (rigid_body.rs:672-705)
       const V3<T> dir = -g.normal;
-      const V3<T> ang = cross(w1 - b1.com, dir);
-      wl1 = dir * b1.inv_mass;
+      const V3<T> ang = cmul(cross(w1 - b1.com, dir), b1.mask_ang);
+      wl1 = cmul(dir, b1.mask_lin) * b1.inv_mass;
       wa1 = mulv(b1.inv_aug, ang);
       inv_r = inv_r + (b1.inv_mass + dot(ang, wa1));
     }
     if (pc.ndofs2) {
       const V3<T> dir = g.normal;
-      const V3<T> ang = cross(w2 - b2.com, dir);
-      wl2 = dir * b2.inv_mass;
+      const V3<T> ang = cmul(cross(w2 - b2.com, dir), b2.mask_ang);
+      wl2 = cmul(dir, b2.mask_lin) * b2.inv_mass;
       wa2 = mulv(b2.inv_aug, ang);
       inv_r = inv_r + (b2.inv_mass + dot(ang, wa2));
     }
@@ -1091,7 +1204,10 @@ class World {
     narrow_phase();
     counters.collision_detection += now_s() - tc;
     prm.t = prm.t + prm.dt;
-    for (auto& b : bodies) b.flags = 0;
+    for (auto& b : bodies) {
+      b.flags = 0;
+      b.f_lin = b.f_ang = V3<T>{T(0), T(0), T(0)};  // clear_forces (mechanical_world.rs:365)
+    }
     counters.step += now_s() - ts;
     counters.nsteps++;
   }

@@ -1,0 +1,468 @@
+"""Pins of the oracle functions that the contact KATs leave open: the gyroscopic terms of
+update_dynamics / update_acceleration, the position correction, the material combine modes and
+lookup table, the surface velocity, damping and velocity clamps, the gravity flag, the jacobian
+mask and external forces.
+
+Every expected value comes from mechanics or from the reference's stated formula evaluated in
+closed form, never from the oracle's own code path, and each test is chosen so that a plausible
+slip (a flipped sign, a dropped factor, another combine mode) fails it:
+
+  * gyroscopic terms (rigid_body.rs:572-580, 598-606): a torque-free body keeps its world-frame
+    angular momentum vector L = I_w w; with the gyroscopic torque's sign flipped, dL/dt = 2 w x L
+    turns L (|L| and the energy would still be conserved, so only the direction pins the sign);
+    without the term w stays constant and an intermediate-axis spin would not depart;
+  * position correction (nonlinear_sor_prox.rs:296-309, 137-152): an isolated ball overlapping
+    the ground is lifted per iteration by exactly max((depth + margins - 0.001) erp, -0.2), so
+    three iterations remove (1 - 0.8^3) of the correctable depth;
+  * combine modes (material.rs:72-86): the friction deceleration of a sliding ball is mu g and the
+    rebound speed of a frictionless ball is e |v|, with mu, e the combined coefficients of unequal
+    materials under each mode;
+  * damping and clamps (rigid_body.rs:467-505), gravity flag and jacobian mask
+    (rigid_body.rs:590-619, 672-722), forces (rigid_body.rs:771-845): free-body closed forms.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from nphysics_amd import abi, scenes
+
+
+def body(status=0, shape=0, dims=(0.5, 0.5, 0.5), pos=(0, 0, 0), rot=(0, 0, 0, 1), linvel=(0, 0, 0),
+         angvel=(0, 0, 0), density=1.0, friction=0.5, restitution=0.0, margin=0.01, **extra):
+    s = scenes._empty(1)
+    s["status"][0] = status
+    s["shape"][0] = shape
+    s["dims"][0] = dims
+    s["pos"][0] = pos
+    s["rot"][0] = rot
+    s["linvel"][0] = linvel
+    s["angvel"][0] = angvel
+    s["density"][0] = density
+    s["friction"][0] = friction
+    s["restitution"][0] = restitution
+    s["margin"][0] = margin
+    for k, v in extra.items():
+        dtype, width = abi.OPTIONAL_BODY_FIELDS[k]
+        s[k] = np.asarray(v, dtype).reshape((1, width) if width > 1 else (1,))
+    return s
+
+
+def merge(*parts):
+    """Concatenates single-body scenes; optional fields missing from a part take their default."""
+    keys = set()
+    for p in parts:
+        keys |= set(p)
+    defaults = {"scene": 0, "sleep_threshold": -1.0, "linear_damping": 0.0, "angular_damping": 0.0,
+                "max_linear_velocity": np.finfo(np.float32).max,
+                "max_angular_velocity": np.finfo(np.float32).max, "gravity_enabled": 1, "kinematic_dofs": 0,
+                "friction_combine": 0, "restitution_combine": 0, "material_id": -1,
+                "surface_velocity": (0.0, 0.0, 0.0)}
+    filled = []
+    for p in parts:
+        q = dict(p)
+        n = p["status"].shape[0]
+        for k in keys:
+            if k not in q:
+                dtype, width = abi.OPTIONAL_BODY_FIELDS[k]
+                q[k] = np.broadcast_to(np.asarray(defaults[k], dtype), (n, width) if width > 1 else (n,)).copy()
+        filled.append(q)
+    return scenes.concat(*filled)
+
+
+def ground(friction=0.5, restitution=0.0, **extra):
+    return body(status=1, shape=1, dims=(50, 0.5, 50), pos=(0, -0.5, 0), friction=friction,
+                restitution=restitution, **extra)
+
+
+def quat_mat(q):
+    x, y, z, w = q
+    return np.array([
+        [1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+        [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+        [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)],
+    ])
+
+
+# ------------------------------------------------------------------ gyroscopic terms
+
+HE = (0.2, 0.5, 0.9)  # principal moments: x largest (major), y intermediate, z smallest
+
+
+def cuboid_inertia(he, density=1.0):
+    m = density * 8 * he[0] * he[1] * he[2]
+    return m, np.diag([m * (4 * he[1] ** 2 + 4 * he[2] ** 2) / 12, m * (4 * he[0] ** 2 + 4 * he[2] ** 2) / 12,
+                       m * (4 * he[0] ** 2 + 4 * he[1] ** 2) / 12])
+
+
+def spin_run(oracle_mod, w0, steps, record_every=10):
+    o = oracle_mod.OracleWorld(abi.default_params(gravity=(0, 0, 0)), precision=64)
+    o.add_bodies(body(shape=1, dims=HE, angvel=w0))
+    _, Il = cuboid_inertia(HE)
+    out = []
+    for k in range(0, steps + 1, record_every):
+        if k:
+            o.step(record_every)
+        st = o.get_state()
+        R = quat_mat(st["rot"][0])
+        w = st["angvel"][0]
+        Iw = R @ Il @ R.T
+        out.append((Iw @ w, 0.5 * w @ Iw @ w, R.T @ w))
+    return out
+
+
+def test_gyroscopic_major_axis_keeps_world_angular_momentum(oracle_mod):
+    """Near-major-axis spin: L = I_w w keeps its world direction (an explicit or sign-flipped
+    gyroscopic torque turns it by radians within a second); |L| and the energy never grow (the
+    reference's linearised implicit step dissipates)."""
+    rec = spin_run(oracle_mod, (5.0, 0.3, 0.3), steps=600)
+    L0, E0, _ = rec[0]
+    for L, E, _ in rec[1:]:
+        cosang = L @ L0 / (np.linalg.norm(L) * np.linalg.norm(L0))
+        assert math.acos(min(1.0, cosang)) < 0.03  # sign-flipped terms: 0.23 rad
+        assert np.linalg.norm(L) <= np.linalg.norm(L0) * (1 + 1e-12)
+        assert E <= E0 * (1 + 1e-12)
+    # slightly dissipative, not collapsing
+    assert np.linalg.norm(rec[-1][0]) > 0.99 * np.linalg.norm(L0)
+
+
+def test_gyroscopic_intermediate_axis_departs(oracle_mod):
+    """Dzhanibekov effect: a spin about the intermediate axis with a 1e-3 perturbation departs and
+    flips its body-frame direction within 10 s; a missing gyroscopic term would keep w constant.
+    The major-axis spin holds its body-frame direction."""
+    inter = spin_run(oracle_mod, (1e-3, 5.0, 1e-3), steps=600)
+    wy = np.array([wb[1] for _, _, wb in inter])
+    assert wy[0] > 4.9 and wy.min() < -4.0
+    major = spin_run(oracle_mod, (5.0, 1e-3, 1e-3), steps=600)
+    wx = np.array([wb[0] for _, _, wb in major])
+    assert wx.min() > 4.99
+
+
+def test_gyroscopic_one_step_matches_implicit_euler(oracle_mod):
+    """One step of the linearised implicit scheme: (I + dt J) (w1 - w0) = -dt w0 x I w0 with
+    J = [w0]x I - [I w0]x, i.e. w1 agrees with the exact implicit Euler solution of
+    I (w1 - w0) = -dt w1 x I w1 to O(dt^2 |w|^3), and differs from w0 at first order."""
+    w0 = np.array([1.0, 4.0, 2.0])
+    dt = 1.0 / 60.0
+    o = oracle_mod.OracleWorld(abi.default_params(gravity=(0, 0, 0)), precision=64)
+    o.add_bodies(body(shape=1, dims=HE, angvel=w0))
+    o.step(1)
+    w1 = o.get_state()["angvel"][0]
+    _, I = cuboid_inertia(tuple(float(np.float32(h)) for h in HE))
+    # exact implicit Euler by fixed-point iteration
+    w = w0.copy()
+    for _ in range(200):
+        w = np.linalg.solve(I, I @ w0 - dt * np.cross(w, I @ w))
+    first_order = np.linalg.norm(np.linalg.solve(I, dt * np.cross(w0, I @ w0)))
+    assert np.linalg.norm(w1 - w0) > 0.5 * first_order
+    assert np.linalg.norm(w1 - w) < 0.05 * first_order
+
+
+# ------------------------------------------------------------------ position correction
+
+def pos_world(oracle_mod, depth, iters, erp=0.2, ball_r=0.5):
+    p = abi.default_params(gravity=(0, 0, 0), max_velocity_iterations=0, max_position_iterations=iters, erp=erp)
+    o = oracle_mod.OracleWorld(p, precision=64)
+    o.add_bodies(scenes.concat(ground(), body(dims=(ball_r, 0, 0), pos=(0, ball_r - depth, 0))))
+    return o
+
+
+@pytest.mark.parametrize("depth", [0.05, 0.3])
+def test_position_correction_single_iteration(oracle_mod, depth):
+    """One position iteration lifts the ball by max((depth + m1 + m2 - 0.001) erp, -0.2)
+    negated: the contact normal is the lever-arm direction, so the angular term vanishes and the
+    displacement is -rhs r / m = -rhs (nonlinear_sor_prox.rs:137-152, 296-309)."""
+    r = float(np.float32(0.5))
+    y0 = float(np.float32(r - depth))
+    o = pos_world(oracle_mod, depth, iters=1)
+    o.step(1)
+    y1 = o.get_state()["pos"][1, 1]
+    d_eff = (r - y0) + 2 * float(np.float32(0.01))  # both margins are FP32 descriptor values
+    expected = min((d_eff - 0.001) * 0.2, 0.2)
+    assert y1 - y0 == pytest.approx(expected, rel=1e-9)
+
+
+def test_position_correction_caps_at_max_linear_correction(oracle_mod):
+    o = pos_world(oracle_mod, 1.4, iters=1, ball_r=2.0)
+    y0 = float(np.float32(2.0 - 1.4))
+    o.step(1)
+    assert o.get_state()["pos"][1, 1] - y0 == pytest.approx(0.2, rel=1e-9)
+
+
+def test_position_correction_three_iterations_geometric(oracle_mod):
+    """Each iteration re-evaluates the contact and removes erp of the remaining correctable
+    depth: three iterations remove (1 - (1 - erp)^3) of it; nothing is done below 0.001."""
+    depth = 0.1
+    r = float(np.float32(0.5))
+    y0 = float(np.float32(r - depth))
+    o = pos_world(oracle_mod, depth, iters=3)
+    o.step(1)
+    d_eff = (r - y0) + 2 * float(np.float32(0.01))
+    assert o.get_state()["pos"][1, 1] - y0 == pytest.approx((d_eff - 0.001) * (1 - 0.8 ** 3), rel=1e-9)
+    # a contact whose dilated depth is below allowed_linear_error is never corrected
+    o2 = pos_world(oracle_mod, -0.0195, iters=3)  # depth + margins = 0.0005
+    y2 = float(np.float32(r + 0.0195))
+    o2.collide()
+    assert o2.contacts()["depth"].shape[0] == 1
+    o2.step(1)
+    assert o2.get_state()["pos"][1, 1] == y2
+
+
+def test_position_correction_with_rotation_moves_the_contact_point(oracle_mod):
+    """A tilted box touching the ground with one vertex: the constraint-space displacement of the
+    contact point along the normal is J M^-1 J^T lambda = inv_r (-rhs r) = -rhs to first order
+    (the rotation's second-order terms aside), whatever the box's inertia; dropping the angular
+    term of inv_r or of the displacement breaks this."""
+    he = (0.5, 0.3, 0.4)
+    ang = math.radians(30.0)
+    q = (0.0, 0.0, math.sin(ang / 2), math.cos(ang / 2))
+    R = quat_mat(q)
+    corners = np.array([[sx * he[0], sy * he[1], sz * he[2]] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+    low = (corners @ R.T)[:, 1].min()
+    depth = 0.04
+    p = abi.default_params(gravity=(0, 0, 0), max_velocity_iterations=0, max_position_iterations=1)
+    o = oracle_mod.OracleWorld(p, precision=64)
+    o.add_bodies(scenes.concat(ground(), body(shape=1, dims=he, pos=(0, -low - depth, 0), rot=q)))
+    o.collide()
+    d0 = o.contacts()["depth"].max()
+    o.step(1)
+    o.collide()
+    d1 = o.contacts()["depth"].max()
+    corr = min((d0 + 0.02 - 0.001) * 0.2, 0.2)
+    assert d0 == pytest.approx(depth, abs=2e-6)
+    assert d0 - d1 == pytest.approx(corr, rel=0.02)
+
+
+def test_position_correction_two_dynamic_bodies(oracle_mod):
+    """A ball (body 1) pressed onto the corner of a free tilted box (body 2), both dynamic: the
+    depth still drops by -rhs to first order, now shared between two bodies with an angular term
+    on the box's side (J M^-1 J^T of both bodies)."""
+    he = (0.5, 0.3, 0.4)
+    ang = math.radians(30.0)
+    q = (0.0, 0.0, math.sin(ang / 2), math.cos(ang / 2))
+    R = quat_mat(q)
+    corners = np.array([[sx * he[0], sy * he[1], sz * he[2]] for sx in (-1, 1) for sy in (-1, 1) for sz in (-1, 1)])
+    top = corners[np.argmax((corners @ R.T)[:, 1])] @ R.T
+    r, depth = 0.4, 0.05
+    p = abi.default_params(gravity=(0, 0, 0), max_velocity_iterations=0, max_position_iterations=1)
+    o = oracle_mod.OracleWorld(p, precision=64)
+    o.add_bodies(scenes.concat(body(dims=(r, 0, 0), pos=(top[0], top[1] + r - depth, top[2]), density=2.0),
+                               body(shape=1, dims=he, pos=(0, 0, 0), rot=q)))
+    o.collide()
+    d0 = o.contacts()["depth"].max()
+    o.step(1)
+    o.collide()
+    d1 = o.contacts()["depth"].max()
+    corr = min((d0 + 0.02 - 0.001) * 0.2, 0.2)
+    assert d0 == pytest.approx(depth, abs=2e-6)
+    assert d0 - d1 == pytest.approx(corr, rel=0.02)
+
+
+# ------------------------------------------------------------------ materials
+
+MODES = {abi.COMBINE_AVERAGE: 0.5, abi.COMBINE_MIN: 0.2, abi.COMBINE_MULTIPLY: 0.16, abi.COMBINE_MAX: 0.8}
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
+def test_combine_mode_friction_deceleration(oracle_mod, mode):
+    """A ball sliding on the ground decelerates at mu g while it slips (the friction rows sit on
+    their limit mu lambda_n, lambda_n = m g dt); mu is the combination of the ball's 0.2 and the
+    ground's 0.8 under the mode (the ball carries the mode; Average on the ground)."""
+    r, v0 = 0.5, 6.0
+    mu = MODES[mode]
+    ball = body(dims=(r, 0, 0), pos=(0, r, 0), linvel=(v0, 0, 0), friction=0.2, friction_combine=mode)
+    o = oracle_mod.OracleWorld(abi.default_params(max_velocity_iterations=30), precision=64)
+    o.add_bodies(merge(ground(friction=0.8), ball))
+    k = 8  # still slipping: rolling starts at t = 2 v0 / (7 mu g), 13 steps for mu = 0.8
+    o.step(k)
+    v2 = o.get_state()["linvel"][1, 0]
+    assert (v0 - v2) / (9.81 * k / 60.0) == pytest.approx(mu, rel=1e-5)
+
+
+@pytest.mark.parametrize("mode", sorted(MODES))
+def test_combine_mode_restitution(oracle_mod, mode):
+    """A frictionless ball hitting the ground at 5 m/s leaves at e 5 m/s, e the combination of
+    0.2 (ball) and 0.8 (ground) under the mode (signorini_model.rs:93-95)."""
+    r = 0.5
+    e = MODES[mode]
+    ball = body(dims=(r, 0, 0), pos=(0, r, 0), linvel=(0, -5.0, 0), friction=0.0, restitution=0.2,
+                restitution_combine=mode)
+    o = oracle_mod.OracleWorld(abi.default_params(gravity=(0, 0, 0), max_velocity_iterations=20,
+                                                  max_position_iterations=0), precision=64)
+    o.add_bodies(merge(ground(friction=0.0, restitution=0.8), ball))
+    o.step(1)
+    assert o.get_state()["linvel"][1, 1] == pytest.approx(5.0 * e, rel=1e-6)
+
+
+def test_material_lookup_table_overrides_modes(oracle_mod):
+    """MaterialsCoefficientsTable: a pair with an entry uses it (either id order); a pair without
+    one falls back to the combine mode; a material without an id never uses the table."""
+    r = 0.5
+
+    def rebound(ids, table, mode=abi.COMBINE_MAX):
+        ball = body(dims=(r, 0, 0), pos=(0, r, 0), linvel=(0, -5.0, 0), friction=0.0, restitution=0.2,
+                    restitution_combine=mode, material_id=ids[1])
+        o = oracle_mod.OracleWorld(abi.default_params(gravity=(0, 0, 0), max_velocity_iterations=20,
+                                                      max_position_iterations=0), precision=64)
+        o.add_bodies(merge(ground(friction=0.0, restitution=0.8, material_id=ids[0]), ball))
+        for (a, b, v) in table:
+            o.set_material_coefficient(1, a, b, v)
+        o.step(1)
+        return o.get_state()["linvel"][1, 1] / 5.0
+
+    assert rebound((3, 7), [(7, 3, 0.35)]) == pytest.approx(0.35, rel=1e-6)
+    assert rebound((3, 7), [(3, 8, 0.35)]) == pytest.approx(0.8, rel=1e-6)
+    assert rebound((3, -1), [(3, 7, 0.35)]) == pytest.approx(0.8, rel=1e-6)
+
+
+def test_surface_velocity_conveyor(oracle_mod):
+    """A box on a static belt whose material has surface velocity (2, 0, 0) in the ground's frame
+    is carried to the belt's speed; rotating the ground by 90 degrees about y carries it along
+    -z instead (the surface velocity is rotated by the collider position)."""
+    for q, expect in (((0, 0, 0, 1), (2.0, 0.0, 0.0)),
+                      ((0, math.sin(math.pi / 4), 0, math.cos(math.pi / 4)), (0.0, 0.0, -2.0))):
+        belt = body(status=1, shape=1, dims=(50, 0.5, 50), pos=(0, -0.5, 0), rot=q, friction=1.0,
+                    surface_velocity=(2.0, 0.0, 0.0))
+        box = body(shape=1, dims=(0.5, 0.5, 0.5), pos=(0, 0.5, 0), friction=1.0)
+        o = oracle_mod.OracleWorld(abi.default_params(max_velocity_iterations=20), precision=64)
+        o.add_bodies(merge(belt, box))
+        o.step(120)
+        np.testing.assert_allclose(o.get_state()["linvel"][1], expect, atol=2e-3)
+
+
+# ------------------------------------------------------------------ body attributes
+
+def test_damping_closed_form(oracle_mod):
+    """RigidBody::integrate: v <- v / (1 + dt d) each step, so v_k = v0 (1 + dt d)^-k."""
+    o = oracle_mod.OracleWorld(abi.default_params(gravity=(0, 0, 0)), precision=64)
+    o.add_bodies(body(linvel=(3.0, 0, 0), angvel=(0, 2.0, 0), linear_damping=0.5, angular_damping=2.0))
+    o.step(30)
+    st = o.get_state()
+    dt = 1.0 / 60.0
+    d_lin, d_ang = float(np.float32(0.5)), 2.0
+    assert st["linvel"][0, 0] == pytest.approx(3.0 * (1 + dt * d_lin) ** -30, rel=1e-12)
+    assert st["angvel"][0, 1] == pytest.approx(2.0 * (1 + dt * d_ang) ** -30, rel=1e-12)
+
+
+def test_velocity_clamps(oracle_mod):
+    """|v| is clamped to max_linear_velocity (direction kept), |w| to max_angular_velocity, and a
+    zero limit zeroes the velocity."""
+    o = oracle_mod.OracleWorld(abi.default_params(), precision=64)
+    s = merge(body(linvel=(1.0, 0, 0), max_linear_velocity=3.0, angvel=(0, 0, 5.0), max_angular_velocity=2.0,
+                   pos=(0, 0, 0)),
+              body(linvel=(1.0, 2.0, 0), max_linear_velocity=0.0, pos=(10, 0, 0)))
+    o.add_bodies(s)
+    o.step(60)
+    st = o.get_state()
+    assert np.linalg.norm(st["linvel"][0]) == pytest.approx(3.0, rel=1e-12)
+    assert np.linalg.norm(st["angvel"][0]) == pytest.approx(2.0, rel=1e-12)
+    # falling: the clamp keeps the direction, mostly downwards after a second of gravity
+    assert st["linvel"][0, 1] < -2.9
+    assert np.all(st["linvel"][1] == 0.0)
+    assert st["pos"][1, 0] == pytest.approx(10.0)
+
+
+def test_gravity_flag(oracle_mod):
+    o = oracle_mod.OracleWorld(abi.default_params(), precision=64)
+    o.add_bodies(merge(body(pos=(0, 10, 0), gravity_enabled=0), body(pos=(5, 10, 0))))
+    o.step(60)
+    st = o.get_state()
+    assert st["pos"][0, 1] == 10.0 and np.all(st["linvel"][0] == 0.0)
+    # symplectic Euler: y = 10 - g dt^2 n (n + 1) / 2
+    assert st["pos"][1, 1] == pytest.approx(10 - 9.81 / 3600 * 60 * 61 / 2, rel=1e-9)
+
+
+def test_kinematic_translation_ignores_gravity(oracle_mod):
+    """acc (.) jacobian_mask: a body whose y translation is kinematic does not fall; x and z stay
+    free (a velocity along x is kept)."""
+    o = oracle_mod.OracleWorld(abi.default_params(), precision=64)
+    o.add_bodies(body(pos=(0, 10, 0), linvel=(1.0, 0, 0), kinematic_dofs=0b000010))
+    o.step(60)
+    st = o.get_state()
+    assert st["pos"][0, 1] == 10.0 and st["linvel"][0, 1] == 0.0
+    assert st["pos"][0, 0] == pytest.approx(1.0, rel=1e-12)
+
+
+def test_kinematic_rotations_box_does_not_turn(oracle_mod):
+    """A tilted box dropped on the ground with all rotations kinematic never rotates: every
+    contact row's angular jacobian is masked to zero, so its impulses change w by I^-1 0."""
+    ang = math.radians(20.0)
+    q = (math.sin(ang / 2), 0.0, 0.0, math.cos(ang / 2))
+    o = oracle_mod.OracleWorld(abi.default_params(), precision=64)
+    o.add_bodies(merge(ground(), body(shape=1, dims=(0.5, 0.5, 0.5), pos=(0, 1.0, 0), rot=q,
+                                      kinematic_dofs=0b111000)))
+    o.step(90)
+    st = o.get_state()
+    assert np.all(st["angvel"][1] == 0.0)
+    np.testing.assert_array_equal(st["rot"][1], np.array(q, np.float32).astype(np.float64))
+    # it lands on an edge and rests there (the position solver lifts it out of the ground)
+    assert st["pos"][1, 1] > 0.5
+
+
+def test_kinematic_x_translation_ball_spins_up(oracle_mod):
+    """A ball sliding along x with x translation kinematic keeps v_x (friction's linear jacobian
+    is masked) while friction spins it up to rolling, w_z = -v_x / (r + margin) (the unmasked
+    velocity, rigid_body.rs:710-718, of the contact point dilated by the margin,
+    signorini_model.rs:65-66, is then zero)."""
+    r, v0 = 0.5, 2.0
+    o = oracle_mod.OracleWorld(abi.default_params(max_velocity_iterations=20), precision=64)
+    o.add_bodies(merge(ground(), body(dims=(r, 0, 0), pos=(0, r, 0), linvel=(v0, 0, 0), kinematic_dofs=0b000001)))
+    o.step(90)
+    st = o.get_state()
+    assert st["linvel"][1, 0] == v0
+    assert st["angvel"][1, 2] == pytest.approx(-v0 / (r + float(np.float32(0.01))), rel=1e-6)
+
+
+def test_forces_closed_forms(oracle_mod):
+    """Force: dv = F dt / m for one step, then cleared; Impulse: dv = J / m at once and
+    dw = I^-1 tau; VelocityChange: dv = f; AccelerationChange: dv = a dt; a force at a point
+    adds the torque (p - com) x f."""
+    r = 0.5
+    m = 4.0 / 3.0 * math.pi * float(np.float32(r)) ** 3
+    inertia = 0.4 * m * float(np.float32(r)) ** 2
+    dt = 1.0 / 60.0
+    o = oracle_mod.OracleWorld(abi.default_params(gravity=(0, 0, 0)), precision=64)
+    o.add_bodies(scenes.concat(*[body(dims=(r, 0, 0), pos=(3.0 * i, 0, 0)) for i in range(5)]))
+    o.apply_force([0], [[2.0, 0, 0, 0, 0, 0]], abi.FORCE)
+    o.apply_force([1], [[0, 3.0, 0, 0, 0, 0.7]], abi.IMPULSE)
+    o.apply_force([2], [[0, 0, 1.5, 0.2, 0, 0]], abi.VELOCITY_CHANGE)
+    o.apply_force([3], [[0.5, 0, 0, 0, 0, 0]], abi.ACCELERATION_CHANGE)
+    o.apply_force([4], [[0, 0, 1.0]], abi.FORCE, points=[[12.0, 0.5, 0]])
+    o.step(1)
+    st = o.get_state()
+    assert st["linvel"][0, 0] == pytest.approx(2.0 * dt / m, rel=1e-12)
+    assert st["linvel"][1, 1] == pytest.approx(3.0 / m, rel=1e-12)
+    assert st["angvel"][1, 2] == pytest.approx(float(np.float32(0.7)) / inertia, rel=1e-12)
+    np.testing.assert_allclose(st["linvel"][2], [0, 0, 1.5], rtol=1e-12)
+    np.testing.assert_allclose(st["angvel"][2], [float(np.float32(0.2)), 0, 0], rtol=1e-12)
+    assert st["linvel"][3, 0] == pytest.approx(0.5 * dt, rel=1e-12)
+    # torque (0, 0.5, 0) x (0, 0, 1) = (0.5, 0, 0)
+    np.testing.assert_allclose(st["angvel"][4], [0.5 * dt / inertia, 0, 0], rtol=1e-12)
+    # cleared after the step: the next step keeps the velocities
+    v = st["linvel"].copy()
+    o.step(1)
+    np.testing.assert_allclose(o.get_state()["linvel"], v, rtol=1e-12)
+
+
+def test_forces_respect_mask_and_accumulate(oracle_mod):
+    """Two forces on one body add up; a kinematic dof drops its component."""
+    r = 0.5
+    m = 4.0 / 3.0 * math.pi * float(np.float32(r)) ** 3
+    o = oracle_mod.OracleWorld(abi.default_params(gravity=(0, 0, 0)), precision=64)
+    o.add_bodies(body(dims=(r, 0, 0), kinematic_dofs=0b000100))
+    o.apply_force([0, 0], [[1.0, 0, 2.0, 0, 0, 0], [0.5, 0, 0, 0, 0, 0]], abi.FORCE)
+    o.step(1)
+    v = o.get_state()["linvel"][0]
+    assert v[0] == pytest.approx(1.5 / 60.0 / m, rel=1e-12)
+    assert v[2] == 0.0
+
+
+def test_force_wakes_a_sleeping_body(oracle_mod):
+    """apply_force with auto_wake_up activates the body: energy 2 x its threshold."""
+    o = oracle_mod.OracleWorld(abi.default_params(), precision=64)
+    s = merge(ground(), body(dims=(0.5, 0, 0), pos=(0, 0.5, 0), sleep_threshold=0.01))
+    o.add_bodies(s)
+    o.step(400)
+    assert o.activation()[1] == 0.0
+    o.apply_force([1], [[0, 0, 0, 0, 0, 0]], abi.FORCE, auto_wake_up=True)
+    assert o.activation()[1] == pytest.approx(0.02)
