@@ -611,9 +611,9 @@ __device__ __forceinline__ void warm_manifold(const A& acc, const DevBodies& b, 
   acc.store(m, v1, v2);
 }
 
-// One SOR-Prox step of one manifold (sor_prox.rs:159-343): its friction rows, bounded by mu times
-// the current normal impulse and zeroed when that is zero, then its normal rows, projected on
-// l >= 0.
+// One SOR-Prox step of one manifold (sor_prox.rs:159-343), contact by contact: the normal row,
+// projected on l >= 0, then the two friction rows, bounded by mu times that normal impulse and
+// zeroed when it is zero.
 template <bool kMask, class A>
 __device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, const SolverBufs& sb, int slot) {
   Manifold<kMask> m;
@@ -623,6 +623,15 @@ __device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, c
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     if (k >= m.x.cnt) break;
+    {
+      const Row v = row_of(m, k, 0);
+      const float w = row_w(v, v1, v2);
+      const float a = m.c[k].lam.x - v.r * w;
+      const float ni = a > 0.0f ? a : 0.0f;
+      const float dl = ni - m.c[k].lam.x;
+      m.c[k].lam.x = ni;
+      row_add(v, dl, m, v1, v2);
+    }
     const float mu = m.c[k].mu;
     const float ln = m.c[k].lam.x;
 #pragma unroll
@@ -647,17 +656,6 @@ __device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, c
       if (f == 1) m.c[k].lam.y = li;
       else m.c[k].lam.z = li;
     }
-  }
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (k >= m.x.cnt) break;
-    const Row v = row_of(m, k, 0);
-    const float w = row_w(v, v1, v2);
-    const float a = m.c[k].lam.x - v.r * w;
-    const float ni = a > 0.0f ? a : 0.0f;
-    const float dl = ni - m.c[k].lam.x;
-    m.c[k].lam.x = ni;
-    row_add(v, dl, m, v1, v2);
   }
   acc.store(m, v1, v2);
 #pragma unroll

@@ -93,20 +93,46 @@ def test_step_bitexact_vs_coloured_oracle(nphys, oracle_mod, name, steps):
     _assert_same_contacts(g.contacts(), o.contacts())
 
 
-def test_residual_vs_reference_order(nphys, oracle_mod):
-    """Colour order converges like the reference's row order at equal iteration count."""
-    s = scenes.small_pile(n=512)
-    g = _gpu_world(nphys, s)
-    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_REFERENCE)
-    rg, ro = [], []
-    for _ in range(60):
+# Residual at equal iteration count against the FP32 oracle in the reference's own row order:
+# scene, velocity iterations, steps, steps skipped (landing), bound on the ratio of the mean R-inf
+# and on the ratio of the mean RMS (GPU / reference order). The bound is 1 (the GPU no worse)
+# except on box stacks, where the reference's pair order sweeps every column bottom-up within one
+# sweep (its lattice numbering puts each body after the one below it) and a colouring cannot: the
+# colour order there is within 1-4% of it in RMS and 1-13% above it in R-inf (measured).
+RESIDUAL_CASES = {
+    "pile512_it8": (lambda: scenes.small_pile(n=512), 8, 60, 20, 1.0, 1.0),
+    "pile2k_it16": (lambda: scenes.small_pile(seed=21, n=2048, lattice=(16, 8, 16), extent=10.0), 16, 150, 50,
+                    1.0, 1.0),
+    "balls1k_it8": (lambda: scenes.balls(), 8, 150, 50, 1.0, 1.0),
+    "boxes384_it10": (lambda: scenes.boxes(nx=8, ny=6, nz=8), 10, 60, 20, 1.15, 1.04),
+    "boxes1k_it10": (lambda: scenes.boxes(nx=10, ny=10, nz=10), 10, 100, 20, 1.15, 1.04),
+}
+
+
+@pytest.mark.parametrize("case", list(RESIDUAL_CASES))
+def test_residual_vs_reference_order(nphys, oracle_mod, case):
+    """The GPU residual (natural-map R-inf and RMS after the last sweep, averaged over the steps)
+    against the reference row order's at equal iteration count."""
+    make, iters, steps, skip, b_inf, b_rms = RESIDUAL_CASES[case]
+    s = make()
+    g = _gpu_world(nphys, s, max_velocity_iterations=iters)
+    o = _oracle_world(oracle_mod, s, oracle_mod.ORDER_REFERENCE, max_velocity_iterations=iters)
+    gi, gr, oi, orr = [], [], [], []
+    for _ in range(steps):
         g.step(1)
         o.step(1)
-        rg.append(g.residual()[1])
-        ro.append(o.residual()[1])
-    rg, ro = np.array(rg[20:]), np.array(ro[20:])
-    print("rms residual gpu/colour", rg.mean(), "oracle/reference order", ro.mean())
-    assert rg.mean() <= 1.5 * ro.mean()
+        a, b = g.residual()
+        gi.append(a)
+        gr.append(b)
+        a, b = o.residual()
+        oi.append(a)
+        orr.append(b)
+    ratio_inf = np.mean(gi[skip:]) / np.mean(oi[skip:])
+    ratio_rms = np.mean(gr[skip:]) / np.mean(orr[skip:])
+    print(f"{case}: R-inf gpu {np.mean(gi[skip:]):.4g} reference order {np.mean(oi[skip:]):.4g} ({ratio_inf:.3f}); "
+          f"RMS {np.mean(gr[skip:]):.4g} / {np.mean(orr[skip:]):.4g} ({ratio_rms:.3f})")
+    assert ratio_inf <= b_inf
+    assert ratio_rms <= b_rms
 
 
 def test_determinism(nphys):
