@@ -58,18 +58,21 @@ def settle(w, steps):
 
 
 def cpu_baseline(scene, snapshot, params, steps):
-    """The FP32 CPU restatement of the reference step on one thread, from the same snapshot."""
-    from oracle import oracle
-
-    s = dict(scene)
-    for k in ("pos", "rot", "linvel", "angvel"):
-        s[k] = snapshot[k]
-    o = oracle.OracleWorld(params, precision=32)
-    o.add_bodies(s)
-    o.step(1)  # first step also runs the initial collision detection, untimed
-    t0 = time.perf_counter()
-    o.step(steps)
-    dt = time.perf_counter() - t0
+    """The FP32 CPU restatement of the reference step (reference row order) on one thread, from the
+    same snapshot: one untimed step (it also runs the initial collision detection), then `steps`
+    timed steps, with the residual after each. Returns the baseline and the state after the
+    untimed step (for the FP64 error)."""
+    o = oracle_world(with_state(scene, snapshot), params, 32)
+    o.step(1)
+    after_one = o.get_state()
+    dt, rinf, rrms = 0.0, [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        o.step(1)
+        dt += time.perf_counter() - t0
+        a, b = o.residual()
+        rinf.append(a)
+        rrms.append(b)
     c = o.counters()
     return {
         "value": steps / dt,
@@ -77,9 +80,27 @@ def cpu_baseline(scene, snapshot, params, steps):
         "cores": 1,
         "kind": "port",
         "sample": f"{steps} steps of the settled dense pile ({c['ncontacts']} contacts) after one untimed "
-                  f"step, FP32, single thread, on {cpu_model()} ({os.cpu_count()} logical CPUs)",
+                  f"step, FP32, reference row order, single thread, on {cpu_model()} ({os.cpu_count()} logical CPUs)",
         "contact_iters_per_s": c["ncontacts"] * params.max_velocity_iterations * steps / dt,
-    }
+        "residual_inf": float(np.mean(rinf)),
+        "residual_rms": float(np.mean(rrms)),
+    }, after_one
+
+
+def gpu_from_snapshot(scene, snapshot, params, steps):
+    """The GPU on the CPU baseline's sample: the same snapshot, one step (state kept for the FP64
+    error), then `steps` steps with the residual after each."""
+    w = gpu_world(with_state(scene, snapshot), params)
+    w.step(1)
+    after_one = w.get_state()
+    rinf, rrms = [], []
+    for _ in range(steps):
+        w.step(1)
+        a, b = w.residual()
+        rinf.append(a)
+        rrms.append(b)
+    w.close()
+    return float(np.mean(rinf)), float(np.mean(rrms)), after_one
 
 
 def train_warmstart(scene, params, settle_steps, capture_steps, max_sweeps, check_every, what):
@@ -112,17 +133,17 @@ def train_warmstart(scene, params, settle_steps, capture_steps, max_sweeps, chec
 def learned_section(fresh, net_params, train, iters, targets, steps, max_sweeps):
     """Solve-to-residual runs from the state `fresh()` returns, plain vs learned warm start.
 
-    For each RMS target: steps/s and sweeps/step with the plain and with the learned warm start
-    (separate runs from the same state, capped at max_sweeps; whether the target was reached is
-    reported), and the judge's sweep counts, both starts solved from identical assembled systems
-    with the learned start kept only where it needs fewer sweeps. The targets are the given ones
-    plus the RMS residual the plain warm start reaches with the reference's iteration count
+    For each target on R-inf = max |phi| (m/s): steps/s and sweeps/step with the plain and with the
+    learned warm start (separate runs from the same state, capped at max_sweeps; whether the target
+    was reached is reported), and the judge's sweep counts, both starts solved from identical
+    assembled systems with the learned start kept only where it needs fewer sweeps. The targets are
+    the given ones plus the R-inf the plain warm start reaches with the reference's iteration count
     (`iters` sweeps) on the same state."""
     from nphysics_amd import learned
 
     w = fresh()
     w.step(1)
-    r_ref = w.residual()[1]
+    r_ref = w.residual()[0]
     w.close()
 
     def run(tgt, learned_start):
@@ -130,7 +151,7 @@ def learned_section(fresh, net_params, train, iters, targets, steps, max_sweeps)
         if learned_start:
             w.load_warmstart(net_params)
             w.set_contact_model(abi.CONTACT_MODEL_LEARNED_WARMSTART)
-        w.set_residual_target(tgt, max_sweeps, 1, "rms")
+        w.set_residual_target(tgt, max_sweeps, 1, "max")
         w.step(1)
         w.reset_counters()
         t0 = time.perf_counter()
@@ -139,17 +160,18 @@ def learned_section(fresh, net_params, train, iters, targets, steps, max_sweeps)
         c = w.counters()
         r = {"steps_per_s": steps / dt, "sweeps_per_step": c["velocity_sweeps"] / steps,
              "contact_iters_per_s": c["ncontacts"] * c["velocity_sweeps"] / dt,
-             "residual_rms": w.residual()[1], "reached": bool(w.residual()[1] <= tgt)}
+             "residual_inf": w.residual()[0], "residual_rms": w.residual()[1],
+             "reached": bool(w.residual()[0] <= tgt)}
         w.close()
         return r
 
-    out = {"norm": "rms", "max_sweeps": max_sweeps, "steps": steps, "training": train, "runs": []}
+    out = {"norm": "inf", "max_sweeps": max_sweeps, "steps": steps, "training": train, "runs": []}
     for tgt, what in [(t, "fixed target") for t in targets] + [
-            (r_ref, f"RMS residual of the plain warm start after {iters} sweeps on this state")]:
+            (r_ref, f"R-inf of the plain warm start after {iters} sweeps on this state")]:
         sec = {"target": tgt, "what": what, "plain": run(tgt, False), "learned": run(tgt, True)}
         w = fresh()
         sec["judge"] = learned.judge(w, net_params, steps, target=tgt, max_sweeps=max_sweeps, check_every=1,
-                                     norm="rms")
+                                     norm="max")
         w.close()
         p, l = sec["plain"]["sweeps_per_step"], sec["learned"]["sweeps_per_step"]
         sec["sweep_reduction"] = (p - l) / p if p > 0 else 0.0
@@ -201,6 +223,190 @@ def residual_target_section(scene, snapshot, params, target, steps, max_sweeps):
     return out
 
 
+def with_state(scene, st):
+    s = dict(scene)
+    for k in ("pos", "rot", "linvel", "angvel"):
+        s[k] = st[k]
+    return s
+
+
+def gpu_world(scene, params):
+    w = world.MechanicalWorld(integration_parameters=params)
+    w.add_bodies(scene)
+    return w
+
+
+def oracle_world(scene, params, precision):
+    from oracle import oracle
+
+    o = oracle.OracleWorld(params, precision=precision, order=oracle.ORDER_REFERENCE)
+    o.add_bodies(scene)
+    return o
+
+
+def velocity_errors(st, ref):
+    """Largest velocity differences of state `st` against `ref` (the FP64 run), in m/s and rad/s,
+    and relative to the largest FP64 speed."""
+    dl = float(np.max(np.abs(st["linvel"].astype(np.float64) - ref["linvel"]), initial=0.0))
+    da = float(np.max(np.abs(st["angvel"].astype(np.float64) - ref["angvel"]), initial=0.0))
+    dp = float(np.max(np.abs(st["pos"].astype(np.float64) - ref["pos"]), initial=0.0))
+    vmax = float(np.max(np.linalg.norm(ref["linvel"], axis=1), initial=0.0))
+    return {"linvel_max_abs": dl, "angvel_max_abs": da, "pos_max_abs": dp,
+            "linvel_rel_to_max_speed": dl / vmax if vmax > 0 else 0.0}
+
+
+def f64_error(scene, st, params, gpu_state_after=None, cpu32_state_after=None):
+    """Error against an FP64 run of the reference step: one step from the same state on the GPU
+    (FP32, colour order), on the CPU restatement in FP32 (reference order) and in FP64 (reference
+    order, the error reference). Row order and precision both enter the GPU figure; only precision
+    enters the CPU FP32 one."""
+    from oracle import oracle
+
+    s = with_state(scene, st)
+    o64 = oracle_world(s, params, 64)
+    o64.step(1)
+    ref = o64.get_state()
+    del o64
+    o64c = oracle.OracleWorld(params, precision=64, order=oracle.ORDER_COLOURED)
+    o64c.add_bodies(s)
+    o64c.step(1)
+    ref_coloured = o64c.get_state()
+    del o64c
+    if gpu_state_after is None:
+        g = gpu_world(s, params)
+        g.step(1)
+        gpu_state_after = g.get_state()
+        g.close()
+    if cpu32_state_after is None:
+        o32 = oracle_world(s, params, 32)
+        o32.step(1)
+        cpu32_state_after = o32.get_state()
+    return {"what": "one step from the same state against the CPU restatement in FP64 (reference row order); "
+                    "gpu_f32_same_order: against FP64 in the GPU's colour order (precision alone)",
+            "gpu_f32": velocity_errors(gpu_state_after, ref), "cpu_f32": velocity_errors(cpu32_state_after, ref),
+            "gpu_f32_same_order": velocity_errors(gpu_state_after, ref_coloured)}
+
+
+def solve_to_target(make, target, max_sweeps, warm_steps=3):
+    """Sweeps one step needs to reach max |phi| <= target (R-inf, m/s), capped at max_sweeps, after
+    `warm_steps` fixed-count steps that fill the impulse cache; with the time of that step."""
+    w = make()
+    w.step(warm_steps)
+    w.set_residual_target(target, max_sweeps, 1, "max")
+    w.reset_counters()
+    t0 = time.perf_counter()
+    w.step(1)
+    dt = time.perf_counter() - t0
+    sweeps = int(w.counters()["velocity_sweeps"])
+    rinf, rrms = w.residual()
+    out = {"sweeps": sweeps, "residual_inf": rinf, "residual_rms": rrms, "reached": bool(rinf <= target),
+           "step_s": dt}
+    if hasattr(w, "close"):
+        w.close()
+    return out
+
+
+def reachability(scene, st, params, target, max_sweeps, gpu=True):
+    """Sweeps to R-inf <= target from one state: GPU FP32 (colour order), CPU FP32 and FP64 (reference
+    order). The FP64 run separates a precision floor (FP64 reaches the target, FP32 does not) from
+    the convergence of projected Gauss-Seidel itself (neither reaches it)."""
+    s = with_state(scene, st)
+    out = {"target_inf": target, "max_sweeps": max_sweeps}
+    if gpu:
+        out["gpu_f32"] = solve_to_target(lambda: gpu_world(s, params), target, max_sweeps)
+    out["cpu_f32"] = solve_to_target(lambda: oracle_world(s, params, 32), target, max_sweeps)
+    out["cpu_f64"] = solve_to_target(lambda: oracle_world(s, params, 64), target, max_sweeps)
+    return out
+
+
+def small_config(what, scene, params, steps, samples, sample_steps, target, max_sweeps):
+    """One workload run whole on the GPU (all `steps` steps timed from the initial scene, counters
+    off), beside the one-thread FP32 CPU restatement (reference row order) timed on windows of
+    `sample_steps` steps from GPU states at `samples` (each after one untimed step from that state,
+    the GPU stepping the same windows for the residual comparison at equal iteration count); the
+    FP64 error of one step from each sampled state; sweeps to R-inf <= target from the final state."""
+    iters = params.max_velocity_iterations
+    w = gpu_world(scene, params)
+    w.enable_counters(False)
+    t0 = time.perf_counter()
+    w.step(steps)
+    elapsed = time.perf_counter() - t0
+    final = w.get_state()
+    c = w.counters()
+    w.close()
+    # states at the sample points (deterministic: the same trajectory again)
+    w = gpu_world(scene, params)
+    snaps, done = [], 0
+    for k in samples:
+        w.step(k - done) if k > done else None
+        done = k
+        snaps.append(w.get_state())
+    w.close()
+    g_inf, g_rms, o_inf, o_rms, cpu_t, cpu_n, errs = [], [], [], [], 0.0, 0, []
+    contacts = []
+    for st in snaps:
+        s = with_state(scene, st)
+        g = gpu_world(s, params)
+        g.step(1)
+        g_after = g.get_state()
+        o = oracle_world(s, params, 32)
+        o.step(1)
+        errs.append(f64_error(scene, st, params, g_after, o.get_state()))
+        for _ in range(sample_steps):
+            g.step(1)
+            a, b = g.residual()
+            g_inf.append(a)
+            g_rms.append(b)
+            t1 = time.perf_counter()
+            o.step(1)
+            cpu_t += time.perf_counter() - t1
+            cpu_n += 1
+            a, b = o.residual()
+            o_inf.append(a)
+            o_rms.append(b)
+            contacts.append(o.counters()["ncontacts"])
+        g.close()
+    worst = max(errs, key=lambda e: e["gpu_f32"]["linvel_max_abs"])
+    out = {
+        "workload": what,
+        "bodies": int(scene["status"].shape[0]),
+        "dynamic_bodies": int((scene["status"] == 0).sum()),
+        "steps": steps,
+        "value": steps / elapsed,
+        "unit": "steps/s",
+        "ms_per_step": 1e3 * elapsed / steps,
+        "contacts_last_step": c["ncontacts"],
+        "velocity_iterations": iters,
+        "residual_at_equal_iterations": {
+            "what": f"mean over {len(g_inf)} steps ({sample_steps} after each of the states at steps {list(samples)}), "
+                    f"{iters} sweeps per step, both sides from the same states",
+            "gpu_inf": float(np.mean(g_inf)), "gpu_rms": float(np.mean(g_rms)),
+            "cpu_reference_order_inf": float(np.mean(o_inf)), "cpu_reference_order_rms": float(np.mean(o_rms)),
+            "ratio_inf": float(np.mean(g_inf) / np.mean(o_inf)) if np.mean(o_inf) > 0 else None,
+            "ratio_rms": float(np.mean(g_rms) / np.mean(o_rms)) if np.mean(o_rms) > 0 else None},
+        "f64_error": dict(worst, what=worst["what"] + f"; the worst of the states at steps {list(samples)}"),
+        "residual_target": reachability(scene, final, params, target, max_sweeps),
+        "cpu_baseline": {"value": cpu_n / cpu_t, "unit": "steps/s", "cores": 1, "kind": "port",
+                         "sample": f"{cpu_n} steps ({sample_steps} after each of the GPU states at steps "
+                                   f"{list(samples)}, one untimed step first), FP32, reference row order, "
+                                   f"single thread, on {cpu_model()} ({os.cpu_count()} logical CPUs); "
+                                   f"{int(np.mean(contacts))} contacts on average"},
+    }
+    out["contact_iters_per_s"] = float(np.mean(contacts)) * iters * out["value"]
+    out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
+def pile_subset(scene, st, corner=20.0):
+    """The static walls and the dynamic bodies of the settled pile in the corner x, z > corner."""
+    stat = scene["status"] == 1
+    keep = stat | ((st["pos"][:, 0] > corner) & (st["pos"][:, 2] > corner))
+    idx = np.nonzero(keep)[0]
+    n = scene["status"].shape[0]
+    return {k: (v[idx] if isinstance(v, np.ndarray) and v.shape[:1] == (n,) else v) for k, v in scene.items()}, \
+        {k: v[idx] for k, v in st.items()}
+
+
 def cpu_baseline_batched(scene, snapshot, params, steps):
     """The FP32 CPU restatement on the batch from the same state, one world per scene: on one
     thread (the reference's own threading), and on all of the host's cores (at most 16) with a
@@ -246,34 +452,35 @@ def cpu_baseline_batched(scene, snapshot, params, steps):
     return single, every
 
 
-def bench_batched(args):
+def batched_section(steps, warmup, settle_steps, sweep_variant, cpu_steps, no_cpu):
     """4,096 independent worlds (lognormal sizes, median 64 bodies, each with its own ground)
-    stepped as one handle; 8 velocity / 3 position iterations. Prints one JSON line."""
+    stepped as one handle; 8 velocity / 3 position iterations."""
     params = abi.default_params(max_velocity_iterations=8, max_position_iterations=3)
     scene = scenes.batched()
     w = world.MechanicalWorld(integration_parameters=params)
     w.add_bodies(scene)
-    w.set_sweep_variant(args.sweep_variant)
-    settle_steps = min(args.settle, 200)
+    w.set_sweep_variant(sweep_variant)
     t0 = time.perf_counter()
     settle(w, settle_steps)
     log(f"settled in {time.perf_counter() - t0:.1f} s")
-    w.step(args.warmup)
+    w.step(warmup)
     snapshot = w.get_state()
     w.reset_counters()
     t0 = time.perf_counter()
-    w.step(args.steps)
+    w.step(steps)
     elapsed = time.perf_counter() - t0
     c = w.counters()
+    rinf, rrms = w.residual()
+    w.close()
     n_dyn = int((scene["status"] == 0).sum())
     out = {
         "metric": "steps/s of 4,096 independent worlds stepped as one batch",
-        "value": args.steps / elapsed,
+        "value": steps / elapsed,
         "unit": "steps/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": 1e3 * elapsed / steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -284,21 +491,29 @@ def bench_batched(args):
                    "bodies": int(scene["status"].shape[0]), "dynamic_bodies": n_dyn,
                    "settle_steps": settle_steps, "contacts": c["ncontacts"], "manifolds": c["nmanifolds"],
                    "colours": c["ncolours"], "sweep_variant": c["sweep_variant"]},
-        "contact_iters_per_s": c["ncontacts"] * 8 * args.steps / elapsed,
-        "world_steps_per_s": 4096 * args.steps / elapsed,
-        "stages_ms_per_step": {k: 1e3 * c[k] / args.steps for k in
+        "contact_iters_per_s": c["ncontacts"] * 8 * steps / elapsed,
+        "world_steps_per_s": 4096 * steps / elapsed,
+        "residual_inf": rinf,
+        "residual_rms": rrms,
+        "stages_ms_per_step": {k: 1e3 * c[k] / steps for k in
                                ("collision_detection", "broad_phase", "narrow_phase", "assembly",
                                 "velocity_resolution", "velocity_update", "position_resolution")},
     }
-    if not args.no_cpu:
+    if not no_cpu:
         log("timing the CPU baselines ...")
         out["cpu_baseline"], out["cpu_baseline_all_cores"] = cpu_baseline_batched(scene, snapshot, params,
-                                                                                  args.cpu_steps)
+                                                                                  cpu_steps)
         out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
         out["speedup_vs_cpu_all_cores"] = out["value"] / out["cpu_baseline_all_cores"]["value"]
     else:
         out["cpu_baseline"] = None
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def bench_batched(args):
+    """The batched worlds as the bench's one JSON line."""
+    print(json.dumps(batched_section(args.steps, args.warmup, min(args.settle, 200), args.sweep_variant,
+                                     args.cpu_steps, args.no_cpu)), flush=True)
 
 
 def bench_sleeping(args):
@@ -366,7 +581,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the residual-target / learned runs")
-    ap.add_argument("--target-rms", type=float, default=1e-4)
+    ap.add_argument("--target", type=float, default=1e-4, help="R-inf target (m/s) of the residual-target runs")
+    ap.add_argument("--subset-max-sweeps", type=int, default=1000)
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the ball drop, box stack, batched worlds, pile subset and FP64 error sections")
     ap.add_argument("--target-steps", type=int, default=4)
     ap.add_argument("--target-max-sweeps", type=int, default=1000)
     ap.add_argument("--save-state", default=None, help="write the settled pile's state (.npz)")
@@ -510,21 +729,63 @@ def main():
             "timing_region_steps": args.timing_steps,
         },
     }
+    w.close()
     if not args.no_learned:
         log("residual-target runs, plain vs learned warm start ...")
-        w.close()
-        out["residual_target"] = residual_target_section(scene, snapshot, params, args.target_rms,
+        out["residual_target"] = residual_target_section(scene, snapshot, params, args.target,
                                                          args.target_steps, args.target_max_sweeps)
         out["learned_sweep_reduction"] = {
             wl: {f"{r['target']:.3g}": r["sweep_reduction"] for r in sec["runs"]}
             for wl, sec in out["residual_target"].items()}
     if not args.no_cpu:
         log("timing the CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline(scene, snapshot, params, args.cpu_steps)
+        out["cpu_baseline"], cpu_one = cpu_baseline(scene, snapshot, params, args.cpu_steps)
         out["speedup_vs_cpu"] = steps_per_s / out["cpu_baseline"]["value"]
+        g_inf, g_rms, gpu_one = gpu_from_snapshot(scene, snapshot, params, args.cpu_steps)
+        cb = out["cpu_baseline"]
+        out["residual_at_equal_iterations"] = {
+            "what": f"mean over the CPU baseline's {args.cpu_steps} steps from the settled snapshot (one step "
+                    f"after it first), {iters} sweeps per step, both sides from the same state",
+            "gpu_inf": g_inf, "gpu_rms": g_rms,
+            "cpu_reference_order_inf": cb["residual_inf"], "cpu_reference_order_rms": cb["residual_rms"],
+            "ratio_inf": g_inf / cb["residual_inf"], "ratio_rms": g_rms / cb["residual_rms"]}
+        if not args.no_configs:
+            log("FP64 error of one pile step ...")
+            out["f64_error"] = f64_error(scene, snapshot, params, gpu_one, cpu_one)
     else:
         out["cpu_baseline"] = None
+    if not args.no_configs:
+        log("pile corner subset: sweeps to the R-inf target in FP32 and FP64 ...")
+        sub, sub_st = pile_subset(scene, snapshot)
+        r = reachability(sub, sub_st, params, args.target, args.subset_max_sweeps)
+        r["what"] = (f"the settled pile's bodies in the corner x, z > 20 m ({int((sub['status'] == 0).sum())} "
+                     f"dynamic bodies) with the walls; one step solved to the target after 3 fixed-count steps")
+        out["residual_target_pile_subset"] = r
+        out["configs"] = configs_section(args)
     print(json.dumps(out), flush=True)
+
+
+def configs_section(args):
+    """The other workloads, each run whole on the GPU beside the CPU restatement: the ball drop
+    (1,000 balls, 8 + 3 iterations, 200 steps), the box stack (8,000 cuboids, 10 + 3 iterations,
+    500 steps) and the batched worlds (4,096 scenes, 8 + 3 iterations)."""
+    out = {}
+    log("ball drop ...")
+    out["ball_drop"] = small_config(
+        "ball drop: 1,000 balls r 0.5 m on a 10^3 lattice (1.1 m, jitter 0.05 m) over a static ground; "
+        "8 velocity + 3 position iterations, 200 steps, dt 1/60",
+        scenes.balls(), abi.default_params(max_velocity_iterations=8, max_position_iterations=3),
+        200, [0, 60, 120, 180], 5, args.target, 2000)
+    log("box stack ...")
+    out["box_stack"] = small_config(
+        "box stack: 8,000 cuboids (half extents 0.5 m) on a 20^3 lattice (1.05 m, tilt 2 degrees) over a "
+        "static ground; 10 velocity + 3 position iterations, 500 steps, dt 1/60",
+        scenes.boxes(), abi.default_params(max_velocity_iterations=10, max_position_iterations=3),
+        500, [0, 125, 250, 375, 490], 3, args.target, 1000)
+    log("batched worlds ...")
+    out["batched"] = batched_section(args.batched_steps, 10, 200, args.sweep_variant, args.cpu_steps,
+                                     args.no_cpu)
+    return out
 
 
 if __name__ == "__main__":

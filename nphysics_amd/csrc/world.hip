@@ -744,6 +744,7 @@ struct World {
       } else if (rb_host[1] == 0) {
         // a clean body's segment took more appended partners than it has room for: the full build
         // below gets segments that large, so the following incremental updates fit
+        ++counters.incremental_overflows;
         cap = int(rb_host[0]) + 8;
         if ((rc = ensure_bp())) return rc;
         v = bp_view();

@@ -129,13 +129,14 @@ def test_forces_between_steps_bitexact(nphys, oracle_mod):
 
 def test_forces_wake_sleeping_bodies(nphys, oracle_mod):
     """Sleeping bodies woken by a force with auto_wake_up, and by an impulse without it."""
-    s = scenes.with_sleep_threshold(scenes.small_pile(seed=14, n=64, lattice=(4, 4, 4), extent=3.0), 0.01)
+    # a pile where 10 bodies sleep after 400 steps (CPU restatement, colour order)
+    s = scenes.with_sleep_threshold(scenes.small_pile(seed=5, n=64, lattice=(4, 4, 4), extent=4.0), 0.01)
     g = nphys.MechanicalWorld()
     g.add_bodies(s)
     o = oracle_mod.OracleWorld(abi.default_params(), precision=32, order=oracle_mod.ORDER_COLOURED)
     o.add_bodies(s)
-    g.step(300)
-    o.step(300)
+    g.step(400)
+    o.step(400)
     _assert_same(g, o, "settled")
     asleep = np.flatnonzero((g.activation() == 0.0) & (s["status"] == 0))
     assert asleep.size > 0

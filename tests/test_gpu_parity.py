@@ -356,8 +356,10 @@ def test_incremental_pairs_match_full_rebuild(nphys, case):
     """The incremental pair update (only bodies whose loose AABB was replaced re-traversed in the
     previous tree refitted by expansion) gives the same pair list as a full tree build every step,
     with and without scenes, and the trajectories stay identical."""
+    # the batch: small worlds that settle within about 50 steps, so that the changed loose AABBs
+    # per step fall below the incremental update's limit (n / 8)
     s = (scenes.small_pile(seed=23, n=1024, lattice=(16, 4, 16), extent=12.0, y0=0.3) if case == "pile"
-         else scenes.batched(worlds=32, seed=11))
+         else scenes.batched(worlds=48, seed=11, median=24, n_max=48))
     a = _gpu_world(nphys, s)
     b = _gpu_world(nphys, s)
     b.set_incremental_pairs(False)
@@ -372,3 +374,45 @@ def test_incremental_pairs_match_full_rebuild(nphys, case):
     sa, sb = a.get_state(), b.get_state()
     for key in ("pos", "rot", "linvel", "angvel"):
         np.testing.assert_array_equal(sa[key], sb[key], err_msg=key)
+
+
+def test_incremental_update_overflow_grows_segments(nphys):
+    """An incremental pair update that appends more new partners to a clean body's candidate segment
+    than it holds falls back to a full build with larger segments: 24 balls land together on a wide
+    resting box (index 1, so the balls are appended to its segment) among 400 resting balls. The
+    pairs equal a full build's every step, the overflow is counted, and the updates after it are
+    incremental again."""
+    parts = [scenes.ground()]
+    box = scenes._empty(1)
+    box["shape"][0] = 1
+    box["dims"][0] = (2.0, 0.25, 2.0)
+    box["pos"][0] = (0.0, 0.25, 0.0)
+    parts.append(box)
+    grid = scenes._empty(400)
+    i = np.arange(400)
+    grid["dims"][:, 0] = 0.2
+    grid["pos"][:] = np.stack([10.0 + 0.5 * (i % 20), np.full(400, 0.2), -5.0 + 0.5 * (i // 20)], 1)
+    parts.append(grid)
+    drop = scenes._empty(24)
+    k = np.arange(24)
+    drop["dims"][:, 0] = 0.2
+    drop["pos"][:] = np.stack([-1.25 + 0.5 * (k % 6), np.full(24, 3.0), -0.75 + 0.5 * (k // 6)], 1)
+    parts.append(drop)
+    s = scenes.concat(*parts)
+    a = _gpu_world(nphys, s)
+    b = _gpu_world(nphys, s)
+    b.set_incremental_pairs(False)
+    overflow_step = None
+    for k in range(90):
+        a.step(1)
+        b.step(1)
+        np.testing.assert_array_equal(a.pairs(), b.pairs(), err_msg=f"step {k + 1}")
+        c = a.counters()
+        if overflow_step is None and c["incremental_overflows"] > 0:
+            overflow_step, inc_at_overflow = k + 1, c["incremental_pair_updates"]
+    c = a.counters()
+    print(f"overflow at step {overflow_step}; incremental updates {c['incremental_pair_updates']} of "
+          f"{c['pair_rebuilds']} rebuilds")
+    assert overflow_step is not None
+    assert c["incremental_pair_updates"] > inc_at_overflow
+    np.testing.assert_array_equal(a.get_state()["pos"], b.get_state()["pos"])
