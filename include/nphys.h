@@ -118,6 +118,26 @@ typedef struct nphys_body_desc {
   const float* surface_velocity;
 } nphys_body_desc;
 
+/* Joint constraints (src/joint/), structure of arrays for n joints. Kinds: BallConstraint
+ * (ball_constraint.rs:14-178: the anchors coincide), FixedConstraint (fixed_constraint.rs:14-249:
+ * anchors and reference frames coincide) and RevoluteConstraint without limits or motor
+ * (revolute_constraint.rs:14-349: anchors coincide, the bodies turn about the axis only). */
+enum { NPHYS_JOINT_BALL = 0, NPHYS_JOINT_FIXED = 1, NPHYS_JOINT_REVOLUTE = 2 };
+typedef struct nphys_joint_desc {
+  int32_t n;
+  const int32_t* kind;        /* n: NPHYS_JOINT_* */
+  const int32_t* body1;       /* n: body handles, distinct, at least one of the two dynamic */
+  const int32_t* body2;       /* n */
+  const float* anchor1;       /* 3n: anchor in body 1's local frame */
+  const float* anchor2;       /* 3n: anchor in body 2's local frame */
+  const float* frame1;        /* 4n: fixed joints: ref_frame1 (i, j, k, w); NULL: identity */
+  const float* frame2;        /* 4n: fixed joints: ref_frame2; NULL: identity */
+  const float* axis1;         /* 3n: revolute joints: unit axis in body 1's frame; NULL: (0, 0, 1) */
+  const float* axis2;         /* 3n: revolute joints: unit axis in body 2's frame; NULL: (0, 0, 1) */
+  const float* break_force;   /* n: set_break_force; NULL or a negative value: unbreakable */
+  const float* break_torque;  /* n: set_break_torque (fixed, revolute); NULL or negative: unbreakable */
+} nphys_joint_desc;
+
 /* MaterialCombineMode (material.rs:46-66); the mode of higher precedence of the two colliders
  * decides: Max > Multiply > Min > Average. */
 enum { NPHYS_COMBINE_AVERAGE = 0, NPHYS_COMBINE_MIN = 1, NPHYS_COMBINE_MULTIPLY = 2, NPHYS_COMBINE_MAX = 3 };
@@ -163,6 +183,9 @@ typedef struct nphys_counters {
    * re-traversed, the tree refitted by expansion) instead of by a full tree build */
   int64_t incremental_pair_updates;
   int64_t active_bodies;  /* awake dynamic bodies after the last step's activation update */
+  /* accumulated incremental pair updates that overflowed a clean body's candidate segment and
+   * fell back to a full build with larger segments */
+  int64_t incremental_overflows;
 } nphys_counters;
 
 /* ---------------------------------------------------------------- lifecycle */

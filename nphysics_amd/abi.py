@@ -29,6 +29,10 @@ IMPULSE = 1
 ACCELERATION_CHANGE = 2
 VELOCITY_CHANGE = 3
 
+JOINT_BALL = 0
+JOINT_FIXED = 1
+JOINT_REVOLUTE = 2
+
 # optional per-body arrays of nphys_body_desc: name -> (dtype, width)
 OPTIONAL_BODY_FIELDS = {
     "scene": (np.int32, 1),
@@ -158,6 +162,7 @@ class Counters(C.Structure):
         ("pair_rebuilds", C.c_int64),
         ("incremental_pair_updates", C.c_int64),
         ("active_bodies", C.c_int64),
+        ("incremental_overflows", C.c_int64),
     ]
 
     def as_dict(self):
@@ -182,6 +187,60 @@ def u32ptr(a):
 
 def u64ptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+class JointDesc(C.Structure):
+    _fields_ = [
+        ("n", C.c_int32),
+        ("kind", C.POINTER(C.c_int32)),
+        ("body1", C.POINTER(C.c_int32)),
+        ("body2", C.POINTER(C.c_int32)),
+        ("anchor1", C.POINTER(C.c_float)),
+        ("anchor2", C.POINTER(C.c_float)),
+        ("frame1", C.POINTER(C.c_float)),
+        ("frame2", C.POINTER(C.c_float)),
+        ("axis1", C.POINTER(C.c_float)),
+        ("axis2", C.POINTER(C.c_float)),
+        ("break_force", C.POINTER(C.c_float)),
+        ("break_torque", C.POINTER(C.c_float)),
+    ]
+
+
+def joints(kind, body1, body2, anchor1, anchor2, frame1=None, frame2=None, axis1=None, axis2=None,
+           break_force=None, break_torque=None):
+    """A joint set as a dict of arrays (the fields of nphys_joint_desc); scalars broadcast."""
+    body1 = np.atleast_1d(np.asarray(body1, np.int32))
+    n = body1.shape[0]
+
+    def arr(v, dtype, width):
+        if v is None:
+            return None
+        a = np.asarray(v, dtype)
+        return np.ascontiguousarray(np.broadcast_to(a, (n, width) if width > 1 else (n,)))
+
+    out = {"kind": arr(kind, np.int32, 1), "body1": body1, "body2": arr(body2, np.int32, 1),
+           "anchor1": arr(anchor1, np.float32, 3), "anchor2": arr(anchor2, np.float32, 3)}
+    for name, v, w in (("frame1", frame1, 4), ("frame2", frame2, 4), ("axis1", axis1, 3), ("axis2", axis2, 3),
+                       ("break_force", break_force, 1), ("break_torque", break_torque, 1)):
+        a = arr(v, np.float32, w)
+        if a is not None:
+            out[name] = a
+    return out
+
+
+def make_joint_desc(js):
+    """Builds a JointDesc over a joints() dict; returns (desc, keepalive)."""
+    keep = {}
+    d = JointDesc()
+    d.n = int(js["body1"].shape[0])
+    for name in ("kind", "body1", "body2"):
+        keep[name] = np.ascontiguousarray(js[name], np.int32).reshape(-1)
+        setattr(d, name, iptr(keep[name]))
+    for name in ("anchor1", "anchor2", "frame1", "frame2", "axis1", "axis2", "break_force", "break_torque"):
+        if js.get(name) is not None:
+            keep[name] = np.ascontiguousarray(js[name], np.float32).reshape(-1)
+            setattr(d, name, fptr(keep[name]))
+    return d, keep
 
 
 def make_body_desc(scene):

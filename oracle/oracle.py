@@ -47,6 +47,10 @@ def lib():
         L.orc_set_residual_target.argtypes = [vp, C.c_double, C.c_int32, C.c_int32, C.c_int32]
         L.orc_add_bodies.argtypes = [vp, C.POINTER(abi.BodyDesc), C.POINTER(C.c_int32)]
         L.orc_num_bodies.argtypes = [vp, C.POINTER(C.c_int32)]
+        L.orc_add_joints.argtypes = [vp, C.POINTER(abi.JointDesc), C.POINTER(C.c_int32)]
+        L.orc_remove_joint.argtypes = [vp, C.c_int32]
+        L.orc_read_joints.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                                      C.c_int64]
         fp = C.POINTER(C.c_float)
         dp = C.POINTER(C.c_double)
         L.orc_set_state.argtypes = [vp, fp, fp, fp, fp]
@@ -124,6 +128,27 @@ class OracleWorld:
         n = C.c_int32()
         _check(self._L.orc_num_bodies(self._h, C.byref(n)))
         return n.value
+
+    def add_joints(self, js):
+        desc, keep = abi.make_joint_desc(js)
+        first = C.c_int32()
+        _check(self._L.orc_add_joints(self._h, C.byref(desc), C.byref(first)))
+        self._njoints = getattr(self, "_njoints", 0) + desc.n
+        return first.value
+
+    def remove_joint(self, j):
+        _check(self._L.orc_remove_joint(self._h, j))
+
+    def joints(self):
+        """Cached impulses (lin (n, 3), ang (n, 3)) and broken flags of every joint."""
+        n = getattr(self, "_njoints", 0)
+        lin = np.zeros((max(n, 1), 3))
+        ang = np.zeros((max(n, 1), 3))
+        br = np.zeros(max(n, 1), np.int32)
+        _check(self._L.orc_read_joints(self._h, lin.ctypes.data_as(C.POINTER(C.c_double)),
+                                       ang.ctypes.data_as(C.POINTER(C.c_double)),
+                                       br.ctypes.data_as(C.POINTER(C.c_int32)), max(n, 1)))
+        return {"lin": lin[:n], "ang": ang[:n], "broken": br[:n].astype(bool)}
 
     def set_state(self, pos=None, rot=None, linvel=None, angvel=None):
         if all(a is None or np.asarray(a).dtype == np.float64 for a in (pos, rot, linvel, angvel)):

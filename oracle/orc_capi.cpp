@@ -105,6 +105,33 @@ int add_bodies_t(orc::World<T>& w, const nphys_body_desc* d, int32_t* first) {
   return NPHYS_OK;
 }
 
+template <class T>
+int add_joints_t(orc::World<T>& w, const nphys_joint_desc* d, int32_t* first) {
+  if (!d || d->n < 0 || (d->n > 0 && (!d->kind || !d->body1 || !d->body2 || !d->anchor1 || !d->anchor2)))
+    return fail(NPHYS_ERR_INVALID, "bad joint descriptor");
+  const int nb = int(w.bodies.size());
+  for (int i = 0; i < d->n; ++i) {
+    const int b1 = d->body1[i], b2 = d->body2[i];
+    if (b1 < 0 || b1 >= nb || b2 < 0 || b2 >= nb || b1 == b2) return fail(NPHYS_ERR_INVALID, "bad joint body");
+    if (d->kind[i] < 0 || d->kind[i] > 2) return fail(NPHYS_ERR_INVALID, "bad joint kind");
+    if (w.bodies[size_t(b1)].status != orc::STATUS_DYNAMIC && w.bodies[size_t(b2)].status != orc::STATUS_DYNAMIC)
+      return fail(NPHYS_ERR_INVALID, "a joint needs a dynamic body");
+  }
+  if (first) *first = int32_t(w.joints.size());
+  auto v3 = [](const float* p, int i, T x, T y, T z) {
+    return p ? orc::V3<T>{T(p[3 * i]), T(p[3 * i + 1]), T(p[3 * i + 2])} : orc::V3<T>{x, y, z};
+  };
+  auto q4 = [](const float* p, int i) {
+    return p ? orc::Q<T>{T(p[4 * i]), T(p[4 * i + 1]), T(p[4 * i + 2]), T(p[4 * i + 3])}
+             : orc::Q<T>{T(0), T(0), T(0), T(1)};
+  };
+  for (int i = 0; i < d->n; ++i)
+    w.add_joint(d->kind[i], d->body1[i], d->body2[i], v3(d->anchor1, i, 0, 0, 0), v3(d->anchor2, i, 0, 0, 0),
+                q4(d->frame1, i), q4(d->frame2, i), v3(d->axis1, i, 0, 0, 1), v3(d->axis2, i, 0, 0, 1),
+                d->break_force ? T(d->break_force[i]) : T(-1), d->break_torque ? T(d->break_torque[i]) : T(-1));
+  return NPHYS_OK;
+}
+
 template <class T, class S>
 int set_state_t(orc::World<T>& w, const S* pos, const S* rot, const S* lv, const S* av) {
   for (size_t i = 0; i < w.bodies.size(); ++i) {
@@ -270,6 +297,32 @@ int orc_set_brute_force_limit(void* h, int32_t n) {
 int orc_add_bodies(void* h, const nphys_body_desc* d, int32_t* first) {
   if (!d || d->n < 0) return fail(NPHYS_ERR_INVALID, "bad body desc");
   return with_world(h, [&](auto& w) { return add_bodies_t(w, d, first); });
+}
+
+int orc_add_joints(void* h, const nphys_joint_desc* d, int32_t* first) {
+  return with_world(h, [&](auto& w) { return add_joints_t(w, d, first); });
+}
+
+int orc_remove_joint(void* h, int32_t j) {
+  return with_world(h, [&](auto& w) {
+    if (j < 0 || j >= int(w.joints.size())) return fail(NPHYS_ERR_INVALID, "bad joint handle");
+    w.joints[size_t(j)].removed = true;
+    return NPHYS_OK;
+  });
+}
+
+// lin [n][3], ang [n][3] (revolute: 2 used), broken [n]
+int orc_read_joints(void* h, double* lin, double* ang, int32_t* broken, int64_t cap) {
+  return with_world(h, [&](auto& w) {
+    if (cap < int64_t(w.joints.size())) return fail(NPHYS_ERR_CAPACITY, "buffer too small");
+    for (size_t i = 0; i < w.joints.size(); ++i) {
+      const auto& j = w.joints[i];
+      if (lin) put3(lin + 3 * i, j.lin_imp);
+      if (ang) put3(ang + 3 * i, j.ang_imp);
+      if (broken) broken[i] = j.broken ? 1 : 0;
+    }
+    return NPHYS_OK;
+  });
 }
 
 int orc_num_bodies(void* h, int32_t* n) {

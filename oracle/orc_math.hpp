@@ -258,6 +258,53 @@ inline Q<T> qexp_scaled_axis(V3<T> axisangle) {
   return Q<T>{h.x * s, h.y * s, h.z * s, c};
 }
 
+// atan(x) for 0 <= x <= 1 by plain arithmetic (the GPU evaluates the same expression tree): three
+// half-angle reductions x -> x / (1 + sqrt(1 + x^2)) (atan x = 2 atan x') bring x below
+// tan(pi/32) ~ 0.0985, then the Maclaurin series of atan through x^19 in Horner form (truncation
+// below 1e-20 relative there).
+template <class T>
+inline T atan_unit(T x) {
+  for (int i = 0; i < 3; ++i) x = x / (T(1) + std::sqrt(T(1) + x * x));
+  const T x2 = x * x;
+  T p = T(-1.0 / 19.0);
+  p = T(1.0 / 17.0) + x2 * p;
+  p = T(-1.0 / 15.0) + x2 * p;
+  p = T(1.0 / 13.0) + x2 * p;
+  p = T(-1.0 / 11.0) + x2 * p;
+  p = T(1.0 / 9.0) + x2 * p;
+  p = T(-1.0 / 7.0) + x2 * p;
+  p = T(1.0 / 5.0) + x2 * p;
+  p = T(-1.0 / 3.0) + x2 * p;
+  p = T(1) + x2 * p;
+  return T(8) * (x * p);
+}
+
+// atan2(y, x) for y >= 0, in [0, pi].
+template <class T>
+inline T angle_of(T y, T x) {
+  const T pi = T(3.14159265358979323846);
+  const T ax = std::fabs(x);
+  if (y == T(0) && ax == T(0)) return T(0);
+  const T base = ax >= y ? atan_unit(y / ax) : pi * T(0.5) - atan_unit(ax / y);
+  return x >= T(0) ? base : pi - base;
+}
+
+template <class T>
+inline Q<T> qconj(Q<T> q) { return Q<T>{-q.i, -q.j, -q.k, q.w}; }
+
+// UnitQuaternion::scaled_axis (nalgebra, unvendored; restated): axis times angle, the axis the
+// imaginary part (negated when w < 0) normalised, the angle 2 atan2(|imag|, |w|) in [0, pi]; zero
+// when |imag| is at most the scalar's epsilon.
+template <class T>
+inline V3<T> q_scaled_axis(Q<T> q) {
+  V3<T> v{q.i, q.j, q.k};
+  if (q.w < T(0)) v = -v;
+  const T s = norm(v);
+  if (s <= std::numeric_limits<T>::epsilon()) return V3<T>{T(0), T(0), T(0)};
+  const T ang = T(2) * angle_of(s, std::fabs(q.w));
+  return V3<T>{v.x / s * ang, v.y / s * ang, v.z / s * ang};
+}
+
 // Rigid transform.
 template <class T>
 struct Iso {

@@ -127,6 +127,30 @@ struct ContactRows {
   int fric[2];     // index in bil or bil_g
 };
 
+// A joint constraint (src/joint/): BallConstraint (ball_constraint.rs), FixedConstraint
+// (fixed_constraint.rs) or RevoluteConstraint without limits or motor (revolute_constraint.rs).
+enum { JOINT_BALL = 0, JOINT_FIXED = 1, JOINT_REVOLUTE = 2 };
+template <class T>
+struct Joint {
+  int kind, b1, b2;
+  V3<T> a1, a2;       // anchors in the bodies' local frames
+  Q<T> f1, f2;        // fixed: reference frames ref_frame1/2
+  V3<T> ax1, ax2;     // revolute: unit axes in the local frames
+  V3<T> lin_imp{T(0), T(0), T(0)}, ang_imp{T(0), T(0), T(0)};  // impulses cached by the last solve
+  T break_force_sq, break_torque_sq;
+  bool broken = false, removed = false;
+  int colour = 0;
+};
+
+// Rows of one active joint: linear rows along x, y, z at the anchors (cancel_relative_linear_velocity,
+// helper.rs:247-296), then angular rows (fixed: x, y, z, cancel_relative_angular_velocity,
+// helper.rs:502-551; revolute: the two directions orthogonal to the world axis,
+// restrict_relative_angular_velocity_to_axis, helper.rs:614-699).
+struct JointRows {
+  int joint, ground, n;
+  int row[6];  // indices in jbil or jbil_g
+};
+
 struct Counters {  // stage names of src/counters/mod.rs:148-209
   double step = 0, collision_detection = 0, broad_phase = 0, narrow_phase = 0, assembly = 0,
          velocity_resolution = 0, velocity_update = 0, position_resolution = 0;
@@ -301,6 +325,15 @@ class World {
   std::vector<char> manifold_active;   // per manifold: handed to the solver this step
   bool any_sleeper = false;            // some dynamic body has a sleep threshold
   std::vector<ContactRows> crow;       // per contact of `solved`
+  // joints (JointConstraintSet), their colouring (joints sharing a dynamic body conflict; fixed
+  // greedy colours in joint order, recomputed when joints are added) and the rows of the joints
+  // active in the last solve (island_joints, mechanical_world.rs:272-277)
+  std::vector<Joint<T>> joints;
+  int njcolours = 0;
+  std::vector<int> active_joints;      // reference order: joint order
+  std::vector<int> joint_order;        // colour order of the active joints
+  std::vector<VelRow<T>> jbil, jbil_g;
+  std::vector<JointRows> jrow;         // per active joint, in active_joints order
   std::vector<int> colours;
   int ncolours = 0;
   T res_inf = T(0), res_rms = T(0);
@@ -648,11 +681,16 @@ class World {
   // ------------------------------------------------------------------ contact model
   // fill_constraint_geometry for one body: J = [d; (p - com) x d], W J = M^-1 J,
   // inv_r += m^-1 + (J_ang . I^-1 J_ang), out_vel += J . v + J . ext (rigid_body.rs:672-722).
+  // With `angular` the force direction is ForceDirection::Angular (force (0, dir)); inv_r still
+  // gains the inverse mass then (inv_mass.mass() at rigid_body.rs:707-708 is added for every
+  // direction).
   void fill_geom(const Body<T>& b, V3<T> point, V3<T> dir, size_t j_id, size_t wj_id, T& inv_r,
-                 const T* ext, T* out_vel) {
+                 const T* ext, T* out_vel, bool angular = false) {
     if (b.status != STATUS_DYNAMIC) return;
+    const V3<T> zero{T(0), T(0), T(0)};
     const V3<T> p = point - b.com;
-    const V3<T> ang_u = cross(p, dir);
+    const V3<T> ang_u = angular ? dir : cross(p, dir);
+    if (angular) dir = zero;
     // the jacobian is masked; the velocity term uses the unmasked force so that kinematic dofs stay
     // controllable at the velocity level (rigid_body.rs:685-718)
     const V3<T> lin = cmul(dir, b.mask_lin);
@@ -683,7 +721,7 @@ class World {
     T r;
   };
   Geom pair_geometry(const Body<T>& b1, const Body<T>& b2, V3<T> c1, V3<T> c2, V3<T> dir,
-                     size_t& ground_j_id, size_t& j_id, T* out_vel) {
+                     size_t& ground_j_id, size_t& j_id, T* out_vel, bool angular = false) {
     Geom g;
     g.nd1 = b1.status == STATUS_DYNAMIC ? 6 : 0;
     g.nd2 = b2.status == STATUS_DYNAMIC ? 6 : 0;
@@ -695,8 +733,8 @@ class World {
     T inv_r = T(0);
     const T* e1 = g.nd1 ? &ext_vels[b1.companion] : nullptr;
     const T* e2 = g.nd2 ? &ext_vels[b2.companion] : nullptr;
-    fill_geom(b1, c1, dir, g.j1, g.wj1, inv_r, e1, out_vel);
-    fill_geom(b2, c2, -dir, g.j2, g.wj2, inv_r, e2, out_vel);
+    fill_geom(b1, c1, dir, g.j1, g.wj1, inv_r, e1, out_vel, angular);
+    fill_geom(b2, c2, -dir, g.j2, g.wj2, inv_r, e2, out_vel, angular);
     g.r = inv_r != T(0) ? T(1) / inv_r : T(1);
     *out_j += size_t(g.nd1 + g.nd2) * 2;
     return g;
@@ -753,8 +791,15 @@ class World {
       const T acc[6] = {b.acc_lin.x, b.acc_lin.y, b.acc_lin.z, b.acc_ang.x, b.acc_ang.y, b.acc_ang.z};
       for (int k = 0; k < 6; ++k) ext_vels[b.companion + k] = prm.dt * acc[k];
     }
-    // jacobian sizing: non-ground region first, ground region after (:203-224)
+    // jacobian sizing: non-ground region first, ground region after (:181-224)
     size_t jac_sz = 0, gjac_sz = 0;
+    for (int ji : active_joints) {
+      const Joint<T>& jt = joints[size_t(ji)];
+      const int nd1 = bodies[size_t(jt.b1)].status == STATUS_DYNAMIC ? 6 : 0;
+      const int nd2 = bodies[size_t(jt.b2)].status == STATUS_DYNAMIC ? 6 : 0;
+      const size_t sz = size_t(joint_num_rows(jt)) * 2 * size_t(nd1 + nd2);
+      if (nd1 == 0 || nd2 == 0) gjac_sz += sz; else jac_sz += sz;
+    }
     for (const auto& m : ms) {
       const int nd1 = bodies[m.b1].status == STATUS_DYNAMIC ? 6 : 0;
       const int nd2 = bodies[m.b2].status == STATUS_DYNAMIC ? 6 : 0;
@@ -764,6 +809,8 @@ class World {
     jacobians.assign(jac_sz + gjac_sz, T(0));
     size_t j_id = 0, ground_j_id = jac_sz;
     uni.clear(); uni_g.clear(); bil.clear(); bil_g.clear(); posc.clear(); crow.clear();
+    jbil.clear(); jbil_g.clear(); jrow.clear();
+    for (int ji : active_joints) joint_velocity_constraints(ji, ground_j_id, j_id);
     for (const auto& m : ms) {
       const Body<T>& b1 = bodies[m.b1];
       const Body<T>& b2 = bodies[m.b2];
@@ -818,7 +865,228 @@ class World {
         crow.push_back(cr);
       }
     }
-    n_rows = long(uni.size() + uni_g.size() + bil.size() + bil_g.size());
+    n_rows = long(uni.size() + uni_g.size() + bil.size() + bil_g.size() + jbil.size() + jbil_g.size());
+  }
+
+  // ------------------------------------------------------------------ joints
+  static int joint_num_rows(const Joint<T>& j) {  // num_velocity_constraints
+    return j.kind == JOINT_BALL ? 3 : (j.kind == JOINT_FIXED ? 6 : 5);
+  }
+  // World anchors and, for the fixed joint, the frames' world rotations
+  // (position_at_material_point * ref_frame, rigid_body.rs:641-648, fixed_constraint.rs:173-177).
+  void joint_frames(const Joint<T>& j, V3<T>& w1, V3<T>& w2, Q<T>& r1, Q<T>& r2) const {
+    const Body<T>& b1 = bodies[size_t(j.b1)];
+    const Body<T>& b2 = bodies[size_t(j.b2)];
+    w1 = iso_apply(b1.pos, j.a1);
+    w2 = iso_apply(b2.pos, j.a2);
+    r1 = qmul(b1.pos.q, j.f1);
+    r2 = qmul(b2.pos.q, j.f2);
+  }
+  // JointConstraint::velocity_constraints for one joint: bilateral rows with Independent
+  // (unbounded) limits, warm-started with the joint's own cached impulses (no warm-start coefficient).
+  void joint_velocity_constraints(int ji, size_t& ground_j_id, size_t& j_id) {
+    const Joint<T>& jt = joints[size_t(ji)];
+    const Body<T>& b1 = bodies[size_t(jt.b1)];
+    const Body<T>& b2 = bodies[size_t(jt.b2)];
+    V3<T> w1, w2;
+    Q<T> r1, r2;
+    joint_frames(jt, w1, w2, r1, r2);
+    JointRows jr;
+    jr.joint = ji;
+    jr.n = 0;
+    jr.ground = 0;
+    auto push = [&](V3<T> dir, bool angular, T impulse) {
+      T rhs = T(0);
+      const Geom g = pair_geometry(b1, b2, w1, w2, dir, ground_j_id, j_id, &rhs, angular);
+      VelRow<T> row = make_row(g, b1, b2, rhs, impulse, 0);
+      jr.ground = row.ground;
+      std::vector<VelRow<T>>& list = row.ground ? jbil_g : jbil;
+      list.push_back(row);
+      jr.row[jr.n++] = int(list.size()) - 1;
+    };
+    const V3<T> e[3] = {V3<T>{T(1), T(0), T(0)}, V3<T>{T(0), T(1), T(0)}, V3<T>{T(0), T(0), T(1)}};
+    for (int i = 0; i < 3; ++i) push(e[i], false, comp(jt.lin_imp, i));
+    if (jt.kind == JOINT_FIXED) {
+      for (int i = 0; i < 3; ++i) push(e[i], true, comp(jt.ang_imp, i));
+    } else if (jt.kind == JOINT_REVOLUTE) {
+      // axis1 = position_at_material_point(anchor1) * axis1 (revolute_constraint.rs:230)
+      const V3<T> axis = qrot(b1.pos.q, jt.ax1);
+      V3<T> t[2];
+      tangents(axis, t[0], t[1]);  // orthonormal_subspace_basis(axis), in that order
+      for (int i = 0; i < 2; ++i) push(t[i], true, comp(jt.ang_imp, i));
+    }
+    jrow.push_back(jr);
+  }
+  void solve_bilateral(VelRow<T>& c) {  // sor_prox.rs:232-343 with Independent limits
+    const T w = row_w(c);
+    const T lo = -std::numeric_limits<T>::max(), hi = std::numeric_limits<T>::max();
+    const T v = c.impulse - c.r * w;
+    const T ni = v > lo ? (v < hi ? v : hi) : lo;
+    const T dl = ni - c.impulse;
+    c.impulse = ni;
+    row_add(c, dl);
+  }
+  VelRow<T>& jrow_at(const JointRows& jr, int k) { return jr.ground ? jbil_g[size_t(jr.row[k])] : jbil[size_t(jr.row[k])]; }
+  // JointConstraint::cache_impulses (ball_constraint.rs:133-147, fixed_constraint.rs:194-218,
+  // revolute_constraint.rs:265-289): impulses kept per joint; broken once |lin|^2 / dt^2 exceeds
+  // the squared break force or |ang|^2 / dt^2 the squared break torque.
+  void cache_joint_impulses() {
+    const T inv_dt2 = prm.inv_dt * prm.inv_dt;
+    for (const JointRows& jr : jrow) {
+      Joint<T>& jt = joints[size_t(jr.joint)];
+      for (int k = 0; k < jr.n; ++k) {
+        const T v = jrow_at(jr, k).impulse;
+        if (k < 3) set_comp(jt.lin_imp, k, v);
+        else set_comp(jt.ang_imp, k - 3, v);
+      }
+      if (norm2(jt.lin_imp) * inv_dt2 > jt.break_force_sq || norm2(jt.ang_imp) * inv_dt2 > jt.break_torque_sq)
+        jt.broken = true;
+    }
+  }
+  // One generic position constraint (GenericNonlinearConstraint, solved by
+  // NonlinearSORProx::solve_generic, nonlinear_sor_prox.rs:85-119): rhs = -|error| when |error|
+  // exceeds the allowed error, clamped by clamp_rhs, displacement W J (-rhs r) applied at once.
+  void solve_joint_generic(Joint<T>& jt, V3<T> w1, V3<T> w2, V3<T> error, bool angular) {
+    const T allowed = angular ? prm.allowed_angular_error : prm.allowed_linear_error;
+    const T depth = norm(error);
+    if (!(depth > allowed)) return;  // Unit::try_new_and_get(error, allowed)
+    const V3<T> dir{error.x / depth, error.y / depth, error.z / depth};
+    Body<T>& b1 = bodies[size_t(jt.b1)];
+    Body<T>& b2 = bodies[size_t(jt.b2)];
+    const T rhs = angular ? tmax((-depth + prm.allowed_angular_error) * prm.erp, -prm.max_angular_correction)
+                          : tmax((-depth + prm.allowed_linear_error) * prm.erp, -prm.max_linear_correction);
+    if (!(rhs < T(0))) return;
+    T inv_r = T(0);
+    V3<T> wl1{}, wa1{}, wl2{}, wa2{};
+    const V3<T> zero{T(0), T(0), T(0)};
+    if (b1.status == STATUS_DYNAMIC) {
+      const V3<T> lin = angular ? zero : cmul(dir, b1.mask_lin);
+      const V3<T> ang = cmul(angular ? dir : cross(w1 - b1.com, dir), b1.mask_ang);
+      wl1 = lin * b1.inv_mass;
+      wa1 = mulv(b1.inv_aug, ang);
+      inv_r = inv_r + (b1.inv_mass + dot(ang, wa1));
+    }
+    if (b2.status == STATUS_DYNAMIC) {
+      const V3<T> d2 = -dir;
+      const V3<T> lin = angular ? zero : cmul(d2, b2.mask_lin);
+      const V3<T> ang = cmul(angular ? d2 : cross(w2 - b2.com, d2), b2.mask_ang);
+      wl2 = lin * b2.inv_mass;
+      wa2 = mulv(b2.inv_aug, ang);
+      inv_r = inv_r + (b2.inv_mass + dot(ang, wa2));
+    }
+    const T r = inv_r != T(0) ? T(1) / inv_r : T(1);
+    const T impulse = -rhs * r;
+    if (b1.status == STATUS_DYNAMIC) apply_displacement(b1, wl1 * impulse, wa1 * impulse);
+    if (b2.status == STATUS_DYNAMIC) apply_displacement(b2, wl2 * impulse, wa2 * impulse);
+  }
+  // NonlinearConstraintGenerator::position_constraint of one joint, its constraints in order, each
+  // evaluated at the poses the previous one left (ball: translation; fixed: rotation, then
+  // translation; revolute: translation, then axis alignment).
+  void joint_position(int ji) {
+    Joint<T>& jt = joints[size_t(ji)];
+    V3<T> w1, w2;
+    Q<T> r1, r2;
+    auto translation = [&]() {
+      joint_frames(jt, w1, w2, r1, r2);
+      solve_joint_generic(jt, w1, w2, w2 - w1, false);  // cancel_relative_translation (helper.rs:364-420)
+    };
+    if (jt.kind == JOINT_FIXED) {
+      // cancel_relative_rotation (helper.rs:553-612): error = (rotation2 / rotation1).scaled_axis()
+      joint_frames(jt, w1, w2, r1, r2);
+      solve_joint_generic(jt, w1, w2, q_scaled_axis(qmul(r2, qconj(r1))), true);
+      translation();
+    } else if (jt.kind == JOINT_REVOLUTE) {
+      translation();
+      // align_axis (helper.rs:701-769): the scaled axis of the rotation taking axis1 to axis2
+      joint_frames(jt, w1, w2, r1, r2);
+      const V3<T> a1 = qrot(bodies[size_t(jt.b1)].pos.q, jt.ax1);
+      const V3<T> a2 = qrot(bodies[size_t(jt.b2)].pos.q, jt.ax2);
+      solve_joint_generic(jt, w1, w2, axis_alignment_error(a1, a2), true);
+    } else {
+      translation();
+    }
+  }
+  // Rotation::rotation_between_axis(a1, a2).scaled_axis() (nalgebra, unvendored; restated): the
+  // axis a1 x a2 normalised times the angle atan2(|a1 x a2|, a1 . a2); for (anti)parallel axes
+  // (|a1 x a2| at most epsilon) zero when a1 . a2 >= 0, otherwise the reference's pi fallback
+  // (helper.rs:721-727): e_imin x a1 normalised times pi, imin the index of a1's smallest |component|.
+  static V3<T> axis_alignment_error(V3<T> a1, V3<T> a2) {
+    const V3<T> c = cross(a1, a2);
+    const T sn = norm(c);
+    const T cs = dot(a1, a2);
+    if (sn > std::numeric_limits<T>::epsilon()) {
+      const T ang = angle_of(sn, cs);
+      return V3<T>{c.x / sn * ang, c.y / sn * ang, c.z / sn * ang};
+    }
+    if (cs >= T(0)) return V3<T>{T(0), T(0), T(0)};
+    int imin = 0;
+    if (std::fabs(a1.y) < std::fabs(comp(a1, imin))) imin = 1;
+    if (std::fabs(a1.z) < std::fabs(comp(a1, imin))) imin = 2;
+    V3<T> e{T(0), T(0), T(0)};
+    set_comp(e, imin, T(1));
+    const V3<T> d = cross(e, a1);
+    const T l = norm(d);
+    const T pi = T(3.14159265358979323846);
+    return V3<T>{d.x / l * pi, d.y / l * pi, d.z / l * pi};
+  }
+  // Greedy colouring of the joints in joint order: a joint takes the smallest colour no earlier
+  // joint sharing one of its dynamic bodies has. Joints between the same bodies conflict.
+  void colour_joints() {
+    std::vector<std::vector<int>> used(bodies.size());
+    njcolours = 0;
+    for (auto& jt : joints) {
+      std::vector<char> taken;
+      for (int b : {jt.b1, jt.b2}) {
+        if (bodies[size_t(b)].status != STATUS_DYNAMIC) continue;
+        for (int c : used[size_t(b)]) {
+          if (int(taken.size()) <= c) taken.resize(size_t(c) + 1, 0);
+          taken[size_t(c)] = 1;
+        }
+      }
+      int c = 0;
+      while (c < int(taken.size()) && taken[size_t(c)]) ++c;
+      jt.colour = c;
+      njcolours = std::max(njcolours, c + 1);
+      for (int b : {jt.b1, jt.b2})
+        if (bodies[size_t(b)].status == STATUS_DYNAMIC) used[size_t(b)].push_back(c);
+    }
+  }
+  int add_joint(int kind, int b1, int b2, V3<T> a1, V3<T> a2, Q<T> f1, Q<T> f2, V3<T> ax1, V3<T> ax2,
+                T break_force, T break_torque) {
+    Joint<T> j;
+    j.kind = kind;
+    j.b1 = b1;
+    j.b2 = b2;
+    j.a1 = a1;
+    j.a2 = a2;
+    j.f1 = f1;
+    j.f2 = f2;
+    j.ax1 = ax1;
+    j.ax2 = ax2;
+    const T mx = std::numeric_limits<T>::max();
+    j.break_force_sq = break_force >= T(0) ? break_force * break_force : mx;
+    j.break_torque_sq = break_torque >= T(0) ? break_torque * break_torque : mx;
+    joints.push_back(j);
+    colour_joints();
+    return int(joints.size()) - 1;
+  }
+  // island_joints: joints neither broken nor removed with a dynamic, awake body
+  // (JointConstraint::is_active, joint_constraint.rs:218-227); `joint_order` lists them by colour.
+  void collect_active_joints() {
+    active_joints.clear();
+    joint_order.clear();
+    for (size_t i = 0; i < joints.size(); ++i) {
+      const Joint<T>& jt = joints[i];
+      if (jt.broken || jt.removed) continue;
+      const Body<T>& b1 = bodies[size_t(jt.b1)];
+      const Body<T>& b2 = bodies[size_t(jt.b2)];
+      const bool a1 = b1.status == STATUS_DYNAMIC && b1.energy != T(0);
+      const bool a2 = b2.status == STATUS_DYNAMIC && b2.energy != T(0);
+      if (a1 || a2) active_joints.push_back(int(i));
+    }
+    for (int c = 0; c < njcolours; ++c)
+      for (size_t k = 0; k < active_joints.size(); ++k)
+        if (joints[size_t(active_joints[k])].colour == c) joint_order.push_back(int(k));
   }
   static T imp_comp(V3<T> v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
 
@@ -881,14 +1149,23 @@ class World {
   }
 
   // SORProx::solve (sor_prox.rs:48-80), split into the warm start and one sweep.
-  void warmstart() {  // sor_prox.rs:57-58, 19-45
+  void warmstart() {  // sor_prox.rs:57-58, 19-45: the contacts' rows, then the joints'
     if (order_mode == ORDER_REFERENCE) {
       for (auto& c : uni) warm_row(c);
       for (auto& c : uni_g) warm_row(c);
       for (auto& c : bil) warm_row(c);
       for (auto& c : bil_g) warm_row(c);
+      for (auto& c : jbil) warm_row(c);
+      for (auto& c : jbil_g) warm_row(c);
       return;
     }
+    warmstart_contacts_coloured();
+    for (int k : joint_order) {
+      const JointRows& jr = jrow[size_t(k)];
+      for (int i = 0; i < jr.n; ++i) warm_row(jrow_at(jr, i));
+    }
+  }
+  void warmstart_contacts_coloured() {
     for (int m : colour_order) {
       const Manifold& mf = solved[m];
       for (int k = 0; k < mf.count; ++k) {
@@ -901,16 +1178,22 @@ class World {
       }
     }
   }
-  void sweep() {  // SORProx::step (sor_prox.rs:159-179)
+  void sweep() {  // SORProx::step (sor_prox.rs:159-179): joint rows first, then the contacts'
     if (order_mode == ORDER_REFERENCE) {
+      for (auto& c : jbil) solve_bilateral(c);
+      for (auto& c : jbil_g) solve_bilateral(c);
       for (auto& c : bil) solve_friction(c, uni);
       for (auto& c : bil_g) solve_friction(c, uni_g);
       for (auto& c : uni) solve_normal(c);
       for (auto& c : uni_g) solve_normal(c);
       return;
     }
-    // colour order: per contact its normal row, then its two friction rows bounded by the normal
-    // impulse just computed
+    // colour order: the joints by joint colour, each its rows in order; then per contact its
+    // normal row and its two friction rows bounded by the normal impulse just computed
+    for (int k : joint_order) {
+      const JointRows& jr = jrow[size_t(k)];
+      for (int i = 0; i < jr.n; ++i) solve_bilateral(jrow_at(jr, i));
+    }
     for (int m : colour_order) {
       const Manifold& mf = solved[m];
       for (int k = 0; k < mf.count; ++k) {
@@ -976,6 +1259,9 @@ class World {
     for (auto& c : uni_g) normal(c);
     for (auto& c : bil) fric(c, uni);
     for (auto& c : bil_g) fric(c, uni_g);
+    // joint rows (unbounded): l - (l - r w) over r, i.e. w
+    for (auto& c : jbil) acc(row_w(c));
+    for (auto& c : jbil_g) acc(row_w(c));
     res_inf = T(mx);
     res_rms = n ? T(std::sqrt(ss / double(n))) : T(0);
   }
@@ -1048,6 +1334,12 @@ class World {
 
   void position_correction() {  // NonlinearSORProx::solve (nonlinear_sor_prox.rs:17-55)
     for (int it = 0; it < prm.max_position_iterations; ++it) {
+      // the joints first (island_joints order, or by joint colour), then the contacts
+      if (order_mode == ORDER_REFERENCE) {
+        for (int ji : active_joints) joint_position(ji);
+      } else {
+        for (int k : joint_order) joint_position(active_joints[size_t(k)]);
+      }
       if (order_mode == ORDER_REFERENCE) {
         for (auto& pc : posc) solve_position_row(pc);
       } else {
@@ -1126,6 +1418,12 @@ class World {
       const int a = uf_find(p, m.b1), c = uf_find(p, m.b2);
       if (a != c) p[size_t(std::max(a, c))] = std::min(a, c);
     }
+    for (const auto& jt : joints) {  // joints that are not broken join their bodies' islands (:154-159)
+      if (jt.broken || jt.removed) continue;
+      if (bodies[size_t(jt.b1)].status != STATUS_DYNAMIC || bodies[size_t(jt.b2)].status != STATUS_DYNAMIC) continue;
+      const int a = uf_find(p, jt.b1), c = uf_find(p, jt.b2);
+      if (a != c) p[size_t(std::max(a, c))] = std::min(a, c);
+    }
     std::vector<char> can(bodies.size(), 1);
     for (size_t i = 0; i < bodies.size(); ++i) {
       const Body<T>& b = bodies[i];
@@ -1152,8 +1450,9 @@ class World {
     // manifold collection (mechanical_world.rs:287-300): every kept manifold has >= 1 contact
     // and a dynamic side that is awake; narrow_phase() stores them with contiguous contacts.
     manifold_active.assign(manifolds.size(), 1);
+    if (any_sleeper) activation_update();
+    collect_active_joints();
     if (any_sleeper) {
-      activation_update();
       solved.clear();
       solved_contacts.clear();
       for (size_t m = 0; m < manifolds.size(); ++m) {
@@ -1179,6 +1478,7 @@ class World {
     if (order_mode == ORDER_COLOURED) build_colour_order();
     sor_prox();
     cache_impulses();
+    cache_joint_impulses();
     counters.velocity_resolution += now_s() - t0;
     t0 = now_s();
     for (auto& b : bodies) {  // update_velocities_and_integrate (:328-347), over the awake bodies
