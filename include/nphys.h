@@ -212,6 +212,20 @@ int nphys_get_params(const nphys_world* w, nphys_params* out);
  * density (volumetric.rs:46-59), proxy creation. *first_handle receives the first new index. */
 int nphys_add_bodies(nphys_world* w, const nphys_body_desc* desc, int32_t* first_handle);
 int nphys_num_bodies(const nphys_world* w, int32_t* n);
+/* DefaultJointConstraintSet::insert (src/joint/joint_constraint.rs:86-107) of n joints
+ * (nphys_joint_desc); joint handles are dense indices in insertion order, the first written to
+ * *first_handle. Joints are solved with the contacts from the next step on: their rows before the
+ * contacts' in every sweep, their position constraints before the contacts' in every position
+ * iteration, and joints that are not broken join their bodies' activation islands. */
+int nphys_add_joints(nphys_world* w, const nphys_joint_desc* desc, int32_t* first_handle);
+/* JointConstraintSet::remove: the joint takes no further part (its handle stays valid). */
+int nphys_remove_joint(nphys_world* w, int32_t joint);
+int nphys_num_joints(const nphys_world* w, int32_t* n);
+/* Impulses cached by each joint's last solve (JointConstraint::cache_impulses): linear [n][3]
+ * along world x, y, z; angular [n][3] (fixed: about x, y, z; revolute: the two directions
+ * orthogonal to the axis, then 0); broken [n] (set once a break force or torque was exceeded).
+ * Any of the pointers may be NULL. */
+int nphys_read_joints(nphys_world* w, float* lin_impulses, float* ang_impulses, int32_t* broken, int64_t cap);
 
 /* RigidBody::set_position / set_velocity for every body (rigid_body.rs:305-324); marks the
  * bodies moved so the next step or nphys_collide re-runs collision detection. Static bodies

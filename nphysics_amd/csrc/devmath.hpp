@@ -175,6 +175,70 @@ NPH_HD q4 qexp_scaled_axis(f3 axisangle) {
   return q4{h.x * s, h.y * s, h.z * s, c};
 }
 
+// atan(x) for 0 <= x <= 1 by plain arithmetic (the CPU restatement evaluates the same
+// expression tree): three half-angle reductions x -> x / (1 + sqrt(1 + x^2)), then the Maclaurin
+// series of atan through x^19 in Horner form, times 8.
+NPH_HD float atan_unit(float x) {
+  for (int i = 0; i < 3; ++i) x = x / (1.0f + sqrtf(1.0f + x * x));
+  const float x2 = x * x;
+  float p = float(-1.0 / 19.0);
+  p = float(1.0 / 17.0) + x2 * p;
+  p = float(-1.0 / 15.0) + x2 * p;
+  p = float(1.0 / 13.0) + x2 * p;
+  p = float(-1.0 / 11.0) + x2 * p;
+  p = float(1.0 / 9.0) + x2 * p;
+  p = float(-1.0 / 7.0) + x2 * p;
+  p = float(1.0 / 5.0) + x2 * p;
+  p = float(-1.0 / 3.0) + x2 * p;
+  p = 1.0f + x2 * p;
+  return 8.0f * (x * p);
+}
+
+// atan2(y, x) for y >= 0, in [0, pi].
+NPH_HD float angle_of(float y, float x) {
+  const float pi = float(3.14159265358979323846);
+  const float ax = fabsf(x);
+  if (y == 0.0f && ax == 0.0f) return 0.0f;
+  const float base = ax >= y ? atan_unit(y / ax) : pi * 0.5f - atan_unit(ax / y);
+  return x >= 0.0f ? base : pi - base;
+}
+
+NPH_HD q4 qconj(q4 q) { return q4{-q.i, -q.j, -q.k, q.w}; }
+
+// Scaled axis (axis times angle) of a unit quaternion: the imaginary part, negated when w < 0,
+// normalised, times 2 atan2(|imag|, |w|); zero when |imag| is at most FLT_EPSILON.
+NPH_HD f3 q_scaled_axis(q4 q) {
+  f3 v{q.i, q.j, q.k};
+  if (q.w < 0.0f) v = neg(v);
+  const float s = sqrtf(dot(v, v));
+  if (s <= 1.1920928955078125e-07f) return f3{0.0f, 0.0f, 0.0f};
+  const float ang = 2.0f * angle_of(s, fabsf(q.w));
+  return f3{v.x / s * ang, v.y / s * ang, v.z / s * ang};
+}
+
+// Scaled axis of the shortest rotation taking unit axis a1 to unit axis a2: (a1 x a2) normalised
+// times atan2(|a1 x a2|, a1 . a2); for (anti)parallel axes zero, or for opposite axes e_imin x a1
+// normalised times pi, imin the index of a1's smallest |component|.
+NPH_HD f3 axis_alignment_error(f3 a1, f3 a2) {
+  const f3 c = cross(a1, a2);
+  const float sn = sqrtf(dot(c, c));
+  const float cs = dot(a1, a2);
+  if (sn > 1.1920928955078125e-07f) {
+    const float ang = angle_of(sn, cs);
+    return f3{c.x / sn * ang, c.y / sn * ang, c.z / sn * ang};
+  }
+  if (cs >= 0.0f) return f3{0.0f, 0.0f, 0.0f};
+  int imin = 0;
+  if (fabsf(a1.y) < fabsf(comp(a1, imin))) imin = 1;
+  if (fabsf(a1.z) < fabsf(comp(a1, imin))) imin = 2;
+  f3 e{0.0f, 0.0f, 0.0f};
+  set_comp(e, imin, 1.0f);
+  const f3 d = cross(e, a1);
+  const float l = sqrtf(dot(d, d));
+  const float pi = float(3.14159265358979323846);
+  return f3{d.x / l * pi, d.y / l * pi, d.z / l * pi};
+}
+
 // Pose update of apply_displacement: T(com) * Iso(lin, exp(ang)) * T(-com) * pose, com = t.
 NPH_HD void apply_displacement(f3& t, q4& q, f3 lin, f3 ang) {
   const q4 qd = qexp_scaled_axis(ang);

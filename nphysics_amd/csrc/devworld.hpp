@@ -27,6 +27,7 @@ struct SimParams {
   float dt, inv_dt, erp, warmstart_coeff, restitution_threshold;
   float allowed_linear_error, max_linear_correction;
   float gx, gy, gz;
+  float allowed_angular_error, max_angular_correction;
 };
 
 // float4 per pose block and per solver block

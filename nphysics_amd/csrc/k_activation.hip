@@ -233,10 +233,12 @@ void launch_wake(const DevBodies& b, hipStream_t s) {
   if (b.n > 0) hipLaunchKernelGGL(k_act_wake, dim3(nb(b.n)), dim3(256), 0, s, b);
 }
 
-void launch_activation(const DevBodies& b, const DevContacts& c, int M, const ActivationBufs& a, hipStream_t s) {
+void launch_activation(const DevBodies& b, const DevContacts& c, int M, const ActivationBufs& a, const JointBufs* j,
+                       hipStream_t s) {
   if (b.n > 0) {
     hipLaunchKernelGGL(k_act_energy, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can, a.touch);
     if (M > 0) hipLaunchKernelGGL(k_act_union, dim3(nb(M)), dim3(256), 0, s, b, c, M, a.parent, a.touch);
+    if (j) launch_joint_union(b, *j, a.parent, a.touch, s);  // joints join islands (:154-159)
     hipLaunchKernelGGL(k_act_flags, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can, a.touch);
     hipLaunchKernelGGL(k_act_apply, dim3(nb(b.n)), dim3(256), 0, s, b, a.parent, a.can, a.totals + 2);
   }

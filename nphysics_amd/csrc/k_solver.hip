@@ -991,10 +991,10 @@ void launch_fix_colour_starts(int ncolours, int M, int C, SolverBufs& sb, uint32
   hipLaunchKernelGGL(k_fix_colour_starts, dim3(1), dim3(1), 0, s, ncolours, M, C, sb.col_start, sb.soff, class_rec);
 }
 
-// One lane per manifold: one 64-lane workgroup per 64 manifolds of the largest colour class,
-// rounded up to a power of two (so the launch shape, and with it the captured sweep graph,
-// changes only when the largest class doubles or halves), at least 64 and at most 8,192;
-// larger classes are walked by the grid-stride loop.
+// One lane per manifold: the cap on a colour launch's grid, one 64-lane workgroup per 64
+// manifolds of the largest colour class rounded up to a power of two, at least 64 and at most
+// 8,192; each class is launched with the workgroups it needs under that cap, and larger classes
+// are walked by the grid-stride loop.
 int sweep_blocks(int max_class) {
   int need = (max_class + 63) / 64;
   int blocks = 64;

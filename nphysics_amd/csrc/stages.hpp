@@ -25,7 +25,10 @@ struct ActivationBufs {
   uint32_t* totals;  // [3] solved manifolds, their contacts, awake dynamic bodies (zeroed by the caller)
 };
 void launch_wake(const DevBodies& b, hipStream_t s);
-void launch_activation(const DevBodies& b, const DevContacts& c, int M, const ActivationBufs& a, hipStream_t s);
+struct JointBufs;
+// j (may be null): joints whose bodies join the islands
+void launch_activation(const DevBodies& b, const DevContacts& c, int M, const ActivationBufs& a, const JointBufs* j,
+                       hipStream_t s);
 void launch_solve_list(const DevContacts& c, int M, const ActivationBufs& a, const DevContacts& out,
                        const DevContacts& prev, int prevM, hipStream_t s);
 
@@ -174,6 +177,42 @@ void launch_features(const DevBodies& b, const DevContacts& c, const SimParams& 
 void launch_apply_prediction(SolverBufs& sb, int C, const float* y, const float* ls, hipStream_t s);
 void launch_sample_targets(SolverBufs& sb, int C, const float* ls, float* t, hipStream_t s);
 void launch_lam_copy(SolverBufs& sb, int C, float4* buf, bool to_buf, hipStream_t s);
+
+// k_joints.hip -----------------------------------------------------------------------------
+// Joint constraints (ball, fixed, revolute): static SoA per joint in joint order, a fixed greedy
+// colouring (joints sharing a dynamic body differ in colour) with the joints listed colour by
+// colour, and per solve one row record per joint at its position in that list.
+enum { JOINT_BALL = 0, JOINT_FIXED = 1, JOINT_REVOLUTE = 2 };
+constexpr int kJointRec = 10;  // float4 per joint row record
+struct JointBufs {
+  int nj;                // joints
+  int ncol;              // joint colours
+  const int4* info;      // [nj] (kind, body1, body2, colour)
+  const float4* a1;      // [nj] local anchors
+  const float4* a2;
+  const float4* f1;      // [nj] fixed: reference frames (i, j, k, w); revolute: local axes (xyz)
+  const float4* f2;
+  const float2* brk;     // [nj] squared break force, squared break torque
+  float4* imp_lin;       // [nj] impulses cached by the last solve
+  float4* imp_ang;
+  int* state;            // [nj] bit 0 broken, bit 1 removed
+  const int* order;      // [nj] joints colour by colour (stable in joint order)
+  const int2* crange;    // [ncol] (first position in order, joints)
+  // per solve, at the joint's position in `order`: (p1 | rows), (p2 | active), 6 x (dir | r),
+  // (rhs 0-3), (rhs 4-5, -, -); impulses of rows 0-3 and 4-5
+  float4* rec;
+  float4* lam;
+};
+void launch_joint_assemble(const DevBodies& b, const SimParams& p, JointBufs& j, hipStream_t s);
+// one workgroup over every joint colour in order (small joint sets), or one launch per colour
+void launch_joint_warm(const DevBodies& b, JointBufs& j, hipStream_t s);
+void launch_joint_sweep(const DevBodies& b, JointBufs& j, hipStream_t s);
+void launch_joint_position(const DevBodies& b, const SimParams& p, JointBufs& j, hipStream_t s);
+void launch_joint_cache(const DevBodies& b, const SimParams& p, JointBufs& j, hipStream_t s);
+// residual of the joint rows (phi = w): (max, sum of squares, rows) into out[0..2]
+void launch_joint_residual(const DevBodies& b, JointBufs& j, double* out, hipStream_t s);
+// union of the bodies of every joint neither broken nor removed (activation islands)
+void launch_joint_union(const DevBodies& b, const JointBufs& j, int* parent, uint32_t* touch, hipStream_t s);
 
 // k_position.hip ---------------------------------------------------------------------------
 void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k,

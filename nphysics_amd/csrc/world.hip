@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cfloat>
 #include <chrono>
 #include <cmath>
 #include <initializer_list>
@@ -202,6 +203,16 @@ struct World {
   std::vector<int> last_colours;
   bool keep_colours = false;
 
+  // joints: static per-joint arrays (host copies in joint order), their greedy colouring with the
+  // joints listed colour by colour, per-solve row records and impulses, and the judge's saved ones
+  std::vector<int4> h_jinfo;
+  std::vector<float4> h_ja1, h_ja2, h_jf1, h_jf2;
+  std::vector<float2> h_jbrk;
+  std::vector<int> h_jorder;
+  std::vector<int2> h_jcrange;
+  int nj = 0, njcol = 0;
+  DBuf j_info, j_a1, j_a2, j_f1, j_f2, j_brk, j_imp_lin, j_imp_ang, j_state, j_order, j_crange, j_rec, j_lam, jd_jlam;
+
   // Readback buffer (pinned, device-mapped, coherent): words [0, 64) small per-step values,
   // [kRbClass, +2 (kMaxColours + 2)) the class offsets, kRbFlag the sequence flag.
   static constexpr int kRbClass = 64, kRbFlag = kRbClass + 2 * (kMaxColours + 2) + 64, kRbWords = kRbFlag + 64;
@@ -241,6 +252,8 @@ struct World {
     sp.restitution_threshold = float(p.restitution_velocity_threshold);
     sp.allowed_linear_error = float(p.allowed_linear_error);
     sp.max_linear_correction = float(p.max_linear_correction);
+    sp.allowed_angular_error = float(p.allowed_angular_error);
+    sp.max_angular_correction = float(p.max_angular_correction);
     sp.gx = float(p.gravity[0]);
     sp.gy = float(p.gravity[1]);
     sp.gz = float(p.gravity[2]);
@@ -312,6 +325,166 @@ struct World {
   }
 
   // ------------------------------------------------------------------ scene
+  JointBufs joint_view() {
+    JointBufs j;
+    j.nj = nj;
+    j.ncol = njcol;
+    j.info = j_info.as<int4>();
+    j.a1 = j_a1.as<float4>();
+    j.a2 = j_a2.as<float4>();
+    j.f1 = j_f1.as<float4>();
+    j.f2 = j_f2.as<float4>();
+    j.brk = j_brk.as<float2>();
+    j.imp_lin = j_imp_lin.as<float4>();
+    j.imp_ang = j_imp_ang.as<float4>();
+    j.state = j_state.as<int>();
+    j.order = j_order.as<int>();
+    j.crange = j_crange.as<int2>();
+    j.rec = j_rec.as<float4>();
+    j.lam = j_lam.as<float4>();
+    return j;
+  }
+
+  // Greedy colouring of the joints in joint order: a joint takes the smallest colour that no
+  // earlier joint sharing one of its dynamic bodies has (the CPU restatement colours the same way).
+  void colour_joints() {
+    std::vector<std::vector<int>> used(size_t(std::max(n, 1)));
+    njcol = 0;
+    for (int ji = 0; ji < nj; ++ji) {
+      int4& inf = h_jinfo[size_t(ji)];
+      std::vector<char> taken;
+      for (int bi : {inf.y, inf.z}) {
+        if (h_status[size_t(bi)] != NPHYS_STATUS_DYNAMIC) continue;
+        for (int c : used[size_t(bi)]) {
+          if (int(taken.size()) <= c) taken.resize(size_t(c) + 1, 0);
+          taken[size_t(c)] = 1;
+        }
+      }
+      int c = 0;
+      while (c < int(taken.size()) && taken[size_t(c)]) ++c;
+      inf.w = c;
+      njcol = std::max(njcol, c + 1);
+      for (int bi : {inf.y, inf.z})
+        if (h_status[size_t(bi)] == NPHYS_STATUS_DYNAMIC) used[size_t(bi)].push_back(c);
+    }
+    h_jorder.clear();
+    h_jcrange.assign(size_t(njcol), int2{0, 0});
+    for (int c = 0; c < njcol; ++c) {
+      h_jcrange[size_t(c)].x = int(h_jorder.size());
+      for (int ji = 0; ji < nj; ++ji)
+        if (h_jinfo[size_t(ji)].w == c) h_jorder.push_back(ji);
+      h_jcrange[size_t(c)].y = int(h_jorder.size()) - h_jcrange[size_t(c)].x;
+    }
+  }
+
+  // JointConstraintSet::insert for every joint of the descriptor (ball_constraint.rs:27-48,
+  // fixed_constraint.rs:28-58, revolute_constraint.rs:37-72).
+  int add_joints(const nphys_joint_desc& d, int32_t* first) {
+    if (d.n < 0 || (d.n > 0 && (!d.kind || !d.body1 || !d.body2 || !d.anchor1 || !d.anchor2)))
+      return set_err(NPHYS_ERR_INVALID, "bad joint descriptor");
+    for (int i = 0; i < d.n; ++i) {
+      const int b1 = d.body1[i], b2 = d.body2[i];
+      if (b1 < 0 || b1 >= n || b2 < 0 || b2 >= n || b1 == b2) return set_err(NPHYS_ERR_INVALID, "bad joint body");
+      if (d.kind[i] < NPHYS_JOINT_BALL || d.kind[i] > NPHYS_JOINT_REVOLUTE)
+        return set_err(NPHYS_ERR_INVALID, "bad joint kind");
+      if (h_status[size_t(b1)] != NPHYS_STATUS_DYNAMIC && h_status[size_t(b2)] != NPHYS_STATUS_DYNAMIC)
+        return set_err(NPHYS_ERR_INVALID, "a joint needs a dynamic body");
+    }
+    if (first) *first = nj;
+    if (d.n == 0) return NPHYS_OK;
+    HIPCK(hipStreamSynchronize(stream));
+    const int old = nj;
+    for (int i = 0; i < d.n; ++i) {
+      h_jinfo.push_back(make_int4(d.kind[i], d.body1[i], d.body2[i], 0));
+      h_ja1.push_back(make_float4(d.anchor1[3 * i], d.anchor1[3 * i + 1], d.anchor1[3 * i + 2], 0.0f));
+      h_ja2.push_back(make_float4(d.anchor2[3 * i], d.anchor2[3 * i + 1], d.anchor2[3 * i + 2], 0.0f));
+      float4 f1 = make_float4(0.0f, 0.0f, 0.0f, 1.0f), f2 = f1;
+      if (d.kind[i] == NPHYS_JOINT_FIXED) {
+        if (d.frame1) f1 = make_float4(d.frame1[4 * i], d.frame1[4 * i + 1], d.frame1[4 * i + 2], d.frame1[4 * i + 3]);
+        if (d.frame2) f2 = make_float4(d.frame2[4 * i], d.frame2[4 * i + 1], d.frame2[4 * i + 2], d.frame2[4 * i + 3]);
+      } else if (d.kind[i] == NPHYS_JOINT_REVOLUTE) {
+        f1 = d.axis1 ? make_float4(d.axis1[3 * i], d.axis1[3 * i + 1], d.axis1[3 * i + 2], 0.0f)
+                     : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+        f2 = d.axis2 ? make_float4(d.axis2[3 * i], d.axis2[3 * i + 1], d.axis2[3 * i + 2], 0.0f)
+                     : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
+      }
+      h_jf1.push_back(f1);
+      h_jf2.push_back(f2);
+      const float bf = d.break_force ? d.break_force[i] : -1.0f;
+      const float bt = d.break_torque ? d.break_torque[i] : -1.0f;
+      h_jbrk.push_back(make_float2(bf >= 0.0f ? bf * bf : FLT_MAX, bt >= 0.0f ? bt * bt : FLT_MAX));
+    }
+    nj += d.n;
+    colour_joints();
+    const size_t J = size_t(nj);
+    HIPCK(j_info.ensure(J * sizeof(int4)));
+    HIPCK(j_a1.ensure(J * sizeof(float4)));
+    HIPCK(j_a2.ensure(J * sizeof(float4)));
+    HIPCK(j_f1.ensure(J * sizeof(float4)));
+    HIPCK(j_f2.ensure(J * sizeof(float4)));
+    HIPCK(j_brk.ensure(J * sizeof(float2)));
+    HIPCK(j_order.ensure(J * sizeof(int)));
+    HIPCK(j_crange.ensure(size_t(std::max(njcol, 1)) * sizeof(int2)));
+    HIPCK(j_rec.ensure(J * kJointRec * sizeof(float4)));
+    HIPCK(j_lam.ensure(J * 2 * sizeof(float4)));
+    // cached impulses and broken / removed flags live on the device: kept, the new ones cleared
+    HIPCK(j_imp_lin.ensure(J * sizeof(float4), true, stream));
+    HIPCK(j_imp_ang.ensure(J * sizeof(float4), true, stream));
+    HIPCK(j_state.ensure(J * sizeof(int), true, stream));
+    const size_t add = size_t(d.n);
+    HIPCK(hipMemsetAsync(j_imp_lin.as<float4>() + old, 0, add * sizeof(float4), stream));
+    HIPCK(hipMemsetAsync(j_imp_ang.as<float4>() + old, 0, add * sizeof(float4), stream));
+    HIPCK(hipMemsetAsync(j_state.as<int>() + old, 0, add * sizeof(int), stream));
+    HIPCK(hipMemcpyAsync(j_info.p, h_jinfo.data(), J * sizeof(int4), hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(j_a1.p, h_ja1.data(), J * sizeof(float4), hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(j_a2.p, h_ja2.data(), J * sizeof(float4), hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(j_f1.p, h_jf1.data(), J * sizeof(float4), hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(j_f2.p, h_jf2.data(), J * sizeof(float4), hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(j_brk.p, h_jbrk.data(), J * sizeof(float2), hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(j_order.p, h_jorder.data(), J * sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCK(hipMemcpyAsync(j_crange.p, h_jcrange.data(), size_t(njcol) * sizeof(int2), hipMemcpyHostToDevice, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    return NPHYS_OK;
+  }
+
+  // JointConstraintSet::remove: the joint no longer takes part in the solve or the islands.
+  int remove_joint(int ji) {
+    if (ji < 0 || ji >= nj) return set_err(NPHYS_ERR_INVALID, "bad joint handle");
+    int st = 0;
+    HIPCK(hipMemcpyAsync(&st, j_state.as<int>() + ji, sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    st |= 2;
+    HIPCK(hipMemcpyAsync(j_state.as<int>() + ji, &st, sizeof(int), hipMemcpyHostToDevice, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    return NPHYS_OK;
+  }
+
+  int read_joints(float* lin, float* ang, int32_t* broken, int64_t cap) {
+    if (cap < nj) return set_err(NPHYS_ERR_CAPACITY, "joint buffers too small");
+    if (nj == 0) return NPHYS_OK;
+    const size_t J = static_cast<size_t>(nj);
+    std::vector<float4> l(J), a(J);
+    std::vector<int> st(J);
+    HIPCK(hipMemcpyAsync(l.data(), j_imp_lin.p, J * sizeof(float4), hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(a.data(), j_imp_ang.p, J * sizeof(float4), hipMemcpyDeviceToHost, stream));
+    HIPCK(hipMemcpyAsync(st.data(), j_state.p, J * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    for (int i = 0; i < nj; ++i) {
+      if (lin) {
+        lin[3 * i] = l[size_t(i)].x;
+        lin[3 * i + 1] = l[size_t(i)].y;
+        lin[3 * i + 2] = l[size_t(i)].z;
+      }
+      if (ang) {
+        ang[3 * i] = a[size_t(i)].x;
+        ang[3 * i + 1] = a[size_t(i)].y;
+        ang[3 * i + 2] = a[size_t(i)].z;
+      }
+      if (broken) broken[i] = st[size_t(i)] & 1;
+    }
+    return NPHYS_OK;
+  }
+
   int add_bodies(const nphys_body_desc& d, int32_t* first) {
     if (d.n < 0) return set_err(NPHYS_ERR_INVALID, "negative body count");
     if (!d.status || !d.shape || !d.dims || !d.pos || !d.rot || !d.density || !d.margin ||
@@ -878,7 +1051,7 @@ struct World {
     HIPCK(sv_pgeo.ensure(m * 2 * sizeof(float4)));
     rec_stride = long(std::min({sv_rec.bytes / (kRecFields * sizeof(float4)), sv_lam.bytes / sizeof(float4),
                                 sv_prec.bytes / (3 * sizeof(float4))}));
-    HIPCK(sv_red.ensure(size_t(residual_blocks(M)) * 2 * sizeof(double)));
+    HIPCK(sv_red.ensure((size_t(residual_blocks(M)) * 2 + 4) * sizeof(double)));  // + the joints' partial
     const size_t tw = std::max({radix_temp_words(int(m)), scan_temp_words(int(m)), scan_temp_words(int(nn))});
     HIPCK(sv_tmp.ensure(tw * 4 + 64));
     return NPHYS_OK;
@@ -1041,21 +1214,33 @@ struct World {
           k = e;
           continue;
         }
+        // one 64-lane workgroup per 64 manifolds of this class (the grid-stride loop covers the
+        // rest above the cap): no idle workgroups for the smaller classes
+        const int cblocks = std::min(sblocks, std::max(1, (h_range[2 * k + 1] + 63) / 64));
         if (w) {
-          launch_warmstart_colour(b, cv, k, sblocks, v, stream);
+          launch_warmstart_colour(b, cv, k, cblocks, v, stream);
         } else if (counters_enabled) {
-          launch_pgs_colour(b, cv, k, sblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
+          launch_pgs_colour(b, cv, k, cblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
           kt_used += 2;
           const int c = h_visit[k];
           counters.sweep_kernel_contacts += int64_t(class_rec[c + 1]) - int64_t(class_rec[c]);
         } else {
-          launch_pgs_colour(b, cv, k, sblocks, v, stream);
+          launch_pgs_colour(b, cv, k, cblocks, v, stream);
         }
         ++k;
       }
     };
-    if (warm) pass(true);
-    for (int it = 0; it < nsweeps; ++it) pass(false);
+    // the joints' rows: warm-started after the contacts' (sor_prox.rs:56-58), solved before the
+    // contacts' in every sweep (sor_prox.rs:167-168)
+    JointBufs jv = joint_view();
+    if (warm) {
+      pass(true);
+      if (nj > 0) launch_joint_warm(b, jv, stream);
+    }
+    for (int it = 0; it < nsweeps; ++it) {
+      if (nj > 0) launch_joint_sweep(b, jv, stream);
+      pass(false);
+    }
     return NPHYS_OK;
   }
 
@@ -1065,7 +1250,9 @@ struct World {
   int residual_deferred(const DevBodies& b, const DevContacts& cv, int M, int C, SolverBufs& v) {
     launch_residual(b, cv, M, v, stream);
     const int nbk = residual_blocks(M);
-    const size_t need = size_t(nbk) * 2;
+    JointBufs jv = joint_view();
+    if (nj > 0) launch_joint_residual(b, jv, v.red + 2 * nbk, stream);
+    const size_t need = size_t(nbk) * 2 + (nj > 0 ? 3 : 0);
     if (need > red_pinned_cap) {
       HIPCK(hipStreamSynchronize(stream));  // the previous buffer may be the target of a copy
       if (red_pinned) (void)hipHostFree(red_pinned);
@@ -1077,19 +1264,27 @@ struct World {
     HIPCK(hipMemcpyAsync(red_pinned, v.red, need * sizeof(double), hipMemcpyDeviceToHost, stream));
     res_pending_blocks = nbk;
     res_pending_C = C;
+    res_pending_joints = nj > 0;
     return NPHYS_OK;
   }
+  bool res_pending_joints = false;
 
   // Reduces a deferred residual; the stream must have passed its copy.
   void finish_residual() {
     if (res_pending_blocks < 0) return;
-    double mx = 0.0, ss = 0.0;
+    double mx = 0.0, ss = 0.0, rows = double(3 * res_pending_C);
     for (int k = 0; k < res_pending_blocks; ++k) {
       mx = std::max(mx, red_pinned[2 * k]);
       ss += red_pinned[2 * k + 1];
     }
+    if (res_pending_joints) {  // the joint rows' partial after the contacts' blocks
+      const double* jp = red_pinned + 2 * res_pending_blocks;
+      mx = std::max(mx, jp[0]);
+      ss += jp[1];
+      rows += jp[2];
+    }
     res_inf = mx;
-    res_rms = res_pending_C > 0 ? std::sqrt(ss / double(3 * res_pending_C)) : 0.0;
+    res_rms = rows > 0.0 ? std::sqrt(ss / rows) : 0.0;
     res_pending_blocks = -1;
   }
 
@@ -1098,16 +1293,24 @@ struct World {
   int residual(const DevBodies& b, const DevContacts& cv, int M, int C, SolverBufs& v) {
     launch_residual(b, cv, M, v, stream);
     const int nbk = residual_blocks(M);
-    red_host.resize(size_t(nbk) * 2);
-    HIPCK(hipMemcpyAsync(red_host.data(), v.red, red_host.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+    JointBufs jv = joint_view();
+    if (nj > 0) launch_joint_residual(b, jv, v.red + 2 * nbk, stream);
+    red_host.resize(size_t(nbk) * 2 + 3);
+    HIPCK(hipMemcpyAsync(red_host.data(), v.red, (size_t(nbk) * 2 + (nj > 0 ? 3 : 0)) * sizeof(double),
+                         hipMemcpyDeviceToHost, stream));
     HIPCK(hipStreamSynchronize(stream));
-    double mx = 0.0, ss = 0.0;
+    double mx = 0.0, ss = 0.0, rows = double(3 * C);
     for (int k = 0; k < nbk; ++k) {
       mx = std::max(mx, red_host[2 * k]);
       ss += red_host[2 * k + 1];
     }
+    if (nj > 0) {
+      mx = std::max(mx, red_host[2 * nbk]);
+      ss += red_host[2 * nbk + 1];
+      rows += red_host[2 * nbk + 2];
+    }
     res_inf = mx;
-    res_rms = C > 0 ? std::sqrt(ss / double(3 * C)) : 0.0;
+    res_rms = rows > 0.0 ? std::sqrt(ss / rows) : 0.0;
     return NPHYS_OK;
   }
 
@@ -1136,7 +1339,8 @@ struct World {
   // candidate to time. All variants give identical results.
   void choose_variant(int M) {
     tune_pending = false;
-    if (!blocked_eligible(ncolours)) {
+    // the scene-blocked kernels solve contacts only: worlds with joints use per-colour launches
+    if (nj > 0 || !blocked_eligible(ncolours)) {
       cur_variant = 0;
       return;
     }
@@ -1239,7 +1443,8 @@ struct World {
     a.totals = bp_small.as<uint32_t>() + 48;
     fill_u32(a.totals, 0u, 3, stream);
     const DevContacts full = cc.view();
-    launch_activation(b, full, cc.M, a, stream);
+    const JointBufs jv = joint_view();
+    launch_activation(b, full, cc.M, a, nj > 0 ? &jv : nullptr, stream);
     DevContacts out = full;
     out.bfirst = nullptr;  // the solved list has no per-body index
     out.bcount = 0;
@@ -1278,7 +1483,8 @@ struct World {
     solved_C = C;
     res_inf = res_rms = 0.0;
     ncolours = 0;
-    if (M == 0) {
+    max_class = 0;
+    if (M == 0 && nj == 0) {
       last_sweeps = res_target > 0.0 ? 0 : prm.max_velocity_iterations;
       counters.velocity_sweeps += last_sweeps;
       HIPCK(hipEventRecord(ev[EV_ASM], stream));
@@ -1288,11 +1494,17 @@ struct World {
     int rc = ensure_solver(M, C);
     if (rc) return rc;
     SolverBufs v = sv_view();
-    if ((rc = colour(b, cv, M, C, v))) return rc;
-    launch_assemble(b, cv, pc.view(), prevM, M, sp, v, mat_table(), stream);
+    if (M > 0) {
+      if ((rc = colour(b, cv, M, C, v))) return rc;
+      launch_assemble(b, cv, pc.view(), prevM, M, sp, v, mat_table(), stream);
+    }
+    // the joints' rows (moreau_jean_solver.rs:211-223); the island joints are those neither broken
+    // nor removed with an awake dynamic body, decided on the device
+    JointBufs jv = joint_view();
+    if (nj > 0) launch_joint_assemble(b, sp, jv, stream);
     HIPCK(hipEventRecord(ev[EV_ASM], stream));
     // class sizes (read back while the assembly runs) give the visit order and the sweep grids
-    if ((rc = visit_order())) return rc;
+    if (M > 0 && (rc = visit_order())) return rc;
     const int sblocks = sweep_blocks(max_class);
     kt_used = 0;
     choose_variant(M);
@@ -1306,8 +1518,8 @@ struct World {
       SceneBlocks sc = sc_view();
       launch_scene_order(b, M, nscenes, ncolours, v, sc, stream);
     }
-    const bool learned = contact_model == NPHYS_CONTACT_MODEL_LEARNED_WARMSTART;
-    if (learned || capture) {
+    const bool learned = contact_model == NPHYS_CONTACT_MODEL_LEARNED_WARMSTART && C > 0;
+    if ((learned || capture) && C > 0) {
       HIPCK(ml_X.ensure(size_t(C) * kFeat * sizeof(float)));
       HIPCK(ml_ls.ensure(size_t(C) * sizeof(float)));
       launch_features(b, cv, sp, M, v, ml_X.as<float>(), ml_ls.as<float>(), stream);
@@ -1321,10 +1533,20 @@ struct World {
       HIPCK(jd_dv.ensure(size_t(n) * 2 * sizeof(float4)));
       int n_plain = 0, n_learned = 0;
       launch_lam_copy(v, C, jd_lam0.as<float4>(), true, stream);  // the cache warm start
+      // the joints' rows start from their cached impulses in both solves
+      const size_t jl = size_t(nj) * 2 * sizeof(float4);
+      if (nj > 0) {
+        HIPCK(jd_jlam.ensure(2 * jl));
+        HIPCK(hipMemcpyAsync(jd_jlam.p, j_lam.p, jl, hipMemcpyDeviceToDevice, stream));
+      }
       if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, n_plain))) return rc;
       const double rinf_p = res_inf, rrms_p = res_rms;
       launch_lam_copy(v, C, jd_lam.as<float4>(), true, stream);
       launch_lam_copy(v, C, jd_lam0.as<float4>(), false, stream);
+      if (nj > 0) {
+        HIPCK(hipMemcpyAsync(jd_jlam.as<char>() + jl, j_lam.p, jl, hipMemcpyDeviceToDevice, stream));
+        HIPCK(hipMemcpyAsync(j_lam.p, jd_jlam.p, jl, hipMemcpyDeviceToDevice, stream));
+      }
       // the velocity increment is the last 24 bytes of each 64-byte solver block
       char* dv0 = sol.as<char>() + 10 * sizeof(float);
       const size_t pitch = kSolverBlock * sizeof(float4), dvb = 6 * sizeof(float);
@@ -1335,6 +1557,7 @@ struct World {
       const bool keep = n_learned < n_plain;
       if (!keep) {
         launch_lam_copy(v, C, jd_lam.as<float4>(), false, stream);
+        if (nj > 0) HIPCK(hipMemcpyAsync(j_lam.p, jd_jlam.as<char>() + jl, jl, hipMemcpyDeviceToDevice, stream));
         HIPCK(hipMemcpy2DAsync(dv0, pitch, jd_dv.p, dvb, dvb, size_t(n), hipMemcpyDeviceToDevice, stream));
         res_inf = rinf_p;
         res_rms = rrms_p;
@@ -1348,15 +1571,16 @@ struct World {
       if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, last_sweeps))) return rc;
     }
     if (tune_pending) HIPCK(hipEventRecord(tune_ev[1], stream));
-    if (capture) {
+    if (capture && C > 0) {
       HIPCK(ml_t.ensure(size_t(C) * kOut * sizeof(float)));
       launch_sample_targets(v, C, ml_ls.as<float>(), ml_t.as<float>(), stream);
       samples = C;
     }
     counters.velocity_sweeps += last_sweeps;
-    launch_cache_impulses(cv, v, C, stream);
+    if (C > 0) launch_cache_impulses(cv, v, C, stream);
+    if (nj > 0) launch_joint_cache(b, sp, jv, stream);
     HIPCK(hipEventRecord(ev[EV_VEL], stream));
-    if (keep_colours) {
+    if (keep_colours && M > 0) {
       std::vector<uint32_t> st(M);
       HIPCK(hipMemcpyAsync(st.data(), sv_state2[cur].p, size_t(M) * 4, hipMemcpyDeviceToHost, stream));
       HIPCK(hipStreamSynchronize(stream));
@@ -1367,15 +1591,19 @@ struct World {
   }
 
   int position_correction() {
-    if (sview_M[cur] == 0) return NPHYS_OK;
+    if (sview_M[cur] == 0 && nj == 0) return NPHYS_OK;
     DevBodies b = bodies();
     SolverBufs v = sv_view();
+    JointBufs jv = joint_view();
     const DevContacts cv = sview[cur];
     const int pblocks = position_blocks(max_class);
-    // direct launches, runs of two or more consecutive small classes in one workgroup (as the sweeps)
+    // direct launches, runs of two or more consecutive small classes in one workgroup (as the
+    // sweeps); the joints' constraints first in every iteration (nonlinear_sor_prox.rs:33-38)
     const int* h_range = h_visit + (kMaxColours + 2);
-    for (int it = 0; it < prm.max_position_iterations; ++it)
-      for (int k = 0; k < ncolours;) {
+    const int ncol = sview_M[cur] > 0 ? ncolours : 0;
+    for (int it = 0; it < prm.max_position_iterations; ++it) {
+      if (nj > 0) launch_joint_position(b, sp, jv, stream);
+      for (int k = 0; k < ncol;) {
         int e = k;
         while (e < ncolours && h_range[2 * e + 1] <= kRunBlock) ++e;
         if (e - k >= 2) {
@@ -1383,9 +1611,11 @@ struct World {
           k = e;
           continue;
         }
-        launch_position_colour(b, cv, sp, k, pblocks, v, stream);
+        launch_position_colour(b, cv, sp, k, std::min(pblocks, std::max(1, (h_range[2 * k + 1] + 63) / 64)), v,
+                               stream);
         ++k;
       }
+    }
     return NPHYS_OK;
   }
 
@@ -1617,6 +1847,27 @@ int nphys_get_params(const nphys_world* w, nphys_params* out) {
 int nphys_add_bodies(nphys_world* w, const nphys_body_desc* desc, int32_t* first_handle) {
   if (!w || !desc) return set_err(NPHYS_ERR_INVALID, "null argument");
   return w->w.add_bodies(*desc, first_handle);
+}
+
+int nphys_add_joints(nphys_world* w, const nphys_joint_desc* desc, int32_t* first_handle) {
+  if (!w || !desc) return set_err(NPHYS_ERR_INVALID, "null argument");
+  return w->w.add_joints(*desc, first_handle);
+}
+
+int nphys_remove_joint(nphys_world* w, int32_t joint) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  return w->w.remove_joint(joint);
+}
+
+int nphys_num_joints(const nphys_world* w, int32_t* n) {
+  if (!w || !n) return set_err(NPHYS_ERR_INVALID, "null argument");
+  *n = w->w.nj;
+  return NPHYS_OK;
+}
+
+int nphys_read_joints(nphys_world* w, float* lin_impulses, float* ang_impulses, int32_t* broken, int64_t cap) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  return w->w.read_joints(lin_impulses, ang_impulses, broken, cap);
 }
 
 int nphys_num_bodies(const nphys_world* w, int32_t* n) {

@@ -31,7 +31,8 @@ EXPORTED = [
     "nphys_mlp_create", "nphys_mlp_destroy", "nphys_mlp_num_params", "nphys_mlp_get_params",
     "nphys_mlp_set_params", "nphys_mlp_set_data", "nphys_mlp_train", "nphys_mlp_forward",
     "nphys_mlp_gradients", "nphys_last_error", "nphys_apply_force", "nphys_apply_force_at_point",
-    "nphys_set_material_coefficient",
+    "nphys_set_material_coefficient", "nphys_add_joints", "nphys_remove_joint", "nphys_num_joints",
+    "nphys_read_joints",
 ]
 
 
@@ -88,6 +89,10 @@ def load_library(path=LIB_PATH):
     L.nphys_apply_force.argtypes = [W, C.c_int32, ip, fp, C.c_int32, C.c_int32]
     L.nphys_apply_force_at_point.argtypes = [W, C.c_int32, ip, fp, fp, C.c_int32, C.c_int32]
     L.nphys_set_material_coefficient.argtypes = [W, C.c_int32, C.c_int32, C.c_int32, C.c_float]
+    L.nphys_add_joints.argtypes = [W, C.POINTER(abi.JointDesc), ip]
+    L.nphys_remove_joint.argtypes = [W, C.c_int32]
+    L.nphys_num_joints.argtypes = [W, ip]
+    L.nphys_read_joints.argtypes = [W, fp, fp, ip, C.c_int64]
     _lib = L
     return L
 
@@ -188,6 +193,28 @@ class MechanicalWorld:
             p = np.ascontiguousarray(points, np.float32).reshape(-1)
             _check(self._L.nphys_apply_force_at_point(self._h, b.shape[0], abi.iptr(b), abi.fptr(f), abi.fptr(p),
                                                       force_type, int(auto_wake_up)))
+
+    # -- joints (DefaultJointConstraintSet, src/joint/joint_constraint.rs:61-107)
+    def add_joints(self, js):
+        """Inserts the joints of an abi.joints() dict; returns the first joint handle."""
+        desc, keep = abi.make_joint_desc(js)
+        first = C.c_int32()
+        _check(self._L.nphys_add_joints(self._h, C.byref(desc), C.byref(first)))
+        return first.value
+
+    def remove_joint(self, j):
+        _check(self._L.nphys_remove_joint(self._h, j))
+
+    def joints(self):
+        """Cached impulses (lin (n, 3), ang (n, 3)) and broken flags of every joint."""
+        n = C.c_int32()
+        _check(self._L.nphys_num_joints(self._h, C.byref(n)))
+        k = max(n.value, 1)
+        lin = np.zeros((k, 3), np.float32)
+        ang = np.zeros((k, 3), np.float32)
+        br = np.zeros(k, np.int32)
+        _check(self._L.nphys_read_joints(self._h, abi.fptr(lin), abi.fptr(ang), abi.iptr(br), k))
+        return {"lin": lin[:n.value], "ang": ang[:n.value], "broken": br[:n.value].astype(bool)}
 
     def set_material_coefficient(self, which, id1, id2, coeff):
         """MaterialsCoefficientsTable: which 0 friction, 1 restitution, for material ids id1, id2."""
