@@ -186,6 +186,10 @@ typedef struct nphys_counters {
   /* accumulated incremental pair updates that overflowed a clean body's candidate segment and
    * fell back to a full build with larger segments */
   int64_t incremental_overflows;
+  /* learned warm start, while counters are enabled: device time of the network's inference
+   * kernel (HIP events around its launch on the world's stream) and the rows it evaluated */
+  double warmstart_inference_time;
+  int64_t warmstart_inference_rows;
 } nphys_counters;
 
 /* ---------------------------------------------------------------- lifecycle */

@@ -163,6 +163,8 @@ class Counters(C.Structure):
         ("incremental_pair_updates", C.c_int64),
         ("active_bodies", C.c_int64),
         ("incremental_overflows", C.c_int64),
+        ("warmstart_inference_time", C.c_double),
+        ("warmstart_inference_rows", C.c_int64),
     ]
 
     def as_dict(self):

@@ -113,10 +113,8 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const SimParams
 }
 
 template <bool kMask>
-__global__ void __launch_bounds__(64) k_position(DevBodies b, SimParams p, SolverBufs sb, int kv) {
-  const int2 range = sb.vrange[kv];
-  const int first_slot = range.x;
-  const int nslots = range.y;
+__global__ void __launch_bounds__(64) k_position(DevBodies b, SimParams p, SolverBufs sb, int first_slot,
+                                                int nslots) {
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nslots; t += gridDim.x * blockDim.x)
     position_one<kMask>(b, p, sb, first_slot + t);
 }
@@ -137,11 +135,11 @@ __global__ void __launch_bounds__(kRunBlock) k_position_run(DevBodies b, SimPara
 // Sized like the sweep grid: from the largest colour class.
 int position_blocks(int max_class) { return sweep_blocks(max_class); }
 
-void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k, int blocks,
-                            SolverBufs& sb, hipStream_t s) {
+void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int first, int n,
+                            int blocks, SolverBufs& sb, hipStream_t s) {
   (void)c;
-  if (b.kmask) hipLaunchKernelGGL(k_position<true>, dim3(blocks), dim3(64), 0, s, b, p, sb, k);
-  else hipLaunchKernelGGL(k_position<false>, dim3(blocks), dim3(64), 0, s, b, p, sb, k);
+  if (b.kmask) hipLaunchKernelGGL(k_position<true>, dim3(blocks), dim3(64), 0, s, b, p, sb, first, n);
+  else hipLaunchKernelGGL(k_position<false>, dim3(blocks), dim3(64), 0, s, b, p, sb, first, n);
 }
 
 void launch_position_run(const DevBodies& b, const SimParams& p, SolverBufs& sb, int kv0, int nk, hipStream_t s) {

@@ -20,12 +20,14 @@
 // of the contact with the same (body1, body2, feature) key (the ImpulseCache lookup at
 // signorini_coulomb_pyramid_model.rs:104-108).
 //
-// Solve (k_warmstart / k_pgs, one lane per manifold of one colour, all of the manifold's data
-// loaded into registers before its serial chain) restates SORProx
-// (src/solver/sor_prox.rs:19-436): warm start dv += l W J, then per sweep the manifold's friction
-// rows (bounded by mu times the current normal impulse, zeroed when it is zero) and then its normal
-// rows (projected on l >= 0). Manifolds of one colour share no dynamic body, so one launch per
-// colour performs exactly the sequential Gauss-Seidel updates of the colour order.
+// Solve (k_warmstart / k_pgs, all of the manifold's data loaded into registers before its serial
+// chain) restates SORProx (src/solver/sor_prox.rs:19-436): warm start dv += l W J, then per sweep
+// and contact the normal row (projected on l >= 0) and the two friction rows (bounded by mu times
+// the normal impulse just computed, zeroed when it is zero). Manifolds of one colour share no
+// dynamic body, so one launch per colour performs exactly the sequential Gauss-Seidel updates of
+// the colour order. k_pgs gives each manifold two lanes, one per body: each lane forms its body's
+// half of a row's J dv and its body's dv update, and the halves meet by a lane exchange, so the
+// serial chain per row is half as long and twice as many lanes are in flight.
 //
 // Row record (4 contiguous float4 per contact, colour order): 0 n|mu, 1 lever arm p1 of body 1
 // (contact point minus centre of mass, zero for a static body)|r_n, 2 p2|r_t1,
@@ -665,21 +667,130 @@ __device__ __forceinline__ void pgs_manifold(const A& acc, const DevBodies& b, c
 
 // One colour of the warm start / of one sweep: one lane per manifold of the colour visited kv-th.
 template <bool kMask>
-__global__ void __launch_bounds__(64) k_warmstart(DevBodies b, DevContacts c, SolverBufs sb, int kv) {
-  int first, n;
-  colour_range(sb, kv, first, n);
+__global__ void __launch_bounds__(64) k_warmstart(DevBodies b, DevContacts c, SolverBufs sb, int first, int n) {
   const GlobalDv acc{b};
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
     warm_manifold<kMask>(acc, b, sb, first + t);
 }
 
+// One SOR-Prox step of one manifold on two lanes (lane h = 0 body 1, h = 1 body 2), the same
+// arithmetic as pgs_manifold: each lane evaluates its body's dot6 of the row, the two partial sums
+// are exchanged between the lanes (adjacent lanes of one wavefront), both lanes form
+// w = (s1 + s2) + rhs in that order and the same new impulse, and each lane updates its own body.
+// A contact's row data as one lane of the pair needs it: the normal, mu, its body's lever arm,
+// the three r and rhs, the impulses.
+struct CHalf {
+  f3 n, p;
+  float mu;
+  float r[3], rhs[3];
+  float4 lam;
+};
+
+__device__ __forceinline__ void load_half(const SolverBufs& sb, uint32_t r, int h, CHalf& c) {
+  const float4* q = sb.rec + long(r) * kRecFields;
+  const float4 v0 = q[0], v1 = q[1], v2 = q[2], v3 = q[3];
+  c.n = xyz(v0);
+  c.mu = v0.w;
+  c.p = h ? xyz(v2) : xyz(v1);
+  c.r[0] = v1.w;
+  c.r[1] = v2.w;
+  c.r[2] = v3.w;
+  c.rhs[0] = v3.x;
+  c.rhs[1] = v3.y;
+  c.rhs[2] = v3.z;
+  c.lam = sb.lam[r];
+}
+
+// One SOR-Prox step of one manifold on two lanes (lane h = 0 body 1, h = 1 body 2), the same
+// arithmetic as pgs_manifold: each lane evaluates its body's dot6 of the row, the two partial sums
+// are exchanged between the lanes (adjacent lanes of one wavefront), both lanes form
+// w = (s1 + s2) + rhs in that order and the same new impulse, and each lane updates its own body.
 template <bool kMask>
-__global__ void __launch_bounds__(64) k_pgs(DevBodies b, DevContacts c, SolverBufs sb, int kv) {
-  int first, n;
-  colour_range(sb, kv, first, n);
-  const GlobalDv acc{b};
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-    pgs_manifold<kMask>(acc, b, sb, first + t);
+__device__ __forceinline__ void pgs_manifold_pair(const DevBodies& b, const SolverBufs& sb, int slot, int h) {
+  const SlotCtx x = slot_ctx(sb, slot);
+  const int body = h ? x.b2 : x.b1;
+  const bool dyn = h ? x.d2 : x.d1;
+  float im = 0.0f;
+  m33 I{};
+  if (dyn) I = load_inv(b, body, im);
+  f3 ml{1.0f, 1.0f, 1.0f}, ma{1.0f, 1.0f, 1.0f};
+  if (kMask) b.mask(body, ml, ma);
+  Dv v{};
+  if (dyn) b.load_dv(body, v.l, v.a);
+  CHalf c[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    if (k < x.cnt) load_half(sb, x.r0 + k, h, c[k]);
+  // body 1 takes d and p1 x d, body 2 -d and p2 x -d (ang1 / ang2), each masked
+  auto half = [&](f3 p, f3 d, f3& d_h, f3& a_h, f3& wa_h) {
+    const f3 dd = h ? neg(d) : d;
+    const f3 a = cross(p, dd);
+    d_h = kMask ? cmul(dd, ml) : dd;
+    a_h = kMask ? cmul(a, ma) : a;
+    wa_h = mulv(I, a_h);
+  };
+  auto w_of = [&](const f3& d_h, const f3& a_h, float rhs) {
+    const float s_h = dot6(d_h, a_h, v);
+    const float s_o = __shfl_xor(s_h, 1);
+    const float s1 = h == 0 ? s_h : s_o;
+    const float s2 = h == 0 ? s_o : s_h;
+    return (s1 + s2) + rhs;
+  };
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k >= x.cnt) break;
+    {
+      f3 d_h, a_h, wa_h;
+      half(c[k].p, neg(c[k].n), d_h, a_h, wa_h);
+      const float w = w_of(d_h, a_h, c[k].rhs[0]);
+      const float a = c[k].lam.x - c[k].r[0] * w;
+      const float ni = a > 0.0f ? a : 0.0f;
+      const float dl = ni - c[k].lam.x;
+      c[k].lam.x = ni;
+      axpy6(dl, scale(d_h, im), wa_h, v);
+    }
+    const float mu = c[k].mu;
+    const float ln = c[k].lam.x;
+    f3 t[2];
+    tangents(c[k].n, t[0], t[1]);
+#pragma unroll
+    for (int f = 1; f <= 2; ++f) {
+      float li = f == 1 ? c[k].lam.y : c[k].lam.z;
+      f3 d_h, a_h, wa_h;
+      half(c[k].p, t[f - 1], d_h, a_h, wa_h);
+      if (ln == 0.0f) {
+        if (li != 0.0f) {
+          axpy6(-li, scale(d_h, im), wa_h, v);
+          li = 0.0f;
+        }
+      } else {
+        const float mx = mu * ln;
+        const float mn = -mx;
+        const float w = w_of(d_h, a_h, c[k].rhs[f]);
+        const float val = li - c[k].r[f] * w;
+        const float ni = val > mn ? (val < mx ? val : mx) : mn;
+        const float dl = ni - li;
+        li = ni;
+        axpy6(dl, scale(d_h, im), wa_h, v);
+      }
+      if (f == 1) c[k].lam.y = li;
+      else c[k].lam.z = li;
+    }
+  }
+  if (dyn) b.store_dv(body, v.l, v.a);
+  if (h == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (k < x.cnt) sb.lam[x.r0 + k] = c[k].lam;
+  }
+}
+
+// One colour of one sweep, two lanes per manifold (the grid covers both lanes of every pair in
+// one wavefront: blockDim and the stride are even).
+template <bool kMask>
+__global__ void __launch_bounds__(64) k_pgs(DevBodies b, DevContacts c, SolverBufs sb, int first, int n) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; (t >> 1) < n; t += gridDim.x * blockDim.x)
+    pgs_manifold_pair<kMask>(b, sb, first + (t >> 1), t & 1);
 }
 
 // A run of consecutive small colour classes of the visiting order in one workgroup, a barrier
@@ -1003,24 +1114,24 @@ int sweep_blocks(int max_class) {
 }
 
 // The masked kernel variants run only in worlds where some dynamic body has a kinematic dof.
-void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
+void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int first, int n, int blocks, SolverBufs& sb,
                              hipStream_t s) {
-  if (b.kmask) hipLaunchKernelGGL(k_warmstart<true>, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
-  else hipLaunchKernelGGL(k_warmstart<false>, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
+  if (b.kmask) hipLaunchKernelGGL(k_warmstart<true>, dim3(blocks), dim3(64), 0, s, b, c, sb, first, n);
+  else hipLaunchKernelGGL(k_warmstart<false>, dim3(blocks), dim3(64), 0, s, b, c, sb, first, n);
 }
 
-void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
+void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int first, int n, int blocks, SolverBufs& sb,
                        hipStream_t s, hipEvent_t ev0, hipEvent_t ev1) {
   if (b.kmask) {
-    if (ev0) hipExtLaunchKernelGGL(k_pgs<true>, dim3(blocks), dim3(64), 0, s, ev0, ev1, 0, b, c, sb, k);
-    else hipLaunchKernelGGL(k_pgs<true>, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
+    if (ev0) hipExtLaunchKernelGGL(k_pgs<true>, dim3(blocks), dim3(64), 0, s, ev0, ev1, 0, b, c, sb, first, n);
+    else hipLaunchKernelGGL(k_pgs<true>, dim3(blocks), dim3(64), 0, s, b, c, sb, first, n);
     return;
   }
   if (ev0) {
     // the dispatch packet itself records both events, timing the kernel alone
-    hipExtLaunchKernelGGL(k_pgs<false>, dim3(blocks), dim3(64), 0, s, ev0, ev1, 0, b, c, sb, k);
+    hipExtLaunchKernelGGL(k_pgs<false>, dim3(blocks), dim3(64), 0, s, ev0, ev1, 0, b, c, sb, first, n);
   } else {
-    hipLaunchKernelGGL(k_pgs<false>, dim3(blocks), dim3(64), 0, s, b, c, sb, k);
+    hipLaunchKernelGGL(k_pgs<false>, dim3(blocks), dim3(64), 0, s, b, c, sb, first, n);
   }
 }
 

@@ -137,13 +137,14 @@ void launch_assemble(const DevBodies& b, const DevContacts& cur, const DevContac
 // grid-stride over the colour).
 int sweep_blocks(int max_class);
 int position_blocks(int max_class);
-void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int k, int blocks,
+// (first, n): the class's slot range, known on the host from the visiting order
+void launch_warmstart_colour(const DevBodies& b, const DevContacts& c, int first, int n, int blocks,
                              SolverBufs& sb, hipStream_t s);
 // Consecutive visited classes kv0 .. kv0 + nk - 1, each of at most kRunBlock manifolds, in one
 // workgroup (warm start or sweep).
 constexpr int kRunBlock = 512;
 void launch_colour_run(const DevBodies& b, SolverBufs& sb, int kv0, int nk, bool warm, hipStream_t s);
-void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int k, int blocks, SolverBufs& sb,
+void launch_pgs_colour(const DevBodies& b, const DevContacts& c, int first, int n, int blocks, SolverBufs& sb,
                        hipStream_t s, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr);
 void launch_residual(const DevBodies& b, const DevContacts& c, int M, SolverBufs& sb, hipStream_t s);
 void launch_cache_impulses(const DevContacts& c, SolverBufs& sb, int C, hipStream_t s);
@@ -215,7 +216,7 @@ void launch_joint_residual(const DevBodies& b, JointBufs& j, double* out, hipStr
 void launch_joint_union(const DevBodies& b, const JointBufs& j, int* parent, uint32_t* touch, hipStream_t s);
 
 // k_position.hip ---------------------------------------------------------------------------
-void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int k,
+void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int first, int n,
                             int blocks, SolverBufs& sb, hipStream_t s);
 void launch_position_run(const DevBodies& b, const SimParams& p, SolverBufs& sb, int kv0, int nk, hipStream_t s);
 

@@ -240,6 +240,8 @@ struct World {
       if (e) (void)hipEventDestroy(e);
     for (auto& e : tune_ev)
       if (e) (void)hipEventDestroy(e);
+    for (auto& e : ml_ev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -1217,15 +1219,18 @@ struct World {
         // one 64-lane workgroup per 64 manifolds of this class (the grid-stride loop covers the
         // rest above the cap): no idle workgroups for the smaller classes
         const int cblocks = std::min(sblocks, std::max(1, (h_range[2 * k + 1] + 63) / 64));
+        // the sweep kernel runs two lanes per manifold
+        const int pblocks2 = std::min(2 * sblocks, std::max(1, (2 * h_range[2 * k + 1] + 63) / 64));
         if (w) {
-          launch_warmstart_colour(b, cv, k, cblocks, v, stream);
+          launch_warmstart_colour(b, cv, h_range[2 * k], h_range[2 * k + 1], cblocks, v, stream);
         } else if (counters_enabled) {
-          launch_pgs_colour(b, cv, k, cblocks, v, stream, kt_ev[kt_used], kt_ev[kt_used + 1]);
+          launch_pgs_colour(b, cv, h_range[2 * k], h_range[2 * k + 1], pblocks2, v, stream, kt_ev[kt_used],
+                            kt_ev[kt_used + 1]);
           kt_used += 2;
           const int c = h_visit[k];
           counters.sweep_kernel_contacts += int64_t(class_rec[c + 1]) - int64_t(class_rec[c]);
         } else {
-          launch_pgs_colour(b, cv, k, cblocks, v, stream);
+          launch_pgs_colour(b, cv, h_range[2 * k], h_range[2 * k + 1], pblocks2, v, stream);
         }
         ++k;
       }
@@ -1415,10 +1420,23 @@ struct World {
   // The network's impulses for every row record (features already computed) as the warm start.
   int predict(SolverBufs& v, int C) {
     HIPCK(ml_y.ensure(size_t(C) * kOut * sizeof(float)));
+    if (counters_enabled) {
+      for (auto& e : ml_ev)
+        if (!e) HIPCK(hipEventCreate(&e));
+      HIPCK(hipEventRecord(ml_ev[0], stream));
+    }
     mlp_infer(MlpDev{ml_img.as<uint16_t>(), ml_params.as<float>()}, ml_X.as<float>(), C, ml_y.as<float>(), stream);
+    if (counters_enabled) {
+      HIPCK(hipEventRecord(ml_ev[1], stream));
+      ml_ev_pending = true;
+      ml_rows_pending = C;
+    }
     launch_apply_prediction(v, C, ml_y.as<float>(), ml_ls.as<float>(), stream);
     return NPHYS_OK;
   }
+  hipEvent_t ml_ev[2] = {nullptr, nullptr};
+  bool ml_ev_pending = false;
+  int ml_rows_pending = 0;
 
   // Activation islands and the solved-manifold list (sleeping worlds): sets cv, M and C to the
   // manifolds with an awake dynamic side; the others' cached impulses are carried over.
@@ -1611,8 +1629,8 @@ struct World {
           k = e;
           continue;
         }
-        launch_position_colour(b, cv, sp, k, std::min(pblocks, std::max(1, (h_range[2 * k + 1] + 63) / 64)), v,
-                               stream);
+        launch_position_colour(b, cv, sp, h_range[2 * k], h_range[2 * k + 1],
+                               std::min(pblocks, std::max(1, (h_range[2 * k + 1] + 63) / 64)), v, stream);
         ++k;
       }
     }
@@ -1682,6 +1700,14 @@ struct World {
       counters.sweep_kernel_launches += 1;
     }
     kt_used = 0;
+    if (ml_ev_pending) {  // the step's final synchronisation has passed the inference
+      float ms = 0.0f;
+      if (hipEventElapsedTime(&ms, ml_ev[0], ml_ev[1]) == hipSuccess) {
+        counters.warmstart_inference_time += double(ms) * 1e-3;
+        counters.warmstart_inference_rows += ml_rows_pending;
+      }
+      ml_ev_pending = false;
+    }
     prm.t += prm.dt;
     // clear_forces (mechanical_world.rs:365): the next accumulation starts from zero
     if (forces_pending) {

@@ -380,24 +380,28 @@ def test_incremental_update_overflow_grows_segments(nphys):
     """An incremental pair update that appends more new partners to a clean body's candidate segment
     than it holds falls back to a full build with larger segments: 48 balls land together on a wide
     resting box (index 1, so the balls are appended to its segment, which holds 32 partners) among
-    900 resting balls. The pairs equal a full build's every step, the overflow is counted, and the
-    updates after it are incremental again."""
+    1,000 resting balls, while 8 more balls are still falling elsewhere. The pairs equal a full
+    build's every step, the overflow is counted, and the updates after it are incremental again."""
     parts = [scenes.ground()]
     box = scenes._empty(1)
     box["shape"][0] = 1
     box["dims"][0] = (2.0, 0.25, 2.0)
     box["pos"][0] = (0.0, 0.25, 0.0)
     parts.append(box)
-    grid = scenes._empty(900)
-    i = np.arange(900)
+    grid = scenes._empty(1000)
+    i = np.arange(1000)
     grid["dims"][:, 0] = 0.2
-    grid["pos"][:] = np.stack([10.0 + 0.5 * (i % 30), np.full(900, 0.2), -7.0 + 0.5 * (i // 30)], 1)
+    grid["pos"][:] = np.stack([10.0 + 0.5 * (i % 40), np.full(1000, 0.2), -6.0 + 0.5 * (i // 40)], 1)
     parts.append(grid)
     drop = scenes._empty(48)
     k = np.arange(48)
     drop["dims"][:, 0] = 0.2
     drop["pos"][:] = np.stack([-1.575 + 0.45 * (k % 8), np.full(48, 3.0), -1.125 + 0.45 * (k // 8)], 1)
     parts.append(drop)
+    late = scenes._empty(8)
+    late["dims"][:, 0] = 0.2
+    late["pos"][:] = np.stack([-10.0 + 0.5 * np.arange(8), np.full(8, 8.0), np.zeros(8)], 1)
+    parts.append(late)
     s = scenes.concat(*parts)
     a = _gpu_world(nphys, s)
     b = _gpu_world(nphys, s)
