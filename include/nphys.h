@@ -391,6 +391,17 @@ int nphys_mlp_forward(nphys_mlp* m, const float* features, int64_t n, float* out
  * normalised outputs) with respect to the 20,099 trained parameters, in parameter order. */
 int nphys_mlp_gradients(nphys_mlp* m, const float* features, const float* targets, int64_t n,
                         float* grad, double* loss);
+/* nphys_mlp_gradients without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight training loops). No counterpart in the reference. All
+ * pointers are DEVICE pointers on the current device (features [n][24], targets [n][3], grad
+ * [20099], loss [1]); n is at most NPHYS_MLP_NOALLOC_MAX_ROWS (larger n: NPHYS_ERR_CAPACITY). The
+ * batch buffers live in a fixed device array of the library (about 9.9 MB per device, held until
+ * the process ends) shared by every call on that device: calls that overlap on one device race.
+ * The work is enqueued on `stream` (a hipStream_t; NULL: the network's own stream, and the call
+ * waits for it); with a caller's stream the call returns once enqueued. */
+#define NPHYS_MLP_NOALLOC_MAX_ROWS 4096
+int nphys_mlp_gradients_noalloc(nphys_mlp* m, const float* features, const float* targets, int64_t n,
+                                float* grad, double* loss, void* stream);
 
 /* Loads a network (NPHYS_MLP_NUM_PARAMS floats, the layout above) into the world; afterwards
  * nphys_contact_model_set(NPHYS_CONTACT_MODEL_LEARNED_WARMSTART) warm-starts every contact with

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "../../include/nphys.h"
@@ -92,6 +93,60 @@ void init_params(uint64_t seed, std::vector<float>& p) {
   for (int i = 0; i < kHid * kOut; ++i) p[kOffW3 + i] = float(normal() * std::sqrt(1.0 / kHid));
   for (int i = 0; i < kFeat; ++i) p[kOffXScale + i] = 1.0f;
   for (int i = 0; i < kOut; ++i) p[kOffYScale + i] = 1.0f;
+}
+
+// Scratch of nphys_mlp_gradients_noalloc: the training-pass buffers of one batch of at most
+// kNoAllocMaxRows rows (what ensure_batch allocates for nphys_mlp_gradients) and the identity row
+// index, carved out of one file-scope device array at 256-byte aligned offsets (about 9.9 MB; every
+// device the library runs on holds its copy until the process ends). Every byte a call reads is
+// written earlier in the same call, so nothing needs clearing between calls.
+constexpr int64_t kNoAllocMaxRows = 4096;
+constexpr size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+constexpr size_t kR = size_t(kNoAllocMaxRows);
+constexpr size_t kScrIdx = 0;
+constexpr size_t kScrA0 = align256(kScrIdx + kR * 4);
+constexpr size_t kScrZ1 = align256(kScrA0 + kR * kK1 * 2);
+constexpr size_t kScrZ2 = align256(kScrZ1 + kR * kHid * 4);
+constexpr size_t kScrH1 = align256(kScrZ2 + kR * kHid * 4);
+constexpr size_t kScrH2 = align256(kScrH1 + kR * kHid * 2);
+constexpr size_t kScrY = align256(kScrH2 + kR * kHid * 2);
+constexpr size_t kScrD3 = align256(kScrY + kR * kOut * 4);
+constexpr size_t kScrD2 = align256(kScrD3 + kR * kK1 * 2);
+constexpr size_t kScrD1 = align256(kScrD2 + kR * kHid * 2);
+constexpr size_t kScrLoss = align256(kScrD1 + kR * kHid * 2);
+constexpr size_t kScrGpart = align256(kScrLoss + size_t(1024) * 8);
+constexpr size_t kScratchBytes = align256(kScrGpart + ((kR + 255) / 256) * size_t(kNumTrained) * 4);
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// Device address of `scratch` on the current device, looked up once per device (with that device
+// current) and kept: host code uses this address, never the array's name.
+int scratch_on_current_device(unsigned char** out) {
+  int dev = 0;
+  HIPCK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) return set_err(NPHYS_ERR_CAPACITY, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return set_err(NPHYS_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = static_cast<unsigned char*>(g_scratch_ptr[dev]);
+  return NPHYS_OK;
+}
+
+__global__ void k_mlp_iota(int* __restrict__ idx, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) idx[i] = i;
 }
 
 }  // namespace
@@ -273,6 +328,39 @@ int nphys_mlp_gradients(nphys_mlp* m, const float* X, const float* T, int64_t n,
   HIPCK(hipMemcpyAsync(grad, m->grad.p, kNumTrained * 4, hipMemcpyDeviceToHost, m->stream));
   HIPCK(hipMemcpyAsync(loss, dl.p, 8, hipMemcpyDeviceToHost, m->stream));
   HIPCK(hipStreamSynchronize(m->stream));
+  return NPHYS_OK;
+}
+
+int nphys_mlp_gradients_noalloc(nphys_mlp* m, const float* X, const float* T, int64_t n, float* grad, double* loss,
+                                void* stream) {
+  if (!m || !X || !T || !grad || !loss) return set_err(NPHYS_ERR_INVALID, "null argument");
+  if (n < 1 || n > kNoAllocMaxRows)
+    return set_err(NPHYS_ERR_CAPACITY, "rows must be in [1, " + std::to_string(kNoAllocMaxRows) +
+                                           "] for nphys_mlp_gradients_noalloc");
+  unsigned char* base = nullptr;
+  if (const int rc = scratch_on_current_device(&base)) return rc;
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : m->stream;
+  MlpBatch b;
+  b.B = int(n);
+  b.X = X;
+  b.T = T;
+  b.idx = reinterpret_cast<const int*>(base + kScrIdx);
+  b.A0 = reinterpret_cast<uint16_t*>(base + kScrA0);
+  b.Z1 = reinterpret_cast<float*>(base + kScrZ1);
+  b.Z2 = reinterpret_cast<float*>(base + kScrZ2);
+  b.H1 = reinterpret_cast<uint16_t*>(base + kScrH1);
+  b.H2 = reinterpret_cast<uint16_t*>(base + kScrH2);
+  b.Y = reinterpret_cast<float*>(base + kScrY);
+  b.D3 = reinterpret_cast<uint16_t*>(base + kScrD3);
+  b.D2 = reinterpret_cast<uint16_t*>(base + kScrD2);
+  b.D1 = reinterpret_cast<uint16_t*>(base + kScrD1);
+  b.loss_part = reinterpret_cast<double*>(base + kScrLoss);
+  b.gpart = reinterpret_cast<float*>(base + kScrGpart);
+  hipLaunchKernelGGL(k_mlp_iota, dim3((b.B + 255) / 256), dim3(256), 0, s, reinterpret_cast<int*>(base + kScrIdx),
+                     b.B);
+  mlp_batch_gradients(m->dev(), b, grad, loss, s);
+  HIPCK(hipGetLastError());
+  if (!stream) HIPCK(hipStreamSynchronize(s));
   return NPHYS_OK;
 }
 

@@ -30,7 +30,7 @@ EXPORTED = [
     "nphys_capture_samples", "nphys_read_samples", "nphys_set_warmstart_judge",
     "nphys_mlp_create", "nphys_mlp_destroy", "nphys_mlp_num_params", "nphys_mlp_get_params",
     "nphys_mlp_set_params", "nphys_mlp_set_data", "nphys_mlp_train", "nphys_mlp_forward",
-    "nphys_mlp_gradients", "nphys_last_error", "nphys_apply_force", "nphys_apply_force_at_point",
+    "nphys_mlp_gradients", "nphys_mlp_gradients_noalloc", "nphys_last_error", "nphys_apply_force", "nphys_apply_force_at_point",
     "nphys_set_material_coefficient", "nphys_add_joints", "nphys_remove_joint", "nphys_num_joints",
     "nphys_read_joints",
 ]
