@@ -116,6 +116,22 @@ typedef struct nphys_body_desc {
   const int32_t* material_id;
   /* 3n: BasicMaterial::surface_velocity in the collider's frame (default none = zero) */
   const float* surface_velocity;
+  /* n: ColliderDesc::ccd_enabled (collider.rs:70,284-291), default 0. With any body CCD-enabled
+   * every step runs MechanicalWorld::solve_ccd (mechanical_world.rs:559-943) after the position
+   * correction: times of impact over the swept broad-phase pairs with a CCD-enabled side (bodies
+   * frozen at their first impact, re-sweeps), then, when some impact falls in the step, a substep
+   * (MoreauJeanSolver::step_ccd, moreau_jean_solver.rs:94-127) on the island around the impacting
+   * bodies moved to the last impact time; bodies still predicted to impact are put back to their
+   * pose at the start of their motion. One scene per world and no sleep thresholds with CCD. */
+  const int32_t* ccd_enabled;
+  /* n: RigidBody::enable_linear_motion_interpolation (rigid_body.rs:91-101), default 0: the CCD
+   * motion of the body keeps its current rotation. */
+  const int32_t* linear_motion_interpolation;
+  /* n: for a static body, an index shared by the static colliders that belong to one static body
+   * (the reference's Ground with several colliders, all at BodyPartHandle(ground, 0)); CCD freezes
+   * and traverses that body as a whole. Negative or NULL: a body of its own. Ignored for dynamic
+   * bodies. */
+  const int32_t* static_body;
 } nphys_body_desc;
 
 /* Joint constraints (src/joint/), structure of arrays for n joints. Kinds: BallConstraint
@@ -190,6 +206,11 @@ typedef struct nphys_counters {
    * kernel (HIP events around its launch on the world's stream) and the rows it evaluated */
   double warmstart_inference_time;
   int64_t warmstart_inference_rows;
+  /* CCD (counters.ccd, src/counters/ccd_counters.rs): time of the CCD phase of every step, the
+   * substeps run and the impacts they resolved, accumulated */
+  double ccd_time;
+  int64_t ccd_substeps;
+  int64_t ccd_impacts;
 } nphys_counters;
 
 /* ---------------------------------------------------------------- lifecycle */

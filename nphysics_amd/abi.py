@@ -47,6 +47,9 @@ OPTIONAL_BODY_FIELDS = {
     "restitution_combine": (np.int32, 1),
     "material_id": (np.int32, 1),
     "surface_velocity": (np.float32, 3),
+    "ccd_enabled": (np.int32, 1),
+    "linear_motion_interpolation": (np.int32, 1),
+    "static_body": (np.int32, 1),
 }
 
 
@@ -130,6 +133,9 @@ class BodyDesc(C.Structure):
         ("restitution_combine", C.POINTER(C.c_int32)),
         ("material_id", C.POINTER(C.c_int32)),
         ("surface_velocity", C.POINTER(C.c_float)),
+        ("ccd_enabled", C.POINTER(C.c_int32)),
+        ("linear_motion_interpolation", C.POINTER(C.c_int32)),
+        ("static_body", C.POINTER(C.c_int32)),
     ]
 
 
@@ -165,6 +171,9 @@ class Counters(C.Structure):
         ("incremental_overflows", C.c_int64),
         ("warmstart_inference_time", C.c_double),
         ("warmstart_inference_rows", C.c_int64),
+        ("ccd_time", C.c_double),
+        ("ccd_substeps", C.c_int64),
+        ("ccd_impacts", C.c_int64),
     ]
 
     def as_dict(self):

@@ -197,6 +197,48 @@ def small_pile(seed=7, n=512, lattice=(8, 16, 8), extent=8.0, y0=2.0, friction=0
     return s
 
 
+def ccd_room(num=4, rad=0.1, half_width=3.0, thickness=0.2, height=1.5, velocity=(-100.0, -10.0, 0.0),
+             shape=1, seed=None):
+    """The reference's CCD example (examples3d/ccd3.rs:18-124): a closed room of six static walls,
+    cuboids of half extents (half_width, thickness, half_width) that are all colliders of one static
+    body (static_body 0), and num^3 CCD-enabled bodies of half extent / radius `rad` (cuboids for
+    shape 1, balls for shape 0; with a seed, alternating balls and cuboids with a random
+    orientation) on a lattice of spacing 2 (rad + margin) at `height`, all moving at `velocity`
+    (100 m/s crosses the room in 0.06 s, 3.6 steps). Density 1, no sleep thresholds (the example
+    keeps RigidBodyDesc's default 0.01)."""
+    hw = float(half_width)
+    half = (hw, thickness, hw)
+    zq = quat_axis(2, np.array([np.pi / 2]))[0]
+    xq = quat_axis(0, np.array([np.pi / 2]))[0]
+    ident = np.array([0.0, 0.0, 0.0, 1.0])
+    walls = statics([half] * 6, [(0, -hw, 0), (0, hw, 0), (hw, 0, 0), (-hw, 0, 0), (0, 0, hw), (0, 0, -hw)])
+    walls["rot"][:] = np.stack([ident, ident, zq, zq, xq, xq]).astype(np.float32)
+    walls["static_body"] = np.zeros(6, np.int32)
+    n = num ** 3
+    b = _empty(n)
+    shift = (rad + DEFAULT_MARGIN) * 2.0
+    cx = shift * num / 2.0
+    i, j, k = np.meshgrid(np.arange(num), np.arange(num), np.arange(num), indexing="ij")
+    b["pos"][:] = np.stack([i.ravel() * shift - cx, j.ravel() * shift + shift / 2.0 + height, k.ravel() * shift - cx], 1)
+    b["shape"][:] = shape
+    b["dims"][:] = (rad, rad, rad) if shape == 1 else (rad, 0.0, 0.0)
+    if seed is not None:
+        v = splitmix_uniform(seed, 3 * n).reshape(n, 3)
+        is_ball = (np.arange(n) % 2) == 0
+        b["shape"][:] = np.where(is_ball, 0, 1)
+        b["dims"][:] = np.where(is_ball[:, None], np.array([rad, 0.0, 0.0]), np.array([rad, rad, rad]))
+        u1, u2, u3 = v[:, 0], 2 * np.pi * v[:, 1], 2 * np.pi * v[:, 2]
+        q = np.stack([np.sqrt(1 - u1) * np.sin(u2), np.sqrt(1 - u1) * np.cos(u2),
+                      np.sqrt(u1) * np.sin(u3), np.sqrt(u1) * np.cos(u3)], 1)
+        q[is_ball] = (0, 0, 0, 1)
+        b["rot"][:] = q
+    b["linvel"][:] = np.asarray(velocity, np.float32)
+    b["ccd_enabled"] = np.ones(n, np.int32)
+    b["static_body"] = np.full(n, -1, np.int32)
+    walls["ccd_enabled"] = np.zeros(6, np.int32)
+    return concat(walls, b)
+
+
 def num_static(scene):
     return int((scene["status"] == 1).sum())
 

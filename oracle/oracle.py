@@ -76,6 +76,8 @@ def lib():
         L.orc_collide_shapes.argtypes = [C.c_int32, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double,
                                          C.POINTER(C.c_uint32), dp, dp, dp, dp, ip]
         L.orc_colour_manifolds.argtypes = [C.c_int32, ip, ip, C.c_int32, ip, ip]
+        L.orc_time_of_impact.argtypes = [C.c_int32, C.c_int32, dp, dp, C.c_int32, dp, dp, C.c_double, C.c_double,
+                                         dp, ip]
         L.orc_apply_force.argtypes = [vp, C.c_int32, ip, fp, fp, C.c_int32, C.c_int32]
         L.orc_set_material_coefficient.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32, C.c_float]
         L.orc_mlp_forward.argtypes = [dp, fp, C.c_int64, C.c_int32, dp]
@@ -290,6 +292,27 @@ def collide_shapes(kind1, dims1, pose1, kind2, dims2, pose2, prediction, precisi
         raise RuntimeError("collide_shapes failed")
     return {"n": nc, "feature": feat[:nc], "world1": w1[:nc], "world2": w2[:nc], "normal": nn[:nc],
             "depth": dep[:nc], "kind": kind[:nc]}
+
+
+def time_of_impact(kind1, dims1, motion1, kind2, dims2, motion2, max_toi, target, precision=64):
+    """Conservative-advancement time of impact of two shapes. motion = dict(kind=0 static / 1 linear /
+    2 rotating, t0, pose=(x, y, z, i, j, k, w), v, w). Returns (toi, status) or None."""
+
+    def mot(m):
+        return np.array([m.get("kind", 2), m.get("t0", 0.0), *m["pose"], *m.get("v", (0, 0, 0)),
+                         *m.get("w", (0, 0, 0))], np.float64)
+
+    d1 = np.ascontiguousarray(dims1, np.float64)
+    d2 = np.ascontiguousarray(dims2, np.float64)
+    m1, m2 = mot(motion1), mot(motion2)
+    toi = np.zeros(1)
+    st = np.zeros(1, np.int32)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    found = lib().orc_time_of_impact(int(precision), int(kind1), d1.ctypes.data_as(dp), m1.ctypes.data_as(dp),
+                                     int(kind2), d2.ctypes.data_as(dp), m2.ctypes.data_as(dp), float(max_toi),
+                                     float(target), toi.ctypes.data_as(dp), st.ctypes.data_as(ip))
+    return (float(toi[0]), int(st[0])) if found else None
 
 
 def colour_manifolds(b1, b2, is_dynamic, initial=None):

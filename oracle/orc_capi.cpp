@@ -48,6 +48,7 @@ void apply_params(orc::World<T>& w, const nphys_params* p) {
   w.prm.max_position_iterations = p->max_position_iterations;
   w.prm.max_ccd_position_iterations = p->max_ccd_position_iterations;
   w.prm.max_ccd_substeps = p->max_ccd_substeps;
+  w.prm.ccd_on_penetration = p->ccd_on_penetration_enabled;
   w.gravity = orc::V3<T>{T(p->gravity[0]), T(p->gravity[1]), T(p->gravity[2])};
 }
 
@@ -94,6 +95,9 @@ int add_bodies_t(orc::World<T>& w, const nphys_body_desc* d, int32_t* first) {
     if (d->restitution_combine) ex.rest_mode = d->restitution_combine[i];
     if (d->material_id) ex.mat_id = d->material_id[i] < 0 ? -1 : d->material_id[i];
     if (d->surface_velocity) ex.surf_vel = v3_of<T>(d->surface_velocity + 3 * i);
+    if (d->ccd_enabled) ex.ccd = d->ccd_enabled[i] != 0;
+    if (d->linear_motion_interpolation) ex.lin_interp = d->linear_motion_interpolation[i] != 0;
+    if (d->static_body) ex.static_body = d->static_body[i] < 0 ? -1 : d->static_body[i];
     if (ex.fric_mode < 0 || ex.fric_mode > 3 || ex.rest_mode < 0 || ex.rest_mode > 3)
       return fail(NPHYS_ERR_INVALID, "bad combine mode");
     w.add_body(d->status[i], d->shape[i], v3_of<T>(d->dims + 3 * i),
@@ -374,7 +378,12 @@ int orc_set_material_coefficient(void* h, int32_t which, int32_t id1, int32_t id
 int orc_step(void* h, int32_t nsteps) {
   if (nsteps < 0) return fail(NPHYS_ERR_INVALID, "negative step count");
   return with_world(h, [&](auto& w) {
-    for (int i = 0; i < nsteps; ++i) w.step();
+    if (w.any_ccd && (w.any_sleeper || w.max_scene() > 0))
+      return fail(NPHYS_ERR_INVALID, "CCD needs one scene and no sleep thresholds");
+    for (int i = 0; i < nsteps; ++i) {
+      w.step();
+      if (!w.step_error.empty()) return fail(NPHYS_ERR_STATE, w.step_error.c_str());
+    }
     return NPHYS_OK;
   });
 }
@@ -470,6 +479,9 @@ int orc_read_counters(void* h, nphys_counters* c) {
     int64_t active = 0;
     for (const auto& b : w.bodies) active += (b.status == orc::STATUS_DYNAMIC && b.energy != 0) ? 1 : 0;
     c->active_bodies = active;
+    c->ccd_time = w.counters.ccd;
+    c->ccd_substeps = w.ccd_substeps;
+    c->ccd_impacts = w.ccd_impacts;
     return NPHYS_OK;
   });
 }
@@ -512,6 +524,39 @@ int orc_collide_shapes(int32_t precision, int32_t k1, const double* d1, const do
   if (precision == 32)
     return collide_shapes_t<float>(k1, d1, p1, k2, d2, p2, prediction, feature, w1, w2, n, depth, kind);
   return collide_shapes_t<double>(k1, d1, p1, k2, d2, p2, prediction, feature, w1, w2, n, depth, kind);
+}
+
+// Stand-alone time of impact of two shapes (the CCD's conservative advancement). Motion m =
+// (kind, t0, x, y, z, i, j, k, w, vx, vy, vz, wx, wy, wz): kind 0 static, 1 linear, 2 rotating;
+// pose at t = p0 moved for t - t0. Returns 1 with *toi and *status when an impact is found by
+// max_toi, else 0.
+int orc_time_of_impact(int32_t precision, int32_t k1, const double* d1, const double* m1, int32_t k2,
+                       const double* d2, const double* m2, double max_toi, double target, double* toi,
+                       int32_t* status) {
+  auto run = [&](auto zero) -> int {
+    using T = decltype(zero);
+    auto shape = [](int32_t k, const double* d) {
+      orc::ShapeDesc<T> sd;
+      sd.kind = k;
+      sd.he = k == orc::SHAPE_BALL ? orc::V3<T>{T(d[0]), T(d[0]), T(d[0])} : orc::V3<T>{T(d[0]), T(d[1]), T(d[2])};
+      return sd;
+    };
+    auto motion = [](const double* m) {
+      orc::CcdMotion<T> c;
+      c.kind = int(m[0]);
+      c.t0 = T(m[1]);
+      c.p0 = orc::Iso<T>{orc::V3<T>{T(m[2]), T(m[3]), T(m[4])}, orc::Q<T>{T(m[5]), T(m[6]), T(m[7]), T(m[8])}};
+      c.v = orc::V3<T>{T(m[9]), T(m[10]), T(m[11])};
+      c.w = orc::V3<T>{T(m[12]), T(m[13]), T(m[14])};
+      return c;
+    };
+    const orc::ToiResult<T> r =
+        orc::time_of_impact(shape(k1, d1), motion(m1), shape(k2, d2), motion(m2), T(max_toi), T(target));
+    *toi = double(r.toi);
+    *status = r.status;
+    return r.found ? 1 : 0;
+  };
+  return precision == 32 ? run(0.0f) : run(0.0);
 }
 
 // The deterministic manifold colouring on its own (colours[m] for manifold m).

@@ -32,6 +32,8 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <queue>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
@@ -61,6 +63,7 @@ struct Params {  // IntegrationParameters (src/solver/integration_parameters.rs:
   T max_stabilization_multiplier;
   int max_velocity_iterations, max_position_iterations, max_ccd_position_iterations;
   int max_ccd_substeps;
+  int ccd_on_penetration = 0;  // ccd_on_penetration_enabled (integration_parameters.rs:56-69)
   void set_dt(T d) {
     dt = d;
     inv_dt = d == T(0) ? T(0) : T(1) / d;
@@ -92,6 +95,11 @@ struct Body {
   V3<T> surf_vel;
   int companion;
   unsigned flags;
+  // CCD: ColliderDesc::ccd_enabled (collider.rs:70), RigidBody::linear_motion_interpolation_enabled
+  // (rigid_body.rs:44,91-101), and the key of the body the collider belongs to (the body itself, or
+  // for a static collider that shares a static body with others, the first of them)
+  bool ccd, lin_interp;
+  int ccd_key;
   AABB<T> loose;
   T energy;     // ActivationStatus::energy (0: asleep)
   T sleep_thr;  // ActivationStatus::threshold, < 0 for None
@@ -153,7 +161,7 @@ struct JointRows {
 
 struct Counters {  // stage names of src/counters/mod.rs:148-209
   double step = 0, collision_detection = 0, broad_phase = 0, narrow_phase = 0, assembly = 0,
-         velocity_resolution = 0, velocity_update = 0, position_resolution = 0;
+         velocity_resolution = 0, velocity_update = 0, position_resolution = 0, ccd = 0;
   long nsteps = 0;
 };
 
@@ -299,6 +307,8 @@ struct BodyExtras {
   int kinematic_dofs = 0;  // bit k: dof k kinematic (0-2 translations, 3-5 rotations)
   int fric_mode = COMBINE_AVERAGE, rest_mode = COMBINE_AVERAGE, mat_id = -1;
   V3<T> surf_vel{T(0), T(0), T(0)};
+  bool ccd = false, lin_interp = false;
+  int static_body = -1;  // static colliders with one index >= 0 (per scene) belong to one body
 };
 
 template <class T>
@@ -341,6 +351,7 @@ class World {
   Counters counters;
   // MaterialsCoefficientsTable (materials_coefficients_table.rs:8-50), keyed by the sorted pair
   std::map<std::pair<uint32_t, uint32_t>, T> fric_table, rest_table;
+  std::map<std::pair<int, int>, int> static_owner;  // (scene, static body index) -> its first collider
   bool any_surface_velocity = false;  // some body's material has a surface velocity
 
   World() {
@@ -404,6 +415,16 @@ class World {
     b.surf_vel = ex.surf_vel;
     if (ex.surf_vel.x != T(0) || ex.surf_vel.y != T(0) || ex.surf_vel.z != T(0)) any_surface_velocity = true;
     b.flags = F_ALL;
+    b.ccd = ex.ccd;
+    b.lin_interp = status == STATUS_DYNAMIC && ex.lin_interp;
+    if (ex.ccd) any_ccd = true;
+    b.ccd_key = int(bodies.size());
+    if (status != STATUS_DYNAMIC && ex.static_body >= 0) {
+      const auto key = std::make_pair(scene, ex.static_body);
+      auto it = static_owner.find(key);
+      if (it == static_owner.end()) static_owner.emplace(key, b.ccd_key);
+      else b.ccd_key = it->second;
+    }
     b.com = pos.t;
     b.mass = T(0);
     b.local_I = m3_zero<T>();
@@ -519,15 +540,10 @@ class World {
 
   // Isometry composition of apply_displacement (rigid_body.rs:371-381):
   // new_pos = T(com) * Iso(lin, exp(ang)) * T(-com) * pos.
+  // (the centre of mass is the pose's translation for every body here: com == pos.t)
   void apply_displacement(Body<T>& b, V3<T> lin, V3<T> ang) {
-    const Q<T> qd = qexp_scaled_axis(ang);
-    const V3<T> com = b.com;
-    const V3<T> dt_ = com + (lin + qrot(qd, -com));
-    Iso<T> np;
-    np.t = dt_ + qrot(qd, b.pos.t);
-    np.q = qmul(qd, b.pos.q);
-    b.pos = np;
-    b.com = np.t;
+    b.pos = iso_displaced(b.pos, lin, ang);
+    b.com = b.pos.t;
     b.flags |= F_POSITION;
   }
 
@@ -553,7 +569,7 @@ class World {
     const double t0 = now_s();
     for (auto& b : bodies) {
       if (!(b.flags & F_POSITION)) continue;  // colliders_need_update (body.rs:510-512)
-      AABB<T> a1 = shape_aabb(b.sd, swept ? b.pos0 : b.pos);
+      AABB<T> a1 = shape_aabb(b.sd, swept ? safe_position(b) : b.pos);
       if (swept) a1 = merged(a1, shape_aabb(b.sd, b.pos));
       else a1 = merged(a1, a1);
       const AABB<T> tight = loosened(a1, b.query_limit());
@@ -1495,6 +1511,308 @@ class World {
     position_correction();
     counters.position_resolution += now_s() - t0;
   }
+
+  // ------------------------------------------------------------------ CCD
+  // MechanicalWorld::solve_ccd, predict_next_impacts and TOIEntry (mechanical_world.rs:400-1052)
+  // with MoreauJeanSolver::step_ccd (moreau_jean_solver.rs:94-127), when some body has
+  // ccd_enabled. Where the reference's order comes from hash maps or its interaction graph, this
+  // restatement fixes one:
+  //  * the first pass bounds every pair's time of impact by the step's end; the reference narrows
+  //    the bound to the earliest impact seen beyond the window so far, which finds the same impacts
+  //    within the window and the same earliest one beyond it;
+  //  * the queue pops the earliest impact first and equal times by pair index (BinaryHeap leaves
+  //    ties unordered);
+  //  * a substep's island is every body reached from the impacting bodies' colliders through pairs
+  //    whose manifold at the advanced poses is not empty, plus the dynamic neighbours of the
+  //    traversed colliders (the reference skips a body that an earlier pair with an empty manifold
+  //    already advanced, which depends on the graph's iteration order); its manifolds are solved
+  //    in pair order.
+  bool any_ccd = false;
+  std::vector<T> body_time;  // SubstepState::body_times (0 for a body not advanced)
+  std::vector<T> frozen;     // per ccd key: time the body was frozen at, < 0 when not frozen
+  struct ToiEntry {
+    T toi;
+    int pair, ts;
+  };
+  std::vector<ToiEntry> impacts;
+  std::string step_error;
+  long ccd_substeps = 0, ccd_impacts = 0;
+
+  // RigidBody::safe_position (rigid_body.rs:904-910)
+  static Iso<T> safe_position(const Body<T>& b) { return b.lin_interp ? Iso<T>{b.pos0.t, b.pos.q} : b.pos0; }
+
+  // Body::part_motion (rigid_body.rs:539-553; a static body: BodyPartMotion::Static at its pose)
+  static CcdMotion<T> part_motion(const Body<T>& b, T t0) {
+    CcdMotion<T> m;
+    m.t0 = t0;
+    m.v = b.linvel;
+    m.w = b.angvel;
+    if (b.status != STATUS_DYNAMIC) {
+      m.kind = MOTION_STATIC;
+      m.p0 = b.pos;
+    } else if (b.lin_interp) {
+      m.kind = MOTION_LINEAR;
+      m.p0 = Iso<T>{b.pos0.t, b.pos.q};
+    } else {
+      m.kind = MOTION_ROTATING;
+      m.p0 = b.pos0;
+    }
+    return m;
+  }
+
+  // RigidBody::clamp_advancement (rigid_body.rs:530-537)
+  static void clamp_advancement(Body<T>& b) {
+    b.pos = safe_position(b);
+    b.com = b.pos.t;
+    b.flags |= F_POSITION;
+  }
+
+  // TOIEntry::try_from_colliders (mechanical_world.rs:946-1022) for pair (i, j); f1, f2: the
+  // bodies' freeze times (< 0: not frozen). The returned time is global (from the step's start).
+  ToiResult<T> pair_toi(int i, int j, T f1, T f2, T end_time) const {
+    const Body<T>& b1 = bodies[size_t(i)];
+    const Body<T>& b2 = bodies[size_t(j)];
+    const T bt1 = body_time[size_t(i)], bt2 = body_time[size_t(j)];
+    const T time1 = f1 >= T(0) ? f1 : bt1;
+    const T time2 = f2 >= T(0) ? f2 : bt2;
+    const T start = tmax(time1, time2);
+    const T o1 = time1 - start, o2 = time2 - start;
+    CcdMotion<T> m1 = part_motion(b1, o1), m2 = part_motion(b2, o2);
+    if (f1 >= T(0)) {
+      m1.p0 = m1.at((f1 + o1) - bt1);
+      m1.kind = MOTION_STATIC;
+    }
+    if (f2 >= T(0)) {
+      m2.p0 = m2.at((f2 + o2) - bt2);
+      m2.kind = MOTION_STATIC;
+    }
+    ToiResult<T> r = time_of_impact(b1.sd, m1, b2.sd, m2, end_time - start, prm.allowed_linear_error);
+    if (r.found && prm.ccd_on_penetration == 0 && r.status == TOI_PENETRATING) r.found = false;
+    if (r.found) r.toi = start + r.toi;
+    return r;
+  }
+
+  int max_scene() const {
+    int m = 0;
+    for (const auto& b : bodies) m = std::max(m, b.scene);
+    return m;
+  }
+  bool is_static(int i) const { return bodies[size_t(i)].status != STATUS_DYNAMIC; }
+  int key_of(int i) const { return bodies[size_t(i)].ccd_key; }
+
+  // predict_next_impacts (mechanical_world.rs:400-557): the impacts up to end_time in the order
+  // they are confirmed, with the freeze times; returns PREDICT_*.
+  enum { PREDICT_NONE = 0, PREDICT_AFTER_END = 1, PREDICT_IMPACTS = 2 };
+  int predict_impacts(T end_time, T dt0, T& min_overstep) {
+    impacts.clear();
+    frozen.assign(bodies.size(), T(-1));
+    std::vector<int> ccd_pairs;
+    for (size_t p = 0; p < pairs.size(); ++p)
+      if (bodies[size_t(pairs[p].first)].ccd || bodies[size_t(pairs[p].second)].ccd) ccd_pairs.push_back(int(p));
+    std::vector<int> ts(pairs.size(), 0);
+    auto later = [](const ToiEntry& a, const ToiEntry& b) {
+      if (a.toi != b.toi) return a.toi > b.toi;
+      if (a.pair != b.pair) return a.pair > b.pair;
+      return a.ts > b.ts;
+    };
+    std::priority_queue<ToiEntry, std::vector<ToiEntry>, decltype(later)> q(later);
+    min_overstep = dt0;
+    for (int p : ccd_pairs) {
+      const ToiResult<T> r = pair_toi(pairs[size_t(p)].first, pairs[size_t(p)].second, T(-1), T(-1), dt0);
+      if (!r.found) continue;
+      if (r.toi > end_time) {
+        min_overstep = tmin(min_overstep, r.toi);
+      } else {
+        min_overstep = end_time;
+        q.push(ToiEntry{r.toi, p, 0});
+      }
+    }
+    if (min_overstep == dt0 && q.empty()) return PREDICT_NONE;
+    if (min_overstep > end_time) return PREDICT_AFTER_END;
+    while (!q.empty()) {
+      const ToiEntry e = q.top();
+      q.pop();
+      if (e.ts != ts[size_t(e.pair)]) continue;  // superseded by a re-sweep
+      const int i = pairs[size_t(e.pair)].first, j = pairs[size_t(e.pair)].second;
+      const T t1 = frozen[size_t(key_of(i))], t2 = frozen[size_t(key_of(j))];
+      if (t1 >= T(0) && t2 >= T(0)) {
+        if (e.toi == tmax(t1, t2)) impacts.push_back(e);
+        continue;
+      }
+      if (t1 < T(0)) frozen[size_t(key_of(i))] = e.toi;
+      if (t2 < T(0)) frozen[size_t(key_of(j))] = e.toi;
+      // re-sweep every pair of the two bodies' dynamic colliders with a CCD-enabled side
+      for (const int s : {i, j}) {
+        if (is_static(s)) continue;
+        for (int p : ccd_pairs) {
+          const int a = pairs[size_t(p)].first, b = pairs[size_t(p)].second;
+          if (a != s && b != s) continue;
+          const T fa = frozen[size_t(key_of(a))], fb = frozen[size_t(key_of(b))];
+          if ((fa >= T(0) || is_static(a)) && (fb >= T(0) || is_static(b))) continue;
+          ts[size_t(p)] += 1;
+          const ToiResult<T> r = pair_toi(a, b, fa, fb, end_time);
+          if (r.found) q.push(ToiEntry{r.toi, p, ts[size_t(p)]});
+        }
+      }
+      impacts.push_back(e);
+    }
+    return PREDICT_IMPACTS;
+  }
+
+  // prepare_body of solve_ccd (mechanical_world.rs:653-681) with the substep's parameters current
+  void prepare_ccd_body(int s, T last_toi) {
+    Body<T>& b = bodies[size_t(s)];
+    b.f_lin = b.f_ang = V3<T>{T(0), T(0), T(0)};
+    update_dynamics(b);
+    update_acceleration(b);
+    const T f = frozen[size_t(b.ccd_key)];
+    const T target = f >= T(0) ? f : last_toi;
+    b.pos0 = part_motion(b, T(0)).at(target - body_time[size_t(s)]);  // RigidBody::advance
+    clamp_advancement(b);
+    body_time[size_t(s)] = last_toi;
+  }
+
+  // One CCD substep (mechanical_world.rs:630-935): the island around the impacting bodies moved to
+  // the last impact time, then MoreauJeanSolver::step_ccd over it with dt0 - last_toi, no warm
+  // start, max_ccd_position_iterations, no joints and no gravity.
+  void ccd_substep(T dt0) {
+    ++ccd_substeps;
+    ccd_impacts += long(impacts.size());
+    const T last_toi = impacts.back().toi;
+    const Params<T> saved_prm = prm;
+    const V3<T> saved_g = gravity;
+    prm.max_position_iterations = prm.max_ccd_position_iterations;
+    prm.warmstart_coeff = T(0);
+    prm.set_dt(dt0 - last_toi);
+    gravity = V3<T>{T(0), T(0), T(0)};
+    const size_t nb = bodies.size(), np = pairs.size();
+    std::vector<char> in_t(nb, 0), prepared(nb, 0), evaluated(np, 0), kept(np, 0);
+    for (const ToiEntry& e : impacts) {
+      in_t[size_t(key_of(pairs[size_t(e.pair)].first))] = 1;
+      in_t[size_t(key_of(pairs[size_t(e.pair)].second))] = 1;
+    }
+    Contact<T> buf[8];
+    for (;;) {
+      auto touches = [&](size_t p) { return in_t[size_t(key_of(pairs[p].first))] || in_t[size_t(key_of(pairs[p].second))]; };
+      for (size_t p = 0; p < np; ++p) {
+        if (evaluated[p] || !touches(p)) continue;
+        for (const int s : {pairs[p].first, pairs[p].second})
+          if (!is_static(s) && !prepared[size_t(s)]) {
+            prepare_ccd_body(s, last_toi);
+            prepared[size_t(s)] = 1;
+          }
+      }
+      std::vector<int> grow;
+      for (size_t p = 0; p < np; ++p) {
+        if (evaluated[p] || !touches(p)) continue;
+        evaluated[p] = 1;
+        const Body<T>& a = bodies[size_t(pairs[p].first)];
+        const Body<T>& b = bodies[size_t(pairs[p].second)];
+        if (collide_pair(a.sd, a.pos, b.sd, b.pos, a.query_limit() + b.query_limit(), buf) == 0) continue;
+        kept[p] = 1;
+        for (const int s : {pairs[p].first, pairs[p].second})
+          if (!is_static(s) && !in_t[size_t(key_of(s))]) grow.push_back(key_of(s));
+      }
+      if (grow.empty()) break;
+      for (int k : grow) in_t[size_t(k)] = 1;
+    }
+    std::vector<Manifold> ms;
+    std::vector<Contact<T>> cs;
+    for (size_t p = 0; p < np; ++p) {
+      if (!kept[p]) continue;
+      const Body<T>& a = bodies[size_t(pairs[p].first)];
+      const Body<T>& b = bodies[size_t(pairs[p].second)];
+      const int nc = collide_pair(a.sd, a.pos, b.sd, b.pos, a.query_limit() + b.query_limit(), buf);
+      ms.push_back(Manifold{pairs[p].first, pairs[p].second, int(cs.size()), nc});
+      for (int k = 0; k < nc; ++k) cs.push_back(buf[k]);
+    }
+    // step_ccd on the island; the main solve's rows, colours and joint list are put back afterwards
+    // (the next step's colours are inherited from the main solve)
+    auto keep_uni = uni, keep_uni_g = uni_g, keep_bil = bil, keep_bil_g = bil_g;
+    auto keep_jbil = jbil, keep_jbil_g = jbil_g;
+    auto keep_posc = posc;
+    auto keep_crow = crow;
+    auto keep_jrow = jrow;
+    auto keep_solved = solved;
+    auto keep_solved_contacts = solved_contacts;
+    auto keep_colours = colours;
+    auto keep_order = colour_order;
+    auto keep_prev = prev_coloured;
+    auto keep_prev_colours = prev_colours;
+    const int keep_ncolours = ncolours;
+    const long keep_rows = n_rows;
+    std::vector<int> keep_joints;
+    keep_joints.swap(active_joints);
+    solved = ms;
+    solved_contacts = cs;
+    assemble(ms, cs);
+    ++solve_count;
+    if (order_mode == ORDER_COLOURED) build_colour_order();
+    position_correction();
+    for (const ToiEntry& e : impacts)  // validate_advancement of the impacting bodies
+      for (const int s : {pairs[size_t(e.pair)].first, pairs[size_t(e.pair)].second})
+        if (!is_static(s)) bodies[size_t(s)].pos0 = bodies[size_t(s)].pos;
+    warmstart();
+    for (int it = 0; it < prm.max_velocity_iterations; ++it) sweep();
+    for (size_t s = 0; s < nb; ++s) {  // update_velocities_and_integrate over the island
+      if (!prepared[s]) continue;
+      Body<T>& b = bodies[s];
+      const int id = b.companion;
+      b.linvel = V3<T>{b.linvel.x + ext_vels[id + 0], b.linvel.y + ext_vels[id + 1], b.linvel.z + ext_vels[id + 2]};
+      b.angvel = V3<T>{b.angvel.x + ext_vels[id + 3], b.angvel.y + ext_vels[id + 4], b.angvel.z + ext_vels[id + 5]};
+      b.linvel = V3<T>{b.linvel.x + mj_lambda[id + 0], b.linvel.y + mj_lambda[id + 1], b.linvel.z + mj_lambda[id + 2]};
+      b.angvel = V3<T>{b.angvel.x + mj_lambda[id + 3], b.angvel.y + mj_lambda[id + 4], b.angvel.z + mj_lambda[id + 5]};
+      integrate(b);
+    }
+    for (auto& b : bodies) update_dynamics(b);  // mechanical_world.rs:915-923
+    keep_joints.swap(active_joints);
+    uni = keep_uni; uni_g = keep_uni_g; bil = keep_bil; bil_g = keep_bil_g;
+    jbil = keep_jbil; jbil_g = keep_jbil_g;
+    posc = keep_posc; crow = keep_crow; jrow = keep_jrow;
+    solved = keep_solved; solved_contacts = keep_solved_contacts;
+    colours = keep_colours; colour_order = keep_order; ncolours = keep_ncolours; n_rows = keep_rows;
+    prev_coloured = keep_prev; prev_colours = keep_prev_colours;
+    prm = saved_prm;
+    gravity = saved_g;
+  }
+
+  // MechanicalWorld::solve_ccd (mechanical_world.rs:559-943)
+  void solve_ccd() {
+    const T dt0 = prm.dt;
+    if (dt0 == T(0) || prm.max_ccd_substeps == 0) return;
+    const T sub_len = dt0 / T(prm.max_ccd_substeps);
+    body_time.assign(bodies.size(), T(0));
+    T end_time = T(0);
+    bool last_iter = false;
+    for (;;) {
+      if (end_time < dt0) end_time = tmin(end_time + sub_len, dt0);
+      else last_iter = true;
+      sync_and_broad_phase(true);
+      T min_ov = T(0);
+      int kind = predict_impacts(end_time, dt0, min_ov);
+      if (kind == PREDICT_AFTER_END) {
+        end_time = std::floor(min_ov / sub_len) * sub_len;
+        end_time = tmin(end_time + sub_len, dt0);
+        kind = predict_impacts(end_time, dt0, min_ov);
+        if (kind != PREDICT_IMPACTS) {
+          step_error = "CCD: no impact in the window of the earliest one";
+          return;
+        }
+      }
+      if (kind == PREDICT_NONE) {
+        impacts.clear();
+        frozen.assign(bodies.size(), T(-1));
+      }
+      if (!last_iter && !impacts.empty()) {
+        ccd_substep(dt0);
+        continue;
+      }
+      for (auto& b : bodies)
+        if (b.status == STATUS_DYNAMIC && frozen[size_t(b.ccd_key)] >= T(0)) clamp_advancement(b);
+      break;
+    }
+  }
+
   void step() {  // MechanicalWorld::step_with_filter (mechanical_world.rs:199-396)
     const double ts = now_s();
     bool any_moved = false;
@@ -1507,8 +1825,12 @@ class World {
     if (any_moved) collide();  // pre-solve CD: only when bodies were moved outside the solver
     solver_step();
     for (auto& b : bodies) update_dynamics(b);  // post-solve update_dynamics (:343-346)
-    // CCD phase: sync with (position0 -> position) and broad phase over swept AABBs (:604-606)
-    sync_and_broad_phase(true);
+    // CCD phase: sync with (position0 -> position) and broad phase over swept AABBs (:604-606),
+    // then with CCD-enabled bodies the impacts and substeps
+    const double tccd = now_s();
+    if (any_ccd) solve_ccd();
+    else sync_and_broad_phase(true);
+    counters.ccd += now_s() - tccd;
     for (auto& b : bodies) b.pos0 = b.pos;  // validate_advancement
     const double tc = now_s();
     sync_and_broad_phase(false);

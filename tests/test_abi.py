@@ -41,7 +41,7 @@ def test_params_struct_layout(nphys):
         else:
             assert getattr(p, name) == pytest.approx(getattr(q, name)), name
     assert C.sizeof(abi.Params) == 8 * 10 + 4 * 8 + 8 * 3
-    assert C.sizeof(abi.Counters) == 8 * 8 + 8 * 7 + 8 + 8 + 8 + 8 + 8 * 11
+    assert C.sizeof(abi.Counters) == 8 * 8 + 8 * 7 + 8 + 8 + 8 + 8 + 8 * 11 + 8 * 3
 
 
 def test_no_device_fails_loudly(nphys):
