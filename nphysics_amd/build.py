@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libnphys_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 SOURCES = ["k_util.hip", "k_bodies.hip", "k_activation.hip", "k_broadphase.hip", "k_narrowphase.hip",
-           "k_solver.hip", "k_joints.hip", "k_position.hip", "k_mlp.hip", "mlp.hip", "world.hip"]
+           "k_solver.hip", "k_joints.hip", "k_position.hip", "k_ccd.hip", "k_mlp.hip", "mlp.hip", "world.hip"]
 ARCH = os.environ.get("NPHYS_OFFLOAD_ARCH", "gfx950")
 
 FLAGS = [

@@ -28,6 +28,8 @@ struct SimParams {
   float allowed_linear_error, max_linear_correction;
   float gx, gy, gz;
   float allowed_angular_error, max_angular_correction;
+  // time step of the bodies' last dynamics update (dt, or a CCD substep's after one)
+  float dyn_dt;
 };
 
 // float4 per pose block and per solver block
@@ -66,7 +68,14 @@ struct DevBodies {
   const float4* surf_vel;  // surface velocity in the collider frame
   float4* f_lin;         // accumulated external forces, null when none are pending
   float4* f_ang;
-  float4* w_dyn;         // angular velocity of the last dynamics update (written by k_integrate)
+  float4* w_dyn;         // angular velocity of the last dynamics update (written by k_integrate);
+                         // w = 2 marks a body put back by the CCD, whose last update used ccd_rot
+  // CCD, null when no body is CCD-enabled: flags (bit 0 ccd_enabled, bit 1 linear motion
+  // interpolation), the key of the body a collider belongs to (itself, or the first collider of its
+  // static body), and the rotation of the last dynamics update of a body the CCD put back
+  const int* ccd;
+  const int* ccd_key;
+  float4* ccd_rot;
 
   // Jacobian mask of body i (1 free, 0 kinematic), linear and angular (rigid_body.rs:46,103-118).
   NPH_HD void mask(int i, f3& ml, f3& ma) const {
@@ -163,6 +172,60 @@ NPH_HD f3 cmul(f3 a, f3 m) { return f3{a.x * m.x, a.y * m.y, a.z * m.z}; }
 // Body flag bits (DevBodies::flags).
 constexpr int kFlagNoGravity = 64;
 NPH_HD int fric_mode(int flags) { return (flags >> 8) & 3; }
+
+// RigidBody::update_dynamics (rigid_body.rs:558-588) at rotation q with angular velocity w over
+// dyn_dt: I_w = R I_l R^T, augmented inertia I_w + [w dt]x I_w - [(I_w w) dt]x and its inverse
+// (stored with 1/m), or, for a body that keeps the augmented mass of an earlier update, that one
+// (`stored`; wd: the angular velocity of a stale update when given); then update_acceleration
+// (:590-619): gyroscopic term, gravity g unless disabled, external forces, the jacobian mask; and
+// the external velocity increment dt * acc (moreau_jean_solver.rs:166-174) of dynamic body i.
+__device__ __forceinline__ void body_prestep(const DevBodies& b, int i, q4 q, f3 w, bool stored, bool stale, f3 wd,
+                                             float dyn_dt, float dt, f3 g) {
+  const m33 R = qmat(q);
+  const float4 li = b.local_i[i];
+  m33 L;
+  L.m[0][0] = li.x; L.m[0][1] = 0.0f; L.m[0][2] = 0.0f;
+  L.m[1][0] = 0.0f; L.m[1][1] = li.y; L.m[1][2] = 0.0f;
+  L.m[2][0] = 0.0f; L.m[2][1] = 0.0f; L.m[2][2] = li.z;
+  const m33 Iw = mmul(mmul(R, L), mtrans(R));
+  const f3 iw = mulv(Iw, w);
+  float inv_mass;
+  m33 inv;
+  if (stored) {
+    inv = b.inv_inertia(i, inv_mass);
+  } else {
+    const f3 wu = stale ? wd : w;
+    const f3 iwd = stale ? mulv(Iw, wd) : iw;
+    const m33 wx = cross_matrix(scale(wu, dyn_dt));
+    const m33 iwx = cross_matrix(scale(iwd, dyn_dt));
+    const m33 wxI = mmul(wx, Iw);
+    m33 aug;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) aug.m[r][c] = Iw.m[r][c] + (wxI.m[r][c] - iwx.m[r][c]);
+    const float mass = b.mass[i];
+    inv_mass = mass == 0.0f ? 0.0f : 1.0f / mass;
+    try_inverse(aug, inv);
+  }
+  b.set_inv_inertia(i, inv, inv_mass);
+  const f3 gyro = neg(cross(w, iw));
+  f3 acc_ang = mulv(inv, gyro);
+  const bool grav = !(b.flags && (b.flags[i] & kFlagNoGravity));
+  f3 acc_lin = (inv_mass != 0.0f && grav) ? g : f3{0.0f, 0.0f, 0.0f};
+  if (b.f_lin) {  // acceleration += inv_augmented_mass * external_forces
+    acc_lin = add(acc_lin, scale(xyz(b.f_lin[i]), inv_mass));
+    acc_ang = add(acc_ang, mulv(inv, xyz(b.f_ang[i])));
+  }
+  if (b.kmask) {  // (.) jacobian_mask
+    f3 ml, ma;
+    b.mask(i, ml, ma);
+    acc_lin = cmul(acc_lin, ml);
+    acc_ang = cmul(acc_ang, ma);
+  }
+  b.ext_lin[i] = make_float4(dt * acc_lin.x, dt * acc_lin.y, dt * acc_lin.z, 0.0f);
+  b.ext_ang[i] = make_float4(dt * acc_ang.x, dt * acc_ang.y, dt * acc_ang.z, 0.0f);
+}
 NPH_HD int rest_mode(int flags) { return (flags >> 10) & 3; }
 
 // MaterialCombineMode::combine (material.rs:72-86): Max > Multiply > Min > Average.

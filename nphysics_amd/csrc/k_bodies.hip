@@ -44,56 +44,15 @@ __global__ void __launch_bounds__(256) k_prestep(DevBodies b, SimParams p) {
     b.ext_ang[i] = z4;
     return;
   }
-  const float dt = p.dt;
-  const m33 R = qmat(toq(q));
-  const float4 li = b.local_i[i];
-  m33 L;
-  L.m[0][0] = li.x; L.m[0][1] = 0.0f; L.m[0][2] = 0.0f;
-  L.m[1][0] = 0.0f; L.m[1][1] = li.y; L.m[1][2] = 0.0f;
-  L.m[2][0] = 0.0f; L.m[2][1] = 0.0f; L.m[2][2] = li.z;
-  const m33 Iw = mmul(mmul(R, L), mtrans(R));
-  const f3 w = xyz(b.angvel[i]);
-  const f3 iw = mulv(Iw, w);
-  float inv_mass;
-  m33 inv;
-  if (b.act && b.act[i].x == 0.0f) {
-    inv = b.inv_inertia(i, inv_mass);
-  } else {
-    // A body woken between steps by a force (no velocity or pose change) has no update flag, so
-    // the reference keeps the augmented mass of its last update (body.rs:498-507), made with the
-    // angular velocity it had before it slept: w_dyn, marked by k_apply_force.
-    const bool stale = b.act && b.w_dyn[i].w != 0.0f;
-    const f3 wd = stale ? xyz(b.w_dyn[i]) : w;
-    const f3 iwd = stale ? mulv(Iw, wd) : iw;
-    const m33 wx = cross_matrix(scale(wd, dt));
-    const m33 iwx = cross_matrix(scale(iwd, dt));
-    const m33 wxI = mmul(wx, Iw);
-    m33 aug;
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) aug.m[r][c] = Iw.m[r][c] + (wxI.m[r][c] - iwx.m[r][c]);
-    const float mass = b.mass[i];
-    inv_mass = mass == 0.0f ? 0.0f : 1.0f / mass;
-    try_inverse(aug, inv);
-  }
-  b.set_inv_inertia(i, inv, inv_mass);
-  const f3 gyro = neg(cross(w, iw));
-  f3 acc_ang = mulv(inv, gyro);
-  const bool grav = !(b.flags && (b.flags[i] & kFlagNoGravity));
-  f3 acc_lin = (inv_mass != 0.0f && grav) ? f3{p.gx, p.gy, p.gz} : f3{0.0f, 0.0f, 0.0f};
-  if (b.f_lin) {  // acceleration += inv_augmented_mass * external_forces
-    acc_lin = add(acc_lin, scale(xyz(b.f_lin[i]), inv_mass));
-    acc_ang = add(acc_ang, mulv(inv, xyz(b.f_ang[i])));
-  }
-  if (b.kmask) {  // (.) jacobian_mask
-    f3 ml, ma;
-    b.mask(i, ml, ma);
-    acc_lin = cmul(acc_lin, ml);
-    acc_ang = cmul(acc_ang, ma);
-  }
-  b.ext_lin[i] = make_float4(dt * acc_lin.x, dt * acc_lin.y, dt * acc_lin.z, 0.0f);
-  b.ext_ang[i] = make_float4(dt * acc_ang.x, dt * acc_ang.y, dt * acc_ang.z, 0.0f);
+  // A body woken between steps by a force (no velocity or pose change) has no update flag, so the
+  // reference keeps the augmented mass of its last update (body.rs:498-507), made with the angular
+  // velocity it had before it slept: w_dyn, marked by k_apply_force. A body the CCD put back at the
+  // end of the last step was last updated before that, at the rotation kept in ccd_rot.
+  const bool asleep = b.act && b.act[i].x == 0.0f;
+  const bool stale = b.act && b.w_dyn[i].w != 0.0f;
+  const bool ccd_back = b.ccd_rot && b.w_dyn[i].w == 2.0f;
+  body_prestep(b, i, ccd_back ? toq(b.ccd_rot[i]) : toq(q), xyz(b.angvel[i]), asleep, stale,
+               stale ? xyz(b.w_dyn[i]) : f3{0.0f, 0.0f, 0.0f}, p.dyn_dt, p.dt, f3{p.gx, p.gy, p.gz});
 }
 
 // Augmented inertia I_w + [w dt]x I_w - [(I_w w) dt]x of a body (rigid_body.rs:572-580) at
@@ -138,8 +97,12 @@ __global__ void __launch_bounds__(256) k_apply_force(DevBodies b, SimParams p, i
   const float mass = b.mass[i];
   const float inv_mass = mass == 0.0f ? 0.0f : 1.0f / mass;
   m33 aug, inv;
-  if (type == 1 || type == 2)
-    augmented_inertia(b, i, rot_dyn ? rot_dyn[i] : b.rot(i), xyz(b.w_dyn[i]), p.dt, aug, inv);
+  if (type == 1 || type == 2) {
+    // the rotation of the last dynamics update: before a set_state of this interval, the one kept
+    // by the CCD for a body it put back, else the current one; p.dyn_dt: that update's time step
+    const float4 rq = rot_dyn ? rot_dyn[i] : ((b.ccd_rot && b.w_dyn[i].w == 2.0f) ? b.ccd_rot[i] : b.rot(i));
+    augmented_inertia(b, i, rq, xyz(b.w_dyn[i]), p.dyn_dt, aug, inv);
+  }
   const f3 com = xyz(b.pos(i));
   f3 fl_acc = b.f_lin ? xyz(b.f_lin[i]) : f3{0.0f, 0.0f, 0.0f};
   f3 fa_acc = b.f_lin ? xyz(b.f_ang[i]) : f3{0.0f, 0.0f, 0.0f};
@@ -189,10 +152,11 @@ __global__ void __launch_bounds__(256) k_apply_force(DevBodies b, SimParams p, i
   }
 }
 
-__global__ void __launch_bounds__(256) k_integrate(DevBodies b, SimParams p) {
+__global__ void __launch_bounds__(256) k_integrate(DevBodies b, SimParams p, const uint32_t* __restrict__ only) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
   if (b.status[i] != STATUS_DYNAMIC) return;
+  if (only && !only[i]) return;  // a CCD substep integrates its island only
   if (b.act && b.act[i].x == 0.0f) return;  // asleep: not in the solver's island
   const float4 v = b.linvel[i], w = b.angvel[i];
   const float4 el = b.ext_lin[i], ea = b.ext_ang[i];
@@ -226,9 +190,20 @@ void launch_prestep(const DevBodies& b, const SimParams& p, hipStream_t s) {
   hipLaunchKernelGGL(k_prestep, dim3((b.n + 255) / 256), dim3(256), 0, s, b, p);
 }
 
-void launch_integrate(const DevBodies& b, const SimParams& p, hipStream_t s) {
+void launch_integrate(const DevBodies& b, const SimParams& p, hipStream_t s, const uint32_t* only) {
   if (b.n == 0) return;
-  hipLaunchKernelGGL(k_integrate, dim3((b.n + 255) / 256), dim3(256), 0, s, b, p);
+  hipLaunchKernelGGL(k_integrate, dim3((b.n + 255) / 256), dim3(256), 0, s, b, p, only);
+}
+
+// The rotation each body's last dynamics update used (the CCD's kept one for a body it put back).
+__global__ void __launch_bounds__(256) k_save_rot_dyn(DevBodies b, float4* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < b.n) out[i] = (b.ccd_rot && b.w_dyn[i].w == 2.0f) ? b.ccd_rot[i] : b.rot(i);
+}
+
+void launch_save_rot_dyn(const DevBodies& b, float4* out, hipStream_t s) {
+  if (b.n == 0) return;
+  hipLaunchKernelGGL(k_save_rot_dyn, dim3((b.n + 255) / 256), dim3(256), 0, s, b, out);
 }
 
 // set_state flags every body for a dynamics update from its new state: no body keeps an old one.

@@ -47,7 +47,11 @@ __global__ void __launch_bounds__(256) k_loose_update(DevBodies b, int mode, uin
   const int kind = b.shape[i];
   const f3 he = xyz(b.dims[i]);
   aabb a = shape_aabb(kind, he, xyz(b.pos(i)), toq(b.rot(i)));
-  if (mode == 2) a = merged(shape_aabb(kind, he, xyz(b.pos0[i]), toq(b.rot0[i])), a);
+  if (mode == 2) {  // safe_position (rigid_body.rs:904-910): position0, with linear motion interpolation
+                   // the current rotation
+    const q4 r0 = (b.ccd && (b.ccd[i] & 2)) ? toq(b.rot(i)) : toq(b.rot0[i]);
+    a = merged(shape_aabb(kind, he, xyz(b.pos0[i]), r0), a);
+  }
   else a = merged(a, a);
   const aabb tight = loosened(a, b.qlim[i]);
   const aabb loose{xyz(b.loose_mn[i]), xyz(b.loose_mx[i])};

@@ -1,0 +1,421 @@
+// nphysics_amd/csrc/k_ccd.hip -- continuous collision detection: times of impact, the impact
+// queue, and the substep island.
+//
+// Restates MechanicalWorld::predict_next_impacts and the island traversal of solve_ccd
+// (src/world/mechanical_world.rs:400-557, 630-861) with TOIEntry::try_from_colliders (:946-1022):
+//   * k_ccd_first_pass, one lane per broad-phase pair with a CCD-enabled side: the pair gets a CCD
+//     slot and its time of impact up to the step's end (conservative advancement, np_geometry.hpp);
+//     impacts within the window go to the queue, later ones lower the earliest-later time;
+//   * k_ccd_events, one workgroup: lane 0 pops the earliest impact (ties by pair index), skips
+//     entries superseded by a re-sweep, confirms pairs of two frozen bodies only at the later
+//     freeze time, and freezes the two bodies (their keys: a static collider shares its static
+//     body's); then every lane re-sweeps the CCD slots of the two bodies' dynamic colliders with
+//     the frozen bodies held at their freeze poses, and lane 0 pushes the results;
+//   * the island of a substep grows in rounds: the dynamic bodies of every pair touching a
+//     traversed collider are moved to their impact time (k_ccd_touch, k_ccd_prepare; the dynamics
+//     update of the substep, no gravity, forces cleared), those pairs' manifolds are computed by
+//     the narrow phase at the moved poses, and the dynamic bodies of the non-empty ones are
+//     traversed from the next round (k_ccd_grow, k_ccd_merge);
+//   * k_ccd_validate (validate_advancement of the impacting bodies) and k_ccd_clamp (bodies still
+//     predicted to impact go back to the start of their motion, rigid_body.rs:530-537).
+// The queue is serial by nature; it holds the few impacts of a step, while the time-of-impact
+// queries (the arithmetic) run one per lane.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+#include "np_geometry.hpp"
+#include "stages.hpp"
+
+namespace nph {
+
+namespace {
+
+constexpr int kEventBlock = 256;
+
+__device__ __forceinline__ unsigned long long toi_key(float toi, int pair) {
+  return (static_cast<unsigned long long>(__float_as_uint(toi)) << 32) | static_cast<unsigned long long>(uint32_t(pair));
+}
+__device__ __forceinline__ float key_toi(unsigned long long k) { return __uint_as_float(uint32_t(k >> 32)); }
+__device__ __forceinline__ int key_pair(unsigned long long k) { return int(uint32_t(k & 0xffffffffull)); }
+
+// Body::part_motion (rigid_body.rs:539-553): static bodies at their pose; with linear motion
+// interpolation from (position0, current rotation); else from position0 with rotation.
+__device__ __forceinline__ CcdMotion part_motion(const DevBodies& b, int i, float t0) {
+  CcdMotion m;
+  m.t0 = t0;
+  m.v = xyz(b.linvel[i]);
+  m.w = xyz(b.angvel[i]);
+  if (b.status[i] != STATUS_DYNAMIC) {
+    m.kind = MOTION_STATIC;
+    m.t0p = xyz(b.pos(i));
+    m.q0 = toq(b.rot(i));
+  } else if (b.ccd[i] & 2) {
+    m.kind = MOTION_LINEAR;
+    m.t0p = xyz(b.pos0[i]);
+    m.q0 = toq(b.rot(i));
+  } else {
+    m.kind = MOTION_ROTATING;
+    m.t0p = xyz(b.pos0[i]);
+    m.q0 = toq(b.rot0[i]);
+  }
+  return m;
+}
+
+// TOIEntry::try_from_colliders for bodies i, j with freeze times f1, f2 (< 0: not frozen): the
+// global time of impact up to end_time.
+__device__ bool pair_toi(const DevBodies& b, const CcdBufs& c, int i, int j, float f1, float f2, float end_time,
+                         float target, int on_penetration, float* out) {
+  const float bt1 = c.body_time[i], bt2 = c.body_time[j];
+  const float time1 = f1 >= 0.0f ? f1 : bt1;
+  const float time2 = f2 >= 0.0f ? f2 : bt2;
+  const float start = time1 > time2 ? time1 : time2;
+  const float o1 = time1 - start, o2 = time2 - start;
+  CcdMotion m1 = part_motion(b, i, o1), m2 = part_motion(b, j, o2);
+  if (f1 >= 0.0f) {
+    m1.at((f1 + o1) - bt1, m1.t0p, m1.q0);
+    m1.kind = MOTION_STATIC;
+  }
+  if (f2 >= 0.0f) {
+    m2.at((f2 + o2) - bt2, m2.t0p, m2.q0);
+    m2.kind = MOTION_STATIC;
+  }
+  float toi = 0.0f;
+  int status = 0;
+  if (!time_of_impact(b.shape[i], xyz(b.dims[i]), m1, b.shape[j], xyz(b.dims[j]), m2, end_time - start, target, &toi,
+                      &status))
+    return false;
+  if (!on_penetration && status == TOI_PENETRATING) return false;
+  *out = start + toi;
+  return true;
+}
+
+__global__ void k_ccd_reset(uint32_t* ctl) {
+  const int t = threadIdx.x;
+  if (t < kCcdWords) ctl[t] = t == kCcdMinAfter ? 0x7f800000u : 0u;  // +inf
+}
+
+__global__ void __launch_bounds__(128) k_ccd_first_pass(DevBodies b, const int2* __restrict__ pairs, int npairs,
+                                                        CcdBufs c, float end_time, float dt0, float target,
+                                                        int on_penetration) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int2 pr = pairs[p];
+  if (!((b.ccd[pr.x] | b.ccd[pr.y]) & 1)) return;
+  const uint32_t slot = atomicAdd(&c.ctl[kCcdSlots], 1u);
+  c.slot_pair[slot] = p;
+  c.ts[slot] = 0u;
+  float toi;
+  if (!pair_toi(b, c, pr.x, pr.y, -1.0f, -1.0f, dt0, target, on_penetration, &toi)) return;
+  if (toi > end_time) {
+    atomicMin(&c.ctl[kCcdMinAfter], __float_as_uint(toi));  // toi > 0: the bits order as the values
+  } else {
+    atomicOr(&c.ctl[kCcdAnyIn], 1u);
+    const uint32_t e = atomicAdd(&c.ctl[kCcdEntries], 1u);
+    if (int(e) < c.cap) c.heap[e] = ToiEntry{toi_key(toi, p), int(slot), 0};
+    else atomicOr(&c.ctl[kCcdOverflow], 1u);
+  }
+}
+
+__device__ __forceinline__ bool entry_before(const ToiEntry& a, const ToiEntry& b) {
+  return a.key < b.key || (a.key == b.key && a.ts < b.ts);
+}
+__device__ void heap_down(ToiEntry* h, int n, int k) {
+  const ToiEntry e = h[k];
+  for (;;) {
+    int ch = 2 * k + 1;
+    if (ch >= n) break;
+    if (ch + 1 < n && entry_before(h[ch + 1], h[ch])) ++ch;
+    if (!entry_before(h[ch], e)) break;
+    h[k] = h[ch];
+    k = ch;
+  }
+  h[k] = e;
+}
+__device__ void heap_up(ToiEntry* h, int k) {
+  const ToiEntry e = h[k];
+  while (k > 0) {
+    const int par = (k - 1) / 2;
+    if (!entry_before(e, h[par])) break;
+    h[k] = h[par];
+    k = par;
+  }
+  h[k] = e;
+}
+
+__global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const int2* __restrict__ pairs, CcdBufs c,
+                                                            float end_time, float dt0, float target,
+                                                            int on_penetration) {
+  __shared__ int s_go, s_ovf, s_i, s_j, s_nstage, s_hsize, s_nimp;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    const int nent = int(c.ctl[kCcdEntries]);
+    const bool any = c.ctl[kCcdAnyIn] != 0u;
+    // min_overstep (mechanical_world.rs:412-451): the step's end lowered to the earliest impact
+    // after the window, or the window's end when an impact lies within it
+    float min_over = dt0;
+    const float after = __uint_as_float(c.ctl[kCcdMinAfter]);
+    if (after < min_over) min_over = after;
+    if (any) min_over = end_time;
+    int kind = PREDICT_IMPACTS;
+    if (min_over == dt0 && !any) kind = PREDICT_NONE;
+    else if (min_over > end_time) kind = PREDICT_AFTER_END;
+    c.ctl[kCcdKind] = uint32_t(kind);
+    c.ctl[kCcdMinOver] = __float_as_uint(min_over);
+    const int go = (kind == PREDICT_IMPACTS && c.ctl[kCcdOverflow] == 0u) ? 1 : 0;
+    if (go)
+      for (int k = nent / 2 - 1; k >= 0; --k) heap_down(c.heap, nent, k);
+    s_go = go;
+    s_ovf = 0;
+    s_hsize = nent;
+    s_nimp = 0;
+  }
+  __syncthreads();
+  if (!s_go) {
+    if (t == 0) c.ctl[kCcdImpacts] = 0u;
+    return;
+  }
+  for (;;) {
+    if (t == 0) {
+      int hs = s_hsize, nimp = s_nimp, found = 0;
+      while (hs > 0) {
+        if (nimp >= c.cap) {  // the impact list is full: reported as an overflow
+          c.ctl[kCcdOverflow] = 1u;
+          break;
+        }
+        const ToiEntry e = c.heap[0];
+        --hs;
+        if (hs > 0) {
+          c.heap[0] = c.heap[hs];
+          heap_down(c.heap, hs, 0);
+        }
+        if (uint32_t(e.ts) != c.ts[e.slot]) continue;  // superseded by a re-sweep
+        const float toi = key_toi(e.key);
+        const int2 pr = pairs[key_pair(e.key)];
+        const int ki = b.ccd_key[pr.x], kj = b.ccd_key[pr.y];
+        const float t1 = c.frozen[ki], t2 = c.frozen[kj];
+        if (t1 >= 0.0f && t2 >= 0.0f) {
+          if (toi == (t1 > t2 ? t1 : t2)) c.impacts[nimp++] = e.key;
+          continue;
+        }
+        if (t1 < 0.0f) c.frozen[ki] = toi;
+        if (t2 < 0.0f) c.frozen[kj] = toi;
+        c.impacts[nimp++] = e.key;
+        s_i = pr.x;
+        s_j = pr.y;
+        found = 1;
+        break;
+      }
+      s_hsize = hs;
+      s_nimp = nimp;
+      s_go = found;
+      s_nstage = 0;
+    }
+    __syncthreads();
+    if (!s_go) break;
+    // re-sweep the CCD slots of the two bodies' dynamic colliders (mechanical_world.rs:503-551)
+    const int si = s_i, sj = s_j;
+    const bool di = b.status[si] == STATUS_DYNAMIC, dj = b.status[sj] == STATUS_DYNAMIC;
+    const int ncc = int(c.ctl[kCcdSlots]);
+    for (int slot = t; slot < ncc; slot += kEventBlock) {
+      const int p = c.slot_pair[slot];
+      const int2 pr = pairs[p];
+      const bool touch = (di && (pr.x == si || pr.y == si)) || (dj && (pr.x == sj || pr.y == sj));
+      if (!touch) continue;
+      const float fa = c.frozen[b.ccd_key[pr.x]], fb = c.frozen[b.ccd_key[pr.y]];
+      const bool sa = b.status[pr.x] != STATUS_DYNAMIC, sb = b.status[pr.y] != STATUS_DYNAMIC;
+      if ((fa >= 0.0f || sa) && (fb >= 0.0f || sb)) continue;
+      const uint32_t stamp = c.ts[slot] + 1u;
+      c.ts[slot] = stamp;
+      float toi;
+      if (pair_toi(b, c, pr.x, pr.y, fa, fb, end_time, target, on_penetration, &toi)) {
+        const int k = atomicAdd(&s_nstage, 1);
+        c.stage[k] = ToiEntry{toi_key(toi, p), slot, int(stamp)};
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      int hs = s_hsize;
+      const int ns = s_nstage;
+      if (hs + ns > c.cap) {
+        c.ctl[kCcdOverflow] = 1u;
+        s_ovf = 1;
+      } else {
+        for (int k = 0; k < ns; ++k) {
+          c.heap[hs] = c.stage[k];
+          heap_up(c.heap, hs);
+          ++hs;
+        }
+      }
+      s_hsize = hs;
+    }
+    __syncthreads();
+    if (s_ovf) break;
+  }
+  if (t == 0) {
+    const int nimp = s_nimp;
+    c.ctl[kCcdImpacts] = uint32_t(nimp);
+    c.ctl[kCcdLastToi] = nimp > 0 ? uint32_t(c.impacts[nimp - 1] >> 32) : 0u;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ccd_seed(DevBodies b, const int2* __restrict__ pairs, CcdBufs c, int nimp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nimp) return;
+  const int2 pr = pairs[key_pair(c.impacts[k])];
+  c.in_t[b.ccd_key[pr.x]] = 1u;
+  c.in_t[b.ccd_key[pr.y]] = 1u;
+}
+
+__global__ void __launch_bounds__(256) k_ccd_touch(DevBodies b, const int2* __restrict__ pairs, int npairs,
+                                                   CcdBufs c) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int2 pr = pairs[p];
+  const bool on = !c.evaluated[p] && (c.in_t[b.ccd_key[pr.x]] || c.in_t[b.ccd_key[pr.y]]);
+  c.cand[p] = on ? 1u : 0u;
+  if (!on) return;
+  if (b.status[pr.x] == STATUS_DYNAMIC) c.prep[pr.x] = 1u;
+  if (b.status[pr.y] == STATUS_DYNAMIC) c.prep[pr.y] = 1u;
+}
+
+// prepare_body (mechanical_world.rs:653-681): forces cleared, the dynamics update of the substep
+// (dt_sub, no gravity) at the current pose, then RigidBody::advance to the body's freeze time (or
+// the last impact time) along its motion and clamp_advancement.
+__global__ void __launch_bounds__(256) k_ccd_prepare(DevBodies b, CcdBufs c, float dt_sub, float last_toi) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n || !c.prep[i] || c.prepared[i]) return;
+  c.prepared[i] = 1u;
+  if (b.f_lin) {
+    b.f_lin[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    b.f_ang[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  }
+  body_prestep(b, i, toq(b.rot(i)), xyz(b.angvel[i]), false, false, f3{0.0f, 0.0f, 0.0f}, dt_sub, dt_sub,
+               f3{0.0f, 0.0f, 0.0f});
+  const float f = c.frozen[b.ccd_key[i]];
+  const float target = f >= 0.0f ? f : last_toi;
+  const CcdMotion m = part_motion(b, i, 0.0f);
+  f3 tp;
+  q4 q;
+  m.at(target - c.body_time[i], tp, q);
+  b.pos0[i] = f4(tp, 0.0f);
+  b.rot0[i] = fromq(q);
+  b.pos(i) = f4(tp, 0.0f);
+  if (!(b.ccd[i] & 2)) b.rot(i) = fromq(q);
+  c.body_time[i] = last_toi;
+}
+
+__global__ void __launch_bounds__(256) k_ccd_scatter(const int2* __restrict__ pairs, int npairs,
+                                                     const uint32_t* __restrict__ flag, CcdBufs c) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const uint32_t at = c.cpos[p];
+  if (p == npairs - 1) c.ctl[kCcdList] = at + flag[p];
+  if (!flag[p]) return;
+  c.list[at] = pairs[p];
+  c.list_pair[at] = p;
+}
+
+__global__ void __launch_bounds__(256) k_ccd_grow(DevBodies b, CcdBufs c, int nlist, const uint32_t* __restrict__ count) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nlist) return;
+  const int p = c.list_pair[k];
+  c.evaluated[p] = 1u;
+  if (count[k] == 0u) return;
+  c.kept[p] = 1u;
+  const int2 pr = c.list[k];
+  if (b.status[pr.x] == STATUS_DYNAMIC && !c.in_t[b.ccd_key[pr.x]]) c.new_t[b.ccd_key[pr.x]] = 1u;
+  if (b.status[pr.y] == STATUS_DYNAMIC && !c.in_t[b.ccd_key[pr.y]]) c.new_t[b.ccd_key[pr.y]] = 1u;
+}
+
+__global__ void __launch_bounds__(256) k_ccd_merge(int n, CcdBufs c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || !c.new_t[i]) return;
+  c.in_t[i] = 1u;
+  c.new_t[i] = 0u;
+}
+
+__global__ void __launch_bounds__(256) k_ccd_validate(DevBodies b, const int2* __restrict__ pairs, CcdBufs c, int nimp) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nimp) return;
+  const int2 pr = pairs[key_pair(c.impacts[k])];
+  if (b.status[pr.x] == STATUS_DYNAMIC) {
+    b.pos0[pr.x] = b.pos(pr.x);
+    b.rot0[pr.x] = b.rot(pr.x);
+  }
+  if (b.status[pr.y] == STATUS_DYNAMIC) {
+    b.pos0[pr.y] = b.pos(pr.y);
+    b.rot0[pr.y] = b.rot(pr.y);
+  }
+}
+
+// The bodies still frozen when the CCD ends go back to their safe position; their last dynamics
+// update (after the substep) used the rotation they have now, kept for the next one.
+__global__ void __launch_bounds__(256) k_ccd_clamp(DevBodies b, CcdBufs c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= b.n || b.status[i] != STATUS_DYNAMIC || !(c.frozen[b.ccd_key[i]] >= 0.0f)) return;
+  b.ccd_rot[i] = b.rot(i);
+  b.w_dyn[i].w = 2.0f;
+  b.pos(i) = b.pos0[i];
+  if (!(b.ccd[i] & 2)) b.rot(i) = b.rot0[i];
+}
+
+int blocks_of(int n, int bs) { return (n + bs - 1) / bs; }
+
+}  // namespace
+
+void launch_ccd_predict(const DevBodies& b, const int2* pairs, int npairs, CcdBufs& c, float end_time, float dt0,
+                        float target, int on_penetration, hipStream_t s) {
+  hipLaunchKernelGGL(k_ccd_reset, dim3(1), dim3(64), 0, s, c.ctl);
+  fill_u32(reinterpret_cast<uint32_t*>(c.frozen), 0xbf800000u, size_t(b.n), s);  // -1
+  if (npairs > 0)
+    hipLaunchKernelGGL(k_ccd_first_pass, dim3(blocks_of(npairs, 128)), dim3(128), 0, s, b, pairs, npairs, c,
+                       end_time, dt0, target, on_penetration);
+  hipLaunchKernelGGL(k_ccd_events, dim3(1), dim3(kEventBlock), 0, s, b, pairs, c, end_time, dt0, target,
+                     on_penetration);
+}
+
+void launch_ccd_seed(const DevBodies& b, const int2* pairs, CcdBufs& c, int nimp, hipStream_t s) {
+  if (nimp <= 0) return;
+  hipLaunchKernelGGL(k_ccd_seed, dim3(blocks_of(nimp, 256)), dim3(256), 0, s, b, pairs, c, nimp);
+}
+
+void launch_ccd_touch(const DevBodies& b, const int2* pairs, int npairs, CcdBufs& c, hipStream_t s) {
+  if (npairs <= 0) return;
+  hipLaunchKernelGGL(k_ccd_touch, dim3(blocks_of(npairs, 256)), dim3(256), 0, s, b, pairs, npairs, c);
+}
+
+void launch_ccd_prepare(const DevBodies& b, CcdBufs& c, float dt_sub, float last_toi, hipStream_t s) {
+  if (b.n <= 0) return;
+  hipLaunchKernelGGL(k_ccd_prepare, dim3(blocks_of(b.n, 256)), dim3(256), 0, s, b, c, dt_sub, last_toi);
+}
+
+void launch_ccd_compact(const int2* pairs, int npairs, const uint32_t* flag, CcdBufs& c, hipStream_t s) {
+  if (npairs <= 0) {
+    fill_u32(c.ctl + kCcdList, 0u, 1, s);
+    return;
+  }
+  scan_exclusive_u32(flag, c.cpos, npairs, c.tmp, s);
+  hipLaunchKernelGGL(k_ccd_scatter, dim3(blocks_of(npairs, 256)), dim3(256), 0, s, pairs, npairs, flag, c);
+}
+
+void launch_ccd_grow(const DevBodies& b, CcdBufs& c, int nlist, const uint32_t* count, hipStream_t s) {
+  if (nlist <= 0) return;
+  hipLaunchKernelGGL(k_ccd_grow, dim3(blocks_of(nlist, 256)), dim3(256), 0, s, b, c, nlist, count);
+}
+
+void launch_ccd_merge(int n, CcdBufs& c, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_ccd_merge, dim3(blocks_of(n, 256)), dim3(256), 0, s, n, c);
+}
+
+void launch_ccd_validate(const DevBodies& b, const int2* pairs, CcdBufs& c, int nimp, hipStream_t s) {
+  if (nimp <= 0) return;
+  hipLaunchKernelGGL(k_ccd_validate, dim3(blocks_of(nimp, 256)), dim3(256), 0, s, b, pairs, c, nimp);
+}
+
+void launch_ccd_clamp(const DevBodies& b, CcdBufs& c, hipStream_t s) {
+  if (b.n <= 0) return;
+  hipLaunchKernelGGL(k_ccd_clamp, dim3(blocks_of(b.n, 256)), dim3(256), 0, s, b, c);
+}
+
+}  // namespace nph

@@ -467,4 +467,177 @@ NPH_HD Contact reevaluate(int kind, f3 lp1, f3 lp2, f3 ln, int k1, f3 h1, f3 t1,
   return c;
 }
 
+// ---------------------------------------------------------------- time of impact
+// Conservative advancement (Mirtich, "Impulse-based dynamic simulation of rigid body systems",
+// 1996, ch. 2) standing in for ncollide's query::time_of_impact / nonlinear_time_of_impact, which
+// the reference's CCD calls (src/world/mechanical_world.rs:975-1002): at time t the separation d of
+// the two shapes along a direction n is measured; no point approaches the other shape along n
+// faster than a = |w1| rho1 + |w2| rho2 - (v2 - v1) . n (rho: the largest distance from the
+// rotation centre to a point of the shape, 0 for a ball), so the separation stays above the target
+// for a further (d - target) / a. Every iterate is a lower bound of the first time the distance
+// reaches the target.
+
+// Separation of two shapes (kinds k1, k2, half extents h1, h2; balls (r, r, r)) at the given poses
+// and the unit direction n from shape 1 to shape 2 it is measured along: the distance for ball/ball
+// and ball/cuboid (negative when overlapping); for cuboid/cuboid the largest separation over the 15
+// axes of collide_box_box, a lower bound of the distance, exact for face and edge-edge closest
+// features.
+NPH_HD float toi_separation(int k1, f3 h1, f3 t1, q4 q1, int k2, f3 h2, f3 t2, q4 q2, f3& n) {
+  if (k1 == SHAPE_BALL && k2 == SHAPE_BALL) {
+    const f3 d = sub(t2, t1);
+    const float dist2 = dot(d, d);
+    float dist = 0.0f;
+    n = f3{1.0f, 0.0f, 0.0f};
+    if (dist2 > 0.0f) {
+      dist = sqrtf(dist2);
+      n = f3{d.x / dist, d.y / dist, d.z / dist};
+    }
+    return dist - (h1.x + h2.x);
+  }
+  if (k1 == SHAPE_BALL || k2 == SHAPE_BALL) {
+    const bool ball_first = k1 == SHAPE_BALL;
+    const f3 c = ball_first ? t1 : t2;
+    const float r = ball_first ? h1.x : h2.x;
+    const f3 bt = ball_first ? t2 : t1;
+    const q4 bq = ball_first ? q2 : q1;
+    const f3 h = ball_first ? h2 : h1;
+    const m33 R = qmat(bq);
+    const f3 p = mulv_t(R, sub(c, bt));
+    const f3 q{fmaxf_(-h.x, fminf_(p.x, h.x)), fmaxf_(-h.y, fminf_(p.y, h.y)), fmaxf_(-h.z, fminf_(p.z, h.z))};
+    f3 nl{0.0f, 0.0f, 0.0f};
+    float sep = 0.0f;
+    bool outside = false;
+    if (!((p.x == q.x) && (p.y == q.y) && (p.z == q.z))) {
+      const f3 dl = sub(p, q);
+      const float dist2 = dot(dl, dl);
+      if (dist2 > 0.0f) {
+        const float dist = sqrtf(dist2);
+        nl = f3{dl.x / dist, dl.y / dist, dl.z / dist};
+        sep = dist - r;
+        outside = true;
+      }
+    }
+    if (!outside) {
+      const float px = h.x - fabsf(p.x), py = h.y - fabsf(p.y), pz = h.z - fabsf(p.z);
+      int ax = 0;
+      float best = px;
+      if (py < best) { best = py; ax = 1; }
+      if (pz < best) { best = pz; ax = 2; }
+      set_comp(nl, ax, comp(p, ax) >= 0.0f ? 1.0f : -1.0f);
+      sep = -(r + best);
+    }
+    const f3 nw = mulv(R, nl);
+    n = ball_first ? neg(nw) : nw;
+    return sep;
+  }
+  const m33 RA = qmat(q1), RB = qmat(q2);
+  const f3 d = sub(t2, t1);
+  f3 a[3], b[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    a[i] = mcol(RA, i);
+    b[i] = mcol(RB, i);
+  }
+  float AC[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) AC[i][j] = fabsf(dot(a[i], b[j]));
+  float best = -INFINITY;
+  f3 axis = a[0];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    const float rb = (h2.x * AC[i][0] + h2.y * AC[i][1]) + h2.z * AC[i][2];
+    const float s = fabsf(dot(d, a[i])) - (comp(h1, i) + rb);
+    if (s > best) { best = s; axis = a[i]; }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const float ra = (h1.x * AC[0][j] + h1.y * AC[1][j]) + h1.z * AC[2][j];
+    const float s = fabsf(dot(d, b[j])) - (ra + comp(h2, j));
+    if (s > best) { best = s; axis = b[j]; }
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const f3 L = cross(a[i], b[j]);
+      const float l2 = dot(L, L);
+      if (l2 < 1e-6f) continue;
+      const float l = sqrtf(l2);
+      const f3 Ln{L.x / l, L.y / l, L.z / l};
+      const float ra = (h1.x * fabsf(dot(a[0], Ln)) + h1.y * fabsf(dot(a[1], Ln))) + h1.z * fabsf(dot(a[2], Ln));
+      const float rb = (h2.x * fabsf(dot(b[0], Ln)) + h2.y * fabsf(dot(b[1], Ln))) + h2.z * fabsf(dot(b[2], Ln));
+      const float s = fabsf(dot(d, Ln)) - (ra + rb);
+      if (s > best) { best = s; axis = Ln; }
+    }
+  n = dot(d, axis) < 0.0f ? neg(axis) : axis;
+  return best;
+}
+
+// Rigid motion over a CCD interval (BodyPartMotion, src/object/body.rs:15-46): the pose at time t
+// is (t0p, q0) moved for t - t0 at velocity v and, when rotating, angular velocity w about the
+// centre (apply_displacement's composition); linear motions keep the rotation; static ones the pose.
+enum : int { MOTION_STATIC = 0, MOTION_LINEAR = 1, MOTION_ROTATING = 2 };
+struct CcdMotion {
+  int kind;
+  f3 t0p;
+  q4 q0;
+  f3 v, w;
+  float t0;
+  NPH_HD void at(float t, f3& tp, q4& q) const {
+    tp = t0p;
+    q = q0;
+    if (kind == MOTION_STATIC) return;
+    const float dt = t - t0;
+    if (kind == MOTION_LINEAR) {
+      tp = add(t0p, scale(v, dt));
+      return;
+    }
+    apply_displacement(tp, q, scale(v, dt), scale(w, dt));
+  }
+};
+
+enum : int { TOI_CONVERGED = 0, TOI_OUT_OF_ITERATIONS = 1, TOI_PENETRATING = 2 };
+constexpr int kToiMaxIterations = 32;
+constexpr float kToiTolerance = 1e-4f;
+
+NPH_HD float rotation_reach(int kind, f3 h) { return kind == SHAPE_BALL ? 0.0f : sqrtf(dot(h, h)); }
+
+// First time in [0, max_toi] at which the separation falls to target (+ kToiTolerance); returns
+// false when the shapes never get that close before max_toi. *status: TOI_*.
+NPH_HD bool time_of_impact(int k1, f3 h1, const CcdMotion& m1, int k2, f3 h2, const CcdMotion& m2, float max_toi,
+                           float target, float* toi, int* status) {
+  const f3 z{0.0f, 0.0f, 0.0f};
+  const f3 v1 = m1.kind == MOTION_STATIC ? z : m1.v, v2 = m2.kind == MOTION_STATIC ? z : m2.v;
+  const f3 w1 = m1.kind == MOTION_ROTATING ? m1.w : z, w2 = m2.kind == MOTION_ROTATING ? m2.w : z;
+  const float spin = sqrtf(dot(w1, w1)) * rotation_reach(k1, h1) + sqrtf(dot(w2, w2)) * rotation_reach(k2, h2);
+  const f3 dv = sub(v2, v1);
+  float tau = 0.0f;
+  for (int it = 0; it < kToiMaxIterations; ++it) {
+    f3 p1, p2, n;
+    q4 r1, r2;
+    m1.at(tau, p1, r1);
+    m2.at(tau, p2, r2);
+    const float d = toi_separation(k1, h1, p1, r1, k2, h2, p2, r2, n);
+    if (it == 0 && d < 0.0f) {
+      *toi = 0.0f;
+      *status = TOI_PENETRATING;
+      return true;
+    }
+    if (d <= target + kToiTolerance) {
+      *toi = tau;
+      *status = TOI_CONVERGED;
+      return true;
+    }
+    const float a = spin - dot(dv, n);
+    if (!(a > 0.0f)) return false;
+    tau = tau + (d - target) / a;
+    if (tau > max_toi) return false;
+  }
+  *toi = tau;
+  *status = TOI_OUT_OF_ITERATIONS;
+  return true;
+}
+
 }  // namespace nph

@@ -8,8 +8,10 @@ namespace nph {
 
 // k_bodies.hip
 void launch_prestep(const DevBodies& b, const SimParams& p, hipStream_t s);
-void launch_integrate(const DevBodies& b, const SimParams& p, hipStream_t s);
+// only: when given, the bodies with only[i] != 0 (a CCD substep's island)
+void launch_integrate(const DevBodies& b, const SimParams& p, hipStream_t s, const uint32_t* only = nullptr);
 void launch_clear_dyn_marks(const DevBodies& b, hipStream_t s);
+void launch_save_rot_dyn(const DevBodies& b, float4* out, hipStream_t s);
 void launch_apply_force(const DevBodies& b, const SimParams& p, int nu, const int* ubody, const int* off,
                         const int* order, const float* forces, const float* points, int type, int wake,
                         const float4* rot_dyn, hipStream_t s);
@@ -214,6 +216,54 @@ void launch_joint_cache(const DevBodies& b, const SimParams& p, JointBufs& j, hi
 void launch_joint_residual(const DevBodies& b, JointBufs& j, double* out, hipStream_t s);
 // union of the bodies of every joint neither broken nor removed (activation islands)
 void launch_joint_union(const DevBodies& b, const JointBufs& j, int* parent, uint32_t* touch, hipStream_t s);
+
+// k_ccd.hip --------------------------------------------------------------------------------
+// A queued time of impact: key = time bits << 32 | pair index (times are >= 0, so keys order by
+// time, then pair), the pair's CCD slot and the re-sweep stamp it was computed with.
+struct ToiEntry {
+  unsigned long long key;
+  int slot, ts;
+};
+struct CcdBufs {
+  float* body_time;        // [n] time each body's motion starts from (SubstepState::body_times)
+  float* frozen;           // [n] per body key: freeze time, < 0 when not frozen
+  int* slot_pair;          // [P] pair index of each CCD slot (pairs with a CCD-enabled side)
+  uint32_t* ts;            // [P] re-sweep stamp of each slot
+  ToiEntry* heap;          // [cap] first-pass entries, then the queue
+  ToiEntry* stage;         // [P] entries of one re-sweep
+  unsigned long long* impacts;  // [cap] keys of the confirmed impacts, in order
+  uint32_t* ctl;           // control words kCcd*
+  int cap;
+  uint32_t* in_t;          // [n] per key: its colliders are traversed
+  uint32_t* new_t;         // [n] per key: traversed from the next round on
+  uint32_t* prep;          // [n] to be moved to its impact time
+  uint32_t* prepared;      // [n] moved: the substep's island
+  uint32_t* evaluated;     // [P]
+  uint32_t* kept;          // [P] manifold not empty at the moved poses
+  uint32_t* cand;          // [P] to evaluate in this round
+  uint32_t* cpos;          // [P] exclusive scan of a flag array
+  int2* list;              // [P] compacted pairs, pair order
+  int* list_pair;          // [P] their pair indices
+  uint32_t* tmp;           // scan scratch
+};
+enum : int {
+  kCcdSlots = 0, kCcdEntries = 1, kCcdAnyIn = 2, kCcdMinAfter = 3, kCcdKind = 4, kCcdImpacts = 5,
+  kCcdLastToi = 6, kCcdOverflow = 7, kCcdMinOver = 8, kCcdList = 9, kCcdWords = 16
+};
+enum : int { PREDICT_NONE = 0, PREDICT_AFTER_END = 1, PREDICT_IMPACTS = 2 };
+// predict_next_impacts up to end_time: control words reset, freeze times cleared, every CCD pair's
+// time of impact, then the queue in one workgroup (kind, impacts, last impact time in ctl)
+void launch_ccd_predict(const DevBodies& b, const int2* pairs, int npairs, CcdBufs& c, float end_time, float dt0,
+                        float target, int on_penetration, hipStream_t s);
+void launch_ccd_seed(const DevBodies& b, const int2* pairs, CcdBufs& c, int nimp, hipStream_t s);
+void launch_ccd_touch(const DevBodies& b, const int2* pairs, int npairs, CcdBufs& c, hipStream_t s);
+void launch_ccd_prepare(const DevBodies& b, CcdBufs& c, float dt_sub, float last_toi, hipStream_t s);
+// the pairs with flag[p] != 0 into c.list / c.list_pair in pair order, their count to ctl[kCcdList]
+void launch_ccd_compact(const int2* pairs, int npairs, const uint32_t* flag, CcdBufs& c, hipStream_t s);
+void launch_ccd_grow(const DevBodies& b, CcdBufs& c, int nlist, const uint32_t* count, hipStream_t s);
+void launch_ccd_merge(int n, CcdBufs& c, hipStream_t s);
+void launch_ccd_validate(const DevBodies& b, const int2* pairs, CcdBufs& c, int nimp, hipStream_t s);
+void launch_ccd_clamp(const DevBodies& b, CcdBufs& c, hipStream_t s);
 
 // k_position.hip ---------------------------------------------------------------------------
 void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimParams& p, int first, int n,

@@ -23,6 +23,7 @@
 #include <cmath>
 #include <initializer_list>
 #include <limits>
+#include <map>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -66,7 +67,7 @@ struct ContactBufs {
   }
 };
 
-enum StageEv { EV_START, EV_PRE_CD, EV_ASM, EV_VEL, EV_INT, EV_POS, EV_BP, EV_NP, EV_COUNT };
+enum StageEv { EV_START, EV_PRE_CD, EV_ASM, EV_VEL, EV_INT, EV_POS, EV_CCD, EV_BP, EV_NP, EV_COUNT };
 
 // Small device values to pinned host memory without a stream synchronisation: one workgroup copies
 // up to four word ranges into the host buffer, fences at system scope and then writes the
@@ -213,6 +214,21 @@ struct World {
   int nj = 0, njcol = 0;
   DBuf j_info, j_a1, j_a2, j_f1, j_f2, j_brk, j_imp_lin, j_imp_ang, j_state, j_order, j_crange, j_rec, j_lam, jd_jlam;
 
+  // CCD, once some body is CCD-enabled: per-body flags (bit 0 ccd_enabled, bit 1 linear motion
+  // interpolation) and keys (a static collider that shares a static body takes the index of that
+  // body's first collider), the CcdBufs arrays, the scratch manifolds of the island rounds, and the
+  // last prediction's results
+  bool any_ccd = false;
+  std::vector<int> h_ccd, h_ccd_key;
+  std::map<std::pair<int, int>, int> static_owner;  // (scene, static body index) -> first collider
+  DBuf ccd_flags, ccd_keys, ccd_rot, ccd_time, ccd_frozen, ccd_slot, ccd_ts, ccd_heap, ccd_stage, ccd_imp, ccd_ctl,
+      ccd_in, ccd_new, ccd_prep, ccd_prepared, ccd_eval, ccd_kept, ccd_cand, ccd_cpos, ccd_list, ccd_lpair, ccd_tmp;
+  ContactBufs ccd_cb;
+  int ccd_cap = 0, ccd_nimp = 0;
+  float ccd_last_toi = 0.0f, ccd_min_over = 0.0f;
+  bool ccd_sub = false;        // a CCD substep is being solved: no joint rows, per-colour launches
+  float dyn_dt_next = 0.0f;    // time step of the bodies' last dynamics update (a substep's after one)
+
   // Readback buffer (pinned, device-mapped, coherent): words [0, 64) small per-step values,
   // [kRbClass, +2 (kMaxColours + 2)) the class offsets, kRbFlag the sequence flag.
   static constexpr int kRbClass = 64, kRbFlag = kRbClass + 2 * (kMaxColours + 2) + 64, kRbWords = kRbFlag + 64;
@@ -259,6 +275,8 @@ struct World {
     sp.gx = float(p.gravity[0]);
     sp.gy = float(p.gravity[1]);
     sp.gz = float(p.gravity[2]);
+    sp.dyn_dt = sp.dt;
+    dyn_dt_next = sp.dt;
   }
 
   int init(const nphys_params& p) {
@@ -312,6 +330,9 @@ struct World {
     b.f_lin = forces_pending ? f_lin.as<float4>() : nullptr;
     b.f_ang = forces_pending ? f_ang.as<float4>() : nullptr;
     b.w_dyn = w_dyn.as<float4>();
+    b.ccd = any_ccd ? ccd_flags.as<int>() : nullptr;
+    b.ccd_key = any_ccd ? ccd_keys.as<int>() : nullptr;
+    b.ccd_rot = any_ccd ? ccd_rot.as<float4>() : nullptr;
     return b;
   }
 
@@ -575,6 +596,18 @@ struct World {
                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       h_surf.push_back(sv);
       any_surf = any_surf || sv.x != 0.0f || sv.y != 0.0f || sv.z != 0.0f;
+      // ColliderDesc::ccd_enabled, RigidBody::enable_linear_motion_interpolation, and the static body
+      // a static collider belongs to
+      const int cf = ((d.ccd_enabled && d.ccd_enabled[k]) ? 1 : 0) |
+                     ((st == NPHYS_STATUS_DYNAMIC && d.linear_motion_interpolation && d.linear_motion_interpolation[k])
+                          ? 2
+                          : 0);
+      h_ccd.push_back(cf);
+      any_ccd = any_ccd || (cf & 1) != 0;
+      int ckey = n0 + k;
+      if (st != NPHYS_STATUS_DYNAMIC && d.static_body && d.static_body[k] >= 0)
+        ckey = static_owner.emplace(std::make_pair(h_scene.back(), int(d.static_body[k])), n0 + k).first->second;
+      h_ccd_key.push_back(ckey);
       const float* p = d.pos + 3 * k;
       const float* q = d.rot + 4 * k;
       hp[k] = make_float4(p[0], p[1], p[2], 0.0f);
@@ -606,8 +639,10 @@ struct World {
     for (DBuf* b : {&w_dyn, &f_lin, &f_ang}) HIPCK(b->ensure(size_t(n1) * f4b, true, stream));
     for (DBuf* b : {&pos0, &rot0, &ext_lin, &ext_ang, &dims, &local_i, &damp, &surf, &rot_dyn})
       HIPCK(b->ensure(size_t(n1) * f4b));
-    for (DBuf* b : {&status, &shape, &scene, &margin, &qlim, &friction, &restitution, &mass, &bflags, &matid})
+    for (DBuf* b : {&status, &shape, &scene, &margin, &qlim, &friction, &restitution, &mass, &bflags, &matid,
+                    &ccd_flags, &ccd_keys})
       HIPCK(b->ensure(size_t(n1) * 4));
+    HIPCK(ccd_rot.ensure(size_t(n1) * f4b));
     // scene bits above the Morton code: the BVH keeps every scene in its own key range
     int max_scene = 0;
     for (int sc : h_scene) max_scene = std::max(max_scene, sc);
@@ -630,6 +665,8 @@ struct World {
     HIPCK(up(surf, h_surf.data(), size_t(n1) * f4b, 0));
     HIPCK(up(bflags, h_flags.data(), size_t(n1) * 4, 0));
     HIPCK(up(matid, h_matid.data(), size_t(n1) * 4, 0));
+    HIPCK(up(ccd_flags, h_ccd.data(), size_t(n1) * 4, 0));
+    HIPCK(up(ccd_keys, h_ccd_key.data(), size_t(n1) * 4, 0));
     // new bodies: no dynamics update yet (the augmented mass is the world inertia, w = 0) and no
     // pending force
     HIPCK(hipMemsetAsync(w_dyn.as<float4>() + n0, 0, size_t(d.n) * f4b, stream));
@@ -712,8 +749,12 @@ struct World {
     if (r && !rot_dyn_saved) {
       // impulses applied before the next step use the augmented mass of the rotation the last
       // dynamics update saw (the reference recomputes it only in the next step)
-      HIPCK(hipMemcpy2DAsync(rot_dyn.p, sizeof(float4), pose.as<float4>() + 1, kPoseBlock * sizeof(float4),
-                             sizeof(float4), size_t(n), hipMemcpyDeviceToDevice, stream));
+      if (any_ccd) {
+        launch_save_rot_dyn(bodies(), rot_dyn.as<float4>(), stream);
+      } else {
+        HIPCK(hipMemcpy2DAsync(rot_dyn.p, sizeof(float4), pose.as<float4>() + 1, kPoseBlock * sizeof(float4),
+                               sizeof(float4), size_t(n), hipMemcpyDeviceToDevice, stream));
+      }
       rot_dyn_saved = true;
     }
     std::vector<float4> hp(n), hr(n), hv(n), hw(n);
@@ -735,7 +776,7 @@ struct World {
     HIPCK(pose_copy(1, hr.data(), 0, n, hipMemcpyHostToDevice));
     HIPCK(hipMemcpyAsync(linvel.p, hv.data(), n * f4b, hipMemcpyHostToDevice, stream));
     HIPCK(hipMemcpyAsync(angvel.p, hw.data(), n * f4b, hipMemcpyHostToDevice, stream));
-    if (sleeping) launch_clear_dyn_marks(bodies(), stream);
+    if (sleeping || any_ccd) launch_clear_dyn_marks(bodies(), stream);
     HIPCK(hipStreamSynchronize(stream));
     moved = true;
     wake_pending = sleeping;
@@ -878,12 +919,14 @@ struct World {
   }
 
   // The end-of-step syncs (CCD-phase swept AABBs, then the final poses) with one readback.
+  // The swept pass is the first of solve_ccd's loop, which returns before it when dt or
+  // max_ccd_substeps is zero (mechanical_world.rs:578-581).
   int loose_passes_end_of_step(uint32_t* ch_swept, uint32_t* ch_final) {
     int rc = ensure_bp();
     if (rc) return rc;
     BroadPhaseBufs v = bp_view();
     fill_u32(v.changed, 0u, 4, stream);
-    launch_loose_update(bodies(), 2, v.changed, v.dirty, stream);
+    if (sp.dt != 0.0f && prm.max_ccd_substeps != 0) launch_loose_update(bodies(), 2, v.changed, v.dirty, stream);
     launch_loose_update(bodies(), 1, v.changed + 2, v.dirty, stream);
     if ((rc = readback({{v.changed, 4, 0}}))) return rc;
     *ch_swept = rb_host[0];
@@ -996,9 +1039,11 @@ struct World {
   // ------------------------------------------------------------------ narrow phase
   // `done`, when given, is recorded after the narrow phase and before its readback (the step's last
   // stage event, complete once the readback is).
-  int narrow_phase(hipEvent_t done = nullptr) {
-    ContactBufs& out = cb[cur];
-    const size_t P = size_t(std::max(npairs, 1));
+  int narrow_phase(hipEvent_t done = nullptr) { return narrow_phase_of(pairs.as<int2>(), npairs, cb[cur], done); }
+
+  // The narrow phase of `np` pairs (in pair order) into `out`.
+  int narrow_phase_of(const int2* list, int np, ContactBufs& out, hipEvent_t done = nullptr) {
+    const size_t P = size_t(std::max(np, 1));
     // a manifold's first slot is packed in 28 bits of the solver's slot record
     if (kPairSlots * P >= (size_t(1) << 28)) return set_err(NPHYS_ERR_CAPACITY, "too many broad-phase pairs");
     for (DBuf* b : {&np_count, &np_coff, &np_has, &np_moff, &np_perm}) HIPCK(b->ensure(P * 4));
@@ -1014,7 +1059,7 @@ struct World {
     v.moff = np_moff.as<uint32_t>();
     v.totals = bp_small.as<uint32_t>() + 24;
     v.tmp = np_tmp.as<uint32_t>();
-    launch_narrow_phase(bodies(), pairs.as<int2>(), npairs, v, out.view(), stream);
+    launch_narrow_phase(bodies(), list, np, v, out.view(), stream);
     if (done) HIPCK(hipEventRecord(done, stream));
     const int rc = readback({{v.totals, 2, 0}});
     if (rc) return rc;
@@ -1238,12 +1283,13 @@ struct World {
     // the joints' rows: warm-started after the contacts' (sor_prox.rs:56-58), solved before the
     // contacts' in every sweep (sor_prox.rs:167-168)
     JointBufs jv = joint_view();
+    const bool joints = nj > 0 && !ccd_sub;  // a CCD substep solves no joints (mechanical_world.rs:909)
     if (warm) {
       pass(true);
-      if (nj > 0) launch_joint_warm(b, jv, stream);
+      if (joints) launch_joint_warm(b, jv, stream);
     }
     for (int it = 0; it < nsweeps; ++it) {
-      if (nj > 0) launch_joint_sweep(b, jv, stream);
+      if (joints) launch_joint_sweep(b, jv, stream);
       pass(false);
     }
     return NPHYS_OK;
@@ -1608,8 +1654,9 @@ struct World {
     return NPHYS_OK;
   }
 
-  int position_correction() {
-    if (sview_M[cur] == 0 && nj == 0) return NPHYS_OK;
+  int position_correction(const SimParams& ps, int iterations) {
+    const bool joints = nj > 0 && !ccd_sub;
+    if (sview_M[cur] == 0 && !joints) return NPHYS_OK;
     DevBodies b = bodies();
     SolverBufs v = sv_view();
     JointBufs jv = joint_view();
@@ -1619,17 +1666,17 @@ struct World {
     // sweeps); the joints' constraints first in every iteration (nonlinear_sor_prox.rs:33-38)
     const int* h_range = h_visit + (kMaxColours + 2);
     const int ncol = sview_M[cur] > 0 ? ncolours : 0;
-    for (int it = 0; it < prm.max_position_iterations; ++it) {
-      if (nj > 0) launch_joint_position(b, sp, jv, stream);
+    for (int it = 0; it < iterations; ++it) {
+      if (joints) launch_joint_position(b, ps, jv, stream);
       for (int k = 0; k < ncol;) {
         int e = k;
         while (e < ncolours && h_range[2 * e + 1] <= kRunBlock) ++e;
         if (e - k >= 2) {
-          launch_position_run(b, sp, v, k, e - k, stream);
+          launch_position_run(b, ps, v, k, e - k, stream);
           k = e;
           continue;
         }
-        launch_position_colour(b, cv, sp, h_range[2 * k], h_range[2 * k + 1],
+        launch_position_colour(b, cv, ps, h_range[2 * k], h_range[2 * k + 1],
                                std::min(pblocks, std::max(1, (h_range[2 * k + 1] + 63) / 64)), v, stream);
         ++k;
       }
@@ -1637,10 +1684,190 @@ struct World {
     return NPHYS_OK;
   }
 
+
+  // ------------------------------------------------------------------ CCD
+  int ensure_ccd() {
+    const size_t nn = size_t(std::max(n, 1)), P = size_t(std::max(npairs, 1));
+    for (DBuf* b : {&ccd_time, &ccd_frozen, &ccd_in, &ccd_new, &ccd_prep, &ccd_prepared}) HIPCK(b->ensure(nn * 4));
+    for (DBuf* b : {&ccd_slot, &ccd_ts, &ccd_eval, &ccd_kept, &ccd_cand, &ccd_cpos, &ccd_lpair}) HIPCK(b->ensure(P * 4));
+    HIPCK(ccd_list.ensure(P * sizeof(int2)));
+    HIPCK(ccd_stage.ensure(P * sizeof(ToiEntry)));
+    // queue capacity: every pair's first-pass entry and re-sweeps of twice as many (a full queue is
+    // an error, never a silent loss)
+    const size_t cap = 3 * P + 4096;
+    HIPCK(ccd_heap.ensure(cap * sizeof(ToiEntry)));
+    HIPCK(ccd_imp.ensure(cap * sizeof(unsigned long long)));
+    ccd_cap = int(std::min<size_t>(ccd_heap.bytes / sizeof(ToiEntry), ccd_imp.bytes / sizeof(unsigned long long)));
+    HIPCK(ccd_ctl.ensure(kCcdWords * 4));
+    HIPCK(ccd_tmp.ensure(scan_temp_words(int(P)) * 4 + 64));
+    return NPHYS_OK;
+  }
+
+  CcdBufs ccd_view() {
+    CcdBufs c;
+    c.body_time = ccd_time.as<float>();
+    c.frozen = ccd_frozen.as<float>();
+    c.slot_pair = ccd_slot.as<int>();
+    c.ts = ccd_ts.as<uint32_t>();
+    c.heap = ccd_heap.as<ToiEntry>();
+    c.stage = ccd_stage.as<ToiEntry>();
+    c.impacts = ccd_imp.as<unsigned long long>();
+    c.ctl = ccd_ctl.as<uint32_t>();
+    c.cap = ccd_cap;
+    c.in_t = ccd_in.as<uint32_t>();
+    c.new_t = ccd_new.as<uint32_t>();
+    c.prep = ccd_prep.as<uint32_t>();
+    c.prepared = ccd_prepared.as<uint32_t>();
+    c.evaluated = ccd_eval.as<uint32_t>();
+    c.kept = ccd_kept.as<uint32_t>();
+    c.cand = ccd_cand.as<uint32_t>();
+    c.cpos = ccd_cpos.as<uint32_t>();
+    c.list = ccd_list.as<int2>();
+    c.list_pair = ccd_lpair.as<int>();
+    c.tmp = ccd_tmp.as<uint32_t>();
+    return c;
+  }
+
+  // The CCD phase's sync and broad phase with swept AABBs (position0 -> position).
+  int ccd_broad_phase() {
+    uint32_t ch = 0;
+    int rc = loose_pass(2, &ch);
+    if (rc) return rc;
+    counters.loose_changes += int64_t(ch);
+    if (ch || pairs_dirty) {
+      if ((rc = rebuild_pairs())) return rc;
+      ++counters.pair_rebuilds;
+    }
+    return ensure_ccd();
+  }
+
+  // predict_next_impacts up to end_time: sets *kind (PREDICT_*), the impact count, the last
+  // impact time and min_overstep.
+  int ccd_predict(float end_time, float dt0, int* kind) {
+    CcdBufs c = ccd_view();
+    launch_ccd_predict(bodies(), pairs.as<int2>(), npairs, c, end_time, dt0, sp.allowed_linear_error,
+                       prm.ccd_on_penetration_enabled, stream);
+    const int rc = readback({{c.ctl, kCcdWords, 0}});
+    if (rc) return rc;
+    if (rb_host[kCcdOverflow]) return set_err(NPHYS_ERR_CAPACITY, "CCD impact queue overflow");
+    *kind = int(rb_host[kCcdKind]);
+    ccd_nimp = *kind == PREDICT_IMPACTS ? int(rb_host[kCcdImpacts]) : 0;
+    uint32_t u = rb_host[kCcdLastToi];
+    std::memcpy(&ccd_last_toi, &u, 4);
+    u = rb_host[kCcdMinOver];
+    std::memcpy(&ccd_min_over, &u, 4);
+    return NPHYS_OK;
+  }
+
+  // One substep (mechanical_world.rs:630-935): the island moved to the last impact time, its
+  // manifolds at the moved poses, then MoreauJeanSolver::step_ccd over dt0 - last_toi: assembly
+  // without warm start (coefficient 0), max_ccd_position_iterations position iterations, the
+  // impacting bodies' advancement validated, the velocity sweeps, integration of the island.
+  int ccd_substep(float dt0) {
+    int rc;
+    ++counters.ccd_substeps;
+    counters.ccd_impacts += ccd_nimp;
+    const float last_toi = ccd_last_toi;
+    const float dt_sub = dt0 - last_toi;
+    const DevBodies b = bodies();
+    CcdBufs c = ccd_view();
+    const size_t nn = size_t(n), P = size_t(std::max(npairs, 1));
+    for (uint32_t* a : {c.in_t, c.new_t, c.prep, c.prepared}) fill_u32(a, 0u, nn, stream);
+    for (uint32_t* a : {c.evaluated, c.kept}) fill_u32(a, 0u, P, stream);
+    launch_ccd_seed(b, pairs.as<int2>(), c, ccd_nimp, stream);
+    for (;;) {  // island rounds
+      launch_ccd_touch(b, pairs.as<int2>(), npairs, c, stream);
+      launch_ccd_prepare(b, c, dt_sub, last_toi, stream);
+      launch_ccd_compact(pairs.as<int2>(), npairs, c.cand, c, stream);
+      if ((rc = readback({{c.ctl + kCcdList, 1, 0}}))) return rc;
+      const int nl = int(rb_host[0]);
+      if (nl == 0) break;
+      if ((rc = narrow_phase_of(c.list, nl, ccd_cb))) return rc;
+      launch_ccd_grow(b, c, nl, np_count.as<uint32_t>(), stream);
+      launch_ccd_merge(n, c, stream);
+    }
+    // the island's manifolds, in pair order, into cb[cur] (the final narrow phase rewrites it)
+    launch_ccd_compact(pairs.as<int2>(), npairs, c.kept, c, stream);
+    if ((rc = readback({{c.ctl + kCcdList, 1, 0}}))) return rc;
+    const int nk = int(rb_host[0]);
+    ContactBufs& cc = cb[cur];
+    if ((rc = narrow_phase_of(c.list, nk, cc))) return rc;
+    const int M = cc.M, C = cc.C;
+    SimParams ss = sp;
+    ss.dt = dt_sub;
+    ss.inv_dt = dt_sub == 0.0f ? 0.0f : 1.0f / dt_sub;
+    ss.warmstart_coeff = 0.0f;
+    ss.gx = ss.gy = ss.gz = 0.0f;
+    ss.dyn_dt = dt_sub;
+    const DevContacts cv = cc.view();
+    ++solve_count;
+    ccd_sub = true;
+    const int saved_variant = cur_variant;
+    cur_variant = 0;
+    SolverBufs v{};
+    if (M > 0) {
+      if ((rc = ensure_solver(M, C))) return rc;
+      v = sv_view();
+      if ((rc = colour(b, cv, M, C, v))) return rc;
+      const ContactBufs& pc = cb[cur ^ 1];
+      launch_assemble(b, cv, pc.view(), have_solved ? pc.M : 0, M, ss, v, mat_table(), stream);
+      if ((rc = visit_order())) return rc;
+      sview[cur] = cv;
+      sview_M[cur] = M;
+      sview_C[cur] = C;
+      if ((rc = position_correction(ss, prm.max_ccd_position_iterations))) return rc;
+    }
+    launch_ccd_validate(b, pairs.as<int2>(), c, ccd_nimp, stream);
+    if (M > 0 && (rc = sweeps(b, cv, v, sweep_blocks(max_class), true, prm.max_velocity_iterations))) return rc;
+    launch_integrate(b, ss, stream, c.prepared);
+    ccd_sub = false;
+    cur_variant = saved_variant;
+    // every moved body's dynamics are updated with the substep's time step after it (:915-923)
+    dyn_dt_next = dt_sub;
+    return NPHYS_OK;
+  }
+
+  // MechanicalWorld::solve_ccd (mechanical_world.rs:559-943).
+  int solve_ccd() {
+    int rc;
+    const float dt0 = sp.dt;
+    if (dt0 == 0.0f || prm.max_ccd_substeps == 0) return NPHYS_OK;
+    const float sub_len = dt0 / float(prm.max_ccd_substeps);
+    if ((rc = ensure_ccd())) return rc;
+    fill_u32(ccd_time.as<uint32_t>(), 0u, size_t(n), stream);
+    float end_time = 0.0f;
+    bool last_iter = false;
+    for (;;) {
+      if (end_time < dt0) end_time = fminf_(end_time + sub_len, dt0);
+      else last_iter = true;
+      if ((rc = ccd_broad_phase())) return rc;
+      int kind = PREDICT_NONE;
+      if ((rc = ccd_predict(end_time, dt0, &kind))) return rc;
+      if (kind == PREDICT_AFTER_END) {
+        end_time = std::floor(ccd_min_over / sub_len) * sub_len;
+        end_time = fminf_(end_time + sub_len, dt0);
+        if ((rc = ccd_predict(end_time, dt0, &kind))) return rc;
+        if (kind != PREDICT_IMPACTS) return set_err(NPHYS_ERR_STATE, "CCD: no impact in the window of the earliest one");
+      }
+      if (!last_iter && ccd_nimp > 0) {
+        if ((rc = ccd_substep(dt0))) return rc;
+        continue;
+      }
+      CcdBufs c = ccd_view();
+      launch_ccd_clamp(bodies(), c, stream);  // the last prediction's frozen bodies
+      return NPHYS_OK;
+    }
+  }
+
   // ------------------------------------------------------------------ step
   int step_once() {
     int rc;
     HIPCK(hipEventRecord(ev[EV_START], stream));
+    // bodies moved from the host are updated at their new state with this step's dt; the others
+    // keep the time step of their last update (a CCD substep's after one)
+    SimParams spp = sp;
+    spp.dyn_dt = moved ? sp.dt : dyn_dt_next;
+    dyn_dt_next = sp.dt;
     if (moved) {
       if ((rc = collide_now(0))) return rc;
       moved = false;
@@ -1650,21 +1877,35 @@ struct World {
       launch_wake(bodies(), stream);
       wake_pending = false;
     }
-    launch_prestep(bodies(), sp, stream);
+    launch_prestep(bodies(), spp, stream);
     if ((rc = solve())) return rc;
     launch_integrate(bodies(), sp, stream);
     HIPCK(hipEventRecord(ev[EV_INT], stream));
-    if ((rc = position_correction())) return rc;
+    if ((rc = position_correction(sp, prm.max_position_iterations))) return rc;
     HIPCK(hipEventRecord(ev[EV_POS], stream));
     // this step's manifolds and impulses become the warm-start cache
     cur ^= 1;
     have_solved = true;
-    uint32_t ch1 = 0, ch2 = 0;
-    if ((rc = loose_passes_end_of_step(&ch1, &ch2))) return rc;  // CCD-phase sync, then the final one
-    counters.loose_changes += int64_t(ch1) + int64_t(ch2);
-    if (ch1 || ch2 || pairs_dirty) {
-      if ((rc = rebuild_pairs())) return rc;
-      ++counters.pair_rebuilds;
+    if (any_ccd) {
+      // the CCD phase (its swept syncs and broad phases, impacts and substeps), then the final sync
+      if ((rc = solve_ccd())) return rc;
+      HIPCK(hipEventRecord(ev[EV_CCD], stream));
+      uint32_t ch = 0;
+      if ((rc = loose_pass(1, &ch))) return rc;
+      counters.loose_changes += int64_t(ch);
+      if (ch || pairs_dirty) {
+        if ((rc = rebuild_pairs())) return rc;
+        ++counters.pair_rebuilds;
+      }
+    } else {
+      HIPCK(hipEventRecord(ev[EV_CCD], stream));
+      uint32_t ch1 = 0, ch2 = 0;
+      if ((rc = loose_passes_end_of_step(&ch1, &ch2))) return rc;  // CCD-phase sync, then the final one
+      counters.loose_changes += int64_t(ch1) + int64_t(ch2);
+      if (ch1 || ch2 || pairs_dirty) {
+        if ((rc = rebuild_pairs())) return rc;
+        ++counters.pair_rebuilds;
+      }
     }
     HIPCK(hipEventRecord(ev[EV_BP], stream));
     // the narrow phase's readback completes the step: every earlier event and copy is done after it
@@ -1676,8 +1917,9 @@ struct World {
       return double(ms) * 1e-3;
     };
     counters.step += el(EV_START, EV_NP);
-    counters.collision_detection += el(EV_START, EV_PRE_CD) + el(EV_POS, EV_NP);
-    counters.broad_phase += el(EV_POS, EV_BP);
+    counters.collision_detection += el(EV_START, EV_PRE_CD) + el(EV_CCD, EV_NP);
+    counters.broad_phase += el(EV_CCD, EV_BP);
+    counters.ccd_time += el(EV_POS, EV_CCD);
     counters.narrow_phase += el(EV_BP, EV_NP);
     counters.assembly += el(EV_PRE_CD, EV_ASM);
     counters.velocity_resolution += el(EV_ASM, EV_VEL);
@@ -1756,7 +1998,9 @@ struct World {
     HIPCK(hipMemcpyAsync(d_f, forces, size_t(count) * fw * 4, hipMemcpyHostToDevice, stream));
     if (points) HIPCK(hipMemcpyAsync(d_p, points, size_t(count) * 12, hipMemcpyHostToDevice, stream));
     if (type == NPHYS_FORCE || type == NPHYS_ACCELERATION_CHANGE) forces_pending = true;
-    launch_apply_force(bodies(), sp, nu, d_ub, d_off, d_order, d_f, d_p, type, wake, rot_dyn_saved ? rot_dyn.as<float4>() : nullptr,
+    SimParams spf = sp;
+    spf.dyn_dt = dyn_dt_next;  // the time step of the augmented mass the forces act through
+    launch_apply_force(bodies(), spf, nu, d_ub, d_off, d_order, d_f, d_p, type, wake, rot_dyn_saved ? rot_dyn.as<float4>() : nullptr,
                        stream);
     HIPCK(hipStreamSynchronize(stream));  // the staging buffer is reused by the next call
     return NPHYS_OK;
@@ -1934,6 +2178,8 @@ int nphys_set_material_coefficient(nphys_world* w, int32_t which, int32_t id1, i
 int nphys_step(nphys_world* w, int32_t nsteps) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
   if (nsteps < 0) return set_err(NPHYS_ERR_INVALID, "negative step count");
+  if (w->w.any_ccd && (w->w.sleeping || w->w.nscenes > 1))
+    return set_err(NPHYS_ERR_INVALID, "CCD needs one scene and no sleep thresholds");
   for (int i = 0; i < nsteps; ++i) {
     const int rc = w->w.step_once();
     if (rc) return rc;

@@ -1741,8 +1741,9 @@ class World {
     auto keep_prev_colours = prev_colours;
     const int keep_ncolours = ncolours;
     const long keep_rows = n_rows;
-    std::vector<int> keep_joints;
+    std::vector<int> keep_joints, keep_joint_order;
     keep_joints.swap(active_joints);
+    keep_joint_order.swap(joint_order);
     solved = ms;
     solved_contacts = cs;
     assemble(ms, cs);
@@ -1766,6 +1767,7 @@ class World {
     }
     for (auto& b : bodies) update_dynamics(b);  // mechanical_world.rs:915-923
     keep_joints.swap(active_joints);
+    keep_joint_order.swap(joint_order);
     uni = keep_uni; uni_g = keep_uni_g; bil = keep_bil; bil_g = keep_bil_g;
     jbil = keep_jbil; jbil_g = keep_jbil_g;
     posc = keep_posc; crow = keep_crow; jrow = keep_jrow;
@@ -1829,7 +1831,7 @@ class World {
     // then with CCD-enabled bodies the impacts and substeps
     const double tccd = now_s();
     if (any_ccd) solve_ccd();
-    else sync_and_broad_phase(true);
+    else if (prm.dt != T(0) && prm.max_ccd_substeps != 0) sync_and_broad_phase(true);  // (:578-581)
     counters.ccd += now_s() - tccd;
     for (auto& b : bodies) b.pos0 = b.pos;  // validate_advancement
     const double tc = now_s();

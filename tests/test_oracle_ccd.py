@@ -310,6 +310,19 @@ def test_reference_ccd_example_keeps_the_boxes_in(oracle_mod):
     assert not np.all(_inside_room(w2.get_state()["pos"][ns:]))
 
 
+def test_substep_leaves_joints_out(oracle_mod):
+    # a pendulum beside a CCD impact: the substep solves contacts only (mechanical_world.rs:909),
+    # and the joint holds again in the following steps
+    s = merge(_wall(), _ball(-1.0, 200.0), body(shape=0, dims=(0.1, 0, 0), pos=(-0.5, 1.0, 1.0)),
+              body(status=1, shape=0, dims=(0.05, 0, 0), pos=(-0.5, 1.5, 1.0)))
+    w = _world(oracle_mod, s, precision=32, gravity=(0.0, -9.81, 0.0))
+    w.add_joints(abi.joints(abi.JOINT_BALL, 3, 2, (0.0, 0.0, 0.0), (0.0, 0.5, 0.0)))
+    w.step(10)
+    assert w.counters()["ccd_substeps"] >= 1
+    p = w.get_state()["pos"]
+    assert np.linalg.norm(p[2] - p[3]) == pytest.approx(0.5, abs=5e-3)
+
+
 def test_ccd_rejects_sleeping_and_scenes(oracle_mod):
     s = merge(_wall(), _ball(-1.0, 200.0, sleep_threshold=0.01))
     w = _world(oracle_mod, s)
