@@ -554,6 +554,19 @@ def bench_batched(args):
                                      args.cpu_steps, args.no_cpu)), flush=True)
 
 
+def bench_ccd(args):
+    """The 4,096-cuboid CCD room (the reference's examples3d/ccd3.rs scaled), args.steps steps from
+    its initial scene, with the CPU restatement on the same steps unless --no-cpu."""
+    out = ccd_config("the reference's CCD example with 16^3 = 4,096 cuboids in a room of half width 4 m; "
+                     f"{args.steps} steps from the start, dt 1/60",
+                     scenes.ccd_room(num=16, half_width=4.0, height=-1.76), abi.default_params(), args.steps,
+                     0 if args.no_cpu else args.steps)
+    out = dict({"metric": "steps/s of the 4,096-cuboid CCD room", "n_gpus": 1, "warmup": 0,
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                "data": "synthetic"}, **out)
+    print(json.dumps(out), flush=True)
+
+
 def bench_sleeping(args):
     """The batched worlds with the reference's default sleep threshold (0.01) on every body,
     settled with sleeping on: islands at rest leave the solve and the integration (a single dense
@@ -633,9 +646,10 @@ def main():
                          "FETCH_SIZE / WRITE_SIZE counter calibration) and exit")
     ap.add_argument("--sweep-variant", type=int, default=-1,
                     help="-1 autotune (default), 0 per-colour launches, 64/128/256 scene-blocked")
-    ap.add_argument("--workload", choices=("pile", "batched", "sleeping"), default="pile",
+    ap.add_argument("--workload", choices=("pile", "batched", "sleeping", "ccd"), default="pile",
                     help="pile: the dense pile (default); batched: 4,096 independent worlds in one handle; "
-                         "sleeping: the batched worlds with sleep threshold 0.01")
+                         "sleeping: the batched worlds with sleep threshold 0.01; ccd: the reference's CCD "
+                         "example scaled to 4,096 cuboids (--steps steps from the start)")
     ap.add_argument("--sleeping-scene", choices=("batched", "pile"), default="batched",
                     help="scene of --workload sleeping")
     args = ap.parse_args()
@@ -661,6 +675,8 @@ def main():
         return bench_batched(args)
     if args.workload == "sleeping":
         return bench_sleeping(args)
+    if args.workload == "ccd":
+        return bench_ccd(args)
     params = abi.default_params(max_velocity_iterations=16, max_position_iterations=3)
     scene = scenes.pile()
     w = world.MechanicalWorld(integration_parameters=params)
@@ -752,6 +768,8 @@ def main():
         "stages_ms_per_step": {k: 1e3 * c[k] / args.steps for k in
                                ("collision_detection", "broad_phase", "narrow_phase", "assembly",
                                 "velocity_resolution", "velocity_update", "position_resolution")},
+        "stage_rooflines": stage_rooflines(c, args.steps, C, M, n_dyn, c["npairs"], iters,
+                                           params.max_position_iterations),
         "roofline": {
             "kernel": "k_pgs (one colour class of one SOR-Prox sweep, classes of more than 512 manifolds; "
                       "average over every k_pgs launch of the timing region)",
@@ -803,10 +821,79 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def ccd_config(what, scene, params, steps, cpu_steps):
+    """A CCD workload run from its initial scene on the GPU (all `steps` steps timed, counters off;
+    a second run with counters gives the CCD phase's share and the impacts), beside the one-thread
+    FP32 CPU restatement (reference row order) timed on the first `cpu_steps` steps of the same
+    trajectory."""
+    w = gpu_world(scene, params)
+    w.enable_counters(False)
+    t0 = time.perf_counter()
+    w.step(steps)
+    elapsed = time.perf_counter() - t0
+    w.close()
+    w = gpu_world(scene, params)
+    w.enable_counters(True)
+    w.step(steps)
+    c = w.counters()
+    w.close()
+    if cpu_steps > 0:
+        o = oracle_world(scene, params, 32)
+        t1 = time.perf_counter()
+        o.step(cpu_steps)
+        cpu_t = time.perf_counter() - t1
+        oc = o.counters()
+    out = {
+        "workload": what,
+        "bodies": int(scene["status"].shape[0]),
+        "dynamic_bodies": int((scene["status"] == 0).sum()),
+        "steps": steps,
+        "value": steps / elapsed,
+        "unit": "steps/s",
+        "ms_per_step": 1e3 * elapsed / steps,
+        "ccd_ms_per_step": 1e3 * c["ccd_time"] / max(c["nsteps"], 1),
+        "ccd_substeps": c["ccd_substeps"],
+        "ccd_impacts": c["ccd_impacts"],
+        "impacts_per_s": c["ccd_impacts"] / elapsed,
+        "cpu_baseline": None,
+    }
+    if cpu_steps > 0:
+        out["cpu_baseline"] = {
+            "value": cpu_steps / cpu_t, "unit": "steps/s", "cores": 1, "kind": "port", "impacts": oc["ccd_impacts"],
+            "sample": f"the first {cpu_steps} steps of the same run ({oc['ccd_impacts']} impacts), FP32, reference "
+                      f"row order, single thread, on {cpu_model()} ({os.cpu_count()} logical CPUs)"}
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
+def stage_rooflines(c, steps, C, M, N, P, sweeps, pos_iters):
+    """Each stage's algorithmic bytes per step (the per-kernel figures of the design notes) over
+    its measured time per step (stage timers: HIP events on the world's stream), against the
+    8 TB/s HBM peak. The velocity stage counts its sweeps only (the warm start, residual and
+    impulse cache add to its time); the broad phase counts a full pass (the incremental update
+    moves less)."""
+    rows = {
+        "velocity_resolution": (sweeps * (168 * C + 76 * N), "k_warmstart + k_pgs sweeps: (168 C + 76 N) per sweep"),
+        "position_resolution": (pos_iters * (48 * C + 84 * N), "k_position: (48 C + 84 N) per iteration"),
+        "assembly": (224 * C + 104 * N, "colouring + k_assemble: 224 C + 104 N"),
+        "narrow_phase": (100 * P + 48 * C, "narrow phase: 100 P + 48 C"),
+        "broad_phase": (124 * N + 8 * P, "broad phase: 124 N + 8 P (a full pass)"),
+        "velocity_update": (152 * N, "k_integrate (with k_prestep): 152 N"),
+    }
+    out = {}
+    for k, (b, what) in rows.items():
+        t = c[k] / steps
+        gbps = b / t / 1e9 if t > 0 else 0.0
+        out[k] = {"bytes_per_step": int(b), "ms_per_step": 1e3 * t, "achieved": gbps, "unit": "GB/s",
+                  "frac": gbps / HBM_PEAK_GBPS, "bytes": what}
+    return out
+
+
 def configs_section(args):
     """The other workloads, each run whole on the GPU beside the CPU restatement: the ball drop
     (1,000 balls, 8 + 3 iterations, 200 steps), the box stack (8,000 cuboids, 10 + 3 iterations,
-    500 steps) and the batched worlds (4,096 scenes, 8 + 3 iterations)."""
+    500 steps), the reference's CCD example and a 4,096-body version of it, and the batched worlds
+    (4,096 scenes, 8 + 3 iterations)."""
     out = {}
     log("ball drop ...")
     out["ball_drop"] = small_config(
@@ -820,6 +907,17 @@ def configs_section(args):
         "static ground; 10 velocity + 3 position iterations, 500 steps, dt 1/60",
         scenes.boxes(), abi.default_params(max_velocity_iterations=10, max_position_iterations=3),
         500, [0, 125, 250, 375, 490], 3, args.target, 1000)
+    log("CCD example ...")
+    out["ccd_example"] = ccd_config(
+        "the reference's CCD example (examples3d/ccd3.rs): 64 CCD-enabled cuboids (half extent 0.1 m) fired at "
+        "(-100, -10, 0) m/s in a closed room of six walls (one static body, half width 3 m); 8 velocity + 3 "
+        "position iterations, 10 CCD position iterations, 200 steps, dt 1/60",
+        scenes.ccd_room(), abi.default_params(), 200, 200)
+    log("CCD room of 4,096 cuboids ...")
+    out["ccd_room_4096"] = ccd_config(
+        "the same example with 16^3 = 4,096 cuboids in a room of half width 4 m: every body freezes in each "
+        "step's impact cascade; 12 steps, dt 1/60",
+        scenes.ccd_room(num=16, half_width=4.0, height=-1.76), abi.default_params(), 12, 12)
     log("batched worlds ...")
     out["batched"] = batched_section(args.batched_steps, 10, 200, args.sweep_variant, args.cpu_steps,
                                      args.no_cpu)

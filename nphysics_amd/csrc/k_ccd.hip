@@ -30,7 +30,11 @@ namespace nph {
 
 namespace {
 
-constexpr int kEventBlock = 256;
+constexpr int kEventBlock = 512;  // eight waves: one freeze re-sweeps a few hundred swept-AABB pairs in one round
+// The queue lives in LDS while it holds at most this many entries (156 KB of gfx950's 160 KB per
+// workgroup): lane 0's heap operations are chains of dependent loads, 20x shorter from LDS than
+// from HBM; a larger queue moves to HBM for the rest of the kernel.
+constexpr int kLdsHeap = 9984;
 
 __device__ __forceinline__ unsigned long long toi_key(float toi, int pair) {
   return (static_cast<unsigned long long>(__float_as_uint(toi)) << 32) | static_cast<unsigned long long>(uint32_t(pair));
@@ -116,6 +120,24 @@ __global__ void __launch_bounds__(128) k_ccd_first_pass(DevBodies b, const int2*
   }
 }
 
+// The CCD slots of every dynamic body (adjacency in slot order is not needed: the re-sweep's
+// results go into the queue, whose order is by key).
+__global__ void __launch_bounds__(256) k_ccd_degree(DevBodies b, const int2* __restrict__ pairs, int npairs, CcdBufs c) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= npairs || slot >= int(c.ctl[kCcdSlots])) return;
+  const int2 pr = pairs[c.slot_pair[slot]];
+  if (b.status[pr.x] == STATUS_DYNAMIC) atomicAdd(&c.deg[pr.x], 1u);
+  if (b.status[pr.y] == STATUS_DYNAMIC) atomicAdd(&c.deg[pr.y], 1u);
+}
+
+__global__ void __launch_bounds__(256) k_ccd_fill(DevBodies b, const int2* __restrict__ pairs, int npairs, CcdBufs c) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= npairs || slot >= int(c.ctl[kCcdSlots])) return;
+  const int2 pr = pairs[c.slot_pair[slot]];
+  if (b.status[pr.x] == STATUS_DYNAMIC) c.adj[c.off[pr.x] + atomicAdd(&c.fill[pr.x], 1u)] = slot;
+  if (b.status[pr.y] == STATUS_DYNAMIC) c.adj[c.off[pr.y] + atomicAdd(&c.fill[pr.y], 1u)] = slot;
+}
+
 __device__ __forceinline__ bool entry_before(const ToiEntry& a, const ToiEntry& b) {
   return a.key < b.key || (a.key == b.key && a.ts < b.ts);
 }
@@ -145,10 +167,20 @@ __device__ void heap_up(ToiEntry* h, int k) {
 __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const int2* __restrict__ pairs, CcdBufs c,
                                                             float end_time, float dt0, float target,
                                                             int on_penetration) {
-  __shared__ int s_go, s_ovf, s_i, s_j, s_nstage, s_hsize, s_nimp;
+  __shared__ ToiEntry s_heap[kLdsHeap];
+  __shared__ int s_go, s_ovf, s_i, s_j, s_nstage, s_hsize, s_nimp, s_lds;
   const int t = threadIdx.x;
+  {
+    const int nent = int(c.ctl[kCcdEntries]);
+    const bool fits = c.ctl[kCcdOverflow] == 0u && nent <= kLdsHeap;
+    if (fits)
+      for (int k = t; k < nent; k += kEventBlock) s_heap[k] = c.heap[k];
+    if (t == 0) s_lds = fits ? 1 : 0;
+    __syncthreads();
+  }
   if (t == 0) {
     const int nent = int(c.ctl[kCcdEntries]);
+    ToiEntry* heap = s_lds ? s_heap : c.heap;
     const bool any = c.ctl[kCcdAnyIn] != 0u;
     // min_overstep (mechanical_world.rs:412-451): the step's end lowered to the earliest impact
     // after the window, or the window's end when an impact lies within it
@@ -163,7 +195,7 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     c.ctl[kCcdMinOver] = __float_as_uint(min_over);
     const int go = (kind == PREDICT_IMPACTS && c.ctl[kCcdOverflow] == 0u) ? 1 : 0;
     if (go)
-      for (int k = nent / 2 - 1; k >= 0; --k) heap_down(c.heap, nent, k);
+      for (int k = nent / 2 - 1; k >= 0; --k) heap_down(heap, nent, k);
     s_go = go;
     s_ovf = 0;
     s_hsize = nent;
@@ -176,17 +208,18 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
   }
   for (;;) {
     if (t == 0) {
+      ToiEntry* heap = s_lds ? s_heap : c.heap;
       int hs = s_hsize, nimp = s_nimp, found = 0;
       while (hs > 0) {
         if (nimp >= c.cap) {  // the impact list is full: reported as an overflow
           c.ctl[kCcdOverflow] = 1u;
           break;
         }
-        const ToiEntry e = c.heap[0];
+        const ToiEntry e = heap[0];
         --hs;
         if (hs > 0) {
-          c.heap[0] = c.heap[hs];
-          heap_down(c.heap, hs, 0);
+          heap[0] = heap[hs];
+          heap_down(heap, hs, 0);
         }
         if (uint32_t(e.ts) != c.ts[e.slot]) continue;  // superseded by a re-sweep
         const float toi = key_toi(e.key);
@@ -212,15 +245,15 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     }
     __syncthreads();
     if (!s_go) break;
-    // re-sweep the CCD slots of the two bodies' dynamic colliders (mechanical_world.rs:503-551)
+    // re-sweep the CCD slots of the two bodies' dynamic colliders (mechanical_world.rs:503-551);
+    // the event's own pair is in both lists and skipped there (both bodies are frozen)
     const int si = s_i, sj = s_j;
-    const bool di = b.status[si] == STATUS_DYNAMIC, dj = b.status[sj] == STATUS_DYNAMIC;
-    const int ncc = int(c.ctl[kCcdSlots]);
-    for (int slot = t; slot < ncc; slot += kEventBlock) {
+    const int ni = b.status[si] == STATUS_DYNAMIC ? int(c.deg[si]) : 0;
+    const int nj = b.status[sj] == STATUS_DYNAMIC ? int(c.deg[sj]) : 0;
+    for (int k = t; k < ni + nj; k += kEventBlock) {
+      const int slot = k < ni ? c.adj[c.off[si] + k] : c.adj[c.off[sj] + (k - ni)];
       const int p = c.slot_pair[slot];
       const int2 pr = pairs[p];
-      const bool touch = (di && (pr.x == si || pr.y == si)) || (dj && (pr.x == sj || pr.y == sj));
-      if (!touch) continue;
       const float fa = c.frozen[b.ccd_key[pr.x]], fb = c.frozen[b.ccd_key[pr.y]];
       const bool sa = b.status[pr.x] != STATUS_DYNAMIC, sb = b.status[pr.y] != STATUS_DYNAMIC;
       if ((fa >= 0.0f || sa) && (fb >= 0.0f || sb)) continue;
@@ -236,13 +269,18 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     if (t == 0) {
       int hs = s_hsize;
       const int ns = s_nstage;
+      if (s_lds && hs + ns > kLdsHeap) {  // the queue outgrows LDS: it continues in HBM
+        for (int k = 0; k < hs; ++k) c.heap[k] = s_heap[k];
+        s_lds = 0;
+      }
+      ToiEntry* heap = s_lds ? s_heap : c.heap;
       if (hs + ns > c.cap) {
         c.ctl[kCcdOverflow] = 1u;
         s_ovf = 1;
       } else {
         for (int k = 0; k < ns; ++k) {
-          c.heap[hs] = c.stage[k];
-          heap_up(c.heap, hs);
+          heap[hs] = c.stage[k];
+          heap_up(heap, hs);
           ++hs;
         }
       }
@@ -367,9 +405,15 @@ void launch_ccd_predict(const DevBodies& b, const int2* pairs, int npairs, CcdBu
                         float target, int on_penetration, hipStream_t s) {
   hipLaunchKernelGGL(k_ccd_reset, dim3(1), dim3(64), 0, s, c.ctl);
   fill_u32(reinterpret_cast<uint32_t*>(c.frozen), 0xbf800000u, size_t(b.n), s);  // -1
-  if (npairs > 0)
+  fill_u32(c.deg, 0u, size_t(b.n), s);
+  fill_u32(c.fill, 0u, size_t(b.n), s);
+  if (npairs > 0) {
     hipLaunchKernelGGL(k_ccd_first_pass, dim3(blocks_of(npairs, 128)), dim3(128), 0, s, b, pairs, npairs, c,
                        end_time, dt0, target, on_penetration);
+    hipLaunchKernelGGL(k_ccd_degree, dim3(blocks_of(npairs, 256)), dim3(256), 0, s, b, pairs, npairs, c);
+    scan_exclusive_u32(c.deg, c.off, b.n, c.tmp, s);
+    hipLaunchKernelGGL(k_ccd_fill, dim3(blocks_of(npairs, 256)), dim3(256), 0, s, b, pairs, npairs, c);
+  }
   hipLaunchKernelGGL(k_ccd_events, dim3(1), dim3(kEventBlock), 0, s, b, pairs, c, end_time, dt0, target,
                      on_penetration);
 }

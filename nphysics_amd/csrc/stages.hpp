@@ -245,6 +245,10 @@ struct CcdBufs {
   int2* list;              // [P] compacted pairs, pair order
   int* list_pair;          // [P] their pair indices
   uint32_t* tmp;           // scan scratch
+  uint32_t* deg;           // [n] CCD slots of each dynamic body
+  uint32_t* off;           // [n] exclusive scan of deg
+  uint32_t* fill;          // [n]
+  int* adj;                // [2P] the slots of each dynamic body, at off
 };
 enum : int {
   kCcdSlots = 0, kCcdEntries = 1, kCcdAnyIn = 2, kCcdMinAfter = 3, kCcdKind = 4, kCcdImpacts = 5,

@@ -222,7 +222,8 @@ struct World {
   std::vector<int> h_ccd, h_ccd_key;
   std::map<std::pair<int, int>, int> static_owner;  // (scene, static body index) -> first collider
   DBuf ccd_flags, ccd_keys, ccd_rot, ccd_time, ccd_frozen, ccd_slot, ccd_ts, ccd_heap, ccd_stage, ccd_imp, ccd_ctl,
-      ccd_in, ccd_new, ccd_prep, ccd_prepared, ccd_eval, ccd_kept, ccd_cand, ccd_cpos, ccd_list, ccd_lpair, ccd_tmp;
+      ccd_in, ccd_new, ccd_prep, ccd_prepared, ccd_eval, ccd_kept, ccd_cand, ccd_cpos, ccd_list, ccd_lpair, ccd_tmp,
+      ccd_deg, ccd_off, ccd_fill, ccd_adj;
   ContactBufs ccd_cb;
   int ccd_cap = 0, ccd_nimp = 0;
   float ccd_last_toi = 0.0f, ccd_min_over = 0.0f;
@@ -1688,7 +1689,9 @@ struct World {
   // ------------------------------------------------------------------ CCD
   int ensure_ccd() {
     const size_t nn = size_t(std::max(n, 1)), P = size_t(std::max(npairs, 1));
-    for (DBuf* b : {&ccd_time, &ccd_frozen, &ccd_in, &ccd_new, &ccd_prep, &ccd_prepared}) HIPCK(b->ensure(nn * 4));
+    for (DBuf* b : {&ccd_time, &ccd_frozen, &ccd_in, &ccd_new, &ccd_prep, &ccd_prepared, &ccd_deg, &ccd_off, &ccd_fill})
+      HIPCK(b->ensure(nn * 4));
+    HIPCK(ccd_adj.ensure(2 * P * 4));
     for (DBuf* b : {&ccd_slot, &ccd_ts, &ccd_eval, &ccd_kept, &ccd_cand, &ccd_cpos, &ccd_lpair}) HIPCK(b->ensure(P * 4));
     HIPCK(ccd_list.ensure(P * sizeof(int2)));
     HIPCK(ccd_stage.ensure(P * sizeof(ToiEntry)));
@@ -1699,7 +1702,7 @@ struct World {
     HIPCK(ccd_imp.ensure(cap * sizeof(unsigned long long)));
     ccd_cap = int(std::min<size_t>(ccd_heap.bytes / sizeof(ToiEntry), ccd_imp.bytes / sizeof(unsigned long long)));
     HIPCK(ccd_ctl.ensure(kCcdWords * 4));
-    HIPCK(ccd_tmp.ensure(scan_temp_words(int(P)) * 4 + 64));
+    HIPCK(ccd_tmp.ensure(std::max(scan_temp_words(int(P)), scan_temp_words(int(nn))) * 4 + 64));
     return NPHYS_OK;
   }
 
@@ -1725,6 +1728,10 @@ struct World {
     c.list = ccd_list.as<int2>();
     c.list_pair = ccd_lpair.as<int>();
     c.tmp = ccd_tmp.as<uint32_t>();
+    c.deg = ccd_deg.as<uint32_t>();
+    c.off = ccd_off.as<uint32_t>();
+    c.fill = ccd_fill.as<uint32_t>();
+    c.adj = ccd_adj.as<int>();
     return c;
   }
 

@@ -1609,6 +1609,11 @@ class World {
     std::vector<int> ccd_pairs;
     for (size_t p = 0; p < pairs.size(); ++p)
       if (bodies[size_t(pairs[p].first)].ccd || bodies[size_t(pairs[p].second)].ccd) ccd_pairs.push_back(int(p));
+    // the CCD pairs of every dynamic body (the re-sweeps visit a frozen body's pairs only)
+    std::vector<std::vector<int>> body_pairs(bodies.size());
+    for (int p : ccd_pairs)
+      for (const int s : {pairs[size_t(p)].first, pairs[size_t(p)].second})
+        if (!is_static(s)) body_pairs[size_t(s)].push_back(p);
     std::vector<int> ts(pairs.size(), 0);
     auto later = [](const ToiEntry& a, const ToiEntry& b) {
       if (a.toi != b.toi) return a.toi > b.toi;
@@ -1644,9 +1649,8 @@ class World {
       // re-sweep every pair of the two bodies' dynamic colliders with a CCD-enabled side
       for (const int s : {i, j}) {
         if (is_static(s)) continue;
-        for (int p : ccd_pairs) {
+        for (int p : body_pairs[size_t(s)]) {
           const int a = pairs[size_t(p)].first, b = pairs[size_t(p)].second;
-          if (a != s && b != s) continue;
           const T fa = frozen[size_t(key_of(a))], fb = frozen[size_t(key_of(b))];
           if ((fa >= T(0) || is_static(a)) && (fb >= T(0) || is_static(b))) continue;
           ts[size_t(p)] += 1;
