@@ -31,10 +31,6 @@ namespace nph {
 namespace {
 
 constexpr int kEventBlock = 512;  // eight waves: one freeze re-sweeps a few hundred swept-AABB pairs in one round
-// The queue lives in LDS while it holds at most this many entries (156 KB of gfx950's 160 KB per
-// workgroup): lane 0's heap operations are chains of dependent loads, 20x shorter from LDS than
-// from HBM; a larger queue moves to HBM for the rest of the kernel.
-constexpr int kLdsHeap = 9984;
 
 __device__ __forceinline__ unsigned long long toi_key(float toi, int pair) {
   return (static_cast<unsigned long long>(__float_as_uint(toi)) << 32) | static_cast<unsigned long long>(uint32_t(pair));
@@ -108,6 +104,7 @@ __global__ void __launch_bounds__(128) k_ccd_first_pass(DevBodies b, const int2*
   const uint32_t slot = atomicAdd(&c.ctl[kCcdSlots], 1u);
   c.slot_pair[slot] = p;
   c.ts[slot] = 0u;
+  c.pos[slot] = -1;
   float toi;
   if (!pair_toi(b, c, pr.x, pr.y, -1.0f, -1.0f, dt0, target, on_penetration, &toi)) return;
   if (toi > end_time) {
@@ -164,23 +161,95 @@ __device__ void heap_up(ToiEntry* h, int k) {
   h[k] = e;
 }
 
+// ---- the queue in LDS: a 64-ary heap indexed by CCD slot, operated by wave 0 ----
+// Each slot holds at most one live entry (its latest time of impact), so a re-sweep replaces or
+// removes it in place instead of leaving a superseded entry behind: the pop sequence of the valid
+// entries is the reference's, without popping the stale ones. A node's 64 children are compared
+// across the wave in one step (three levels cover 266,305 entries); keys are unique (they hold the
+// pair index), so the order is total. pos[slot] (HBM) is written and read by lane 0 only.
+constexpr int kHeapD = 64;
+constexpr int kLdsCap = 13312;  // 12 B per entry: 156 KB of the 160 KB a gfx950 workgroup may hold
+constexpr unsigned long long kNoKey = ~0ull;
+
+__device__ __forceinline__ void wave_argmin(unsigned long long& k, int& l) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint32_t lo = __shfl_xor(uint32_t(k & 0xffffffffull), off);
+    const uint32_t hi = __shfl_xor(uint32_t(k >> 32), off);
+    const int ol = __shfl_xor(l, off);
+    const unsigned long long ok = (static_cast<unsigned long long>(hi) << 32) | lo;
+    if (ok < k || (ok == k && ol < l)) {
+      k = ok;
+      l = ol;
+    }
+  }
+}
+
+// Moves entry (ek, es) down from node k of an n-entry heap (all 64 lanes of wave 0, uniformly).
+__device__ void wave_sift_down(unsigned long long* hk, int* hs, int n, int k, unsigned long long ek, int es, int lane,
+                               int* pos) {
+  for (;;) {
+    const int c0 = kHeapD * k + 1;
+    if (c0 >= n) break;
+    const int ch = c0 + lane;
+    unsigned long long mk = ch < n ? hk[ch] : kNoKey;
+    int ml = lane;
+    wave_argmin(mk, ml);
+    if (!(mk < ek)) break;
+    const int best = c0 + ml;
+    const int bslot = hs[best];
+    if (lane == 0) {
+      hk[k] = mk;
+      hs[k] = bslot;
+      pos[bslot] = k;
+    }
+    k = best;
+  }
+  if (lane == 0) {
+    hk[k] = ek;
+    hs[k] = es;
+    pos[es] = k;
+  }
+}
+
+// Moves entry (ek, es) up from node k (lane 0 alone: at most three levels).
+__device__ void lane_sift_up(unsigned long long* hk, int* hs, int k, unsigned long long ek, int es, int* pos) {
+  while (k > 0) {
+    const int par = (k - 1) / kHeapD;
+    if (!(ek < hk[par])) break;
+    hk[k] = hk[par];
+    hs[k] = hs[par];
+    pos[hs[k]] = k;
+    k = par;
+  }
+  hk[k] = ek;
+  hs[k] = es;
+  pos[es] = k;
+}
+
+// Entry at node p takes key (ek, es) (wave 0, uniformly): up or down as the key decreased or grew.
+__device__ void wave_place(unsigned long long* hk, int* hs, int n, int p, unsigned long long ek, int es, int lane,
+                           int* pos) {
+  const unsigned long long old = hk[p];
+  if (ek < old) {
+    if (lane == 0) lane_sift_up(hk, hs, p, ek, es, pos);
+  } else {
+    wave_sift_down(hk, hs, n, p, ek, es, lane, pos);
+  }
+}
+
+// predict_next_impacts' queue. The queue starts in LDS (indexed, above); when its live entries
+// outgrow LDS it moves, for the rest of the kernel, to a lazily deleted binary heap in HBM run by
+// lane 0 (superseded entries are skipped by their re-sweep stamp, as the reference does).
 __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const int2* __restrict__ pairs, CcdBufs c,
                                                             float end_time, float dt0, float target,
                                                             int on_penetration) {
-  __shared__ ToiEntry s_heap[kLdsHeap];
+  __shared__ unsigned long long s_hk[kLdsCap];
+  __shared__ int s_hs[kLdsCap];
   __shared__ int s_go, s_ovf, s_i, s_j, s_nstage, s_hsize, s_nimp, s_lds;
-  const int t = threadIdx.x;
-  {
-    const int nent = int(c.ctl[kCcdEntries]);
-    const bool fits = c.ctl[kCcdOverflow] == 0u && nent <= kLdsHeap;
-    if (fits)
-      for (int k = t; k < nent; k += kEventBlock) s_heap[k] = c.heap[k];
-    if (t == 0) s_lds = fits ? 1 : 0;
-    __syncthreads();
-  }
+  const int t = threadIdx.x, lane = t & 63;
+  const int nent = int(c.ctl[kCcdEntries]);
   if (t == 0) {
-    const int nent = int(c.ctl[kCcdEntries]);
-    ToiEntry* heap = s_lds ? s_heap : c.heap;
     const bool any = c.ctl[kCcdAnyIn] != 0u;
     // min_overstep (mechanical_world.rs:412-451): the step's end lowered to the earliest impact
     // after the window, or the window's end when an impact lies within it
@@ -193,10 +262,8 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     else if (min_over > end_time) kind = PREDICT_AFTER_END;
     c.ctl[kCcdKind] = uint32_t(kind);
     c.ctl[kCcdMinOver] = __float_as_uint(min_over);
-    const int go = (kind == PREDICT_IMPACTS && c.ctl[kCcdOverflow] == 0u) ? 1 : 0;
-    if (go)
-      for (int k = nent / 2 - 1; k >= 0; --k) heap_down(heap, nent, k);
-    s_go = go;
+    s_go = (kind == PREDICT_IMPACTS && c.ctl[kCcdOverflow] == 0u) ? 1 : 0;
+    s_lds = nent <= kLdsCap ? 1 : 0;
     s_ovf = 0;
     s_hsize = nent;
     s_nimp = 0;
@@ -206,20 +273,79 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     if (t == 0) c.ctl[kCcdImpacts] = 0u;
     return;
   }
+  if (s_lds) {
+    for (int k = t; k < nent; k += kEventBlock) {
+      s_hk[k] = c.heap[k].key;
+      s_hs[k] = c.heap[k].slot;
+    }
+    __syncthreads();
+    if (t < 64)
+      for (int k = (nent - 2) / kHeapD; k >= 0 && nent > 1; --k)
+        wave_sift_down(s_hk, s_hs, nent, k, s_hk[k], s_hs[k], lane, c.pos);
+    __syncthreads();
+    for (int k = t; k < nent; k += kEventBlock) c.pos[s_hs[k]] = k;
+  } else if (t == 0) {
+    for (int k = nent / 2 - 1; k >= 0; --k) heap_down(c.heap, nent, k);
+  }
+  __syncthreads();
   for (;;) {
-    if (t == 0) {
-      ToiEntry* heap = s_lds ? s_heap : c.heap;
+    // ---- pop to the next impact that freezes a body
+    if (s_lds) {
+      if (t < 64) {
+        int n = s_hsize, nimp = s_nimp, found = 0;
+        while (n > 0) {
+          if (nimp >= c.cap) {
+            if (lane == 0) c.ctl[kCcdOverflow] = 1u;
+            break;
+          }
+          const unsigned long long ek = s_hk[0];
+          const int es = s_hs[0];
+          --n;
+          if (n > 0) wave_sift_down(s_hk, s_hs, n, 0, s_hk[n], s_hs[n], lane, c.pos);
+          int d = 0;  // 0 next entry, 1 freeze
+          if (lane == 0) {
+            c.pos[es] = -1;
+            const float toi = key_toi(ek);
+            const int2 pr = pairs[key_pair(ek)];
+            const int ki = b.ccd_key[pr.x], kj = b.ccd_key[pr.y];
+            const float t1 = c.frozen[ki], t2 = c.frozen[kj];
+            if (t1 >= 0.0f && t2 >= 0.0f) {
+              if (toi == (t1 > t2 ? t1 : t2)) c.impacts[nimp++] = ek;
+            } else {
+              if (t1 < 0.0f) c.frozen[ki] = toi;
+              if (t2 < 0.0f) c.frozen[kj] = toi;
+              c.impacts[nimp++] = ek;
+              s_i = pr.x;
+              s_j = pr.y;
+              d = 1;
+            }
+          }
+          d = __shfl(d, 0);
+          nimp = __shfl(nimp, 0);
+          if (d) {
+            found = 1;
+            break;
+          }
+        }
+        if (lane == 0) {
+          s_hsize = n;
+          s_nimp = nimp;
+          s_go = found;
+          s_nstage = 0;
+        }
+      }
+    } else if (t == 0) {
       int hs = s_hsize, nimp = s_nimp, found = 0;
       while (hs > 0) {
         if (nimp >= c.cap) {  // the impact list is full: reported as an overflow
           c.ctl[kCcdOverflow] = 1u;
           break;
         }
-        const ToiEntry e = heap[0];
+        const ToiEntry e = c.heap[0];
         --hs;
         if (hs > 0) {
-          heap[0] = heap[hs];
-          heap_down(heap, hs, 0);
+          c.heap[0] = c.heap[hs];
+          heap_down(c.heap, hs, 0);
         }
         if (uint32_t(e.ts) != c.ts[e.slot]) continue;  // superseded by a re-sweep
         const float toi = key_toi(e.key);
@@ -245,9 +371,10 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     }
     __syncthreads();
     if (!s_go) break;
-    // re-sweep the CCD slots of the two bodies' dynamic colliders (mechanical_world.rs:503-551);
+    // ---- re-sweep the CCD slots of the two bodies' dynamic colliders (mechanical_world.rs:503-551);
     // the event's own pair is in both lists and skipped there (both bodies are frozen)
     const int si = s_i, sj = s_j;
+    const bool lds = s_lds != 0;
     const int ni = b.status[si] == STATUS_DYNAMIC ? int(c.deg[si]) : 0;
     const int nj = b.status[sj] == STATUS_DYNAMIC ? int(c.deg[sj]) : 0;
     for (int k = t; k < ni + nj; k += kEventBlock) {
@@ -257,37 +384,95 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
       const float fa = c.frozen[b.ccd_key[pr.x]], fb = c.frozen[b.ccd_key[pr.y]];
       const bool sa = b.status[pr.x] != STATUS_DYNAMIC, sb = b.status[pr.y] != STATUS_DYNAMIC;
       if ((fa >= 0.0f || sa) && (fb >= 0.0f || sb)) continue;
-      const uint32_t stamp = c.ts[slot] + 1u;
-      c.ts[slot] = stamp;
       float toi;
-      if (pair_toi(b, c, pr.x, pr.y, fa, fb, end_time, target, on_penetration, &toi)) {
-        const int k = atomicAdd(&s_nstage, 1);
-        c.stage[k] = ToiEntry{toi_key(toi, p), slot, int(stamp)};
+      const bool hit = pair_toi(b, c, pr.x, pr.y, fa, fb, end_time, target, on_penetration, &toi);
+      if (lds) {
+        // the slot's entry is replaced by the new time, or removed when there is none
+        if (hit || c.pos[slot] >= 0) {
+          const int q = atomicAdd(&s_nstage, 1);
+          c.stage[q] = ToiEntry{hit ? toi_key(toi, p) : kNoKey, slot, 0};
+        }
+      } else {
+        const uint32_t stamp = c.ts[slot] + 1u;
+        c.ts[slot] = stamp;
+        if (hit) {
+          const int q = atomicAdd(&s_nstage, 1);
+          c.stage[q] = ToiEntry{toi_key(toi, p), slot, int(stamp)};
+        }
       }
     }
     __syncthreads();
-    if (t == 0) {
+    // ---- the re-sweep's results into the queue
+    if (lds) {
+      if (t < 64) {
+        int n = s_hsize, spill = 0;
+        const int ns = s_nstage;
+        for (int k = 0; k < ns; ++k) {
+          const ToiEntry u = c.stage[k];
+          int p = lane == 0 ? c.pos[u.slot] : 0;
+          p = __shfl(p, 0);
+          if (u.key == kNoKey) {  // removed (p >= 0 here)
+            --n;
+            if (p < n) wave_place(s_hk, s_hs, n, p, s_hk[n], s_hs[n], lane, c.pos);
+            if (lane == 0) c.pos[u.slot] = -1;
+          } else if (p >= 0) {
+            wave_place(s_hk, s_hs, n, p, u.key, u.slot, lane, c.pos);
+          } else if (n < kLdsCap) {
+            if (lane == 0) lane_sift_up(s_hk, s_hs, n, u.key, u.slot, c.pos);
+            ++n;
+          } else {
+            spill = 1;  // LDS is full: the queue moves to HBM with this entry and the rest
+            // live entries carry stamp 0; every slot's stamp restarts at 0 below
+            // (entries keep their LDS index, so pos still finds them; a removed one gets a stamp that
+            // never matches)
+            if (lane == 0) {
+              for (int q = 0; q < n; ++q) c.heap[q] = ToiEntry{s_hk[q], s_hs[q], 0};
+              for (int q = k; q < ns; ++q) {
+                const ToiEntry r = c.stage[q];
+                const int rp = c.pos[r.slot];
+                if (r.key == kNoKey) c.heap[rp].ts = -1;
+                else if (rp >= 0) c.heap[rp] = ToiEntry{r.key, r.slot, 0};
+                else c.heap[n++] = ToiEntry{r.key, r.slot, 0};
+              }
+            }
+            break;
+          }
+        }
+        if (lane == 0) {
+          s_hsize = n;
+          if (spill) s_lds = 0;
+          s_ovf = spill ? 2 : 0;
+        }
+      }
+    } else if (t == 0) {
       int hs = s_hsize;
       const int ns = s_nstage;
-      if (s_lds && hs + ns > kLdsHeap) {  // the queue outgrows LDS: it continues in HBM
-        for (int k = 0; k < hs; ++k) c.heap[k] = s_heap[k];
-        s_lds = 0;
-      }
-      ToiEntry* heap = s_lds ? s_heap : c.heap;
       if (hs + ns > c.cap) {
         c.ctl[kCcdOverflow] = 1u;
         s_ovf = 1;
       } else {
         for (int k = 0; k < ns; ++k) {
-          heap[hs] = c.stage[k];
-          heap_up(heap, hs);
+          c.heap[hs] = c.stage[k];
+          heap_up(c.heap, hs);
           ++hs;
         }
       }
       s_hsize = hs;
     }
     __syncthreads();
-    if (s_ovf) break;
+    if (s_ovf == 1) break;
+    if (s_ovf == 2) {
+      // the move to HBM: stamps restart at 0 (the copied entries are all current), then a heap
+      const int ncc = int(c.ctl[kCcdSlots]);
+      for (int k = t; k < ncc; k += kEventBlock) c.ts[k] = 0u;
+      __syncthreads();
+      if (t == 0) {
+        const int hs = s_hsize;
+        for (int k = hs / 2 - 1; k >= 0; --k) heap_down(c.heap, hs, k);
+        s_ovf = 0;
+      }
+      __syncthreads();
+    }
   }
   if (t == 0) {
     const int nimp = s_nimp;

@@ -545,6 +545,7 @@ NPH_HD float toi_separation(int k1, f3 h1, f3 t1, q4 q1, int k2, f3 h2, f3 t2, q
     for (int j = 0; j < 3; ++j) AC[i][j] = fabsf(dot(a[i], b[j]));
   float best = -INFINITY;
   f3 axis = a[0];
+  float scale = 1.0f;  // the best axis is axis / scale (edge axes are compared unnormalised)
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     const float rb = (h2.x * AC[i][0] + h2.y * AC[i][1]) + h2.z * AC[i][2];
@@ -557,6 +558,7 @@ NPH_HD float toi_separation(int k1, f3 h1, f3 t1, q4 q1, int k2, f3 h2, f3 t2, q
     const float s = fabsf(dot(d, b[j])) - (ra + comp(h2, j));
     if (s > best) { best = s; axis = b[j]; }
   }
+  // edge axes L = a_i x b_j: (|d.L| - (ra + rb)) / |L| with ra, rb projected on the unnormalised L
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -565,12 +567,12 @@ NPH_HD float toi_separation(int k1, f3 h1, f3 t1, q4 q1, int k2, f3 h2, f3 t2, q
       const float l2 = dot(L, L);
       if (l2 < 1e-6f) continue;
       const float l = sqrtf(l2);
-      const f3 Ln{L.x / l, L.y / l, L.z / l};
-      const float ra = (h1.x * fabsf(dot(a[0], Ln)) + h1.y * fabsf(dot(a[1], Ln))) + h1.z * fabsf(dot(a[2], Ln));
-      const float rb = (h2.x * fabsf(dot(b[0], Ln)) + h2.y * fabsf(dot(b[1], Ln))) + h2.z * fabsf(dot(b[2], Ln));
-      const float s = fabsf(dot(d, Ln)) - (ra + rb);
-      if (s > best) { best = s; axis = Ln; }
+      const float ra = (h1.x * fabsf(dot(a[0], L)) + h1.y * fabsf(dot(a[1], L))) + h1.z * fabsf(dot(a[2], L));
+      const float rb = (h2.x * fabsf(dot(b[0], L)) + h2.y * fabsf(dot(b[1], L))) + h2.z * fabsf(dot(b[2], L));
+      const float s = (fabsf(dot(d, L)) - (ra + rb)) / l;
+      if (s > best) { best = s; axis = L; scale = l; }
     }
+  if (scale != 1.0f) axis = f3{axis.x / scale, axis.y / scale, axis.z / scale};
   n = dot(d, axis) < 0.0f ? neg(axis) : axis;
   return best;
 }

@@ -228,7 +228,8 @@ struct CcdBufs {
   float* body_time;        // [n] time each body's motion starts from (SubstepState::body_times)
   float* frozen;           // [n] per body key: freeze time, < 0 when not frozen
   int* slot_pair;          // [P] pair index of each CCD slot (pairs with a CCD-enabled side)
-  uint32_t* ts;            // [P] re-sweep stamp of each slot
+  uint32_t* ts;            // [P] re-sweep stamp of each slot (queue in HBM)
+  int* pos;                // [P] node of each slot's entry in the LDS queue, -1 when it has none
   ToiEntry* heap;          // [cap] first-pass entries, then the queue
   ToiEntry* stage;         // [P] entries of one re-sweep
   unsigned long long* impacts;  // [cap] keys of the confirmed impacts, in order

@@ -573,6 +573,7 @@ T toi_separation(const ShapeDesc<T>& s1, const Iso<T>& p1, const ShapeDesc<T>& s
     for (int j = 0; j < 3; ++j) AC[i][j] = std::fabs(dot(a[i], b[j]));
   T best = -std::numeric_limits<T>::infinity();
   V3<T> axis = a[0];
+  T scale = T(1);  // the best axis is axis / scale (edge axes are compared unnormalised)
   for (int i = 0; i < 3; ++i) {
     const T rb = (comp(hb, 0) * AC[i][0] + comp(hb, 1) * AC[i][1]) + comp(hb, 2) * AC[i][2];
     const T s = std::fabs(dot(d, a[i])) - (comp(ha, i) + rb);
@@ -583,20 +584,22 @@ T toi_separation(const ShapeDesc<T>& s1, const Iso<T>& p1, const ShapeDesc<T>& s
     const T s = std::fabs(dot(d, b[j])) - (ra + comp(hb, j));
     if (s > best) { best = s; axis = b[j]; }
   }
+  // edge axes L = a_i x b_j: the separation along L / |L| is (|d.L| - (ra + rb)) / |L| with ra, rb
+  // the projected half extents on the unnormalised L (one division per axis)
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
       const V3<T> L = cross(a[i], b[j]);
       const T l2 = dot(L, L);
       if (l2 < T(1e-6)) continue;
       const T l = std::sqrt(l2);
-      const V3<T> Ln{L.x / l, L.y / l, L.z / l};
-      const T ra = (comp(ha, 0) * std::fabs(dot(a[0], Ln)) + comp(ha, 1) * std::fabs(dot(a[1], Ln))) +
-                   comp(ha, 2) * std::fabs(dot(a[2], Ln));
-      const T rb = (comp(hb, 0) * std::fabs(dot(b[0], Ln)) + comp(hb, 1) * std::fabs(dot(b[1], Ln))) +
-                   comp(hb, 2) * std::fabs(dot(b[2], Ln));
-      const T s = std::fabs(dot(d, Ln)) - (ra + rb);
-      if (s > best) { best = s; axis = Ln; }
+      const T ra = (comp(ha, 0) * std::fabs(dot(a[0], L)) + comp(ha, 1) * std::fabs(dot(a[1], L))) +
+                   comp(ha, 2) * std::fabs(dot(a[2], L));
+      const T rb = (comp(hb, 0) * std::fabs(dot(b[0], L)) + comp(hb, 1) * std::fabs(dot(b[1], L))) +
+                   comp(hb, 2) * std::fabs(dot(b[2], L));
+      const T s = (std::fabs(dot(d, L)) - (ra + rb)) / l;
+      if (s > best) { best = s; axis = L; scale = l; }
     }
+  if (scale != T(1)) axis = V3<T>{axis.x / scale, axis.y / scale, axis.z / scale};
   n = dot(d, axis) < T(0) ? -axis : axis;
   return best;
 }
