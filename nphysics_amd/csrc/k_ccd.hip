@@ -6,11 +6,11 @@
 //   * k_ccd_first_pass, one lane per broad-phase pair with a CCD-enabled side: the pair gets a CCD
 //     slot and its time of impact up to the step's end (conservative advancement, np_geometry.hpp);
 //     impacts within the window go to the queue, later ones lower the earliest-later time;
-//   * k_ccd_events, one workgroup: lane 0 pops the earliest impact (ties by pair index), skips
-//     entries superseded by a re-sweep, confirms pairs of two frozen bodies only at the later
-//     freeze time, and freezes the two bodies (their keys: a static collider shares its static
-//     body's); then every lane re-sweeps the CCD slots of the two bodies' dynamic colliders with
-//     the frozen bodies held at their freeze poses, and lane 0 pushes the results;
+//   * k_ccd_events, one workgroup: wave 0 pops the earliest impact (ties by pair index) from an
+//     indexed 64-ary heap, confirms pairs of two frozen bodies only at the later freeze time, and
+//     freezes the two bodies (their keys: a static collider shares its static body's); then every
+//     lane re-sweeps the CCD slots of the two bodies' dynamic colliders with the frozen bodies held
+//     at their freeze poses, and wave 0 replaces, removes or adds those slots' entries;
 //   * the island of a substep grows in rounds: the dynamic bodies of every pair touching a
 //     traversed collider are moved to their impact time (k_ccd_touch, k_ccd_prepare; the dynamics
 //     update of the substep, no gravity, forces cleared), those pairs' manifolds are computed by
@@ -103,7 +103,6 @@ __global__ void __launch_bounds__(128) k_ccd_first_pass(DevBodies b, const int2*
   if (!((b.ccd[pr.x] | b.ccd[pr.y]) & 1)) return;
   const uint32_t slot = atomicAdd(&c.ctl[kCcdSlots], 1u);
   c.slot_pair[slot] = p;
-  c.ts[slot] = 0u;
   c.pos[slot] = -1;
   float toi;
   if (!pair_toi(b, c, pr.x, pr.y, -1.0f, -1.0f, dt0, target, on_penetration, &toi)) return;
@@ -112,7 +111,7 @@ __global__ void __launch_bounds__(128) k_ccd_first_pass(DevBodies b, const int2*
   } else {
     atomicOr(&c.ctl[kCcdAnyIn], 1u);
     const uint32_t e = atomicAdd(&c.ctl[kCcdEntries], 1u);
-    if (int(e) < c.cap) c.heap[e] = ToiEntry{toi_key(toi, p), int(slot), 0};
+    if (int(e) < c.cap) c.heap[e] = ToiEntry{toi_key(toi, p), int(slot)};
     else atomicOr(&c.ctl[kCcdOverflow], 1u);
   }
 }
@@ -135,120 +134,118 @@ __global__ void __launch_bounds__(256) k_ccd_fill(DevBodies b, const int2* __res
   if (b.status[pr.y] == STATUS_DYNAMIC) c.adj[c.off[pr.y] + atomicAdd(&c.fill[pr.y], 1u)] = slot;
 }
 
-__device__ __forceinline__ bool entry_before(const ToiEntry& a, const ToiEntry& b) {
-  return a.key < b.key || (a.key == b.key && a.ts < b.ts);
-}
-__device__ void heap_down(ToiEntry* h, int n, int k) {
-  const ToiEntry e = h[k];
-  for (;;) {
-    int ch = 2 * k + 1;
-    if (ch >= n) break;
-    if (ch + 1 < n && entry_before(h[ch + 1], h[ch])) ++ch;
-    if (!entry_before(h[ch], e)) break;
-    h[k] = h[ch];
-    k = ch;
-  }
-  h[k] = e;
-}
-__device__ void heap_up(ToiEntry* h, int k) {
-  const ToiEntry e = h[k];
-  while (k > 0) {
-    const int par = (k - 1) / 2;
-    if (!entry_before(e, h[par])) break;
-    h[k] = h[par];
-    k = par;
-  }
-  h[k] = e;
-}
-
-// ---- the queue in LDS: a 64-ary heap indexed by CCD slot, operated by wave 0 ----
-// Each slot holds at most one live entry (its latest time of impact), so a re-sweep replaces or
+// ---- the impact queue: an indexed 64-ary heap, its top three levels in LDS, operated by wave 0 ----
+// Each CCD slot holds at most one entry (its latest time of impact), so a re-sweep replaces or
 // removes it in place instead of leaving a superseded entry behind: the pop sequence of the valid
-// entries is the reference's, without popping the stale ones. A node's 64 children are compared
-// across the wave in one step (three levels cover 266,305 entries); keys are unique (they hold the
-// pair index), so the order is total. pos[slot] (HBM) is written and read by lane 0 only.
+// entries is the reference's, without popping stale ones. A node's 64 children are compared across
+// the wave in one step; nodes 0..4160 (levels 0-2) live in LDS and deeper ones in HBM, so a node's
+// children are all in one of the two. Keys are unique (they hold the pair index): the order is
+// total. pos[slot] (HBM) is the node of each slot's entry, -1 when it has none.
 constexpr int kHeapD = 64;
-constexpr int kLdsCap = 13312;  // 12 B per entry: 156 KB of the 160 KB a gfx950 workgroup may hold
+constexpr int kTop = 1 + kHeapD + kHeapD * kHeapD;  // 4,161 nodes, 12 B each in LDS
 constexpr unsigned long long kNoKey = ~0ull;
 
-__device__ __forceinline__ void wave_argmin(unsigned long long& k, int& l) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const uint32_t lo = __shfl_xor(uint32_t(k & 0xffffffffull), off);
-    const uint32_t hi = __shfl_xor(uint32_t(k >> 32), off);
-    const int ol = __shfl_xor(l, off);
-    const unsigned long long ok = (static_cast<unsigned long long>(hi) << 32) | lo;
-    if (ok < k || (ok == k && ol < l)) {
-      k = ok;
-      l = ol;
+struct Queue {
+  unsigned long long* lk;  // LDS keys, nodes < kTop
+  int* ls;                 // LDS slots
+  ToiEntry* g;             // HBM entries, nodes >= kTop
+  int* pos;
+  __device__ unsigned long long key(int i) const { return i < kTop ? lk[i] : g[i].key; }
+  __device__ int slot(int i) const { return i < kTop ? ls[i] : g[i].slot; }
+  __device__ void set(int i, unsigned long long k, int s) const {  // one lane
+    if (i < kTop) {
+      lk[i] = k;
+      ls[i] = s;
+    } else {
+      g[i] = ToiEntry{k, s};
     }
+    pos[s] = i;
   }
+};
+
+// Smallest value across the wave, in every lane: DPP steps within quads, half rows and rows, then
+// row broadcasts; lane 63 ends with the minimum.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint32_t dpp_min_step(uint32_t v) {
+  const uint32_t o = uint32_t(__builtin_amdgcn_update_dpp(int(v), int(v), Ctrl, RowMask, 0xf, false));
+  return o < v ? o : v;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+  v = dpp_min_step<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v = dpp_min_step<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]
+  v = dpp_min_step<0x141, 0xf>(v);  // row_half_mirror
+  v = dpp_min_step<0x140, 0xf>(v);  // row_mirror
+  v = dpp_min_step<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v = dpp_min_step<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return uint32_t(__builtin_amdgcn_readlane(int(v), 63));
 }
 
-// Moves entry (ek, es) down from node k of an n-entry heap (all 64 lanes of wave 0, uniformly).
-__device__ void wave_sift_down(unsigned long long* hk, int* hs, int n, int k, unsigned long long ek, int es, int lane,
-                               int* pos) {
+// The smallest of the wave's 64 keys (uniform) and the lane holding it: the time bits first, then
+// the pair index among the lanes with that time.
+__device__ __forceinline__ int wave_argmin(unsigned long long k, unsigned long long& mk) {
+  const uint32_t hi = uint32_t(k >> 32), lo = uint32_t(k & 0xffffffffull);
+  const uint32_t mh = wave_min_u32(hi);
+  const uint32_t ml = wave_min_u32(hi == mh ? lo : 0xffffffffu);
+  mk = (static_cast<unsigned long long>(mh) << 32) | ml;
+  return __ffsll(static_cast<long long>(__ballot(hi == mh && lo == ml))) - 1;
+}
+
+// Every lane of wave 0 runs the queue operations with the same values (lane 0 writes); a lane may
+// track one slot's node (tslot, tnode), kept current as entries move.
+__device__ void sift_down(const Queue& q, int n, int k, unsigned long long ek, int es, int lane, int tslot,
+                          int& tnode) {
   for (;;) {
     const int c0 = kHeapD * k + 1;
     if (c0 >= n) break;
     const int ch = c0 + lane;
-    unsigned long long mk = ch < n ? hk[ch] : kNoKey;
-    int ml = lane;
-    wave_argmin(mk, ml);
-    if (!(mk < ek)) break;
-    const int best = c0 + ml;
-    const int bslot = hs[best];
-    if (lane == 0) {
-      hk[k] = mk;
-      hs[k] = bslot;
-      pos[bslot] = k;
+    unsigned long long ck = kNoKey;
+    int cs = -1;
+    if (ch < n) {
+      ck = q.key(ch);
+      cs = q.slot(ch);
     }
-    k = best;
+    unsigned long long mk;
+    const int wl = wave_argmin(ck, mk);
+    if (!(mk < ek)) break;
+    const int bs = __builtin_amdgcn_readlane(cs, wl);
+    if (lane == 0) q.set(k, mk, bs);
+    if (tslot == bs) tnode = k;
+    k = c0 + wl;
   }
-  if (lane == 0) {
-    hk[k] = ek;
-    hs[k] = es;
-    pos[es] = k;
-  }
+  if (lane == 0) q.set(k, ek, es);
+  if (tslot == es) tnode = k;
 }
 
-// Moves entry (ek, es) up from node k (lane 0 alone: at most three levels).
-__device__ void lane_sift_up(unsigned long long* hk, int* hs, int k, unsigned long long ek, int es, int* pos) {
+__device__ void sift_up(const Queue& q, int k, unsigned long long ek, int es, int lane, int tslot, int& tnode) {
   while (k > 0) {
     const int par = (k - 1) / kHeapD;
-    if (!(ek < hk[par])) break;
-    hk[k] = hk[par];
-    hs[k] = hs[par];
-    pos[hs[k]] = k;
+    const unsigned long long pk = q.key(par);
+    if (!(ek < pk)) break;
+    const int ps = q.slot(par);
+    if (lane == 0) q.set(k, pk, ps);
+    if (tslot == ps) tnode = k;
     k = par;
   }
-  hk[k] = ek;
-  hs[k] = es;
-  pos[es] = k;
+  if (lane == 0) q.set(k, ek, es);
+  if (tslot == es) tnode = k;
 }
 
-// Entry at node p takes key (ek, es) (wave 0, uniformly): up or down as the key decreased or grew.
-__device__ void wave_place(unsigned long long* hk, int* hs, int n, int p, unsigned long long ek, int es, int lane,
-                           int* pos) {
-  const unsigned long long old = hk[p];
-  if (ek < old) {
-    if (lane == 0) lane_sift_up(hk, hs, p, ek, es, pos);
-  } else {
-    wave_sift_down(hk, hs, n, p, ek, es, lane, pos);
-  }
+// The entry at node p takes key (ek, es): up or down as the key decreased or grew.
+__device__ void place(const Queue& q, int n, int p, unsigned long long ek, int es, int lane, int tslot, int& tnode) {
+  if (ek < q.key(p)) sift_up(q, p, ek, es, lane, tslot, tnode);
+  else sift_down(q, n, p, ek, es, lane, tslot, tnode);
 }
 
-// predict_next_impacts' queue. The queue starts in LDS (indexed, above); when its live entries
-// outgrow LDS it moves, for the rest of the kernel, to a lazily deleted binary heap in HBM run by
-// lane 0 (superseded entries are skipped by their re-sweep stamp, as the reference does).
+// predict_next_impacts' event loop (mechanical_world.rs:453-557) in one workgroup.
 __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const int2* __restrict__ pairs, CcdBufs c,
                                                             float end_time, float dt0, float target,
                                                             int on_penetration) {
-  __shared__ unsigned long long s_hk[kLdsCap];
-  __shared__ int s_hs[kLdsCap];
-  __shared__ int s_go, s_ovf, s_i, s_j, s_nstage, s_hsize, s_nimp, s_lds;
-  const int t = threadIdx.x, lane = t & 63;
+  __shared__ unsigned long long s_hk[kTop];
+  __shared__ int s_hs[kTop];
+  __shared__ int s_go, s_i, s_j, s_nstage, s_hsize, s_nimp;
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int nent = int(c.ctl[kCcdEntries]);
+  const Queue q{s_hk, s_hs, c.heap, c.pos};
   if (t == 0) {
     const bool any = c.ctl[kCcdAnyIn] != 0u;
     // min_overstep (mechanical_world.rs:412-451): the step's end lowered to the earliest impact
@@ -263,8 +260,6 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     c.ctl[kCcdKind] = uint32_t(kind);
     c.ctl[kCcdMinOver] = __float_as_uint(min_over);
     s_go = (kind == PREDICT_IMPACTS && c.ctl[kCcdOverflow] == 0u) ? 1 : 0;
-    s_lds = nent <= kLdsCap ? 1 : 0;
-    s_ovf = 0;
     s_hsize = nent;
     s_nimp = 0;
   }
@@ -273,108 +268,84 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     if (t == 0) c.ctl[kCcdImpacts] = 0u;
     return;
   }
-  if (s_lds) {
-    for (int k = t; k < nent; k += kEventBlock) {
-      s_hk[k] = c.heap[k].key;
-      s_hs[k] = c.heap[k].slot;
-    }
-    __syncthreads();
-    if (t < 64)
-      for (int k = (nent - 2) / kHeapD; k >= 0 && nent > 1; --k)
-        wave_sift_down(s_hk, s_hs, nent, k, s_hk[k], s_hs[k], lane, c.pos);
-    __syncthreads();
-    for (int k = t; k < nent; k += kEventBlock) c.pos[s_hs[k]] = k;
-  } else if (t == 0) {
-    for (int k = nent / 2 - 1; k >= 0; --k) heap_down(c.heap, nent, k);
+  // ---- the first pass's entries into a heap: the LDS levels copied in, then the internal nodes
+  // sifted down level by level from the deepest (the subtrees of one level are disjoint: one wave
+  // per node), then every entry's node recorded
+  for (int k = t; k < nent && k < kTop; k += kEventBlock) {
+    s_hk[k] = c.heap[k].key;
+    s_hs[k] = c.heap[k].slot;
   }
+  __syncthreads();
+  if (nent > 1) {
+    const int last = (nent - 2) / kHeapD;  // the last internal node
+    int first = 0, width = 1;
+    while (first + width <= last) {  // the first node of the deepest internal level
+      first += width;
+      width *= kHeapD;
+    }
+    for (;;) {
+      const int end = first + width - 1 < last ? first + width - 1 : last;
+      int none = -1;
+      for (int k = first + wave; k <= end; k += kEventBlock / 64) sift_down(q, nent, k, q.key(k), q.slot(k), lane, -1, none);
+      __syncthreads();
+      if (first == 0) break;
+      width /= kHeapD;
+      first -= width;
+    }
+  }
+  for (int k = t; k < nent; k += kEventBlock) c.pos[q.slot(k)] = k;
   __syncthreads();
   for (;;) {
     // ---- pop to the next impact that freezes a body
-    if (s_lds) {
-      if (t < 64) {
-        int n = s_hsize, nimp = s_nimp, found = 0;
-        while (n > 0) {
-          if (nimp >= c.cap) {
-            if (lane == 0) c.ctl[kCcdOverflow] = 1u;
-            break;
-          }
-          const unsigned long long ek = s_hk[0];
-          const int es = s_hs[0];
-          --n;
-          if (n > 0) wave_sift_down(s_hk, s_hs, n, 0, s_hk[n], s_hs[n], lane, c.pos);
-          int d = 0;  // 0 next entry, 1 freeze
-          if (lane == 0) {
-            c.pos[es] = -1;
-            const float toi = key_toi(ek);
-            const int2 pr = pairs[key_pair(ek)];
-            const int ki = b.ccd_key[pr.x], kj = b.ccd_key[pr.y];
-            const float t1 = c.frozen[ki], t2 = c.frozen[kj];
-            if (t1 >= 0.0f && t2 >= 0.0f) {
-              if (toi == (t1 > t2 ? t1 : t2)) c.impacts[nimp++] = ek;
-            } else {
-              if (t1 < 0.0f) c.frozen[ki] = toi;
-              if (t2 < 0.0f) c.frozen[kj] = toi;
-              c.impacts[nimp++] = ek;
-              s_i = pr.x;
-              s_j = pr.y;
-              d = 1;
-            }
-          }
-          d = __shfl(d, 0);
-          nimp = __shfl(nimp, 0);
-          if (d) {
-            found = 1;
-            break;
-          }
-        }
-        if (lane == 0) {
-          s_hsize = n;
-          s_nimp = nimp;
-          s_go = found;
-          s_nstage = 0;
-        }
-      }
-    } else if (t == 0) {
-      int hs = s_hsize, nimp = s_nimp, found = 0;
-      while (hs > 0) {
+    if (t < 64) {
+      int n = s_hsize, nimp = s_nimp, found = 0, none = -1;
+      while (n > 0) {
         if (nimp >= c.cap) {  // the impact list is full: reported as an overflow
-          c.ctl[kCcdOverflow] = 1u;
+          if (lane == 0) c.ctl[kCcdOverflow] = 1u;
           break;
         }
-        const ToiEntry e = c.heap[0];
-        --hs;
-        if (hs > 0) {
-          c.heap[0] = c.heap[hs];
-          heap_down(c.heap, hs, 0);
+        const unsigned long long ek = s_hk[0];
+        const int es = s_hs[0];
+        --n;
+        if (n > 0) sift_down(q, n, 0, q.key(n), q.slot(n), lane, -1, none);
+        int d = 0;  // 0 next entry, 1 freeze
+        if (lane == 0) {
+          c.pos[es] = -1;
+          const float toi = key_toi(ek);
+          const int2 pr = pairs[key_pair(ek)];
+          const int ki = b.ccd_key[pr.x], kj = b.ccd_key[pr.y];
+          const float t1 = c.frozen[ki], t2 = c.frozen[kj];
+          if (t1 >= 0.0f && t2 >= 0.0f) {
+            if (toi == (t1 > t2 ? t1 : t2)) c.impacts[nimp++] = ek;
+          } else {
+            if (t1 < 0.0f) c.frozen[ki] = toi;
+            if (t2 < 0.0f) c.frozen[kj] = toi;
+            c.impacts[nimp++] = ek;
+            s_i = pr.x;
+            s_j = pr.y;
+            d = 1;
+          }
         }
-        if (uint32_t(e.ts) != c.ts[e.slot]) continue;  // superseded by a re-sweep
-        const float toi = key_toi(e.key);
-        const int2 pr = pairs[key_pair(e.key)];
-        const int ki = b.ccd_key[pr.x], kj = b.ccd_key[pr.y];
-        const float t1 = c.frozen[ki], t2 = c.frozen[kj];
-        if (t1 >= 0.0f && t2 >= 0.0f) {
-          if (toi == (t1 > t2 ? t1 : t2)) c.impacts[nimp++] = e.key;
-          continue;
+        d = __shfl(d, 0);
+        nimp = __shfl(nimp, 0);
+        if (d) {
+          found = 1;
+          break;
         }
-        if (t1 < 0.0f) c.frozen[ki] = toi;
-        if (t2 < 0.0f) c.frozen[kj] = toi;
-        c.impacts[nimp++] = e.key;
-        s_i = pr.x;
-        s_j = pr.y;
-        found = 1;
-        break;
       }
-      s_hsize = hs;
-      s_nimp = nimp;
-      s_go = found;
-      s_nstage = 0;
+      if (lane == 0) {
+        s_hsize = n;
+        s_nimp = nimp;
+        s_go = found;
+        s_nstage = 0;
+      }
     }
     __syncthreads();
     if (!s_go) break;
     // ---- re-sweep the CCD slots of the two bodies' dynamic colliders (mechanical_world.rs:503-551);
-    // the event's own pair is in both lists and skipped there (both bodies are frozen)
+    // the event's own pair is in both lists and skipped there (both bodies are frozen). A slot's
+    // entry is replaced by its new time, or removed when there is none.
     const int si = s_i, sj = s_j;
-    const bool lds = s_lds != 0;
     const int ni = b.status[si] == STATUS_DYNAMIC ? int(c.deg[si]) : 0;
     const int nj = b.status[sj] == STATUS_DYNAMIC ? int(c.deg[sj]) : 0;
     for (int k = t; k < ni + nj; k += kEventBlock) {
@@ -386,93 +357,59 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
       if ((fa >= 0.0f || sa) && (fb >= 0.0f || sb)) continue;
       float toi;
       const bool hit = pair_toi(b, c, pr.x, pr.y, fa, fb, end_time, target, on_penetration, &toi);
-      if (lds) {
-        // the slot's entry is replaced by the new time, or removed when there is none
-        if (hit || c.pos[slot] >= 0) {
-          const int q = atomicAdd(&s_nstage, 1);
-          c.stage[q] = ToiEntry{hit ? toi_key(toi, p) : kNoKey, slot, 0};
-        }
-      } else {
-        const uint32_t stamp = c.ts[slot] + 1u;
-        c.ts[slot] = stamp;
-        if (hit) {
-          const int q = atomicAdd(&s_nstage, 1);
-          c.stage[q] = ToiEntry{toi_key(toi, p), slot, int(stamp)};
-        }
+      if (hit || c.pos[slot] >= 0) {
+        const int st = atomicAdd(&s_nstage, 1);
+        c.stage[st] = ToiEntry{hit ? toi_key(toi, p) : kNoKey, slot};
       }
     }
     __syncthreads();
-    // ---- the re-sweep's results into the queue
-    if (lds) {
-      if (t < 64) {
-        int n = s_hsize, spill = 0;
-        const int ns = s_nstage;
-        for (int k = 0; k < ns; ++k) {
-          const ToiEntry u = c.stage[k];
-          int p = lane == 0 ? c.pos[u.slot] : 0;
-          p = __shfl(p, 0);
-          if (u.key == kNoKey) {  // removed (p >= 0 here)
-            --n;
-            if (p < n) wave_place(s_hk, s_hs, n, p, s_hk[n], s_hs[n], lane, c.pos);
-            if (lane == 0) c.pos[u.slot] = -1;
-          } else if (p >= 0) {
-            wave_place(s_hk, s_hs, n, p, u.key, u.slot, lane, c.pos);
-          } else if (n < kLdsCap) {
-            if (lane == 0) lane_sift_up(s_hk, s_hs, n, u.key, u.slot, c.pos);
-            ++n;
-          } else {
-            spill = 1;  // LDS is full: the queue moves to HBM with this entry and the rest
-            // live entries carry stamp 0; every slot's stamp restarts at 0 below
-            // (entries keep their LDS index, so pos still finds them; a removed one gets a stamp that
-            // never matches)
-            if (lane == 0) {
-              for (int q = 0; q < n; ++q) c.heap[q] = ToiEntry{s_hk[q], s_hs[q], 0};
-              for (int q = k; q < ns; ++q) {
-                const ToiEntry r = c.stage[q];
-                const int rp = c.pos[r.slot];
-                if (r.key == kNoKey) c.heap[rp].ts = -1;
-                else if (rp >= 0) c.heap[rp] = ToiEntry{r.key, r.slot, 0};
-                else c.heap[n++] = ToiEntry{r.key, r.slot, 0};
-              }
+    // ---- the re-sweep's results into the queue, 64 at a time: each lane loads one entry and its
+    // slot's node, and keeps the node current while the entries before it move
+    if (t < 64) {
+      int n = s_hsize, ovf = 0;
+      const int ns = s_nstage;
+      for (int base = 0; base < ns && !ovf; base += 64) {
+        const int cnt = ns - base < 64 ? ns - base : 64;
+        unsigned long long mkey = kNoKey;
+        int tslot = -1, tnode = -1;
+        if (lane < cnt) {
+          const ToiEntry u = c.stage[base + lane];
+          mkey = u.key;
+          tslot = u.slot;
+          tnode = c.pos[u.slot];
+        }
+        for (int k = 0; k < cnt; ++k) {
+          const uint32_t khi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(mkey >> 32)), k));
+          const uint32_t klo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(mkey & 0xffffffffull)), k));
+          const unsigned long long uk = (static_cast<unsigned long long>(khi) << 32) | klo;
+          const int us = __builtin_amdgcn_readlane(tslot, k);
+          const int p = __builtin_amdgcn_readlane(tnode, k);
+          if (lane == k) tslot = -1;  // this lane's slot is handled now
+          if (uk == kNoKey) {  // removed: the last entry takes its node (a removal always has one)
+            if (p >= 0) {
+              --n;
+              if (p < n) place(q, n, p, q.key(n), q.slot(n), lane, tslot, tnode);
+              if (lane == 0) c.pos[us] = -1;
             }
+          } else if (p >= 0) {
+            place(q, n, p, uk, us, lane, tslot, tnode);
+          } else if (n < c.cap) {
+            sift_up(q, n, uk, us, lane, tslot, tnode);
+            ++n;
+          } else {  // the queue is full: reported as an overflow
+            if (lane == 0) c.ctl[kCcdOverflow] = 1u;
+            ovf = 1;
             break;
           }
         }
-        if (lane == 0) {
-          s_hsize = n;
-          if (spill) s_lds = 0;
-          s_ovf = spill ? 2 : 0;
-        }
       }
-    } else if (t == 0) {
-      int hs = s_hsize;
-      const int ns = s_nstage;
-      if (hs + ns > c.cap) {
-        c.ctl[kCcdOverflow] = 1u;
-        s_ovf = 1;
-      } else {
-        for (int k = 0; k < ns; ++k) {
-          c.heap[hs] = c.stage[k];
-          heap_up(c.heap, hs);
-          ++hs;
-        }
+      if (lane == 0) {
+        s_hsize = n;
+        if (ovf) s_go = 0;
       }
-      s_hsize = hs;
     }
     __syncthreads();
-    if (s_ovf == 1) break;
-    if (s_ovf == 2) {
-      // the move to HBM: stamps restart at 0 (the copied entries are all current), then a heap
-      const int ncc = int(c.ctl[kCcdSlots]);
-      for (int k = t; k < ncc; k += kEventBlock) c.ts[k] = 0u;
-      __syncthreads();
-      if (t == 0) {
-        const int hs = s_hsize;
-        for (int k = hs / 2 - 1; k >= 0; --k) heap_down(c.heap, hs, k);
-        s_ovf = 0;
-      }
-      __syncthreads();
-    }
+    if (!s_go) break;
   }
   if (t == 0) {
     const int nimp = s_nimp;

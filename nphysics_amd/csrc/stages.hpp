@@ -219,18 +219,17 @@ void launch_joint_union(const DevBodies& b, const JointBufs& j, int* parent, uin
 
 // k_ccd.hip --------------------------------------------------------------------------------
 // A queued time of impact: key = time bits << 32 | pair index (times are >= 0, so keys order by
-// time, then pair), the pair's CCD slot and the re-sweep stamp it was computed with.
+// time, then pair) and the pair's CCD slot.
 struct ToiEntry {
   unsigned long long key;
-  int slot, ts;
+  int slot;
 };
 struct CcdBufs {
   float* body_time;        // [n] time each body's motion starts from (SubstepState::body_times)
   float* frozen;           // [n] per body key: freeze time, < 0 when not frozen
   int* slot_pair;          // [P] pair index of each CCD slot (pairs with a CCD-enabled side)
-  uint32_t* ts;            // [P] re-sweep stamp of each slot (queue in HBM)
-  int* pos;                // [P] node of each slot's entry in the LDS queue, -1 when it has none
-  ToiEntry* heap;          // [cap] first-pass entries, then the queue
+  int* pos;                // [P] node of each slot's entry in the queue, -1 when it has none
+  ToiEntry* heap;          // [cap] first-pass entries, then the queue's nodes below its LDS levels
   ToiEntry* stage;         // [P] entries of one re-sweep
   unsigned long long* impacts;  // [cap] keys of the confirmed impacts, in order
   uint32_t* ctl;           // control words kCcd*

@@ -221,7 +221,7 @@ struct World {
   bool any_ccd = false;
   std::vector<int> h_ccd, h_ccd_key;
   std::map<std::pair<int, int>, int> static_owner;  // (scene, static body index) -> first collider
-  DBuf ccd_flags, ccd_keys, ccd_rot, ccd_time, ccd_frozen, ccd_slot, ccd_ts, ccd_heap, ccd_stage, ccd_imp, ccd_ctl,
+  DBuf ccd_flags, ccd_keys, ccd_rot, ccd_time, ccd_frozen, ccd_slot, ccd_heap, ccd_stage, ccd_imp, ccd_ctl,
       ccd_in, ccd_new, ccd_prep, ccd_prepared, ccd_eval, ccd_kept, ccd_cand, ccd_cpos, ccd_list, ccd_lpair, ccd_tmp,
       ccd_deg, ccd_off, ccd_fill, ccd_adj, ccd_pos;
   ContactBufs ccd_cb;
@@ -1692,12 +1692,12 @@ struct World {
     for (DBuf* b : {&ccd_time, &ccd_frozen, &ccd_in, &ccd_new, &ccd_prep, &ccd_prepared, &ccd_deg, &ccd_off, &ccd_fill})
       HIPCK(b->ensure(nn * 4));
     HIPCK(ccd_adj.ensure(2 * P * 4));
-    for (DBuf* b : {&ccd_slot, &ccd_ts, &ccd_pos, &ccd_eval, &ccd_kept, &ccd_cand, &ccd_cpos, &ccd_lpair})
+    for (DBuf* b : {&ccd_slot, &ccd_pos, &ccd_eval, &ccd_kept, &ccd_cand, &ccd_cpos, &ccd_lpair})
       HIPCK(b->ensure(P * 4));
     HIPCK(ccd_list.ensure(P * sizeof(int2)));
     HIPCK(ccd_stage.ensure(P * sizeof(ToiEntry)));
-    // queue capacity: every pair's first-pass entry and re-sweeps of twice as many (a full queue is
-    // an error, never a silent loss)
+    // the queue holds at most one entry per CCD slot (at most P); the impact list takes 3P + 4096
+    // confirmed impacts (a full list is an error, never a silent loss)
     const size_t cap = 3 * P + 4096;
     HIPCK(ccd_heap.ensure(cap * sizeof(ToiEntry)));
     HIPCK(ccd_imp.ensure(cap * sizeof(unsigned long long)));
@@ -1712,7 +1712,6 @@ struct World {
     c.body_time = ccd_time.as<float>();
     c.frozen = ccd_frozen.as<float>();
     c.slot_pair = ccd_slot.as<int>();
-    c.ts = ccd_ts.as<uint32_t>();
     c.pos = ccd_pos.as<int>();
     c.heap = ccd_heap.as<ToiEntry>();
     c.stage = ccd_stage.as<ToiEntry>();
