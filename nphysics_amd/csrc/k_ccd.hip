@@ -103,6 +103,8 @@ __global__ void __launch_bounds__(128) k_ccd_first_pass(DevBodies b, const int2*
   if (!((b.ccd[pr.x] | b.ccd[pr.y]) & 1)) return;
   const uint32_t slot = atomicAdd(&c.ctl[kCcdSlots], 1u);
   c.slot_pair[slot] = p;
+  c.slot_bodies[slot] = int4{pr.x, pr.y, b.ccd_key[pr.x] | (b.status[pr.x] != STATUS_DYNAMIC ? kSlotStatic : 0),
+                             b.ccd_key[pr.y] | (b.status[pr.y] != STATUS_DYNAMIC ? kSlotStatic : 0)};
   c.pos[slot] = -1;
   float toi;
   if (!pair_toi(b, c, pr.x, pr.y, -1.0f, -1.0f, dt0, target, on_penetration, &toi)) return;
@@ -118,20 +120,20 @@ __global__ void __launch_bounds__(128) k_ccd_first_pass(DevBodies b, const int2*
 
 // The CCD slots of every dynamic body (adjacency in slot order is not needed: the re-sweep's
 // results go into the queue, whose order is by key).
-__global__ void __launch_bounds__(256) k_ccd_degree(DevBodies b, const int2* __restrict__ pairs, int npairs, CcdBufs c) {
+__global__ void __launch_bounds__(256) k_ccd_degree(int npairs, CcdBufs c) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= npairs || slot >= int(c.ctl[kCcdSlots])) return;
-  const int2 pr = pairs[c.slot_pair[slot]];
-  if (b.status[pr.x] == STATUS_DYNAMIC) atomicAdd(&c.deg[pr.x], 1u);
-  if (b.status[pr.y] == STATUS_DYNAMIC) atomicAdd(&c.deg[pr.y], 1u);
+  const int4 sb = c.slot_bodies[slot];
+  if (!(sb.z & kSlotStatic)) atomicAdd(&c.deg[sb.x], 1u);
+  if (!(sb.w & kSlotStatic)) atomicAdd(&c.deg[sb.y], 1u);
 }
 
-__global__ void __launch_bounds__(256) k_ccd_fill(DevBodies b, const int2* __restrict__ pairs, int npairs, CcdBufs c) {
+__global__ void __launch_bounds__(256) k_ccd_fill(int npairs, CcdBufs c) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= npairs || slot >= int(c.ctl[kCcdSlots])) return;
-  const int2 pr = pairs[c.slot_pair[slot]];
-  if (b.status[pr.x] == STATUS_DYNAMIC) c.adj[c.off[pr.x] + atomicAdd(&c.fill[pr.x], 1u)] = slot;
-  if (b.status[pr.y] == STATUS_DYNAMIC) c.adj[c.off[pr.y] + atomicAdd(&c.fill[pr.y], 1u)] = slot;
+  const int4 sb = c.slot_bodies[slot];
+  if (!(sb.z & kSlotStatic)) c.adj[c.off[sb.x] + atomicAdd(&c.fill[sb.x], 1u)] = slot;
+  if (!(sb.w & kSlotStatic)) c.adj[c.off[sb.y] + atomicAdd(&c.fill[sb.y], 1u)] = slot;
 }
 
 // ---- the impact queue: an indexed 64-ary heap, its top three levels in LDS, operated by wave 0 ----
@@ -236,13 +238,20 @@ __device__ void place(const Queue& q, int n, int p, unsigned long long ek, int e
   else sift_down(q, n, p, ek, es, lane, tslot, tnode);
 }
 
-// predict_next_impacts' event loop (mechanical_world.rs:453-557) in one workgroup.
-__global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const int2* __restrict__ pairs, CcdBufs c,
-                                                            float end_time, float dt0, float target,
-                                                            int on_penetration) {
+// Freeze times held in LDS for up to this many body keys (the rest of the 160 KB beside the queue).
+constexpr int kFzCap = 27648;
+
+// predict_next_impacts' event loop (mechanical_world.rs:453-557) in one workgroup; with LdsFrozen
+// the freeze times live in LDS for the loop and are written back at its end.
+template <bool LdsFrozen>
+__global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, CcdBufs c, float end_time, float dt0,
+                                                            float target, int on_penetration) {
   __shared__ unsigned long long s_hk[kTop];
   __shared__ int s_hs[kTop];
+  __shared__ float s_fz[LdsFrozen ? kFzCap : 1];
   __shared__ int s_go, s_i, s_j, s_nstage, s_hsize, s_nimp;
+  float* const fz = LdsFrozen ? s_fz : c.frozen;
+  constexpr int kKey = kSlotStatic - 1;
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int nent = int(c.ctl[kCcdEntries]);
   const Queue q{s_hk, s_hs, c.heap, c.pos};
@@ -275,6 +284,8 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     s_hk[k] = c.heap[k].key;
     s_hs[k] = c.heap[k].slot;
   }
+  if (LdsFrozen)
+    for (int k = t; k < b.n; k += kEventBlock) s_fz[k] = c.frozen[k];
   __syncthreads();
   if (nent > 1) {
     const int last = (nent - 2) / kHeapD;  // the last internal node
@@ -312,17 +323,17 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
         if (lane == 0) {
           c.pos[es] = -1;
           const float toi = key_toi(ek);
-          const int2 pr = pairs[key_pair(ek)];
-          const int ki = b.ccd_key[pr.x], kj = b.ccd_key[pr.y];
-          const float t1 = c.frozen[ki], t2 = c.frozen[kj];
+          const int4 sb = c.slot_bodies[es];
+          const int ki = sb.z & kKey, kj = sb.w & kKey;
+          const float t1 = fz[ki], t2 = fz[kj];
           if (t1 >= 0.0f && t2 >= 0.0f) {
             if (toi == (t1 > t2 ? t1 : t2)) c.impacts[nimp++] = ek;
           } else {
-            if (t1 < 0.0f) c.frozen[ki] = toi;
-            if (t2 < 0.0f) c.frozen[kj] = toi;
+            if (t1 < 0.0f) fz[ki] = toi;
+            if (t2 < 0.0f) fz[kj] = toi;
             c.impacts[nimp++] = ek;
-            s_i = pr.x;
-            s_j = pr.y;
+            s_i = sb.x;
+            s_j = sb.y;
             d = 1;
           }
         }
@@ -346,18 +357,17 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     // the event's own pair is in both lists and skipped there (both bodies are frozen). A slot's
     // entry is replaced by its new time, or removed when there is none.
     const int si = s_i, sj = s_j;
-    const int ni = b.status[si] == STATUS_DYNAMIC ? int(c.deg[si]) : 0;
-    const int nj = b.status[sj] == STATUS_DYNAMIC ? int(c.deg[sj]) : 0;
+    const int ni = int(c.deg[si]), nj = int(c.deg[sj]);  // 0 for a body that is not dynamic
     for (int k = t; k < ni + nj; k += kEventBlock) {
       const int slot = k < ni ? c.adj[c.off[si] + k] : c.adj[c.off[sj] + (k - ni)];
+      const int4 sb = c.slot_bodies[slot];
       const int p = c.slot_pair[slot];
-      const int2 pr = pairs[p];
-      const float fa = c.frozen[b.ccd_key[pr.x]], fb = c.frozen[b.ccd_key[pr.y]];
-      const bool sa = b.status[pr.x] != STATUS_DYNAMIC, sb = b.status[pr.y] != STATUS_DYNAMIC;
-      if ((fa >= 0.0f || sa) && (fb >= 0.0f || sb)) continue;
+      const int had = c.pos[slot];
+      const float fa = fz[sb.z & kKey], fb = fz[sb.w & kKey];
+      if ((fa >= 0.0f || (sb.z & kSlotStatic)) && (fb >= 0.0f || (sb.w & kSlotStatic))) continue;
       float toi;
-      const bool hit = pair_toi(b, c, pr.x, pr.y, fa, fb, end_time, target, on_penetration, &toi);
-      if (hit || c.pos[slot] >= 0) {
+      const bool hit = pair_toi(b, c, sb.x, sb.y, fa, fb, end_time, target, on_penetration, &toi);
+      if (hit || had >= 0) {
         const int st = atomicAdd(&s_nstage, 1);
         c.stage[st] = ToiEntry{hit ? toi_key(toi, p) : kNoKey, slot};
       }
@@ -410,6 +420,10 @@ __global__ void __launch_bounds__(kEventBlock) k_ccd_events(DevBodies b, const i
     }
     __syncthreads();
     if (!s_go) break;
+  }
+  if (LdsFrozen) {
+    __syncthreads();
+    for (int k = t; k < b.n; k += kEventBlock) c.frozen[k] = s_fz[k];
   }
   if (t == 0) {
     const int nimp = s_nimp;
@@ -532,12 +546,16 @@ void launch_ccd_predict(const DevBodies& b, const int2* pairs, int npairs, CcdBu
   if (npairs > 0) {
     hipLaunchKernelGGL(k_ccd_first_pass, dim3(blocks_of(npairs, 128)), dim3(128), 0, s, b, pairs, npairs, c,
                        end_time, dt0, target, on_penetration);
-    hipLaunchKernelGGL(k_ccd_degree, dim3(blocks_of(npairs, 256)), dim3(256), 0, s, b, pairs, npairs, c);
+    hipLaunchKernelGGL(k_ccd_degree, dim3(blocks_of(npairs, 256)), dim3(256), 0, s, npairs, c);
     scan_exclusive_u32(c.deg, c.off, b.n, c.tmp, s);
-    hipLaunchKernelGGL(k_ccd_fill, dim3(blocks_of(npairs, 256)), dim3(256), 0, s, b, pairs, npairs, c);
+    hipLaunchKernelGGL(k_ccd_fill, dim3(blocks_of(npairs, 256)), dim3(256), 0, s, npairs, c);
   }
-  hipLaunchKernelGGL(k_ccd_events, dim3(1), dim3(kEventBlock), 0, s, b, pairs, c, end_time, dt0, target,
-                     on_penetration);
+  if (b.n <= kFzCap)
+    hipLaunchKernelGGL(k_ccd_events<true>, dim3(1), dim3(kEventBlock), 0, s, b, c, end_time, dt0, target,
+                       on_penetration);
+  else
+    hipLaunchKernelGGL(k_ccd_events<false>, dim3(1), dim3(kEventBlock), 0, s, b, c, end_time, dt0, target,
+                       on_penetration);
 }
 
 void launch_ccd_seed(const DevBodies& b, const int2* pairs, CcdBufs& c, int nimp, hipStream_t s) {

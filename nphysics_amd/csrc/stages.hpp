@@ -228,6 +228,8 @@ struct CcdBufs {
   float* body_time;        // [n] time each body's motion starts from (SubstepState::body_times)
   float* frozen;           // [n] per body key: freeze time, < 0 when not frozen
   int* slot_pair;          // [P] pair index of each CCD slot (pairs with a CCD-enabled side)
+  int4* slot_bodies;       // [P] the slot's two bodies and their keys; a key | kSlotStatic when its
+                           // body is not dynamic
   int* pos;                // [P] node of each slot's entry in the queue, -1 when it has none
   ToiEntry* heap;          // [cap] first-pass entries, then the queue's nodes below its LDS levels
   ToiEntry* stage;         // [P] entries of one re-sweep
@@ -255,6 +257,7 @@ enum : int {
   kCcdLastToi = 6, kCcdOverflow = 7, kCcdMinOver = 8, kCcdList = 9, kCcdWords = 16
 };
 enum : int { PREDICT_NONE = 0, PREDICT_AFTER_END = 1, PREDICT_IMPACTS = 2 };
+constexpr int kSlotStatic = 1 << 30;  // body counts stay below 2^30
 // predict_next_impacts up to end_time: control words reset, freeze times cleared, every CCD pair's
 // time of impact, then the queue in one workgroup (kind, impacts, last impact time in ctl)
 void launch_ccd_predict(const DevBodies& b, const int2* pairs, int npairs, CcdBufs& c, float end_time, float dt0,

@@ -223,7 +223,7 @@ struct World {
   std::map<std::pair<int, int>, int> static_owner;  // (scene, static body index) -> first collider
   DBuf ccd_flags, ccd_keys, ccd_rot, ccd_time, ccd_frozen, ccd_slot, ccd_heap, ccd_stage, ccd_imp, ccd_ctl,
       ccd_in, ccd_new, ccd_prep, ccd_prepared, ccd_eval, ccd_kept, ccd_cand, ccd_cpos, ccd_list, ccd_lpair, ccd_tmp,
-      ccd_deg, ccd_off, ccd_fill, ccd_adj, ccd_pos;
+      ccd_deg, ccd_off, ccd_fill, ccd_adj, ccd_pos, ccd_sbodies;
   ContactBufs ccd_cb;
   int ccd_cap = 0, ccd_nimp = 0;
   float ccd_last_toi = 0.0f, ccd_min_over = 0.0f;
@@ -1688,6 +1688,7 @@ struct World {
 
   // ------------------------------------------------------------------ CCD
   int ensure_ccd() {
+    if (n >= kSlotStatic) return set_err(NPHYS_ERR_CAPACITY, "CCD supports fewer than 2^30 bodies");
     const size_t nn = size_t(std::max(n, 1)), P = size_t(std::max(npairs, 1));
     for (DBuf* b : {&ccd_time, &ccd_frozen, &ccd_in, &ccd_new, &ccd_prep, &ccd_prepared, &ccd_deg, &ccd_off, &ccd_fill})
       HIPCK(b->ensure(nn * 4));
@@ -1695,6 +1696,7 @@ struct World {
     for (DBuf* b : {&ccd_slot, &ccd_pos, &ccd_eval, &ccd_kept, &ccd_cand, &ccd_cpos, &ccd_lpair})
       HIPCK(b->ensure(P * 4));
     HIPCK(ccd_list.ensure(P * sizeof(int2)));
+    HIPCK(ccd_sbodies.ensure(P * sizeof(int4)));
     HIPCK(ccd_stage.ensure(P * sizeof(ToiEntry)));
     // the queue holds at most one entry per CCD slot (at most P); the impact list takes 3P + 4096
     // confirmed impacts (a full list is an error, never a silent loss)
@@ -1712,6 +1714,7 @@ struct World {
     c.body_time = ccd_time.as<float>();
     c.frozen = ccd_frozen.as<float>();
     c.slot_pair = ccd_slot.as<int>();
+    c.slot_bodies = ccd_sbodies.as<int4>();
     c.pos = ccd_pos.as<int>();
     c.heap = ccd_heap.as<ToiEntry>();
     c.stage = ccd_stage.as<ToiEntry>();
