@@ -645,7 +645,7 @@ def main():
                     help="only run a plain 512 MiB device copy 5 times (the reference kernel of the "
                          "FETCH_SIZE / WRITE_SIZE counter calibration) and exit")
     ap.add_argument("--sweep-variant", type=int, default=-1,
-                    help="-1 autotune (default), 0 per-colour launches, 64/128/256 scene-blocked")
+                    help="-1 autotune (default), 0 per-colour launches, 256/512/1024 scene-blocked")
     ap.add_argument("--workload", choices=("pile", "batched", "sleeping", "ccd"), default="pile",
                     help="pile: the dense pile (default); batched: 4,096 independent worlds in one handle; "
                          "sleeping: the batched worlds with sleep threshold 0.01; ccd: the reference's CCD "

@@ -192,7 +192,7 @@ typedef struct nphys_counters {
   int64_t judge_plain_sweeps;
   int64_t judge_learned_sweeps;
   int64_t judge_kept;
-  int64_t sweep_variant;  /* sweeps of the last step: 0 per-colour launches, 64/128/256 scene-blocked */
+  int64_t sweep_variant;  /* solve of the last step: 0 per-colour launches, 256/512/1024 scene-blocked */
   int64_t loose_changes;  /* accumulated loose AABBs replaced by the end-of-step syncs */
   int64_t pair_rebuilds;  /* accumulated end-of-step broad-phase pair rebuilds */
   /* accumulated pair rebuilds done incrementally (only the bodies whose loose AABB was replaced
@@ -351,12 +351,15 @@ int nphys_counters_enable(nphys_world* w, int32_t enabled);
 int nphys_set_residual_target(nphys_world* w, double target, int32_t max_sweeps, int32_t check_every,
                               int32_t norm);
 
-/* Launch shape of the velocity sweeps. 0: one launch per colour and sweep over all manifolds of
- * the colour; 64, 128, 256: one launch per solve with one workgroup of that size per scene,
- * the scene's velocity increments in LDS (only when every scene has at most 2048 dynamic
- * bodies; otherwise 0 is used); -1 (default): an autotuner times every eligible variant over
- * the first solves and keeps the fastest. All variants give bit-identical results. No reference
- * counterpart (the reference has one sequential loop). */
+/* Launch shape of the velocity sweeps and the position correction. 0: one launch per colour and
+ * pass over all manifolds of the colour; 256, 512, 1024: per solve one launch for the warm start
+ * and sweeps and one for the position passes, one workgroup per scene (the scene's velocity
+ * increments, then its poses, in LDS; that many lanes for a scene of more than 256 dynamic bodies,
+ * 64 for the others, which run beside them on a second stream). Scene-blocked only when every
+ * scene has at most 2048 dynamic bodies and the world has no joints; otherwise 0 is used.
+ * -1 (default): an autotuner times every eligible variant over the first steps and keeps the
+ * fastest. All variants give bit-identical results. No reference counterpart (the reference has
+ * one sequential loop). */
 int nphys_set_sweep_variant(nphys_world* w, int32_t variant);
 
 /* How the pair set follows replaced loose AABBs. 1 (default): when only dynamic bodies' loose

@@ -20,9 +20,46 @@ namespace nph {
 
 namespace {
 
-template <bool kMask>
-__device__ __forceinline__ void position_one(const DevBodies& b, const SimParams& p, const SolverBufs& sb,
-                                             int slot) {
+// Poses of a manifold's bodies: in HBM (the per-colour kernels) ...
+struct GlobalPose {
+  DevBodies b;
+  __device__ void load(int i, bool, f3& t, q4& q) const {
+    t = xyz(b.pos(i));
+    q = toq(b.rot(i));
+  }
+  __device__ void store(int i, f3 t, q4 q) const {
+    b.pos(i) = f4(t, 0.0f);
+    b.rot(i) = fromq(q);
+  }
+};
+
+// ... or in LDS for the scene's dynamic bodies, indexed by their rank in the scene (the
+// scene-blocked kernel); a static body's pose is read from HBM, and never written.
+struct LdsPose {
+  DevBodies b;
+  float4* pos;
+  float4* rot;
+  const int* local;
+  __device__ void load(int i, bool dyn, f3& t, q4& q) const {
+    if (dyn) {
+      const int k = local[i];
+      t = xyz(pos[k]);
+      q = toq(rot[k]);
+    } else {
+      t = xyz(b.pos(i));
+      q = toq(b.rot(i));
+    }
+  }
+  __device__ void store(int i, f3 t, q4 q) const {
+    const int k = local[i];
+    pos[k] = f4(t, 0.0f);
+    rot[k] = fromq(q);
+  }
+};
+
+template <bool kMask, class A>
+__device__ __forceinline__ void position_one(const A& acc, const DevBodies& b, const SimParams& p,
+                                             const SolverBufs& sb, int slot) {
   const int4 q = sb.sinfo[slot];
   const int b1 = q.x & 0x3fffffff, b2 = q.y & 0x3fffffff;
   const bool d1 = (q.x >> 30) & 1;
@@ -32,8 +69,10 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const SimParams
   const float4 g1 = sb.pgeo[2 * long(slot)], g2 = sb.pgeo[2 * long(slot) + 1];
   const f3 h1 = xyz(g1), h2 = xyz(g2);
   const float m1 = g1.w, m2 = g2.w;
-  f3 t1 = xyz(b.pos(b1)), t2 = xyz(b.pos(b2));
-  q4 q1 = toq(b.rot(b1)), q2 = toq(b.rot(b2));
+  f3 t1, t2;
+  q4 q1, q2;
+  acc.load(b1, d1, t1, q1);
+  acc.load(b2, d2, t2, q2);
   float im1 = 0.0f, im2 = 0.0f;
   m33 I1, I2;
   if (d1) I1 = load_inv(b, b1, im1);
@@ -102,31 +141,60 @@ __device__ __forceinline__ void position_one(const DevBodies& b, const SimParams
       moved2 = true;
     }
   }
-  if (moved1) {
-    b.pos(b1) = f4(t1, 0.0f);
-    b.rot(b1) = fromq(q1);
-  }
-  if (moved2) {
-    b.pos(b2) = f4(t2, 0.0f);
-    b.rot(b2) = fromq(q2);
-  }
+  if (moved1) acc.store(b1, t1, q1);
+  if (moved2) acc.store(b2, t2, q2);
 }
 
 template <bool kMask>
 __global__ void __launch_bounds__(64) k_position(DevBodies b, SimParams p, SolverBufs sb, int first_slot,
                                                 int nslots) {
+  const GlobalPose acc{b};
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nslots; t += gridDim.x * blockDim.x)
-    position_one<kMask>(b, p, sb, first_slot + t);
+    position_one<kMask>(acc, b, p, sb, first_slot + t);
 }
 
 // Consecutive small classes of the visiting order in one workgroup, a barrier between classes
 // (the same corrections in the same order per body as one launch per class).
 template <bool kMask>
 __global__ void __launch_bounds__(kRunBlock) k_position_run(DevBodies b, SimParams p, SolverBufs sb, int kv0, int nk) {
+  const GlobalPose acc{b};
   for (int kv = kv0; kv < kv0 + nk; ++kv) {
     const int2 range = sb.vrange[kv];
-    for (int t = threadIdx.x; t < range.y; t += kRunBlock) position_one<kMask>(b, p, sb, range.x + t);
+    for (int t = threadIdx.x; t < range.y; t += kRunBlock) position_one<kMask>(acc, b, p, sb, range.x + t);
     __syncthreads();
+  }
+}
+
+// Scene-blocked position correction: one workgroup per scene, the scene's dynamic poses in LDS,
+// `iterations` passes over the colours in the visiting order with a barrier between colours.
+template <int kBlock, int kCap, bool kMask>
+__global__ void __launch_bounds__(kBlock) k_position_blocked(DevBodies b, SimParams p, SolverBufs sb, SceneBlocks sc,
+                                                           const int* __restrict__ scene_list, int ncolours,
+                                                           int iterations) {
+  __shared__ float4 s_pos[kCap];
+  __shared__ float4 s_rot[kCap];
+  const int s = scene_list[blockIdx.x];
+  const int d0 = sc.dyn_start[s], nd = sc.dyn_start[s + 1] - d0;
+  for (int i = threadIdx.x; i < nd; i += kBlock) {
+    const int body = sc.dyn_list[d0 + i];
+    s_pos[i] = b.pos(body);
+    s_rot[i] = b.rot(body);
+  }
+  __syncthreads();
+  const LdsPose acc{b, s_pos, s_rot, sc.local};
+  const uint32_t* st = sc.start + long(s) * (ncolours + 1);
+  for (int it = 0; it < iterations; ++it) {
+    for (int k = 0; k < ncolours; ++k) {
+      const int col = sb.visit[k];
+      const int e = int(st[col + 1]);
+      for (int i = int(st[col]) + threadIdx.x; i < e; i += kBlock) position_one<kMask>(acc, b, p, sb, int(sc.slot[i]));
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < nd; i += kBlock) {
+    const int body = sc.dyn_list[d0 + i];
+    b.pos(body) = s_pos[i];
+    b.rot(body) = s_rot[i];
   }
 }
 
@@ -140,6 +208,31 @@ void launch_position_colour(const DevBodies& b, const DevContacts& c, const SimP
   (void)c;
   if (b.kmask) hipLaunchKernelGGL(k_position<true>, dim3(blocks), dim3(64), 0, s, b, p, sb, first, n);
   else hipLaunchKernelGGL(k_position<false>, dim3(blocks), dim3(64), 0, s, b, p, sb, first, n);
+}
+
+template <bool kMask>
+void position_blocked(const DevBodies& b, const SimParams& p, SolverBufs& sb, const SceneBlocks& sc, int ncolours,
+                      int iterations, int block, hipStream_t s_small, hipStream_t s_large) {
+  if (sc.n_large > 0) {
+    if (block == 256)
+      hipLaunchKernelGGL((k_position_blocked<256, kLdsBodies, kMask>), dim3(sc.n_large), dim3(256), 0, s_large, b, p,
+                         sb, sc, sc.large_list, ncolours, iterations);
+    else if (block == 512)
+      hipLaunchKernelGGL((k_position_blocked<512, kLdsBodies, kMask>), dim3(sc.n_large), dim3(512), 0, s_large, b, p,
+                         sb, sc, sc.large_list, ncolours, iterations);
+    else
+      hipLaunchKernelGGL((k_position_blocked<1024, kLdsBodies, kMask>), dim3(sc.n_large), dim3(1024), 0, s_large, b,
+                         p, sb, sc, sc.large_list, ncolours, iterations);
+  }
+  if (sc.n_small > 0)
+    hipLaunchKernelGGL((k_position_blocked<64, kSmallScene, kMask>), dim3(sc.n_small), dim3(64), 0, s_small, b, p, sb,
+                       sc, sc.small_list, ncolours, iterations);
+}
+
+void launch_position_blocked(const DevBodies& b, const SimParams& p, SolverBufs& sb, const SceneBlocks& sc,
+                             int ncolours, int iterations, int block, hipStream_t s_small, hipStream_t s_large) {
+  if (b.kmask) position_blocked<true>(b, p, sb, sc, ncolours, iterations, block, s_small, s_large);
+  else position_blocked<false>(b, p, sb, sc, ncolours, iterations, block, s_small, s_large);
 }
 
 void launch_position_run(const DevBodies& b, const SimParams& p, SolverBufs& sb, int kv0, int nk, hipStream_t s) {

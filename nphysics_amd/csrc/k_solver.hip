@@ -555,6 +555,8 @@ struct GlobalDv {
   __device__ void load(const Manifold<kMask>& m, Dv& v1, Dv& v2) const { load_dv(b, m, v1, v2); }
   template <bool kMask>
   __device__ void store(const Manifold<kMask>& m, const Dv& v1, const Dv& v2) const { store_dv(b, m, v1, v2); }
+  __device__ void load1(int body, Dv& v) const { b.load_dv(body, v.l, v.a); }
+  __device__ void store1(int body, const Dv& v) const { b.store_dv(body, v.l, v.a); }
 };
 
 // ... or in LDS, indexed by the body's rank among its scene's dynamic bodies (the scene-blocked
@@ -588,6 +590,16 @@ struct LdsDv {
       lin[i] = f4(v2.l, 0.0f);
       ang[i] = f4(v2.a, 0.0f);
     }
+  }
+  __device__ void load1(int body, Dv& v) const {
+    const int i = local[body];
+    v.l = xyz(lin[i]);
+    v.a = xyz(ang[i]);
+  }
+  __device__ void store1(int body, const Dv& v) const {
+    const int i = local[body];
+    lin[i] = f4(v.l, 0.0f);
+    ang[i] = f4(v.a, 0.0f);
   }
 };
 
@@ -705,8 +717,9 @@ __device__ __forceinline__ void load_half(const SolverBufs& sb, uint32_t r, int 
 // arithmetic as pgs_manifold: each lane evaluates its body's dot6 of the row, the two partial sums
 // are exchanged between the lanes (adjacent lanes of one wavefront), both lanes form
 // w = (s1 + s2) + rhs in that order and the same new impulse, and each lane updates its own body.
-template <bool kMask>
-__device__ __forceinline__ void pgs_manifold_pair(const DevBodies& b, const SolverBufs& sb, int slot, int h) {
+template <bool kMask, class A>
+__device__ __forceinline__ void pgs_manifold_pair(const A& acc, const DevBodies& b, const SolverBufs& sb, int slot,
+                                                  int h) {
   const SlotCtx x = slot_ctx(sb, slot);
   const int body = h ? x.b2 : x.b1;
   const bool dyn = h ? x.d2 : x.d1;
@@ -716,7 +729,7 @@ __device__ __forceinline__ void pgs_manifold_pair(const DevBodies& b, const Solv
   f3 ml{1.0f, 1.0f, 1.0f}, ma{1.0f, 1.0f, 1.0f};
   if (kMask) b.mask(body, ml, ma);
   Dv v{};
-  if (dyn) b.load_dv(body, v.l, v.a);
+  if (dyn) acc.load1(body, v);
   CHalf c[4];
 #pragma unroll
   for (int k = 0; k < 4; ++k)
@@ -777,7 +790,7 @@ __device__ __forceinline__ void pgs_manifold_pair(const DevBodies& b, const Solv
       else c[k].lam.z = li;
     }
   }
-  if (dyn) b.store_dv(body, v.l, v.a);
+  if (dyn) acc.store1(body, v);
   if (h == 0) {
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -789,8 +802,9 @@ __device__ __forceinline__ void pgs_manifold_pair(const DevBodies& b, const Solv
 // one wavefront: blockDim and the stride are even).
 template <bool kMask>
 __global__ void __launch_bounds__(64) k_pgs(DevBodies b, DevContacts c, SolverBufs sb, int first, int n) {
+  const GlobalDv acc{b};
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; (t >> 1) < n; t += gridDim.x * blockDim.x)
-    pgs_manifold_pair<kMask>(b, sb, first + (t >> 1), t & 1);
+    pgs_manifold_pair<kMask>(acc, b, sb, first + (t >> 1), t & 1);
 }
 
 // A run of consecutive small colour classes of the visiting order in one workgroup, a barrier
@@ -810,11 +824,12 @@ __global__ void __launch_bounds__(kRunBlock) k_colour_run(DevBodies b, SolverBuf
   }
 }
 
-// Scene-blocked sweeps: one workgroup per scene runs the warm start (when warm) and nsweeps
-// sweeps over all colours in the visiting order, with the scene's dynamic-body velocity
-// increments resident in LDS and a workgroup barrier between colours. Every body sees the same
-// sequence of row updates as under the per-colour launches (manifolds of one colour share no
-// dynamic body; colours and sweeps keep their order), so the results are identical.
+// Scene-blocked sweeps: one workgroup per scene runs the warm start (when warm, one lane per
+// manifold) and nsweeps sweeps (two lanes per manifold, as k_pgs) over all colours in the visiting
+// order, with the scene's dynamic-body velocity increments resident in LDS and a workgroup barrier
+// between colours. Every body sees the same sequence of row updates as under the per-colour
+// launches (manifolds of one colour share no dynamic body; colours and sweeps keep their order),
+// so the results are identical.
 template <int kBlock, int kCap, bool kMask>
 __global__ void __launch_bounds__(kBlock) k_sweeps_blocked(DevBodies b, SolverBufs sb, SceneBlocks sc,
                                                          const int* __restrict__ scene_list, int ncolours,
@@ -836,11 +851,13 @@ __global__ void __launch_bounds__(kBlock) k_sweeps_blocked(DevBodies b, SolverBu
   for (int pass = warm ? -1 : 0; pass < nsweeps; ++pass) {
     for (int k = 0; k < ncolours; ++k) {
       const int col = sb.visit[k];
-      const int e = int(st[col + 1]);
-      for (int i = int(st[col]) + threadIdx.x; i < e; i += kBlock) {
-        const int slot = int(sc.slot[i]);
-        if (pass < 0) warm_manifold<kMask>(acc, b, sb, slot);
-        else pgs_manifold<kMask>(acc, b, sb, slot);
+      const int s0 = int(st[col]), e = int(st[col + 1]);
+      if (pass < 0) {
+        for (int i = s0 + threadIdx.x; i < e; i += kBlock) warm_manifold<kMask>(acc, b, sb, int(sc.slot[i]));
+      } else {
+        // both lanes of a manifold are adjacent lanes of one wavefront (kBlock and 2 * s0 are even)
+        for (int i = 2 * s0 + threadIdx.x; i < 2 * e; i += kBlock)
+          pgs_manifold_pair<kMask>(acc, b, sb, int(sc.slot[i >> 1]), i & 1);
       }
       __syncthreads();
     }
@@ -1187,31 +1204,34 @@ void launch_scene_order(const DevBodies& b, int M, int nscenes, int ncolours, So
   hipLaunchKernelGGL(k_scene_starts, dim3(nb(total)), dim3(256), 0, s, sc.keys, M, nscenes, ncolours, sc.start);
 }
 
-// Two launches: scenes with at most kSmallScene dynamic bodies (64-lane workgroups, 8 KB of LDS,
-// many resident per CU), then the larger ones (`block` lanes, 64 KB of LDS).
+// Two launches: the scenes with more than kSmallScene dynamic bodies (`block` lanes, 64 KB of LDS)
+// on s_large, the others (64-lane workgroups, 8 KB of LDS, many resident per CU) on s_small, so the
+// few large scenes run beside the many small ones.
 template <bool kMask>
 void sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int ncolours, int nsweeps, bool warm,
-                    int block, hipStream_t s) {
+                    int block, hipStream_t s_small, hipStream_t s_large) {
   const int w = warm ? 1 : 0;
+  if (kMask && block == 1024) block = 512;  // the masked kernel would spill at 1024 lanes (128 VGPRs)
+  if (sc.n_large > 0) {
+    if (block == 256)
+      hipLaunchKernelGGL((k_sweeps_blocked<256, kLdsBodies, kMask>), dim3(sc.n_large), dim3(256), 0, s_large, b, sb,
+                         sc, sc.large_list, ncolours, nsweeps, w);
+    else if (kMask || block == 512)
+      hipLaunchKernelGGL((k_sweeps_blocked<512, kLdsBodies, kMask>), dim3(sc.n_large), dim3(512), 0, s_large, b, sb,
+                         sc, sc.large_list, ncolours, nsweeps, w);
+    else
+      hipLaunchKernelGGL((k_sweeps_blocked<1024, kLdsBodies, false>), dim3(sc.n_large), dim3(1024), 0, s_large, b,
+                         sb, sc, sc.large_list, ncolours, nsweeps, w);
+  }
   if (sc.n_small > 0)
-    hipLaunchKernelGGL((k_sweeps_blocked<64, kSmallScene, kMask>), dim3(sc.n_small), dim3(64), 0, s, b, sb, sc,
+    hipLaunchKernelGGL((k_sweeps_blocked<64, kSmallScene, kMask>), dim3(sc.n_small), dim3(64), 0, s_small, b, sb, sc,
                        sc.small_list, ncolours, nsweeps, w);
-  if (sc.n_large == 0) return;
-  if (block == 64)
-    hipLaunchKernelGGL((k_sweeps_blocked<64, kLdsBodies, kMask>), dim3(sc.n_large), dim3(64), 0, s, b, sb, sc,
-                       sc.large_list, ncolours, nsweeps, w);
-  else if (block == 128)
-    hipLaunchKernelGGL((k_sweeps_blocked<128, kLdsBodies, kMask>), dim3(sc.n_large), dim3(128), 0, s, b, sb, sc,
-                       sc.large_list, ncolours, nsweeps, w);
-  else
-    hipLaunchKernelGGL((k_sweeps_blocked<256, kLdsBodies, kMask>), dim3(sc.n_large), dim3(256), 0, s, b, sb, sc,
-                       sc.large_list, ncolours, nsweeps, w);
 }
 
 void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int ncolours, int nsweeps,
-                           bool warm, int block, hipStream_t s) {
-  if (b.kmask) sweeps_blocked<true>(b, sb, sc, ncolours, nsweeps, warm, block, s);
-  else sweeps_blocked<false>(b, sb, sc, ncolours, nsweeps, warm, block, s);
+                           bool warm, int block, hipStream_t s_small, hipStream_t s_large) {
+  if (b.kmask) sweeps_blocked<true>(b, sb, sc, ncolours, nsweeps, warm, block, s_small, s_large);
+  else sweeps_blocked<false>(b, sb, sc, ncolours, nsweeps, warm, block, s_small, s_large);
 }
 
 }  // namespace nph

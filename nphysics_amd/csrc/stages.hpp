@@ -170,8 +170,14 @@ struct SceneBlocks {
 };
 void launch_scene_order(const DevBodies& b, int M, int nscenes, int ncolours, SolverBufs& sb, SceneBlocks& sc,
                         hipStream_t s);
+// block: lanes per large scene (256, 512 or 1024); the large scenes go on s_large, the small on s_small
 void launch_sweeps_blocked(const DevBodies& b, SolverBufs& sb, const SceneBlocks& sc, int ncolours, int nsweeps,
-                           bool warm, int block, hipStream_t s);
+                           bool warm, int block, hipStream_t s_small, hipStream_t s_large);
+// Scene-blocked position correction: `iterations` passes over the colours in the visiting order,
+// one workgroup per scene with the scene's dynamic poses in LDS (the same corrections in the same
+// order per body as the per-colour launches); streams and block as launch_sweeps_blocked.
+void launch_position_blocked(const DevBodies& b, const SimParams& p, SolverBufs& sb, const SceneBlocks& sc,
+                             int ncolours, int iterations, int block, hipStream_t s_small, hipStream_t s_large);
 
 // learned warm start (k_solver.hip): features [C][24] and impulse units ls [C] per row record,
 // the prediction y [C][3] applied as the warm start, post-solve targets t [C][3].

@@ -91,6 +91,10 @@ __global__ void __launch_bounds__(256) k_readback(RbJob j, uint32_t* host, int f
 
 struct World {
   hipStream_t stream = nullptr;
+  // the scene-blocked solve runs its large scenes on stream_large beside the small ones on stream
+  // (fork and join by events)
+  hipStream_t stream_large = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   int device = 0;
   nphys_params prm{};
   SimParams sp{};
@@ -183,9 +187,9 @@ struct World {
   std::vector<double> red_host;
   // learned warm start: network (FP32 parameters + BF16 images), per-record features, impulse
   // units, predictions and recorded targets; the judge's saved plain solve
-  // scene-blocked sweeps: per-scene dynamic-body lists (host-built at insertion), the per-solve
+  // scene-blocked solve: per-scene dynamic-body lists (host-built at insertion), the per-solve
   // scene-major slot order, and the variant chosen by the autotuner (0 per-colour launches,
-  // 64/128/256 the blocked kernel's workgroup size)
+  // 256/512/1024 the blocked kernels' workgroup size for the scenes of more than 256 dynamic bodies)
   int nscenes = 1, max_scene_dyn = 0;
   DBuf sc_dyn_start, sc_dyn_list, sc_local, sc_keys, sc_slot, sc_keys_alt, sc_slot_alt, sc_start, sc_small, sc_large;
   int n_small_scenes = 0, n_large_scenes = 0;
@@ -259,6 +263,9 @@ struct World {
       if (e) (void)hipEventDestroy(e);
     for (auto& e : ml_ev)
       if (e) (void)hipEventDestroy(e);
+    if (fork_ev) (void)hipEventDestroy(fork_ev);
+    if (join_ev) (void)hipEventDestroy(join_ev);
+    if (stream_large) (void)hipStreamDestroy(stream_large);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -286,6 +293,9 @@ struct World {
       return set_err(NPHYS_ERR_NO_DEVICE, "no HIP device");
     HIPCK(hipGetDevice(&device));
     HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    HIPCK(hipStreamCreateWithFlags(&stream_large, hipStreamNonBlocking));
+    HIPCK(hipEventCreateWithFlags(&fork_ev, hipEventDisableTiming));
+    HIPCK(hipEventCreateWithFlags(&join_ev, hipEventDisableTiming));
     HIPCK(hipHostMalloc(reinterpret_cast<void**>(&rb_host), kRbWords * sizeof(uint32_t),
                         hipHostMallocMapped | hipHostMallocCoherent));
     std::memset(rb_host, 0, kRbWords * sizeof(uint32_t));
@@ -1233,7 +1243,11 @@ struct World {
         }
         HIPCK(hipEventRecord(kt_ev[kt_used], stream));
       }
-      launch_sweeps_blocked(b, v, sc, ncolours, nsweeps, warm, cur_variant, stream);
+      HIPCK(hipEventRecord(fork_ev, stream));
+      HIPCK(hipStreamWaitEvent(stream_large, fork_ev, 0));
+      launch_sweeps_blocked(b, v, sc, ncolours, nsweeps, warm, cur_variant, stream, stream_large);
+      HIPCK(hipEventRecord(join_ev, stream_large));
+      HIPCK(hipStreamWaitEvent(stream, join_ev, 0));
       if (counters_enabled) {
         HIPCK(hipEventRecord(kt_ev[kt_used + 1], stream));
         kt_used += 2;
@@ -1408,7 +1422,7 @@ struct World {
     }
     if (tune_cands.empty()) {
       tune_M = M;
-      tune_cands = {0, 64, 128, 256};
+      tune_cands = {0, 256, 512, 1024};
       tune_time.assign(tune_cands.size(), 0.0);
       tune_count.assign(tune_cands.size(), 0);
     }
@@ -1635,7 +1649,6 @@ struct World {
       if (learned && (rc = predict(v, C))) return rc;
       if ((rc = solve_sweeps(b, cv, v, M, C, sblocks, last_sweeps))) return rc;
     }
-    if (tune_pending) HIPCK(hipEventRecord(tune_ev[1], stream));
     if (capture && C > 0) {
       HIPCK(ml_t.ensure(size_t(C) * kOut * sizeof(float)));
       launch_sample_targets(v, C, ml_ls.as<float>(), ml_t.as<float>(), stream);
@@ -1662,6 +1675,16 @@ struct World {
     SolverBufs v = sv_view();
     JointBufs jv = joint_view();
     const DevContacts cv = sview[cur];
+    if (cur_variant > 0 && !joints) {
+      // scene-blocked, as the sweeps of this solve (worlds without joints)
+      SceneBlocks sc = sc_view();
+      HIPCK(hipEventRecord(fork_ev, stream));
+      HIPCK(hipStreamWaitEvent(stream_large, fork_ev, 0));
+      launch_position_blocked(b, ps, v, sc, ncolours, iterations, cur_variant, stream, stream_large);
+      HIPCK(hipEventRecord(join_ev, stream_large));
+      HIPCK(hipStreamWaitEvent(stream, join_ev, 0));
+      return NPHYS_OK;
+    }
     const int pblocks = position_blocks(max_class);
     // direct launches, runs of two or more consecutive small classes in one workgroup (as the
     // sweeps); the joints' constraints first in every iteration (nonlinear_sor_prox.rs:33-38)
@@ -1894,6 +1917,7 @@ struct World {
     HIPCK(hipEventRecord(ev[EV_INT], stream));
     if ((rc = position_correction(sp, prm.max_position_iterations))) return rc;
     HIPCK(hipEventRecord(ev[EV_POS], stream));
+    if (tune_pending) HIPCK(hipEventRecord(tune_ev[1], stream));  // the variant's sweeps and position passes
     // this step's manifolds and impulses become the warm-start cache
     cur ^= 1;
     have_solved = true;
@@ -2258,8 +2282,8 @@ int nphys_read_samples(nphys_world* w, float* features, float* targets, int64_t 
 
 int nphys_set_sweep_variant(nphys_world* w, int32_t variant) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
-  if (variant != -1 && variant != 0 && variant != 64 && variant != 128 && variant != 256)
-    return set_err(NPHYS_ERR_INVALID, "variant must be -1 (autotune), 0, 64, 128 or 256");
+  if (variant != -1 && variant != 0 && variant != 256 && variant != 512 && variant != 1024)
+    return set_err(NPHYS_ERR_INVALID, "variant must be -1 (autotune), 0, 256, 512 or 1024");
   w->w.variant_request = variant;
   w->w.tune_cands.clear();
   w->w.tuned = false;

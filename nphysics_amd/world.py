@@ -302,7 +302,7 @@ class MechanicalWorld:
         return X, T
 
     def set_sweep_variant(self, variant=-1):
-        """-1 autotune (default), 0 per-colour launches, 64/128/256 scene-blocked workgroups."""
+        """-1 autotune (default), 0 per-colour launches, 256/512/1024 scene-blocked (lanes per large scene)."""
         _check(self._L.nphys_set_sweep_variant(self._h, variant))
 
     def set_incremental_pairs(self, on=True):
