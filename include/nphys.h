@@ -132,6 +132,13 @@ typedef struct nphys_body_desc {
    * and traverses that body as a whole. Negative or NULL: a body of its own. Ignored for dynamic
    * bodies. */
   const int32_t* static_body;
+  /* 3n: ColliderDesc::collision_groups (collider.rs:444,506), ncollide3d 0.31's CollisionGroups as
+   * (membership, whitelist, blacklist) bit masks over groups 0-29 (bits 30-31 are ignored): two
+   * colliders interact only when each is a member of some group the other whitelists and of no
+   * group the other blacklists. Pairs that cannot interact are never formed (no contacts, no
+   * CCD). NULL: CollisionGroups::new() for every collider (member of all groups, whitelisting all,
+   * blacklisting none). */
+  const uint32_t* collision_groups;
 } nphys_body_desc;
 
 /* Joint constraints (src/joint/), structure of arrays for n joints. Kinds: BallConstraint

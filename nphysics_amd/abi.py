@@ -50,7 +50,27 @@ OPTIONAL_BODY_FIELDS = {
     "ccd_enabled": (np.int32, 1),
     "linear_motion_interpolation": (np.int32, 1),
     "static_body": (np.int32, 1),
+    "collision_groups": (np.uint32, 3),
 }
+
+# CollisionGroups masks (ncollide3d 0.31): groups 0-29
+ALL_GROUPS = (1 << 30) - 1
+
+
+def collision_groups(membership=None, whitelist=None, blacklist=()):
+    """(membership, whitelist, blacklist) masks of CollisionGroups::new() followed by
+    set_membership / set_whitelist / set_blacklist with these group lists (None keeps the
+    default: all groups for the first two)."""
+    def mask(groups, default):
+        if groups is None:
+            return default
+        m = 0
+        for g in groups:
+            if not 0 <= g < 30:
+                raise ValueError("collision group ids are 0-29")
+            m |= 1 << g
+        return m
+    return (mask(membership, ALL_GROUPS), mask(whitelist, ALL_GROUPS), mask(blacklist, 0))
 
 
 class Params(C.Structure):
@@ -136,6 +156,7 @@ class BodyDesc(C.Structure):
         ("ccd_enabled", C.POINTER(C.c_int32)),
         ("linear_motion_interpolation", C.POINTER(C.c_int32)),
         ("static_body", C.POINTER(C.c_int32)),
+        ("collision_groups", C.POINTER(C.c_uint32)),
     ]
 
 
@@ -281,5 +302,10 @@ def make_body_desc(scene):
     for name, (dtype, width) in OPTIONAL_BODY_FIELDS.items():
         if name in scene:
             a = arr(name, dtype, width)
-            setattr(d, name, iptr(a) if dtype == np.int32 else fptr(a))
+            if dtype == np.int32:
+                setattr(d, name, iptr(a))
+            elif dtype == np.uint32:
+                setattr(d, name, a.ctypes.data_as(C.POINTER(C.c_uint32)))
+            else:
+                setattr(d, name, fptr(a))
     return d, keep

@@ -36,6 +36,12 @@ struct SimParams {
 constexpr int kPoseBlock = 2;
 constexpr int kSolverBlock = 4;
 
+// CollisionGroups::can_interact_with_groups (ncollide3d 0.31, unvendored): each collider is a
+// member of some group the other whitelists and of no group the other blacklists.
+NPH_HD inline bool groups_interact(uint4 a, uint4 c) {
+  return (a.x & c.z) == 0u && (c.x & a.z) == 0u && (a.x & c.y) != 0u && (c.x & a.y) != 0u;
+}
+
 struct DevBodies {
   int n;
   const int* status;
@@ -76,6 +82,8 @@ struct DevBodies {
   const int* ccd;
   const int* ccd_key;
   float4* ccd_rot;
+  // collision groups (membership, whitelist, blacklist, -), null when every collider has the default
+  const uint4* groups;
 
   // Jacobian mask of body i (1 free, 0 kinematic), linear and angular (rigid_body.rs:46,103-118).
   NPH_HD void mask(int i, f3& ml, f3& ma) const {

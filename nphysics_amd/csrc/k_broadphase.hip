@@ -256,7 +256,8 @@ __global__ void __launch_bounds__(256) k_pack_nodes(DevBodies b, const uint32_t*
 }
 
 // Walks the packed tree for the loose AABB of body q and calls leaf(o, dynamic) for every other
-// body of q's scene whose loose AABB overlaps it. Returns false on a stack overflow.
+// body of q's scene whose loose AABB overlaps it and whose collision groups can interact with q's.
+// Returns false on a stack overflow.
 // The traversal stack lives in LDS, interleaved across the workgroup's lanes (entry k of lane t at
 // stack[k * kTraverseBlock + t]): a stack in private memory would be scratch, one memory round trip
 // per push and pop on the traversal's dependent chain.
@@ -267,6 +268,7 @@ __device__ __forceinline__ bool walk_tree(const DevBodies& b, const float4* __re
                                           int* __restrict__ stack, F&& leaf) {
   const aabb Q{xyz(b.loose_mn[q]), xyz(b.loose_mx[q])};
   const int qs = kScenes ? b.scene[q] : 0;
+  const uint4 qg = b.groups ? b.groups[q] : uint4{};
   int sp = 0;
   int node = 0;
   bool ok = true;
@@ -282,7 +284,8 @@ __device__ __forceinline__ bool walk_tree(const DevBodies& b, const float4* __re
         const uint32_t c = ref[k];
         if (c & kLeafRef) {
           const int o = int(c & 0x3fffffffu);
-          if (o != q && (!kScenes || b.scene[o] == qs)) leaf(o, ((c >> 30) & 1u) != 0);
+          if (o != q && (!kScenes || b.scene[o] == qs) && (!b.groups || groups_interact(qg, b.groups[o])))
+            leaf(o, ((c >> 30) & 1u) != 0);
         } else if (sp < kStack) {
           stack[(sp++) * kTraverseBlock] = int(c);
         } else {

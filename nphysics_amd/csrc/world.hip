@@ -117,9 +117,11 @@ struct World {
   // Optional per-body attributes (damping and clamps, flags, material ids, surface velocities);
   // each is passed to the kernels only when some body differs from the default.
   std::vector<float4> h_damp, h_surf;
+  std::vector<uint4> h_groups;
   std::vector<int> h_flags, h_matid;
   bool any_damp = false, any_flags = false, any_mask = false, any_matid = false, any_surf = false;
-  DBuf damp, bflags, matid, surf;
+  bool any_groups = false;
+  DBuf damp, bflags, matid, surf, groups;
   // External forces accumulated for the next step (cleared after it), the angular velocity of the
   // last dynamics update, and the rotations from before a set_state of the current interval.
   DBuf f_lin, f_ang, w_dyn, rot_dyn, af_tmp;
@@ -344,6 +346,7 @@ struct World {
     b.ccd = any_ccd ? ccd_flags.as<int>() : nullptr;
     b.ccd_key = any_ccd ? ccd_keys.as<int>() : nullptr;
     b.ccd_rot = any_ccd ? ccd_rot.as<float4>() : nullptr;
+    b.groups = any_groups ? groups.as<uint4>() : nullptr;
     return b;
   }
 
@@ -607,6 +610,14 @@ struct World {
                             : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       h_surf.push_back(sv);
       any_surf = any_surf || sv.x != 0.0f || sv.y != 0.0f || sv.z != 0.0f;
+      // ColliderDesc::collision_groups: groups 0-29 (CollisionGroups::new() without the array)
+      constexpr uint32_t kAllGroups = (1u << 30) - 1u;
+      const uint4 cg = d.collision_groups
+                           ? make_uint4(d.collision_groups[3 * k] & kAllGroups, d.collision_groups[3 * k + 1] & kAllGroups,
+                                        d.collision_groups[3 * k + 2] & kAllGroups, 0u)
+                           : make_uint4(kAllGroups, kAllGroups, 0u, 0u);
+      h_groups.push_back(cg);
+      any_groups = any_groups || cg.x != kAllGroups || cg.y != kAllGroups || cg.z != 0u;
       // ColliderDesc::ccd_enabled, RigidBody::enable_linear_motion_interpolation, and the static body
       // a static collider belongs to
       const int cf = ((d.ccd_enabled && d.ccd_enabled[k]) ? 1 : 0) |
@@ -648,7 +659,7 @@ struct World {
     HIPCK(sol.ensure(size_t(n1) * kSolverBlock * f4b, true, stream));
     HIPCK(act.ensure(size_t(n1) * sizeof(float2), true, stream));
     for (DBuf* b : {&w_dyn, &f_lin, &f_ang}) HIPCK(b->ensure(size_t(n1) * f4b, true, stream));
-    for (DBuf* b : {&pos0, &rot0, &ext_lin, &ext_ang, &dims, &local_i, &damp, &surf, &rot_dyn})
+    for (DBuf* b : {&pos0, &rot0, &ext_lin, &ext_ang, &dims, &local_i, &damp, &surf, &rot_dyn, &groups})
       HIPCK(b->ensure(size_t(n1) * f4b));
     for (DBuf* b : {&status, &shape, &scene, &margin, &qlim, &friction, &restitution, &mass, &bflags, &matid,
                     &ccd_flags, &ccd_keys})
@@ -674,6 +685,7 @@ struct World {
     HIPCK(up(restitution, h_restitution.data(), size_t(n1) * 4, 0));
     HIPCK(up(damp, h_damp.data(), size_t(n1) * f4b, 0));
     HIPCK(up(surf, h_surf.data(), size_t(n1) * f4b, 0));
+    HIPCK(up(groups, h_groups.data(), size_t(n1) * f4b, 0));
     HIPCK(up(bflags, h_flags.data(), size_t(n1) * 4, 0));
     HIPCK(up(matid, h_matid.data(), size_t(n1) * 4, 0));
     HIPCK(up(ccd_flags, h_ccd.data(), size_t(n1) * 4, 0));

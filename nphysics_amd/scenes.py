@@ -239,6 +239,35 @@ def ccd_room(num=4, rad=0.1, half_width=3.0, thickness=0.2, height=1.5, velocity
     return concat(walls, b)
 
 
+def collision_groups_example(num=8, layers=4, rad=0.1):
+    """The reference's collision groups example (examples3d/collision_groups3.rs:18-132): one
+    static body (static_body 0) with a main floor of half extents (3, 0.2, 3) whose top is at y = 0
+    (default groups: it collides with everything), a green floor (1, 0.1, 1) at y = 1 in group 0
+    whitelisting group 0, and a blue floor at y = 2 in group 1 whitelisting group 1; above them
+    `layers` layers of num x num cuboids of half extent `rad` on a lattice of spacing 2 (rad +
+    margin) from y = 2.5, the even layers green and the odd ones blue. Green and blue bodies never
+    touch: the green layers fall through the blue floor onto the green one."""
+    from nphysics_amd.abi import collision_groups
+    green = collision_groups(membership=[0], whitelist=[0])
+    blue = collision_groups(membership=[1], whitelist=[1])
+    floors = statics([(3.0, 0.2, 3.0), (1.0, 0.1, 1.0), (1.0, 0.1, 1.0)], [(0.0, -0.2, 0.0), (0.0, 1.0, 0.0),
+                                                                          (0.0, 2.0, 0.0)])
+    floors["static_body"] = np.zeros(3, np.int32)
+    floors["collision_groups"] = np.array([collision_groups(), green, blue], np.uint32)
+    n = layers * num * num
+    b = _empty(n)
+    b["shape"][:] = 1
+    b["dims"][:] = (rad, rad, rad)
+    shift = (rad + DEFAULT_MARGIN) * 2.0
+    c = shift * num / 2.0
+    k, i, j = np.meshgrid(np.arange(layers), np.arange(num), np.arange(num), indexing="ij")
+    b["pos"][:] = np.stack([i.ravel() * shift - c, k.ravel() * shift + 2.5, j.ravel() * shift - c], 1)
+    b["static_body"] = np.full(n, -1, np.int32)
+    b["collision_groups"] = np.where((k.ravel() % 2 == 0)[:, None], np.array(green, np.uint32),
+                                     np.array(blue, np.uint32)).astype(np.uint32)
+    return concat(floors, b)
+
+
 def num_static(scene):
     return int((scene["status"] == 1).sum())
 

@@ -98,6 +98,12 @@ int add_bodies_t(orc::World<T>& w, const nphys_body_desc* d, int32_t* first) {
     if (d->ccd_enabled) ex.ccd = d->ccd_enabled[i] != 0;
     if (d->linear_motion_interpolation) ex.lin_interp = d->linear_motion_interpolation[i] != 0;
     if (d->static_body) ex.static_body = d->static_body[i] < 0 ? -1 : d->static_body[i];
+    if (d->collision_groups) {
+      const uint32_t all = (1u << 30) - 1u;  // groups 0-29
+      ex.g_member = d->collision_groups[3 * i] & all;
+      ex.g_white = d->collision_groups[3 * i + 1] & all;
+      ex.g_black = d->collision_groups[3 * i + 2] & all;
+    }
     if (ex.fric_mode < 0 || ex.fric_mode > 3 || ex.rest_mode < 0 || ex.rest_mode > 3)
       return fail(NPHYS_ERR_INVALID, "bad combine mode");
     w.add_body(d->status[i], d->shape[i], v3_of<T>(d->dims + 3 * i),

@@ -100,6 +100,8 @@ struct Body {
   // for a static collider that shares a static body with others, the first of them)
   bool ccd, lin_interp;
   int ccd_key;
+  // ColliderDesc::collision_groups (collider.rs:444): membership, whitelist, blacklist masks
+  uint32_t g_member, g_white, g_black;
   AABB<T> loose;
   T energy;     // ActivationStatus::energy (0: asleep)
   T sleep_thr;  // ActivationStatus::threshold, < 0 for None
@@ -309,6 +311,8 @@ struct BodyExtras {
   V3<T> surf_vel{T(0), T(0), T(0)};
   bool ccd = false, lin_interp = false;
   int static_body = -1;  // static colliders with one index >= 0 (per scene) belong to one body
+  // CollisionGroups::new(): member of groups 0-29, whitelisting them all, blacklisting none
+  uint32_t g_member = (1u << 30) - 1u, g_white = (1u << 30) - 1u, g_black = 0u;
 };
 
 template <class T>
@@ -419,6 +423,9 @@ class World {
     b.lin_interp = status == STATUS_DYNAMIC && ex.lin_interp;
     if (ex.ccd) any_ccd = true;
     b.ccd_key = int(bodies.size());
+    b.g_member = ex.g_member;
+    b.g_white = ex.g_white;
+    b.g_black = ex.g_black;
     if (status != STATUS_DYNAMIC && ex.static_body >= 0) {
       const auto key = std::make_pair(scene, ex.static_body);
       auto it = static_owner.find(key);
@@ -586,9 +593,16 @@ class World {
   }
 
   // DefaultCollisionFilter::is_pair_valid (geometrical_world.rs:933-971) for single-collider
-  // bodies: at least one side has dofs; plus the scene rule (independent worlds in one batch).
+  // bodies: at least one side has dofs; the collision groups ncollide's pipeline checks before it
+  // (CollisionGroups::can_interact_with_groups, ncollide3d 0.31, unvendored: each collider is a
+  // member of some group the other whitelists and of no group the other blacklists); plus the
+  // scene rule (independent worlds in one batch).
+  static bool groups_interact(const Body<T>& a, const Body<T>& b) {
+    return (a.g_member & b.g_black) == 0u && (b.g_member & a.g_black) == 0u && (a.g_member & b.g_white) != 0u &&
+           (b.g_member & a.g_white) != 0u;
+  }
   static bool pair_allowed(const Body<T>& a, const Body<T>& b) {
-    return (a.status == STATUS_DYNAMIC || b.status == STATUS_DYNAMIC) && a.scene == b.scene;
+    return (a.status == STATUS_DYNAMIC || b.status == STATUS_DYNAMIC) && a.scene == b.scene && groups_interact(a, b);
   }
 
   // canonical pair: first is dynamic; second is static or a higher index

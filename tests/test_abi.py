@@ -44,6 +44,17 @@ def test_params_struct_layout(nphys):
     assert C.sizeof(abi.Counters) == 8 * 8 + 8 * 7 + 8 + 8 + 8 + 8 + 8 * 11 + 8 * 3
 
 
+def test_body_desc_fields_follow_the_header():
+    """abi.BodyDesc lists nphys_body_desc's members in the header's order."""
+    import re
+
+    text = open(os.path.join(os.path.dirname(__file__), "..", "include", "nphys.h")).read()
+    body = text[text.index("typedef struct nphys_body_desc {"):text.index("} nphys_body_desc;")]
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+    names = re.findall(r"(\w+)\s*;", body)
+    assert names == [f for f, _ in abi.BodyDesc._fields_]
+
+
 def test_no_device_fails_loudly(nphys):
     import torch
 

@@ -58,7 +58,7 @@ def merge(*parts):
                 "max_angular_velocity": np.finfo(np.float32).max, "gravity_enabled": 1, "kinematic_dofs": 0,
                 "friction_combine": 0, "restitution_combine": 0, "material_id": -1,
                 "surface_velocity": (0.0, 0.0, 0.0), "ccd_enabled": 0, "linear_motion_interpolation": 0,
-                "static_body": -1}
+                "static_body": -1, "collision_groups": (abi.ALL_GROUPS, abi.ALL_GROUPS, 0)}
     filled = []
     for p in parts:
         q = dict(p)
