@@ -100,6 +100,16 @@ def test_reference_ccd_example_bitexact(nphys, oracle_mod):
     assert g.counters()["ccd_substeps"] > 0
 
 
+def test_ccd_room_4096_cascade_bitexact(nphys, oracle_mod):
+    """The 4,096-cuboid room at full size: every body freezes in each step's cascade (about 3,000
+    impacts per step), and the first pass leaves about 24,000 queue entries, so the queue runs with
+    its deeper heap levels in HBM."""
+    s = scenes.ccd_room(num=16, half_width=4.0, height=-1.76)
+    g, o = _pair(nphys, oracle_mod, s)
+    _run(g, o, 4, "ccd room 4096")
+    assert g.counters()["ccd_impacts"] > 5000
+
+
 def test_ccd_room_balls_and_rotated_boxes_bitexact(nphys, oracle_mod):
     s = scenes.ccd_room(num=6, rad=0.12, seed=11, velocity=(-60.0, -25.0, 40.0))
     g, o = _pair(nphys, oracle_mod, s)
