@@ -19,6 +19,10 @@
 //   (add_local_inertia_and_com); mass properties src/volumetric/volumetric_ball.rs:9-44,
 //   src/volumetric/volumetric_cuboid.rs:10-75; material combine src/material/material.rs:72-178.
 //
+// Collision groups in the pair rule restate ncollide3d 0.31's CollisionGroups (unvendored): pinned
+// by a truth table and the reference's collision groups example; parity with ncollide's own code is
+// unpinned.
+//
 // Two row orders are provided for the velocity and position sweeps:
 //   ORDER_REFERENCE -- the reference's: friction rows (non-ground, then ground), then normal rows
 //                      (non-ground, then ground), manifolds in pair order (sor_prox.rs:159-179);
