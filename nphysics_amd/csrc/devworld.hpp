@@ -38,7 +38,7 @@ constexpr int kSolverBlock = 4;
 
 // CollisionGroups::can_interact_with_groups (ncollide3d 0.31, unvendored): each collider is a
 // member of some group the other whitelists and of no group the other blacklists.
-NPH_HD inline bool groups_interact(uint4 a, uint4 c) {
+NPH_HD bool groups_interact(uint4 a, uint4 c) {
   return (a.x & c.z) == 0u && (c.x & a.z) == 0u && (a.x & c.y) != 0u && (c.x & a.y) != 0u;
 }
 
