@@ -266,6 +266,12 @@ int nphys_set_state(nphys_world* w, const float* pos, const float* rot, const fl
                     const float* angvel);
 int nphys_get_state(nphys_world* w, float* pos, float* rot, float* linvel, float* angvel);
 
+/* Collider::set_collision_groups (collider.rs:386-388) for `count` bodies: groups holds 3 masks
+ * (membership, whitelist, blacklist) per entry, as nphys_body_desc.collision_groups. The pair set
+ * is rebuilt by the collision detection at the start of the next step (or by nphys_collide), as
+ * ncollide re-filters the collider's pairs at its next broad phase. */
+int nphys_set_collision_groups(nphys_world* w, int32_t count, const int32_t* bodies, const uint32_t* groups);
+
 /* Body::apply_force (rigid_body.rs:771-822) for `count` entries: body handle, world-frame force
  * (fx, fy, fz, torque x, y, z) and the ForceType `type` (NPHYS_FORCE ...). Forces and acceleration
  * changes accumulate into the body's external forces, applied by the next step and then cleared

@@ -151,6 +151,7 @@ struct World {
   int npairs = 0;
   bool pairs_dirty = true;
   bool moved = true;
+  bool recollide = false;  // collision groups changed: the next step starts with collision detection
 
   // narrow phase
   DBuf np_count, np_coff, np_has, np_moff, np_perm, np_tmp;
@@ -803,6 +804,23 @@ struct World {
     HIPCK(hipStreamSynchronize(stream));
     moved = true;
     wake_pending = sleeping;
+    return NPHYS_OK;
+  }
+
+  int set_collision_groups(int32_t count, const int32_t* bodies_, const uint32_t* g) {
+    if (count < 0 || (count > 0 && (!bodies_ || !g))) return set_err(NPHYS_ERR_INVALID, "bad collision group arrays");
+    for (int k = 0; k < count; ++k)
+      if (bodies_[k] < 0 || bodies_[k] >= n) return set_err(NPHYS_ERR_INVALID, "bad body handle");
+    if (count == 0) return NPHYS_OK;
+    constexpr uint32_t kAllGroups = (1u << 30) - 1u;
+    for (int k = 0; k < count; ++k)
+      h_groups[size_t(bodies_[k])] = make_uint4(g[3 * k] & kAllGroups, g[3 * k + 1] & kAllGroups, g[3 * k + 2] & kAllGroups, 0u);
+    any_groups = false;
+    for (const uint4& cg : h_groups) any_groups = any_groups || cg.x != kAllGroups || cg.y != kAllGroups || cg.z != 0u;
+    HIPCK(hipMemcpyAsync(groups.p, h_groups.data(), size_t(n) * sizeof(uint4), hipMemcpyHostToDevice, stream));
+    HIPCK(hipStreamSynchronize(stream));
+    pairs_dirty = true;
+    recollide = true;
     return NPHYS_OK;
   }
 
@@ -1914,9 +1932,10 @@ struct World {
     SimParams spp = sp;
     spp.dyn_dt = moved ? sp.dt : dyn_dt_next;
     dyn_dt_next = sp.dt;
-    if (moved) {
+    if (moved || recollide) {
       if ((rc = collide_now(0))) return rc;
       moved = false;
+      recollide = false;
     }
     HIPCK(hipEventRecord(ev[EV_PRE_CD], stream));
     if (wake_pending) {
@@ -2199,6 +2218,11 @@ int nphys_set_state(nphys_world* w, const float* pos, const float* rot, const fl
   return w->w.set_state(pos, rot, linvel, angvel);
 }
 
+int nphys_set_collision_groups(nphys_world* w, int32_t count, const int32_t* bodies, const uint32_t* groups) {
+  if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
+  return w->w.set_collision_groups(count, bodies, groups);
+}
+
 int nphys_get_state(nphys_world* w, float* pos, float* rot, float* linvel, float* angvel) {
   if (!w) return set_err(NPHYS_ERR_INVALID, "null world");
   return w->w.get_state(pos, rot, linvel, angvel);
@@ -2239,6 +2263,7 @@ int nphys_collide(nphys_world* w) {
   const int rc = w->w.collide_now(0);
   if (rc) return rc;
   w->w.moved = false;
+  w->w.recollide = false;
   if (hipStreamSynchronize(w->w.stream) != hipSuccess) return set_err(NPHYS_ERR_HIP, "sync failed");
   return NPHYS_OK;
 }

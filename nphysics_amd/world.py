@@ -21,7 +21,7 @@ _lib = None
 
 EXPORTED = [
     "nphys_params_default", "nphys_world_create", "nphys_world_destroy", "nphys_set_params",
-    "nphys_get_params", "nphys_add_bodies", "nphys_num_bodies", "nphys_set_state",
+    "nphys_get_params", "nphys_add_bodies", "nphys_num_bodies", "nphys_set_state", "nphys_set_collision_groups",
     "nphys_get_state", "nphys_step", "nphys_collide", "nphys_contact_model_set",
     "nphys_num_pairs", "nphys_read_pairs", "nphys_num_contacts", "nphys_read_contacts",
     "nphys_num_manifolds", "nphys_read_manifolds", "nphys_num_solved_contacts", "nphys_read_impulses", "nphys_residual", "nphys_read_counters",
@@ -61,6 +61,7 @@ def load_library(path=LIB_PATH):
     L.nphys_add_bodies.argtypes = [W, C.POINTER(abi.BodyDesc), ip]
     L.nphys_num_bodies.argtypes = [W, ip]
     L.nphys_set_state.argtypes = [W, fp, fp, fp, fp]
+    L.nphys_set_collision_groups.argtypes = [W, C.c_int32, ip, C.POINTER(C.c_uint32)]
     L.nphys_get_state.argtypes = [W, fp, fp, fp, fp]
     L.nphys_step.argtypes = [W, C.c_int32]
     L.nphys_collide.argtypes = [W]
@@ -165,6 +166,13 @@ class MechanicalWorld:
         n = C.c_int32()
         _check(self._L.nphys_num_bodies(self._h, C.byref(n)))
         return n.value
+
+    def set_collision_groups(self, bodies, groups):
+        """Collider::set_collision_groups for the given bodies; groups: (count, 3) masks."""
+        b = np.ascontiguousarray(bodies, np.int32).reshape(-1)
+        g = np.ascontiguousarray(groups, np.uint32).reshape(-1, 3)
+        _check(self._L.nphys_set_collision_groups(self._h, int(b.shape[0]), abi.iptr(b),
+                                                  g.ctypes.data_as(C.POINTER(C.c_uint32))))
 
     def set_state(self, pos=None, rot=None, linvel=None, angvel=None):
         arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (pos, rot, linvel, angvel)]

@@ -54,6 +54,7 @@ def lib():
         fp = C.POINTER(C.c_float)
         dp = C.POINTER(C.c_double)
         L.orc_set_state.argtypes = [vp, fp, fp, fp, fp]
+        L.orc_set_collision_groups.argtypes = [vp, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_uint32)]
         L.orc_set_state_f64.argtypes = [vp, dp, dp, dp, dp]
         L.orc_get_state.argtypes = [vp, dp, dp, dp, dp]
         L.orc_step.argtypes = [vp, C.c_int32]
@@ -151,6 +152,12 @@ class OracleWorld:
                                        ang.ctypes.data_as(C.POINTER(C.c_double)),
                                        br.ctypes.data_as(C.POINTER(C.c_int32)), max(n, 1)))
         return {"lin": lin[:n], "ang": ang[:n], "broken": br[:n].astype(bool)}
+
+    def set_collision_groups(self, bodies, groups):
+        b = np.ascontiguousarray(bodies, np.int32).reshape(-1)
+        g = np.ascontiguousarray(groups, np.uint32).reshape(-1, 3)
+        _check(self._L.orc_set_collision_groups(self._h, int(b.shape[0]), abi.iptr(b),
+                                                g.ctypes.data_as(C.POINTER(C.c_uint32))))
 
     def set_state(self, pos=None, rot=None, linvel=None, angvel=None):
         if all(a is None or np.asarray(a).dtype == np.float64 for a in (pos, rot, linvel, angvel)):

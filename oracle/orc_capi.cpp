@@ -339,6 +339,17 @@ int orc_num_bodies(void* h, int32_t* n) {
   return with_world(h, [&](auto& w) { *n = int32_t(w.bodies.size()); return NPHYS_OK; });
 }
 
+int orc_set_collision_groups(void* h, int32_t count, const int32_t* bodies, const uint32_t* groups) {
+  return with_world(h, [&](auto& w) {
+    const uint32_t all = (1u << 30) - 1u;  // groups 0-29
+    for (int k = 0; k < count; ++k) {
+      if (bodies[k] < 0 || size_t(bodies[k]) >= w.bodies.size()) return fail(NPHYS_ERR_INVALID, "bad body handle");
+      w.set_collision_groups(bodies[k], groups[3 * k] & all, groups[3 * k + 1] & all, groups[3 * k + 2] & all);
+    }
+    return int(NPHYS_OK);
+  });
+}
+
 int orc_set_state(void* h, const float* pos, const float* rot, const float* lv, const float* av) {
   return with_world(h, [&](auto& w) { return set_state_t(w, pos, rot, lv, av); });
 }

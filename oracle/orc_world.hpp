@@ -477,6 +477,17 @@ class World {
     return int(bodies.size()) - 1;
   }
 
+  // Collider::set_collision_groups (collider.rs:386-388): the collider's pairs are re-filtered by
+  // the collision detection at the start of the next step
+  void set_collision_groups(int i, uint32_t member, uint32_t white, uint32_t black) {
+    Body<T>& b = bodies[i];
+    b.g_member = member;
+    b.g_white = white;
+    b.g_black = black;
+    pairs_dirty = true;
+    groups_changed = true;
+  }
+
   void set_state(int i, Iso<T> pos, V3<T> lv, V3<T> av) {
     Body<T>& b = bodies[i];
     b.pos = pos;
@@ -575,6 +586,7 @@ class World {
   // only when it no longer contains tight. Pairs = overlapping loose AABBs that pass
   // DefaultCollisionFilter (geometrical_world.rs:933-971): not the same body, not both static.
   bool pairs_dirty = true;
+  bool groups_changed = false;
 
   void sync_and_broad_phase(bool swept) {
     const double t0 = now_s();
@@ -1846,7 +1858,9 @@ class World {
       if (b.flags & F_POSITION) any_moved = true;
     }
     for (auto& b : bodies) update_acceleration(b);
-    if (any_moved) collide();  // pre-solve CD: only when bodies were moved outside the solver
+    // pre-solve CD: only when bodies were moved outside the solver or collision groups changed
+    if (any_moved || groups_changed) collide();
+    groups_changed = false;
     solver_step();
     for (auto& b : bodies) update_dynamics(b);  // post-solve update_dynamics (:343-346)
     // CCD phase: sync with (position0 -> position) and broad phase over swept AABBs (:604-606),

@@ -69,3 +69,25 @@ def test_groups_example_bodies_rest_on_their_floors(oracle_mod):
     assert np.all(y[green] > 1.1 + h - 0.005) and np.all(y[green] < 1.1 + 3 * h + 2 * 0.02 + 0.01)
     assert np.all(y[~green] > 2.1 + h - 0.005) and np.all(y[~green] < 2.1 + 3 * h + 2 * 0.02 + 0.01)
     assert np.abs(o.get_state()["linvel"][ns:]).max() < 0.05
+
+
+def test_changed_groups_refilter_the_pairs(oracle_mod):
+    """Collider::set_collision_groups between steps: a ball resting on a floor falls freely from
+    the next step on once the floor stops accepting its group (v = -g t exactly), and lands again
+    on a lower floor that accepts it."""
+    s = merge(body(status=1, shape=1, dims=(2.0, 0.1, 2.0), pos=(0.0, -0.1, 0.0)),
+              body(status=1, shape=1, dims=(2.0, 0.1, 2.0), pos=(0.0, -3.1, 0.0)),
+              body(shape=0, dims=(0.25, 0, 0), pos=(0.0, 0.25, 0.0), collision_groups=cg([1])))
+    o = oracle_mod.OracleWorld(abi.default_params(), precision=64)
+    o.add_bodies(s)
+    o.step(30)
+    assert abs(o.get_state()["linvel"][2, 1]) < 1e-6
+    o.set_collision_groups([0], [cg([0], [0])])  # the upper floor now accepts group 0 only
+    dt = 1.0 / 60.0
+    for k in range(1, 11):
+        o.step(1)
+        assert len(o.pairs()) == 0  # the lower floor is still out of reach
+        np.testing.assert_allclose(o.get_state()["linvel"][2, 1], -9.81 * dt * k, rtol=1e-12)
+    o.step(80)
+    y = o.get_state()["pos"][2, 1]
+    assert abs(y - (-3.0 + 0.25)) < 0.03  # resting on the lower floor
