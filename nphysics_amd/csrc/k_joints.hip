@@ -1,8 +1,10 @@
 // nphysics_amd/csrc/k_joints.hip -- joint constraints in the coloured solver.
 //
 // Restates the JointConstraint implementations BallConstraint (src/joint/ball_constraint.rs:64-178),
-// FixedConstraint (src/joint/fixed_constraint.rs:91-249) and RevoluteConstraint without limits or
-// motor (src/joint/revolute_constraint.rs:172-349), with their solver helpers
+// FixedConstraint (src/joint/fixed_constraint.rs:91-249), RevoluteConstraint without limits or
+// motor (src/joint/revolute_constraint.rs:172-349) and PrismaticConstraint without limits
+// (src/joint/prismatic_constraint.rs:132-345: two linear rows orthogonal to the axis, then three
+// angular rows; rotation cancelled, then anchor 2 projected onto the axis), with their solver helpers
 // (src/solver/helper.rs:53-135, 167-296, 364-420, 422-612, 614-769), inside the per-step solve:
 //   k_joint_assemble   velocity_constraints of every active joint (moreau_jean_solver.rs:211-223):
 //                      bilateral rows with unbounded (Independent) limits, linear along x, y, z at
@@ -134,6 +136,14 @@ __global__ void __launch_bounds__(256) k_joint_assemble(DevBodies b, SimParams p
     dirs[3] = dirs[0];
     dirs[4] = dirs[1];
     dirs[5] = dirs[2];
+  } else if (kind == JOINT_PRISMATIC) {
+    // restrict_relative_linear_velocity_to_axis (helper.rs:788-856): the orthonormal_subspace_basis
+    // of the world axis, then cancel_relative_angular_velocity: x, y, z
+    const f3 axis = qrot(toq(b.rot(b1)), xyz(j.f1[ji]));
+    tangents_of(axis, dirs[0], dirs[1]);
+    dirs[2] = f3{1.0f, 0.0f, 0.0f};
+    dirs[3] = f3{0.0f, 1.0f, 0.0f};
+    dirs[4] = f3{0.0f, 0.0f, 1.0f};
   } else if (kind == JOINT_REVOLUTE) {
     // axis1 = position_at_material_point(anchor1) * axis1 (revolute_constraint.rs:230), and the
     // orthonormal_subspace_basis of it in that order
@@ -141,10 +151,17 @@ __global__ void __launch_bounds__(256) k_joint_assemble(DevBodies b, SimParams p
     tangents_of(axis, dirs[3], dirs[4]);
   }
   const float4 il = j.imp_lin[ji], ia = j.imp_ang[ji];
-  const float imp[6] = {il.x, il.y, il.z, ia.x, ia.y, ia.z};
+  const int nlin = joint_linear_rows(kind);
+  float imp[6] = {il.x, il.y, il.z, ia.x, ia.y, ia.z};
+  if (nlin == 2) {
+    imp[2] = ia.x;
+    imp[3] = ia.y;
+    imp[4] = ia.z;
+    imp[5] = 0.0f;
+  }
   float rhs_all[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   for (int k = 0; k < nrows; ++k) {
-    const bool angular = k >= 3;
+    const bool angular = k >= nlin;
     float inv_r = 0.0f, rhs = 0.0f;
     if (d1) jfill(v1, w1, dirs[k], angular, inv_r, rhs);
     if (d2) jfill(v2, w2, neg(dirs[k]), angular, inv_r, rhs);
@@ -165,7 +182,7 @@ __global__ void __launch_bounds__(256) k_joint_assemble(DevBodies b, SimParams p
 
 // One joint's rows in registers for the velocity phases.
 struct JointCtx {
-  int b1, b2, nrows;
+  int b1, b2, nrows, nlin;
   bool active, d1, d2;
   f3 p1, p2;
   f3 dir[6];
@@ -185,6 +202,7 @@ __device__ __forceinline__ void load_ctx(const DevBodies& b, const JointBufs& j,
   c.d1 = (fl >> 1) & 1;
   c.d2 = (fl >> 2) & 1;
   c.nrows = __float_as_int(r0.w) & 0xff;
+  c.nlin = joint_linear_rows(__float_as_int(r0.w) >> 8);
   const int4 inf = j.info[j.order[s]];
   c.b1 = inf.y;
   c.b2 = inf.z;
@@ -237,7 +255,7 @@ __device__ __forceinline__ JRow jrow(const JointCtx& c, int k) {
   const f3 zero{0.0f, 0.0f, 0.0f};
   const f3 d = c.dir[k];
   const f3 nd = neg(d);
-  const bool angular = k >= 3;
+  const bool angular = k >= c.nlin;
   v.d1 = cmul(angular ? zero : d, c.ml1);
   v.d2 = cmul(angular ? zero : nd, c.ml2);
   v.a1 = cmul(angular ? d : cross(c.p1, d), c.ma1);
@@ -376,6 +394,17 @@ __device__ __forceinline__ void joint_position_one(const DevBodies& b, const Sim
     generic_solve(p, true, q_scaled_axis(qmul(r2, qconj(r1))), d1, d2, w1, w2, t1, q1, t2, q2, I1, im1, I2, im2,
                   ml1, ma1, ml2, ma2);
     translation();
+  } else if (kind == JOINT_PRISMATIC) {
+    // cancel_relative_rotation of the bodies' rotations (helper.rs:553-612), then
+    // project_anchor_to_axis (helper.rs:858-912): anchor 2 onto the line through anchor 1 along the axis
+    f3 w1, w2;
+    anchors(w1, w2);
+    generic_solve(p, true, q_scaled_axis(qmul(q2, qconj(q1))), d1, d2, w1, w2, t1, q1, t2, q2, I1, im1, I2, im2, ml1,
+                  ma1, ml2, ma2);
+    anchors(w1, w2);
+    const f3 axis = qrot(q1, xyz(j.f1[ji]));
+    const f3 proj = add(w1, scale(axis, dot(axis, sub(w2, w1))));
+    generic_solve(p, false, sub(w2, proj), d1, d2, w1, w2, t1, q1, t2, q2, I1, im1, I2, im2, ml1, ma1, ml2, ma2);
   } else if (kind == JOINT_REVOLUTE) {
     translation();
     f3 w1, w2;
@@ -430,9 +459,12 @@ __global__ void __launch_bounds__(256) k_joint_cache(DevBodies b, SimParams p, J
   const int ji = j.order[s];
   const int nrows = __float_as_int(R[0].w) & 0xff;
   const float4 l0 = j.lam[2 * long(s)], l1 = j.lam[2 * long(s) + 1];
-  const f3 lin{l0.x, l0.y, l0.z};
-  const f3 ang = nrows == 6 ? f3{l0.w, l1.x, l1.y} : (nrows == 5 ? f3{l0.w, l1.x, 0.0f} : f3{0.0f, 0.0f, 0.0f});
-  // the revolute joint keeps a third angular impulse of zero; the ball joint has none
+  const bool prismatic = (__float_as_int(R[0].w) >> 8) == JOINT_PRISMATIC;
+  const f3 lin = prismatic ? f3{l0.x, l0.y, 0.0f} : f3{l0.x, l0.y, l0.z};
+  const f3 ang = prismatic ? f3{l0.z, l0.w, l1.x}
+                           : (nrows == 6 ? f3{l0.w, l1.x, l1.y} : (nrows == 5 ? f3{l0.w, l1.x, 0.0f} : f3{0.0f, 0.0f, 0.0f}));
+  // the revolute joint keeps a third angular impulse of zero, the prismatic joint a third linear
+  // one; the ball joint has no angular impulse
   j.imp_lin[ji] = f4(lin, 0.0f);
   j.imp_ang[ji] = f4(ang, 0.0f);
   const float inv_dt2 = p.inv_dt * p.inv_dt;

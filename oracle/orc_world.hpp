@@ -143,7 +143,7 @@ struct ContactRows {
 
 // A joint constraint (src/joint/): BallConstraint (ball_constraint.rs), FixedConstraint
 // (fixed_constraint.rs) or RevoluteConstraint without limits or motor (revolute_constraint.rs).
-enum { JOINT_BALL = 0, JOINT_FIXED = 1, JOINT_REVOLUTE = 2 };
+enum { JOINT_BALL = 0, JOINT_FIXED = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3 };
 template <class T>
 struct Joint {
   int kind, b1, b2;
@@ -951,6 +951,19 @@ class World {
       jr.row[jr.n++] = int(list.size()) - 1;
     };
     const V3<T> e[3] = {V3<T>{T(1), T(0), T(0)}, V3<T>{T(0), T(1), T(0)}, V3<T>{T(0), T(0), T(1)}};
+    if (jt.kind == JOINT_PRISMATIC) {
+      // PrismaticConstraint::velocity_constraints (prismatic_constraint.rs:132-206):
+      // restrict_relative_linear_velocity_to_axis (helper.rs:788-856), the two directions of
+      // orthonormal_subspace_basis(axis), axis = position_at_material_point(anchor1) * axis1; then
+      // cancel_relative_angular_velocity (helper.rs:422-473)
+      const V3<T> axis = qrot(b1.pos.q, jt.ax1);
+      V3<T> t[2];
+      tangents(axis, t[0], t[1]);
+      for (int i = 0; i < 2; ++i) push(t[i], false, comp(jt.lin_imp, i));
+      for (int i = 0; i < 3; ++i) push(e[i], true, comp(jt.ang_imp, i));
+      jrow.push_back(jr);
+      return;
+    }
     for (int i = 0; i < 3; ++i) push(e[i], false, comp(jt.lin_imp, i));
     if (jt.kind == JOINT_FIXED) {
       for (int i = 0; i < 3; ++i) push(e[i], true, comp(jt.ang_imp, i));
@@ -980,10 +993,11 @@ class World {
     const T inv_dt2 = prm.inv_dt * prm.inv_dt;
     for (const JointRows& jr : jrow) {
       Joint<T>& jt = joints[size_t(jr.joint)];
+      const int nlin = jt.kind == JOINT_PRISMATIC ? 2 : 3;  // linear rows first
       for (int k = 0; k < jr.n; ++k) {
         const T v = jrow_at(jr, k).impulse;
-        if (k < 3) set_comp(jt.lin_imp, k, v);
-        else set_comp(jt.ang_imp, k - 3, v);
+        if (k < nlin) set_comp(jt.lin_imp, k, v);
+        else set_comp(jt.ang_imp, k - nlin, v);
       }
       if (norm2(jt.lin_imp) * inv_dt2 > jt.break_force_sq || norm2(jt.ang_imp) * inv_dt2 > jt.break_torque_sq)
         jt.broken = true;
@@ -1041,6 +1055,17 @@ class World {
       joint_frames(jt, w1, w2, r1, r2);
       solve_joint_generic(jt, w1, w2, q_scaled_axis(qmul(r2, qconj(r1))), true);
       translation();
+    } else if (jt.kind == JOINT_PRISMATIC) {
+      // PrismaticConstraint::position_constraint (prismatic_constraint.rs:279-345): i = 0
+      // cancel_relative_rotation of the bodies' rotations, i = 1 project_anchor_to_axis
+      // (helper.rs:858-912): error = anchor2 - (anchor1 + axis (axis . (anchor2 - anchor1)))
+      joint_frames(jt, w1, w2, r1, r2);
+      const Q<T> q1 = bodies[size_t(jt.b1)].pos.q, q2 = bodies[size_t(jt.b2)].pos.q;
+      solve_joint_generic(jt, w1, w2, q_scaled_axis(qmul(q2, qconj(q1))), true);
+      joint_frames(jt, w1, w2, r1, r2);
+      const V3<T> axis = qrot(bodies[size_t(jt.b1)].pos.q, jt.ax1);
+      const V3<T> proj = w1 + axis * dot(axis, w2 - w1);
+      solve_joint_generic(jt, w1, w2, w2 - proj, false);
     } else if (jt.kind == JOINT_REVOLUTE) {
       translation();
       // align_axis (helper.rs:701-769): the scaled axis of the rotation taking axis1 to axis2

@@ -252,3 +252,30 @@ def test_removed_joint_is_ignored(oracle_mod):
     o.remove_joint(0)
     o.step(10)
     assert o.get_state()["linvel"][1, 1] == pytest.approx(-10 * G * DT, rel=1e-9)
+
+
+def test_prismatic_slide_closed_form(oracle_mod):
+    """A cuboid on a frictionless inclined rail (PrismaticConstraint, prismatic_constraint.rs:132-345;
+    the anchor at its centre of mass, so the rows do not couple rotation): the two linear rows
+    orthogonal to the axis are decoupled under the isotropic mass, and each step's solve keeps
+    exactly the gravity component along the axis, v_k = k dt (g . a) a, with no rotation. Three
+    linear rows (the fixed joint's) would hold it still; an axis row instead of an orthogonal one
+    would let it fall freely."""
+    c = math.sqrt(0.5)
+    a = np.array([c, -c, 0.0])
+    s = merge(body(status=1, shape=1, dims=(0.05, 0.05, 0.05), pos=(0, -40, 0)),
+              body(shape=1, dims=(0.2, 0.3, 0.25), pos=(0.0, 0.0, 0.0)))
+    js = abi.joints(abi.JOINT_PRISMATIC, 0, 1, (0.0, 40.0, 0.0), (0.0, 0.0, 0.0), axis1=tuple(a))
+    o = oracle_mod.OracleWorld(abi.default_params(), precision=64)
+    o.add_bodies(s)
+    o.add_joints(js)
+    dt = 1.0 / 60.0
+    g = np.array([0.0, -9.81, 0.0])
+    for k in range(1, 31):
+        o.step(1)
+        st = o.get_state()
+        np.testing.assert_allclose(st["linvel"][1], k * dt * g.dot(a) * a, rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(st["angvel"][1], 0.0, atol=1e-12)
+    # the anchor stays on the rail
+    p = st["pos"][1]
+    assert np.linalg.norm(p - a * p.dot(a)) < 1e-9
