@@ -146,8 +146,16 @@ typedef struct nphys_body_desc {
  * anchors and reference frames coincide), RevoluteConstraint without limits or motor
  * (revolute_constraint.rs:14-349: anchors coincide, the bodies turn about the axis only) and
  * PrismaticConstraint without limits (prismatic_constraint.rs:14-345: the relative rotation is
- * cancelled and anchor 2 stays on the line through anchor 1 along axis1). */
-enum { NPHYS_JOINT_BALL = 0, NPHYS_JOINT_FIXED = 1, NPHYS_JOINT_REVOLUTE = 2, NPHYS_JOINT_PRISMATIC = 3 };
+ * cancelled and anchor 2 stays on the line through anchor 1 along axis1) and CylindricalConstraint
+ * without limits or motor (cylindrical_constraint.rs:14-292: the axes stay aligned and anchor 2 on
+ * the line along axis1; sliding along and turning about it are free). */
+enum {
+  NPHYS_JOINT_BALL = 0,
+  NPHYS_JOINT_FIXED = 1,
+  NPHYS_JOINT_REVOLUTE = 2,
+  NPHYS_JOINT_PRISMATIC = 3,
+  NPHYS_JOINT_CYLINDRICAL = 4
+};
 typedef struct nphys_joint_desc {
   int32_t n;
   const int32_t* kind;        /* n: NPHYS_JOINT_* */
@@ -157,8 +165,8 @@ typedef struct nphys_joint_desc {
   const float* anchor2;       /* 3n: anchor in body 2's local frame */
   const float* frame1;        /* 4n: fixed joints: ref_frame1 (i, j, k, w); NULL: identity */
   const float* frame2;        /* 4n: fixed joints: ref_frame2; NULL: identity */
-  const float* axis1;         /* 3n: revolute and prismatic joints: unit axis in body 1's frame; NULL: (0, 0, 1) */
-  const float* axis2;         /* 3n: revolute joints: unit axis in body 2's frame; NULL: (0, 0, 1) */
+  const float* axis1;         /* 3n: revolute, prismatic, cylindrical joints: unit axis in body 1's frame; NULL: (0, 0, 1) */
+  const float* axis2;         /* 3n: revolute, cylindrical joints: unit axis in body 2's frame; NULL: (0, 0, 1) */
   const float* break_force;   /* n: set_break_force; NULL or a negative value: unbreakable */
   const float* break_torque;  /* n: set_break_torque (fixed, revolute); NULL or negative: unbreakable */
 } nphys_joint_desc;

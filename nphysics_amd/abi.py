@@ -33,6 +33,7 @@ JOINT_BALL = 0
 JOINT_FIXED = 1
 JOINT_REVOLUTE = 2
 JOINT_PRISMATIC = 3
+JOINT_CYLINDRICAL = 4
 
 # optional per-body arrays of nphys_body_desc: name -> (dtype, width)
 OPTIONAL_BODY_FIELDS = {

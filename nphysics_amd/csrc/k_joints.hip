@@ -4,7 +4,9 @@
 // FixedConstraint (src/joint/fixed_constraint.rs:91-249), RevoluteConstraint without limits or
 // motor (src/joint/revolute_constraint.rs:172-349) and PrismaticConstraint without limits
 // (src/joint/prismatic_constraint.rs:132-345: two linear rows orthogonal to the axis, then three
-// angular rows; rotation cancelled, then anchor 2 projected onto the axis), with their solver helpers
+// angular rows; rotation cancelled, then anchor 2 projected onto the axis) and CylindricalConstraint
+// without limits or motor (src/joint/cylindrical_constraint.rs:125-292: two linear and two angular
+// rows orthogonal to the axis; axes aligned, then anchor 2 projected), with their solver helpers
 // (src/solver/helper.rs:53-135, 167-296, 364-420, 422-612, 614-769), inside the per-step solve:
 //   k_joint_assemble   velocity_constraints of every active joint (moreau_jean_solver.rs:211-223):
 //                      bilateral rows with unbounded (Independent) limits, linear along x, y, z at
@@ -91,7 +93,9 @@ __device__ __forceinline__ void tangents_of(f3 n, f3& t1, f3& t2) {
   t2 = a;
 }
 
-__device__ __forceinline__ int joint_rows(int kind) { return kind == JOINT_BALL ? 3 : (kind == JOINT_FIXED ? 6 : 5); }
+__device__ __forceinline__ int joint_rows(int kind) {
+  return kind == JOINT_BALL ? 3 : (kind == JOINT_FIXED ? 6 : (kind == JOINT_CYLINDRICAL ? 4 : 5));
+}
 
 __device__ __forceinline__ bool body_awake(const DevBodies& b, int i) {
   return b.status[i] == STATUS_DYNAMIC && (!b.act || b.act[i].x != 0.0f);
@@ -136,6 +140,13 @@ __global__ void __launch_bounds__(256) k_joint_assemble(DevBodies b, SimParams p
     dirs[3] = dirs[0];
     dirs[4] = dirs[1];
     dirs[5] = dirs[2];
+  } else if (kind == JOINT_CYLINDRICAL) {
+    // restrict_relative_linear_velocity_to_axis, then restrict_relative_angular_velocity_to_axis
+    // (helper.rs:614-699): both the orthonormal_subspace_basis of the world axis
+    const f3 axis = qrot(toq(b.rot(b1)), xyz(j.f1[ji]));
+    tangents_of(axis, dirs[0], dirs[1]);
+    dirs[2] = dirs[0];
+    dirs[3] = dirs[1];
   } else if (kind == JOINT_PRISMATIC) {
     // restrict_relative_linear_velocity_to_axis (helper.rs:788-856): the orthonormal_subspace_basis
     // of the world axis, then cancel_relative_angular_velocity: x, y, z
@@ -153,7 +164,7 @@ __global__ void __launch_bounds__(256) k_joint_assemble(DevBodies b, SimParams p
   const float4 il = j.imp_lin[ji], ia = j.imp_ang[ji];
   const int nlin = joint_linear_rows(kind);
   float imp[6] = {il.x, il.y, il.z, ia.x, ia.y, ia.z};
-  if (nlin == 2) {
+  if (nlin == 2) {  // prismatic: angular x, y, z; cylindrical: the two angular rows
     imp[2] = ia.x;
     imp[3] = ia.y;
     imp[4] = ia.z;
@@ -394,6 +405,17 @@ __device__ __forceinline__ void joint_position_one(const DevBodies& b, const Sim
     generic_solve(p, true, q_scaled_axis(qmul(r2, qconj(r1))), d1, d2, w1, w2, t1, q1, t2, q2, I1, im1, I2, im2,
                   ml1, ma1, ml2, ma2);
     translation();
+  } else if (kind == JOINT_CYLINDRICAL) {
+    // align_axis (helper.rs:701-769), then project_anchor_to_axis (helper.rs:858-912)
+    f3 w1, w2;
+    anchors(w1, w2);
+    const f3 a1 = qrot(q1, xyz(j.f1[ji])), a2 = qrot(q2, xyz(j.f2[ji]));
+    generic_solve(p, true, axis_alignment_error(a1, a2), d1, d2, w1, w2, t1, q1, t2, q2, I1, im1, I2, im2, ml1,
+                  ma1, ml2, ma2);
+    anchors(w1, w2);
+    const f3 axis = qrot(q1, xyz(j.f1[ji]));
+    const f3 proj = add(w1, scale(axis, dot(axis, sub(w2, w1))));
+    generic_solve(p, false, sub(w2, proj), d1, d2, w1, w2, t1, q1, t2, q2, I1, im1, I2, im2, ml1, ma1, ml2, ma2);
   } else if (kind == JOINT_PRISMATIC) {
     // cancel_relative_rotation of the bodies' rotations (helper.rs:553-612), then
     // project_anchor_to_axis (helper.rs:858-912): anchor 2 onto the line through anchor 1 along the axis
@@ -459,12 +481,18 @@ __global__ void __launch_bounds__(256) k_joint_cache(DevBodies b, SimParams p, J
   const int ji = j.order[s];
   const int nrows = __float_as_int(R[0].w) & 0xff;
   const float4 l0 = j.lam[2 * long(s)], l1 = j.lam[2 * long(s) + 1];
-  const bool prismatic = (__float_as_int(R[0].w) >> 8) == JOINT_PRISMATIC;
-  const f3 lin = prismatic ? f3{l0.x, l0.y, 0.0f} : f3{l0.x, l0.y, l0.z};
-  const f3 ang = prismatic ? f3{l0.z, l0.w, l1.x}
-                           : (nrows == 6 ? f3{l0.w, l1.x, l1.y} : (nrows == 5 ? f3{l0.w, l1.x, 0.0f} : f3{0.0f, 0.0f, 0.0f}));
-  // the revolute joint keeps a third angular impulse of zero, the prismatic joint a third linear
-  // one; the ball joint has no angular impulse
+  // the rows' impulses in order: nlin linear ones, then the angular ones; missing components are
+  // zero (the ball joint has no angular impulse, the revolute joint no third angular one, the
+  // prismatic and cylindrical joints no third linear one)
+  const int nlin = joint_linear_rows(__float_as_int(R[0].w) >> 8);
+  const float l[6] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y};
+  float lv[3] = {0.0f, 0.0f, 0.0f}, av[3] = {0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < nrows; ++k) {
+    if (k < nlin) lv[k] = l[k];
+    else av[k - nlin] = l[k];
+  }
+  const f3 lin{lv[0], lv[1], lv[2]};
+  const f3 ang{av[0], av[1], av[2]};
   j.imp_lin[ji] = f4(lin, 0.0f);
   j.imp_ang[ji] = f4(ang, 0.0f);
   const float inv_dt2 = p.inv_dt * p.inv_dt;

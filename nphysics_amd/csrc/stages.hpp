@@ -191,9 +191,12 @@ void launch_lam_copy(SolverBufs& sb, int C, float4* buf, bool to_buf, hipStream_
 // Joint constraints (ball, fixed, revolute): static SoA per joint in joint order, a fixed greedy
 // colouring (joints sharing a dynamic body differ in colour) with the joints listed colour by
 // colour, and per solve one row record per joint at its position in that list.
-enum { JOINT_BALL = 0, JOINT_FIXED = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3 };
-// linear rows of a joint kind (they come first): the prismatic joint's two, the others' three
-__host__ __device__ inline int joint_linear_rows(int kind) { return kind == JOINT_PRISMATIC ? 2 : 3; }
+enum { JOINT_BALL = 0, JOINT_FIXED = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3, JOINT_CYLINDRICAL = 4 };
+// linear rows of a joint kind (they come first): the prismatic and cylindrical joints' two (the
+// directions orthogonal to the axis), the others' three
+__host__ __device__ inline int joint_linear_rows(int kind) {
+  return kind == JOINT_PRISMATIC || kind == JOINT_CYLINDRICAL ? 2 : 3;
+}
 constexpr int kJointRec = 10;  // float4 per joint row record
 struct JointBufs {
   int nj;                // joints

@@ -423,7 +423,7 @@ struct World {
     for (int i = 0; i < d.n; ++i) {
       const int b1 = d.body1[i], b2 = d.body2[i];
       if (b1 < 0 || b1 >= n || b2 < 0 || b2 >= n || b1 == b2) return set_err(NPHYS_ERR_INVALID, "bad joint body");
-      if (d.kind[i] < NPHYS_JOINT_BALL || d.kind[i] > NPHYS_JOINT_PRISMATIC)
+      if (d.kind[i] < NPHYS_JOINT_BALL || d.kind[i] > NPHYS_JOINT_CYLINDRICAL)
         return set_err(NPHYS_ERR_INVALID, "bad joint kind");
       if (h_status[size_t(b1)] != NPHYS_STATUS_DYNAMIC && h_status[size_t(b2)] != NPHYS_STATUS_DYNAMIC)
         return set_err(NPHYS_ERR_INVALID, "a joint needs a dynamic body");
@@ -443,7 +443,7 @@ struct World {
       } else if (d.kind[i] == NPHYS_JOINT_PRISMATIC) {
         f1 = d.axis1 ? make_float4(d.axis1[3 * i], d.axis1[3 * i + 1], d.axis1[3 * i + 2], 0.0f)
                      : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
-      } else if (d.kind[i] == NPHYS_JOINT_REVOLUTE) {
+      } else if (d.kind[i] == NPHYS_JOINT_REVOLUTE || d.kind[i] == NPHYS_JOINT_CYLINDRICAL) {
         f1 = d.axis1 ? make_float4(d.axis1[3 * i], d.axis1[3 * i + 1], d.axis1[3 * i + 2], 0.0f)
                      : make_float4(0.0f, 0.0f, 1.0f, 0.0f);
         f2 = d.axis2 ? make_float4(d.axis2[3 * i], d.axis2[3 * i + 1], d.axis2[3 * i + 2], 0.0f)

@@ -123,7 +123,7 @@ int add_joints_t(orc::World<T>& w, const nphys_joint_desc* d, int32_t* first) {
   for (int i = 0; i < d->n; ++i) {
     const int b1 = d->body1[i], b2 = d->body2[i];
     if (b1 < 0 || b1 >= nb || b2 < 0 || b2 >= nb || b1 == b2) return fail(NPHYS_ERR_INVALID, "bad joint body");
-    if (d->kind[i] < 0 || d->kind[i] > 3) return fail(NPHYS_ERR_INVALID, "bad joint kind");
+    if (d->kind[i] < 0 || d->kind[i] > 4) return fail(NPHYS_ERR_INVALID, "bad joint kind");
     if (w.bodies[size_t(b1)].status != orc::STATUS_DYNAMIC && w.bodies[size_t(b2)].status != orc::STATUS_DYNAMIC)
       return fail(NPHYS_ERR_INVALID, "a joint needs a dynamic body");
   }

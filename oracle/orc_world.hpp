@@ -143,7 +143,7 @@ struct ContactRows {
 
 // A joint constraint (src/joint/): BallConstraint (ball_constraint.rs), FixedConstraint
 // (fixed_constraint.rs) or RevoluteConstraint without limits or motor (revolute_constraint.rs).
-enum { JOINT_BALL = 0, JOINT_FIXED = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3 };
+enum { JOINT_BALL = 0, JOINT_FIXED = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3, JOINT_CYLINDRICAL = 4 };
 template <class T>
 struct Joint {
   int kind, b1, b2;
@@ -916,7 +916,7 @@ class World {
 
   // ------------------------------------------------------------------ joints
   static int joint_num_rows(const Joint<T>& j) {  // num_velocity_constraints
-    return j.kind == JOINT_BALL ? 3 : (j.kind == JOINT_FIXED ? 6 : 5);
+    return j.kind == JOINT_BALL ? 3 : (j.kind == JOINT_FIXED ? 6 : (j.kind == JOINT_CYLINDRICAL ? 4 : 5));
   }
   // World anchors and, for the fixed joint, the frames' world rotations
   // (position_at_material_point * ref_frame, rigid_body.rs:641-648, fixed_constraint.rs:173-177).
@@ -951,6 +951,18 @@ class World {
       jr.row[jr.n++] = int(list.size()) - 1;
     };
     const V3<T> e[3] = {V3<T>{T(1), T(0), T(0)}, V3<T>{T(0), T(1), T(0)}, V3<T>{T(0), T(0), T(1)}};
+    if (jt.kind == JOINT_CYLINDRICAL) {
+      // CylindricalConstraint::velocity_constraints (cylindrical_constraint.rs:125-205):
+      // restrict_relative_linear_velocity_to_axis, then restrict_relative_angular_velocity_to_axis
+      // (helper.rs:614-699), both along orthonormal_subspace_basis(axis)
+      const V3<T> axis = qrot(b1.pos.q, jt.ax1);
+      V3<T> t[2];
+      tangents(axis, t[0], t[1]);
+      for (int i = 0; i < 2; ++i) push(t[i], false, comp(jt.lin_imp, i));
+      for (int i = 0; i < 2; ++i) push(t[i], true, comp(jt.ang_imp, i));
+      jrow.push_back(jr);
+      return;
+    }
     if (jt.kind == JOINT_PRISMATIC) {
       // PrismaticConstraint::velocity_constraints (prismatic_constraint.rs:132-206):
       // restrict_relative_linear_velocity_to_axis (helper.rs:788-856), the two directions of
@@ -993,7 +1005,7 @@ class World {
     const T inv_dt2 = prm.inv_dt * prm.inv_dt;
     for (const JointRows& jr : jrow) {
       Joint<T>& jt = joints[size_t(jr.joint)];
-      const int nlin = jt.kind == JOINT_PRISMATIC ? 2 : 3;  // linear rows first
+      const int nlin = jt.kind == JOINT_PRISMATIC || jt.kind == JOINT_CYLINDRICAL ? 2 : 3;  // linear rows first
       for (int k = 0; k < jr.n; ++k) {
         const T v = jrow_at(jr, k).impulse;
         if (k < nlin) set_comp(jt.lin_imp, k, v);
@@ -1055,6 +1067,17 @@ class World {
       joint_frames(jt, w1, w2, r1, r2);
       solve_joint_generic(jt, w1, w2, q_scaled_axis(qmul(r2, qconj(r1))), true);
       translation();
+    } else if (jt.kind == JOINT_CYLINDRICAL) {
+      // CylindricalConstraint::position_constraint (cylindrical_constraint.rs:256-292): i = 0
+      // align_axis, i = 1 project_anchor_to_axis
+      joint_frames(jt, w1, w2, r1, r2);
+      const V3<T> a1 = qrot(bodies[size_t(jt.b1)].pos.q, jt.ax1);
+      const V3<T> a2 = qrot(bodies[size_t(jt.b2)].pos.q, jt.ax2);
+      solve_joint_generic(jt, w1, w2, axis_alignment_error(a1, a2), true);
+      joint_frames(jt, w1, w2, r1, r2);
+      const V3<T> axis = qrot(bodies[size_t(jt.b1)].pos.q, jt.ax1);
+      const V3<T> proj = w1 + axis * dot(axis, w2 - w1);
+      solve_joint_generic(jt, w1, w2, w2 - proj, false);
     } else if (jt.kind == JOINT_PRISMATIC) {
       // PrismaticConstraint::position_constraint (prismatic_constraint.rs:279-345): i = 0
       // cancel_relative_rotation of the bodies' rotations, i = 1 project_anchor_to_axis

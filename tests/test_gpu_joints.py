@@ -65,11 +65,11 @@ def chain_scene(n=6, he=(0.15, 0.3, 0.15), top=(0.0, 4.0, 0.0), tilt=0.6, ground
     return s, b1, b2, a1, a2
 
 
-@pytest.mark.parametrize("kind", ["ball", "fixed", "revolute", "prismatic"])
+@pytest.mark.parametrize("kind", ["ball", "fixed", "revolute", "prismatic", "cylindrical"])
 def test_chain_swinging_into_the_ground(nphys, oracle_mod, kind):
     s, b1, b2, a1, a2 = chain_scene(n=6, top=(0.0, 2.5, 0.0))
     k = {"ball": abi.JOINT_BALL, "fixed": abi.JOINT_FIXED, "revolute": abi.JOINT_REVOLUTE,
-         "prismatic": abi.JOINT_PRISMATIC}[kind]
+         "prismatic": abi.JOINT_PRISMATIC, "cylindrical": abi.JOINT_CYLINDRICAL}[kind]
     js = abi.joints(k, b1, b2, a1, a2, axis1=(0, 0, 1), axis2=(0, 0, 1))
     g, o = _pair(nphys, oracle_mod, s, js)
     _run(g, o, 150, f"{kind} chain")
@@ -178,8 +178,8 @@ def test_many_joints_use_per_colour_launches(nphys, oracle_mod):
 
 
 def test_prismatic_rails_into_the_ground(nphys, oracle_mod):
-    """Cuboids on inclined rails (prismatic joints to a static post, the anchor off the centre of
-    mass) sliding into the ground, beside a revolute hinge: joint rows, the rotation and axis
+    """Cuboids on rails (a prismatic joint on an incline with the anchor off the centre of mass, a
+    cylindrical one on a vertical axis) sliding into the ground, beside a revolute hinge: joint rows, the rotation and axis
     position constraints and the contacts, bit for bit."""
     s = merge(body(status=1, shape=1, dims=(20.0, 0.5, 20.0), pos=(0, -0.5, 0)),
               body(status=1, shape=1, dims=(0.05, 0.05, 0.05), pos=(0, 6, 0)),
@@ -187,7 +187,7 @@ def test_prismatic_rails_into_the_ground(nphys, oracle_mod):
               body(shape=1, dims=(0.25, 0.25, 0.25), pos=(-1.0, 3.5, 0.5)),
               body(shape=0, dims=(0.3, 0, 0), pos=(0.0, 2.0, 1.5), angvel=(0.0, 0.0, 2.0)))
     c = math.sqrt(0.5)
-    js = abi.joints([abi.JOINT_PRISMATIC, abi.JOINT_PRISMATIC, abi.JOINT_REVOLUTE], [1, 1, 1], [2, 3, 4],
+    js = abi.joints([abi.JOINT_PRISMATIC, abi.JOINT_CYLINDRICAL, abi.JOINT_REVOLUTE], [1, 1, 1], [2, 3, 4],
                     [(1.0, -3.0, 0.0), (-1.0, -2.5, 0.5), (0.0, -4.0, 1.5)],
                     [(0.1, 0.05, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)],
                     axis1=[(c, -c, 0.0), (0.0, -1.0, 0.0), (0.0, 0.0, 1.0)], axis2=(0, 0, 1))

@@ -279,3 +279,26 @@ def test_prismatic_slide_closed_form(oracle_mod):
     # the anchor stays on the rail
     p = st["pos"][1]
     assert np.linalg.norm(p - a * p.dot(a)) < 1e-9
+
+
+def test_cylindrical_slide_and_spin_closed_form(oracle_mod):
+    """A cube on a vertical cylindrical joint (CylindricalConstraint, cylindrical_constraint.rs:125-292;
+    the anchor at its centre of mass, isotropic inertia, so the two linear and the two angular rows
+    are each decoupled): the first step's solve removes the velocity and spin across the axis
+    exactly, and from then on the cube falls freely along the axis while its spin about it stays.
+    The prismatic joint's third angular row would stop the spin, the revolute joint's third linear
+    row the fall."""
+    s = merge(body(status=1, shape=1, dims=(0.05, 0.05, 0.05), pos=(30, 0, 0)),
+              body(shape=1, dims=(0.25, 0.25, 0.25), pos=(0.0, 0.0, 0.0), linvel=(1.0, 2.0, 3.0),
+                   angvel=(0.4, 5.0, -0.7)))
+    js = abi.joints(abi.JOINT_CYLINDRICAL, 0, 1, (-30.0, 0.0, 0.0), (0.0, 0.0, 0.0), axis1=(0, 1, 0),
+                    axis2=(0, 1, 0))
+    o = oracle_mod.OracleWorld(abi.default_params(), precision=64)
+    o.add_bodies(s)
+    o.add_joints(js)
+    dt = 1.0 / 60.0
+    for k in range(1, 21):
+        o.step(1)
+        st = o.get_state()
+        np.testing.assert_allclose(st["linvel"][1], (0.0, 2.0 - 9.81 * dt * k, 0.0), rtol=1e-9, atol=1e-10)
+        np.testing.assert_allclose(st["angvel"][1], (0.0, 5.0, 0.0), rtol=1e-9, atol=1e-10)
